@@ -1,0 +1,101 @@
+// CPU FP64 oracle for the cuVHS decode hot path.
+//
+// TEST INFRASTRUCTURE ONLY.  This library restates the reference's stage
+// logic (hunterschuler/cuVHS, src/pipeline/*.cu, src/io/*.cpp,
+// src/format/*.cpp) statement by statement on the CPU in FP64.  Only tests/,
+// __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it, and
+// only as the checker.  The product (cuvhs_amd/, include/cuvhs.h) never links
+// or calls it.
+//
+// Parity pins (see tests/): FFT vs numpy.fft + brute-force DFT; filter
+// design vs scipy.signal; VideoFormat known answers; u8 conversion
+// exhaustive; K2/VSYNC vs brute-force run scans; K4 vs closed-form linear
+// edges; K5 vs closed-form Catmull-Rom on polynomials; JSON vs golden text.
+// Parity unpinned (restatement only, reviewed line by line against the
+// reference): K2b line location (pipeline.cu:410-893), the chunk/VSYNC
+// clustering heuristics (pipeline.cu:1595-1651), chroma track detection,
+// retry and phase-ID selection (chroma_decode.cu:674-1001).
+#pragma once
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OR_MAX_PULSES 800          /* sync_pulses.h:6 */
+#define OR_MAX_DROPOUTS 512        /* dropout_detect.h:5 */
+#define OR_LINES_PER_FRAME 525
+#define OR_FIELD_LINES 263
+#define OR_LINE_LEN 910
+
+typedef struct or_format {
+    double sample_rate;
+    int samples_per_line;
+    int samples_per_field;
+    int spfp;                 /* samples_per_field + 20 lines (pipeline.cu:1339-1342) */
+    int k1_fft_size;          /* next 7-smooth >= spfp + 2048 */
+    int chroma_fft_size;      /* next 7-smooth >= spfp */
+    double hsync_width, eq_pulse_width, vsync_width;
+    double ire0, hz_ire, vsync_ire, sync_tip_hz, pulse_threshold_hz;
+    double output_zero, output_scale, output_rate, fsc, chroma_under;
+    double burst_abs_ref, black16b, white16b;
+    int burst_start, burst_end;
+} or_format;
+
+int or_format_init(double mhz, or_format* out);
+
+/* --- filter tables (for design checks) --------------------------------- */
+/* which: 0 = K1 RF (real, fb bins), 1 = K1 Fvideo (complex, interleaved),
+ *        2 = K1 Fvideo05 (complex), 3 = chroma pre-BPF |H|^2/N (real)
+ * returns number of bins written (or required when out == NULL) */
+int or_filter_table(double mhz, int which, double* out);
+
+/* --- FFT (for tests) ----------------------------------------------------- */
+int or_fft(int n, const double* in_interleaved, double* out_interleaved, int sign);
+
+/* --- stage functions on host buffers (mirroring the product's C ABI) ----- */
+/* K1: raw[nfields*spfp] (already converted to s16 scale) */
+int or_fm_demod(double mhz, const double* raw, int nfields,
+                double* demod, double* demod05, double* env);
+/* VSYNC discovery: returns raw count (may exceed cap); idx holds the first
+ * min(count,cap) candidates in ascending order */
+int or_vsync_discover(double mhz, const double* d05, int total, int* idx, int cap);
+/* K2 */
+int or_sync_pulses(double mhz, const double* d05, const uint64_t* offsets, int nfields,
+                   int* starts, int* lens, int* count);
+/* K4 (in place) */
+int or_hsync_refine(double mhz, const double* d05, int total, double* linelocs, int nfields);
+/* K5 */
+int or_tbc_resample(double mhz, const double* demod, int total, const double* linelocs,
+                    int nfields, uint16_t* luma);
+
+/* --- whole decode ------------------------------------------------------- */
+typedef struct or_dec or_dec;
+or_dec* or_create(double mhz, int batch, int record_trace);
+void or_destroy(or_dec* d);
+/* Decode a whole capture held in memory (u8 samples).  Returns field count. */
+int or_decode(or_dec* d, const uint8_t* data, size_t n);
+int or_num_fields(const or_dec* d);
+const uint16_t* or_luma(const or_dec* d);
+const uint16_t* or_chroma(const or_dec* d);
+/* per-field metadata arrays (length num_fields each); any pointer may be NULL */
+void or_field_meta(const or_dec* d, uint64_t* file_loc, int* is_first, int* phase_id,
+                   int* do_count);
+/* dropouts of one field; returns count */
+int or_field_dropouts(const or_dec* d, int field, int* lines, int* startx, int* endx);
+int or_write_outputs(const or_dec* d, const char* out_base);
+/* trace of chunk c (record_trace=1); returns fields in chunk or -1 */
+int or_trace_chunk(const or_dec* d, int c, uint64_t* raw_offset, int* fields_k1,
+                   uint64_t* next_offset, uint64_t* offsets, int* pulse_count,
+                   int* pulse_starts, int* pulse_lens, int* vpulse_count, int* vpulse_starts,
+                   int* vpulse_lens, int* vpulse_types, double* ll_coarse, double* ll_refined,
+                   int* is_first_k2b);
+int or_num_chunks(const or_dec* d);
+
+/* CLI-equivalent: decode file -> out_base.{tbc,_chroma.tbc,tbc.json} */
+int or_decode_file(const char* in_path, const char* out_base, double mhz, int batch);
+
+#ifdef __cplusplus
+}
+#endif

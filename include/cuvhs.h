@@ -1,0 +1,162 @@
+/*
+ * cuvhs-mi355x: C ABI of the MI355X-native VHS RF decoder.
+ *
+ * The drop-in boundary is the reference's stage API (hunterschuler/cuVHS,
+ * src/pipeline/*.h) plus its chunk loop (pipeline.cu:1525-2713).  Every
+ * entry point takes plain pointers and sizes; device pointers are HIP device
+ * memory on the context's device.  Every call returns an int status:
+ *   0  OK
+ *   <0 error (CUVHS_E_*)
+ *   >0 data warning bits (CUVHS_W_*), the reference's soft paths kept
+ *      observable (pipeline.cu:1584-1587, 1614-1620).
+ *
+ * Work is enqueued on the context's stream; entry points that return
+ * host-visible results synchronise that stream.  A context is not
+ * thread-safe; distinct contexts are independent.
+ */
+#ifndef CUVHS_H
+#define CUVHS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CUVHS_MAX_PULSES 800          /* sync_pulses.h:6 */
+#define CUVHS_MAX_DROPOUTS 512        /* dropout_detect.h:5 */
+#define CUVHS_LINES_PER_FRAME 525     /* linelocs per field (video_format.cpp:26) */
+#define CUVHS_FIELD_LINES 263         /* TBC rows per field */
+#define CUVHS_LINE_LEN 910            /* TBC samples per row */
+
+enum {
+    CUVHS_OK = 0,
+    CUVHS_E_ARG = -1,
+    CUVHS_E_HIP = -2,
+    CUVHS_E_ALLOC = -3,
+    CUVHS_E_UNSUPPORTED = -4,
+    CUVHS_E_CAPACITY = -5
+};
+
+enum {
+    CUVHS_W_VSYNC_OVERFLOW = 1,       /* pipeline.cu:1584: candidates truncated */
+    CUVHS_W_NO_VSYNC = 2,             /* pipeline.cu:1614: nominal spacing fallback */
+    CUVHS_W_EOF = 4                   /* pipeline.cu:1535: fewer than one field left */
+};
+
+enum { CUVHS_SYSTEM_NTSC = 0 };
+enum { CUVHS_PRECISION_FP64 = 0 };
+
+typedef struct cuvhs_ctx cuvhs_ctx;
+
+typedef struct cuvhs_config {
+    int system;               /* CUVHS_SYSTEM_NTSC (PAL: CUVHS_E_UNSUPPORTED) */
+    double sample_rate_mhz;   /* capture rate, e.g. 28.0 */
+    int batch_fields;         /* F: fields per chunk; results depend on F */
+    int precision;            /* CUVHS_PRECISION_* */
+    void* stream;             /* hipStream_t, NULL = a stream owned by the context */
+    int device;               /* HIP device ordinal */
+} cuvhs_config;
+
+/* Derived timing constants (VideoFormat, video_format.cpp:23-104). */
+typedef struct cuvhs_format {
+    double sample_rate;
+    int samples_per_line;     /* round(fs / 15734.264) */
+    int samples_per_field;    /* samples_per_line * 263 */
+    int spfp;                 /* samples_per_field + 20 lines (pipeline.cu:1339-1342) */
+    double pulse_threshold_hz, ire0, hz_ire, output_scale, output_zero, output_rate;
+    double black16b, white16b;
+    int burst_start, burst_end;
+} cuvhs_format;
+
+/* Create/destroy.  Replaces Pipeline::allocate_buffers + FMDemodState::init
+ * (pipeline.cu:1383-1475, fm_demod.cu:326-427). */
+int cuvhs_create(const cuvhs_config* cfg, cuvhs_ctx** out);
+int cuvhs_destroy(cuvhs_ctx* ctx);
+int cuvhs_get_format(const cuvhs_ctx* ctx, cuvhs_format* out);
+int cuvhs_format_for(double sample_rate_mhz, cuvhs_format* out);
+/* Clears the state carried across chunks (K2b cadence, ChromaState, field
+ * parity: pipeline.h:93-97, chroma_decode.h:9-15, pipeline.cu:2575). */
+int cuvhs_reset(cuvhs_ctx* ctx);
+
+/* ---- chunk-level decode (Pipeline::process_chunk, pipeline.cu:1525-2624) ---- */
+typedef struct cuvhs_chunk_info {
+    int fields;               /* fields decoded in this chunk (0 = stop) */
+    int fields_k1;            /* segments demodulated (n_read / spf) */
+    int vsync_candidates;     /* raw candidate count before truncation */
+    int status;               /* warning bits for this chunk */
+    uint64_t raw_offset;      /* chunk start in the capture (samples) */
+    uint64_t next_raw_offset; /* start of the next chunk */
+} cuvhs_chunk_info;
+
+/* Device pointers to the last chunk's outputs, owned by the context and valid
+ * until the next decode call.  Field i of the chunk:
+ *   luma/chroma: [fields][263][910] u16
+ *   file_loc[i] (u64), is_first[i], phase_id[i], do_count[i],
+ *   do_line/do_startx/do_endx: [fields][512] */
+typedef struct cuvhs_chunk_outputs {
+    const uint16_t* luma;
+    const uint16_t* chroma;
+    const uint64_t* file_loc;
+    const int32_t* is_first;
+    const int32_t* phase_id;
+    const int32_t* do_count;
+    const int32_t* do_line;
+    const int32_t* do_startx;
+    const int32_t* do_endx;
+} cuvhs_chunk_outputs;
+
+/* Decode the chunk starting at raw_offset of a device-resident u8 capture of
+ * capture_len samples, with at most max_fields fields (<= batch_fields).
+ * Mirrors Pipeline::run's chunk step (pipeline.cu:2650-2665). */
+int cuvhs_decode_chunk(cuvhs_ctx* ctx, const uint8_t* d_capture, uint64_t capture_len, uint64_t raw_offset,
+                       int max_fields, cuvhs_chunk_info* info);
+int cuvhs_chunk_outputs_get(const cuvhs_ctx* ctx, cuvhs_chunk_outputs* out);
+
+/* Decode a whole device-resident capture (Pipeline::run, pipeline.cu:2626-2713)
+ * into caller device buffers sized for max_out_fields fields.  Returns the
+ * field count through *fields. */
+int cuvhs_decode_capture(cuvhs_ctx* ctx, const uint8_t* d_capture, uint64_t capture_len, int max_out_fields,
+                         uint16_t* d_luma, uint16_t* d_chroma, uint64_t* d_file_loc, int32_t* d_is_first,
+                         int32_t* d_phase_id, int32_t* d_do_count, int32_t* d_do_line, int32_t* d_do_startx,
+                         int32_t* d_do_endx, int* fields);
+
+/* ---- stage entry points on caller device buffers (parity tests) ----
+ * All FP64 signal buffers hold nfields*spfp samples, laid out as the
+ * reference's chunk buffers. */
+
+/* fm_demod (fm_demod.h:46-53): u8 chunk (n_valid samples, zero past it). */
+int cuvhs_fm_demod(cuvhs_ctx* ctx, const uint8_t* d_raw_u8, uint64_t n_valid, int nfields, double* d_demod,
+                   double* d_demod05, double* d_env);
+/* discover_vsyncs (vsync_discover.h:6-11): ascending candidates, returns the
+ * raw count through *count (may exceed cap). */
+int cuvhs_vsync_discover(cuvhs_ctx* ctx, const double* d_demod05, int total, int* d_idx, int cap, int* count);
+/* sync_pulses (sync_pulses.h:20-27) with per-field offsets. */
+int cuvhs_sync_pulses(cuvhs_ctx* ctx, const double* d_demod05, int total, const uint64_t* d_offsets, int nfields,
+                      int* d_starts, int* d_lens, int* d_count);
+/* K2b line location (pipeline.cu:1694-1791) on one chunk's K2 pulses, using
+ * and updating the context's carried cadence state. */
+int cuvhs_linelocs(cuvhs_ctx* ctx, uint64_t raw_offset, const uint64_t* d_offsets, int nfields, int* d_starts,
+                   int* d_lens, int* d_count, int* d_types, double* d_linelocs, int* d_is_first);
+/* hsync_refine (hsync_refine.h:18-22), in place. */
+int cuvhs_hsync_refine(cuvhs_ctx* ctx, const double* d_demod05, int total, double* d_linelocs, int nfields);
+/* tbc_resample (tbc_resample.h:16-27), debug counters omitted. */
+int cuvhs_tbc_resample(cuvhs_ctx* ctx, const double* d_demod, int total, const double* d_linelocs, int nfields,
+                       uint16_t* d_luma);
+/* chroma_decode (chroma_decode.h:38-46): u8 chunk in, phase IDs out
+ * (host array of nfields), using and updating the carried ChromaState. */
+int cuvhs_chroma(cuvhs_ctx* ctx, const uint8_t* d_raw_u8, uint64_t n_valid, const double* d_linelocs,
+                 int nfields, uint16_t* d_chroma, int* h_phase_ids);
+/* dropout_detect (dropout_detect.h:22-32), concealment in place. */
+int cuvhs_dropouts(cuvhs_ctx* ctx, const double* d_env, const double* d_linelocs, int nfields, uint16_t* d_luma,
+                   uint16_t* d_chroma, int* d_lines, int* d_starts, int* d_ends, int* d_count);
+
+/* Last HIP error text for the context (or a static string). */
+const char* cuvhs_last_error(const cuvhs_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CUVHS_H */

@@ -1,0 +1,259 @@
+// `cuvhs` command line: the reference's drop-in surface (src/main.cpp:24-165,
+// Pipeline::run pipeline.cu:2626-2713) on top of the C ABI in include/cuvhs.h.
+//
+//   cuvhs [--system NTSC] [-f MHz] [--format u8] [--gpu id] [--batch F]
+//         [--overwrite] <input.u8> <output_base>
+//
+// Writes output_base.tbc, output_base_chroma.tbc and output_base.tbc.json.
+// Exit status 0 on success, 1 on any failure.  --batch sets the chunk size F
+// (default 64); the reference derives F from free VRAM and its results
+// depend on it, so it is explicit here.
+#include <fcntl.h>
+#include <hip/hip_runtime.h>
+#include <strings.h>
+#include <sys/stat.h>
+#include <unistd.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../../include/cuvhs.h"
+#include "tbc_io.h"
+
+namespace {
+
+struct Args {
+    std::string input, output;
+    bool pal = false;
+    double mhz = 28.0;
+    std::string format;
+    bool dc_correct = false;
+    int gpu = 0;
+    int batch = 64;
+    bool overwrite = false;
+};
+
+void usage(const char* prog) {
+    fprintf(stderr,
+            "cuvhs (MI355X) - VHS RF decoder\n\n"
+            "Usage: %s [options] <input_file> <output_base>\n\n"
+            "Arguments:\n"
+            "  input_file       Raw RF capture (.u8 or .raw, 8-bit unsigned)\n"
+            "  output_base      Output prefix (creates .tbc, _chroma.tbc, .tbc.json)\n\n"
+            "Options:\n"
+            "  --system <NTSC>       Video system (default: NTSC)\n"
+            "  -f <MHz>              Sample rate in MHz (default: 28)\n"
+            "  --format <u8>         Input sample format (default: from extension)\n"
+            "  --gpu <id>            GPU device ID (default: 0)\n"
+            "  --batch <fields>      Fields per chunk (default: 64)\n"
+            "  --overwrite           Overwrite existing output files\n"
+            "  -h, --help            Show this help\n",
+            prog);
+}
+
+bool parse(int argc, char** argv, Args& a) {
+    int pos = 0;
+    for (int i = 1; i < argc; i++) {
+        if (!strcmp(argv[i], "-h") || !strcmp(argv[i], "--help")) {
+            usage(argv[0]);
+            exit(0);
+        } else if (!strcmp(argv[i], "--system") && i + 1 < argc) {
+            i++;
+            if (!strcasecmp(argv[i], "NTSC")) a.pal = false;
+            else if (!strcasecmp(argv[i], "PAL")) a.pal = true;
+            else { fprintf(stderr, "Unknown system: %s\n", argv[i]); return false; }
+        } else if (!strcmp(argv[i], "-f") && i + 1 < argc) {
+            a.mhz = atof(argv[++i]);
+        } else if (!strcmp(argv[i], "--format") && i + 1 < argc) {
+            a.format = argv[++i];
+            if (a.format != "u8" && a.format != "s16" && a.format != "u16") {
+                fprintf(stderr, "Unknown format: %s\n", a.format.c_str());
+                return false;
+            }
+        } else if (!strcmp(argv[i], "--dc-correct")) {
+            a.dc_correct = true;
+        } else if (!strcmp(argv[i], "--gpu") && i + 1 < argc) {
+            a.gpu = atoi(argv[++i]);
+        } else if (!strcmp(argv[i], "--batch") && i + 1 < argc) {
+            a.batch = atoi(argv[++i]);
+        } else if (!strcmp(argv[i], "--overwrite")) {
+            a.overwrite = true;
+        } else if (argv[i][0] == '-' && strlen(argv[i]) > 1) {
+            fprintf(stderr, "Unknown option: %s\n", argv[i]);
+            return false;
+        } else {
+            if (pos == 0) a.input = argv[i];
+            else if (pos == 1) a.output = argv[i];
+            else { fprintf(stderr, "Too many arguments\n"); return false; }
+            pos++;
+        }
+    }
+    if (pos < 2) {
+        fprintf(stderr, "Error: input_file and output_base are required\n\n");
+        usage(argv[0]);
+        return false;
+    }
+    return true;
+}
+
+std::string detect_format(const std::string& path) {
+    auto ext = path.substr(path.find_last_of('.') + 1);
+    if (ext == "s16") return "s16";
+    if (ext == "u16") return "u16";
+    return "u8";  // .u8, .raw and anything else (main.cpp:89-95)
+}
+
+#define CHECK_HIP(x)                                                                       \
+    do {                                                                                   \
+        hipError_t e_ = (x);                                                               \
+        if (e_ != hipSuccess) {                                                            \
+            fprintf(stderr, "HIP error %s at %s\n", hipGetErrorString(e_), #x);           \
+            return 1;                                                                      \
+        }                                                                                  \
+    } while (0)
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    Args a;
+    if (!parse(argc, argv, a)) return 1;
+    if (a.format.empty()) a.format = detect_format(a.input);
+    if (a.pal) { fprintf(stderr, "PAL is not supported by this decoder\n"); return 1; }
+    if (a.format != "u8") { fprintf(stderr, "Input format %s is not supported (u8 only)\n", a.format.c_str()); return 1; }
+    if (a.dc_correct) { fprintf(stderr, "--dc-correct is not supported for u8 input in this decoder\n"); return 1; }
+    if (a.input == "-") { fprintf(stderr, "stdin streaming is not supported; pass a file\n"); return 1; }
+    if (a.batch < 1) { fprintf(stderr, "--batch must be >= 1\n"); return 1; }
+
+    cuvhs_config cfg{};
+    cfg.system = CUVHS_SYSTEM_NTSC;
+    cfg.sample_rate_mhz = a.mhz;
+    cfg.batch_fields = a.batch;
+    cfg.precision = CUVHS_PRECISION_FP64;
+    cfg.stream = nullptr;
+    cfg.device = a.gpu;
+    cuvhs_ctx* ctx = nullptr;
+    int rc = cuvhs_create(&cfg, &ctx);
+    if (rc != 0) {
+        fprintf(stderr, "Failed to initialize GPU decoder (%d)\n", rc);
+        return 1;
+    }
+    cuvhs_format fmt;
+    cuvhs_get_format(ctx, &fmt);
+    fprintf(stderr, "Format: NTSC @ %.1f MHz capture rate\n  Samples/line: %d  Samples/field: %d\n", a.mhz,
+            fmt.samples_per_line, fmt.samples_per_field);
+
+    int fd = open(a.input.c_str(), O_RDONLY);
+    if (fd < 0) {
+        perror(a.input.c_str());
+        fprintf(stderr, "Failed to open input: %s\n", a.input.c_str());
+        cuvhs_destroy(ctx);
+        return 1;
+    }
+    struct stat st;
+    fstat(fd, &st);
+    const uint64_t total = (uint64_t)st.st_size;
+    fprintf(stderr, "Input: %s (%.1f MHz, u8, %.2f GB)\n", a.input.c_str(), a.mhz, total / (1024.0 * 1024.0 * 1024.0));
+
+    cuvhs_cli::TbcWriter writer;
+    if (!writer.open(a.output, a.overwrite)) {
+        fprintf(stderr, "Failed to open output: %s\n", a.output.c_str());
+        close(fd);
+        cuvhs_destroy(ctx);
+        return 1;
+    }
+
+    const size_t chunk_cap = (size_t)a.batch * (size_t)fmt.spfp;
+    const size_t tbc = (size_t)CUVHS_FIELD_LINES * CUVHS_LINE_LEN;
+    uint8_t* h_chunk = nullptr;
+    uint8_t* d_chunk = nullptr;
+    CHECK_HIP(hipHostMalloc((void**)&h_chunk, chunk_cap, hipHostMallocDefault));
+    CHECK_HIP(hipMalloc((void**)&d_chunk, chunk_cap));
+    std::vector<uint16_t> luma(a.batch * tbc), chroma(a.batch * tbc);
+    std::vector<uint64_t> file_loc(a.batch);
+    std::vector<int32_t> is_first(a.batch), phase(a.batch), doc(a.batch);
+    std::vector<int32_t> dol((size_t)a.batch * CUVHS_MAX_DROPOUTS), dos(dol.size()), doe(dol.size());
+
+    const int total_est = (int)(total / (uint64_t)fmt.samples_per_field);
+    fprintf(stderr, "Starting decode: ~%d fields in chunks of %d\n\n\n", total_est, a.batch);
+    auto t0 = std::chrono::steady_clock::now();
+    uint64_t raw_offset = 0;
+    int done = 0;
+    bool ok = true;
+    while (true) {
+        if (raw_offset >= total) break;
+        uint64_t remaining = (total - raw_offset) / (uint64_t)fmt.samples_per_field;
+        if (remaining == 0) break;
+        int this_chunk = (int)std::min<uint64_t>((uint64_t)a.batch, remaining);
+        size_t want = (size_t)this_chunk * (size_t)fmt.spfp;
+        size_t n_read = (size_t)std::min<uint64_t>(want, total - raw_offset);
+        ssize_t got = pread(fd, h_chunk, n_read, (off_t)raw_offset);
+        if (got <= 0) break;
+        n_read = (size_t)got;
+        CHECK_HIP(hipMemcpy(d_chunk, h_chunk, n_read, hipMemcpyHostToDevice));
+        cuvhs_chunk_info info;
+        rc = cuvhs_decode_chunk(ctx, d_chunk, n_read, raw_offset, this_chunk, &info);
+        if (rc < 0) {
+            fprintf(stderr, "Decode failed (%d): %s\n", rc, cuvhs_last_error(ctx));
+            ok = false;
+            break;
+        }
+        if (info.status & CUVHS_W_VSYNC_OVERFLOW)
+            fprintf(stderr, "Warning: VSYNC candidate buffer overflow (%d > %d), truncating chunk results\n",
+                    info.vsync_candidates, info.fields_k1 * 20);
+        if (info.status & CUVHS_W_NO_VSYNC)
+            fprintf(stderr, "Warning: no VSYNC candidates found in chunk at raw offset %llu, using nominal field spacing fallback\n",
+                    (unsigned long long)raw_offset);
+        if (info.fields == 0) break;
+        const int n = info.fields;
+        cuvhs_chunk_outputs o;
+        cuvhs_chunk_outputs_get(ctx, &o);
+        CHECK_HIP(hipMemcpy(luma.data(), o.luma, n * tbc * 2, hipMemcpyDeviceToHost));
+        CHECK_HIP(hipMemcpy(chroma.data(), o.chroma, n * tbc * 2, hipMemcpyDeviceToHost));
+        CHECK_HIP(hipMemcpy(file_loc.data(), o.file_loc, n * 8, hipMemcpyDeviceToHost));
+        CHECK_HIP(hipMemcpy(is_first.data(), o.is_first, n * 4, hipMemcpyDeviceToHost));
+        CHECK_HIP(hipMemcpy(phase.data(), o.phase_id, n * 4, hipMemcpyDeviceToHost));
+        CHECK_HIP(hipMemcpy(doc.data(), o.do_count, n * 4, hipMemcpyDeviceToHost));
+        CHECK_HIP(hipMemcpy(dol.data(), o.do_line, (size_t)n * CUVHS_MAX_DROPOUTS * 4, hipMemcpyDeviceToHost));
+        CHECK_HIP(hipMemcpy(dos.data(), o.do_startx, (size_t)n * CUVHS_MAX_DROPOUTS * 4, hipMemcpyDeviceToHost));
+        CHECK_HIP(hipMemcpy(doe.data(), o.do_endx, (size_t)n * CUVHS_MAX_DROPOUTS * 4, hipMemcpyDeviceToHost));
+        if (!writer.write_fields(luma.data(), chroma.data(), n)) { fprintf(stderr, "TBC write failed\n"); ok = false; break; }
+        for (int i = 0; i < n; i++) {
+            cuvhs_cli::FieldMeta m;
+            m.is_first = is_first[i] != 0;
+            m.phase_id = phase[i];
+            m.file_loc = file_loc[i];
+            const size_t b = (size_t)i * CUVHS_MAX_DROPOUTS;
+            for (int k = 0; k < doc[i]; k++) {
+                m.do_line.push_back(dol[b + k]);
+                m.do_startx.push_back(dos[b + k]);
+                m.do_endx.push_back(doe[b + k]);
+            }
+            writer.add_field(std::move(m));
+        }
+        writer.write_json();
+        done += n;
+        raw_offset = info.next_raw_offset;
+        double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        double pct = total_est > 0 ? 100.0 * done / total_est : 100.0;
+        fprintf(stderr, "\033[2A\033[K  %d/%d fields (%.1f%%) | %.1f FPS\n\033[K\n", done, total_est, pct,
+                (done / 2.0) / std::max(el, 1e-9));
+    }
+    double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    fprintf(stderr, "\033[2A\033[KDecode complete: %d fields in %.1f seconds (%.1f FPS)\n\033[K\n", done, el,
+            (done / 2.0) / std::max(el, 1e-9));
+    (void)hipFree(d_chunk);
+    (void)hipHostFree(h_chunk);
+    close(fd);
+    cuvhs_destroy(ctx);
+    if (!writer.finalize() || !ok) {
+        fprintf(stderr, "Pipeline failed\n");
+        return 1;
+    }
+    fprintf(stderr, "Done.\n");
+    return 0;
+}

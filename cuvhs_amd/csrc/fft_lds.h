@@ -1,0 +1,168 @@
+// LDS-resident Stockham FFT for one workgroup (gfx950, wave64).
+//
+// B points of complex T (double2 / float2) held in LDS; B/16 threads each own
+// one radix-16 butterfly per stage (16 complex values in VGPRs), so a stage
+// is: load 16 values -> barrier -> twiddle + DFT16 in registers -> store to
+// the Stockham position -> barrier.  The buffer is reused in place because
+// every load of a stage completes before any store of that stage.
+// Sizes: 1024 = 16*16*4, 4096 = 16^3, 8192 = 16^3*2, 16384 = 16^3*4.
+// Twiddles come from a table of W_B^m = exp(-2 pi i m / B) in global memory:
+// four loads per butterfly (w1, w2, w4, w8), the other powers by at most
+// three complex products.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace cuvhs {
+
+template <typename T> struct C2;
+template <> struct C2<double> { using type = double2; };
+template <> struct C2<float> { using type = float2; };
+
+template <typename V> __device__ __forceinline__ V cmul(V a, V b) {
+    V r;
+    r.x = a.x * b.x - a.y * b.y;
+    r.y = a.x * b.y + a.y * b.x;
+    return r;
+}
+template <typename V> __device__ __forceinline__ V cadd(V a, V b) { V r; r.x = a.x + b.x; r.y = a.y + b.y; return r; }
+template <typename V> __device__ __forceinline__ V csub(V a, V b) { V r; r.x = a.x - b.x; r.y = a.y - b.y; return r; }
+// multiply by sign*i  (sign = -1: -i, +1: +i)
+template <int S, typename V> __device__ __forceinline__ V mul_si(V a) {
+    V r;
+    if (S < 0) { r.x = a.y; r.y = -a.x; } else { r.x = -a.y; r.y = a.x; }
+    return r;
+}
+template <typename V> __device__ __forceinline__ V cconj(V a) { V r; r.x = a.x; r.y = -a.y; return r; }
+
+template <int S, typename V> __device__ __forceinline__ void dft2(V& a, V& b) {
+    V t = a;
+    a = cadd(t, b);
+    b = csub(t, b);
+}
+
+template <int S, typename V> __device__ __forceinline__ void dft4(V& a0, V& a1, V& a2, V& a3) {
+    V t0 = cadd(a0, a2), t1 = csub(a0, a2), t2 = cadd(a1, a3), t3 = mul_si<S>(csub(a1, a3));
+    a0 = cadd(t0, t2);
+    a2 = csub(t0, t2);
+    a1 = cadd(t1, t3);
+    a3 = csub(t1, t3);
+}
+
+// In-place DFT16 of v[0..15] (natural order in and out).
+template <int S, typename T> __device__ __forceinline__ void dft16(typename C2<T>::type* v) {
+    using V = typename C2<T>::type;
+    // cos/sin(2 pi k / 16), k = 1, 2, 3
+    const T c1 = (T)0.92387953251128675613, s1 = (T)0.38268343236508977173;
+    const T c2 = (T)0.70710678118654752440;
+    // step 1: DFT4 over n1 for each n2 (elements n2, n2+4, n2+8, n2+12)
+#pragma unroll
+    for (int n2 = 0; n2 < 4; n2++) dft4<S>(v[n2], v[n2 + 4], v[n2 + 8], v[n2 + 12]);
+    // now v[n2 + 4*k1] = a[n2][k1]; step 2: twiddle by W16^(n2*k1)
+    auto tw = [&](V a, int e) -> V {
+        // W16^e with sign S, e in [0, 9]
+        T c, s;
+        switch (e) {
+            case 0: return a;
+            case 1: c = c1; s = s1; break;
+            case 2: c = c2; s = c2; break;
+            case 3: c = s1; s = c1; break;
+            case 4: return mul_si<S>(a);
+            case 6: c = -c2; s = c2; break;
+            case 9: c = -c1; s = -s1; break;
+            default: c = 1; s = 0; break;
+        }
+        V w;
+        w.x = c;
+        w.y = (S < 0) ? -s : s;
+        return cmul(a, w);
+    };
+#pragma unroll
+    for (int n2 = 1; n2 < 4; n2++)
+#pragma unroll
+        for (int k1 = 1; k1 < 4; k1++) v[n2 + 4 * k1] = tw(v[n2 + 4 * k1], n2 * k1);
+    // step 3: DFT4 over n2 for each k1 -> X[k1 + 4*k2]
+    V out[16];
+#pragma unroll
+    for (int k1 = 0; k1 < 4; k1++) {
+        V a0 = v[0 + 4 * k1], a1 = v[1 + 4 * k1], a2 = v[2 + 4 * k1], a3 = v[3 + 4 * k1];
+        dft4<S>(a0, a1, a2, a3);
+        out[k1 + 0] = a0;
+        out[k1 + 4] = a1;
+        out[k1 + 8] = a2;
+        out[k1 + 12] = a3;
+    }
+#pragma unroll
+    for (int i = 0; i < 16; i++) v[i] = out[i];
+}
+
+// One Stockham stage with radix R over B points, ns = product of previous radices.
+// Each of the NT threads handles 16/R butterflies (16 values).
+template <int S, typename T, int B, int R, int NT>
+__device__ __forceinline__ void fft_stage(typename C2<T>::type* buf, const typename C2<T>::type* tw, int ns) {
+    using V = typename C2<T>::type;
+    constexpr int NB = B / R;          // butterflies in the stage
+    constexpr int PER = NB / NT;       // butterflies per thread
+    static_assert(PER * R == 16, "each thread owns 16 values");
+    V v[16];
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int p = 0; p < PER; p++) {
+        int j = t + p * NT;
+#pragma unroll
+        for (int r = 0; r < R; r++) v[p * R + r] = buf[j + r * NB];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int p = 0; p < PER; p++) {
+        int j = t + p * NT;
+        int k = j % ns;
+        V* u = v + p * R;
+        if (ns > 1) {
+            // twiddles W_{ns*R}^{r*k} = W_B^{r*k*B/(ns*R)}
+            int step = k * (B / (ns * R));
+            V w[16];
+            w[0].x = (T)1; w[0].y = (T)0;
+            auto ld = [&](int e) -> V { V x = tw[e & (B - 1)]; return S < 0 ? x : cconj(x); };
+            if (R >= 2) w[1] = ld(step);
+            if (R >= 4) { w[2] = ld(2 * step); w[3] = cmul(w[1], w[2]); }
+            if (R >= 8) { w[4] = ld(4 * step); w[5] = cmul(w[4], w[1]); w[6] = cmul(w[4], w[2]); w[7] = cmul(w[4], w[3]); }
+            if (R >= 16) {
+                w[8] = ld(8 * step);
+#pragma unroll
+                for (int r = 9; r < 16; r++) w[r] = cmul(w[8], w[r - 8]);
+            }
+#pragma unroll
+            for (int r = 1; r < R; r++) u[r] = cmul(u[r], w[r]);
+        }
+        if (R == 16) dft16<S, T>(u);
+        else if (R == 4) dft4<S>(u[0], u[1], u[2], u[3]);
+        else if (R == 2) dft2<S>(u[0], u[1]);
+    }
+#pragma unroll
+    for (int p = 0; p < PER; p++) {
+        int j = t + p * NT;
+        int k = j % ns;
+        int idx = (j / ns) * ns * R + k;
+#pragma unroll
+        for (int r = 0; r < R; r++) buf[idx + r * ns] = v[p * R + r];
+    }
+    __syncthreads();
+}
+
+// Full transform; the caller has written buf and synchronised.
+template <int S, typename T, int B>
+__device__ __forceinline__ void fft_lds(typename C2<T>::type* buf, const typename C2<T>::type* tw) {
+    constexpr int NT = B / 16;
+    static_assert(B == 1024 || B == 4096 || B == 8192 || B == 16384, "unsupported FFT size");
+    fft_stage<S, T, B, 16, NT>(buf, tw, 1);
+    fft_stage<S, T, B, 16, NT>(buf, tw, 16);
+    if (B == 1024) {
+        fft_stage<S, T, B, 4, NT>(buf, tw, 256);
+    } else {
+        fft_stage<S, T, B, 16, NT>(buf, tw, 256);
+        if (B == 8192) fft_stage<S, T, B, 2, NT>(buf, tw, 4096);
+        if (B == 16384) fft_stage<S, T, B, 4, NT>(buf, tw, 4096);
+    }
+}
+
+}  // namespace cuvhs

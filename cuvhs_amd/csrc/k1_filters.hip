@@ -1,0 +1,264 @@
+// K1 FM demodulation and the chroma capture-rate pre-band-pass as blocked
+// overlap-save convolutions in LDS.
+//
+// Reference: fm_demod.cu:449-539 (whole-segment D2Z -> filter -> Z2Z/Z2D over
+// N = next_7smooth(spfp + 2048) points per field segment) and
+// chroma_decode.cu:594-633 (per-segment circular pre-band-pass over
+// N2 = next_7smooth(spfp)).
+//
+// Equivalence: the reference computes, for segment f and window position q,
+//   y_f(q) = sum_{m} g_N[m] * x_f((q - m) mod N)
+// with x_f(p) = src[clamp(f*spfp - 2048 + p, 0, T-1)] (fm_demod.cu:186-206).
+// Each workgroup here computes V consecutive outputs of one segment as a
+// B-point circular convolution of B loaded inputs with the taps of g_N kept
+// in [-lm, lp]; the loaded input indices wrap modulo N exactly like the
+// reference's window, so the only difference is the dropped taps
+// (|g| < eps * max) plus rounding.
+#include <hip/hip_runtime.h>
+
+#include "common.h"
+#include "fft_lds.h"
+#include "launch.h"
+
+namespace cuvhs {
+
+__device__ __forceinline__ double u8_to_s16(const uint8_t* cap, long long s, long long n_valid) {
+    // RawReader::convert for u8 (raw_reader.cpp:74-77); zero past the read
+    // (pipeline.cu:1539-1541)
+    return s < n_valid ? ((double)cap[s] - 128.0) * 256.0 : 0.0;
+}
+
+__device__ __forceinline__ long long wrap_mod(long long q, int n) {
+    long long r = q % n;
+    return r < 0 ? r + n : r;
+}
+
+// Part A: analytic signal, envelope and instantaneous frequency.
+template <int B>
+__global__ __launch_bounds__(B / 16) void k1_part_a(K1ALaunch a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    double2* buf = reinterpret_cast<double2*>(smem);
+    constexpr int NT = B / 16;
+    const int f = blockIdx.y;
+    const int b = blockIdx.x;
+    const long long qs = (long long)(kOverlap - 1) + (long long)b * a.V;  // first analytic output
+    if (qs + 1 >= (long long)kOverlap + a.spfp) return;
+    const long long seg0 = (long long)f * a.spfp - kOverlap;
+    for (int j = threadIdx.x; j < B; j += NT) {
+        long long q = qs - a.lp + j;
+        long long s = seg0 + wrap_mod(q, a.N);
+        if (s < 0) s = 0;
+        if (s >= a.total) s = a.total - 1;
+        double2 v;
+        v.x = u8_to_s16(a.cap, s, a.n_valid);
+        v.y = 0.0;
+        buf[j] = v;
+    }
+    __syncthreads();
+    fft_lds<-1, double, B>(buf, a.tw);
+    for (int j = threadIdx.x; j < B; j += NT) buf[j] = cmul(buf[j], a.G[j]);
+    __syncthreads();
+    fft_lds<+1, double, B>(buf, a.tw);
+    // outputs o = 0..V: analytic a(qs + o) = buf[lp + o]
+    constexpr int MAXO = (B + NT - 1) / NT;
+    double ang[MAXO];
+    const long long qend = (long long)kOverlap + a.spfp;  // one past the last demod output
+#pragma unroll
+    for (int i = 0; i < MAXO; i++) {
+        int o = threadIdx.x + i * NT;
+        ang[i] = 0.0;
+        if (o <= a.V) {
+            double2 z = buf[a.lp + o];
+            ang[i] = atan2(z.y, z.x);
+            long long q = qs + o;
+            if (q >= kOverlap && q < qend) {
+                long long s = (long long)f * a.spfp + (q - kOverlap);
+                a.env[s] = sqrt(z.x * z.x + z.y * z.y);
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < MAXO; i++) {
+        int o = threadIdx.x + i * NT;
+        if (o <= a.V) buf[a.lp + o].x = ang[i];
+    }
+    __syncthreads();
+    const double PI = 3.14159265358979323846;
+    const double TAU = 2.0 * PI;
+#pragma unroll
+    for (int i = 0; i < MAXO; i++) {
+        int o = threadIdx.x + i * NT;
+        if (o >= 1 && o <= a.V) {
+            long long q = qs + o;
+            if (q < qend) {
+                double delta = ang[i] - buf[a.lp + o - 1].x;
+                if (delta > PI) delta -= TAU;
+                if (delta < -PI) delta += TAU;
+                long long s = (long long)f * a.spfp + (q - kOverlap);
+                a.rawdemod[s] = delta * a.scale;
+            }
+        }
+    }
+}
+
+// Part B: video and sync filters in one complex transform
+// (real input, kernel g_video + i * g_sync -> real part demod, imag demod_05).
+template <int B>
+__global__ __launch_bounds__(B / 16) void k1_part_b(K1BLaunch a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    double2* buf = reinterpret_cast<double2*>(smem);
+    constexpr int NT = B / 16;
+    const int f = blockIdx.y;
+    const int b = blockIdx.x;
+    const long long qs = (long long)kOverlap + (long long)b * a.V;
+    const long long qend = (long long)kOverlap + a.spfp;
+    if (qs >= qend) return;
+    const long long seg0 = (long long)f * a.spfp - kOverlap;
+    for (int j = threadIdx.x; j < B; j += NT) {
+        long long q = qs - a.lp + j;
+        long long s = seg0 + wrap_mod(q, a.N);
+        if (s < 0) s = 0;
+        if (s >= a.total) s = a.total - 1;
+        double2 v;
+        v.x = a.rawdemod[s];
+        v.y = 0.0;
+        buf[j] = v;
+    }
+    __syncthreads();
+    fft_lds<-1, double, B>(buf, a.tw);
+    for (int j = threadIdx.x; j < B; j += NT) buf[j] = cmul(buf[j], a.G[j]);
+    __syncthreads();
+    fft_lds<+1, double, B>(buf, a.tw);
+    for (int o = threadIdx.x; o < a.V; o += NT) {
+        long long q = qs + o;
+        if (q >= qend) break;
+        double2 z = buf[a.lp + o];
+        long long s = (long long)f * a.spfp + (q - kOverlap);
+        a.demod[s] = z.x;
+        a.demod05[s] = z.y;
+    }
+}
+
+// Chroma pre-band-pass: segment f is raw[f*spfp, (f+1)*spfp) zero-padded to N2
+// and filtered circularly (chroma_decode.cu:606-627).
+template <int B>
+__global__ __launch_bounds__(B / 16) void k_prebpf(PreLaunch a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    float2* buf = reinterpret_cast<float2*>(smem);
+    constexpr int NT = B / 16;
+    const int f = blockIdx.y;
+    const int b = blockIdx.x;
+    const long long ns = (long long)b * a.V;
+    if (ns >= a.spfp) return;
+    const long long seg0 = (long long)f * a.spfp;
+    for (int j = threadIdx.x; j < B; j += NT) {
+        long long n = wrap_mod(ns - a.lp + j, a.N2);
+        float2 v;
+        v.x = n < a.spfp ? (float)u8_to_s16(a.cap, seg0 + n, a.n_valid) : 0.0f;
+        v.y = 0.0f;
+        buf[j] = v;
+    }
+    __syncthreads();
+    fft_lds<-1, float, B>(buf, a.tw);
+    for (int j = threadIdx.x; j < B; j += NT) buf[j] = cmul(buf[j], a.G[j]);
+    __syncthreads();
+    fft_lds<+1, float, B>(buf, a.tw);
+    for (int o = threadIdx.x; o < a.V; o += NT) {
+        long long n = ns + o;
+        if (n >= a.spfp) break;
+        a.out[seg0 + n] = buf[a.lp + o].x;
+    }
+}
+
+__global__ void k_fill_f64(double* p, long long n, double v) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long st = (long long)gridDim.x * blockDim.x;
+    for (; i < n; i += st) p[i] = v;
+}
+
+// ---- host launchers ----
+template <int B>
+static hipError_t launch_a(const K1ALaunch& a, int nf, hipStream_t st) {
+    int nb = (int)((a.spfp + a.V - 1) / a.V);
+    size_t sh = (size_t)B * sizeof(double2);
+    (void)hipFuncSetAttribute((const void*)k1_part_a<B>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
+    hipLaunchKernelGGL(k1_part_a<B>, dim3(nb, nf), dim3(B / 16), sh, st, a);
+    return hipGetLastError();
+}
+template <int B>
+static hipError_t launch_b(const K1BLaunch& a, int nf, hipStream_t st) {
+    int nb = (int)((a.spfp + a.V - 1) / a.V);
+    size_t sh = (size_t)B * sizeof(double2);
+    (void)hipFuncSetAttribute((const void*)k1_part_b<B>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
+    hipLaunchKernelGGL(k1_part_b<B>, dim3(nb, nf), dim3(B / 16), sh, st, a);
+    return hipGetLastError();
+}
+template <int B>
+static hipError_t launch_pre(const PreLaunch& a, int nf, hipStream_t st) {
+    int nb = (int)((a.spfp + a.V - 1) / a.V);
+    size_t sh = (size_t)B * sizeof(float2);
+    (void)hipFuncSetAttribute((const void*)k_prebpf<B>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
+    hipLaunchKernelGGL(k_prebpf<B>, dim3(nb, nf), dim3(B / 16), sh, st, a);
+    return hipGetLastError();
+}
+
+hipError_t run_k1_part_a(int B, const K1ALaunch& a, int nf, hipStream_t st) {
+    if (B == 4096) return launch_a<4096>(a, nf, st);
+    if (B == 8192) return launch_a<8192>(a, nf, st);
+    return hipErrorInvalidValue;
+}
+hipError_t run_k1_part_b(int B, const K1BLaunch& a, int nf, hipStream_t st) {
+    if (B == 4096) return launch_b<4096>(a, nf, st);
+    if (B == 8192) return launch_b<8192>(a, nf, st);
+    return hipErrorInvalidValue;
+}
+hipError_t run_prebpf(int B, const PreLaunch& a, int nf, hipStream_t st) {
+    if (B == 8192) return launch_pre<8192>(a, nf, st);
+    if (B == 16384) return launch_pre<16384>(a, nf, st);
+    return hipErrorInvalidValue;
+}
+hipError_t run_fill_f64(double* p, long long n, double v, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_f64, dim3(1024), dim3(256), 0, st, p, n, v);
+    return hipGetLastError();
+}
+
+// FFT self-test: one LDS transform of B points (parity of the FFT building block).
+template <int B, typename T>
+__global__ __launch_bounds__(B / 16) void k_fft_selftest(const typename C2<T>::type* in, typename C2<T>::type* out,
+                                                         const typename C2<T>::type* tw, int sign) {
+    using V = typename C2<T>::type;
+    extern __shared__ __align__(16) unsigned char smem[];
+    V* buf = reinterpret_cast<V*>(smem);
+    for (int j = threadIdx.x; j < B; j += B / 16) buf[j] = in[j];
+    __syncthreads();
+    if (sign < 0) fft_lds<-1, T, B>(buf, tw);
+    else fft_lds<+1, T, B>(buf, tw);
+    for (int j = threadIdx.x; j < B; j += B / 16) out[j] = buf[j];
+}
+
+template <int B, typename T>
+static hipError_t st_launch(const void* in, void* out, const void* tw, int sign, hipStream_t st) {
+    using V = typename C2<T>::type;
+    size_t sh = (size_t)B * sizeof(V);
+    (void)hipFuncSetAttribute((const void*)k_fft_selftest<B, T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
+    hipLaunchKernelGGL((k_fft_selftest<B, T>), dim3(1), dim3(B / 16), sh, st, (const V*)in, (V*)out, (const V*)tw,
+                       sign);
+    return hipGetLastError();
+}
+
+hipError_t run_fft_selftest(int B, int is_f32, const void* in, void* out, const void* tw, int sign, hipStream_t st) {
+    if (!is_f32) {
+        if (B == 1024) return st_launch<1024, double>(in, out, tw, sign, st);
+        if (B == 4096) return st_launch<4096, double>(in, out, tw, sign, st);
+        if (B == 8192) return st_launch<8192, double>(in, out, tw, sign, st);
+    } else {
+        if (B == 1024) return st_launch<1024, float>(in, out, tw, sign, st);
+        if (B == 8192) return st_launch<8192, float>(in, out, tw, sign, st);
+        if (B == 16384) return st_launch<16384, float>(in, out, tw, sign, st);
+    }
+    return hipErrorInvalidValue;
+}
+
+}  // namespace cuvhs

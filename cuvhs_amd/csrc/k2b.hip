@@ -1,0 +1,582 @@
+// K2b line location on the GPU.
+//
+// Reference: the host path pipeline.cu:410-893 (functions) and its driver
+// :1694-1791.  The reference runs it serially on the CPU after copying every
+// pulse list to the host and copying line locations back.  Here:
+//   k_k2b_field   one workgroup per field: pulse refinement with the vblank
+//                 state machine, mean line length, vblank pulse scan and the
+//                 first-HSYNC anchor for both field parities and both
+//                 "no previous anchor" cases -- everything that does not
+//                 depend on the previous field.  Data stays in LDS.
+//   k_k2b_chain   one thread: the cross-field cadence (pipeline.h:92-97),
+//                 O(1) per field except the rare estimate-only anchor.
+//   k_k2b_linelocs one workgroup per field: sort pulse starts (bitonic) and
+//                 snap the 525 line locations (pipeline.cu:857-893).
+// The statement order of every floating-point expression follows the
+// reference so results round identically.
+#include <hip/hip_runtime.h>
+
+#include "common.h"
+#include "launch.h"
+
+namespace cuvhs {
+
+struct VP {            // validated pulse, structure of arrays in LDS/global
+    int* start;
+    int* len;
+    unsigned char* tg;  // type | good << 4
+};
+
+struct Timings {
+    double hsync_min, hsync_max, eq_min, eq_max, vsync_min, vsync_max;
+};
+
+struct AnchorVariant {
+    int count;              // valid_location_count (0 = none)
+    double loc;             // first_hsync_loc after rounding and hsync offset
+    double loc_rounded;     // first_hsync_loc after rounding, before the hsync offset
+};
+
+// Per-field summary written by k_k2b_field.
+struct K2bField {
+    int nvp;                // validated pulses (possibly > 800)
+    int npulses_k2;         // K2 count (capped at 800)
+    int overflow;
+    int id_zero;            // interlaced_detected == 0
+    int round_consensus1;   // round(ic / max(1, id)) == 1
+    double ic, id;
+    double meanlinelen;
+    int first_k2_start;
+    AnchorVariant var[2][2];  // [first_field][prev_loc_nonpositive]
+    int vblank_pulses[8];
+};
+
+struct K2bAnchorOut {
+    int valid;
+    int first_field;
+    double first_hsync_loc;
+    double first_hsync_line;
+    double meanlinelen;
+};
+
+__device__ __forceinline__ bool inr(double v, double lo, double hi) { return v >= lo && v <= hi; }
+
+__device__ __forceinline__ bool qualitycheck(int prev_type, int type, int start_delta, int in_line_len) {
+    double mn, mx;
+    if (prev_type > 0 && type > 0) { mn = 0.4; mx = 0.6; }
+    else if (prev_type == 0 && type == 0) { mn = 0.9; mx = 1.1; }
+    else { mn = 0.4; mx = 1.1; }
+    double span = (double)start_delta / (double)in_line_len;
+    return span >= mn && span <= mx;
+}
+
+__device__ __forceinline__ double round_line_loc(double line_number) {
+    return round(0.5 * round(line_number / 0.5) * 10.0) / 10.0;
+}
+
+// host_run_vblank_state_machine (pipeline.cu:519-609) appending to vp[n..)
+// Returns done; *n_out is the new length (entries past the cap are dropped
+// and flagged).
+__device__ bool vblank_state_machine(const int* rs, const int* rl, int npc, int start_idx, const Timings& lt,
+                                     int num_pulses, int in_line_len, VP vp, int n, int* n_out, int* ovf) {
+    bool done = false;
+    double num_pulses_half = num_pulses / 2.0;
+    int vsync_start = -1;
+    double state_end = 0.0;
+    double state_length = -1.0;
+    for (int i = start_idx; i < npc && i < start_idx + 26; i++) {
+        int ps = rs[i], pl = rl[i];
+        double len = (double)pl;
+        int type = -1;
+        bool have = false;
+        int state = n == 0 ? -1 : (vp.tg[n - 1] & 15);
+        if (state == -1) {
+            if (inr(len, lt.hsync_min, lt.hsync_max)) { type = kPulseHsync; have = true; }
+        } else if (state == kPulseHsync) {
+            if (inr(len, lt.hsync_min, lt.hsync_max)) { type = kPulseHsync; have = true; }
+            else if (inr(len, lt.eq_min, lt.eq_max)) { type = kPulseEq1; state_length = num_pulses_half; have = true; }
+            else if (inr(len, lt.vsync_min, lt.vsync_max)) { vsync_start = n - 1; type = kPulseVsync; have = true; }
+        } else if (state == kPulseEq1) {
+            if (inr(len, lt.eq_min, lt.eq_max)) { type = kPulseEq1; have = true; }
+            else if (inr(len, lt.vsync_min, lt.vsync_max)) { vsync_start = n - 1; type = kPulseVsync; state_length = num_pulses_half; have = true; }
+            else if (inr(len, lt.hsync_min, lt.hsync_max)) { type = kPulseHsync; have = true; }
+        } else if (state == kPulseVsync) {
+            if (inr(len, lt.eq_min, lt.eq_max)) { type = kPulseEq2; state_length = num_pulses_half; have = true; }
+            else if (inr(len, lt.vsync_min, lt.vsync_max)) { type = kPulseVsync; have = true; }
+            else if ((double)ps > state_end && inr(len, lt.hsync_min, lt.hsync_max)) { type = kPulseHsync; have = true; }
+        } else if (state == kPulseEq2) {
+            if (inr(len, lt.eq_min, lt.eq_max)) { type = kPulseEq2; have = true; }
+            else if (inr(len, lt.hsync_min, lt.hsync_max)) { type = kPulseHsync; have = true; done = true; }
+        }
+        if (have && type != state) {
+            if ((double)ps < state_end) {
+                have = false;
+            } else if (state_length > 0.0) {
+                state_end = (double)ps + ((state_length - 0.1) * (double)in_line_len);
+                state_length = -1.0;
+            }
+        }
+        if (have) {
+            bool good = n == 0 ? false : qualitycheck(vp.tg[n - 1] & 15, type, ps - vp.start[n - 1], in_line_len);
+            if (n < kVpCap) {
+                vp.start[n] = ps;
+                vp.len[n] = pl;
+                vp.tg[n] = (unsigned char)(type | (good ? 16 : 0));
+                n++;
+            } else {
+                *ovf = 1;
+            }
+        }
+        if (done && vsync_start >= 0) { *n_out = n; return true; }
+    }
+    *n_out = n;
+    return done;
+}
+
+// first-HSYNC anchor given the field parity (pipeline.cu:747-845, without the
+// estimate-only branch which needs the previous field)
+__device__ AnchorVariant anchor_variant(VP vp, int nvp, const int* vbp, double mll, bool first_field,
+                                        bool prev_nonpos, int num_eq, int fl0, int fl1) {
+    AnchorVariant r;
+    r.count = 0;
+    r.loc = 0.0;
+    r.loc_rounded = 0.0;
+    const double TOL = 0.5;
+    double ffl[4] = {1, 0.5, 0.5, 1};
+    double sfl[4] = {0.5, 1, 1, 0.5};
+    double* cfl = first_field ? ffl : sfl;
+    double vsl = num_eq / 2.0;
+    double cur_lines = first_field ? fl0 : fl1;
+    double vl[8];
+    vl[0] = 0.0 + cfl[0];
+    vl[1] = vl[0] + vsl;
+    vl[2] = vl[1] + vsl;
+    vl[3] = vl[2] + vsl - 0.5;
+    double hsl = vl[3] + cfl[1];
+    vl[4] = cur_lines + cfl[2];
+    vl[5] = vl[4] + vsl;
+    vl[6] = vl[5] + vsl;
+    vl[7] = vl[6] + vsl - 0.5;
+    auto calc = [&](int a, int b, double& loc, int& cnt, double& off) {
+        int p1 = vbp[a], p2 = vbp[b];
+        if (p1 == -1 || p2 == -1 || mll == 0.0) return;
+        double actual = ((double)p1 - (double)p2) / mll;
+        double expected = vl[a] - vl[b];
+        if (actual < expected + TOL && actual > expected - TOL) {
+            off += actual - expected;
+            loc += (double)p2 + mll * (hsl - vl[b]);
+            cnt += 1;
+        }
+    };
+    double fv_loc = 0.0, fv_off = 0.0, lv_loc = 0.0, lv_off = 0.0;
+    int fv_n = 0, lv_n = 0;
+    for (int a = 0; a < 4; a++)
+        for (int b = a + 1; b < 4; b++) calc(a, b, fv_loc, fv_n, fv_off);
+    for (int a = 0; a < 4; a++)
+        for (int b = a + 1; b < 4; b++) calc(4 + a, 4 + b, lv_loc, lv_n, lv_off);
+    double loc = 0.0, off = 0.0;
+    int cnt = 0;
+    double fe = fv_n ? fv_loc / fv_n : 0.0;
+    double le = lv_n ? lv_loc / lv_n : 0.0;
+    if (fv_n && lv_n && fe < le + TOL * mll && fe > le - TOL * mll) {
+        loc = fv_loc + lv_loc;
+        cnt = fv_n + lv_n;
+        off = fv_off + lv_off;
+        for (int a = 0; a < 4; a++)
+            for (int b = 0; b < 4; b++) calc(a, 4 + b, loc, cnt, off);
+    } else if (fv_n == 6 || (prev_nonpos && fv_n > lv_n)) {
+        loc = fv_loc; cnt = fv_n; off = fv_off;
+    } else if (lv_n == 6 || (prev_nonpos && lv_n > fv_n)) {
+        loc = lv_loc; cnt = lv_n; off = lv_off;
+    }
+    if (cnt <= 0) return r;
+    off /= cnt;
+    loc = round((loc + off) / cnt);
+    r.loc_rounded = loc;
+    double hoff = 0.0;
+    int hc = 0;
+    for (int i = 0; i < nvp; i++) {
+        int tg = vp.tg[i];
+        if ((tg & 15) != kPulseHsync || !(tg & 16)) continue;
+        double lineloc = ((double)vp.start[i] - loc) / mll + hsl;
+        int rl = (int)llround(lineloc);
+        if (rl > cur_lines) break;
+        if (rl >= hsl) {
+            hoff += loc + mll * ((double)rl - hsl) - (double)vp.start[i];
+            hc++;
+        }
+    }
+    if (hc > 0) {
+        hoff /= hc;
+        loc -= hoff;
+    }
+    r.count = cnt;
+    r.loc = loc;
+    return r;
+}
+
+// One workgroup (64 threads) per field.
+__global__ __launch_bounds__(64) void k_k2b_field(const ChunkCtl* ctl, int nf_override, const int* k2_starts,
+                                                  const int* k2_lens, const int* k2_count, double tol_half_us,
+                                                  double hsync_width, double eq_width, double vsync_width,
+                                                  int spl, int num_eq, int fl0, int fl1, int* g_vp_start,
+                                                  int* g_vp_len, unsigned char* g_vp_tg, K2bField* summ) {
+    __shared__ int rs[kMaxPulses], rl[kMaxPulses];
+    __shared__ int vs[kVpCap], vlen[kVpCap];
+    __shared__ unsigned char vtg[kVpCap];
+    __shared__ int s_nvp;
+    __shared__ int s_vbp[8];
+    __shared__ double s_mll;
+    const int f = blockIdx.x;
+    const int nf = nf_override > 0 ? nf_override : ctl->nf;
+    if (f >= nf) return;
+    const int npc = min(max(k2_count[f], 0), kMaxPulses);
+    for (int i = threadIdx.x; i < npc; i += 64) {
+        rs[i] = k2_starts[(size_t)f * kMaxPulses + i];
+        rl[i] = k2_lens[(size_t)f * kMaxPulses + i];
+    }
+    __syncthreads();
+    VP vp{vs, vlen, vtg};
+    K2bField& S = summ[f];
+    if (threadIdx.x == 0) {
+        int ovf = 0;
+        int n = 0;
+        // host_get_line_timings (pipeline.cu:450-477)
+        Timings lt;
+        double tol = tol_half_us;
+        double gmin = hsync_width - 3.5 * tol, gmax = hsync_width + 4.0 * tol;
+        double sum = 0.0;
+        int cnt = 0;
+        for (int i = 0; i < npc; i++) {
+            double len = (double)rl[i];
+            if (len >= gmin && len <= gmax) { sum += len; cnt++; }
+        }
+        double est = cnt > 0 ? (sum / cnt) : hsync_width;
+        double hoff = est - hsync_width;
+        double nom = hsync_width + hoff;
+        lt.hsync_min = nom - tol;
+        lt.hsync_max = nom + tol;
+        lt.eq_min = eq_width + hoff - tol;
+        lt.eq_max = eq_width + hoff + tol;
+        lt.vsync_min = (vsync_width + hoff) * 0.5;
+        lt.vsync_max = vsync_width + hoff + 2.0 * tol;
+        // host_refine_pulses_exactish (pipeline.cu:611-648)
+        for (int i = 0; i < npc;) {
+            int cs = rs[i], cl = rl[i];
+            if (inr((double)cl, lt.hsync_min, lt.hsync_max)) {
+                bool good = n > 0 && qualitycheck(vtg[n - 1] & 15, kPulseHsync, cs - vs[n - 1], spl);
+                if (n < kVpCap) {
+                    vs[n] = cs; vlen[n] = cl; vtg[n] = (unsigned char)(kPulseHsync | (good ? 16 : 0)); n++;
+                } else {
+                    ovf = 1;
+                }
+                i++;
+            } else if (i > 2 && inr((double)cl, lt.eq_min, lt.eq_max) && n > 0 && (vtg[n - 1] & 15) == kPulseHsync) {
+                int n2 = n;
+                bool done = vblank_state_machine(rs, rl, npc, i - 2, lt, num_eq, spl, vp, n, &n2, &ovf);
+                if (done && n2 > n) n = n2;  // keep the appended sequence
+                i++;                          // advances by one either way (pipeline.cu:637-642)
+            } else {
+                i++;
+            }
+        }
+        // host_compute_meanlinelen_from_valid_pulses (pipeline.cu:479-517)
+        int lr_start = -1, lr_len = -1, cr_start = -1, cr_len = 0;
+        for (int i = 0; i < n; i++) {
+            if ((vtg[i] & 15) != kPulseHsync) {
+                if (cr_start >= 0 && cr_len > lr_len) { lr_start = cr_start; lr_len = cr_len; }
+                cr_start = -1;
+                cr_len = 0;
+            } else if (cr_start < 0) {
+                cr_start = i;
+                cr_len = 0;
+            } else {
+                cr_len++;
+            }
+        }
+        if (cr_start >= 0 && cr_len > lr_len) { lr_start = cr_start; lr_len = cr_len; }
+        double nominal = (double)spl;
+        double lsum = 0.0;
+        int lcnt = 0;
+        if (lr_start >= 1 && lr_len >= 1) {
+            for (int i = lr_start + 1; i < lr_start + lr_len; i++) {
+                double ll = (double)(vs[i] - vs[i - 1]);
+                double ratio = ll / nominal;
+                if (ratio >= 0.95 && ratio <= 1.05) { lsum += ll; lcnt++; }
+            }
+        }
+        double mll = lcnt == 0 ? nominal : lsum / (double)lcnt;
+        // vblank pulse scan (pipeline.cu:675-727)
+        double fol[4] = {-1, -1, -1, -1};
+        int vbp[8] = {-1, -1, -1, -1, -1, -1, -1, -1};
+        int last = -1, group = 0, fgroup = 0;
+        for (int i = 0; i < n; i++) {
+            if (last != -1 && (vtg[i] & 16)) {
+                if (group == 0 && vs[i] > vs[0] + fl0 * mll) { group = 4; fgroup = 2; }
+                int lt_ = vtg[last] & 15, ty = vtg[i] & 15;
+                if (lt_ == kPulseHsync && ty > kPulseHsync) {
+                    vbp[0 + group] = vs[i];
+                    fol[0 + fgroup] = round_line_loc((vs[i] - vs[last]) / mll);
+                } else if (lt_ == kPulseEq1 && ty == kPulseVsync) {
+                    vbp[1 + group] = vs[i];
+                } else if (lt_ == kPulseVsync && ty == kPulseEq2) {
+                    vbp[2 + group] = vs[i];
+                } else if (lt_ > kPulseHsync && ty == kPulseHsync) {
+                    vbp[3 + group] = vs[last];
+                    fol[1 + fgroup] = round_line_loc((vs[i] - vs[last]) / mll);
+                }
+            }
+            last = i;
+        }
+        double ffl[4] = {1, 0.5, 0.5, 1}, sfl[4] = {0.5, 1, 1, 0.5};
+        double ic = 0.0, id = 0.0;
+        for (int i = 0; i < 4; i++) {
+            if (fol[i] == ffl[i]) { ic += 1.0; id += 1.0; }
+            if (fol[i] == sfl[i]) id += 1.0;
+        }
+        S.nvp = n;
+        S.npulses_k2 = npc;
+        S.overflow = ovf;
+        S.ic = ic;
+        S.id = id;
+        S.id_zero = id == 0.0;
+        S.round_consensus1 = round(ic / fmax(1.0, id)) == 1.0;
+        S.meanlinelen = mll;
+        S.first_k2_start = npc > 0 ? rs[0] : 0;
+        for (int k = 0; k < 8; k++) { S.vblank_pulses[k] = vbp[k]; s_vbp[k] = vbp[k]; }
+        s_nvp = n;
+        s_mll = mll;
+    }
+    __syncthreads();
+    const int n = s_nvp;
+    if (threadIdx.x < 4) {
+        bool ff = (threadIdx.x & 1) != 0;
+        bool pn = (threadIdx.x & 2) != 0;
+        S.var[ff ? 1 : 0][pn ? 1 : 0] = anchor_variant(vp, n, s_vbp, s_mll, ff, pn, num_eq, fl0, fl1);
+    }
+    for (int i = threadIdx.x; i < n; i += 64) {
+        g_vp_start[(size_t)f * kVpCap + i] = vs[i];
+        g_vp_len[(size_t)f * kVpCap + i] = vlen[i];
+        g_vp_tg[(size_t)f * kVpCap + i] = vtg[i];
+    }
+}
+
+// Cross-field cadence (pipeline.cu:1717-1772, :729-745, :811-827), one thread.
+__global__ void k_k2b_chain(ChunkCtl* ctl, int nf_override, const unsigned long long* offsets,
+                            const K2bField* summ, const int* g_vp_start, const unsigned char* g_vp_tg,
+                            K2bAnchorOut* anchors, int* is_first, int num_eq, int fl0, int fl1) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const int nf = nf_override > 0 ? nf_override : ctl->nf;
+    K2bState st = ctl->k2b;
+    const unsigned long long raw_offset = ctl->raw_offset;
+    int ovf = 0;
+    for (int f = 0; f < nf; f++) {
+        const K2bField& S = summ[f];
+        ovf |= S.overflow;
+        K2bAnchorOut A;
+        A.valid = 0;
+        A.first_field = -1;
+        A.first_hsync_loc = 0.0;
+        A.first_hsync_line = 0.0;
+        double mll = S.meanlinelen;
+        A.meanlinelen = mll;
+        double prev_offset_lines = 0.0;
+        if (st.prev_first_hsync_readloc != -1 && mll > 0.0)
+            prev_offset_lines = ((double)st.prev_first_hsync_readloc - (double)(raw_offset + offsets[f])) / mll;
+        bool valid = false;
+        bool first_field = false;
+        double loc = 0.0, hsl = 0.0, new_diff = st.prev_first_hsync_diff;
+        if (S.nvp > 0 && mll > 0.0) {
+            if (st.prev_first_field == -1) first_field = S.id_zero || S.round_consensus1;
+            else first_field = !st.prev_first_field;
+            int conf = 75;
+            if (S.id > 0.0) {
+                if (st.prev_first_hsync_loc < 0.0) conf = min(conf, 50);
+                double weighting = S.id / 4.0;
+                int ffc = (int)llround((S.ic / S.id) * weighting * 100.0);
+                int sfc = (int)llround(((S.id - S.ic) / S.id) * weighting * 100.0);
+                if (ffc >= conf && ffc > sfc) first_field = true;
+                else if (sfc >= conf && sfc > ffc) first_field = false;
+            }
+            double ffl[4] = {1, 0.5, 0.5, 1}, sfl[4] = {0.5, 1, 1, 0.5};
+            double* cfl = first_field ? ffl : sfl;
+            double vsl = num_eq / 2.0;
+            hsl = 0.0 + cfl[0] + vsl + vsl + vsl - 0.5 + cfl[1];
+            {   // same association as the reference's chained vblank_lines
+                double v0 = 0.0 + cfl[0];
+                double v1 = v0 + vsl;
+                double v2 = v1 + vsl;
+                double v3 = v2 + vsl - 0.5;
+                hsl = v3 + cfl[1];
+            }
+            double cur_lines = first_field ? fl0 : fl1;
+            double prev_lines = first_field ? fl1 : fl0;
+            bool prev_nonpos = st.prev_first_hsync_loc <= 0.0;
+            const AnchorVariant& V = S.var[first_field ? 1 : 0][prev_nonpos ? 1 : 0];
+            int est_loc = (int)llround((prev_offset_lines + prev_lines + st.prev_first_hsync_loc / mll) * mll);
+            if (V.count > 0) {
+                valid = true;
+                loc = V.loc;
+                new_diff = (V.loc_rounded - est_loc) / mll;
+            } else if (st.prev_first_hsync_loc > 0.0) {
+                // estimate-only anchor (pipeline.cu:814-844)
+                double ewo = (st.prev_first_hsync_diff <= 0.5 && st.prev_first_hsync_diff >= -0.5)
+                                 ? est_loc + mll * st.prev_first_hsync_diff
+                                 : (double)est_loc;
+                if (ewo <= 0.0) ewo = (double)g_vp_start[(size_t)f * kVpCap + 0];
+                double l = 0.0 + ewo;
+                l = round((l + 0.0 / 1) / 1);
+                double hoff = 0.0;
+                int hc = 0;
+                for (int i = 0; i < S.nvp && i < kVpCap; i++) {
+                    int tg = g_vp_tg[(size_t)f * kVpCap + i];
+                    if ((tg & 15) != kPulseHsync || !(tg & 16)) continue;
+                    double ps = (double)g_vp_start[(size_t)f * kVpCap + i];
+                    double lineloc = (ps - l) / mll + hsl;
+                    int rl = (int)llround(lineloc);
+                    if (rl > cur_lines) break;
+                    if (rl >= hsl) {
+                        hoff += l + mll * ((double)rl - hsl) - ps;
+                        hc++;
+                    }
+                }
+                if (hc > 0) {
+                    hoff /= hc;
+                    l -= hoff;
+                }
+                valid = true;
+                loc = l;
+                new_diff = st.prev_first_hsync_diff;
+            }
+        }
+        if (valid) {
+            A.valid = 1;
+            A.first_field = first_field ? 1 : 0;
+            A.first_hsync_loc = loc;
+            A.first_hsync_line = hsl;
+            is_first[f] = A.first_field;
+            st.prev_first_field = A.first_field;
+            st.prev_progressive_field = 0;
+            st.prev_first_hsync_readloc = (long long)(raw_offset + offsets[f]);
+            st.prev_first_hsync_loc = loc;
+            st.prev_first_hsync_diff = new_diff;
+        } else {
+            is_first[f] = (st.prev_first_field >= 0) ? (1 - st.prev_first_field) : 1;
+        }
+        anchors[f] = A;
+    }
+    ctl->k2b = st;
+    ctl->vp_overflow = ovf;
+}
+
+// Line locations + pulse outputs, one workgroup (256 threads) per field.
+__global__ __launch_bounds__(256) void k_k2b_linelocs(const ChunkCtl* ctl, int nf_override, const K2bField* summ,
+                                                      const K2bAnchorOut* anchors, const int* g_vp_start,
+                                                      const int* g_vp_len, const unsigned char* g_vp_tg,
+                                                      int* out_starts, int* out_lens, int* out_count,
+                                                      int* out_types, double* linelocs) {
+    __shared__ int key[kVpCap];
+    __shared__ double s_ll[kLinesPerFrame];
+    const int f = blockIdx.x;
+    const int nf = nf_override > 0 ? nf_override : ctl->nf;
+    if (f >= nf) return;
+    const K2bField& S = summ[f];
+    const K2bAnchorOut A = anchors[f];
+    const int n = min(S.nvp, kVpCap);
+    // pulse outputs (pipeline.cu:1739-1750)
+    const int after = min(n, kMaxPulses);
+    for (int i = threadIdx.x; i < kMaxPulses; i += 256) {
+        size_t o = (size_t)f * kMaxPulses + i;
+        if (i < after) {
+            size_t g = (size_t)f * kVpCap + i;
+            out_starts[o] = g_vp_start[g];
+            out_lens[o] = g_vp_len[g];
+            out_types[o] = g_vp_tg[g] & 15;
+        } else {
+            out_starts[o] = 0;
+            out_lens[o] = 0;
+            out_types[o] = kPulseHsync;
+        }
+    }
+    if (threadIdx.x == 0) out_count[f] = after;
+    double* ll = linelocs + (size_t)f * kLinesPerFrame;
+    const double mll = S.meanlinelen;
+    if (!A.valid) {
+        double ref = (double)S.first_k2_start;
+        if (S.npulses_k2 <= 0) ref = 0.0;
+        for (int line = threadIdx.x; line < kLinesPerFrame; line += 256) ll[line] = ref + mll * (double)line;
+        return;
+    }
+    // bitonic sort of the starts (pad with INT_MAX)
+    int P = 1;
+    while (P < n) P <<= 1;
+    for (int i = threadIdx.x; i < P; i += 256) key[i] = i < n ? g_vp_start[(size_t)f * kVpCap + i] : 0x7fffffff;
+    __syncthreads();
+    for (int k = 2; k <= P; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = threadIdx.x; i < P; i += 256) {
+                int ixj = i ^ j;
+                if (ixj > i) {
+                    int a = key[i], b = key[ixj];
+                    bool up = (i & k) == 0;
+                    if ((a > b) == up) { key[i] = b; key[ixj] = a; }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    if (threadIdx.x == 0) {
+        // host_valid_pulses_to_linelocs_exactish (pipeline.cu:857-893)
+        double max_allowed = mll / 1.5;
+        int cur = 0;
+        for (int line = 0; line < kLinesPerFrame; line++) {
+            double l = A.first_hsync_loc + mll * ((double)line - A.first_hsync_line);
+            s_ll[line] = l;
+            if (cur >= n) continue;
+            double cd = fabs((double)key[cur] - l);
+            double smallest = max_allowed;
+            double loc = -1.0;
+            int si = cur;
+            while (si + 1 < n) {
+                if (cd <= smallest) {
+                    smallest = cd;
+                    cur = si;
+                    loc = (double)key[si];
+                }
+                double nd = fabs((double)key[si + 1] - l);
+                if (nd > cd) break;
+                cd = nd;
+                si++;
+            }
+            if (loc >= 0.0) {
+                s_ll[line] = loc;
+                cur++;
+            }
+        }
+    }
+    __syncthreads();
+    for (int line = threadIdx.x; line < kLinesPerFrame; line += 256) ll[line] = s_ll[line];
+}
+
+// ---- host launchers ----
+hipError_t run_k2b(const ChunkCtl* ctl_dev, ChunkCtl* ctl_mut, int nf_override, int max_fields, const Fmt& fmt,
+                   const unsigned long long* offsets, const int* k2_starts, const int* k2_lens, const int* k2_count,
+                   int* vp_start, int* vp_len, unsigned char* vp_tg, K2bField* summ, K2bAnchorOut* anchors,
+                   int* out_starts, int* out_lens, int* out_count, int* out_types, double* linelocs, int* is_first,
+                   hipStream_t st) {
+    double tol = fmt.sample_rate * 0.5e-6;
+    hipLaunchKernelGGL(k_k2b_field, dim3(max_fields), dim3(64), 0, st, ctl_dev, nf_override, k2_starts, k2_lens,
+                       k2_count, tol, fmt.hsync_width, fmt.eq_pulse_width, fmt.vsync_width, fmt.samples_per_line,
+                       fmt.num_eq_pulses, fmt.field_lines_first, fmt.field_lines_second, vp_start, vp_len, vp_tg,
+                       summ);
+    hipLaunchKernelGGL(k_k2b_chain, dim3(1), dim3(64), 0, st, ctl_mut, nf_override, offsets, summ, vp_start, vp_tg,
+                       anchors, is_first, fmt.num_eq_pulses, fmt.field_lines_first, fmt.field_lines_second);
+    hipLaunchKernelGGL(k_k2b_linelocs, dim3(max_fields), dim3(256), 0, st, ctl_dev, nf_override, summ, anchors,
+                       vp_start, vp_len, vp_tg, out_starts, out_lens, out_count, out_types, linelocs);
+    return hipGetLastError();
+}
+
+size_t k2b_field_summary_bytes() { return sizeof(K2bField); }
+size_t k2b_anchor_bytes() { return sizeof(K2bAnchorOut); }
+
+}  // namespace cuvhs

@@ -1,0 +1,252 @@
+// K6 colour-under chroma decode.
+//
+// Reference: chroma_decode.cu:526-1002.  The reference resolves head-track
+// flips with a host loop: per 15-field sub-chunk it re-runs resample ->
+// 1024-point FFT band-pass -> inverse FFT -> burst metric up to three
+// times, copying one int to the host per pass, and detects the first
+// track and the NTSC phase from field 0 with two more host round trips.
+//
+// Here the decision logic needs only per-field burst metrics for both track
+// hypotheses.  For even output lines the heterodyne phase is the same for
+// both tracks; for odd lines switching track adds pi, i.e. negates the
+// line.  The band-pass is linear, so one pass over lines 16..246 with track 0
+// gives the track-0 metric (|a + b|) and the track-1 metric (|a - b|) of
+// every field.  A single-thread kernel then replays the reference's
+// detection, retry and state logic exactly, and the output pass recomputes
+// every line with the resolved track.
+#include <hip/hip_runtime.h>
+
+#include "common.h"
+#include "fft_lds.h"
+#include "launch.h"
+
+namespace cuvhs {
+
+constexpr int kBurstLen = 36;      // 112 - 76 at 4 fsc
+constexpr int kSkip = 16;          // chroma_decode.cu:255
+constexpr int kPairs = (kFieldLines - 2 * kSkip - 1) / 2;  // 115
+
+// One wavefront per line: resample + heterodyne (chroma_decode.cu:75-157),
+// FFT, band-pass (:163-182), inverse FFT, then either burst capture or
+// ACC + u16 (:188-231).
+template <int OUT>
+__global__ __launch_bounds__(64) void k_chroma_line(ChromaLaunch a) {
+    __shared__ __align__(16) double2 buf[kChromaFFT];
+    __shared__ double s_acc;
+    const int f = blockIdx.y;
+    const int nf = a.nf_override > 0 ? a.nf_override : a.ctl->nf;
+    if (f >= nf) return;
+    const int line = a.line0 + blockIdx.x;
+    const long long total = (long long)nf * a.spfp;
+    const double* ll = a.linelocs + (size_t)f * kLinesPerFrame;
+    const double ls = ll[line], le = ll[line + 1];
+    const int tr = a.track ? a.track[f] : 0;
+    const int rot = tr ? 1 : 3;
+    const int line_phase = (line * rot + 0) & 3;
+    const double phase_offset = (double)line_phase * (M_PI * 0.5);
+    for (int col = threadIdx.x; col < kChromaFFT; col += 64) {
+        double v = 0.0;
+        if (col < kLineLen) {
+            double frac = (double)col / (double)kLineLen;
+            double coord = ls + frac * (le - ls);
+            int ci = (int)coord;
+            double x = coord - (double)ci;
+            if (!(ci < 1 || (long long)ci + 2 >= total)) {
+                double p0 = a.craw[ci - 1], p1 = a.craw[ci], p2 = a.craw[ci + 1], p3 = a.craw[ci + 2];
+                double aa = p2 - p0;
+                double bb = 2.0 * p0 - 5.0 * p1 + 4.0 * p2 - p3;
+                double cc = 3.0 * (p1 - p2) + p3 - p0;
+                double raw_val = p1 + 0.5 * x * (aa + x * (bb + x * cc));
+                double abs_sample = (double)(line * kLineLen + col);
+                double het = -cos(2.0 * M_PI * a.het_scale * abs_sample + phase_offset);
+                v = raw_val * het;
+            }
+        }
+        double2 z;
+        z.x = v;
+        z.y = 0.0;
+        buf[col] = z;
+    }
+    __syncthreads();
+    fft_lds<-1, double, kChromaFFT>(buf, a.tw);
+    for (int k = threadIdx.x; k < kChromaFFT; k += 64) {
+        int kk = k <= kChromaFFT / 2 ? k : kChromaFFT - k;
+        if (kk < a.lo || kk > a.hi) {
+            double2 z;
+            z.x = 0.0;
+            z.y = 0.0;
+            buf[k] = z;
+        }
+    }
+    __syncthreads();
+    fft_lds<+1, double, kChromaFFT>(buf, a.tw);
+    if (!OUT) {
+        double* dst = a.burst + ((size_t)f * kFieldLines + line) * kBurstLen;
+        for (int i = threadIdx.x; i < kBurstLen; i += 64) dst[i] = buf[a.burst_start + i].x * a.fft_scale;
+        return;
+    }
+    if (threadIdx.x == 0) {
+        double ssq = 0.0;
+        for (int i = 0; i < kBurstLen; i++) {
+            double v = buf[a.burst_start + i].x * a.fft_scale;
+            ssq += v * v;
+        }
+        double rms = sqrt(ssq / (double)kBurstLen);
+        s_acc = (rms > 1.0) ? (a.burst_abs_ref / rms) : 0.0;
+    }
+    __syncthreads();
+    const double acc = s_acc;
+    uint16_t* o = a.out + ((size_t)f * kFieldLines + line) * kLineLen;
+    for (int col = threadIdx.x; col < kLineLen; col += 64) {
+        double v = buf[col].x * a.fft_scale * acc;
+        double ov = v + 32768.0;
+        if (ov < 0.0) ov = 0.0;
+        if (ov > 65535.0) ov = 65535.0;
+        o[col] = (uint16_t)(ov + 0.5);
+    }
+}
+
+// Per-field burst-cancellation metrics for both tracks (chroma_decode.cu:383-411
+// summation order) and, for field 0, per-line burst I/Q (:415-450).
+__global__ __launch_bounds__(64) void k_chroma_metrics(const double* burst, const ChunkCtl* ctl, int nf_override,
+                                                       double* m0, double* m1, const double* cs_tab, double* iq0) {
+    const int nf = nf_override > 0 ? nf_override : ctl->nf;
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f < nf) {
+        double t0 = 0.0, t1 = 0.0;
+        for (int p = 0; p < kPairs; p++) {
+            const double* ra = burst + ((size_t)f * kFieldLines + kSkip + 2 * p) * kBurstLen;
+            const double* rb = ra + kBurstLen;
+            double s0 = 0.0, s1 = 0.0;
+            for (int i = 0; i < kBurstLen; i++) {
+                s0 += fabs(ra[i] + rb[i]);
+                s1 += fabs(ra[i] + (-rb[i]));
+            }
+            t0 += s0 / kBurstLen;
+            t1 += s1 / kBurstLen;
+        }
+        m0[f] = t0 / kPairs;
+        m1[f] = t1 / kPairs;
+    }
+    if (blockIdx.x == 0 && iq0) {
+        for (int line = kSkip + threadIdx.x; line < kFieldLines - kSkip; line += blockDim.x) {
+            const double* r = burst + (size_t)line * kBurstLen;
+            double I = 0.0, Q = 0.0;
+            for (int i = 0; i < kBurstLen; i++) {
+                I += r[i] * cs_tab[2 * i];
+                Q += r[i] * cs_tab[2 * i + 1];
+            }
+            iq0[2 * line] = I;
+            iq0[2 * line + 1] = Q;
+        }
+    }
+}
+
+__device__ int ntsc_phase_id_first(int quadrant) {
+    // lookup_ntsc_phase(1, quadrant, -1): chroma_decode.cu:337-377
+    switch (quadrant) {
+        case 2: return 3;
+        case 1: return 1;
+        case 0: return 3;
+        case 3: return 1;
+    }
+    return 1;
+}
+
+// Detection, retry replay and state update (chroma_decode.cu:674-1001).
+__global__ void k_chroma_resolve(ChunkCtl* ctl, int nf_override, const double* m0, const double* m1,
+                                 const double* iq0, int* out_track, int* phase_ids) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const int nf = nf_override > 0 ? nf_override : ctl->nf;
+    if (nf <= 0) return;
+    ChromaState cs = ctl->chroma;
+    const int cycle[8] = {3, 2, 1, 4, 3, 2, 1, 4};
+    int det, cycle_start;
+    double thr;
+    if (cs.valid) {
+        det = cs.current_track;
+        thr = cs.good_metric_threshold;
+        cycle_start = cs.cycle_start;
+    } else {
+        double metric[2] = {m0[0], m1[0]};
+        det = (metric[1] < metric[0]) ? 1 : 0;
+        thr = metric[det] * 4.0;
+        double It = 0.0, Qt = 0.0;
+        for (int line = kSkip; line < kFieldLines - kSkip; line++) {
+            double I = iq0[2 * line], Q = iq0[2 * line + 1];
+            if (det == 1 && (line & 1)) { I = -I; Q = -Q; }
+            double mag = sqrt(I * I + Q * Q);
+            if (mag > 0.0) { It += I / mag; Qt += Q / mag; }
+        }
+        double ph = atan2(Qt, It);
+        double deg = fmod(ph * 180.0 / M_PI + 360.0, 360.0);
+        int q0 = ((int)(deg / 90.0 + 0.5)) % 4;
+        int pid = ntsc_phase_id_first(q0);
+        cycle_start = 0;
+        for (int i = 0; i < 8; i += 2)
+            if (cycle[i] == pid) { cycle_start = i; break; }
+    }
+    int current = det;
+    for (int f = 0; f < nf; f++) {
+        out_track[f] = (f & 1) ? (1 - current) : current;
+        phase_ids[f] = cycle[(cycle_start + f) % 8];
+    }
+    // out_track doubles as h_track; the tracks of the last computed pass of
+    // each sub-chunk are what the output uses.
+    int fpc = (4096 / kFieldLines) * kFieldLines;
+    int tl = nf * kFieldLines;
+    int cl = tl < fpc ? tl : fpc;
+    fpc = cl / kFieldLines;
+    for (int fs = 0; fs < nf; fs += fpc) {
+        int fthis = min(fpc, nf - fs);
+        int used[16];
+        int retries = 3;
+        bool ok = false;
+        while (!ok && retries-- > 0) {
+            for (int i = 0; i < fthis; i++) used[i] = out_track[fs + i];
+            int first_bad = -1;
+            for (int i = 0; i < fthis; i++) {
+                double m = used[i] ? m1[fs + i] : m0[fs + i];
+                if (m > thr) { first_bad = i; break; }
+            }
+            if (first_bad < 0) {
+                ok = true;
+            } else {
+                current = 1 - current;
+                for (int ff = fs + first_bad; ff < nf; ff++) out_track[ff] = (ff & 1) ? (1 - current) : current;
+            }
+        }
+        // output uses the tracks of the last pass; later fields keep the flip
+        for (int i = 0; i < fthis; i++) out_track[fs + i] = used[i];
+        if (!ok) {
+            // the flip of the final pass applies to the following sub-chunks only
+            for (int ff = fs + fthis; ff < nf; ff++) out_track[ff] = (ff & 1) ? (1 - current) : current;
+        }
+    }
+    cs.valid = 1;
+    cs.current_track = (nf & 1) ? (1 - current) : current;
+    cs.good_metric_threshold = thr;
+    cs.cycle_start = (cycle_start + nf) % 8;
+    ctl->chroma = cs;
+}
+
+hipError_t run_chroma(ChromaLaunch a, ChunkCtl* ctl_mut, int max_fields, double* m0, double* m1,
+                      const double* cs_tab, double* iq0, int* out_track, int* phase_ids, int need_iq,
+                      hipStream_t st) {
+    // metric pass: lines 16..246, track 0
+    ChromaLaunch m = a;
+    m.track = nullptr;
+    m.line0 = kSkip;
+    hipLaunchKernelGGL(k_chroma_line<0>, dim3(kFieldLines - 2 * kSkip, max_fields), dim3(64), 0, st, m);
+    hipLaunchKernelGGL(k_chroma_metrics, dim3((max_fields + 63) / 64), dim3(64), 0, st, a.burst, a.ctl,
+                       a.nf_override, m0, m1, cs_tab, need_iq ? iq0 : nullptr);
+    hipLaunchKernelGGL(k_chroma_resolve, dim3(1), dim3(64), 0, st, ctl_mut, a.nf_override, m0, m1, iq0, out_track,
+                       phase_ids);
+    ChromaLaunch o = a;
+    o.track = out_track;
+    o.line0 = 0;
+    hipLaunchKernelGGL(k_chroma_line<1>, dim3(kFieldLines, max_fields), dim3(64), 0, st, o);
+    return hipGetLastError();
+}
+
+}  // namespace cuvhs

@@ -1,0 +1,241 @@
+// K7 dropout detection, TBC mapping and concealment, plus the per-field
+// output metadata (file location, field parity).
+//
+// Reference: dropout_detect.cu:17-339 (one thread per field scanning the
+// whole window serially; 32-lane shuffles for the mean) and
+// pipeline.cu:2575-2610 (parity alternation, fileLoc).
+//
+// Here one 256-thread workgroup handles a field: block means are computed in
+// parallel, the hysteresis state machine (start at mean <= 0.18*env, end at
+// mean >= 1.25 * that) is resolved with a scan of "last decisive class"
+// across threads, and only the short event list is processed serially.
+// Concealment keeps the reference's row copy (row l-1, or l+1 for row 0,
+// into row l over [startx, endx)) and applies entries in order; each thread
+// owns whole columns, so a row written by one entry and read by a later one
+// is handled by the same thread in program order.
+#include <hip/hip_runtime.h>
+
+#include "common.h"
+#include "launch.h"
+
+namespace cuvhs {
+
+constexpr int kDoThreads = 256;
+constexpr int kEnvBlock = 16;            // dropout_detect.cu:11
+constexpr int kMaxClassBlocks = 45056;   // >= spfp / 16 for captures up to 40 MHz
+constexpr int kMaxEvents = 2048;
+
+__device__ __forceinline__ int find_tbc_line(const double* ll, int als, int n_lines, double rf) {
+    if (rf < ll[als] || rf >= ll[als + n_lines]) return -1;
+    int lo = 0, hi = n_lines - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) / 2;
+        if (ll[als + mid] <= rf) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(kDoThreads) void k_dropouts(const double* __restrict__ env, const double* __restrict__ linelocs,
+                                                          const ChunkCtl* ctl, int nf_override, int spfp, int als,
+                                                          uint16_t* luma, uint16_t* chroma, int* do_lines,
+                                                          int* do_starts, int* do_ends, int* do_count) {
+    __shared__ unsigned char cls[kMaxClassBlocks];
+    __shared__ int ev[kMaxEvents];
+    __shared__ double red[kDoThreads];
+    __shared__ int s_last[kDoThreads];
+    __shared__ int s_cnt[kDoThreads];
+    __shared__ int s_ent[3 * kMaxDropouts];
+    __shared__ int s_nent;
+    __shared__ int s_nev;
+    const int f = blockIdx.x;
+    const int nf = nf_override > 0 ? nf_override : ctl->nf;
+    if (f >= nf) return;
+    const int t = threadIdx.x;
+    // field mean envelope over [f*spfp, (f+1)*spfp) (dropout_detect.cu:17-53)
+    const double* fe = env + (size_t)f * spfp;
+    double s = 0.0;
+    for (int i = t; i < spfp; i += kDoThreads) s += fe[i];
+    red[t] = s;
+    __syncthreads();
+    for (int o = kDoThreads / 2; o > 0; o >>= 1) {
+        if (t < o) red[t] += red[t + o];
+        __syncthreads();
+    }
+    const double mean_env = red[0] / (double)spfp;
+    const double down = mean_env * 0.18;
+    const double up = down * 1.25;
+    const double* fll = linelocs + (size_t)f * kLinesPerFrame;
+    const int field_start = f * spfp;
+    const int scan_start = max((int)fll[als], field_start);
+    const int scan_end = min((int)fll[als + kFieldLines], field_start + spfp);
+    if (t == 0) s_nent = 0;
+    if (scan_end > scan_start) {
+        const int nblk = (scan_end - scan_start + kEnvBlock - 1) / kEnvBlock;
+        const int nb = min(nblk, kMaxClassBlocks);
+        // classes: 0 neutral, 1 down (<= down), 2 up (>= up)
+        for (int b = t; b < nb; b += kDoThreads) {
+            int bs = scan_start + b * kEnvBlock;
+            int be = min(bs + kEnvBlock, scan_end);
+            double bsum = 0.0;
+            for (int i = bs; i < be; i++) bsum += env[i];
+            double bm = bsum / (double)(be - bs);
+            cls[b] = bm <= down ? 1 : (bm >= up ? 2 : 0);
+        }
+        __syncthreads();
+        const int per = (nb + kDoThreads - 1) / kDoThreads;
+        const int b0 = t * per, b1 = min(nb, b0 + per);
+        int last = 0;
+        for (int b = b0; b < b1; b++)
+            if (cls[b]) last = cls[b];
+        s_last[t] = last;
+        __syncthreads();
+        if (t == 0) {
+            int state = 2;  // not in a dropout
+            for (int k = 0; k < kDoThreads; k++) {
+                int l = s_last[k];
+                s_last[k] = state;  // incoming state of thread k
+                if (l) state = l;
+            }
+        }
+        __syncthreads();
+        int state = s_last[t];
+        int cnt = 0;
+        for (int b = b0; b < b1; b++) {
+            int c = cls[b];
+            if ((c == 1 && state == 2) || (c == 2 && state == 1)) cnt++;
+            if (c) state = c;
+        }
+        s_cnt[t] = cnt;
+        __syncthreads();
+        if (t == 0) {
+            int acc = 0;
+            for (int k = 0; k < kDoThreads; k++) { int c = s_cnt[k]; s_cnt[k] = acc; acc += c; }
+            s_nev = acc;
+        }
+        __syncthreads();
+        const int nev = s_nev;
+        if (nev <= kMaxEvents) {
+            int pos = s_cnt[t];
+            state = s_last[t];
+            for (int b = b0; b < b1; b++) {
+                int c = cls[b];
+                if ((c == 1 && state == 2) || (c == 2 && state == 1)) ev[pos++] = b;
+                if (c) state = c;
+            }
+        }
+        __syncthreads();
+        if (t == 0) {
+            // serial replay of dropout_detect.cu:121-182 on the event list
+            int rs[kMaxRfDropouts], re[kMaxRfDropouts];
+            int n = 0;
+            int rf_start = -1;
+            auto close_at = [&](int end_pos) {
+                if (n > 0 && rf_start - re[n - 1] <= 30) re[n - 1] = end_pos;
+                else if (n < kMaxRfDropouts) { rs[n] = rf_start; re[n] = end_pos; n++; }
+                rf_start = -1;
+            };
+            if (nev <= kMaxEvents) {
+                for (int k = 0; k < nev; k++) {
+                    int bpos = scan_start + ev[k] * kEnvBlock;
+                    if (rf_start < 0) rf_start = bpos;  // start event
+                    else close_at(bpos);                // end event
+                }
+            } else {
+                int st = 2;
+                for (int b = 0; b < nb; b++) {
+                    int c = cls[b];
+                    int bpos = scan_start + b * kEnvBlock;
+                    if (c == 1 && st == 2) rf_start = bpos;
+                    else if (c == 2 && st == 1) close_at(bpos);
+                    if (c) st = c;
+                }
+            }
+            if (rf_start >= 0) close_at(scan_end);
+            int m = 0;
+            for (int i = 0; i < n; i++)
+                if (re[i] - rs[i] >= 10) { rs[m] = rs[i]; re[m] = re[i]; m++; }
+            int tc = 0;
+            for (int i = 0; i < m && tc < kMaxDropouts; i++) {
+                double srf = (double)rs[i], erf = (double)re[i];
+                int sl = find_tbc_line(fll, als, kFieldLines, srf);
+                if (sl < 0) continue;
+                int el = find_tbc_line(fll, als, kFieldLines, erf - 1.0);
+                if (el < 0) el = kFieldLines - 1;
+                for (int l = sl; l <= el && tc < kMaxDropouts; l++) {
+                    double lls = fll[als + l], lle = fll[als + l + 1];
+                    double len = lle - lls;
+                    if (len <= 0.0) continue;
+                    int sx, ex;
+                    if (l == sl) { sx = (int)((srf - lls) / len * (double)kLineLen); sx = max(0, sx); }
+                    else sx = 0;
+                    if (l == el) { ex = (int)((erf - lls) / len * (double)kLineLen + 0.999); ex = min(kLineLen, ex); }
+                    else ex = kLineLen;
+                    if (ex > sx) {
+                        s_ent[3 * tc] = l;
+                        s_ent[3 * tc + 1] = sx;
+                        s_ent[3 * tc + 2] = ex;
+                        tc++;
+                    }
+                }
+            }
+            s_nent = tc;
+        }
+    }
+    __syncthreads();
+    const int ne = s_nent;
+    for (int i = t; i < ne; i += kDoThreads) {
+        size_t o = (size_t)f * kMaxDropouts + i;
+        do_lines[o] = s_ent[3 * i];
+        do_starts[o] = s_ent[3 * i + 1];
+        do_ends[o] = s_ent[3 * i + 2];
+    }
+    if (t == 0) do_count[f] = ne;
+    // concealment, column-owned threads, entries in order
+    uint16_t* lu = luma + (size_t)f * kFieldLines * kLineLen;
+    uint16_t* ch = chroma + (size_t)f * kFieldLines * kLineLen;
+    for (int d = 0; d < ne; d++) {
+        int line = s_ent[3 * d], sx = s_ent[3 * d + 1], ex = s_ent[3 * d + 2];
+        int src = (line > 0) ? line - 1 : line + 1;
+        if (src >= kFieldLines) continue;
+        for (int col = sx + t; col < ex; col += kDoThreads) {
+            lu[line * kLineLen + col] = lu[src * kLineLen + col];
+            ch[line * kLineLen + col] = ch[src * kLineLen + col];
+        }
+    }
+}
+
+// Output metadata: fileLoc and the alternating field parity
+// (pipeline.cu:2577-2593), phase IDs passed through.
+__global__ void k_field_meta(ChunkCtl* ctl, const unsigned long long* offsets, const int* is_first_k2b,
+                             unsigned long long* file_loc, int* is_first_out) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const int nf = ctl->nf;
+    int last = ctl->last_parity;
+    for (int i = 0; i < nf; i++) {
+        file_loc[i] = ctl->raw_offset + offsets[i];
+        int h = is_first_k2b[i];
+        int first;
+        if (h >= 0 && (h == 1) != (last != 0)) first = (h == 1);
+        else first = !last;
+        last = first;
+        is_first_out[i] = first;
+    }
+    ctl->last_parity = last;
+}
+
+hipError_t run_dropouts(const double* env, const double* ll, const ChunkCtl* ctl, int nf_override, int max_fields,
+                        const Fmt& fmt, uint16_t* luma, uint16_t* chroma, int* lines, int* starts, int* ends,
+                        int* count, hipStream_t st) {
+    hipLaunchKernelGGL(k_dropouts, dim3(max_fields), dim3(kDoThreads), 0, st, env, ll, ctl, nf_override, fmt.spfp,
+                       fmt.active_line_start, luma, chroma, lines, starts, ends, count);
+    return hipGetLastError();
+}
+
+hipError_t run_field_meta(ChunkCtl* ctl, const unsigned long long* offsets, const int* is_first_k2b,
+                          unsigned long long* file_loc, int* is_first_out, hipStream_t st) {
+    hipLaunchKernelGGL(k_field_meta, dim3(1), dim3(64), 0, st, ctl, offsets, is_first_k2b, file_loc, is_first_out);
+    return hipGetLastError();
+}
+
+}  // namespace cuvhs

@@ -1,0 +1,90 @@
+// Kernel launch parameter blocks and host launchers shared by the decoder's
+// translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "common.h"
+
+namespace cuvhs {
+
+struct K2bField;
+struct K2bAnchorOut;
+
+struct K1ALaunch {
+    const uint8_t* cap;
+    long long n_valid, total;
+    int spfp, N, lm, lp, V;
+    double scale;
+    const double2* G;
+    const double2* tw;
+    double* rawdemod;
+    double* env;
+};
+struct K1BLaunch {
+    const double* rawdemod;
+    long long total;
+    int spfp, N, lm, lp, V;
+    const double2* G;
+    const double2* tw;
+    double* demod;
+    double* demod05;
+};
+struct PreLaunch {
+    const uint8_t* cap;
+    long long n_valid;
+    int spfp, N2, lm, lp, V;
+    const float2* G;
+    const float2* tw;
+    float* out;
+};
+struct ChromaLaunch {
+    const float* craw;
+    const double* linelocs;
+    const ChunkCtl* ctl;
+    int nf_override;
+    int spfp;
+    double het_scale, fft_scale, burst_abs_ref;
+    int burst_start, lo, hi;
+    const int* track;
+    int line0;
+    double* burst;
+    uint16_t* out;
+    const double2* tw;
+};
+
+
+hipError_t run_k1_part_a(int B, const K1ALaunch& a, int nf, hipStream_t st);
+hipError_t run_k1_part_b(int B, const K1BLaunch& a, int nf, hipStream_t st);
+hipError_t run_prebpf(int B, const PreLaunch& a, int nf, hipStream_t st);
+hipError_t run_fill_f64(double* p, long long n, double v, hipStream_t st);
+hipError_t run_fft_selftest(int B, int is_f32, const void* in, void* out, const void* tw, int sign, hipStream_t st);
+int events_nwg(long long total);
+hipError_t run_events(const double* x, long long total, double thr, int* wg_count, int* wg_off, int* events,
+                      ChunkCtl* ctl, hipStream_t st);
+hipError_t run_vsync_cluster(const int* events, ChunkCtl* ctl, long long total, int min_w, int max_w, int cap,
+                             int* cand, unsigned long long* offsets, int num_fields, int spf, int spl, int fields_k1,
+                             int do_cluster, hipStream_t st);
+hipError_t run_sync_pulses(const int* events, const ChunkCtl* ctl, const unsigned long long* offsets, int spfp,
+                           int* starts, int* lens, int* count, int max_fields, int nf_override, hipStream_t st);
+hipError_t run_k2b(const ChunkCtl* ctl_dev, ChunkCtl* ctl_mut, int nf_override, int max_fields, const Fmt& fmt,
+                   const unsigned long long* offsets, const int* k2_starts, const int* k2_lens, const int* k2_count,
+                   int* vp_start, int* vp_len, unsigned char* vp_tg, K2bField* summ, K2bAnchorOut* anchors,
+                   int* out_starts, int* out_lens, int* out_count, int* out_types, double* linelocs, int* is_first,
+                   hipStream_t st);
+size_t k2b_field_summary_bytes();
+size_t k2b_anchor_bytes();
+hipError_t run_hsync_refine(const double* d05, double* ll, const ChunkCtl* ctl, int nf_override, int max_fields,
+                            const Fmt& fmt, hipStream_t st);
+hipError_t run_tbc_resample(const double* demod, const double* ll, uint16_t* luma, const ChunkCtl* ctl,
+                            int nf_override, int max_fields, const Fmt& fmt, hipStream_t st);
+hipError_t run_chroma(ChromaLaunch a, ChunkCtl* ctl_mut, int max_fields, double* m0, double* m1,
+                      const double* cs_tab, double* iq0, int* out_track, int* phase_ids, int need_iq,
+                      hipStream_t st);
+hipError_t run_dropouts(const double* env, const double* ll, const ChunkCtl* ctl, int nf_override, int max_fields,
+                        const Fmt& fmt, uint16_t* luma, uint16_t* chroma, int* lines, int* starts, int* ends,
+                        int* count, hipStream_t st);
+hipError_t run_field_meta(ChunkCtl* ctl, const unsigned long long* offsets, const int* is_first_k2b,
+                          unsigned long long* file_loc, int* is_first_out, hipStream_t st);
+
+// launch parameter blocks (layouts shared with the kernel files)
+}  // namespace cuvhs

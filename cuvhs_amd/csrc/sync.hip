@@ -1,0 +1,258 @@
+// Sync threshold searches: run boundaries of demod_05 <= threshold, VSYNC
+// discovery, chunk scheduling (VSYNC clustering) and K2 pulse lists.
+//
+// Reference: vsync_discover.cu:5-72 (one thread per sample, serial width
+// count, atomic append), pipeline.cu:1567-1651 (host sort + clustering),
+// sync_pulses.cu:10-82 (one thread per field scanning ~5e5 samples).
+//
+// Here the signal is scanned once: every workgroup turns 8192 samples into
+// ordered run-boundary events (start = first sample <= thr, end = first
+// sample > thr after a run) with wavefront ballots and a workgroup scan, and
+// all three consumers work on that short, sorted event list.  Both the
+// reference's serial loops are functions of these boundaries alone:
+//   VSYNC width = min(end - start, max_w + 1), end = chunk end if no end;
+//   K2 keeps runs with off < start and end < off + spfp, length end - start.
+#include <hip/hip_runtime.h>
+
+#include "common.h"
+#include "launch.h"
+
+namespace cuvhs {
+
+constexpr int kEvThreads = 256;
+constexpr int kEvPerThread = 32;
+constexpr int kEvSpan = kEvThreads * kEvPerThread;  // samples per workgroup
+
+__device__ __forceinline__ bool below(const double* x, long long i, double thr) { return x[i] <= thr; }
+
+// events of sample i: bit0 = start, bit1 = end
+__device__ __forceinline__ int event_at(const double* x, long long i, double thr) {
+    bool b = x[i] <= thr;
+    if (i == 0) return b ? 1 : 0;
+    bool p = x[i - 1] <= thr;
+    if (b && !p) return 1;
+    if (!b && p) return 2;
+    return 0;
+}
+
+template <typename F>
+__device__ __forceinline__ int block_exclusive_scan(int v, int* sh, int* total, F) {
+    // sh: kEvThreads ints
+    const int t = threadIdx.x;
+    sh[t] = v;
+    __syncthreads();
+    for (int off = 1; off < kEvThreads; off <<= 1) {
+        int add = t >= off ? sh[t - off] : 0;
+        __syncthreads();
+        sh[t] += add;
+        __syncthreads();
+    }
+    int incl = sh[t];
+    *total = sh[kEvThreads - 1];
+    __syncthreads();
+    return incl - v;
+}
+
+struct Dummy {};
+
+__global__ __launch_bounds__(kEvThreads) void k_ev_count(const double* x, long long total, double thr, int* wg_count) {
+    __shared__ int sh[kEvThreads];
+    long long base = (long long)blockIdx.x * kEvSpan + (long long)threadIdx.x * kEvPerThread;
+    int c = 0;
+    for (int k = 0; k < kEvPerThread; k++) {
+        long long i = base + k;
+        if (i < total) c += event_at(x, i, thr) != 0;
+    }
+    int tot;
+    block_exclusive_scan(c, sh, &tot, Dummy{});
+    if (threadIdx.x == 0) wg_count[blockIdx.x] = tot;
+}
+
+// single workgroup: exclusive scan of per-workgroup counts
+__global__ __launch_bounds__(1024) void k_ev_scan(const int* wg_count, int nwg, int* wg_off, ChunkCtl* ctl) {
+    __shared__ int sh[1024];
+    __shared__ int carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (int base = 0; base < nwg; base += 1024) {
+        int i = base + threadIdx.x;
+        int v = i < nwg ? wg_count[i] : 0;
+        sh[threadIdx.x] = v;
+        __syncthreads();
+        for (int off = 1; off < 1024; off <<= 1) {
+            int add = threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
+            __syncthreads();
+            sh[threadIdx.x] += add;
+            __syncthreads();
+        }
+        if (i < nwg) wg_off[i] = carry + sh[threadIdx.x] - v;
+        __syncthreads();
+        if (threadIdx.x == 0) carry += sh[1023];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) ctl->nevents = carry;
+}
+
+// events encoded as 2*pos + is_end, ascending
+__global__ __launch_bounds__(kEvThreads) void k_ev_write(const double* x, long long total, double thr,
+                                                         const int* wg_off, int* events) {
+    __shared__ int sh[kEvThreads];
+    long long base = (long long)blockIdx.x * kEvSpan + (long long)threadIdx.x * kEvPerThread;
+    int c = 0;
+    for (int k = 0; k < kEvPerThread; k++) {
+        long long i = base + k;
+        if (i < total) c += event_at(x, i, thr) != 0;
+    }
+    int tot;
+    int off = block_exclusive_scan(c, sh, &tot, Dummy{}) + wg_off[blockIdx.x];
+    for (int k = 0; k < kEvPerThread; k++) {
+        long long i = base + k;
+        if (i >= total) break;
+        int e = event_at(x, i, thr);
+        if (e) events[off++] = (int)(2 * i + (e == 2 ? 1 : 0));
+    }
+}
+
+// VSYNC discovery over the event list + chunk clustering (one workgroup).
+// offsets[] and ctl->{nf, next_raw_offset, vsync_count, status} are written.
+__global__ __launch_bounds__(1024) void k_vsync_cluster(const int* events, ChunkCtl* ctl, long long total,
+                                                        int min_w, int max_w, int cap, int* cand,
+                                                        unsigned long long* offsets, int num_fields, int spf,
+                                                        int spl, int fields_k1, int do_cluster) {
+    __shared__ int sh[1024];
+    __shared__ int carry_all, carry_kept;
+    const int nev = ctl->nevents;
+    if (threadIdx.x == 0) { carry_all = 0; carry_kept = 0; }
+    __syncthreads();
+    for (int base = 0; base < nev; base += 1024) {
+        int k = base + threadIdx.x;
+        int is_c = 0;
+        int pos = 0;
+        if (k < nev) {
+            int e = events[k];
+            if ((e & 1) == 0) {
+                pos = e >> 1;
+                long long end = total;
+                if (k + 1 < nev) end = events[k + 1] >> 1;  // next event is this run's end
+                long long w = end - pos;
+                if (w > max_w + 1) w = max_w + 1;
+                is_c = (w >= min_w && w <= max_w) ? 1 : 0;
+            }
+        }
+        sh[threadIdx.x] = is_c;
+        __syncthreads();
+        for (int off = 1; off < 1024; off <<= 1) {
+            int add = threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
+            __syncthreads();
+            sh[threadIdx.x] += add;
+            __syncthreads();
+        }
+        int idx = carry_all + sh[threadIdx.x] - is_c;
+        if (is_c && idx < cap) cand[idx] = pos;
+        __syncthreads();
+        if (threadIdx.x == 0) carry_all += sh[1023];
+        __syncthreads();
+    }
+    if (threadIdx.x != 0 || !do_cluster) {
+        if (threadIdx.x == 0) ctl->vsync_count = carry_all;
+        return;
+    }
+    // pipeline.cu:1595-1651 (sequential, <= 20 * fields candidates)
+    int count = carry_all;
+    ctl->vsync_count = count;
+    int status = 0;
+    int stored = count < cap ? count : cap;
+    if (count > cap) status |= 1;
+    int n = 0;
+    unsigned long long last = 0;
+    for (int i = 0; i < stored; i++) {
+        int pos = cand[i];
+        if (n == 0 || ((unsigned long long)pos - last > (unsigned long long)(spf / 2))) {
+            unsigned long long safe = (pos > 5 * spl) ? (unsigned long long)(pos - 5 * spl) : 0ull;
+            if (n < num_fields) offsets[n] = safe;
+            n++;
+            last = safe;
+        }
+    }
+    if (n == 0) {
+        status |= 2;
+        for (int i = 0; i < fields_k1; i++) offsets[i] = (unsigned long long)i * (unsigned long long)spf;
+        n = fields_k1;
+    }
+    if (n > num_fields) n = num_fields;
+    unsigned long long actual = (unsigned long long)spf;
+    if (n >= 2) actual = (offsets[n - 1] - offsets[0]) / (unsigned long long)(n - 1);
+    unsigned long long predicted = offsets[n - 1] + actual;
+    unsigned long long margin = 10ull * (unsigned long long)spl;
+    ctl->next_raw_offset = predicted > margin ? ctl->raw_offset + predicted - margin : ctl->raw_offset + predicted;
+    ctl->nf = n;
+    ctl->status |= status;
+}
+
+// K2: one thread per field, walking the event list from the window start.
+__global__ void k_sync_pulses(const int* events, const ChunkCtl* ctl, const unsigned long long* offsets, int spfp,
+                              int* starts, int* lens, int* count, int nf_override) {
+    int f = blockIdx.x * blockDim.x + threadIdx.x;
+    int nf = nf_override > 0 ? nf_override : ctl->nf;
+    if (f >= nf) return;
+    const int nev = ctl->nevents;
+    long long off = (long long)offsets[f];
+    long long wend = off + spfp;
+    // first event with position > off
+    int lo = 0, hi = nev;
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if ((long long)(events[mid] >> 1) <= off) lo = mid + 1;
+        else hi = mid;
+    }
+    int c = 0;
+    int* st = starts + (size_t)f * kMaxPulses;
+    int* ln = lens + (size_t)f * kMaxPulses;
+    for (int k = lo; k < nev; k++) {
+        int e = events[k];
+        long long p = e >> 1;
+        if (p >= wend) break;
+        if (e & 1) continue;                 // an end whose start precedes the window
+        if (k + 1 >= nev) break;             // run never ends inside the buffer
+        long long pe = events[k + 1] >> 1;
+        if (pe >= wend) break;               // run still open at the window end
+        if (c < kMaxPulses) {
+            st[c] = (int)p;
+            ln[c] = (int)(pe - p);
+            c++;
+        }
+        k++;
+    }
+    count[f] = c;
+}
+
+// ---- host launchers ----
+int events_nwg(long long total) { return (int)((total + kEvSpan - 1) / kEvSpan); }
+
+hipError_t run_events(const double* x, long long total, double thr, int* wg_count, int* wg_off, int* events,
+                      ChunkCtl* ctl, hipStream_t st) {
+    int nwg = events_nwg(total);
+    if (nwg == 0) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_ev_count, dim3(nwg), dim3(kEvThreads), 0, st, x, total, thr, wg_count);
+    hipLaunchKernelGGL(k_ev_scan, dim3(1), dim3(1024), 0, st, wg_count, nwg, wg_off, ctl);
+    hipLaunchKernelGGL(k_ev_write, dim3(nwg), dim3(kEvThreads), 0, st, x, total, thr, wg_off, events);
+    return hipGetLastError();
+}
+
+hipError_t run_vsync_cluster(const int* events, ChunkCtl* ctl, long long total, int min_w, int max_w, int cap,
+                             int* cand, unsigned long long* offsets, int num_fields, int spf, int spl, int fields_k1,
+                             int do_cluster, hipStream_t st) {
+    hipLaunchKernelGGL(k_vsync_cluster, dim3(1), dim3(1024), 0, st, events, ctl, total, min_w, max_w, cap, cand,
+                       offsets, num_fields, spf, spl, fields_k1, do_cluster);
+    return hipGetLastError();
+}
+
+hipError_t run_sync_pulses(const int* events, const ChunkCtl* ctl, const unsigned long long* offsets, int spfp,
+                           int* starts, int* lens, int* count, int max_fields, int nf_override, hipStream_t st) {
+    int bs = 64;
+    hipLaunchKernelGGL(k_sync_pulses, dim3((max_fields + bs - 1) / bs), dim3(bs), 0, st, events, ctl, offsets, spfp,
+                       starts, lens, count, nf_override);
+    return hipGetLastError();
+}
+
+}  // namespace cuvhs

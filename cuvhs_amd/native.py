@@ -1,0 +1,220 @@
+"""ctypes bindings of include/cuvhs.h and the synthetic capture generator.
+
+The decoder runs only through libcuvhs.so (HIP, gfx950).  Loading fails with
+an exception when the library has not been built; there is no fallback path.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_DIR = os.path.join(HERE, "lib")
+
+MAX_PULSES = 800
+MAX_DROPOUTS = 512
+LINES_PER_FRAME = 525
+FIELD_LINES = 263
+LINE_LEN = 910
+
+E_ARG, E_HIP, E_ALLOC, E_UNSUPPORTED, E_CAPACITY = -1, -2, -3, -4, -5
+W_VSYNC_OVERFLOW, W_NO_VSYNC, W_EOF = 1, 2, 4
+
+_vp = ctypes.c_void_p
+_i = ctypes.c_int
+_u64 = ctypes.c_uint64
+_d = ctypes.c_double
+
+
+class CuvhsError(RuntimeError):
+    pass
+
+
+class Config(ctypes.Structure):
+    _fields_ = [("system", _i), ("sample_rate_mhz", _d), ("batch_fields", _i), ("precision", _i),
+                ("stream", _vp), ("device", _i)]
+
+
+class Format(ctypes.Structure):
+    _fields_ = [("sample_rate", _d), ("samples_per_line", _i), ("samples_per_field", _i), ("spfp", _i),
+                ("pulse_threshold_hz", _d), ("ire0", _d), ("hz_ire", _d), ("output_scale", _d),
+                ("output_zero", _d), ("output_rate", _d), ("black16b", _d), ("white16b", _d),
+                ("burst_start", _i), ("burst_end", _i)]
+
+
+class ChunkInfo(ctypes.Structure):
+    _fields_ = [("fields", _i), ("fields_k1", _i), ("vsync_candidates", _i), ("status", _i),
+                ("raw_offset", _u64), ("next_raw_offset", _u64)]
+
+
+class ChunkOutputs(ctypes.Structure):
+    _fields_ = [(n, _vp) for n in ("luma", "chroma", "file_loc", "is_first", "phase_id", "do_count", "do_line",
+                                   "do_startx", "do_endx")]
+
+
+class GenParams(ctypes.Structure):
+    _fields_ = [("sample_rate_mhz", _d), ("n_samples", ctypes.c_size_t), ("seed", _u64), ("lead_lines", _d),
+                ("snr_lo_db", _d), ("snr_hi_db", _d), ("chroma_db", _d), ("dropouts_per_field", _d),
+                ("dropout_len_lo_us", _d), ("dropout_len_hi_us", _d), ("wow_max_us", _d), ("wow_f_lo_hz", _d),
+                ("wow_f_hi_hz", _d), ("headswitch_max_us", _d), ("headswitch_group_max", _i)]
+
+
+_SIGS = {
+    "cuvhs_create": (_i, [ctypes.POINTER(Config), ctypes.POINTER(_vp)]),
+    "cuvhs_destroy": (_i, [_vp]),
+    "cuvhs_get_format": (_i, [_vp, ctypes.POINTER(Format)]),
+    "cuvhs_format_for": (_i, [_d, ctypes.POINTER(Format)]),
+    "cuvhs_reset": (_i, [_vp]),
+    "cuvhs_decode_chunk": (_i, [_vp, _vp, _u64, _u64, _i, ctypes.POINTER(ChunkInfo)]),
+    "cuvhs_chunk_outputs_get": (_i, [_vp, ctypes.POINTER(ChunkOutputs)]),
+    "cuvhs_decode_capture": (_i, [_vp, _vp, _u64, _i] + [_vp] * 9 + [ctypes.POINTER(_i)]),
+    "cuvhs_fm_demod": (_i, [_vp, _vp, _u64, _i, _vp, _vp, _vp]),
+    "cuvhs_vsync_discover": (_i, [_vp, _vp, _i, _vp, _i, ctypes.POINTER(_i)]),
+    "cuvhs_sync_pulses": (_i, [_vp, _vp, _i, _vp, _i, _vp, _vp, _vp]),
+    "cuvhs_linelocs": (_i, [_vp, _u64, _vp, _i, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "cuvhs_hsync_refine": (_i, [_vp, _vp, _i, _vp, _i]),
+    "cuvhs_tbc_resample": (_i, [_vp, _vp, _i, _vp, _i, _vp]),
+    "cuvhs_chroma": (_i, [_vp, _vp, _u64, _vp, _i, _vp, _vp]),
+    "cuvhs_dropouts": (_i, [_vp, _vp, _vp, _i, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "cuvhs_selftest_fft": (_i, [_i, _i, _vp, _vp, _i]),
+    "cuvhs_last_error": (ctypes.c_char_p, [_vp]),
+}
+
+_lib = None
+_gen = None
+
+
+def header_symbols():
+    """Function names declared in include/cuvhs.h."""
+    import re
+    path = os.path.join(os.path.dirname(HERE), "include", "cuvhs.h")
+    with open(path) as f:
+        text = f.read()
+    return sorted(set(re.findall(r"\b(cuvhs_[a-z_0-9]+)\s*\(", text)))
+
+
+def lib():
+    """The decoder library (raises CuvhsError when it is not built)."""
+    global _lib
+    if _lib is None:
+        path = os.path.join(LIB_DIR, "libcuvhs.so")
+        if not os.path.exists(path):
+            raise CuvhsError(f"{path} is missing: build it with __graft_entry__.build()")
+        L = ctypes.CDLL(path)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def gen_lib():
+    global _gen
+    if _gen is None:
+        path = os.path.join(LIB_DIR, "librfgen.so")
+        if not os.path.exists(path):
+            raise CuvhsError(f"{path} is missing: build it with __graft_entry__.build()")
+        G = ctypes.CDLL(path)
+        G.cuvhs_gen_defaults.argtypes = [ctypes.POINTER(GenParams), _i]
+        G.cuvhs_gen_defaults.restype = None
+        G.cuvhs_gen.argtypes = [ctypes.POINTER(GenParams), _vp]
+        G.cuvhs_gen.restype = _i
+        _gen = G
+    return _gen
+
+
+def gen_params(config, **overrides):
+    p = GenParams()
+    gen_lib().cuvhs_gen_defaults(ctypes.byref(p), int(config))
+    for k, v in overrides.items():
+        setattr(p, k, v)
+    return p
+
+
+def generate(config=0, **overrides):
+    """Synthetic u8 capture of one of the benchmark configurations (numpy array)."""
+    import numpy as np
+    p = gen_params(config, **overrides)
+    out = np.empty(p.n_samples, dtype=np.uint8)
+    rc = gen_lib().cuvhs_gen(ctypes.byref(p), out.ctypes.data)
+    if rc != 0:
+        raise CuvhsError(f"cuvhs_gen failed ({rc})")
+    return out
+
+
+def format_for(mhz):
+    f = Format()
+    rc = lib().cuvhs_format_for(float(mhz), ctypes.byref(f))
+    if rc != 0:
+        raise CuvhsError(f"cuvhs_format_for({mhz}) -> {rc}")
+    return f
+
+
+def check(rc, ctx=None, what="call"):
+    if rc < 0:
+        msg = lib().cuvhs_last_error(ctx).decode() if ctx else ""
+        raise CuvhsError(f"{what} failed with {rc} {msg}")
+    return rc
+
+
+def ptr(t):
+    """Device address of a torch tensor (or None)."""
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+class Decoder:
+    """One decode context (Pipeline, pipeline.h:28-114) on a HIP device."""
+
+    def __init__(self, sample_rate_mhz=28.0, batch_fields=64, device=0):
+        cfg = Config(0, float(sample_rate_mhz), int(batch_fields), 0, None, int(device))
+        h = _vp()
+        rc = lib().cuvhs_create(ctypes.byref(cfg), ctypes.byref(h))
+        if rc != 0:
+            raise CuvhsError(f"cuvhs_create failed with {rc}")
+        self.h = h
+        self.mhz = float(sample_rate_mhz)
+        self.batch = int(batch_fields)
+        self.device = int(device)
+        self.fmt = Format()
+        check(lib().cuvhs_get_format(self.h, ctypes.byref(self.fmt)), self.h, "get_format")
+
+    def close(self):
+        if self.h:
+            lib().cuvhs_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def reset(self):
+        check(lib().cuvhs_reset(self.h), self.h, "reset")
+
+    def decode_capture(self, d_capture, max_fields=None):
+        """Decode a device-resident u8 capture (torch uint8 tensor on the GPU).
+        Returns a dict of device tensors trimmed to the decoded field count."""
+        import torch
+        n = d_capture.numel()
+        if max_fields is None:
+            max_fields = int(n // self.fmt.samples_per_field) + 2
+        dev = d_capture.device
+        luma = torch.empty((max_fields, FIELD_LINES, LINE_LEN), dtype=torch.uint16, device=dev)
+        chroma = torch.empty_like(luma)
+        file_loc = torch.empty(max_fields, dtype=torch.uint64, device=dev)
+        is_first = torch.empty(max_fields, dtype=torch.int32, device=dev)
+        phase = torch.empty(max_fields, dtype=torch.int32, device=dev)
+        doc = torch.empty(max_fields, dtype=torch.int32, device=dev)
+        dol = torch.empty((max_fields, MAX_DROPOUTS), dtype=torch.int32, device=dev)
+        dos = torch.empty_like(dol)
+        doe = torch.empty_like(dol)
+        nf = _i()
+        torch.cuda.synchronize(dev)
+        rc = lib().cuvhs_decode_capture(self.h, ptr(d_capture), n, max_fields, ptr(luma), ptr(chroma), ptr(file_loc),
+                                        ptr(is_first), ptr(phase), ptr(doc), ptr(dol), ptr(dos), ptr(doe),
+                                        ctypes.byref(nf))
+        check(rc, self.h, "decode_capture")
+        k = nf.value
+        return {"fields": k, "status": rc, "luma": luma[:k], "chroma": chroma[:k], "file_loc": file_loc[:k],
+                "is_first": is_first[:k], "phase_id": phase[:k], "do_count": doc[:k], "do_line": dol[:k],
+                "do_startx": dos[:k], "do_endx": doe[:k]}

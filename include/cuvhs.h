@@ -2,7 +2,7 @@
  * cuvhs-mi355x: C ABI of the MI355X-native VHS RF decoder.
  *
  * The drop-in boundary is the reference's stage API (hunterschuler/cuVHS,
- * src/pipeline/*.h) plus its chunk loop (pipeline.cu:1525-2713).  Every
+ * src/pipeline/ headers) plus its chunk loop (pipeline.cu:1525-2713).  Every
  * entry point takes plain pointers and sizes; device pointers are HIP device
  * memory on the context's device.  Every call returns an int status:
  *   0  OK
@@ -107,10 +107,14 @@ typedef struct cuvhs_chunk_outputs {
     const int32_t* do_endx;
 } cuvhs_chunk_outputs;
 
-/* Decode the chunk starting at raw_offset of a device-resident u8 capture of
- * capture_len samples, with at most max_fields fields (<= batch_fields).
- * Mirrors Pipeline::run's chunk step (pipeline.cu:2650-2665). */
-int cuvhs_decode_chunk(cuvhs_ctx* ctx, const uint8_t* d_capture, uint64_t capture_len, uint64_t raw_offset,
+/* Decode one chunk (Pipeline::process_chunk).  d_chunk points at capture
+ * sample raw_offset in device memory, with `avail` samples readable from
+ * there (the rest of the capture, or at least max_fields * spfp); samples
+ * past `avail` read as zero (pipeline.cu:1539-1541).  raw_offset is the
+ * chunk's position in the capture (fileLoc, K2b cadence).  At most
+ * max_fields (<= batch_fields) fields are decoded; info->next_raw_offset is
+ * where the next chunk starts (pipeline.cu:1636-1651). */
+int cuvhs_decode_chunk(cuvhs_ctx* ctx, const uint8_t* d_chunk, uint64_t avail, uint64_t raw_offset,
                        int max_fields, cuvhs_chunk_info* info);
 int cuvhs_chunk_outputs_get(const cuvhs_ctx* ctx, cuvhs_chunk_outputs* out);
 
@@ -151,6 +155,11 @@ int cuvhs_chroma(cuvhs_ctx* ctx, const uint8_t* d_raw_u8, uint64_t n_valid, cons
 /* dropout_detect (dropout_detect.h:22-32), concealment in place. */
 int cuvhs_dropouts(cuvhs_ctx* ctx, const double* d_env, const double* d_linelocs, int nfields, uint16_t* d_luma,
                    uint16_t* d_chroma, int* d_lines, int* d_starts, int* d_ends, int* d_count);
+
+/* Self-test of the LDS FFT building block (no counterpart in the reference):
+ * one B-point transform (B in 1024/4096/8192 FP64, 1024/8192/16384 FP32) of
+ * device data, sign -1 forward / +1 inverse, unnormalised. */
+int cuvhs_selftest_fft(int B, int is_f32, const void* d_in, void* d_out, int sign);
 
 /* Last HIP error text for the context (or a static string). */
 const char* cuvhs_last_error(const cuvhs_ctx* ctx);

@@ -1537,10 +1537,12 @@ or_dec* or_create(double mhz, int batch, int record_trace) {
     d->trace = record_trace != 0;
     d->k1 = make_k1_filters(d->fmt);
     size_t n = (size_t)batch * d->fmt.spfp;
+    // one segment of zero padding: a K2 window may start in the last segment
+    size_t np = n + (size_t)d->fmt.spfp;
     d->raw.assign(n, 0.0);
-    d->demod.assign(n, 0.0);
-    d->d05.assign(n, 0.0);
-    d->env.assign(n, 0.0);
+    d->demod.assign(np, 0.0);
+    d->d05.assign(np, 0.0);
+    d->env.assign(np, 0.0);
     return d;
 }
 

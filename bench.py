@@ -1,0 +1,146 @@
+"""Decode throughput of the MI355X VHS decoder on the 600-field 28 MHz workload.
+
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+
+One step decodes the whole synthetic capture (600 NTSC fields of u8 RF at
+28 MHz: SNR 25-40 dB per field, Poisson dropouts, wow/flutter; chunks of
+F = 64 fields) with the capture already resident in HBM and outputs left in
+HBM.  Prints one JSON line on stdout.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "NTSC fields/s at 28 MHz (MFMA mixed vs plain FP32, 1 GPU) + achieved HBM GB/s"
+PEAK_HBM_GBS = 8000.0
+
+# Algorithmic bytes per unit of each stage (the stage's compulsory reads and
+# writes of its own inputs/outputs, counted once).
+STAGE_BYTES = {
+    # per capture sample: u8 in, analytic envelope (f64) + instantaneous frequency (f64) out
+    "k1_analytic": ("sample", 1 + 8 + 8),
+    # per capture sample: instantaneous frequency in, demod + demod_05 out (f64)
+    "k1_video": ("sample", 8 + 8 + 8),
+    # per capture sample: u8 in, band-passed f32 out
+    "k6_prebpf": ("sample", 1 + 4),
+}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(cap, fmt, batch):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_py
+    sample = cap[: batch * fmt.spfp + fmt.samples_per_field]
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    t = time.perf_counter()
+    ref = oracle_py.decode(sample, 28.0, batch, trace=False)
+    dt = time.perf_counter() - t
+    return {"value": ref["fields"] / dt, "unit": "fields/s", "cores": threads, "kind": "port",
+            "sample": f"first {ref['fields']} fields of the same 28 MHz capture ({sample.size} u8 samples), "
+                      f"oracle liboracle.so (FP64, own FFT), OpenMP over fields, {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--batch", type=int, default=64)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the decode path shards by "
+              "capture segment, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+    from cuvhs_amd import native
+
+    t = time.perf_counter()
+    cap = native.generate(1)
+    log(f"generated {cap.size} samples in {time.perf_counter() - t:.1f} s")
+    dev = torch.device("cuda:0")
+    d_cap = torch.from_numpy(cap).to(dev)
+    dec = native.Decoder(28.0, args.batch, device=0)
+    fmt = dec.fmt
+    fields = 0
+    for _ in range(args.warmup):
+        dec.reset()
+        fields = dec.decode_capture(d_cap)["fields"]
+    log(f"warmup done: {fields} fields")
+    dec.profile(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        dec.reset()
+        out = dec.decode_capture(d_cap)
+        fields = out["fields"]
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    prof = dec.profile_read()
+    dec.profile(False)
+    ms_per_step = dt * 1000.0 / args.steps
+    value = fields * args.steps / dt
+    bytes_per_field = cap.size / fields + 2 * 2 * 910 * 263  # u8 in + luma/chroma u16 out
+    log(f"{fields} fields/step, {ms_per_step:.2f} ms/step, {value:.1f} fields/s")
+    stage_ms = {k: v / args.steps for k, v in prof.items()}
+    log("per-step stage ms: " + ", ".join(f"{k}={v:.3f}" for k, v in stage_ms.items()))
+
+    # dominant kernel with an algorithmic byte count
+    chunks = int(np.ceil(fields / args.batch))
+    dom = max((k for k in STAGE_BYTES if k in stage_ms), key=lambda k: stage_ms[k])
+    unit, per_unit = STAGE_BYTES[dom]
+    samples_per_step = chunks * args.batch * fmt.spfp
+    launches_per_step = chunks
+    alg_bytes_per_launch = per_unit * samples_per_step / launches_per_step
+    avg_launch_s = stage_ms[dom] / 1000.0 / launches_per_step
+    achieved = alg_bytes_per_launch / avg_launch_s / 1e9
+    traffic = None
+    pmc_path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    if os.path.exists(pmc_path):
+        with open(pmc_path) as f:
+            traffic = json.load(f).get(dom)
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
+                "alg_bytes_per_launch": int(alg_bytes_per_launch), "avg_launch_ms": round(avg_launch_s * 1e3, 4)}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("running CPU baseline ...")
+        cpu = cpu_baseline(cap, fmt, args.batch)
+        log(f"cpu baseline {cpu['value']:.2f} fields/s on {cpu['cores']} threads")
+
+    line = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "fields/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "600 synthetic NTSC fields of u8 RF at 28 MHz (SNR 25-40 dB, dropouts, wow/flutter)",
+                   "fields_per_step": fields, "batch_fields": args.batch, "sample_rate_mhz": 28.0,
+                   "precision": "fp64 (chroma pre-band-pass fp32)"},
+        "hbm_gbs_algorithmic": round(value * bytes_per_field / 1e9, 2),
+        "stage_ms_per_step": {k: round(v, 3) for k, v in stage_ms.items()},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

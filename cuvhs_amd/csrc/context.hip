@@ -75,6 +75,12 @@ struct cuvhs_ctx {
     ChunkCtl* ctl = nullptr;       // device
     ChunkCtl* hctl = nullptr;      // pinned host shadow
     std::vector<void*> allocs;
+    // per-stage timing with HIP events on the context stream
+    static constexpr int kStages = 12;
+    bool prof = false;
+    hipEvent_t ev[kStages + 1] = {};
+    double stage_ms[kStages] = {};
+    long long stage_calls = 0;
 };
 
 namespace {
@@ -155,6 +161,10 @@ ChromaLaunch chroma_launch(cuvhs_ctx* c, int nf_override) {
     return a;
 }
 
+void mark(cuvhs_ctx* c, int i) {
+    if (c->prof) (void)hipEventRecord(c->ev[i], c->stream);
+}
+
 int k1_launch(cuvhs_ctx* c, const uint8_t* chunk, long long n_valid, int nfk1) {
     const Fmt& f = c->fmt;
     long long T = (long long)nfk1 * f.spfp;
@@ -173,6 +183,7 @@ int k1_launch(cuvhs_ctx* c, const uint8_t* chunk, long long n_valid, int nfk1) {
     a.rawdemod = c->rawdemod;
     a.env = c->env;
     HIPCHK(c, run_k1_part_a(c->k1.rf.B, a, nfk1, c->stream));
+    mark(c, 1);
     K1BLaunch b{};
     b.rawdemod = c->rawdemod;
     b.total = T;
@@ -356,6 +367,8 @@ int cuvhs_create(const cuvhs_config* cfg, cuvhs_ctx** out) {
 int cuvhs_destroy(cuvhs_ctx* c) {
     if (!c) return CUVHS_E_ARG;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (hipEvent_t e : c->ev)
+        if (e) (void)hipEventDestroy(e);
     for (void* p : c->allocs) (void)hipFree(p);
     if (c->hctl) (void)hipHostFree(c->hctl);
     if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
@@ -405,27 +418,46 @@ int cuvhs_decode_chunk(cuvhs_ctx* c, const uint8_t* chunk, uint64_t avail, uint6
     c->hctl->raw_offset = raw_offset;
     c->hctl->next_raw_offset = 0;
     HIPCHK(c, hipMemcpyAsync(c->ctl, c->hctl, sizeof(ChunkCtl), hipMemcpyHostToDevice, c->stream));
+    mark(c, 0);
     int r = k1_launch(c, chunk, (long long)n_read, fk1);
     if (r) return r;
+    mark(c, 2);
     HIPCHK(c, run_events(c->d05, T, f.pulse_threshold_hz, c->wg_count, c->wg_off, c->events, c->ctl, c->stream));
+    mark(c, 3);
     HIPCHK(c, run_vsync_cluster(c->events, c->ctl, T, f.vsync_min_w, f.vsync_max_w, fk1 * 20, c->cand, c->offsets,
                                 max_fields, f.samples_per_field, f.samples_per_line, fk1, 1, c->stream));
+    mark(c, 4);
     HIPCHK(c, run_sync_pulses(c->events, c->ctl, c->offsets, f.spfp, c->k2s, c->k2l, c->k2c, max_fields, 0,
                               c->stream));
+    mark(c, 5);
     HIPCHK(c, run_k2b(c->ctl, c->ctl, 0, max_fields, f, c->offsets, c->k2s, c->k2l, c->k2c, c->vps, c->vpl, c->vptg,
                       (K2bField*)c->summ, (K2bAnchorOut*)c->anchors, c->ps, c->pl, c->pc, c->pt, c->ll, c->isf_k2b,
                       c->stream));
+    mark(c, 6);
     HIPCHK(c, run_hsync_refine(c->d05, c->ll, c->ctl, 0, max_fields, f, c->stream));
+    mark(c, 7);
     HIPCHK(c, run_tbc_resample(c->demod, c->ll, c->luma, c->ctl, 0, max_fields, f, c->stream));
+    mark(c, 8);
     r = prebpf_launch(c, chunk, (long long)n_read, max_fields);
     if (r) return r;
+    mark(c, 9);
     HIPCHK(c, run_chroma(chroma_launch(c, 0), c->ctl, max_fields, c->m0, c->m1, c->cs_tab, c->iq0, c->track, c->phase,
                          c->hctl->chroma.valid ? 0 : 1, c->stream));
+    mark(c, 10);
     HIPCHK(c, run_dropouts(c->env, c->ll, c->ctl, 0, max_fields, f, c->luma, c->chroma, c->dol, c->dos, c->doe,
                            c->doc, c->stream));
+    mark(c, 11);
     HIPCHK(c, run_field_meta(c->ctl, c->offsets, c->isf_k2b, c->file_loc, c->isf_out, c->stream));
+    mark(c, 12);
     HIPCHK(c, hipMemcpyAsync(c->hctl, c->ctl, sizeof(ChunkCtl), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (c->prof) {
+        for (int i = 0; i < cuvhs_ctx::kStages; i++) {
+            float ms = 0.0f;
+            if (hipEventElapsedTime(&ms, c->ev[i], c->ev[i + 1]) == hipSuccess) c->stage_ms[i] += ms;
+        }
+        c->stage_calls++;
+    }
     info->fields = c->hctl->nf;
     info->fields_k1 = fk1;
     info->vsync_candidates = c->hctl->vsync_count;
@@ -492,6 +524,30 @@ int cuvhs_decode_capture(cuvhs_ctx* c, const uint8_t* cap, uint64_t cap_len, int
     HIPCHK(c, hipStreamSynchronize(c->stream));
     *fields = total;
     return warn;
+}
+
+static const char* kStageNames[cuvhs_ctx::kStages] = {
+    "k1_analytic", "k1_video", "sync_events", "vsync_cluster", "k2_pulses", "k2b_linelocs",
+    "k4_hsync_refine", "k5_tbc_resample", "k6_prebpf", "k6_chroma", "k7_dropouts", "field_meta"};
+
+int cuvhs_profile(cuvhs_ctx* c, int enable) {
+    if (!c) return CUVHS_E_ARG;
+    if (enable && !c->ev[0]) {
+        for (hipEvent_t& e : c->ev) HIPCHK(c, hipEventCreate(&e));
+    }
+    c->prof = enable != 0;
+    for (double& v : c->stage_ms) v = 0.0;
+    c->stage_calls = 0;
+    return 0;
+}
+
+int cuvhs_profile_read(const cuvhs_ctx* c, double* ms, const char** names, int n) {
+    if (!c || n < 0) return CUVHS_E_ARG;
+    for (int i = 0; i < n && i < cuvhs_ctx::kStages; i++) {
+        if (ms) ms[i] = c->stage_ms[i];
+        if (names) names[i] = kStageNames[i];
+    }
+    return cuvhs_ctx::kStages;
 }
 
 // ---- stage entry points ----

@@ -76,6 +76,8 @@ _SIGS = {
     "cuvhs_dropouts": (_i, [_vp, _vp, _vp, _i, _vp, _vp, _vp, _vp, _vp, _vp]),
     "cuvhs_selftest_fft": (_i, [_i, _i, _vp, _vp, _i]),
     "cuvhs_last_error": (ctypes.c_char_p, [_vp]),
+    "cuvhs_profile": (_i, [_vp, _i]),
+    "cuvhs_profile_read": (_i, [_vp, _vp, _vp, _i]),
 }
 
 _lib = None
@@ -187,6 +189,17 @@ class Decoder:
             self.close()
         except Exception:
             pass
+
+    def profile(self, enable=True):
+        check(lib().cuvhs_profile(self.h, 1 if enable else 0), self.h, "profile")
+
+    def profile_read(self):
+        """{stage name: accumulated device ms} since profiling was enabled."""
+        n = 32
+        ms = (ctypes.c_double * n)()
+        names = (ctypes.c_char_p * n)()
+        k = lib().cuvhs_profile_read(self.h, ms, names, n)
+        return {names[i].decode(): ms[i] for i in range(k)}
 
     def reset(self):
         check(lib().cuvhs_reset(self.h), self.h, "reset")

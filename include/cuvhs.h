@@ -161,6 +161,14 @@ int cuvhs_dropouts(cuvhs_ctx* ctx, const double* d_env, const double* d_linelocs
  * device data, sign -1 forward / +1 inverse, unnormalised. */
 int cuvhs_selftest_fft(int B, int is_f32, const void* d_in, void* d_out, int sign);
 
+/* Per-stage device time (HIP events on the context stream) accumulated over
+ * cuvhs_decode_chunk calls while enabled; enabling clears the totals.
+ * cuvhs_profile_read fills up to n stage totals (ms) and names and returns
+ * the number of stages.  (Counterpart of nothing in the reference, which
+ * only times the whole chunk loop: pipeline.cu:2642-2708.) */
+int cuvhs_profile(cuvhs_ctx* ctx, int enable);
+int cuvhs_profile_read(const cuvhs_ctx* ctx, double* ms, const char** names, int n);
+
 /* Last HIP error text for the context (or a static string). */
 const char* cuvhs_last_error(const cuvhs_ctx* ctx);
 

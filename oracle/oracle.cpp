@@ -1435,6 +1435,8 @@ static int process_chunk(or_dec* d, const uint8_t* data, size_t total_n, size_t 
     return nf;
 }
 
+static int write_json_meta(const char* path, const std::vector<or_dec::Meta>& meta, const Fmt& fmt);
+
 extern "C" {
 
 int or_format_init(double mhz, or_format* o) {
@@ -1627,46 +1629,25 @@ int or_write_outputs(const or_dec* d, const char* out_base) {
     fwrite(d->chroma_out.data(), sizeof(uint16_t), d->meta.size() * tbc, fc);
     fclose(fl);
     fclose(fc);
-    FILE* fp = fopen((base + ".tbc.json").c_str(), "w");
-    if (!fp) return -1;
-    double black16b = (0.0 - fmt.vsync_ire) * fmt.output_scale + fmt.output_zero;
-    double white16b = (100.0 - fmt.vsync_ire) * fmt.output_scale + fmt.output_zero;
-    int burst_start = (int)(fmt.burst_start_us * 1e-6 * fmt.output_rate + 0.5);
-    int burst_end = (int)(fmt.burst_end_us * 1e-6 * fmt.output_rate + 0.5);
-    int n = (int)d->meta.size();
-    fprintf(fp, "{\n  \"videoParameters\": {\n");
-    fprintf(fp, "    \"system\": \"NTSC\",\n    \"isSubcarrierLocked\": false,\n    \"isSourcePal\": false,\n");
-    fprintf(fp, "    \"numberOfSequentialFields\": %d,\n", n);
-    fprintf(fp, "    \"black16bIre\": %.1f,\n", black16b);
-    fprintf(fp, "    \"white16bIre\": %.1f,\n", white16b);
-    fprintf(fp, "    \"sampleRate\": %.0f,\n", fmt.output_rate);
-    fprintf(fp, "    \"fieldWidth\": %d,\n", fmt.output_line_len);
-    fprintf(fp, "    \"fieldHeight\": %d,\n", fmt.output_field_lines);
-    fprintf(fp, "    \"colourBurstStart\": %d,\n", burst_start);
-    fprintf(fp, "    \"colourBurstEnd\": %d,\n", burst_end);
-    fprintf(fp, "    \"activeVideoStart\": %d,\n", 134);
-    fprintf(fp, "    \"activeVideoEnd\": %d,\n", 894);
-    fprintf(fp, "    \"tapeFormat\": \"VHS\",\n    \"isMapped\": false\n  },\n");
-    fprintf(fp, "  \"fields\": [\n");
+    return write_json_meta((base + ".tbc.json").c_str(), d->meta, fmt);
+}
+
+int or_write_json_meta(double mhz, const char* path, int n, const uint64_t* file_loc, const int* is_first,
+                       const int* phase_id, const int* do_count, const int* do_line, const int* do_startx,
+                       const int* do_endx) {
+    std::vector<or_dec::Meta> meta((size_t)n);
+    int k = 0;
     for (int i = 0; i < n; i++) {
-        const auto& m = d->meta[(size_t)i];
-        fprintf(fp, "    {\n");
-        fprintf(fp, "      \"isFirstField\": %s,\n", m.is_first ? "true" : "false");
-        fprintf(fp, "      \"seqNo\": %d,\n", i + 1);
-        fprintf(fp, "      \"fileLoc\": %zu,\n", m.file_loc);
-        if (m.phase_id > 0) fprintf(fp, "      \"fieldPhaseID\": %d,\n", m.phase_id);
-        fprintf(fp, "      \"dropOuts\": {\n        \"fieldLine\": [");
-        for (size_t j = 0; j < m.dl.size(); j++) fprintf(fp, j ? ", %d" : "%d", m.dl[j]);
-        fprintf(fp, "],\n        \"startx\": [");
-        for (size_t j = 0; j < m.ds.size(); j++) fprintf(fp, j ? ", %d" : "%d", m.ds[j]);
-        fprintf(fp, "],\n        \"endx\": [");
-        for (size_t j = 0; j < m.de.size(); j++) fprintf(fp, j ? ", %d" : "%d", m.de[j]);
-        fprintf(fp, "]\n      }\n");
-        fprintf(fp, "    }%s\n", (i < n - 1) ? "," : "");
+        meta[(size_t)i].file_loc = file_loc[i];
+        meta[(size_t)i].is_first = is_first[i];
+        meta[(size_t)i].phase_id = phase_id[i];
+        for (int j = 0; j < do_count[i]; j++, k++) {
+            meta[(size_t)i].dl.push_back(do_line[k]);
+            meta[(size_t)i].ds.push_back(do_startx[k]);
+            meta[(size_t)i].de.push_back(do_endx[k]);
+        }
     }
-    fprintf(fp, "  ]\n}\n");
-    fclose(fp);
-    return 0;
+    return write_json_meta(path, meta, make_fmt(mhz));
 }
 
 int or_decode_file(const char* in_path, const char* out_base, double mhz, int batch) {
@@ -1688,3 +1669,48 @@ int or_decode_file(const char* in_path, const char* out_base, double mhz, int ba
 }
 
 }  // extern "C"
+
+// TBCWriter::write_json (tbc_writer.cpp:89-175)
+static int write_json_meta(const char* path, const std::vector<or_dec::Meta>& meta, const Fmt& fmt) {
+    FILE* fp = fopen(path, "w");
+    if (!fp) return -1;
+    double black16b = (0.0 - fmt.vsync_ire) * fmt.output_scale + fmt.output_zero;
+    double white16b = (100.0 - fmt.vsync_ire) * fmt.output_scale + fmt.output_zero;
+    int burst_start = (int)(fmt.burst_start_us * 1e-6 * fmt.output_rate + 0.5);
+    int burst_end = (int)(fmt.burst_end_us * 1e-6 * fmt.output_rate + 0.5);
+    int n = (int)meta.size();
+    fprintf(fp, "{\n  \"videoParameters\": {\n");
+    fprintf(fp, "    \"system\": \"NTSC\",\n    \"isSubcarrierLocked\": false,\n    \"isSourcePal\": false,\n");
+    fprintf(fp, "    \"numberOfSequentialFields\": %d,\n", n);
+    fprintf(fp, "    \"black16bIre\": %.1f,\n", black16b);
+    fprintf(fp, "    \"white16bIre\": %.1f,\n", white16b);
+    fprintf(fp, "    \"sampleRate\": %.0f,\n", fmt.output_rate);
+    fprintf(fp, "    \"fieldWidth\": %d,\n", fmt.output_line_len);
+    fprintf(fp, "    \"fieldHeight\": %d,\n", fmt.output_field_lines);
+    fprintf(fp, "    \"colourBurstStart\": %d,\n", burst_start);
+    fprintf(fp, "    \"colourBurstEnd\": %d,\n", burst_end);
+    fprintf(fp, "    \"activeVideoStart\": %d,\n", 134);
+    fprintf(fp, "    \"activeVideoEnd\": %d,\n", 894);
+    fprintf(fp, "    \"tapeFormat\": \"VHS\",\n    \"isMapped\": false\n  },\n");
+    fprintf(fp, "  \"fields\": [\n");
+    for (int i = 0; i < n; i++) {
+        const auto& m = meta[(size_t)i];
+        fprintf(fp, "    {\n");
+        fprintf(fp, "      \"isFirstField\": %s,\n", m.is_first ? "true" : "false");
+        fprintf(fp, "      \"seqNo\": %d,\n", i + 1);
+        fprintf(fp, "      \"fileLoc\": %zu,\n", m.file_loc);
+        if (m.phase_id > 0) fprintf(fp, "      \"fieldPhaseID\": %d,\n", m.phase_id);
+        fprintf(fp, "      \"dropOuts\": {\n        \"fieldLine\": [");
+        for (size_t j = 0; j < m.dl.size(); j++) fprintf(fp, j ? ", %d" : "%d", m.dl[j]);
+        fprintf(fp, "],\n        \"startx\": [");
+        for (size_t j = 0; j < m.ds.size(); j++) fprintf(fp, j ? ", %d" : "%d", m.ds[j]);
+        fprintf(fp, "],\n        \"endx\": [");
+        for (size_t j = 0; j < m.de.size(); j++) fprintf(fp, j ? ", %d" : "%d", m.de[j]);
+        fprintf(fp, "]\n      }\n");
+        fprintf(fp, "    }%s\n", (i < n - 1) ? "," : "");
+    }
+    fprintf(fp, "  ]\n}\n");
+    fclose(fp);
+    return 0;
+}
+

@@ -93,6 +93,11 @@ int or_trace_chunk(const or_dec* d, int c, uint64_t* raw_offset, int* fields_k1,
                    int* is_first_k2b);
 int or_num_chunks(const or_dec* d);
 
+/* .tbc.json from explicit metadata (do_* arrays concatenated over fields) */
+int or_write_json_meta(double mhz, const char* path, int n, const uint64_t* file_loc, const int* is_first,
+                       const int* phase_id, const int* do_count, const int* do_line, const int* do_startx,
+                       const int* do_endx);
+
 /* CLI-equivalent: decode file -> out_base.{tbc,_chroma.tbc,tbc.json} */
 int or_decode_file(const char* in_path, const char* out_base, double mhz, int batch);
 

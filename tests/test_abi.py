@@ -1,0 +1,48 @@
+"""The C ABI library loads and exports every symbol include/cuvhs.h declares;
+argument validation that needs no GPU."""
+import ctypes
+import subprocess
+
+import pytest
+
+from cuvhs_amd import native
+
+
+def test_header_symbols_exported():
+    L = native.lib()
+    syms = native.header_symbols()
+    assert len(syms) >= 20
+    for s in syms:
+        assert hasattr(L, s), s
+    out = subprocess.run(["nm", "-D", "--defined-only", native.os.path.join(native.LIB_DIR, "libcuvhs.so")],
+                         capture_output=True, text=True).stdout
+    for s in syms:
+        assert f" T {s}\n" in out, s
+
+
+def test_create_rejects_unsupported_configs():
+    h = ctypes.c_void_p()
+    for cfg in (native.Config(1, 28.0, 64, 0, None, 0),     # PAL
+                native.Config(0, 50.0, 64, 0, None, 0),     # above 40 MHz
+                native.Config(0, 28.0, 64, 7, None, 0)):    # unknown precision
+        assert native.lib().cuvhs_create(ctypes.byref(cfg), ctypes.byref(h)) == native.E_UNSUPPORTED
+    assert native.lib().cuvhs_create(ctypes.byref(native.Config(0, 28.0, 0, 0, None, 0)), ctypes.byref(h)) == native.E_ARG
+    assert native.lib().cuvhs_create(None, ctypes.byref(h)) == native.E_ARG
+
+
+def test_null_context_calls_fail_cleanly():
+    L = native.lib()
+    info = native.ChunkInfo()
+    assert L.cuvhs_decode_chunk(None, None, 0, 0, 1, ctypes.byref(info)) == native.E_ARG
+    assert L.cuvhs_destroy(None) == native.E_ARG
+    assert L.cuvhs_reset(None) == native.E_ARG
+    assert L.cuvhs_last_error(None) == b"no context"
+
+
+@pytest.mark.parametrize("mhz,spl", [(28.0, 1780), (35.8, 2275), (40.0, 2542)])
+def test_format_known_answers(mhz, spl):
+    f = native.format_for(mhz)
+    assert f.samples_per_line == spl
+    assert f.samples_per_field == spl * 263
+    assert f.spfp == spl * 283
+    assert abs(f.pulse_threshold_hz - 3542857.142857143) < 1e-6

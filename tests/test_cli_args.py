@@ -1,0 +1,34 @@
+"""`cuvhs` argument handling that exits before any GPU work (main.cpp:45-95)."""
+import os
+import subprocess
+
+import pytest
+
+from cuvhs_amd import native
+
+EXE = os.path.join(native.LIB_DIR, "cuvhs")
+
+
+def run(*args):
+    return subprocess.run([EXE, *args], capture_output=True, text=True, timeout=60)
+
+
+def test_help_exits_zero():
+    r = run("--help")
+    assert r.returncode == 0 and "Usage" in r.stderr
+
+
+@pytest.mark.parametrize("args,msg", [
+    ((), "input_file and output_base are required"),
+    (("--bogus", "a.u8", "o"), "Unknown option: --bogus"),
+    (("--system", "SECAM", "a.u8", "o"), "Unknown system"),
+    (("--system", "PAL", "a.u8", "o"), "PAL is not supported"),
+    (("a.s16", "o"), "not supported"),
+    (("--format", "u16", "a.u8", "o"), "not supported"),
+    (("a.u8", "o", "extra"), "Too many arguments"),
+    (("-", "o"), "stdin streaming is not supported"),
+])
+def test_bad_arguments_exit_one(args, msg):
+    r = run(*args)
+    assert r.returncode == 1
+    assert msg in r.stderr

@@ -1,0 +1,89 @@
+"""The oracle's and the decoder's restatements of the reference's host-only
+code (video_format.cpp, raw_reader.cpp, tbc_writer.cpp) against output of the
+reference's own compiled sources: committed fixtures, plus a live run of
+oracle/_ref/ref_host where the reference is mounted."""
+import ctypes
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, ROOT
+
+REF = os.path.join(ROOT, "oracle", "_ref", "ref_host")
+
+
+class OrFormat(ctypes.Structure):
+    _fields_ = [("sample_rate", ctypes.c_double), ("samples_per_line", ctypes.c_int),
+                ("samples_per_field", ctypes.c_int), ("spfp", ctypes.c_int), ("k1_fft_size", ctypes.c_int),
+                ("chroma_fft_size", ctypes.c_int)] + [(n, ctypes.c_double) for n in (
+                    "hsync_width", "eq_pulse_width", "vsync_width", "ire0", "hz_ire", "vsync_ire", "sync_tip_hz",
+                    "pulse_threshold_hz", "output_zero", "output_scale", "output_rate", "fsc", "chroma_under",
+                    "burst_abs_ref", "black16b", "white16b")] + [("burst_start", ctypes.c_int),
+                                                                  ("burst_end", ctypes.c_int)]
+
+
+def ref_formats():
+    with open(os.path.join(GOLDEN, "ref_format.json")) as f:
+        return json.load(f)
+
+
+@pytest.mark.parametrize("mhz", ["28", "35.8", "40"])
+def test_oracle_format_matches_reference(oracle, mhz):
+    ref = ref_formats()[mhz]
+    o = OrFormat()
+    assert oracle.lib().or_format_init(float(mhz), ctypes.byref(o)) == 0
+    for k, v in ref.items():
+        assert getattr(o, k) == v, k
+
+
+@pytest.mark.parametrize("mhz", ["28", "35.8", "40"])
+def test_decoder_format_matches_reference(mhz):
+    from cuvhs_amd import native
+    ref = ref_formats()[mhz]
+    f = native.format_for(float(mhz))
+    for k in ("sample_rate", "samples_per_line", "samples_per_field", "pulse_threshold_hz", "ire0", "hz_ire",
+              "output_scale", "output_zero", "output_rate"):
+        assert getattr(f, k) == ref[k], k
+    assert f.spfp == ref["samples_per_field"] + 20 * ref["samples_per_line"]
+    assert (f.black16b, f.white16b, f.burst_start, f.burst_end) == (15360.0, 51200.0, 76, 112)
+
+
+def test_u8_conversion_exhaustive():
+    with open(os.path.join(GOLDEN, "ref_convert_u8.txt")) as f:
+        vals = np.array([float(x) for x in f.read().split()])
+    assert vals.size == 256
+    assert np.array_equal(vals, (np.arange(256) - 128.0) * 256.0)
+
+
+META = dict(file_loc=[32032, 499165, 966298], is_first=[0, 1, 0], phase_id=[1, 4, 0], do_count=[0, 2, 0],
+            do_line=[12, 13], do_startx=[100, 0], do_endx=[910, 57])
+
+
+def oracle_json(oracle, path):
+    L = oracle.lib()
+    L.or_write_json_meta.argtypes = [ctypes.c_double, ctypes.c_char_p, ctypes.c_int] + [ctypes.c_void_p] * 7
+    arrs = [np.array(META["file_loc"], np.uint64)] + [np.array(META[k], np.int32) for k in (
+        "is_first", "phase_id", "do_count", "do_line", "do_startx", "do_endx")]
+    assert L.or_write_json_meta(28.0, path.encode(), 3, *[a.ctypes.data for a in arrs]) == 0
+    with open(path) as f:
+        return f.read()
+
+
+def test_oracle_json_matches_reference_writer(oracle, tmp_path):
+    with open(os.path.join(GOLDEN, "ref_writer_3fields.tbc.json")) as f:
+        golden = f.read()
+    assert oracle_json(oracle, str(tmp_path / "o.tbc.json")) == golden
+
+
+@pytest.mark.skipif(not os.path.exists(REF), reason="reference host build not present")
+def test_live_reference_build_matches_fixtures(tmp_path):
+    out = subprocess.run([REF, "format", "28"], capture_output=True, text=True, check=True).stdout
+    live = {k: float(v) for k, v in (l.split("=") for l in out.split())}
+    assert live == ref_formats()["28"]
+    base = str(tmp_path / "r")
+    subprocess.run([REF, "json", base], check=True, capture_output=True)
+    with open(base + ".tbc.json") as f, open(os.path.join(GOLDEN, "ref_writer_3fields.tbc.json")) as g:
+        assert f.read() == g.read()

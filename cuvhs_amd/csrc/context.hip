@@ -28,6 +28,10 @@ struct cuvhs_ctx {
     int F = 0;
     hipStream_t stream = nullptr;
     bool own_stream = false;
+    // the chroma pre-band-pass needs only the raw chunk: it runs on a second
+    // stream (default priority) next to the sync/line-location stages
+    hipStream_t side = nullptr;
+    hipEvent_t ev_k1 = nullptr, ev_pre = nullptr, ev_pre0 = nullptr;
     std::string err;
     // designs
     K1Design k1;
@@ -70,6 +74,8 @@ struct cuvhs_ctx {
     int* track = nullptr;
     int* phase = nullptr;
     int *dol = nullptr, *dos = nullptr, *doe = nullptr, *doc = nullptr;
+    double* do_partial = nullptr;
+    unsigned char* do_cls = nullptr;
     unsigned long long* file_loc = nullptr;
     int* isf_out = nullptr;
     ChunkCtl* ctl = nullptr;       // device
@@ -79,7 +85,7 @@ struct cuvhs_ctx {
     static constexpr int kStages = 12;
     bool prof = false;
     hipEvent_t ev[kStages + 1] = {};
-    double stage_ms[kStages] = {};
+    double stage_ms[kStages + 1] = {};  // + the side-stream pre-band-pass
     long long stage_calls = 0;
 };
 
@@ -204,7 +210,7 @@ int k1_launch(cuvhs_ctx* c, const uint8_t* chunk, long long n_valid, int nfk1) {
     return 0;
 }
 
-int prebpf_launch(cuvhs_ctx* c, const uint8_t* chunk, long long n_valid, int nf_max) {
+int prebpf_launch(cuvhs_ctx* c, const uint8_t* chunk, long long n_valid, int nf_max, hipStream_t st) {
     PreLaunch p{};
     p.cap = chunk;
     p.n_valid = n_valid;
@@ -216,7 +222,7 @@ int prebpf_launch(cuvhs_ctx* c, const uint8_t* chunk, long long n_valid, int nf_
     p.G = c->G_p;
     p.tw = c->tw_p;
     p.out = c->craw;
-    HIPCHK(c, run_prebpf(c->pre.bpf.B, p, nf_max, c->stream));
+    HIPCHK(c, run_prebpf(c->pre.bpf.B, p, nf_max, st));
     return 0;
 }
 
@@ -265,6 +271,12 @@ int cuvhs_create(const cuvhs_config* cfg, cuvhs_ctx** out) {
         e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
         if (e != hipSuccess) return bad(e, "hipStreamCreate");
         c->own_stream = true;
+    }
+    e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking);
+    if (e != hipSuccess) return bad(e, "hipStreamCreate side");
+    for (hipEvent_t* ev : {&c->ev_k1, &c->ev_pre, &c->ev_pre0}) {
+        e = hipEventCreate(ev);
+        if (e != hipSuccess) return bad(e, "hipEventCreate");
     }
     try {
         const int bk1[2] = {4096, 8192};
@@ -351,6 +363,8 @@ int cuvhs_create(const cuvhs_config* cfg, cuvhs_ctx** out) {
     ALLOC(c->dos, Fz * kMaxDropouts);
     ALLOC(c->doe, Fz * kMaxDropouts);
     ALLOC(c->doc, Fz);
+    ALLOC(c->do_partial, dropout_partial_count(c->F));
+    ALLOC(c->do_cls, dropout_class_bytes(c->F));
     ALLOC(c->file_loc, Fz);
     ALLOC(c->isf_out, Fz);
     ALLOC(c->ctl, 1);
@@ -367,8 +381,12 @@ int cuvhs_create(const cuvhs_config* cfg, cuvhs_ctx** out) {
 int cuvhs_destroy(cuvhs_ctx* c) {
     if (!c) return CUVHS_E_ARG;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->side) (void)hipStreamSynchronize(c->side);
     for (hipEvent_t e : c->ev)
         if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t e : {c->ev_k1, c->ev_pre, c->ev_pre0})
+        if (e) (void)hipEventDestroy(e);
+    if (c->side) (void)hipStreamDestroy(c->side);
     for (void* p : c->allocs) (void)hipFree(p);
     if (c->hctl) (void)hipHostFree(c->hctl);
     if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
@@ -422,6 +440,12 @@ int cuvhs_decode_chunk(cuvhs_ctx* c, const uint8_t* chunk, uint64_t avail, uint6
     int r = k1_launch(c, chunk, (long long)n_read, fk1);
     if (r) return r;
     mark(c, 2);
+    HIPCHK(c, hipEventRecord(c->ev_k1, c->stream));
+    HIPCHK(c, hipStreamWaitEvent(c->side, c->ev_k1, 0));
+    HIPCHK(c, hipEventRecord(c->ev_pre0, c->side));
+    r = prebpf_launch(c, chunk, (long long)n_read, max_fields, c->side);
+    if (r) return r;
+    HIPCHK(c, hipEventRecord(c->ev_pre, c->side));
     HIPCHK(c, run_events(c->d05, T, f.pulse_threshold_hz, c->wg_count, c->wg_off, c->events, c->ctl, c->stream));
     mark(c, 3);
     HIPCHK(c, run_vsync_cluster(c->events, c->ctl, T, f.vsync_min_w, f.vsync_max_w, fk1 * 20, c->cand, c->offsets,
@@ -438,13 +462,12 @@ int cuvhs_decode_chunk(cuvhs_ctx* c, const uint8_t* chunk, uint64_t avail, uint6
     mark(c, 7);
     HIPCHK(c, run_tbc_resample(c->demod, c->ll, c->luma, c->ctl, 0, max_fields, f, c->stream));
     mark(c, 8);
-    r = prebpf_launch(c, chunk, (long long)n_read, max_fields);
-    if (r) return r;
+    HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_pre, 0));
     mark(c, 9);
     HIPCHK(c, run_chroma(chroma_launch(c, 0), c->ctl, max_fields, c->m0, c->m1, c->cs_tab, c->iq0, c->track, c->phase,
                          c->hctl->chroma.valid ? 0 : 1, c->stream));
     mark(c, 10);
-    HIPCHK(c, run_dropouts(c->env, c->ll, c->ctl, 0, max_fields, f, c->luma, c->chroma, c->dol, c->dos, c->doe,
+    HIPCHK(c, run_dropouts(c->env, c->ll, c->ctl, 0, max_fields, f, c->do_partial, c->do_cls, c->luma, c->chroma, c->dol, c->dos, c->doe,
                            c->doc, c->stream));
     mark(c, 11);
     HIPCHK(c, run_field_meta(c->ctl, c->offsets, c->isf_k2b, c->file_loc, c->isf_out, c->stream));
@@ -456,6 +479,10 @@ int cuvhs_decode_chunk(cuvhs_ctx* c, const uint8_t* chunk, uint64_t avail, uint6
             float ms = 0.0f;
             if (hipEventElapsedTime(&ms, c->ev[i], c->ev[i + 1]) == hipSuccess) c->stage_ms[i] += ms;
         }
+        // the pre-band-pass runs on the side stream: its own duration, and
+        // the main-stream wait for it stays in its slot
+        float pre = 0.0f;
+        if (hipEventElapsedTime(&pre, c->ev_pre0, c->ev_pre) == hipSuccess) c->stage_ms[cuvhs_ctx::kStages] += pre;
         c->stage_calls++;
     }
     info->fields = c->hctl->nf;
@@ -526,9 +553,10 @@ int cuvhs_decode_capture(cuvhs_ctx* c, const uint8_t* cap, uint64_t cap_len, int
     return warn;
 }
 
-static const char* kStageNames[cuvhs_ctx::kStages] = {
+static const char* kStageNames[cuvhs_ctx::kStages + 1] = {
     "k1_analytic", "k1_video", "sync_events", "vsync_cluster", "k2_pulses", "k2b_linelocs",
-    "k4_hsync_refine", "k5_tbc_resample", "k6_prebpf", "k6_chroma", "k7_dropouts", "field_meta"};
+    "k4_hsync_refine", "k5_tbc_resample", "k6_prebpf_wait", "k6_chroma", "k7_dropouts", "field_meta",
+    "k6_prebpf"};
 
 int cuvhs_profile(cuvhs_ctx* c, int enable) {
     if (!c) return CUVHS_E_ARG;
@@ -543,11 +571,11 @@ int cuvhs_profile(cuvhs_ctx* c, int enable) {
 
 int cuvhs_profile_read(const cuvhs_ctx* c, double* ms, const char** names, int n) {
     if (!c || n < 0) return CUVHS_E_ARG;
-    for (int i = 0; i < n && i < cuvhs_ctx::kStages; i++) {
+    for (int i = 0; i < n && i <= cuvhs_ctx::kStages; i++) {
         if (ms) ms[i] = c->stage_ms[i];
         if (names) names[i] = kStageNames[i];
     }
-    return cuvhs_ctx::kStages;
+    return cuvhs_ctx::kStages + 1;
 }
 
 // ---- stage entry points ----
@@ -624,7 +652,7 @@ int cuvhs_tbc_resample(cuvhs_ctx* c, const double* demod, int total, const doubl
 int cuvhs_chroma(cuvhs_ctx* c, const uint8_t* raw, uint64_t n_valid, const double* ll, int nf, uint16_t* out,
                  int* h_phase) {
     if (!c || !raw || !ll || !out || nf < 1 || nf > c->F) return CUVHS_E_ARG;
-    int r = prebpf_launch(c, raw, (long long)n_valid, nf);
+    int r = prebpf_launch(c, raw, (long long)n_valid, nf, c->stream);
     if (r) return r;
     HIPCHK(c, hipMemcpyAsync(c->ll, ll, (size_t)nf * kLinesPerFrame * 8, hipMemcpyDeviceToDevice, c->stream));
     ChromaLaunch a = chroma_launch(c, nf);
@@ -640,7 +668,7 @@ int cuvhs_chroma(cuvhs_ctx* c, const uint8_t* raw, uint64_t n_valid, const doubl
 int cuvhs_dropouts(cuvhs_ctx* c, const double* env, const double* ll, int nf, uint16_t* luma, uint16_t* chroma,
                    int* lines, int* starts, int* ends, int* count) {
     if (!c || !env || !ll || nf < 1 || nf > c->F) return CUVHS_E_ARG;
-    HIPCHK(c, run_dropouts(env, ll, c->ctl, nf, nf, c->fmt, luma, chroma, lines, starts, ends, count, c->stream));
+    HIPCHK(c, run_dropouts(env, ll, c->ctl, nf, nf, c->fmt, c->do_partial, c->do_cls, luma, chroma, lines, starts, ends, count, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return 0;
 }

@@ -6,6 +6,9 @@
 // the Stockham position -> barrier.  The buffer is reused in place because
 // every load of a stage completes before any store of that stage.
 // Sizes: 1024 = 16*16*4, 4096 = 16^3, 8192 = 16^3*2, 16384 = 16^3*4.
+// LDS layout is padded by one element per 16 (element p lives at p + p/16):
+// the first stage stores at a stride of 16 elements, which without padding
+// puts every lane of a wavefront on the same LDS banks.
 // Twiddles come from a table of W_B^m = exp(-2 pi i m / B) in global memory:
 // four loads per butterfly (w1, w2, w4, w8), the other powers by at most
 // three complex products.
@@ -13,6 +16,10 @@
 #include <hip/hip_runtime.h>
 
 namespace cuvhs {
+
+// padded LDS index, and the LDS elements a B-point buffer needs
+__device__ __forceinline__ int pidx(int p) { return p + (p >> 4); }
+constexpr int lds_elems(int B) { return B + B / 16; }
 
 template <typename T> struct C2;
 template <> struct C2<double> { using type = double2; };
@@ -105,29 +112,41 @@ __device__ __forceinline__ void fft_stage(typename C2<T>::type* buf, const typen
     static_assert(PER * R == 16, "each thread owns 16 values");
     V v[16];
     const int t = threadIdx.x;
+    // base twiddles W_B^{e}, e = step * {1, 2, 4, 8}, loaded before the
+    // stage's LDS reads so their latency overlaps the barrier
+    V wb[PER][4];
+    if (ns > 1) {
+#pragma unroll
+        for (int p = 0; p < PER; p++) {
+            int step = ((t + p * NT) % ns) * (B / (ns * R));
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                if ((1 << q) < R) {
+                    V x = tw[(step << q) & (B - 1)];
+                    wb[p][q] = S < 0 ? x : cconj(x);
+                }
+            }
+        }
+    }
 #pragma unroll
     for (int p = 0; p < PER; p++) {
         int j = t + p * NT;
 #pragma unroll
-        for (int r = 0; r < R; r++) v[p * R + r] = buf[j + r * NB];
+        for (int r = 0; r < R; r++) v[p * R + r] = buf[pidx(j + r * NB)];
     }
     __syncthreads();
 #pragma unroll
     for (int p = 0; p < PER; p++) {
-        int j = t + p * NT;
-        int k = j % ns;
         V* u = v + p * R;
         if (ns > 1) {
-            // twiddles W_{ns*R}^{r*k} = W_B^{r*k*B/(ns*R)}
-            int step = k * (B / (ns * R));
+            // twiddles W_{ns*R}^{r*k} = W_B^{r*k*B/(ns*R)}, k = j mod ns
             V w[16];
             w[0].x = (T)1; w[0].y = (T)0;
-            auto ld = [&](int e) -> V { V x = tw[e & (B - 1)]; return S < 0 ? x : cconj(x); };
-            if (R >= 2) w[1] = ld(step);
-            if (R >= 4) { w[2] = ld(2 * step); w[3] = cmul(w[1], w[2]); }
-            if (R >= 8) { w[4] = ld(4 * step); w[5] = cmul(w[4], w[1]); w[6] = cmul(w[4], w[2]); w[7] = cmul(w[4], w[3]); }
+            if (R >= 2) w[1] = wb[p][0];
+            if (R >= 4) { w[2] = wb[p][1]; w[3] = cmul(w[1], w[2]); }
+            if (R >= 8) { w[4] = wb[p][2]; w[5] = cmul(w[4], w[1]); w[6] = cmul(w[4], w[2]); w[7] = cmul(w[4], w[3]); }
             if (R >= 16) {
-                w[8] = ld(8 * step);
+                w[8] = wb[p][3];
 #pragma unroll
                 for (int r = 9; r < 16; r++) w[r] = cmul(w[8], w[r - 8]);
             }
@@ -144,7 +163,23 @@ __device__ __forceinline__ void fft_stage(typename C2<T>::type* buf, const typen
         int k = j % ns;
         int idx = (j / ns) * ns * R + k;
 #pragma unroll
-        for (int r = 0; r < R; r++) buf[idx + r * ns] = v[p * R + r];
+        for (int r = 0; r < R; r++) buf[pidx(idx + r * ns)] = v[p * R + r];
+    }
+    __syncthreads();
+}
+
+// buf[k] *= G[k] for all B bins (G in global memory, loads issued together).
+template <typename T, int B>
+__device__ __forceinline__ void spectrum_multiply(typename C2<T>::type* buf, const typename C2<T>::type* G) {
+    using V = typename C2<T>::type;
+    constexpr int NT = B / 16;
+    V g[16];
+#pragma unroll
+    for (int i = 0; i < 16; i++) g[i] = G[threadIdx.x + i * NT];
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        int j = pidx(threadIdx.x + i * NT);
+        buf[j] = cmul(buf[j], g[i]);
     }
     __syncthreads();
 }

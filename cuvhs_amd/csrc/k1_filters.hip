@@ -28,10 +28,9 @@ __device__ __forceinline__ double u8_to_s16(const uint8_t* cap, long long s, lon
     return s < n_valid ? ((double)cap[s] - 128.0) * 256.0 : 0.0;
 }
 
-__device__ __forceinline__ long long wrap_mod(long long q, int n) {
-    long long r = q % n;
-    return r < 0 ? r + n : r;
-}
+// q mod n for q in (-n, 2n): every window index of these kernels lies there
+// (the block start is within one segment and a block is shorter than n).
+__device__ __forceinline__ int wrap_n(int q, int n) { return q < 0 ? q + n : (q >= n ? q - n : q); }
 
 // Part A: analytic signal, envelope and instantaneous frequency.
 template <int B>
@@ -41,38 +40,42 @@ __global__ __launch_bounds__(B / 16) void k1_part_a(K1ALaunch a) {
     constexpr int NT = B / 16;
     const int f = blockIdx.y;
     const int b = blockIdx.x;
-    const long long qs = (long long)(kOverlap - 1) + (long long)b * a.V;  // first analytic output
-    if (qs + 1 >= (long long)kOverlap + a.spfp) return;
-    const long long seg0 = (long long)f * a.spfp - kOverlap;
-    for (int j = threadIdx.x; j < B; j += NT) {
-        long long q = qs - a.lp + j;
-        long long s = seg0 + wrap_mod(q, a.N);
-        if (s < 0) s = 0;
-        if (s >= a.total) s = a.total - 1;
+    const int qs = (kOverlap - 1) + b * a.V;  // first analytic output
+    if (qs + 1 >= kOverlap + a.spfp) return;
+    const int seg0 = f * a.spfp - kOverlap;
+    const int last = (int)a.total - 1;
+    double xin[16];
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        int s = seg0 + wrap_n(qs - a.lp + (int)threadIdx.x + i * NT, a.N);
+        s = s < 0 ? 0 : (s > last ? last : s);
+        xin[i] = u8_to_s16(a.cap, s, a.n_valid);
+    }
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
         double2 v;
-        v.x = u8_to_s16(a.cap, s, a.n_valid);
+        v.x = xin[i];
         v.y = 0.0;
-        buf[j] = v;
+        buf[pidx(threadIdx.x + i * NT)] = v;
     }
     __syncthreads();
     fft_lds<-1, double, B>(buf, a.tw);
-    for (int j = threadIdx.x; j < B; j += NT) buf[j] = cmul(buf[j], a.G[j]);
-    __syncthreads();
+    spectrum_multiply<double, B>(buf, a.G);
     fft_lds<+1, double, B>(buf, a.tw);
     // outputs o = 0..V: analytic a(qs + o) = buf[lp + o]
     constexpr int MAXO = (B + NT - 1) / NT;
     double ang[MAXO];
-    const long long qend = (long long)kOverlap + a.spfp;  // one past the last demod output
+    const int qend = kOverlap + a.spfp;  // one past the last demod output
 #pragma unroll
     for (int i = 0; i < MAXO; i++) {
         int o = threadIdx.x + i * NT;
         ang[i] = 0.0;
         if (o <= a.V) {
-            double2 z = buf[a.lp + o];
+            double2 z = buf[pidx(a.lp + o)];
             ang[i] = atan2(z.y, z.x);
-            long long q = qs + o;
+            int q = qs + o;
             if (q >= kOverlap && q < qend) {
-                long long s = (long long)f * a.spfp + (q - kOverlap);
+                int s = f * a.spfp + (q - kOverlap);
                 a.env[s] = sqrt(z.x * z.x + z.y * z.y);
             }
         }
@@ -81,7 +84,7 @@ __global__ __launch_bounds__(B / 16) void k1_part_a(K1ALaunch a) {
 #pragma unroll
     for (int i = 0; i < MAXO; i++) {
         int o = threadIdx.x + i * NT;
-        if (o <= a.V) buf[a.lp + o].x = ang[i];
+        if (o <= a.V) buf[pidx(a.lp + o)].x = ang[i];
     }
     __syncthreads();
     const double PI = 3.14159265358979323846;
@@ -90,12 +93,12 @@ __global__ __launch_bounds__(B / 16) void k1_part_a(K1ALaunch a) {
     for (int i = 0; i < MAXO; i++) {
         int o = threadIdx.x + i * NT;
         if (o >= 1 && o <= a.V) {
-            long long q = qs + o;
+            int q = qs + o;
             if (q < qend) {
-                double delta = ang[i] - buf[a.lp + o - 1].x;
+                double delta = ang[i] - buf[pidx(a.lp + o - 1)].x;
                 if (delta > PI) delta -= TAU;
                 if (delta < -PI) delta += TAU;
-                long long s = (long long)f * a.spfp + (q - kOverlap);
+                int s = f * a.spfp + (q - kOverlap);
                 a.rawdemod[s] = delta * a.scale;
             }
         }
@@ -111,30 +114,34 @@ __global__ __launch_bounds__(B / 16) void k1_part_b(K1BLaunch a) {
     constexpr int NT = B / 16;
     const int f = blockIdx.y;
     const int b = blockIdx.x;
-    const long long qs = (long long)kOverlap + (long long)b * a.V;
-    const long long qend = (long long)kOverlap + a.spfp;
+    const int qs = kOverlap + b * a.V;
+    const int qend = kOverlap + a.spfp;
     if (qs >= qend) return;
-    const long long seg0 = (long long)f * a.spfp - kOverlap;
-    for (int j = threadIdx.x; j < B; j += NT) {
-        long long q = qs - a.lp + j;
-        long long s = seg0 + wrap_mod(q, a.N);
-        if (s < 0) s = 0;
-        if (s >= a.total) s = a.total - 1;
+    const int seg0 = f * a.spfp - kOverlap;
+    const int last = (int)a.total - 1;
+    double xin[16];
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        int s = seg0 + wrap_n(qs - a.lp + (int)threadIdx.x + i * NT, a.N);
+        s = s < 0 ? 0 : (s > last ? last : s);
+        xin[i] = a.rawdemod[s];
+    }
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
         double2 v;
-        v.x = a.rawdemod[s];
+        v.x = xin[i];
         v.y = 0.0;
-        buf[j] = v;
+        buf[pidx(threadIdx.x + i * NT)] = v;
     }
     __syncthreads();
     fft_lds<-1, double, B>(buf, a.tw);
-    for (int j = threadIdx.x; j < B; j += NT) buf[j] = cmul(buf[j], a.G[j]);
-    __syncthreads();
+    spectrum_multiply<double, B>(buf, a.G);
     fft_lds<+1, double, B>(buf, a.tw);
     for (int o = threadIdx.x; o < a.V; o += NT) {
-        long long q = qs + o;
+        int q = qs + o;
         if (q >= qend) break;
-        double2 z = buf[a.lp + o];
-        long long s = (long long)f * a.spfp + (q - kOverlap);
+        double2 z = buf[pidx(a.lp + o)];
+        int s = f * a.spfp + (q - kOverlap);
         a.demod[s] = z.x;
         a.demod05[s] = z.y;
     }
@@ -149,25 +156,30 @@ __global__ __launch_bounds__(B / 16) void k_prebpf(PreLaunch a) {
     constexpr int NT = B / 16;
     const int f = blockIdx.y;
     const int b = blockIdx.x;
-    const long long ns = (long long)b * a.V;
+    const int ns = b * a.V;
     if (ns >= a.spfp) return;
-    const long long seg0 = (long long)f * a.spfp;
-    for (int j = threadIdx.x; j < B; j += NT) {
-        long long n = wrap_mod(ns - a.lp + j, a.N2);
+    const int seg0 = f * a.spfp;
+    float xin[16];
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        int n = wrap_n(ns - a.lp + (int)threadIdx.x + i * NT, a.N2);
+        xin[i] = n < a.spfp ? (float)u8_to_s16(a.cap, seg0 + n, a.n_valid) : 0.0f;
+    }
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
         float2 v;
-        v.x = n < a.spfp ? (float)u8_to_s16(a.cap, seg0 + n, a.n_valid) : 0.0f;
+        v.x = xin[i];
         v.y = 0.0f;
-        buf[j] = v;
+        buf[pidx(threadIdx.x + i * NT)] = v;
     }
     __syncthreads();
     fft_lds<-1, float, B>(buf, a.tw);
-    for (int j = threadIdx.x; j < B; j += NT) buf[j] = cmul(buf[j], a.G[j]);
-    __syncthreads();
+    spectrum_multiply<float, B>(buf, a.G);
     fft_lds<+1, float, B>(buf, a.tw);
     for (int o = threadIdx.x; o < a.V; o += NT) {
-        long long n = ns + o;
+        int n = ns + o;
         if (n >= a.spfp) break;
-        a.out[seg0 + n] = buf[a.lp + o].x;
+        a.out[seg0 + n] = buf[pidx(a.lp + o)].x;
     }
 }
 
@@ -181,7 +193,7 @@ __global__ void k_fill_f64(double* p, long long n, double v) {
 template <int B>
 static hipError_t launch_a(const K1ALaunch& a, int nf, hipStream_t st) {
     int nb = (int)((a.spfp + a.V - 1) / a.V);
-    size_t sh = (size_t)B * sizeof(double2);
+    size_t sh = (size_t)lds_elems(B) * sizeof(double2);
     (void)hipFuncSetAttribute((const void*)k1_part_a<B>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
     hipLaunchKernelGGL(k1_part_a<B>, dim3(nb, nf), dim3(B / 16), sh, st, a);
     return hipGetLastError();
@@ -189,7 +201,7 @@ static hipError_t launch_a(const K1ALaunch& a, int nf, hipStream_t st) {
 template <int B>
 static hipError_t launch_b(const K1BLaunch& a, int nf, hipStream_t st) {
     int nb = (int)((a.spfp + a.V - 1) / a.V);
-    size_t sh = (size_t)B * sizeof(double2);
+    size_t sh = (size_t)lds_elems(B) * sizeof(double2);
     (void)hipFuncSetAttribute((const void*)k1_part_b<B>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
     hipLaunchKernelGGL(k1_part_b<B>, dim3(nb, nf), dim3(B / 16), sh, st, a);
     return hipGetLastError();
@@ -197,7 +209,7 @@ static hipError_t launch_b(const K1BLaunch& a, int nf, hipStream_t st) {
 template <int B>
 static hipError_t launch_pre(const PreLaunch& a, int nf, hipStream_t st) {
     int nb = (int)((a.spfp + a.V - 1) / a.V);
-    size_t sh = (size_t)B * sizeof(float2);
+    size_t sh = (size_t)lds_elems(B) * sizeof(float2);
     (void)hipFuncSetAttribute((const void*)k_prebpf<B>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
     hipLaunchKernelGGL(k_prebpf<B>, dim3(nb, nf), dim3(B / 16), sh, st, a);
     return hipGetLastError();
@@ -231,17 +243,17 @@ __global__ __launch_bounds__(B / 16) void k_fft_selftest(const typename C2<T>::t
     using V = typename C2<T>::type;
     extern __shared__ __align__(16) unsigned char smem[];
     V* buf = reinterpret_cast<V*>(smem);
-    for (int j = threadIdx.x; j < B; j += B / 16) buf[j] = in[j];
+    for (int j = threadIdx.x; j < B; j += B / 16) buf[pidx(j)] = in[j];
     __syncthreads();
     if (sign < 0) fft_lds<-1, T, B>(buf, tw);
     else fft_lds<+1, T, B>(buf, tw);
-    for (int j = threadIdx.x; j < B; j += B / 16) out[j] = buf[j];
+    for (int j = threadIdx.x; j < B; j += B / 16) out[j] = buf[pidx(j)];
 }
 
 template <int B, typename T>
 static hipError_t st_launch(const void* in, void* out, const void* tw, int sign, hipStream_t st) {
     using V = typename C2<T>::type;
-    size_t sh = (size_t)B * sizeof(V);
+    size_t sh = (size_t)lds_elems(B) * sizeof(V);
     (void)hipFuncSetAttribute((const void*)k_fft_selftest<B, T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
     hipLaunchKernelGGL((k_fft_selftest<B, T>), dim3(1), dim3(B / 16), sh, st, (const V*)in, (V*)out, (const V*)tw,
                        sign);

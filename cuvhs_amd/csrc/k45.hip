@@ -11,90 +11,108 @@
 
 namespace cuvhs {
 
-__device__ __forceinline__ double find_falling(const double* sig, int start, int count, double target, int total) {
+// Wavefront helpers: all 64 lanes of a wavefront cooperate on one line; every
+// lane returns the same value.
+__device__ __forceinline__ double wv_find_falling(const double* sig, int start, int count, double target, int total,
+                                                  int lane) {
     if (start < 0 || start + count > total || count < 2) return -1.0;
-    double prev = sig[start];
-    for (int i = start + 1; i < start + count; i++) {
-        double curr = sig[i];
-        if (prev >= target && curr < target) {
-            double a = prev - target, b = curr - target;
-            double frac = (a - b != 0.0) ? (a / (a - b)) : 0.0;
-            return (double)(i - 1) + frac;
+    for (int b = start + 1; b < start + count; b += 64) {
+        int i = b + lane;
+        bool c = false;
+        if (i < start + count) c = sig[i - 1] >= target && sig[i] < target;
+        unsigned long long m = __ballot(c);
+        if (m) {
+            int f = b + __ffsll((long long)m) - 1;   // first crossing (hsync_refine.cu:29-39)
+            double a = sig[f - 1] - target, bb = sig[f] - target;
+            double frac = (a - bb != 0.0) ? (a / (a - bb)) : 0.0;
+            return (double)(f - 1) + frac;
         }
-        prev = curr;
     }
     return -1.0;
 }
 
-__device__ __forceinline__ double find_rising(const double* sig, int start, int count, double target, int total) {
+__device__ __forceinline__ double wv_find_rising(const double* sig, int start, int count, double target, int total,
+                                                 int lane) {
     if (start < 0 || start + count > total || count < 2) return -1.0;
     if (sig[start] > target) return -1.0;
-    double prev = sig[start];
-    for (int i = start + 1; i < start + count; i++) {
-        double curr = sig[i];
-        if (prev < target && curr >= target) {
-            double a = target - prev, b = target - curr;
-            double frac = (a + b != 0.0) ? (a / (a - b)) : 0.0;
-            return (double)(i - 1) + frac;
+    for (int b = start + 1; b < start + count; b += 64) {
+        int i = b + lane;
+        bool c = false;
+        if (i < start + count) c = sig[i - 1] < target && sig[i] >= target;
+        unsigned long long m = __ballot(c);
+        if (m) {
+            int f = b + __ffsll((long long)m) - 1;   // hsync_refine.cu:57-67
+            double a = target - sig[f - 1], bb = target - sig[f];
+            double frac = (a + bb != 0.0) ? (a / (a - bb)) : 0.0;
+            return (double)(f - 1) + frac;
         }
-        prev = curr;
     }
     return -1.0;
 }
 
-// Median of up to 64 samples by partial selection (hsync_refine.cu:72-97).
-__device__ __forceinline__ double window_median(const double* sig, int start, int count, int total) {
+// The n/2-th smallest of up to 64 samples, which is what the reference's
+// partial selection sort leaves at buf[n/2] (hsync_refine.cu:72-97): lane j
+// holds sample j and counts its stable rank.
+__device__ __forceinline__ double wv_window_median(const double* sig, int start, int count, int total, int lane) {
     if (start < 0 || start + count > total || count <= 0) return 0.0;
-    double buf[64];
-    int n = count < 64 ? count : 64;
-    for (int i = 0; i < n; i++) buf[i] = sig[start + i];
-    int mid = n / 2;
-    for (int i = 0; i <= mid; i++)
-        for (int j = i + 1; j < n; j++)
-            if (buf[j] < buf[i]) { double t = buf[i]; buf[i] = buf[j]; buf[j] = t; }
-    return buf[mid];
+    const int n = count < 64 ? count : 64;
+    const int mid = n / 2;
+    double v = lane < n ? sig[start + lane] : 0.0;
+    int r = 0;
+    for (int k = 0; k < n; k++) {
+        double vk = __shfl(v, k, 64);
+        r += (vk < v) || (vk == v && k < lane);
+    }
+    unsigned long long m = __ballot(lane < n && r == mid);
+    return __shfl(v, __ffsll((long long)m) - 1, 64);
 }
 
+// One wavefront per line (hsync_refine.cu:99-239).
 __global__ __launch_bounds__(256) void k_hsync_refine(const double* __restrict__ d05, double* __restrict__ ll,
                                                       const ChunkCtl* ctl, int nf_override, int total_per_field,
                                                       double thr, double one_usec_samples, int als, double hw,
                                                       double right_edge_offset) {
     const int nf = nf_override > 0 ? nf_override : ctl->nf;
-    int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const int idx = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (idx >= nf * kLinesPerFrame) return;
-    int line = idx % kLinesPerFrame;
+    const int line = idx % kLinesPerFrame;
     if (line < als) return;
     const int total = nf * total_per_field;
-    double original = ll[idx];
-    int one_usec = (int)(one_usec_samples + 0.5);
-    int search_start = (int)(original - one_usec);
-    int search_count = one_usec * 2;
-    double zc = find_falling(d05, search_start, search_count, thr, total);
+    const double original = ll[idx];
+    const int one_usec = (int)(one_usec_samples + 0.5);
+    const int search_start = (int)(original - one_usec);
+    const int search_count = one_usec * 2;
+    double zc = wv_find_falling(d05, search_start, search_count, thr, total, lane);
     double left = -1.0;
     if (zc >= 0.0) {
         int zci = (int)(zc + 0.5);
-        double sync_level = window_median(d05, zci + (int)(1.0 * one_usec_samples), (int)(1.5 * one_usec_samples), total);
-        double porch_level = window_median(d05, zci + (int)(8.0 * one_usec_samples), (int)(1.0 * one_usec_samples), total);
+        double sync_level = wv_window_median(d05, zci + (int)(1.0 * one_usec_samples), (int)(1.5 * one_usec_samples),
+                                             total, lane);
+        double porch_level = wv_window_median(d05, zci + (int)(8.0 * one_usec_samples), (int)(1.0 * one_usec_samples),
+                                              total, lane);
         if (porch_level > sync_level) {
             double mid = (sync_level + porch_level) / 2.0;
             int rs = (int)(original - one_usec * 2);
             int rc = (int)(one_usec_samples * 4);
-            double zc2 = find_falling(d05, rs, rc, mid, total);
+            double zc2 = wv_find_falling(d05, rs, rc, mid, total, lane);
             if (zc2 >= 0.0 && fabs(zc2 - zc) <= one_usec_samples * 0.5) left = zc2;
         }
     }
     double right = -1.0;
-    int rs0 = (int)(original + hw - one_usec_samples);
-    int rc0 = (int)(hw * 2);
-    double rcross = find_rising(d05, rs0, rc0, thr, total);
+    const int rs0 = (int)(original + hw - one_usec_samples);
+    const int rc0 = (int)(hw * 2);
+    double rcross = wv_find_rising(d05, rs0, rc0, thr, total, lane);
     if (rcross >= 0.0) {
         double zc_fr = rcross - hw;
         int zfi = (int)(zc_fr + 0.5);
-        double rsync = window_median(d05, zfi + (int)(1.0 * one_usec_samples), (int)(1.5 * one_usec_samples), total);
-        double rporch = window_median(d05, zfi + (int)(hw + 1.0 * one_usec_samples), (int)(1.0 * one_usec_samples), total);
+        double rsync = wv_window_median(d05, zfi + (int)(1.0 * one_usec_samples), (int)(1.5 * one_usec_samples),
+                                        total, lane);
+        double rporch = wv_window_median(d05, zfi + (int)(hw + 1.0 * one_usec_samples), (int)(1.0 * one_usec_samples),
+                                         total, lane);
         if (rporch > rsync) {
             double rmid = (rsync + rporch) / 2.0;
-            double zc2r = find_rising(d05, rs0, rc0, rmid, total);
+            double zc2r = wv_find_rising(d05, rs0, rc0, rmid, total, lane);
             if (zc2r >= 0.0 && fabs(zc2r - rcross) <= one_usec_samples * 0.5) {
                 double cand = rcross - hw + right_edge_offset;
                 double ref = (left >= 0.0) ? left : original;
@@ -102,8 +120,10 @@ __global__ __launch_bounds__(256) void k_hsync_refine(const double* __restrict__
             }
         }
     }
-    if (right >= 0.0) ll[idx] = right;
-    else if (left >= 0.0) ll[idx] = left;
+    if (lane == 0) {
+        if (right >= 0.0) ll[idx] = right;
+        else if (left >= 0.0) ll[idx] = left;
+    }
 }
 
 // One thread per output pixel; consecutive threads walk a TBC row, so the
@@ -151,7 +171,7 @@ hipError_t run_hsync_refine(const double* d05, double* ll, const ChunkCtl* ctl, 
     int n = max_fields * kLinesPerFrame;
     double ous = 1.0e-6 * fmt.sample_rate;
     double reo = 2.25 * ((fmt.sample_rate / 1.0e6) / 40.0);
-    hipLaunchKernelGGL(k_hsync_refine, dim3((n + 255) / 256), dim3(256), 0, st, d05, ll, ctl, nf_override, fmt.spfp,
+    hipLaunchKernelGGL(k_hsync_refine, dim3((n + 3) / 4), dim3(256), 0, st, d05, ll, ctl, nf_override, fmt.spfp,
                        fmt.pulse_threshold_hz, ous, fmt.active_line_start, fmt.hsync_width, reo);
     return hipGetLastError();
 }

@@ -31,7 +31,7 @@ constexpr int kPairs = (kFieldLines - 2 * kSkip - 1) / 2;  // 115
 // ACC + u16 (:188-231).
 template <int OUT>
 __global__ __launch_bounds__(64) void k_chroma_line(ChromaLaunch a) {
-    __shared__ __align__(16) double2 buf[kChromaFFT];
+    __shared__ __align__(16) double2 buf[lds_elems(kChromaFFT)];
     __shared__ double s_acc;
     const int f = blockIdx.y;
     const int nf = a.nf_override > 0 ? a.nf_override : a.ctl->nf;
@@ -65,7 +65,7 @@ __global__ __launch_bounds__(64) void k_chroma_line(ChromaLaunch a) {
         double2 z;
         z.x = v;
         z.y = 0.0;
-        buf[col] = z;
+        buf[pidx(col)] = z;
     }
     __syncthreads();
     fft_lds<-1, double, kChromaFFT>(buf, a.tw);
@@ -75,20 +75,20 @@ __global__ __launch_bounds__(64) void k_chroma_line(ChromaLaunch a) {
             double2 z;
             z.x = 0.0;
             z.y = 0.0;
-            buf[k] = z;
+            buf[pidx(k)] = z;
         }
     }
     __syncthreads();
     fft_lds<+1, double, kChromaFFT>(buf, a.tw);
     if (!OUT) {
         double* dst = a.burst + ((size_t)f * kFieldLines + line) * kBurstLen;
-        for (int i = threadIdx.x; i < kBurstLen; i += 64) dst[i] = buf[a.burst_start + i].x * a.fft_scale;
+        for (int i = threadIdx.x; i < kBurstLen; i += 64) dst[i] = buf[pidx(a.burst_start + i)].x * a.fft_scale;
         return;
     }
     if (threadIdx.x == 0) {
         double ssq = 0.0;
         for (int i = 0; i < kBurstLen; i++) {
-            double v = buf[a.burst_start + i].x * a.fft_scale;
+            double v = buf[pidx(a.burst_start + i)].x * a.fft_scale;
             ssq += v * v;
         }
         double rms = sqrt(ssq / (double)kBurstLen);
@@ -98,7 +98,7 @@ __global__ __launch_bounds__(64) void k_chroma_line(ChromaLaunch a) {
     const double acc = s_acc;
     uint16_t* o = a.out + ((size_t)f * kFieldLines + line) * kLineLen;
     for (int col = threadIdx.x; col < kLineLen; col += 64) {
-        double v = buf[col].x * a.fft_scale * acc;
+        double v = buf[pidx(col)].x * a.fft_scale * acc;
         double ov = v + 32768.0;
         if (ov < 0.0) ov = 0.0;
         if (ov > 65535.0) ov = 65535.0;
@@ -106,30 +106,35 @@ __global__ __launch_bounds__(64) void k_chroma_line(ChromaLaunch a) {
     }
 }
 
-// Per-field burst-cancellation metrics for both tracks (chroma_decode.cu:383-411
-// summation order) and, for field 0, per-line burst I/Q (:415-450).
+// Per-field burst-cancellation metrics for both tracks and, for field 0,
+// per-line burst I/Q.  One wavefront per field: lanes evaluate pairs, lane 0
+// accumulates them in pair order (chroma_decode.cu:383-411, :415-450).
 __global__ __launch_bounds__(64) void k_chroma_metrics(const double* burst, const ChunkCtl* ctl, int nf_override,
                                                        double* m0, double* m1, const double* cs_tab, double* iq0) {
+    __shared__ double s0[128], s1[128];
     const int nf = nf_override > 0 ? nf_override : ctl->nf;
-    const int f = blockIdx.x * blockDim.x + threadIdx.x;
-    if (f < nf) {
-        double t0 = 0.0, t1 = 0.0;
-        for (int p = 0; p < kPairs; p++) {
-            const double* ra = burst + ((size_t)f * kFieldLines + kSkip + 2 * p) * kBurstLen;
-            const double* rb = ra + kBurstLen;
-            double s0 = 0.0, s1 = 0.0;
-            for (int i = 0; i < kBurstLen; i++) {
-                s0 += fabs(ra[i] + rb[i]);
-                s1 += fabs(ra[i] + (-rb[i]));
-            }
-            t0 += s0 / kBurstLen;
-            t1 += s1 / kBurstLen;
+    const int f = blockIdx.x;
+    if (f >= nf) return;
+    for (int p = threadIdx.x; p < kPairs; p += 64) {
+        const double* ra = burst + ((size_t)f * kFieldLines + kSkip + 2 * p) * kBurstLen;
+        const double* rb = ra + kBurstLen;
+        double a0 = 0.0, a1 = 0.0;
+        for (int i = 0; i < kBurstLen; i++) {
+            a0 += fabs(ra[i] + rb[i]);
+            a1 += fabs(ra[i] + (-rb[i]));
         }
+        s0[p] = a0 / kBurstLen;
+        s1[p] = a1 / kBurstLen;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t0 = 0.0, t1 = 0.0;
+        for (int p = 0; p < kPairs; p++) { t0 += s0[p]; t1 += s1[p]; }
         m0[f] = t0 / kPairs;
         m1[f] = t1 / kPairs;
     }
-    if (blockIdx.x == 0 && iq0) {
-        for (int line = kSkip + threadIdx.x; line < kFieldLines - kSkip; line += blockDim.x) {
+    if (f == 0 && iq0) {
+        for (int line = kSkip + threadIdx.x; line < kFieldLines - kSkip; line += 64) {
             const double* r = burst + (size_t)line * kBurstLen;
             double I = 0.0, Q = 0.0;
             for (int i = 0; i < kBurstLen; i++) {
@@ -238,7 +243,7 @@ hipError_t run_chroma(ChromaLaunch a, ChunkCtl* ctl_mut, int max_fields, double*
     m.track = nullptr;
     m.line0 = kSkip;
     hipLaunchKernelGGL(k_chroma_line<0>, dim3(kFieldLines - 2 * kSkip, max_fields), dim3(64), 0, st, m);
-    hipLaunchKernelGGL(k_chroma_metrics, dim3((max_fields + 63) / 64), dim3(64), 0, st, a.burst, a.ctl,
+    hipLaunchKernelGGL(k_chroma_metrics, dim3(max_fields), dim3(64), 0, st, a.burst, a.ctl,
                        a.nf_override, m0, m1, cs_tab, need_iq ? iq0 : nullptr);
     hipLaunchKernelGGL(k_chroma_resolve, dim3(1), dim3(64), 0, st, ctl_mut, a.nf_override, m0, m1, iq0, out_track,
                        phase_ids);

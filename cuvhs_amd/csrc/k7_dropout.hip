@@ -36,13 +36,85 @@ __device__ __forceinline__ int find_tbc_line(const double* ll, int als, int n_li
     return lo;
 }
 
-__global__ __launch_bounds__(kDoThreads) void k_dropouts(const double* __restrict__ env, const double* __restrict__ linelocs,
+constexpr int kSumSlices = 16;
+constexpr int kClassSlices = 32;
+
+struct DoWindow {
+    int start, end;
+};
+
+// scan window of dropout_detect.cu:106-114 (field_start uses the padded stride)
+__device__ __forceinline__ DoWindow do_window(const double* linelocs, int f, int spfp, int als) {
+    const double* fll = linelocs + (size_t)f * kLinesPerFrame;
+    const int field_start = f * spfp;
+    DoWindow w;
+    w.start = max((int)fll[als], field_start);
+    w.end = min((int)fll[als + kFieldLines], field_start + spfp);
+    return w;
+}
+
+// Partial sums of the field's envelope over [f*spfp, (f+1)*spfp), only for
+// fields with a non-empty scan window (the mean is unused otherwise).
+__global__ __launch_bounds__(kDoThreads) void k_do_sum(const double* __restrict__ env, const double* __restrict__ linelocs,
+                                                        const ChunkCtl* ctl, int nf_override, int spfp, int als,
+                                                        double* partial) {
+    __shared__ double red[kDoThreads];
+    const int f = blockIdx.x, sl = blockIdx.y, t = threadIdx.x;
+    const int nf = nf_override > 0 ? nf_override : ctl->nf;
+    if (f >= nf) return;
+    DoWindow w = do_window(linelocs, f, spfp, als);
+    if (w.end <= w.start) return;
+    const int L = (spfp + kSumSlices - 1) / kSumSlices;
+    const int a = sl * L, b = min(spfp, a + L);
+    const double* fe = env + (size_t)f * spfp;
+    double s = 0.0;
+    for (int i = a + t; i < b; i += kDoThreads) s += fe[i];
+    red[t] = s;
+    __syncthreads();
+    for (int o = kDoThreads / 2; o > 0; o >>= 1) {
+        if (t < o) red[t] += red[t + o];
+        __syncthreads();
+    }
+    if (t == 0) partial[f * kSumSlices + sl] = red[0];
+}
+
+__device__ __forceinline__ double field_mean(const double* partial, int f, int spfp) {
+    double s = 0.0;
+    for (int k = 0; k < kSumSlices; k++) s += partial[f * kSumSlices + k];
+    return s / (double)spfp;
+}
+
+// 16-sample block means vs the hysteresis thresholds: 1 = <= down, 2 = >= up.
+__global__ __launch_bounds__(kDoThreads) void k_do_class(const double* __restrict__ env, const double* __restrict__ linelocs,
+                                                          const ChunkCtl* ctl, int nf_override, int spfp, int als,
+                                                          const double* partial, unsigned char* cls) {
+    const int f = blockIdx.x, sl = blockIdx.y;
+    const int nf = nf_override > 0 ? nf_override : ctl->nf;
+    if (f >= nf) return;
+    DoWindow w = do_window(linelocs, f, spfp, als);
+    if (w.end <= w.start) return;
+    const double down = field_mean(partial, f, spfp) * 0.18;
+    const double up = down * 1.25;
+    const int nblk = min((w.end - w.start + kEnvBlock - 1) / kEnvBlock, kMaxClassBlocks);
+    const int per = (nblk + kClassSlices - 1) / kClassSlices;
+    const int b0 = sl * per, b1 = min(nblk, b0 + per);
+    for (int b = b0 + threadIdx.x; b < b1; b += kDoThreads) {
+        int bs = w.start + b * kEnvBlock;
+        int be = min(bs + kEnvBlock, w.end);
+        double bsum = 0.0;
+        for (int i = bs; i < be; i++) bsum += env[i];
+        double bm = bsum / (double)(be - bs);
+        cls[(size_t)f * kMaxClassBlocks + b] = bm <= down ? 1 : (bm >= up ? 2 : 0);
+    }
+}
+
+__global__ __launch_bounds__(kDoThreads) void k_dropouts(const unsigned char* __restrict__ gcls,
+                                                          const double* __restrict__ linelocs,
                                                           const ChunkCtl* ctl, int nf_override, int spfp, int als,
                                                           uint16_t* luma, uint16_t* chroma, int* do_lines,
                                                           int* do_starts, int* do_ends, int* do_count) {
     __shared__ unsigned char cls[kMaxClassBlocks];
     __shared__ int ev[kMaxEvents];
-    __shared__ double red[kDoThreads];
     __shared__ int s_last[kDoThreads];
     __shared__ int s_cnt[kDoThreads];
     __shared__ int s_ent[3 * kMaxDropouts];
@@ -52,19 +124,6 @@ __global__ __launch_bounds__(kDoThreads) void k_dropouts(const double* __restric
     const int nf = nf_override > 0 ? nf_override : ctl->nf;
     if (f >= nf) return;
     const int t = threadIdx.x;
-    // field mean envelope over [f*spfp, (f+1)*spfp) (dropout_detect.cu:17-53)
-    const double* fe = env + (size_t)f * spfp;
-    double s = 0.0;
-    for (int i = t; i < spfp; i += kDoThreads) s += fe[i];
-    red[t] = s;
-    __syncthreads();
-    for (int o = kDoThreads / 2; o > 0; o >>= 1) {
-        if (t < o) red[t] += red[t + o];
-        __syncthreads();
-    }
-    const double mean_env = red[0] / (double)spfp;
-    const double down = mean_env * 0.18;
-    const double up = down * 1.25;
     const double* fll = linelocs + (size_t)f * kLinesPerFrame;
     const int field_start = f * spfp;
     const int scan_start = max((int)fll[als], field_start);
@@ -73,15 +132,8 @@ __global__ __launch_bounds__(kDoThreads) void k_dropouts(const double* __restric
     if (scan_end > scan_start) {
         const int nblk = (scan_end - scan_start + kEnvBlock - 1) / kEnvBlock;
         const int nb = min(nblk, kMaxClassBlocks);
-        // classes: 0 neutral, 1 down (<= down), 2 up (>= up)
-        for (int b = t; b < nb; b += kDoThreads) {
-            int bs = scan_start + b * kEnvBlock;
-            int be = min(bs + kEnvBlock, scan_end);
-            double bsum = 0.0;
-            for (int i = bs; i < be; i++) bsum += env[i];
-            double bm = bsum / (double)(be - bs);
-            cls[b] = bm <= down ? 1 : (bm >= up ? 2 : 0);
-        }
+        // classes from k_do_class: 0 neutral, 1 down, 2 up
+        for (int b = t; b < nb; b += kDoThreads) cls[b] = gcls[(size_t)f * kMaxClassBlocks + b];
         __syncthreads();
         const int per = (nb + kDoThreads - 1) / kDoThreads;
         const int b0 = t * per, b1 = min(nb, b0 + per);
@@ -224,10 +276,18 @@ __global__ void k_field_meta(ChunkCtl* ctl, const unsigned long long* offsets, c
     ctl->last_parity = last;
 }
 
+size_t dropout_class_bytes(int max_fields) { return (size_t)max_fields * kMaxClassBlocks; }
+size_t dropout_partial_count(int max_fields) { return (size_t)max_fields * kSumSlices; }
+
 hipError_t run_dropouts(const double* env, const double* ll, const ChunkCtl* ctl, int nf_override, int max_fields,
-                        const Fmt& fmt, uint16_t* luma, uint16_t* chroma, int* lines, int* starts, int* ends,
-                        int* count, hipStream_t st) {
-    hipLaunchKernelGGL(k_dropouts, dim3(max_fields), dim3(kDoThreads), 0, st, env, ll, ctl, nf_override, fmt.spfp,
+                        const Fmt& fmt, double* partial, unsigned char* cls, uint16_t* luma, uint16_t* chroma,
+                        int* lines, int* starts, int* ends, int* count, hipStream_t st) {
+    if ((fmt.spfp + kEnvBlock - 1) / kEnvBlock > kMaxClassBlocks) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_do_sum, dim3(max_fields, kSumSlices), dim3(kDoThreads), 0, st, env, ll, ctl, nf_override,
+                       fmt.spfp, fmt.active_line_start, partial);
+    hipLaunchKernelGGL(k_do_class, dim3(max_fields, kClassSlices), dim3(kDoThreads), 0, st, env, ll, ctl, nf_override,
+                       fmt.spfp, fmt.active_line_start, partial, cls);
+    hipLaunchKernelGGL(k_dropouts, dim3(max_fields), dim3(kDoThreads), 0, st, cls, ll, ctl, nf_override, fmt.spfp,
                        fmt.active_line_start, luma, chroma, lines, starts, ends, count);
     return hipGetLastError();
 }

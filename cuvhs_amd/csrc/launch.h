@@ -80,9 +80,11 @@ hipError_t run_tbc_resample(const double* demod, const double* ll, uint16_t* lum
 hipError_t run_chroma(ChromaLaunch a, ChunkCtl* ctl_mut, int max_fields, double* m0, double* m1,
                       const double* cs_tab, double* iq0, int* out_track, int* phase_ids, int need_iq,
                       hipStream_t st);
+size_t dropout_class_bytes(int max_fields);
+size_t dropout_partial_count(int max_fields);
 hipError_t run_dropouts(const double* env, const double* ll, const ChunkCtl* ctl, int nf_override, int max_fields,
-                        const Fmt& fmt, uint16_t* luma, uint16_t* chroma, int* lines, int* starts, int* ends,
-                        int* count, hipStream_t st);
+                        const Fmt& fmt, double* partial, unsigned char* cls, uint16_t* luma, uint16_t* chroma,
+                        int* lines, int* starts, int* ends, int* count, hipStream_t st);
 hipError_t run_field_meta(ChunkCtl* ctl, const unsigned long long* offsets, const int* is_first_k2b,
                           unsigned long long* file_loc, int* is_first_out, hipStream_t st);
 

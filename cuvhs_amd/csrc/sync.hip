@@ -23,8 +23,6 @@ constexpr int kEvThreads = 256;
 constexpr int kEvPerThread = 32;
 constexpr int kEvSpan = kEvThreads * kEvPerThread;  // samples per workgroup
 
-__device__ __forceinline__ bool below(const double* x, long long i, double thr) { return x[i] <= thr; }
-
 // events of sample i: bit0 = start, bit1 = end
 __device__ __forceinline__ int event_at(const double* x, long long i, double thr) {
     bool b = x[i] <= thr;
@@ -35,37 +33,24 @@ __device__ __forceinline__ int event_at(const double* x, long long i, double thr
     return 0;
 }
 
-template <typename F>
-__device__ __forceinline__ int block_exclusive_scan(int v, int* sh, int* total, F) {
-    // sh: kEvThreads ints
-    const int t = threadIdx.x;
-    sh[t] = v;
-    __syncthreads();
-    for (int off = 1; off < kEvThreads; off <<= 1) {
-        int add = t >= off ? sh[t - off] : 0;
-        __syncthreads();
-        sh[t] += add;
-        __syncthreads();
-    }
-    int incl = sh[t];
-    *total = sh[kEvThreads - 1];
-    __syncthreads();
-    return incl - v;
-}
-
-struct Dummy {};
-
+// Sample i of workgroup g, iteration k, thread t: g*kEvSpan + k*256 + t, so
+// each load instruction of a wavefront reads 64 consecutive doubles.
 __global__ __launch_bounds__(kEvThreads) void k_ev_count(const double* x, long long total, double thr, int* wg_count) {
-    __shared__ int sh[kEvThreads];
-    long long base = (long long)blockIdx.x * kEvSpan + (long long)threadIdx.x * kEvPerThread;
+    __shared__ int s_total;
+    if (threadIdx.x == 0) s_total = 0;
+    __syncthreads();
+    const long long base = (long long)blockIdx.x * kEvSpan + threadIdx.x;
     int c = 0;
+#pragma unroll 8
     for (int k = 0; k < kEvPerThread; k++) {
-        long long i = base + k;
+        long long i = base + (long long)k * kEvThreads;
         if (i < total) c += event_at(x, i, thr) != 0;
     }
-    int tot;
-    block_exclusive_scan(c, sh, &tot, Dummy{});
-    if (threadIdx.x == 0) wg_count[blockIdx.x] = tot;
+    // wavefront sum, then one LDS atomic per wavefront
+    for (int off = 32; off > 0; off >>= 1) c += __shfl_down(c, off, 64);
+    if ((threadIdx.x & 63) == 0) atomicAdd(&s_total, c);
+    __syncthreads();
+    if (threadIdx.x == 0) wg_count[blockIdx.x] = s_total;
 }
 
 // single workgroup: exclusive scan of per-workgroup counts
@@ -93,23 +78,43 @@ __global__ __launch_bounds__(1024) void k_ev_scan(const int* wg_count, int nwg, 
     if (threadIdx.x == 0) ctl->nevents = carry;
 }
 
-// events encoded as 2*pos + is_end, ascending
+// events encoded as 2*pos + is_end, ascending.  Order inside a workgroup is
+// (iteration k, wavefront w, lane) = ascending sample index; the offsets come
+// from per-(k, w) ballot counts.
 __global__ __launch_bounds__(kEvThreads) void k_ev_write(const double* x, long long total, double thr,
                                                          const int* wg_off, int* events) {
-    __shared__ int sh[kEvThreads];
-    long long base = (long long)blockIdx.x * kEvSpan + (long long)threadIdx.x * kEvPerThread;
-    int c = 0;
+    constexpr int kWaves = kEvThreads / 64;
+    __shared__ int s_cnt[kEvPerThread][kWaves];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const long long base = (long long)blockIdx.x * kEvSpan + threadIdx.x;
+    unsigned int fs = 0, fe = 0;
+#pragma unroll 8
     for (int k = 0; k < kEvPerThread; k++) {
-        long long i = base + k;
-        if (i < total) c += event_at(x, i, thr) != 0;
+        long long i = base + (long long)k * kEvThreads;
+        int e = i < total ? event_at(x, i, thr) : 0;
+        if (e == 1) fs |= 1u << k;
+        if (e == 2) fe |= 1u << k;
     }
-    int tot;
-    int off = block_exclusive_scan(c, sh, &tot, Dummy{}) + wg_off[blockIdx.x];
+    const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     for (int k = 0; k < kEvPerThread; k++) {
-        long long i = base + k;
-        if (i >= total) break;
-        int e = event_at(x, i, thr);
-        if (e) events[off++] = (int)(2 * i + (e == 2 ? 1 : 0));
+        unsigned long long m = __ballot(((fs | fe) >> k) & 1u);
+        if (lane == 0) s_cnt[k][wave] = __popcll(m);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int acc = wg_off[blockIdx.x];
+        for (int k = 0; k < kEvPerThread; k++)
+            for (int w = 0; w < kWaves; w++) { int c = s_cnt[k][w]; s_cnt[k][w] = acc; acc += c; }
+    }
+    __syncthreads();
+    for (int k = 0; k < kEvPerThread; k++) {
+        unsigned int bit = ((fs | fe) >> k) & 1u;
+        unsigned long long m = __ballot(bit);
+        if (bit) {
+            long long i = base + (long long)k * kEvThreads;
+            int pos = s_cnt[k][wave] + __popcll(m & lt);
+            events[pos] = (int)(2 * i + ((fe >> k) & 1u));
+        }
     }
 }
 

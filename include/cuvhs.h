@@ -164,7 +164,8 @@ int cuvhs_selftest_fft(int B, int is_f32, const void* d_in, void* d_out, int sig
 /* Per-stage device time (HIP events on the context stream) accumulated over
  * cuvhs_decode_chunk calls while enabled; enabling clears the totals.
  * cuvhs_profile_read fills up to n stage totals (ms) and names and returns
- * the number of stages.  (Counterpart of nothing in the reference, which
+ * the number of stages; the last one is the chroma pre-band-pass, which runs
+ * on a second stream concurrently with the sync and line-location stages.  (Counterpart of nothing in the reference, which
  * only times the whole chunk loop: pipeline.cu:2642-2708.) */
 int cuvhs_profile(cuvhs_ctx* ctx, int enable);
 int cuvhs_profile_read(const cuvhs_ctx* ctx, double* ms, const char** names, int n);

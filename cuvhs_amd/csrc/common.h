@@ -85,8 +85,8 @@ struct ChromaState {
 };
 
 // Per-chunk control block in device memory: written by the scheduling
-// kernels, read by later kernels (so no host round trip is needed inside a
-// chunk) and copied to the host once per chunk.
+// kernels, read by later kernels of the same chunk (so no host round trip is
+// needed inside a chunk) and copied to the host.
 struct ChunkCtl {
     int fields_k1;            // segments K1 demodulated
     int nf;                   // fields after VSYNC clustering
@@ -97,10 +97,15 @@ struct ChunkCtl {
     int pad_[2];
     uint64_t raw_offset;
     uint64_t next_raw_offset;
+};
+
+// State carried from chunk to chunk; only the line-location, chroma and
+// output-metadata stages touch it, in chunk order.
+struct DecodeState {
     K2bState k2b;
     ChromaState chroma;
     int last_parity;          // pipeline.cu:2575 (function-static in the reference)
-    int pad2_;
+    int pad_;
 };
 
 }  // namespace cuvhs

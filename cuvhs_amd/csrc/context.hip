@@ -2,10 +2,13 @@
 //
 // Reference: Pipeline::allocate_buffers / process_chunk / run
 // (pipeline.cu:1383-1475, 1525-2624, 2626-2713).  A chunk is enqueued as a
-// fixed sequence of kernels on the context's stream; the field count that
-// VSYNC clustering produces stays in a device control block that every later
-// kernel reads, so the only host synchronisation per chunk is the final copy
-// of that control block (next chunk offset, field count).
+// fixed sequence of kernels; the field count that VSYNC clustering produces
+// stays in a device control block that every later kernel reads.  A chunk has
+// a head stage (K1, sync events, VSYNC clustering) and a tail stage (pulses,
+// line locations, refinement, resampling, chroma, dropouts, metadata) on two
+// streams with two sets of per-chunk buffers, so the head of chunk c+1 runs
+// while the tail of chunk c does.  The host waits once per chunk, for the
+// head's control block (next chunk offset, field count).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -22,43 +25,55 @@
 
 using namespace cuvhs;
 
+constexpr int kStages = 12;
+constexpr int kMarks = 14;
+
+// Buffers and control block of one chunk in flight.  Two slots alternate so
+// that the head stage of chunk c+1 (K1, sync events, VSYNC clustering) can
+// run while the tail stage of chunk c (pulses, line locations, refinement,
+// resampling, chroma, dropouts) still reads chunk c's signals.
+struct Slot {
+    double* demod = nullptr;
+    double* d05 = nullptr;
+    double* env = nullptr;
+    int* events = nullptr;
+    int* wg_count = nullptr;
+    int* wg_off = nullptr;
+    unsigned long long* offsets = nullptr;
+    ChunkCtl* ctl = nullptr;      // device
+    ChunkCtl* hctl = nullptr;     // pinned host copy
+    hipEvent_t head_done = nullptr, tail_done = nullptr, pre0 = nullptr, pre1 = nullptr, tail0 = nullptr;
+    hipEvent_t mark[kMarks] = {};
+    // host view of the chunk
+    const uint8_t* chunk = nullptr;
+    uint64_t n_read = 0, raw_offset = 0;
+    int max_fields = 0, fk1 = 0;
+};
+
 struct cuvhs_ctx {
     cuvhs_config cfg{};
     Fmt fmt{};
     int F = 0;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;   // head stage (and the caller's stream, if given)
     bool own_stream = false;
-    // the chroma pre-band-pass needs only the raw chunk: it runs on a second
-    // stream (default priority) next to the sync/line-location stages
-    hipStream_t side = nullptr;
-    hipEvent_t ev_k1 = nullptr, ev_pre = nullptr, ev_pre0 = nullptr;
+    hipStream_t tail = nullptr;     // tail stage
+    hipStream_t side = nullptr;     // chroma pre-band-pass, next to the tail's sync stages
     std::string err;
-    // designs
     K1Design k1;
     PreDesign pre;
-    // device tables
     double2* G_a = nullptr;
     double2* G_b = nullptr;
     float2* G_p = nullptr;
-    double2* tw_a = nullptr;   // W_B for part A's B
+    double2* tw_a = nullptr;
     double2* tw_b = nullptr;
     float2* tw_p = nullptr;
     double2* tw_1024 = nullptr;
-    double* cs_tab = nullptr;  // cos/sin of the burst detector
-    // signal buffers
-    long long buf_n = 0;       // F * spfp + pad
-    double* rawdemod = nullptr;
-    double* demod = nullptr;
-    double* d05 = nullptr;
-    double* env = nullptr;
-    float* craw = nullptr;
-    // events
-    int* events = nullptr;
-    int* wg_count = nullptr;
-    int* wg_off = nullptr;
-    int* cand = nullptr;
-    // per field
-    unsigned long long* offsets = nullptr;
+    double* cs_tab = nullptr;       // cos/sin of the burst phase detector
+    long long buf_n = 0;            // F * spfp + one segment of zero padding
+    Slot slot[2];
+    double* rawdemod = nullptr;     // head only
+    int* cand = nullptr;            // head only
+    float* craw = nullptr;          // tail only
     int *k2s = nullptr, *k2l = nullptr, *k2c = nullptr;
     int *ps = nullptr, *pl = nullptr, *pc = nullptr, *pt = nullptr;
     int *vps = nullptr, *vpl = nullptr;
@@ -78,15 +93,11 @@ struct cuvhs_ctx {
     unsigned char* do_cls = nullptr;
     unsigned long long* file_loc = nullptr;
     int* isf_out = nullptr;
-    ChunkCtl* ctl = nullptr;       // device
-    ChunkCtl* hctl = nullptr;      // pinned host shadow
+    DecodeState* state = nullptr;   // device
+    DecodeState* hstate = nullptr;  // pinned, for resets
     std::vector<void*> allocs;
-    // per-stage timing with HIP events on the context stream
-    static constexpr int kStages = 12;
     bool prof = false;
-    hipEvent_t ev[kStages + 1] = {};
-    double stage_ms[kStages + 1] = {};  // + the side-stream pre-band-pass
-    long long stage_calls = 0;
+    double stage_ms[kStages + 1] = {};
 };
 
 namespace {
@@ -134,7 +145,7 @@ hipError_t upload_spectrum(cuvhs_ctx* c, V** dst, const std::vector<cdbl>& s) {
     return hipMemcpy(*dst, h.data(), h.size() * sizeof(V), hipMemcpyHostToDevice);
 }
 
-void reset_state(ChunkCtl* h) {
+void reset_state(DecodeState* h) {
     memset(h, 0, sizeof(*h));
     h->k2b.prev_first_field = -1;
     h->k2b.prev_progressive_field = -1;
@@ -145,11 +156,15 @@ void reset_state(ChunkCtl* h) {
     h->last_parity = 1;
 }
 
-ChromaLaunch chroma_launch(cuvhs_ctx* c, int nf_override) {
+void mark(cuvhs_ctx* c, Slot& s, int i, hipStream_t st) {
+    if (c->prof) (void)hipEventRecord(s.mark[i], st);
+}
+
+ChromaLaunch chroma_launch(cuvhs_ctx* c, const ChunkCtl* ctl, int nf_override) {
     ChromaLaunch a{};
     a.craw = c->craw;
     a.linelocs = c->ll;
-    a.ctl = c->ctl;
+    a.ctl = ctl;
     a.nf_override = nf_override;
     a.spfp = c->fmt.spfp;
     a.het_scale = (c->fmt.fsc + c->fmt.chroma_under) / c->fmt.output_rate;
@@ -167,11 +182,7 @@ ChromaLaunch chroma_launch(cuvhs_ctx* c, int nf_override) {
     return a;
 }
 
-void mark(cuvhs_ctx* c, int i) {
-    if (c->prof) (void)hipEventRecord(c->ev[i], c->stream);
-}
-
-int k1_launch(cuvhs_ctx* c, const uint8_t* chunk, long long n_valid, int nfk1) {
+int k1_launch(cuvhs_ctx* c, Slot& s, const uint8_t* chunk, long long n_valid, int nfk1, hipStream_t st) {
     const Fmt& f = c->fmt;
     long long T = (long long)nfk1 * f.spfp;
     K1ALaunch a{};
@@ -187,9 +198,9 @@ int k1_launch(cuvhs_ctx* c, const uint8_t* chunk, long long n_valid, int nfk1) {
     a.G = c->G_a;
     a.tw = c->tw_a;
     a.rawdemod = c->rawdemod;
-    a.env = c->env;
-    HIPCHK(c, run_k1_part_a(c->k1.rf.B, a, nfk1, c->stream));
-    mark(c, 1);
+    a.env = s.env;
+    HIPCHK(c, run_k1_part_a(c->k1.rf.B, a, nfk1, st));
+    mark(c, s, 1, st);
     K1BLaunch b{};
     b.rawdemod = c->rawdemod;
     b.total = T;
@@ -200,13 +211,13 @@ int k1_launch(cuvhs_ctx* c, const uint8_t* chunk, long long n_valid, int nfk1) {
     b.V = c->k1.video.valid;
     b.G = c->G_b;
     b.tw = c->tw_b;
-    b.demod = c->demod;
-    b.demod05 = c->d05;
-    HIPCHK(c, run_k1_part_b(c->k1.video.B, b, nfk1, c->stream));
+    b.demod = s.demod;
+    b.demod05 = s.d05;
+    HIPCHK(c, run_k1_part_b(c->k1.video.B, b, nfk1, st));
     // beyond the demodulated region the buffers read as zero
-    HIPCHK(c, run_fill_f64(c->demod + T, c->buf_n - T, 0.0, c->stream));
-    HIPCHK(c, run_fill_f64(c->d05 + T, c->buf_n - T, 0.0, c->stream));
-    HIPCHK(c, run_fill_f64(c->env + T, c->buf_n - T, 0.0, c->stream));
+    HIPCHK(c, run_fill_f64(s.demod + T, c->buf_n - T, 0.0, st));
+    HIPCHK(c, run_fill_f64(s.d05 + T, c->buf_n - T, 0.0, st));
+    HIPCHK(c, run_fill_f64(s.env + T, c->buf_n - T, 0.0, st));
     return 0;
 }
 
@@ -223,6 +234,131 @@ int prebpf_launch(cuvhs_ctx* c, const uint8_t* chunk, long long n_valid, int nf_
     p.tw = c->tw_p;
     p.out = c->craw;
     HIPCHK(c, run_prebpf(c->pre.bpf.B, p, nf_max, st));
+    return 0;
+}
+
+// RawReader::read_at + the chunk sizing of Pipeline::run / process_chunk
+// (pipeline.cu:1530-1548, 2651-2658).  Returns false when no chunk starts here.
+bool plan_chunk(cuvhs_ctx* c, Slot& s, const uint8_t* chunk, uint64_t avail, uint64_t raw_offset, int max_fields) {
+    const uint64_t spf = (uint64_t)c->fmt.samples_per_field;
+    uint64_t n_read = std::min<uint64_t>((uint64_t)max_fields * (uint64_t)c->fmt.spfp, avail);
+    if (n_read < spf) return false;
+    s.chunk = chunk;
+    s.n_read = n_read;
+    s.raw_offset = raw_offset;
+    s.max_fields = max_fields;
+    s.fk1 = (int)std::min<uint64_t>(n_read / spf, (uint64_t)max_fields);
+    return true;
+}
+
+// Head stage on the head stream: K1, sync events, VSYNC clustering.
+int head_stage(cuvhs_ctx* c, Slot& s) {
+    const Fmt& f = c->fmt;
+    hipStream_t st = c->stream;
+    ChunkCtl* h = s.hctl;
+    memset(h, 0, sizeof(*h));
+    h->fields_k1 = s.fk1;
+    h->raw_offset = s.raw_offset;
+    HIPCHK(c, hipMemcpyAsync(s.ctl, h, sizeof(ChunkCtl), hipMemcpyHostToDevice, st));
+    mark(c, s, 0, st);
+    int r = k1_launch(c, s, s.chunk, (long long)s.n_read, s.fk1, st);
+    if (r) return r;
+    mark(c, s, 2, st);
+    const long long T = (long long)s.fk1 * f.spfp;
+    HIPCHK(c, run_events(s.d05, T, f.pulse_threshold_hz, s.wg_count, s.wg_off, s.events, s.ctl, st));
+    mark(c, s, 3, st);
+    HIPCHK(c, run_vsync_cluster(s.events, s.ctl, T, f.vsync_min_w, f.vsync_max_w, s.fk1 * 20, c->cand, s.offsets,
+                                s.max_fields, f.samples_per_field, f.samples_per_line, s.fk1, 1, st));
+    mark(c, s, 4, st);
+    HIPCHK(c, hipMemcpyAsync(h, s.ctl, sizeof(ChunkCtl), hipMemcpyDeviceToHost, st));
+    HIPCHK(c, hipEventRecord(s.head_done, st));
+    return 0;
+}
+
+struct OutCopy {
+    uint16_t* luma;
+    uint16_t* chroma;
+    uint64_t* file_loc;
+    int32_t *is_first, *phase_id, *do_count, *do_line, *do_startx, *do_endx;
+};
+
+// Tail stage on the tail stream (pre-band-pass on the side stream); n is the
+// field count the head produced, out (optional) receives the chunk's outputs.
+int tail_stage(cuvhs_ctx* c, Slot& s, const OutCopy* out, int n) {
+    const Fmt& f = c->fmt;
+    hipStream_t st = c->tail;
+    HIPCHK(c, hipStreamWaitEvent(st, s.head_done, 0));
+    mark(c, s, 5, st);
+    HIPCHK(c, hipEventRecord(s.tail0, st));
+    HIPCHK(c, hipStreamWaitEvent(c->side, s.tail0, 0));
+    HIPCHK(c, hipEventRecord(s.pre0, c->side));
+    int r = prebpf_launch(c, s.chunk, (long long)s.n_read, s.max_fields, c->side);
+    if (r) return r;
+    HIPCHK(c, hipEventRecord(s.pre1, c->side));
+    HIPCHK(c, run_sync_pulses(s.events, s.ctl, s.offsets, f.spfp, c->k2s, c->k2l, c->k2c, s.max_fields, 0, st));
+    mark(c, s, 6, st);
+    HIPCHK(c, run_k2b(s.ctl, s.ctl, c->state, 0, s.max_fields, f, s.offsets, c->k2s, c->k2l, c->k2c, c->vps, c->vpl,
+                      c->vptg, (K2bField*)c->summ, (K2bAnchorOut*)c->anchors, c->ps, c->pl, c->pc, c->pt, c->ll,
+                      c->isf_k2b, st));
+    mark(c, s, 7, st);
+    HIPCHK(c, run_hsync_refine(s.d05, c->ll, s.ctl, 0, s.max_fields, f, st));
+    mark(c, s, 8, st);
+    HIPCHK(c, run_tbc_resample(s.demod, c->ll, c->luma, s.ctl, 0, s.max_fields, f, st));
+    mark(c, s, 9, st);
+    HIPCHK(c, hipStreamWaitEvent(st, s.pre1, 0));
+    mark(c, s, 10, st);
+    HIPCHK(c, run_chroma(chroma_launch(c, s.ctl, 0), c->state, s.max_fields, c->m0, c->m1, c->cs_tab, c->iq0, c->track,
+                         c->phase, 1, st));
+    mark(c, s, 11, st);
+    HIPCHK(c, run_dropouts(s.env, c->ll, s.ctl, 0, s.max_fields, f, c->do_partial, c->do_cls, c->luma, c->chroma,
+                           c->dol, c->dos, c->doe, c->doc, st));
+    mark(c, s, 12, st);
+    HIPCHK(c, run_field_meta(s.ctl, c->state, s.offsets, c->isf_k2b, c->file_loc, c->isf_out, st));
+    mark(c, s, 13, st);
+    if (out && n > 0) {
+        const size_t tbc = (size_t)n * kFieldLines * kLineLen * 2, dsz = (size_t)n * kMaxDropouts * 4;
+        if (out->luma) HIPCHK(c, hipMemcpyAsync(out->luma, c->luma, tbc, hipMemcpyDeviceToDevice, st));
+        if (out->chroma) HIPCHK(c, hipMemcpyAsync(out->chroma, c->chroma, tbc, hipMemcpyDeviceToDevice, st));
+        if (out->file_loc) HIPCHK(c, hipMemcpyAsync(out->file_loc, c->file_loc, n * 8, hipMemcpyDeviceToDevice, st));
+        if (out->is_first) HIPCHK(c, hipMemcpyAsync(out->is_first, c->isf_out, n * 4, hipMemcpyDeviceToDevice, st));
+        if (out->phase_id) HIPCHK(c, hipMemcpyAsync(out->phase_id, c->phase, n * 4, hipMemcpyDeviceToDevice, st));
+        if (out->do_count) HIPCHK(c, hipMemcpyAsync(out->do_count, c->doc, n * 4, hipMemcpyDeviceToDevice, st));
+        if (out->do_line) HIPCHK(c, hipMemcpyAsync(out->do_line, c->dol, dsz, hipMemcpyDeviceToDevice, st));
+        if (out->do_startx) HIPCHK(c, hipMemcpyAsync(out->do_startx, c->dos, dsz, hipMemcpyDeviceToDevice, st));
+        if (out->do_endx) HIPCHK(c, hipMemcpyAsync(out->do_endx, c->doe, dsz, hipMemcpyDeviceToDevice, st));
+    }
+    HIPCHK(c, hipMemcpyAsync(s.hctl, s.ctl, sizeof(ChunkCtl), hipMemcpyDeviceToHost, st));
+    HIPCHK(c, hipEventRecord(s.tail_done, st));
+    return 0;
+}
+
+// after tail_done of slot s: accumulate its stage times
+void collect_profile(cuvhs_ctx* c, Slot& s) {
+    if (!c->prof) return;
+    for (int i = 0; i < kStages; i++) {
+        int a = i < 4 ? i : i + 1;
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, s.mark[a], s.mark[a + 1]) == hipSuccess) c->stage_ms[i] += ms;
+    }
+    float pre = 0.0f;
+    if (hipEventElapsedTime(&pre, s.pre0, s.pre1) == hipSuccess) c->stage_ms[kStages] += pre;
+}
+
+int sync_all(cuvhs_ctx* c) {
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->tail));
+    HIPCHK(c, hipStreamSynchronize(c->side));
+    return 0;
+}
+
+// Host wait for a slot's tail stage; collects its stage times and capacity flag.
+int finish_tail(cuvhs_ctx* c, Slot& s) {
+    HIPCHK(c, hipEventSynchronize(s.tail_done));
+    collect_profile(c, s);
+    if (s.hctl->vp_overflow) {
+        c->err = "K2b validated-pulse capacity exceeded";
+        return CUVHS_E_CAPACITY;
+    }
     return 0;
 }
 
@@ -272,11 +408,19 @@ int cuvhs_create(const cuvhs_config* cfg, cuvhs_ctx** out) {
         if (e != hipSuccess) return bad(e, "hipStreamCreate");
         c->own_stream = true;
     }
+    e = hipStreamCreateWithFlags(&c->tail, hipStreamNonBlocking);
+    if (e != hipSuccess) return bad(e, "hipStreamCreate tail");
     e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking);
     if (e != hipSuccess) return bad(e, "hipStreamCreate side");
-    for (hipEvent_t* ev : {&c->ev_k1, &c->ev_pre, &c->ev_pre0}) {
-        e = hipEventCreate(ev);
-        if (e != hipSuccess) return bad(e, "hipEventCreate");
+    for (Slot& s : c->slot) {
+        for (hipEvent_t* ev : {&s.head_done, &s.tail_done, &s.tail0}) {
+            e = hipEventCreateWithFlags(ev, hipEventDisableTiming);
+            if (e != hipSuccess) return bad(e, "hipEventCreate");
+        }
+        for (hipEvent_t* ev : {&s.pre0, &s.pre1}) {
+            e = hipEventCreate(ev);
+            if (e != hipSuccess) return bad(e, "hipEventCreate");
+        }
     }
     try {
         const int bk1[2] = {4096, 8192};
@@ -319,18 +463,24 @@ int cuvhs_create(const cuvhs_config* cfg, cuvhs_ctx** out) {
         if ((e = hipMemcpy(c->cs_tab, cs.data(), cs.size() * sizeof(double), hipMemcpyHostToDevice)) != hipSuccess) return bad(e, "burst table");
     }
     const long long S = (long long)c->F * f.spfp;
+    const size_t Fz = (size_t)c->F;
     c->buf_n = S + f.spfp;  // one segment of zero padding past the chunk
     ALLOC(c->rawdemod, S);
-    ALLOC(c->demod, c->buf_n);
-    ALLOC(c->d05, c->buf_n);
-    ALLOC(c->env, c->buf_n);
-    ALLOC(c->craw, S);
-    ALLOC(c->events, S + 2);
-    ALLOC(c->wg_count, events_nwg(S) + 1);
-    ALLOC(c->wg_off, events_nwg(S) + 1);
     ALLOC(c->cand, 20 * c->F);
-    const size_t Fz = (size_t)c->F;
-    ALLOC(c->offsets, Fz);
+    ALLOC(c->craw, S);
+    for (Slot& s : c->slot) {
+        ALLOC(s.demod, c->buf_n);
+        ALLOC(s.d05, c->buf_n);
+        ALLOC(s.env, c->buf_n);
+        ALLOC(s.events, S + 2);
+        ALLOC(s.wg_count, events_nwg(S) + 1);
+        ALLOC(s.wg_off, events_nwg(S) + 1);
+        ALLOC(s.offsets, Fz);
+        ALLOC(s.ctl, 1);
+        e = hipHostMalloc((void**)&s.hctl, sizeof(ChunkCtl), hipHostMallocDefault);
+        if (e != hipSuccess) return bad(e, "hipHostMalloc");
+        memset(s.hctl, 0, sizeof(ChunkCtl));
+    }
     ALLOC(c->k2s, Fz * kMaxPulses);
     ALLOC(c->k2l, Fz * kMaxPulses);
     ALLOC(c->k2c, Fz);
@@ -367,28 +517,32 @@ int cuvhs_create(const cuvhs_config* cfg, cuvhs_ctx** out) {
     ALLOC(c->do_cls, dropout_class_bytes(c->F));
     ALLOC(c->file_loc, Fz);
     ALLOC(c->isf_out, Fz);
-    ALLOC(c->ctl, 1);
+    ALLOC(c->state, 1);
 #undef ALLOC
-    e = hipHostMalloc((void**)&c->hctl, sizeof(ChunkCtl), hipHostMallocDefault);
+    e = hipHostMalloc((void**)&c->hstate, sizeof(DecodeState), hipHostMallocDefault);
     if (e != hipSuccess) return bad(e, "hipHostMalloc");
-    reset_state(c->hctl);
-    e = hipMemcpy(c->ctl, c->hctl, sizeof(ChunkCtl), hipMemcpyHostToDevice);
-    if (e != hipSuccess) return bad(e, "ctl upload");
+    reset_state(c->hstate);
+    e = hipMemcpy(c->state, c->hstate, sizeof(DecodeState), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return bad(e, "state upload");
     *out = c;
     return 0;
 }
 
 int cuvhs_destroy(cuvhs_ctx* c) {
     if (!c) return CUVHS_E_ARG;
-    if (c->stream) (void)hipStreamSynchronize(c->stream);
-    if (c->side) (void)hipStreamSynchronize(c->side);
-    for (hipEvent_t e : c->ev)
-        if (e) (void)hipEventDestroy(e);
-    for (hipEvent_t e : {c->ev_k1, c->ev_pre, c->ev_pre0})
-        if (e) (void)hipEventDestroy(e);
+    for (hipStream_t st : {c->stream, c->tail, c->side})
+        if (st) (void)hipStreamSynchronize(st);
+    for (Slot& s : c->slot) {
+        for (hipEvent_t e : s.mark)
+            if (e) (void)hipEventDestroy(e);
+        for (hipEvent_t e : {s.head_done, s.tail_done, s.pre0, s.pre1, s.tail0})
+            if (e) (void)hipEventDestroy(e);
+        if (s.hctl) (void)hipHostFree(s.hctl);
+    }
     if (c->side) (void)hipStreamDestroy(c->side);
+    if (c->tail) (void)hipStreamDestroy(c->tail);
     for (void* p : c->allocs) (void)hipFree(p);
-    if (c->hctl) (void)hipHostFree(c->hctl);
+    if (c->hstate) (void)hipHostFree(c->hstate);
     if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return 0;
@@ -399,11 +553,13 @@ int cuvhs_get_format(const cuvhs_ctx* c, cuvhs_format* out) {
     return cuvhs_format_for(c->cfg.sample_rate_mhz, out);
 }
 
+
 int cuvhs_reset(cuvhs_ctx* c) {
     if (!c) return CUVHS_E_ARG;
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    reset_state(c->hctl);
-    HIPCHK(c, hipMemcpy(c->ctl, c->hctl, sizeof(ChunkCtl), hipMemcpyHostToDevice));
+    int r = sync_all(c);
+    if (r) return r;
+    reset_state(c->hstate);
+    HIPCHK(c, hipMemcpy(c->state, c->hstate, sizeof(DecodeState), hipMemcpyHostToDevice));
     return 0;
 }
 
@@ -414,85 +570,25 @@ int cuvhs_decode_chunk(cuvhs_ctx* c, const uint8_t* chunk, uint64_t avail, uint6
     if (!c || !chunk || !info || max_fields < 1 || max_fields > c->F) return CUVHS_E_ARG;
     memset(info, 0, sizeof(*info));
     info->raw_offset = raw_offset;
-    const Fmt& f = c->fmt;
-    const uint64_t spf = (uint64_t)f.samples_per_field;
-    const uint64_t total_samples = (uint64_t)max_fields * (uint64_t)f.spfp;
-    // RawReader::read_at (raw_reader.cpp:128-152) + pipeline.cu:1534-1548
-    uint64_t n_read = std::min<uint64_t>(total_samples, avail);
-    if (n_read < spf) {
+    Slot& s = c->slot[0];
+    if (!plan_chunk(c, s, chunk, avail, raw_offset, max_fields)) {
         info->status = CUVHS_W_EOF;
         return 0;
     }
-    int fk1 = (int)(n_read / spf);
-    if (fk1 > max_fields) fk1 = max_fields;
-    const long long T = (long long)fk1 * f.spfp;
-    // control block: carried state from the host shadow, per-chunk fields reset
-    c->hctl->fields_k1 = fk1;
-    c->hctl->nf = 0;
-    c->hctl->vsync_count = 0;
-    c->hctl->status = 0;
-    c->hctl->vp_overflow = 0;
-    c->hctl->nevents = 0;
-    c->hctl->raw_offset = raw_offset;
-    c->hctl->next_raw_offset = 0;
-    HIPCHK(c, hipMemcpyAsync(c->ctl, c->hctl, sizeof(ChunkCtl), hipMemcpyHostToDevice, c->stream));
-    mark(c, 0);
-    int r = k1_launch(c, chunk, (long long)n_read, fk1);
-    if (r) return r;
-    mark(c, 2);
-    HIPCHK(c, hipEventRecord(c->ev_k1, c->stream));
-    HIPCHK(c, hipStreamWaitEvent(c->side, c->ev_k1, 0));
-    HIPCHK(c, hipEventRecord(c->ev_pre0, c->side));
-    r = prebpf_launch(c, chunk, (long long)n_read, max_fields, c->side);
-    if (r) return r;
-    HIPCHK(c, hipEventRecord(c->ev_pre, c->side));
-    HIPCHK(c, run_events(c->d05, T, f.pulse_threshold_hz, c->wg_count, c->wg_off, c->events, c->ctl, c->stream));
-    mark(c, 3);
-    HIPCHK(c, run_vsync_cluster(c->events, c->ctl, T, f.vsync_min_w, f.vsync_max_w, fk1 * 20, c->cand, c->offsets,
-                                max_fields, f.samples_per_field, f.samples_per_line, fk1, 1, c->stream));
-    mark(c, 4);
-    HIPCHK(c, run_sync_pulses(c->events, c->ctl, c->offsets, f.spfp, c->k2s, c->k2l, c->k2c, max_fields, 0,
-                              c->stream));
-    mark(c, 5);
-    HIPCHK(c, run_k2b(c->ctl, c->ctl, 0, max_fields, f, c->offsets, c->k2s, c->k2l, c->k2c, c->vps, c->vpl, c->vptg,
-                      (K2bField*)c->summ, (K2bAnchorOut*)c->anchors, c->ps, c->pl, c->pc, c->pt, c->ll, c->isf_k2b,
-                      c->stream));
-    mark(c, 6);
-    HIPCHK(c, run_hsync_refine(c->d05, c->ll, c->ctl, 0, max_fields, f, c->stream));
-    mark(c, 7);
-    HIPCHK(c, run_tbc_resample(c->demod, c->ll, c->luma, c->ctl, 0, max_fields, f, c->stream));
-    mark(c, 8);
-    HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_pre, 0));
-    mark(c, 9);
-    HIPCHK(c, run_chroma(chroma_launch(c, 0), c->ctl, max_fields, c->m0, c->m1, c->cs_tab, c->iq0, c->track, c->phase,
-                         c->hctl->chroma.valid ? 0 : 1, c->stream));
-    mark(c, 10);
-    HIPCHK(c, run_dropouts(c->env, c->ll, c->ctl, 0, max_fields, f, c->do_partial, c->do_cls, c->luma, c->chroma, c->dol, c->dos, c->doe,
-                           c->doc, c->stream));
-    mark(c, 11);
-    HIPCHK(c, run_field_meta(c->ctl, c->offsets, c->isf_k2b, c->file_loc, c->isf_out, c->stream));
-    mark(c, 12);
-    HIPCHK(c, hipMemcpyAsync(c->hctl, c->ctl, sizeof(ChunkCtl), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    if (c->prof) {
-        for (int i = 0; i < cuvhs_ctx::kStages; i++) {
-            float ms = 0.0f;
-            if (hipEventElapsedTime(&ms, c->ev[i], c->ev[i + 1]) == hipSuccess) c->stage_ms[i] += ms;
-        }
-        // the pre-band-pass runs on the side stream: its own duration, and
-        // the main-stream wait for it stays in its slot
-        float pre = 0.0f;
-        if (hipEventElapsedTime(&pre, c->ev_pre0, c->ev_pre) == hipSuccess) c->stage_ms[cuvhs_ctx::kStages] += pre;
-        c->stage_calls++;
-    }
-    info->fields = c->hctl->nf;
-    info->fields_k1 = fk1;
-    info->vsync_candidates = c->hctl->vsync_count;
-    info->status = c->hctl->status;
-    info->next_raw_offset = c->hctl->next_raw_offset;
-    if (c->hctl->vp_overflow) {
-        c->err = "K2b validated-pulse capacity exceeded";
-        return CUVHS_E_CAPACITY;
+    int r = head_stage(c, s);
+    if (r) { sync_all(c); return r; }
+    HIPCHK(c, hipEventSynchronize(s.head_done));
+    const ChunkCtl h = *s.hctl;
+    info->fields = h.nf;
+    info->fields_k1 = s.fk1;
+    info->vsync_candidates = h.vsync_count;
+    info->status = h.status;
+    info->next_raw_offset = h.next_raw_offset;
+    if (h.nf > 0) {
+        r = tail_stage(c, s, nullptr, h.nf);
+        if (r) { sync_all(c); return r; }
+        r = finish_tail(c, s);
+        if (r) return r;
     }
     return info->status;
 }
@@ -519,76 +615,102 @@ int cuvhs_decode_capture(cuvhs_ctx* c, const uint8_t* cap, uint64_t cap_len, int
     *fields = 0;
     const uint64_t spf = (uint64_t)c->fmt.samples_per_field;
     const size_t tbc = (size_t)kFieldLines * kLineLen;
-    uint64_t raw_offset = 0;
-    int total = 0;
-    int warn = 0;
-    while (true) {
-        if (raw_offset >= cap_len) break;
-        uint64_t remaining = (cap_len - raw_offset) / spf;
-        if (remaining == 0) break;
+    // chunk sizing of Pipeline::run (pipeline.cu:2651-2658)
+    auto plan = [&](Slot& s, uint64_t off) -> bool {
+        if (off >= cap_len) return false;
+        uint64_t remaining = (cap_len - off) / spf;
+        if (remaining == 0) return false;
         int this_chunk = (int)std::min<uint64_t>((uint64_t)c->F, remaining);
-        cuvhs_chunk_info info;
-        int r = cuvhs_decode_chunk(c, cap + raw_offset, cap_len - raw_offset, raw_offset, this_chunk, &info);
-        if (r < 0) return r;
-        warn |= info.status & ~CUVHS_W_EOF;
-        if (info.fields == 0) break;
-        int n = info.fields;
-        if (total + n > max_out) return CUVHS_E_CAPACITY;
-        hipStream_t st = c->stream;
-        if (d_luma) HIPCHK(c, hipMemcpyAsync(d_luma + (size_t)total * tbc, c->luma, n * tbc * 2, hipMemcpyDeviceToDevice, st));
-        if (d_chroma) HIPCHK(c, hipMemcpyAsync(d_chroma + (size_t)total * tbc, c->chroma, n * tbc * 2, hipMemcpyDeviceToDevice, st));
-        if (d_file_loc) HIPCHK(c, hipMemcpyAsync(d_file_loc + total, c->file_loc, n * 8, hipMemcpyDeviceToDevice, st));
-        if (d_is_first) HIPCHK(c, hipMemcpyAsync(d_is_first + total, c->isf_out, n * 4, hipMemcpyDeviceToDevice, st));
-        if (d_phase_id) HIPCHK(c, hipMemcpyAsync(d_phase_id + total, c->phase, n * 4, hipMemcpyDeviceToDevice, st));
-        if (d_do_count) HIPCHK(c, hipMemcpyAsync(d_do_count + total, c->doc, n * 4, hipMemcpyDeviceToDevice, st));
-        size_t dsz = (size_t)n * kMaxDropouts * 4;
-        if (d_do_line) HIPCHK(c, hipMemcpyAsync(d_do_line + (size_t)total * kMaxDropouts, c->dol, dsz, hipMemcpyDeviceToDevice, st));
-        if (d_do_startx) HIPCHK(c, hipMemcpyAsync(d_do_startx + (size_t)total * kMaxDropouts, c->dos, dsz, hipMemcpyDeviceToDevice, st));
-        if (d_do_endx) HIPCHK(c, hipMemcpyAsync(d_do_endx + (size_t)total * kMaxDropouts, c->doe, dsz, hipMemcpyDeviceToDevice, st));
-        total += n;
-        raw_offset = info.next_raw_offset;
+        return plan_chunk(c, s, cap + off, cap_len - off, off, this_chunk);
+    };
+    if (!plan(c->slot[0], 0)) return 0;
+    int rc = head_stage(c, c->slot[0]);
+    if (rc) { sync_all(c); return rc; }
+    int total = 0, warn = 0, cur = 0, pending = -1;
+    while (true) {
+        Slot& s = c->slot[cur];
+        const hipError_t he = hipEventSynchronize(s.head_done);
+        if (he != hipSuccess) { rc = fail(c, he, "head stage"); break; }
+        const int nf = s.hctl->nf;
+        const uint64_t next = s.hctl->next_raw_offset;
+        warn |= s.hctl->status & ~CUVHS_W_EOF;
+        if (nf == 0) break;
+        if (total + nf > max_out) { rc = CUVHS_E_CAPACITY; break; }
+        OutCopy o{};
+        o.luma = d_luma ? d_luma + (size_t)total * tbc : nullptr;
+        o.chroma = d_chroma ? d_chroma + (size_t)total * tbc : nullptr;
+        o.file_loc = d_file_loc ? d_file_loc + total : nullptr;
+        o.is_first = d_is_first ? d_is_first + total : nullptr;
+        o.phase_id = d_phase_id ? d_phase_id + total : nullptr;
+        o.do_count = d_do_count ? d_do_count + total : nullptr;
+        o.do_line = d_do_line ? d_do_line + (size_t)total * kMaxDropouts : nullptr;
+        o.do_startx = d_do_startx ? d_do_startx + (size_t)total * kMaxDropouts : nullptr;
+        o.do_endx = d_do_endx ? d_do_endx + (size_t)total * kMaxDropouts : nullptr;
+        if ((rc = tail_stage(c, s, &o, nf)) != 0) break;
+        total += nf;
+        // the other slot is free once the previous chunk's tail has finished
+        if (pending >= 0) {
+            int p = pending;
+            pending = -1;
+            if ((rc = finish_tail(c, c->slot[p])) != 0) break;
+        }
+        pending = cur;
+        const int nxt = cur ^ 1;
+        if (!plan(c->slot[nxt], next)) break;
+        if ((rc = head_stage(c, c->slot[nxt])) != 0) break;
+        cur = nxt;
     }
-    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (pending >= 0) {
+        int r = finish_tail(c, c->slot[pending]);
+        if (!rc) rc = r;
+    }
+    int r = sync_all(c);
+    if (!rc) rc = r;
+    if (rc) return rc;
     *fields = total;
     return warn;
 }
 
-static const char* kStageNames[cuvhs_ctx::kStages + 1] = {
+static const char* kStageNames[kStages + 1] = {
     "k1_analytic", "k1_video", "sync_events", "vsync_cluster", "k2_pulses", "k2b_linelocs",
     "k4_hsync_refine", "k5_tbc_resample", "k6_prebpf_wait", "k6_chroma", "k7_dropouts", "field_meta",
     "k6_prebpf"};
 
 int cuvhs_profile(cuvhs_ctx* c, int enable) {
     if (!c) return CUVHS_E_ARG;
-    if (enable && !c->ev[0]) {
-        for (hipEvent_t& e : c->ev) HIPCHK(c, hipEventCreate(&e));
+    if (enable && !c->slot[0].mark[0]) {
+        for (Slot& s : c->slot)
+            for (hipEvent_t& e : s.mark) HIPCHK(c, hipEventCreate(&e));
     }
     c->prof = enable != 0;
     for (double& v : c->stage_ms) v = 0.0;
-    c->stage_calls = 0;
     return 0;
 }
 
 int cuvhs_profile_read(const cuvhs_ctx* c, double* ms, const char** names, int n) {
     if (!c || n < 0) return CUVHS_E_ARG;
-    for (int i = 0; i < n && i <= cuvhs_ctx::kStages; i++) {
+    for (int i = 0; i < n && i <= kStages; i++) {
         if (ms) ms[i] = c->stage_ms[i];
         if (names) names[i] = kStageNames[i];
     }
-    return cuvhs_ctx::kStages + 1;
+    return kStages + 1;
 }
 
-// ---- stage entry points ----
+// ---- stage entry points (synchronous, on the context's stream, slot 0) ----
 
 int cuvhs_fm_demod(cuvhs_ctx* c, const uint8_t* raw, uint64_t n_valid, int nf, double* demod, double* d05,
                    double* env) {
     if (!c || !raw || nf < 1 || nf > c->F || !demod || !d05 || !env) return CUVHS_E_ARG;
-    int r = k1_launch(c, raw, (long long)n_valid, nf);
+    Slot& s = c->slot[0];
+    const bool prof = c->prof;
+    c->prof = false;
+    int r = k1_launch(c, s, raw, (long long)n_valid, nf, c->stream);
+    c->prof = prof;
     if (r) return r;
     size_t n = (size_t)nf * c->fmt.spfp * sizeof(double);
-    HIPCHK(c, hipMemcpyAsync(demod, c->demod, n, hipMemcpyDeviceToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(d05, c->d05, n, hipMemcpyDeviceToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(env, c->env, n, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(demod, s.demod, n, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(d05, s.d05, n, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(env, s.env, n, hipMemcpyDeviceToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return 0;
 }
@@ -596,13 +718,13 @@ int cuvhs_fm_demod(cuvhs_ctx* c, const uint8_t* raw, uint64_t n_valid, int nf, d
 int cuvhs_vsync_discover(cuvhs_ctx* c, const double* d05, int total, int* idx, int cap, int* count) {
     if (!c || !d05 || !idx || !count || total < 1 || cap < 0 || total > (long long)c->F * c->fmt.spfp) return CUVHS_E_ARG;
     const Fmt& f = c->fmt;
-    HIPCHK(c, run_events(d05, total, f.pulse_threshold_hz, c->wg_count, c->wg_off, c->events, c->ctl, c->stream));
-    HIPCHK(c, run_vsync_cluster(c->events, c->ctl, total, f.vsync_min_w, f.vsync_max_w, cap, idx, c->offsets, 1,
+    Slot& s = c->slot[0];
+    HIPCHK(c, run_events(d05, total, f.pulse_threshold_hz, s.wg_count, s.wg_off, s.events, s.ctl, c->stream));
+    HIPCHK(c, run_vsync_cluster(s.events, s.ctl, total, f.vsync_min_w, f.vsync_max_w, cap, idx, s.offsets, 1,
                                 f.samples_per_field, f.samples_per_line, 1, 0, c->stream));
-    ChunkCtl h;
-    HIPCHK(c, hipMemcpyAsync(&h, c->ctl, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(s.hctl, s.ctl, sizeof(ChunkCtl), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    *count = h.vsync_count;
+    *count = s.hctl->vsync_count;
     return 0;
 }
 
@@ -610,8 +732,9 @@ int cuvhs_sync_pulses(cuvhs_ctx* c, const double* d05, int total, const uint64_t
                       int* lens, int* count) {
     if (!c || !d05 || !offsets || nf < 1 || total < 1 || total > (long long)c->F * c->fmt.spfp) return CUVHS_E_ARG;
     const Fmt& f = c->fmt;
-    HIPCHK(c, run_events(d05, total, f.pulse_threshold_hz, c->wg_count, c->wg_off, c->events, c->ctl, c->stream));
-    HIPCHK(c, run_sync_pulses(c->events, c->ctl, (const unsigned long long*)offsets, f.spfp, starts, lens, count, nf,
+    Slot& s = c->slot[0];
+    HIPCHK(c, run_events(d05, total, f.pulse_threshold_hz, s.wg_count, s.wg_off, s.events, s.ctl, c->stream));
+    HIPCHK(c, run_sync_pulses(s.events, s.ctl, (const unsigned long long*)offsets, f.spfp, starts, lens, count, nf,
                               nf, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return 0;
@@ -620,31 +743,36 @@ int cuvhs_sync_pulses(cuvhs_ctx* c, const double* d05, int total, const uint64_t
 int cuvhs_linelocs(cuvhs_ctx* c, uint64_t raw_offset, const uint64_t* offsets, int nf, int* starts, int* lens,
                    int* count, int* types, double* linelocs, int* is_first) {
     if (!c || !offsets || nf < 1 || nf > c->F) return CUVHS_E_ARG;
-    c->hctl->raw_offset = raw_offset;
-    c->hctl->nf = nf;
-    c->hctl->vp_overflow = 0;
-    HIPCHK(c, hipMemcpyAsync(c->ctl, c->hctl, sizeof(ChunkCtl), hipMemcpyHostToDevice, c->stream));
+    Slot& s = c->slot[0];
+    memset(s.hctl, 0, sizeof(ChunkCtl));
+    s.hctl->raw_offset = raw_offset;
+    s.hctl->nf = nf;
+    HIPCHK(c, hipMemcpyAsync(s.ctl, s.hctl, sizeof(ChunkCtl), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->k2s, starts, (size_t)nf * kMaxPulses * 4, hipMemcpyDeviceToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->k2l, lens, (size_t)nf * kMaxPulses * 4, hipMemcpyDeviceToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->k2c, count, (size_t)nf * 4, hipMemcpyDeviceToDevice, c->stream));
-    HIPCHK(c, run_k2b(c->ctl, c->ctl, nf, nf, c->fmt, (const unsigned long long*)offsets, c->k2s, c->k2l, c->k2c,
-                      c->vps, c->vpl, c->vptg, (K2bField*)c->summ, (K2bAnchorOut*)c->anchors, starts, lens, count,
-                      types, linelocs, is_first, c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->hctl, c->ctl, sizeof(ChunkCtl), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, run_k2b(s.ctl, s.ctl, c->state, nf, nf, c->fmt, (const unsigned long long*)offsets, c->k2s, c->k2l,
+                      c->k2c, c->vps, c->vpl, c->vptg, (K2bField*)c->summ, (K2bAnchorOut*)c->anchors, starts, lens,
+                      count, types, linelocs, is_first, c->stream));
+    HIPCHK(c, hipMemcpyAsync(s.hctl, s.ctl, sizeof(ChunkCtl), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    return c->hctl->vp_overflow ? CUVHS_E_CAPACITY : 0;
+    if (s.hctl->vp_overflow) {
+        c->err = "K2b validated-pulse capacity exceeded";
+        return CUVHS_E_CAPACITY;
+    }
+    return 0;
 }
 
 int cuvhs_hsync_refine(cuvhs_ctx* c, const double* d05, int total, double* ll, int nf) {
     if (!c || !d05 || !ll || nf < 1 || total != nf * c->fmt.spfp) return CUVHS_E_ARG;
-    HIPCHK(c, run_hsync_refine(d05, ll, c->ctl, nf, nf, c->fmt, c->stream));
+    HIPCHK(c, run_hsync_refine(d05, ll, c->slot[0].ctl, nf, nf, c->fmt, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return 0;
 }
 
 int cuvhs_tbc_resample(cuvhs_ctx* c, const double* demod, int total, const double* ll, int nf, uint16_t* luma) {
     if (!c || !demod || !ll || !luma || nf < 1 || total != nf * c->fmt.spfp) return CUVHS_E_ARG;
-    HIPCHK(c, run_tbc_resample(demod, ll, luma, c->ctl, nf, nf, c->fmt, c->stream));
+    HIPCHK(c, run_tbc_resample(demod, ll, luma, c->slot[0].ctl, nf, nf, c->fmt, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return 0;
 }
@@ -655,11 +783,9 @@ int cuvhs_chroma(cuvhs_ctx* c, const uint8_t* raw, uint64_t n_valid, const doubl
     int r = prebpf_launch(c, raw, (long long)n_valid, nf, c->stream);
     if (r) return r;
     HIPCHK(c, hipMemcpyAsync(c->ll, ll, (size_t)nf * kLinesPerFrame * 8, hipMemcpyDeviceToDevice, c->stream));
-    ChromaLaunch a = chroma_launch(c, nf);
+    ChromaLaunch a = chroma_launch(c, c->slot[0].ctl, nf);
     a.out = out;
-    HIPCHK(c, run_chroma(a, c->ctl, nf, c->m0, c->m1, c->cs_tab, c->iq0, c->track, c->phase,
-                         c->hctl->chroma.valid ? 0 : 1, c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->hctl, c->ctl, sizeof(ChunkCtl), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, run_chroma(a, c->state, nf, c->m0, c->m1, c->cs_tab, c->iq0, c->track, c->phase, 1, c->stream));
     if (h_phase) HIPCHK(c, hipMemcpyAsync(h_phase, c->phase, (size_t)nf * 4, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return 0;
@@ -668,7 +794,8 @@ int cuvhs_chroma(cuvhs_ctx* c, const uint8_t* raw, uint64_t n_valid, const doubl
 int cuvhs_dropouts(cuvhs_ctx* c, const double* env, const double* ll, int nf, uint16_t* luma, uint16_t* chroma,
                    int* lines, int* starts, int* ends, int* count) {
     if (!c || !env || !ll || nf < 1 || nf > c->F) return CUVHS_E_ARG;
-    HIPCHK(c, run_dropouts(env, ll, c->ctl, nf, nf, c->fmt, c->do_partial, c->do_cls, luma, chroma, lines, starts, ends, count, c->stream));
+    HIPCHK(c, run_dropouts(env, ll, c->slot[0].ctl, nf, nf, c->fmt, c->do_partial, c->do_cls, luma, chroma, lines,
+                           starts, ends, count, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return 0;
 }

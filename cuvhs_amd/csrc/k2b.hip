@@ -362,12 +362,12 @@ __global__ __launch_bounds__(64) void k_k2b_field(const ChunkCtl* ctl, int nf_ov
 }
 
 // Cross-field cadence (pipeline.cu:1717-1772, :729-745, :811-827), one thread.
-__global__ void k_k2b_chain(ChunkCtl* ctl, int nf_override, const unsigned long long* offsets,
+__global__ void k_k2b_chain(ChunkCtl* ctl, DecodeState* dst, int nf_override, const unsigned long long* offsets,
                             const K2bField* summ, const int* g_vp_start, const unsigned char* g_vp_tg,
                             K2bAnchorOut* anchors, int* is_first, int num_eq, int fl0, int fl1) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     const int nf = nf_override > 0 ? nf_override : ctl->nf;
-    K2bState st = ctl->k2b;
+    K2bState st = dst->k2b;
     const unsigned long long raw_offset = ctl->raw_offset;
     int ovf = 0;
     for (int f = 0; f < nf; f++) {
@@ -465,7 +465,7 @@ __global__ void k_k2b_chain(ChunkCtl* ctl, int nf_override, const unsigned long 
         }
         anchors[f] = A;
     }
-    ctl->k2b = st;
+    dst->k2b = st;
     ctl->vp_overflow = ovf;
 }
 
@@ -559,7 +559,7 @@ __global__ __launch_bounds__(256) void k_k2b_linelocs(const ChunkCtl* ctl, int n
 }
 
 // ---- host launchers ----
-hipError_t run_k2b(const ChunkCtl* ctl_dev, ChunkCtl* ctl_mut, int nf_override, int max_fields, const Fmt& fmt,
+hipError_t run_k2b(const ChunkCtl* ctl_dev, ChunkCtl* ctl_mut, DecodeState* dst, int nf_override, int max_fields, const Fmt& fmt,
                    const unsigned long long* offsets, const int* k2_starts, const int* k2_lens, const int* k2_count,
                    int* vp_start, int* vp_len, unsigned char* vp_tg, K2bField* summ, K2bAnchorOut* anchors,
                    int* out_starts, int* out_lens, int* out_count, int* out_types, double* linelocs, int* is_first,
@@ -569,7 +569,7 @@ hipError_t run_k2b(const ChunkCtl* ctl_dev, ChunkCtl* ctl_mut, int nf_override, 
                        k2_count, tol, fmt.hsync_width, fmt.eq_pulse_width, fmt.vsync_width, fmt.samples_per_line,
                        fmt.num_eq_pulses, fmt.field_lines_first, fmt.field_lines_second, vp_start, vp_len, vp_tg,
                        summ);
-    hipLaunchKernelGGL(k_k2b_chain, dim3(1), dim3(64), 0, st, ctl_mut, nf_override, offsets, summ, vp_start, vp_tg,
+    hipLaunchKernelGGL(k_k2b_chain, dim3(1), dim3(64), 0, st, ctl_mut, dst, nf_override, offsets, summ, vp_start, vp_tg,
                        anchors, is_first, fmt.num_eq_pulses, fmt.field_lines_first, fmt.field_lines_second);
     hipLaunchKernelGGL(k_k2b_linelocs, dim3(max_fields), dim3(256), 0, st, ctl_dev, nf_override, summ, anchors,
                        vp_start, vp_len, vp_tg, out_starts, out_lens, out_count, out_types, linelocs);

@@ -159,12 +159,12 @@ __device__ int ntsc_phase_id_first(int quadrant) {
 }
 
 // Detection, retry replay and state update (chroma_decode.cu:674-1001).
-__global__ void k_chroma_resolve(ChunkCtl* ctl, int nf_override, const double* m0, const double* m1,
+__global__ void k_chroma_resolve(const ChunkCtl* ctl, DecodeState* dst, int nf_override, const double* m0, const double* m1,
                                  const double* iq0, int* out_track, int* phase_ids) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     const int nf = nf_override > 0 ? nf_override : ctl->nf;
     if (nf <= 0) return;
-    ChromaState cs = ctl->chroma;
+    ChromaState cs = dst->chroma;
     const int cycle[8] = {3, 2, 1, 4, 3, 2, 1, 4};
     int det, cycle_start;
     double thr;
@@ -232,10 +232,10 @@ __global__ void k_chroma_resolve(ChunkCtl* ctl, int nf_override, const double* m
     cs.current_track = (nf & 1) ? (1 - current) : current;
     cs.good_metric_threshold = thr;
     cs.cycle_start = (cycle_start + nf) % 8;
-    ctl->chroma = cs;
+    dst->chroma = cs;
 }
 
-hipError_t run_chroma(ChromaLaunch a, ChunkCtl* ctl_mut, int max_fields, double* m0, double* m1,
+hipError_t run_chroma(ChromaLaunch a, DecodeState* dst, int max_fields, double* m0, double* m1,
                       const double* cs_tab, double* iq0, int* out_track, int* phase_ids, int need_iq,
                       hipStream_t st) {
     // metric pass: lines 16..246, track 0
@@ -245,7 +245,7 @@ hipError_t run_chroma(ChromaLaunch a, ChunkCtl* ctl_mut, int max_fields, double*
     hipLaunchKernelGGL(k_chroma_line<0>, dim3(kFieldLines - 2 * kSkip, max_fields), dim3(64), 0, st, m);
     hipLaunchKernelGGL(k_chroma_metrics, dim3(max_fields), dim3(64), 0, st, a.burst, a.ctl,
                        a.nf_override, m0, m1, cs_tab, need_iq ? iq0 : nullptr);
-    hipLaunchKernelGGL(k_chroma_resolve, dim3(1), dim3(64), 0, st, ctl_mut, a.nf_override, m0, m1, iq0, out_track,
+    hipLaunchKernelGGL(k_chroma_resolve, dim3(1), dim3(64), 0, st, a.ctl, dst, a.nf_override, m0, m1, iq0, out_track,
                        phase_ids);
     ChromaLaunch o = a;
     o.track = out_track;

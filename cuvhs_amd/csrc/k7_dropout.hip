@@ -259,11 +259,11 @@ __global__ __launch_bounds__(kDoThreads) void k_dropouts(const unsigned char* __
 
 // Output metadata: fileLoc and the alternating field parity
 // (pipeline.cu:2577-2593), phase IDs passed through.
-__global__ void k_field_meta(ChunkCtl* ctl, const unsigned long long* offsets, const int* is_first_k2b,
-                             unsigned long long* file_loc, int* is_first_out) {
+__global__ void k_field_meta(const ChunkCtl* ctl, DecodeState* dst, const unsigned long long* offsets,
+                             const int* is_first_k2b, unsigned long long* file_loc, int* is_first_out) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     const int nf = ctl->nf;
-    int last = ctl->last_parity;
+    int last = dst->last_parity;
     for (int i = 0; i < nf; i++) {
         file_loc[i] = ctl->raw_offset + offsets[i];
         int h = is_first_k2b[i];
@@ -273,7 +273,7 @@ __global__ void k_field_meta(ChunkCtl* ctl, const unsigned long long* offsets, c
         last = first;
         is_first_out[i] = first;
     }
-    ctl->last_parity = last;
+    dst->last_parity = last;
 }
 
 size_t dropout_class_bytes(int max_fields) { return (size_t)max_fields * kMaxClassBlocks; }
@@ -292,9 +292,10 @@ hipError_t run_dropouts(const double* env, const double* ll, const ChunkCtl* ctl
     return hipGetLastError();
 }
 
-hipError_t run_field_meta(ChunkCtl* ctl, const unsigned long long* offsets, const int* is_first_k2b,
-                          unsigned long long* file_loc, int* is_first_out, hipStream_t st) {
-    hipLaunchKernelGGL(k_field_meta, dim3(1), dim3(64), 0, st, ctl, offsets, is_first_k2b, file_loc, is_first_out);
+hipError_t run_field_meta(const ChunkCtl* ctl, DecodeState* dst, const unsigned long long* offsets,
+                          const int* is_first_k2b, unsigned long long* file_loc, int* is_first_out, hipStream_t st) {
+    hipLaunchKernelGGL(k_field_meta, dim3(1), dim3(64), 0, st, ctl, dst, offsets, is_first_k2b, file_loc,
+                       is_first_out);
     return hipGetLastError();
 }
 

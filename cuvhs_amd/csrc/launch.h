@@ -66,7 +66,7 @@ hipError_t run_vsync_cluster(const int* events, ChunkCtl* ctl, long long total, 
                              int do_cluster, hipStream_t st);
 hipError_t run_sync_pulses(const int* events, const ChunkCtl* ctl, const unsigned long long* offsets, int spfp,
                            int* starts, int* lens, int* count, int max_fields, int nf_override, hipStream_t st);
-hipError_t run_k2b(const ChunkCtl* ctl_dev, ChunkCtl* ctl_mut, int nf_override, int max_fields, const Fmt& fmt,
+hipError_t run_k2b(const ChunkCtl* ctl_dev, ChunkCtl* ctl_mut, DecodeState* dst, int nf_override, int max_fields, const Fmt& fmt,
                    const unsigned long long* offsets, const int* k2_starts, const int* k2_lens, const int* k2_count,
                    int* vp_start, int* vp_len, unsigned char* vp_tg, K2bField* summ, K2bAnchorOut* anchors,
                    int* out_starts, int* out_lens, int* out_count, int* out_types, double* linelocs, int* is_first,
@@ -77,7 +77,7 @@ hipError_t run_hsync_refine(const double* d05, double* ll, const ChunkCtl* ctl, 
                             const Fmt& fmt, hipStream_t st);
 hipError_t run_tbc_resample(const double* demod, const double* ll, uint16_t* luma, const ChunkCtl* ctl,
                             int nf_override, int max_fields, const Fmt& fmt, hipStream_t st);
-hipError_t run_chroma(ChromaLaunch a, ChunkCtl* ctl_mut, int max_fields, double* m0, double* m1,
+hipError_t run_chroma(ChromaLaunch a, DecodeState* dst, int max_fields, double* m0, double* m1,
                       const double* cs_tab, double* iq0, int* out_track, int* phase_ids, int need_iq,
                       hipStream_t st);
 size_t dropout_class_bytes(int max_fields);
@@ -85,8 +85,8 @@ size_t dropout_partial_count(int max_fields);
 hipError_t run_dropouts(const double* env, const double* ll, const ChunkCtl* ctl, int nf_override, int max_fields,
                         const Fmt& fmt, double* partial, unsigned char* cls, uint16_t* luma, uint16_t* chroma,
                         int* lines, int* starts, int* ends, int* count, hipStream_t st);
-hipError_t run_field_meta(ChunkCtl* ctl, const unsigned long long* offsets, const int* is_first_k2b,
-                          unsigned long long* file_loc, int* is_first_out, hipStream_t st);
+hipError_t run_field_meta(const ChunkCtl* ctl, DecodeState* dst, const unsigned long long* offsets,
+                          const int* is_first_k2b, unsigned long long* file_loc, int* is_first_out, hipStream_t st);
 
 // launch parameter blocks (layouts shared with the kernel files)
 }  // namespace cuvhs

@@ -55,6 +55,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run 4 plain device copies of MIB MiB (calibrates HBM byte counters) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the decode path shards by "
@@ -63,6 +65,17 @@ def main():
     import numpy as np
     import torch
     from cuvhs_amd import native
+
+    if args.copy_probe > 0:
+        n = args.copy_probe << 20
+        src = torch.randint(0, 255, (n,), dtype=torch.uint8, device="cuda:0")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        for _ in range(4):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy probe: 4 copies of {n} bytes (read {n} B + write {n} B each)")
+        return
 
     t = time.perf_counter()
     cap = native.generate(1)

@@ -97,6 +97,7 @@ struct cuvhs_ctx {
     DecodeState* hstate = nullptr;  // pinned, for resets
     std::vector<void*> allocs;
     bool prof = false;
+    bool serial = false;            // profiling: every stage on the head stream, no overlap
     double stage_ms[kStages + 1] = {};
 };
 
@@ -286,15 +287,18 @@ struct OutCopy {
 // field count the head produced, out (optional) receives the chunk's outputs.
 int tail_stage(cuvhs_ctx* c, Slot& s, const OutCopy* out, int n) {
     const Fmt& f = c->fmt;
-    hipStream_t st = c->tail;
+    hipStream_t st = c->serial ? c->stream : c->tail;
+    hipStream_t side = c->serial ? c->stream : c->side;
     HIPCHK(c, hipStreamWaitEvent(st, s.head_done, 0));
     mark(c, s, 5, st);
     HIPCHK(c, hipEventRecord(s.tail0, st));
-    HIPCHK(c, hipStreamWaitEvent(c->side, s.tail0, 0));
-    HIPCHK(c, hipEventRecord(s.pre0, c->side));
-    int r = prebpf_launch(c, s.chunk, (long long)s.n_read, s.max_fields, c->side);
+    HIPCHK(c, hipStreamWaitEvent(side, s.tail0, 0));
+    HIPCHK(c, hipEventRecord(s.pre0, side));
+    int r = prebpf_launch(c, s.chunk, (long long)s.n_read, s.max_fields, side);
     if (r) return r;
-    HIPCHK(c, hipEventRecord(s.pre1, c->side));
+    HIPCHK(c, hipEventRecord(s.pre1, side));
+    // in serial profiling the pre-band-pass runs here, before the sync stages
+    if (c->serial) mark(c, s, 5, st);
     HIPCHK(c, run_sync_pulses(s.events, s.ctl, s.offsets, f.spfp, c->k2s, c->k2l, c->k2c, s.max_fields, 0, st));
     mark(c, s, 6, st);
     HIPCHK(c, run_k2b(s.ctl, s.ctl, c->state, 0, s.max_fields, f, s.offsets, c->k2s, c->k2l, c->k2c, c->vps, c->vpl,
@@ -682,7 +686,11 @@ int cuvhs_profile(cuvhs_ctx* c, int enable) {
         for (Slot& s : c->slot)
             for (hipEvent_t& e : s.mark) HIPCHK(c, hipEventCreate(&e));
     }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->tail));
+    HIPCHK(c, hipStreamSynchronize(c->side));
     c->prof = enable != 0;
+    c->serial = enable == 2;
     for (double& v : c->stage_ms) v = 0.0;
     return 0;
 }

@@ -103,9 +103,12 @@ template <int S, typename T> __device__ __forceinline__ void dft16(typename C2<T
 }
 
 // One Stockham stage with radix R over B points, ns = product of previous radices.
-// Each of the NT threads handles 16/R butterflies (16 values).
+// Each of the NT threads handles 16/R butterflies (16 values).  With G set
+// (last stage of a forward transform, whose outputs are in natural order)
+// every output bin is multiplied by G[bin] before it is stored.
 template <int S, typename T, int B, int R, int NT>
-__device__ __forceinline__ void fft_stage(typename C2<T>::type* buf, const typename C2<T>::type* tw, int ns) {
+__device__ __forceinline__ void fft_stage(typename C2<T>::type* buf, const typename C2<T>::type* tw, int ns,
+                                          const typename C2<T>::type* G = nullptr) {
     using V = typename C2<T>::type;
     constexpr int NB = B / R;          // butterflies in the stage
     constexpr int PER = NB / NT;       // butterflies per thread
@@ -157,6 +160,18 @@ __device__ __forceinline__ void fft_stage(typename C2<T>::type* buf, const typen
         else if (R == 4) dft4<S>(u[0], u[1], u[2], u[3]);
         else if (R == 2) dft2<S>(u[0], u[1]);
     }
+    if (G) {
+        V g[16];
+#pragma unroll
+        for (int p = 0; p < PER; p++) {
+            int j = t + p * NT;
+            int idx = (j / ns) * ns * R + j % ns;
+#pragma unroll
+            for (int r = 0; r < R; r++) g[p * R + r] = G[idx + r * ns];
+        }
+#pragma unroll
+        for (int i = 0; i < 16; i++) v[i] = cmul(v[i], g[i]);
+    }
 #pragma unroll
     for (int p = 0; p < PER; p++) {
         int j = t + p * NT;
@@ -166,6 +181,42 @@ __device__ __forceinline__ void fft_stage(typename C2<T>::type* buf, const typen
         for (int r = 0; r < R; r++) buf[pidx(idx + r * ns)] = v[p * R + r];
     }
     __syncthreads();
+}
+
+// First stage (ns = 1, radix 16) on values the calling thread already holds:
+// thread t owns inputs t + r * B/16, r = 0..15, which is exactly what the
+// stage would load from LDS, so the input never needs a round trip through
+// LDS.  The stage output is stored and synchronised as usual.
+template <int S, typename T, int B>
+__device__ __forceinline__ void fft_first_stage_regs(typename C2<T>::type* buf, typename C2<T>::type* v) {
+    dft16<S, T>(v);
+    const int idx = (int)threadIdx.x * 16;
+#pragma unroll
+    for (int r = 0; r < 16; r++) buf[pidx(idx + r)] = v[r];
+    __syncthreads();
+}
+
+// Forward transform of register-held input (layout of fft_first_stage_regs)
+// followed by the pointwise product with the spectrum G (natural bin order),
+// fused into the last stage's stores.  Result: buf[k] = FFT(x)[k] * G[k].
+template <typename T, int B>
+__device__ __forceinline__ void fft_fwd_regs_mul(typename C2<T>::type* buf, typename C2<T>::type* v,
+                                                 const typename C2<T>::type* tw, const typename C2<T>::type* G) {
+    constexpr int NT = B / 16;
+    static_assert(B == 1024 || B == 4096 || B == 8192 || B == 16384, "unsupported FFT size");
+    fft_first_stage_regs<-1, T, B>(buf, v);
+    if (B == 1024) {
+        fft_stage<-1, T, B, 16, NT>(buf, tw, 16);
+        fft_stage<-1, T, B, 4, NT>(buf, tw, 256, G);
+    } else if (B == 4096) {
+        fft_stage<-1, T, B, 16, NT>(buf, tw, 16);
+        fft_stage<-1, T, B, 16, NT>(buf, tw, 256, G);
+    } else {
+        fft_stage<-1, T, B, 16, NT>(buf, tw, 16);
+        fft_stage<-1, T, B, 16, NT>(buf, tw, 256);
+        if (B == 8192) fft_stage<-1, T, B, 2, NT>(buf, tw, 4096, G);
+        if (B == 16384) fft_stage<-1, T, B, 4, NT>(buf, tw, 4096, G);
+    }
 }
 
 // buf[k] *= G[k] for all B bins (G in global memory, loads issued together).

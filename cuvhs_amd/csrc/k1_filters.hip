@@ -51,16 +51,13 @@ __global__ __launch_bounds__(B / 16) void k1_part_a(K1ALaunch a) {
         s = s < 0 ? 0 : (s > last ? last : s);
         xin[i] = u8_to_s16(a.cap, s, a.n_valid);
     }
+    double2 v[16];
 #pragma unroll
     for (int i = 0; i < 16; i++) {
-        double2 v;
-        v.x = xin[i];
-        v.y = 0.0;
-        buf[pidx(threadIdx.x + i * NT)] = v;
+        v[i].x = xin[i];
+        v[i].y = 0.0;
     }
-    __syncthreads();
-    fft_lds<-1, double, B>(buf, a.tw);
-    spectrum_multiply<double, B>(buf, a.G);
+    fft_fwd_regs_mul<double, B>(buf, v, a.tw, a.G);
     fft_lds<+1, double, B>(buf, a.tw);
     // outputs o = 0..V: analytic a(qs + o) = buf[lp + o]
     constexpr int MAXO = (B + NT - 1) / NT;
@@ -126,16 +123,13 @@ __global__ __launch_bounds__(B / 16) void k1_part_b(K1BLaunch a) {
         s = s < 0 ? 0 : (s > last ? last : s);
         xin[i] = a.rawdemod[s];
     }
+    double2 v[16];
 #pragma unroll
     for (int i = 0; i < 16; i++) {
-        double2 v;
-        v.x = xin[i];
-        v.y = 0.0;
-        buf[pidx(threadIdx.x + i * NT)] = v;
+        v[i].x = xin[i];
+        v[i].y = 0.0;
     }
-    __syncthreads();
-    fft_lds<-1, double, B>(buf, a.tw);
-    spectrum_multiply<double, B>(buf, a.G);
+    fft_fwd_regs_mul<double, B>(buf, v, a.tw, a.G);
     fft_lds<+1, double, B>(buf, a.tw);
     for (int o = threadIdx.x; o < a.V; o += NT) {
         int q = qs + o;
@@ -165,16 +159,13 @@ __global__ __launch_bounds__(B / 16) void k_prebpf(PreLaunch a) {
         int n = wrap_n(ns - a.lp + (int)threadIdx.x + i * NT, a.N2);
         xin[i] = n < a.spfp ? (float)u8_to_s16(a.cap, seg0 + n, a.n_valid) : 0.0f;
     }
+    float2 v[16];
 #pragma unroll
     for (int i = 0; i < 16; i++) {
-        float2 v;
-        v.x = xin[i];
-        v.y = 0.0f;
-        buf[pidx(threadIdx.x + i * NT)] = v;
+        v[i].x = xin[i];
+        v[i].y = 0.0f;
     }
-    __syncthreads();
-    fft_lds<-1, float, B>(buf, a.tw);
-    spectrum_multiply<float, B>(buf, a.G);
+    fft_fwd_regs_mul<float, B>(buf, v, a.tw, a.G);
     fft_lds<+1, float, B>(buf, a.tw);
     for (int o = threadIdx.x; o < a.V; o += NT) {
         int n = ns + o;
@@ -243,10 +234,17 @@ __global__ __launch_bounds__(B / 16) void k_fft_selftest(const typename C2<T>::t
     using V = typename C2<T>::type;
     extern __shared__ __align__(16) unsigned char smem[];
     V* buf = reinterpret_cast<V*>(smem);
-    for (int j = threadIdx.x; j < B; j += B / 16) buf[pidx(j)] = in[j];
-    __syncthreads();
-    if (sign < 0) fft_lds<-1, T, B>(buf, tw);
-    else fft_lds<+1, T, B>(buf, tw);
+    if (sign == -2) {
+        // forward transform fed from registers (the blocked-convolution path)
+        V v[16];
+        for (int r = 0; r < 16; r++) v[r] = in[threadIdx.x + r * (B / 16)];
+        fft_fwd_regs_mul<T, B>(buf, v, tw, nullptr);
+    } else {
+        for (int j = threadIdx.x; j < B; j += B / 16) buf[pidx(j)] = in[j];
+        __syncthreads();
+        if (sign < 0) fft_lds<-1, T, B>(buf, tw);
+        else fft_lds<+1, T, B>(buf, tw);
+    }
     for (int j = threadIdx.x; j < B; j += B / 16) out[j] = buf[pidx(j)];
 }
 

@@ -120,44 +120,55 @@ __global__ __launch_bounds__(kEvThreads) void k_ev_write(const double* x, long l
 
 // VSYNC discovery over the event list + chunk clustering (one workgroup).
 // offsets[] and ctl->{nf, next_raw_offset, vsync_count, status} are written.
+// The candidate compaction takes 1024 events per step: wavefront ballots give
+// each candidate its rank, the 16 wavefront counts are exchanged through LDS
+// (double-buffered, one barrier per step), and the next step's events are
+// loaded before the barrier.
 __global__ __launch_bounds__(1024) void k_vsync_cluster(const int* events, ChunkCtl* ctl, long long total,
                                                         int min_w, int max_w, int cap, int* cand,
                                                         unsigned long long* offsets, int num_fields, int spf,
                                                         int spl, int fields_k1, int do_cluster) {
-    __shared__ int sh[1024];
-    __shared__ int carry_all, carry_kept;
+    constexpr int kWaves = 1024 / 64;
+    __shared__ int wcnt[2][kWaves];
     const int nev = ctl->nevents;
-    if (threadIdx.x == 0) { carry_all = 0; carry_kept = 0; }
-    __syncthreads();
-    for (int base = 0; base < nev; base += 1024) {
-        int k = base + threadIdx.x;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    int carry_all = 0;
+    int k = threadIdx.x;
+    int e0 = k < nev ? events[k] : 0;
+    int e1 = k + 1 < nev ? events[k + 1] : 0;
+    for (int base = 0, it = 0; base < nev; base += 1024, it ^= 1) {
         int is_c = 0;
         int pos = 0;
-        if (k < nev) {
-            int e = events[k];
-            if ((e & 1) == 0) {
-                pos = e >> 1;
-                long long end = total;
-                if (k + 1 < nev) end = events[k + 1] >> 1;  // next event is this run's end
-                long long w = end - pos;
-                if (w > max_w + 1) w = max_w + 1;
-                is_c = (w >= min_w && w <= max_w) ? 1 : 0;
-            }
+        if (k < nev && (e0 & 1) == 0) {
+            pos = e0 >> 1;
+            long long end = total;
+            if (k + 1 < nev) end = e1 >> 1;  // next event is this run's end
+            long long w = end - pos;
+            if (w > max_w + 1) w = max_w + 1;
+            is_c = (w >= min_w && w <= max_w) ? 1 : 0;
         }
-        sh[threadIdx.x] = is_c;
+        const int kn = k + 1024;
+        const int n0 = kn < nev ? events[kn] : 0;
+        const int n1 = kn + 1 < nev ? events[kn + 1] : 0;
+        const unsigned long long m = __ballot(is_c);
+        if (lane == 0) wcnt[it][wave] = __popcll(m);
         __syncthreads();
-        for (int off = 1; off < 1024; off <<= 1) {
-            int add = threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
-            __syncthreads();
-            sh[threadIdx.x] += add;
-            __syncthreads();
+        int woff = 0, tot = 0;
+#pragma unroll
+        for (int w = 0; w < kWaves; w++) {
+            int v = wcnt[it][w];
+            woff += w < wave ? v : 0;
+            tot += v;
         }
-        int idx = carry_all + sh[threadIdx.x] - is_c;
+        const int idx = carry_all + woff + __popcll(m & lt);
         if (is_c && idx < cap) cand[idx] = pos;
-        __syncthreads();
-        if (threadIdx.x == 0) carry_all += sh[1023];
-        __syncthreads();
+        carry_all += tot;
+        k = kn;
+        e0 = n0;
+        e1 = n1;
     }
+    __syncthreads();
     if (threadIdx.x != 0 || !do_cluster) {
         if (threadIdx.x == 0) ctl->vsync_count = carry_all;
         return;
@@ -194,41 +205,59 @@ __global__ __launch_bounds__(1024) void k_vsync_cluster(const int* events, Chunk
     ctl->status |= status;
 }
 
-// K2: one thread per field, walking the event list from the window start.
-__global__ void k_sync_pulses(const int* events, const ChunkCtl* ctl, const unsigned long long* offsets, int spfp,
-                              int* starts, int* lens, int* count, int nf_override) {
-    int f = blockIdx.x * blockDim.x + threadIdx.x;
-    int nf = nf_override > 0 ? nf_override : ctl->nf;
+// K2: one wavefront per field.  Events alternate start/end, so after an
+// optional leading end (a run already open at the window start, whose pulse
+// the reference drops) pulse j is the event pair (k0 + 2j, k0 + 2j + 1), and
+// the pulses the reference's serial scan keeps are the prefix of pairs whose
+// end lies inside the window (end positions increase with j).
+__global__ __launch_bounds__(64) void k_sync_pulses(const int* events, const ChunkCtl* ctl,
+                                                    const unsigned long long* offsets, int spfp, int* starts,
+                                                    int* lens, int* count, int nf_override) {
+    const int f = blockIdx.x;
+    const int nf = nf_override > 0 ? nf_override : ctl->nf;
     if (f >= nf) return;
+    const int lane = threadIdx.x;
     const int nev = ctl->nevents;
-    long long off = (long long)offsets[f];
-    long long wend = off + spfp;
-    // first event with position > off
+    const long long off = (long long)offsets[f];
+    const long long wend = off + spfp;
+    // first event with position > off: 64-way search, the lanes probing
+    // evenly spaced events of the remaining range [lo, hi]
     int lo = 0, hi = nev;
-    while (lo < hi) {
-        int mid = (lo + hi) >> 1;
-        if ((long long)(events[mid] >> 1) <= off) lo = mid + 1;
-        else hi = mid;
+    while (hi > lo) {
+        const int step = (hi - lo + 63) / 64;
+        const int idx = lo + lane * step;
+        const bool le = idx < hi && (long long)(events[idx] >> 1) <= off;
+        const int t = __popcll(__ballot(le));  // the probes at or before off form a prefix
+        if (t == 0) break;
+        const int nlo = lo + (t - 1) * step + 1;
+        hi = min(lo + t * step, hi);
+        lo = nlo;
     }
-    int c = 0;
+    int k0 = lo;
+    if (k0 < nev && (events[k0] & 1)) k0++;
     int* st = starts + (size_t)f * kMaxPulses;
     int* ln = lens + (size_t)f * kMaxPulses;
-    for (int k = lo; k < nev; k++) {
-        int e = events[k];
-        long long p = e >> 1;
-        if (p >= wend) break;
-        if (e & 1) continue;                 // an end whose start precedes the window
-        if (k + 1 >= nev) break;             // run never ends inside the buffer
-        long long pe = events[k + 1] >> 1;
-        if (pe >= wend) break;               // run still open at the window end
-        if (c < kMaxPulses) {
-            st[c] = (int)p;
-            ln[c] = (int)(pe - p);
-            c++;
+    int c = 0;
+    for (int base = 0; base < kMaxPulses; base += 64) {
+        const int j = base + lane;
+        const long long ke = (long long)k0 + 2LL * j + 1;
+        bool ok = false;
+        int ps = 0, pe = 0;
+        if (ke < nev) {
+            ps = events[ke - 1] >> 1;
+            pe = events[ke] >> 1;
+            ok = (long long)pe < wend;
         }
-        k++;
+        const unsigned long long inv = ~__ballot(ok);
+        const int run = inv ? __builtin_ctzll(inv) : 64;
+        if (lane < run && j < kMaxPulses) {
+            st[j] = ps;
+            ln[j] = pe - ps;
+        }
+        c = base + run;
+        if (run < 64) break;
     }
-    count[f] = c;
+    if (lane == 0) count[f] = c < kMaxPulses ? c : kMaxPulses;
 }
 
 // ---- host launchers ----
@@ -254,9 +283,9 @@ hipError_t run_vsync_cluster(const int* events, ChunkCtl* ctl, long long total, 
 
 hipError_t run_sync_pulses(const int* events, const ChunkCtl* ctl, const unsigned long long* offsets, int spfp,
                            int* starts, int* lens, int* count, int max_fields, int nf_override, hipStream_t st) {
-    int bs = 64;
-    hipLaunchKernelGGL(k_sync_pulses, dim3((max_fields + bs - 1) / bs), dim3(bs), 0, st, events, ctl, offsets, spfp,
-                       starts, lens, count, nf_override);
+    if (max_fields < 1) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_sync_pulses, dim3(max_fields), dim3(64), 0, st, events, ctl, offsets, spfp, starts, lens,
+                       count, nf_override);
     return hipGetLastError();
 }
 

@@ -190,8 +190,11 @@ class Decoder:
         except Exception:
             pass
 
-    def profile(self, enable=True):
-        check(lib().cuvhs_profile(self.h, 1 if enable else 0), self.h, "profile")
+    def profile(self, enable=True, serial=False):
+        """Stage timing; serial=True runs the stages one after another so each
+        total is that stage's own device time."""
+        mode = (2 if serial else 1) if enable else 0
+        check(lib().cuvhs_profile(self.h, mode), self.h, "profile")
 
     def profile_read(self):
         """{stage name: accumulated device ms} since profiling was enabled."""

@@ -10,9 +10,13 @@
  *   >0 data warning bits (CUVHS_W_*), the reference's soft paths kept
  *      observable (pipeline.cu:1584-1587, 1614-1620).
  *
- * Work is enqueued on the context's stream; entry points that return
- * host-visible results synchronise that stream.  A context is not
- * thread-safe; distinct contexts are independent.
+ * Stage entry points enqueue on the context's stream and synchronise it.
+ * Chunk decodes run a chunk's head (K1, sync events, VSYNC clustering) on
+ * the context's stream and its tail (pulses through metadata) on a second
+ * stream owned by the context, with the chroma pre-band-pass on a third, so
+ * that consecutive chunks overlap; they return once all three are idle.  All
+ * streams have the default priority.  A context is not thread-safe; distinct
+ * contexts are independent.
  */
 #ifndef CUVHS_H
 #define CUVHS_H
@@ -158,15 +162,19 @@ int cuvhs_dropouts(cuvhs_ctx* ctx, const double* d_env, const double* d_linelocs
 
 /* Self-test of the LDS FFT building block (no counterpart in the reference):
  * one B-point transform (B in 1024/4096/8192 FP64, 1024/8192/16384 FP32) of
- * device data, sign -1 forward / +1 inverse, unnormalised. */
+ * device data, sign -1 forward / +1 inverse, unnormalised; sign -2 is the
+ * forward transform fed from registers, as the blocked convolutions use it. */
 int cuvhs_selftest_fft(int B, int is_f32, const void* d_in, void* d_out, int sign);
 
-/* Per-stage device time (HIP events on the context stream) accumulated over
- * cuvhs_decode_chunk calls while enabled; enabling clears the totals.
- * cuvhs_profile_read fills up to n stage totals (ms) and names and returns
- * the number of stages; the last one is the chroma pre-band-pass, which runs
- * on a second stream concurrently with the sync and line-location stages.  (Counterpart of nothing in the reference, which
- * only times the whole chunk loop: pipeline.cu:2642-2708.) */
+/* Per-stage device time (HIP events on the stream each stage runs on)
+ * accumulated over chunk decodes while enabled; enabling clears the totals.
+ * enable = 1 keeps the overlapped schedule (stages that share the GPU with
+ * another chunk's stages read longer); enable = 2 runs every stage of every
+ * chunk in sequence on the context's stream, so each total is the stage's
+ * own time.  cuvhs_profile_read fills up to n stage totals (ms) and names
+ * and returns the number of stages; the last one is the chroma
+ * pre-band-pass.  (Counterpart of nothing in the reference, which only times
+ * the whole chunk loop: pipeline.cu:2642-2708.) */
 int cuvhs_profile(cuvhs_ctx* ctx, int enable);
 int cuvhs_profile_read(const cuvhs_ctx* ctx, double* ms, const char** names, int n);
 

@@ -42,7 +42,7 @@ def test_lds_fft_vs_numpy(torch, N, B, f32):
     dt, rt = (np.complex64, np.float32) if f32 else (np.complex128, np.float64)
     xin = dev(torch, x.astype(dt).view(rt))
     out = torch.empty_like(xin)
-    for sign in (-1, 1):
+    for sign in (-1, 1, -2):  # -2: forward transform fed from registers
         assert N.lib().cuvhs_selftest_fft(B, f32, N.ptr(xin), N.ptr(out), sign) == 0
         y = out.cpu().numpy().view(dt)
         ref = np.fft.fft(x) if sign < 0 else np.fft.ifft(x) * B

@@ -38,7 +38,7 @@ def log(*a):
 def cpu_baseline(cap, fmt, batch):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle_py
-    sample = cap[: batch * fmt.spfp + fmt.samples_per_field]
+    sample = cap[: 2 * batch * fmt.spfp]  # two chunks
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     t = time.perf_counter()
     ref = oracle_py.decode(sample, 28.0, batch, trace=False)
@@ -99,31 +99,39 @@ def main():
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     prof = dec.profile_read()
+    k1_samples, chunks, _ = dec.profile_work()
+    # one more decode, untimed, with the stages serialised: each stage's own time
+    dec.profile(True, serial=True)
+    dec.reset()
+    dec.decode_capture(d_cap)
+    iso = dec.profile_read()
     dec.profile(False)
     ms_per_step = dt * 1000.0 / args.steps
     value = fields * args.steps / dt
     bytes_per_field = cap.size / fields + 2 * 2 * 910 * 263  # u8 in + luma/chroma u16 out
     log(f"{fields} fields/step, {ms_per_step:.2f} ms/step, {value:.1f} fields/s")
     stage_ms = {k: v / args.steps for k, v in prof.items()}
-    log("per-step stage ms: " + ", ".join(f"{k}={v:.3f}" for k, v in stage_ms.items()))
+    log("per-step stage ms (overlapped): " + ", ".join(f"{k}={v:.3f}" for k, v in stage_ms.items()))
+    log("per-step stage ms (serialised): " + ", ".join(f"{k}={v:.3f}" for k, v in iso.items()))
 
-    # dominant kernel with an algorithmic byte count
-    chunks = int(np.ceil(fields / args.batch))
-    dom = max((k for k in STAGE_BYTES if k in stage_ms), key=lambda k: stage_ms[k])
+    # dominant kernel (largest serialised time) among those with a byte count
+    dom = max((k for k in STAGE_BYTES if k in iso), key=lambda k: iso[k])
     unit, per_unit = STAGE_BYTES[dom]
-    samples_per_step = chunks * args.batch * fmt.spfp
-    launches_per_step = chunks
-    alg_bytes_per_launch = per_unit * samples_per_step / launches_per_step
-    avg_launch_s = stage_ms[dom] / 1000.0 / launches_per_step
+    launches = chunks                      # one launch per chunk in the timed region
+    alg_bytes_per_launch = per_unit * k1_samples / launches
+    avg_launch_s = stage_ms[dom] * args.steps / 1000.0 / launches
     achieved = alg_bytes_per_launch / avg_launch_s / 1e9
+    iso_launch_s = iso[dom] / 1000.0 / (launches / args.steps)
     traffic = None
     pmc_path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
     if os.path.exists(pmc_path):
         with open(pmc_path) as f:
-            traffic = json.load(f).get(dom)
+            traffic = json.load(f).get("bytes_per_launch", {}).get(dom)
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                 "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
-                "alg_bytes_per_launch": int(alg_bytes_per_launch), "avg_launch_ms": round(avg_launch_s * 1e3, 4)}
+                "alg_bytes_per_launch": int(alg_bytes_per_launch), "avg_launch_ms": round(avg_launch_s * 1e3, 4),
+                "achieved_isolated": round(alg_bytes_per_launch / iso_launch_s / 1e9, 1),
+                "avg_launch_ms_isolated": round(iso_launch_s * 1e3, 4)}
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -149,6 +157,7 @@ def main():
                    "precision": "fp64 (chroma pre-band-pass fp32)"},
         "hbm_gbs_algorithmic": round(value * bytes_per_field / 1e9, 2),
         "stage_ms_per_step": {k: round(v, 3) for k, v in stage_ms.items()},
+        "stage_ms_per_step_isolated": {k: round(v, 3) for k, v in iso.items()},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

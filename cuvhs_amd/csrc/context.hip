@@ -42,7 +42,8 @@ struct Slot {
     unsigned long long* offsets = nullptr;
     ChunkCtl* ctl = nullptr;      // device
     ChunkCtl* hctl = nullptr;     // pinned host copy
-    hipEvent_t head_done = nullptr, tail_done = nullptr, pre0 = nullptr, pre1 = nullptr, tail0 = nullptr;
+    float* craw = nullptr;        // chroma pre-band-pass output
+    hipEvent_t head_done = nullptr, tail_done = nullptr, pre0 = nullptr, pre1 = nullptr;
     hipEvent_t mark[kMarks] = {};
     // host view of the chunk
     const uint8_t* chunk = nullptr;
@@ -73,7 +74,6 @@ struct cuvhs_ctx {
     Slot slot[2];
     double* rawdemod = nullptr;     // head only
     int* cand = nullptr;            // head only
-    float* craw = nullptr;          // tail only
     int *k2s = nullptr, *k2l = nullptr, *k2c = nullptr;
     int *ps = nullptr, *pl = nullptr, *pc = nullptr, *pt = nullptr;
     int *vps = nullptr, *vpl = nullptr;
@@ -99,6 +99,8 @@ struct cuvhs_ctx {
     bool prof = false;
     bool serial = false;            // profiling: every stage on the head stream, no overlap
     double stage_ms[kStages + 1] = {};
+    uint64_t work_samples = 0;      // profiling: K1 samples demodulated
+    int work_chunks = 0, work_fields = 0;
 };
 
 namespace {
@@ -161,9 +163,9 @@ void mark(cuvhs_ctx* c, Slot& s, int i, hipStream_t st) {
     if (c->prof) (void)hipEventRecord(s.mark[i], st);
 }
 
-ChromaLaunch chroma_launch(cuvhs_ctx* c, const ChunkCtl* ctl, int nf_override) {
+ChromaLaunch chroma_launch(cuvhs_ctx* c, const ChunkCtl* ctl, int nf_override, const float* craw) {
     ChromaLaunch a{};
-    a.craw = c->craw;
+    a.craw = craw;
     a.linelocs = c->ll;
     a.ctl = ctl;
     a.nf_override = nf_override;
@@ -222,7 +224,7 @@ int k1_launch(cuvhs_ctx* c, Slot& s, const uint8_t* chunk, long long n_valid, in
     return 0;
 }
 
-int prebpf_launch(cuvhs_ctx* c, const uint8_t* chunk, long long n_valid, int nf_max, hipStream_t st) {
+int prebpf_launch(cuvhs_ctx* c, const uint8_t* chunk, long long n_valid, int nf_max, float* out, hipStream_t st) {
     PreLaunch p{};
     p.cap = chunk;
     p.n_valid = n_valid;
@@ -233,7 +235,7 @@ int prebpf_launch(cuvhs_ctx* c, const uint8_t* chunk, long long n_valid, int nf_
     p.V = c->pre.bpf.valid;
     p.G = c->G_p;
     p.tw = c->tw_p;
-    p.out = c->craw;
+    p.out = out;
     HIPCHK(c, run_prebpf(c->pre.bpf.B, p, nf_max, st));
     return 0;
 }
@@ -252,17 +254,24 @@ bool plan_chunk(cuvhs_ctx* c, Slot& s, const uint8_t* chunk, uint64_t avail, uin
     return true;
 }
 
-// Head stage on the head stream: K1, sync events, VSYNC clustering.
+// Head stage on the head stream: K1, sync events, VSYNC clustering; the
+// chroma pre-band-pass, which needs only the capture, starts at the same time
+// on the side stream.
 int head_stage(cuvhs_ctx* c, Slot& s) {
     const Fmt& f = c->fmt;
     hipStream_t st = c->stream;
+    hipStream_t side = c->serial ? c->stream : c->side;
+    HIPCHK(c, hipEventRecord(s.pre0, side));
+    int r = prebpf_launch(c, s.chunk, (long long)s.n_read, s.max_fields, s.craw, side);
+    if (r) return r;
+    HIPCHK(c, hipEventRecord(s.pre1, side));
     ChunkCtl* h = s.hctl;
     memset(h, 0, sizeof(*h));
     h->fields_k1 = s.fk1;
     h->raw_offset = s.raw_offset;
     HIPCHK(c, hipMemcpyAsync(s.ctl, h, sizeof(ChunkCtl), hipMemcpyHostToDevice, st));
     mark(c, s, 0, st);
-    int r = k1_launch(c, s, s.chunk, (long long)s.n_read, s.fk1, st);
+    r = k1_launch(c, s, s.chunk, (long long)s.n_read, s.fk1, st);
     if (r) return r;
     mark(c, s, 2, st);
     const long long T = (long long)s.fk1 * f.spfp;
@@ -283,22 +292,13 @@ struct OutCopy {
     int32_t *is_first, *phase_id, *do_count, *do_line, *do_startx, *do_endx;
 };
 
-// Tail stage on the tail stream (pre-band-pass on the side stream); n is the
-// field count the head produced, out (optional) receives the chunk's outputs.
+// Tail stage on the tail stream; n is the field count the head produced, out
+// (optional) receives the chunk's outputs.
 int tail_stage(cuvhs_ctx* c, Slot& s, const OutCopy* out, int n) {
     const Fmt& f = c->fmt;
     hipStream_t st = c->serial ? c->stream : c->tail;
-    hipStream_t side = c->serial ? c->stream : c->side;
     HIPCHK(c, hipStreamWaitEvent(st, s.head_done, 0));
     mark(c, s, 5, st);
-    HIPCHK(c, hipEventRecord(s.tail0, st));
-    HIPCHK(c, hipStreamWaitEvent(side, s.tail0, 0));
-    HIPCHK(c, hipEventRecord(s.pre0, side));
-    int r = prebpf_launch(c, s.chunk, (long long)s.n_read, s.max_fields, side);
-    if (r) return r;
-    HIPCHK(c, hipEventRecord(s.pre1, side));
-    // in serial profiling the pre-band-pass runs here, before the sync stages
-    if (c->serial) mark(c, s, 5, st);
     HIPCHK(c, run_sync_pulses(s.events, s.ctl, s.offsets, f.spfp, c->k2s, c->k2l, c->k2c, s.max_fields, 0, st));
     mark(c, s, 6, st);
     HIPCHK(c, run_k2b(s.ctl, s.ctl, c->state, 0, s.max_fields, f, s.offsets, c->k2s, c->k2l, c->k2c, c->vps, c->vpl,
@@ -311,7 +311,7 @@ int tail_stage(cuvhs_ctx* c, Slot& s, const OutCopy* out, int n) {
     mark(c, s, 9, st);
     HIPCHK(c, hipStreamWaitEvent(st, s.pre1, 0));
     mark(c, s, 10, st);
-    HIPCHK(c, run_chroma(chroma_launch(c, s.ctl, 0), c->state, s.max_fields, c->m0, c->m1, c->cs_tab, c->iq0, c->track,
+    HIPCHK(c, run_chroma(chroma_launch(c, s.ctl, 0, s.craw), c->state, s.max_fields, c->m0, c->m1, c->cs_tab, c->iq0, c->track,
                          c->phase, 1, st));
     mark(c, s, 11, st);
     HIPCHK(c, run_dropouts(s.env, c->ll, s.ctl, 0, s.max_fields, f, c->do_partial, c->do_cls, c->luma, c->chroma,
@@ -339,6 +339,9 @@ int tail_stage(cuvhs_ctx* c, Slot& s, const OutCopy* out, int n) {
 // after tail_done of slot s: accumulate its stage times
 void collect_profile(cuvhs_ctx* c, Slot& s) {
     if (!c->prof) return;
+    c->work_samples += (uint64_t)s.fk1 * (uint64_t)c->fmt.spfp;
+    c->work_chunks++;
+    c->work_fields += s.hctl->nf;
     for (int i = 0; i < kStages; i++) {
         int a = i < 4 ? i : i + 1;
         float ms = 0.0f;
@@ -417,7 +420,7 @@ int cuvhs_create(const cuvhs_config* cfg, cuvhs_ctx** out) {
     e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking);
     if (e != hipSuccess) return bad(e, "hipStreamCreate side");
     for (Slot& s : c->slot) {
-        for (hipEvent_t* ev : {&s.head_done, &s.tail_done, &s.tail0}) {
+        for (hipEvent_t* ev : {&s.head_done, &s.tail_done}) {
             e = hipEventCreateWithFlags(ev, hipEventDisableTiming);
             if (e != hipSuccess) return bad(e, "hipEventCreate");
         }
@@ -430,7 +433,7 @@ int cuvhs_create(const cuvhs_config* cfg, cuvhs_ctx** out) {
         const int bk1[2] = {4096, 8192};
         c->k1 = design_k1(f, 1e-13, bk1, 2, 1);
         const int bpre[2] = {8192, 16384};
-        c->pre = design_prebpf(f, 1e-9, bpre, 2);
+        c->pre = design_prebpf(f, 1e-7, bpre, 2);
     } catch (...) {
         cuvhs_destroy(c);
         return CUVHS_E_UNSUPPORTED;
@@ -471,7 +474,6 @@ int cuvhs_create(const cuvhs_config* cfg, cuvhs_ctx** out) {
     c->buf_n = S + f.spfp;  // one segment of zero padding past the chunk
     ALLOC(c->rawdemod, S);
     ALLOC(c->cand, 20 * c->F);
-    ALLOC(c->craw, S);
     for (Slot& s : c->slot) {
         ALLOC(s.demod, c->buf_n);
         ALLOC(s.d05, c->buf_n);
@@ -480,6 +482,7 @@ int cuvhs_create(const cuvhs_config* cfg, cuvhs_ctx** out) {
         ALLOC(s.wg_count, events_nwg(S) + 1);
         ALLOC(s.wg_off, events_nwg(S) + 1);
         ALLOC(s.offsets, Fz);
+        ALLOC(s.craw, S);
         ALLOC(s.ctl, 1);
         e = hipHostMalloc((void**)&s.hctl, sizeof(ChunkCtl), hipHostMallocDefault);
         if (e != hipSuccess) return bad(e, "hipHostMalloc");
@@ -539,7 +542,7 @@ int cuvhs_destroy(cuvhs_ctx* c) {
     for (Slot& s : c->slot) {
         for (hipEvent_t e : s.mark)
             if (e) (void)hipEventDestroy(e);
-        for (hipEvent_t e : {s.head_done, s.tail_done, s.pre0, s.pre1, s.tail0})
+        for (hipEvent_t e : {s.head_done, s.tail_done, s.pre0, s.pre1})
             if (e) (void)hipEventDestroy(e);
         if (s.hctl) (void)hipHostFree(s.hctl);
     }
@@ -692,6 +695,17 @@ int cuvhs_profile(cuvhs_ctx* c, int enable) {
     c->prof = enable != 0;
     c->serial = enable == 2;
     for (double& v : c->stage_ms) v = 0.0;
+    c->work_samples = 0;
+    c->work_chunks = 0;
+    c->work_fields = 0;
+    return 0;
+}
+
+int cuvhs_profile_work(const cuvhs_ctx* c, uint64_t* samples, int* chunks, int* fields) {
+    if (!c) return CUVHS_E_ARG;
+    if (samples) *samples = c->work_samples;
+    if (chunks) *chunks = c->work_chunks;
+    if (fields) *fields = c->work_fields;
     return 0;
 }
 
@@ -788,10 +802,11 @@ int cuvhs_tbc_resample(cuvhs_ctx* c, const double* demod, int total, const doubl
 int cuvhs_chroma(cuvhs_ctx* c, const uint8_t* raw, uint64_t n_valid, const double* ll, int nf, uint16_t* out,
                  int* h_phase) {
     if (!c || !raw || !ll || !out || nf < 1 || nf > c->F) return CUVHS_E_ARG;
-    int r = prebpf_launch(c, raw, (long long)n_valid, nf, c->stream);
+    Slot& s = c->slot[0];
+    int r = prebpf_launch(c, raw, (long long)n_valid, nf, s.craw, c->stream);
     if (r) return r;
     HIPCHK(c, hipMemcpyAsync(c->ll, ll, (size_t)nf * kLinesPerFrame * 8, hipMemcpyDeviceToDevice, c->stream));
-    ChromaLaunch a = chroma_launch(c, c->slot[0].ctl, nf);
+    ChromaLaunch a = chroma_launch(c, s.ctl, nf, s.craw);
     a.out = out;
     HIPCHK(c, run_chroma(a, c->state, nf, c->m0, c->m1, c->cs_tab, c->iq0, c->track, c->phase, 1, c->stream));
     if (h_phase) HIPCHK(c, hipMemcpyAsync(h_phase, c->phase, (size_t)nf * 4, hipMemcpyDeviceToHost, c->stream));

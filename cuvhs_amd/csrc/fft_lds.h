@@ -25,7 +25,10 @@ template <typename T> struct C2;
 template <> struct C2<double> { using type = double2; };
 template <> struct C2<float> { using type = float2; };
 
+// Complex product, contracted to FMAs: the transforms are compared with the
+// oracle by tolerance, not bit for bit, and FMAs cut the twiddle cost.
 template <typename V> __device__ __forceinline__ V cmul(V a, V b) {
+#pragma clang fp contract(fast)
     V r;
     r.x = a.x * b.x - a.y * b.y;
     r.y = a.x * b.y + a.y * b.x;
@@ -137,6 +140,17 @@ __device__ __forceinline__ void fft_stage(typename C2<T>::type* buf, const typen
 #pragma unroll
         for (int r = 0; r < R; r++) v[p * R + r] = buf[pidx(j + r * NB)];
     }
+    // spectrum bins of this thread's outputs, in flight across the barrier
+    V g[16];
+    if (G) {
+#pragma unroll
+        for (int p = 0; p < PER; p++) {
+            int j = t + p * NT;
+            int idx = (j / ns) * ns * R + j % ns;
+#pragma unroll
+            for (int r = 0; r < R; r++) g[p * R + r] = G[idx + r * ns];
+        }
+    }
     __syncthreads();
 #pragma unroll
     for (int p = 0; p < PER; p++) {
@@ -161,14 +175,6 @@ __device__ __forceinline__ void fft_stage(typename C2<T>::type* buf, const typen
         else if (R == 2) dft2<S>(u[0], u[1]);
     }
     if (G) {
-        V g[16];
-#pragma unroll
-        for (int p = 0; p < PER; p++) {
-            int j = t + p * NT;
-            int idx = (j / ns) * ns * R + j % ns;
-#pragma unroll
-            for (int r = 0; r < R; r++) g[p * R + r] = G[idx + r * ns];
-        }
 #pragma unroll
         for (int i = 0; i < 16; i++) v[i] = cmul(v[i], g[i]);
     }

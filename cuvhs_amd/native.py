@@ -78,6 +78,7 @@ _SIGS = {
     "cuvhs_last_error": (ctypes.c_char_p, [_vp]),
     "cuvhs_profile": (_i, [_vp, _i]),
     "cuvhs_profile_read": (_i, [_vp, _vp, _vp, _i]),
+    "cuvhs_profile_work": (_i, [_vp, ctypes.POINTER(_u64), ctypes.POINTER(_i), ctypes.POINTER(_i)]),
 }
 
 _lib = None
@@ -203,6 +204,13 @@ class Decoder:
         names = (ctypes.c_char_p * n)()
         k = lib().cuvhs_profile_read(self.h, ms, names, n)
         return {names[i].decode(): ms[i] for i in range(k)}
+
+    def profile_work(self):
+        """(K1 samples demodulated, chunks, fields) since profiling was enabled."""
+        smp, ch, fl = _u64(), _i(), _i()
+        check(lib().cuvhs_profile_work(self.h, ctypes.byref(smp), ctypes.byref(ch), ctypes.byref(fl)), self.h,
+              "profile_work")
+        return smp.value, ch.value, fl.value
 
     def reset(self):
         check(lib().cuvhs_reset(self.h), self.h, "reset")

@@ -177,6 +177,10 @@ int cuvhs_selftest_fft(int B, int is_f32, const void* d_in, void* d_out, int sig
  * the whole chunk loop: pipeline.cu:2642-2708.) */
 int cuvhs_profile(cuvhs_ctx* ctx, int enable);
 int cuvhs_profile_read(const cuvhs_ctx* ctx, double* ms, const char** names, int n);
+/* Work done by the chunk decodes counted in the same window: capture samples
+ * demodulated by K1 (fields_k1 * spfp per chunk), chunks, and fields decoded.
+ * Any pointer may be NULL.  (No counterpart in the reference.) */
+int cuvhs_profile_work(const cuvhs_ctx* ctx, uint64_t* samples, int* chunks, int* fields);
 
 /* Last HIP error text for the context (or a static string). */
 const char* cuvhs_last_error(const cuvhs_ctx* ctx);

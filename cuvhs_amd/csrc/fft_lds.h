@@ -140,17 +140,6 @@ __device__ __forceinline__ void fft_stage(typename C2<T>::type* buf, const typen
 #pragma unroll
         for (int r = 0; r < R; r++) v[p * R + r] = buf[pidx(j + r * NB)];
     }
-    // spectrum bins of this thread's outputs, in flight across the barrier
-    V g[16];
-    if (G) {
-#pragma unroll
-        for (int p = 0; p < PER; p++) {
-            int j = t + p * NT;
-            int idx = (j / ns) * ns * R + j % ns;
-#pragma unroll
-            for (int r = 0; r < R; r++) g[p * R + r] = G[idx + r * ns];
-        }
-    }
     __syncthreads();
 #pragma unroll
     for (int p = 0; p < PER; p++) {
@@ -175,6 +164,14 @@ __device__ __forceinline__ void fft_stage(typename C2<T>::type* buf, const typen
         else if (R == 2) dft2<S>(u[0], u[1]);
     }
     if (G) {
+        V g[16];
+#pragma unroll
+        for (int p = 0; p < PER; p++) {
+            int j = t + p * NT;
+            int idx = (j / ns) * ns * R + j % ns;
+#pragma unroll
+            for (int r = 0; r < R; r++) g[p * R + r] = G[idx + r * ns];
+        }
 #pragma unroll
         for (int i = 0; i < 16; i++) v[i] = cmul(v[i], g[i]);
     }

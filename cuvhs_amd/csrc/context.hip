@@ -70,6 +70,7 @@ struct cuvhs_ctx {
     float2* tw_p = nullptr;
     double2* tw_1024 = nullptr;
     double* cs_tab = nullptr;       // cos/sin of the burst phase detector
+    double2* chroma_mask = nullptr; // per-line band-pass: (1, 0) kept bins, (0, 0) others
     long long buf_n = 0;            // F * spfp + one segment of zero padding
     Slot slot[2];
     double* rawdemod = nullptr;     // head only
@@ -163,6 +164,15 @@ void mark(cuvhs_ctx* c, Slot& s, int i, hipStream_t st) {
     if (c->prof) (void)hipEventRecord(s.mark[i], st);
 }
 
+// Per-line chroma band-pass bins (chroma_decode.cu:538-545).
+void chroma_band(const Fmt& f, int* lo, int* hi) {
+    int fsc_bin = (int)(f.fsc / f.output_rate * kChromaFFT + 0.5);
+    int bw = (int)(500000.0 / f.output_rate * kChromaFFT + 0.5);
+    if (bw < 10) bw = 10;
+    *lo = fsc_bin - bw;
+    *hi = fsc_bin + bw;
+}
+
 ChromaLaunch chroma_launch(cuvhs_ctx* c, const ChunkCtl* ctl, int nf_override, const float* craw) {
     ChromaLaunch a{};
     a.craw = craw;
@@ -174,11 +184,8 @@ ChromaLaunch chroma_launch(cuvhs_ctx* c, const ChunkCtl* ctl, int nf_override, c
     a.fft_scale = 1.0 / (double)kChromaFFT;
     a.burst_abs_ref = c->fmt.burst_abs_ref;
     a.burst_start = c->fmt.burst_start;
-    int fsc_bin = (int)(c->fmt.fsc / c->fmt.output_rate * kChromaFFT + 0.5);
-    int bw = (int)(500000.0 / c->fmt.output_rate * kChromaFFT + 0.5);
-    if (bw < 10) bw = 10;
-    a.lo = fsc_bin - bw;
-    a.hi = fsc_bin + bw;
+    chroma_band(c->fmt, &a.lo, &a.hi);
+    a.mask = c->chroma_mask;
     a.burst = c->burst;
     a.out = c->chroma;
     a.tw = c->tw_1024;
@@ -468,6 +475,17 @@ int cuvhs_create(const cuvhs_config* cfg, cuvhs_ctx** out) {
         }
         ALLOC(c->cs_tab, cs.size());
         if ((e = hipMemcpy(c->cs_tab, cs.data(), cs.size() * sizeof(double), hipMemcpyHostToDevice)) != hipSuccess) return bad(e, "burst table");
+        // band-pass bins kept by the per-line chroma filter (chroma_decode.cu:163-182)
+        int lo = 0, hi = 0;
+        chroma_band(f, &lo, &hi);
+        std::vector<double2> mk((size_t)kChromaFFT);
+        for (int k = 0; k < kChromaFFT; k++) {
+            int kk = k <= kChromaFFT / 2 ? k : kChromaFFT - k;
+            mk[(size_t)k].x = (kk >= lo && kk <= hi) ? 1.0 : 0.0;
+            mk[(size_t)k].y = 0.0;
+        }
+        ALLOC(c->chroma_mask, mk.size());
+        if ((e = hipMemcpy(c->chroma_mask, mk.data(), mk.size() * sizeof(double2), hipMemcpyHostToDevice)) != hipSuccess) return bad(e, "chroma mask");
     }
     const long long S = (long long)c->F * f.spfp;
     const size_t Fz = (size_t)c->F;

@@ -26,83 +26,103 @@ constexpr int kBurstLen = 36;      // 112 - 76 at 4 fsc
 constexpr int kSkip = 16;          // chroma_decode.cu:255
 constexpr int kPairs = (kFieldLines - 2 * kSkip - 1) / 2;  // 115
 
-// One wavefront per line: resample + heterodyne (chroma_decode.cu:75-157),
-// FFT, band-pass (:163-182), inverse FFT, then either burst capture or
-// ACC + u16 (:188-231).
+// Resampled, heterodyned sample of one line (chroma_decode.cu:75-157).
+__device__ __forceinline__ double chroma_het_sample(const ChromaLaunch& a, double ls, double le, int line, int col,
+                                                   double phase_offset, long long total) {
+    if (col >= kLineLen) return 0.0;
+    double frac = (double)col / (double)kLineLen;
+    double coord = ls + frac * (le - ls);
+    int ci = (int)coord;
+    double x = coord - (double)ci;
+    if (ci < 1 || (long long)ci + 2 >= total) return 0.0;
+    double p0 = a.craw[ci - 1], p1 = a.craw[ci], p2 = a.craw[ci + 1], p3 = a.craw[ci + 2];
+    double aa = p2 - p0;
+    double bb = 2.0 * p0 - 5.0 * p1 + 4.0 * p2 - p3;
+    double cc = 3.0 * (p1 - p2) + p3 - p0;
+    double raw_val = p1 + 0.5 * x * (aa + x * (bb + x * cc));
+    double abs_sample = (double)(line * kLineLen + col);
+    double het = -cos(2.0 * M_PI * a.het_scale * abs_sample + phase_offset);
+    return raw_val * het;
+}
+
+// One wavefront per pair of lines (la, la + 1): resample + heterodyne
+// (chroma_decode.cu:75-157), 1024-point FFT, band-pass (:163-182), inverse
+// FFT, then either burst capture or ACC + u16 (:188-231).  The two real
+// lines go in as the real and imaginary parts of one complex transform; the
+// band-pass mask is symmetric in frequency, so the band-passed line la comes
+// back as the real part and la + 1 as the imaginary part.  (The reference
+// transforms each line on its own; the results agree to rounding.)  Thread t
+// computes columns t + 64 r, which is the register layout the first FFT
+// stage takes, and the mask is applied in the last forward stage.
 template <int OUT>
 __global__ __launch_bounds__(64) void k_chroma_line(ChromaLaunch a) {
     __shared__ __align__(16) double2 buf[lds_elems(kChromaFFT)];
-    __shared__ double s_acc;
+    __shared__ double s_acc[2];
     const int f = blockIdx.y;
     const int nf = a.nf_override > 0 ? a.nf_override : a.ctl->nf;
     if (f >= nf) return;
-    const int line = a.line0 + blockIdx.x;
+    const int la = a.line0 + 2 * (int)blockIdx.x;
+    if (la >= a.line_end) return;
+    const int lb = la + 1;
+    const bool has_b = lb < a.line_end;
     const long long total = (long long)nf * a.spfp;
     const double* ll = a.linelocs + (size_t)f * kLinesPerFrame;
-    const double ls = ll[line], le = ll[line + 1];
     const int tr = a.track ? a.track[f] : 0;
     const int rot = tr ? 1 : 3;
-    const int line_phase = (line * rot + 0) & 3;
-    const double phase_offset = (double)line_phase * (M_PI * 0.5);
-    for (int col = threadIdx.x; col < kChromaFFT; col += 64) {
-        double v = 0.0;
-        if (col < kLineLen) {
-            double frac = (double)col / (double)kLineLen;
-            double coord = ls + frac * (le - ls);
-            int ci = (int)coord;
-            double x = coord - (double)ci;
-            if (!(ci < 1 || (long long)ci + 2 >= total)) {
-                double p0 = a.craw[ci - 1], p1 = a.craw[ci], p2 = a.craw[ci + 1], p3 = a.craw[ci + 2];
-                double aa = p2 - p0;
-                double bb = 2.0 * p0 - 5.0 * p1 + 4.0 * p2 - p3;
-                double cc = 3.0 * (p1 - p2) + p3 - p0;
-                double raw_val = p1 + 0.5 * x * (aa + x * (bb + x * cc));
-                double abs_sample = (double)(line * kLineLen + col);
-                double het = -cos(2.0 * M_PI * a.het_scale * abs_sample + phase_offset);
-                v = raw_val * het;
-            }
-        }
-        double2 z;
-        z.x = v;
-        z.y = 0.0;
-        buf[pidx(col)] = z;
+    const double ls_a = ll[la], le_a = ll[la + 1];
+    const double ph_a = (double)((la * rot + 0) & 3) * (M_PI * 0.5);
+    double ls_b = 0.0, le_b = 0.0, ph_b = 0.0;
+    if (has_b) {
+        ls_b = ll[lb];
+        le_b = ll[lb + 1];
+        ph_b = (double)((lb * rot + 0) & 3) * (M_PI * 0.5);
     }
-    __syncthreads();
-    fft_lds<-1, double, kChromaFFT>(buf, a.tw);
-    for (int k = threadIdx.x; k < kChromaFFT; k += 64) {
-        int kk = k <= kChromaFFT / 2 ? k : kChromaFFT - k;
-        if (kk < a.lo || kk > a.hi) {
-            double2 z;
-            z.x = 0.0;
-            z.y = 0.0;
-            buf[pidx(k)] = z;
-        }
+    double2 v[16];
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        const int col = (int)threadIdx.x + 64 * r;
+        v[r].x = chroma_het_sample(a, ls_a, le_a, la, col, ph_a, total);
+        v[r].y = has_b ? chroma_het_sample(a, ls_b, le_b, lb, col, ph_b, total) : 0.0;
     }
-    __syncthreads();
+    fft_fwd_regs_mul<double, kChromaFFT>(buf, v, a.tw, a.mask);
     fft_lds<+1, double, kChromaFFT>(buf, a.tw);
     if (!OUT) {
-        double* dst = a.burst + ((size_t)f * kFieldLines + line) * kBurstLen;
-        for (int i = threadIdx.x; i < kBurstLen; i += 64) dst[i] = buf[pidx(a.burst_start + i)].x * a.fft_scale;
+        double* da = a.burst + ((size_t)f * kFieldLines + la) * kBurstLen;
+        for (int i = threadIdx.x; i < kBurstLen; i += 64) {
+            double2 z = buf[pidx(a.burst_start + i)];
+            da[i] = z.x * a.fft_scale;
+            if (has_b) da[kBurstLen + i] = z.y * a.fft_scale;
+        }
         return;
     }
-    if (threadIdx.x == 0) {
+    if (threadIdx.x < 2 && (threadIdx.x == 0 || has_b)) {
         double ssq = 0.0;
         for (int i = 0; i < kBurstLen; i++) {
-            double v = buf[pidx(a.burst_start + i)].x * a.fft_scale;
-            ssq += v * v;
+            double2 z = buf[pidx(a.burst_start + i)];
+            double v1 = (threadIdx.x == 0 ? z.x : z.y) * a.fft_scale;
+            ssq += v1 * v1;
         }
         double rms = sqrt(ssq / (double)kBurstLen);
-        s_acc = (rms > 1.0) ? (a.burst_abs_ref / rms) : 0.0;
+        s_acc[threadIdx.x] = (rms > 1.0) ? (a.burst_abs_ref / rms) : 0.0;
     }
     __syncthreads();
-    const double acc = s_acc;
-    uint16_t* o = a.out + ((size_t)f * kFieldLines + line) * kLineLen;
+    const double acc_a = s_acc[0], acc_b = s_acc[1];
+    uint16_t* oa = a.out + ((size_t)f * kFieldLines + la) * kLineLen;
+    uint16_t* ob = oa + kLineLen;
     for (int col = threadIdx.x; col < kLineLen; col += 64) {
-        double v = buf[pidx(col)].x * a.fft_scale * acc;
-        double ov = v + 32768.0;
+        double2 z = buf[pidx(col)];
+        double va = z.x * a.fft_scale * acc_a;
+        double ov = va + 32768.0;
         if (ov < 0.0) ov = 0.0;
         if (ov > 65535.0) ov = 65535.0;
-        o[col] = (uint16_t)(ov + 0.5);
+        oa[col] = (uint16_t)(ov + 0.5);
+        if (has_b) {
+            double vb = z.y * a.fft_scale * acc_b;
+            double ob_ = vb + 32768.0;
+            if (ob_ < 0.0) ob_ = 0.0;
+            if (ob_ > 65535.0) ob_ = 65535.0;
+            ob[col] = (uint16_t)(ob_ + 0.5);
+        }
     }
 }
 
@@ -242,7 +262,8 @@ hipError_t run_chroma(ChromaLaunch a, DecodeState* dst, int max_fields, double* 
     ChromaLaunch m = a;
     m.track = nullptr;
     m.line0 = kSkip;
-    hipLaunchKernelGGL(k_chroma_line<0>, dim3(kFieldLines - 2 * kSkip, max_fields), dim3(64), 0, st, m);
+    m.line_end = kFieldLines - kSkip;
+    hipLaunchKernelGGL(k_chroma_line<0>, dim3((m.line_end - m.line0 + 1) / 2, max_fields), dim3(64), 0, st, m);
     hipLaunchKernelGGL(k_chroma_metrics, dim3(max_fields), dim3(64), 0, st, a.burst, a.ctl,
                        a.nf_override, m0, m1, cs_tab, need_iq ? iq0 : nullptr);
     hipLaunchKernelGGL(k_chroma_resolve, dim3(1), dim3(64), 0, st, a.ctl, dst, a.nf_override, m0, m1, iq0, out_track,
@@ -250,7 +271,8 @@ hipError_t run_chroma(ChromaLaunch a, DecodeState* dst, int max_fields, double* 
     ChromaLaunch o = a;
     o.track = out_track;
     o.line0 = 0;
-    hipLaunchKernelGGL(k_chroma_line<1>, dim3(kFieldLines, max_fields), dim3(64), 0, st, o);
+    o.line_end = kFieldLines;
+    hipLaunchKernelGGL(k_chroma_line<1>, dim3((o.line_end - o.line0 + 1) / 2, max_fields), dim3(64), 0, st, o);
     return hipGetLastError();
 }
 

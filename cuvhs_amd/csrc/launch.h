@@ -46,10 +46,11 @@ struct ChromaLaunch {
     double het_scale, fft_scale, burst_abs_ref;
     int burst_start, lo, hi;
     const int* track;
-    int line0;
+    int line0, line_end;        // lines [line0, line_end) of each field
     double* burst;
     uint16_t* out;
     const double2* tw;
+    const double2* mask;        // band-pass bins: (1, 0) kept, (0, 0) removed
 };
 
 

@@ -165,7 +165,11 @@ __global__ __launch_bounds__(B / 16) void k_prebpf(PreLaunch a) {
         v[i].x = xin[i];
         v[i].y = 0.0f;
     }
-    fft_fwd_regs_mul<float, B>(buf, v, a.tw, a.G);
+    // the spectrum product stays a separate pass here: fused into the last
+    // (radix-4) stage it needs more than the 128 VGPRs a 1024-thread
+    // workgroup allows and spills
+    fft_fwd_regs_mul<float, B>(buf, v, a.tw, nullptr);
+    spectrum_multiply<float, B>(buf, a.G);
     fft_lds<+1, float, B>(buf, a.tw);
     for (int o = threadIdx.x; o < a.V; o += NT) {
         int n = ns + o;

@@ -144,6 +144,21 @@ def generate(config=0, **overrides):
     return out
 
 
+_hip = None
+
+
+def _hip_d2d(dst, src, nbytes):
+    """Synchronous device-to-device copy through the HIP runtime."""
+    global _hip
+    if _hip is None:
+        _hip = ctypes.CDLL("libamdhip64.so")
+        _hip.hipMemcpy.argtypes = [_vp, _vp, ctypes.c_size_t, _i]
+        _hip.hipMemcpy.restype = _i
+    rc = _hip.hipMemcpy(ctypes.c_void_p(dst), ctypes.c_void_p(src), nbytes, 3)  # hipMemcpyDeviceToDevice
+    if rc != 0:
+        raise CuvhsError(f"hipMemcpy failed ({rc})")
+
+
 def format_for(mhz):
     f = Format()
     rc = lib().cuvhs_format_for(float(mhz), ctypes.byref(f))
@@ -211,6 +226,49 @@ class Decoder:
         check(lib().cuvhs_profile_work(self.h, ctypes.byref(smp), ctypes.byref(ch), ctypes.byref(fl)), self.h,
               "profile_work")
         return smp.value, ch.value, fl.value
+
+    def decode_chunked(self, d_capture):
+        """Decode a device-resident u8 capture chunk by chunk through
+        cuvhs_decode_chunk (the loop of Pipeline::run, pipeline.cu:2626-2713,
+        as the CLI runs it), gathering each chunk's outputs.  Returns the same
+        dict as decode_capture plus the per-chunk info records."""
+        import torch
+        n = d_capture.numel()
+        spf = self.fmt.samples_per_field
+        keys = ("luma", "chroma", "file_loc", "is_first", "phase_id", "do_count", "do_line", "do_startx", "do_endx")
+        parts = {k: [] for k in keys}
+        torch.cuda.synchronize(d_capture.device)
+        infos = []
+        off = 0
+        base = d_capture.data_ptr()
+        while off < n and (n - off) // spf > 0:
+            this_chunk = int(min(self.batch, (n - off) // spf))
+            info = ChunkInfo()
+            rc = lib().cuvhs_decode_chunk(self.h, ctypes.c_void_p(base + off), n - off, off, this_chunk,
+                                          ctypes.byref(info))
+            check(rc, self.h, "decode_chunk")
+            infos.append({"fields": info.fields, "fields_k1": info.fields_k1, "raw_offset": info.raw_offset,
+                          "next_raw_offset": info.next_raw_offset, "status": info.status,
+                          "vsync_candidates": info.vsync_candidates})
+            if info.fields == 0:
+                break
+            o = ChunkOutputs()
+            check(lib().cuvhs_chunk_outputs_get(self.h, ctypes.byref(o)), self.h, "chunk_outputs_get")
+            k = info.fields
+            shapes = {"luma": ((k, FIELD_LINES, LINE_LEN), torch.uint16), "chroma": ((k, FIELD_LINES, LINE_LEN), torch.uint16),
+                      "file_loc": ((k,), torch.uint64), "is_first": ((k,), torch.int32), "phase_id": ((k,), torch.int32),
+                      "do_count": ((k,), torch.int32), "do_line": ((k, MAX_DROPOUTS), torch.int32),
+                      "do_startx": ((k, MAX_DROPOUTS), torch.int32), "do_endx": ((k, MAX_DROPOUTS), torch.int32)}
+            for key in keys:
+                shape, dt = shapes[key]
+                t = torch.empty(shape, dtype=dt, device=d_capture.device)
+                _hip_d2d(t.data_ptr(), getattr(o, key), t.numel() * t.element_size())
+                parts[key].append(t)
+            off = info.next_raw_offset
+        out = {k: (torch.cat(v) if v else None) for k, v in parts.items()}
+        out["fields"] = sum(i["fields"] for i in infos)
+        out["chunks"] = infos
+        return out
 
     def reset(self):
         check(lib().cuvhs_reset(self.h), self.h, "reset")

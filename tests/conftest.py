@@ -32,6 +32,18 @@ def ref0(oracle, cap0):
     return oracle.decode(cap0, 28.0, 64, trace=True)
 
 
+@pytest.fixture(scope="session")
+def cap1_prefix():
+    """First 140 fields of the 600-field 28 MHz workload."""
+    from cuvhs_amd import native
+    return native.generate(1, n_samples=140 * 468140)
+
+
+@pytest.fixture(scope="session")
+def ref1_prefix(oracle, cap1_prefix):
+    return oracle.decode(cap1_prefix, 28.0, 64, trace=True)
+
+
 def s16(cap, n):
     """RawReader conversion of a u8 chunk, zero-filled to n samples."""
     raw = np.zeros(n)

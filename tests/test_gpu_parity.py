@@ -226,16 +226,6 @@ def test_decode_capture0(torch, N, cap0, ref0):
     dec.close()
 
 
-@pytest.fixture(scope="module")
-def cap1_prefix(N):
-    return N.generate(1, n_samples=140 * SPF)
-
-
-@pytest.fixture(scope="module")
-def ref1_prefix(oracle, cap1_prefix):
-    return oracle.decode(cap1_prefix, MHZ, 64, trace=True)
-
-
 def test_decode_capture1_prefix(torch, N, cap1_prefix, ref1_prefix):
     """140 fields of the 600-field workload: three chunks (64, 64, 12), SNR
     25-40 dB, dropouts and wow/flutter; carried K2b/chroma/parity state."""

@@ -1,0 +1,149 @@
+"""GPU decodes of the benchmark workloads beyond the 28 MHz prefix: the
+40 MHz head-switch capture, another chunk size, and the full 600-field
+capture through size-independent properties (the oracle needs about a
+minute of CPU for 600 fields, so at full size the GPU result is tied to the
+oracle through its first two chunks instead)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch as t
+    assert t.cuda.is_available()
+    return t
+
+
+@pytest.fixture(scope="module")
+def N():
+    from cuvhs_amd import native
+    native.lib()
+    return native
+
+
+def dev(torch, a):
+    return torch.from_numpy(np.ascontiguousarray(a)).to("cuda:0")
+
+
+def compare(out, ref):
+    n = out["fields"]
+    assert n == ref["fields"]
+    for k in ("file_loc", "is_first", "phase_id", "do_count"):
+        assert np.array_equal(out[k].cpu().numpy().astype(np.int64), np.asarray(ref[k]).astype(np.int64)), k
+    dc = ref["do_count"]
+    for k in ("do_line", "do_startx", "do_endx"):
+        g = out[k].cpu().numpy()
+        for i in range(n):
+            assert np.array_equal(g[i, :dc[i]], ref[k][i, :dc[i]]), (k, i)
+    for k in ("luma", "chroma"):
+        d = np.abs(out[k].cpu().numpy().astype(np.int64) - ref[k].astype(np.int64))
+        assert int(d.max()) <= 1, (k, int(d.max()), int((d > 1).sum()))
+
+
+def test_decode_40mhz_headswitch_prefix(torch, N, oracle):
+    """130 fields of the 6000-field 40 MHz capture (head-switch offsets of
+    0-4 us in field groups, SNR 25-40 dB, dropouts, wow/flutter): two chunks
+    of 64 plus a short one, 8192-point video blocks."""
+    fmt = N.format_for(40.0)
+    cap = N.generate(2, n_samples=130 * fmt.samples_per_field)
+    ref = oracle.decode(cap, 40.0, 64, trace=True)
+    assert ref["fields"] >= 128
+    assert ref["do_count"].sum() > 0
+    dec = N.Decoder(40.0, 64)
+    out = dec.decode_capture(dev(torch, cap))
+    compare(out, ref)
+    dec.close()
+
+
+def test_decode_batch16_chunk_driver(torch, N, oracle):
+    """F = 16 on 70 fields of the 28 MHz capture: five chunks, decoded with
+    the chunk-by-chunk entry point (the CLI's loop) and with the capture
+    entry point; both equal the oracle's decode with the same F."""
+    fmt = N.format_for(28.0)
+    cap = N.generate(1, n_samples=70 * fmt.samples_per_field)
+    ref = oracle.decode(cap, 28.0, 16, trace=True)
+    assert len(ref["chunks"]) >= 4
+    dec = N.Decoder(28.0, 16)
+    d = dev(torch, cap)
+    out = dec.decode_chunked(d)
+    assert [c["fields"] for c in out["chunks"] if c["fields"]] == [c["nf"] for c in ref["chunks"]]
+    assert [c["raw_offset"] for c in out["chunks"] if c["fields"]] == [int(c["raw_offset"]) for c in ref["chunks"]]
+    compare(out, ref)
+    dec.reset()
+    out2 = dec.decode_capture(d)
+    for k in ("luma", "chroma", "file_loc", "is_first", "phase_id", "do_count"):
+        assert torch.equal(out[k], out2[k]), k
+    dec.close()
+
+
+@pytest.fixture(scope="module")
+def full(torch, N):
+    cap = N.generate(1)
+    dec = N.Decoder(28.0, 64)
+    out = dec.decode_capture(dev(torch, cap))
+    yield cap, dec, out
+    dec.close()
+
+
+def test_full_600_field_properties(torch, N, full):
+    """The bench workload at full size: field schedule, metadata sequences
+    and output ranges that hold for any correct decode."""
+    cap, dec, out = full
+    spf = dec.fmt.samples_per_field
+    n = out["fields"]
+    assert 598 <= n <= 602
+    fl = out["file_loc"].cpu().numpy().astype(np.int64)
+    gaps = np.diff(fl)
+    assert (gaps > 0.95 * spf).all() and (gaps < 1.05 * spf).all()   # one field per VSYNC
+    isf = out["is_first"].cpu().numpy()
+    assert isf[0] == 0 and (isf[1:] != isf[:-1]).all()               # strict alternation from false
+    cycle = [3, 2, 1, 4, 3, 2, 1, 4]
+    ph = out["phase_id"].cpu().numpy()
+    start = [i for i in range(8) if all(ph[j] == cycle[(i + j) % 8] for j in range(n))]
+    assert start, "phase IDs do not follow the 8-field NTSC cycle"
+    dc = out["do_count"].cpu().numpy()
+    assert dc.sum() > 0 and (dc >= 0).all() and (dc <= 512).all()
+    dl, ds, de = (out[k].cpu().numpy() for k in ("do_line", "do_startx", "do_endx"))
+    for i in range(n):
+        k = dc[i]
+        assert ((dl[i, :k] >= 0) & (dl[i, :k] < 263)).all()
+        assert ((ds[i, :k] >= 0) & (ds[i, :k] <= de[i, :k]) & (de[i, :k] <= 910)).all()
+    # the picture (ramps and bars between blanking and white) averages well
+    # inside the black (15360) to white (51200) range of the u16 scale
+    mean = float(out["luma"].to(torch.float32).mean())
+    assert 10000.0 < mean < 51200.0, mean
+
+
+def test_full_decode_deterministic(torch, N, full):
+    """A second decode of the full capture after reset is bit-identical."""
+    cap, dec, out = full
+    dec.reset()
+    out2 = dec.decode_capture(dev(torch, cap))
+    assert out2["fields"] == out["fields"]
+    for k in ("luma", "chroma", "file_loc", "is_first", "phase_id", "do_count"):
+        assert torch.equal(out[k], out2[k]), k
+    # dropout lists are defined up to do_count (the entries past it are the
+    # detector's scratch, as in the reference: dropout_detect.cu:118-119)
+    dc = out["do_count"].cpu().numpy()
+    for k in ("do_line", "do_startx", "do_endx"):
+        a, b = out[k].cpu().numpy(), out2[k].cpu().numpy()
+        for i in range(out["fields"]):
+            assert np.array_equal(a[i, :dc[i]], b[i, :dc[i]]), (k, i)
+
+
+def test_full_prefix_matches_oracle(torch, N, full, cap1_prefix, ref1_prefix):
+    """The first two chunks (128 fields) of the full decode equal the
+    oracle's decode of the 140-field prefix of the same capture: both chunks
+    read their whole 64 * spfp window in either case, so they are the same
+    computation."""
+    cap, dec, out = full
+    assert np.array_equal(cap[:cap1_prefix.size], cap1_prefix)
+    ref = ref1_prefix
+    assert [c["nf"] for c in ref["chunks"]][:2] == [64, 64]
+    cut = {k: (v[:128] if hasattr(v, "__getitem__") and k != "fields" else v) for k, v in out.items()}
+    cut["fields"] = 128
+    refc = {k: (v[:128] if isinstance(v, np.ndarray) else v) for k, v in ref.items()}
+    refc["fields"] = 128
+    compare(cut, refc)

@@ -43,10 +43,13 @@ struct Slot {
     ChunkCtl* ctl = nullptr;      // device
     ChunkCtl* hctl = nullptr;     // pinned host copy
     float* craw = nullptr;        // chroma pre-band-pass output
+    double* dc = nullptr;         // per-read mean (--dc-correct)
+    double* dc_partial = nullptr; // its partial sums
+    hipEvent_t dc_done = nullptr;
     hipEvent_t head_done = nullptr, tail_done = nullptr, pre0 = nullptr, pre1 = nullptr;
     hipEvent_t mark[kMarks] = {};
     // host view of the chunk
-    const uint8_t* chunk = nullptr;
+    const void* chunk = nullptr;
     uint64_t n_read = 0, raw_offset = 0;
     int max_fields = 0, fk1 = 0;
 };
@@ -149,6 +152,8 @@ hipError_t upload_spectrum(cuvhs_ctx* c, V** dst, const std::vector<cdbl>& s) {
     return hipMemcpy(*dst, h.data(), h.size() * sizeof(V), hipMemcpyHostToDevice);
 }
 
+int sample_bytes(const cuvhs_ctx* c) { return c->cfg.input_format == CUVHS_INPUT_U8 ? 1 : 2; }
+
 void reset_state(DecodeState* h) {
     memset(h, 0, sizeof(*h));
     h->k2b.prev_first_field = -1;
@@ -192,11 +197,13 @@ ChromaLaunch chroma_launch(cuvhs_ctx* c, const ChunkCtl* ctl, int nf_override, c
     return a;
 }
 
-int k1_launch(cuvhs_ctx* c, Slot& s, const uint8_t* chunk, long long n_valid, int nfk1, hipStream_t st) {
+int k1_launch(cuvhs_ctx* c, Slot& s, const void* chunk, long long n_valid, int nfk1, hipStream_t st) {
     const Fmt& f = c->fmt;
     long long T = (long long)nfk1 * f.spfp;
     K1ALaunch a{};
     a.cap = chunk;
+    a.in_fmt = c->cfg.input_format;
+    a.dc = c->cfg.dc_correct ? s.dc : nullptr;
     a.n_valid = n_valid;
     a.total = T;
     a.spfp = f.spfp;
@@ -231,9 +238,12 @@ int k1_launch(cuvhs_ctx* c, Slot& s, const uint8_t* chunk, long long n_valid, in
     return 0;
 }
 
-int prebpf_launch(cuvhs_ctx* c, const uint8_t* chunk, long long n_valid, int nf_max, float* out, hipStream_t st) {
+int prebpf_launch(cuvhs_ctx* c, const void* chunk, long long n_valid, int nf_max, float* out, const double* dc,
+                  hipStream_t st) {
     PreLaunch p{};
     p.cap = chunk;
+    p.in_fmt = c->cfg.input_format;
+    p.dc = c->cfg.dc_correct ? dc : nullptr;
     p.n_valid = n_valid;
     p.spfp = c->fmt.spfp;
     p.N2 = c->pre.N2;
@@ -249,7 +259,7 @@ int prebpf_launch(cuvhs_ctx* c, const uint8_t* chunk, long long n_valid, int nf_
 
 // RawReader::read_at + the chunk sizing of Pipeline::run / process_chunk
 // (pipeline.cu:1530-1548, 2651-2658).  Returns false when no chunk starts here.
-bool plan_chunk(cuvhs_ctx* c, Slot& s, const uint8_t* chunk, uint64_t avail, uint64_t raw_offset, int max_fields) {
+bool plan_chunk(cuvhs_ctx* c, Slot& s, const void* chunk, uint64_t avail, uint64_t raw_offset, int max_fields) {
     const uint64_t spf = (uint64_t)c->fmt.samples_per_field;
     uint64_t n_read = std::min<uint64_t>((uint64_t)max_fields * (uint64_t)c->fmt.spfp, avail);
     if (n_read < spf) return false;
@@ -268,8 +278,15 @@ int head_stage(cuvhs_ctx* c, Slot& s) {
     const Fmt& f = c->fmt;
     hipStream_t st = c->stream;
     hipStream_t side = c->serial ? c->stream : c->side;
+    if (c->cfg.dc_correct) {
+        // per-read mean of the chunk's samples, needed by both K1 and the
+        // pre-band-pass (raw_reader.cpp:93-100)
+        HIPCHK(c, run_input_mean(s.chunk, c->cfg.input_format, (long long)s.n_read, s.dc_partial, s.dc, st));
+        HIPCHK(c, hipEventRecord(s.dc_done, st));
+        HIPCHK(c, hipStreamWaitEvent(side, s.dc_done, 0));
+    }
     HIPCHK(c, hipEventRecord(s.pre0, side));
-    int r = prebpf_launch(c, s.chunk, (long long)s.n_read, s.max_fields, s.craw, side);
+    int r = prebpf_launch(c, s.chunk, (long long)s.n_read, s.max_fields, s.craw, s.dc, side);
     if (r) return r;
     HIPCHK(c, hipEventRecord(s.pre1, side));
     ChunkCtl* h = s.hctl;
@@ -405,6 +422,7 @@ int cuvhs_create(const cuvhs_config* cfg, cuvhs_ctx** out) {
     *out = nullptr;
     if (cfg->system != CUVHS_SYSTEM_NTSC || cfg->precision != CUVHS_PRECISION_FP64) return CUVHS_E_UNSUPPORTED;
     if (!(cfg->sample_rate_mhz >= 16.0 && cfg->sample_rate_mhz <= 40.0)) return CUVHS_E_UNSUPPORTED;
+    if (cfg->input_format < CUVHS_INPUT_U8 || cfg->input_format > CUVHS_INPUT_U16) return CUVHS_E_UNSUPPORTED;
     if (cfg->batch_fields < 1 || cfg->batch_fields > 4096) return CUVHS_E_ARG;
     cuvhs_ctx* c = new cuvhs_ctx();
     c->cfg = *cfg;
@@ -427,7 +445,7 @@ int cuvhs_create(const cuvhs_config* cfg, cuvhs_ctx** out) {
     e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking);
     if (e != hipSuccess) return bad(e, "hipStreamCreate side");
     for (Slot& s : c->slot) {
-        for (hipEvent_t* ev : {&s.head_done, &s.tail_done}) {
+        for (hipEvent_t* ev : {&s.head_done, &s.tail_done, &s.dc_done}) {
             e = hipEventCreateWithFlags(ev, hipEventDisableTiming);
             if (e != hipSuccess) return bad(e, "hipEventCreate");
         }
@@ -501,6 +519,8 @@ int cuvhs_create(const cuvhs_config* cfg, cuvhs_ctx** out) {
         ALLOC(s.wg_off, events_nwg(S) + 1);
         ALLOC(s.offsets, Fz);
         ALLOC(s.craw, S);
+        ALLOC(s.dc, 1);
+        ALLOC(s.dc_partial, 256);
         ALLOC(s.ctl, 1);
         e = hipHostMalloc((void**)&s.hctl, sizeof(ChunkCtl), hipHostMallocDefault);
         if (e != hipSuccess) return bad(e, "hipHostMalloc");
@@ -560,7 +580,7 @@ int cuvhs_destroy(cuvhs_ctx* c) {
     for (Slot& s : c->slot) {
         for (hipEvent_t e : s.mark)
             if (e) (void)hipEventDestroy(e);
-        for (hipEvent_t e : {s.head_done, s.tail_done, s.pre0, s.pre1})
+        for (hipEvent_t e : {s.head_done, s.tail_done, s.pre0, s.pre1, s.dc_done})
             if (e) (void)hipEventDestroy(e);
         if (s.hctl) (void)hipHostFree(s.hctl);
     }
@@ -590,7 +610,7 @@ int cuvhs_reset(cuvhs_ctx* c) {
 
 const char* cuvhs_last_error(const cuvhs_ctx* c) { return c ? c->err.c_str() : "no context"; }
 
-int cuvhs_decode_chunk(cuvhs_ctx* c, const uint8_t* chunk, uint64_t avail, uint64_t raw_offset, int max_fields,
+int cuvhs_decode_chunk(cuvhs_ctx* c, const void* chunk, uint64_t avail, uint64_t raw_offset, int max_fields,
                        cuvhs_chunk_info* info) {
     if (!c || !chunk || !info || max_fields < 1 || max_fields > c->F) return CUVHS_E_ARG;
     memset(info, 0, sizeof(*info));
@@ -632,7 +652,7 @@ int cuvhs_chunk_outputs_get(const cuvhs_ctx* c, cuvhs_chunk_outputs* o) {
     return 0;
 }
 
-int cuvhs_decode_capture(cuvhs_ctx* c, const uint8_t* cap, uint64_t cap_len, int max_out, uint16_t* d_luma,
+int cuvhs_decode_capture(cuvhs_ctx* c, const void* cap, uint64_t cap_len, int max_out, uint16_t* d_luma,
                          uint16_t* d_chroma, uint64_t* d_file_loc, int32_t* d_is_first, int32_t* d_phase_id,
                          int32_t* d_do_count, int32_t* d_do_line, int32_t* d_do_startx, int32_t* d_do_endx,
                          int* fields) {
@@ -646,7 +666,8 @@ int cuvhs_decode_capture(cuvhs_ctx* c, const uint8_t* cap, uint64_t cap_len, int
         uint64_t remaining = (cap_len - off) / spf;
         if (remaining == 0) return false;
         int this_chunk = (int)std::min<uint64_t>((uint64_t)c->F, remaining);
-        return plan_chunk(c, s, cap + off, cap_len - off, off, this_chunk);
+        return plan_chunk(c, s, static_cast<const uint8_t*>(cap) + off * sample_bytes(c), cap_len - off, off,
+                          this_chunk);
     };
     if (!plan(c->slot[0], 0)) return 0;
     int rc = head_stage(c, c->slot[0]);
@@ -738,10 +759,12 @@ int cuvhs_profile_read(const cuvhs_ctx* c, double* ms, const char** names, int n
 
 // ---- stage entry points (synchronous, on the context's stream, slot 0) ----
 
-int cuvhs_fm_demod(cuvhs_ctx* c, const uint8_t* raw, uint64_t n_valid, int nf, double* demod, double* d05,
+int cuvhs_fm_demod(cuvhs_ctx* c, const void* raw, uint64_t n_valid, int nf, double* demod, double* d05,
                    double* env) {
     if (!c || !raw || nf < 1 || nf > c->F || !demod || !d05 || !env) return CUVHS_E_ARG;
     Slot& s = c->slot[0];
+    if (c->cfg.dc_correct && n_valid > 0)
+        HIPCHK(c, run_input_mean(raw, c->cfg.input_format, (long long)n_valid, s.dc_partial, s.dc, c->stream));
     const bool prof = c->prof;
     c->prof = false;
     int r = k1_launch(c, s, raw, (long long)n_valid, nf, c->stream);
@@ -817,11 +840,13 @@ int cuvhs_tbc_resample(cuvhs_ctx* c, const double* demod, int total, const doubl
     return 0;
 }
 
-int cuvhs_chroma(cuvhs_ctx* c, const uint8_t* raw, uint64_t n_valid, const double* ll, int nf, uint16_t* out,
+int cuvhs_chroma(cuvhs_ctx* c, const void* raw, uint64_t n_valid, const double* ll, int nf, uint16_t* out,
                  int* h_phase) {
     if (!c || !raw || !ll || !out || nf < 1 || nf > c->F) return CUVHS_E_ARG;
     Slot& s = c->slot[0];
-    int r = prebpf_launch(c, raw, (long long)n_valid, nf, s.craw, c->stream);
+    if (c->cfg.dc_correct && n_valid > 0)
+        HIPCHK(c, run_input_mean(raw, c->cfg.input_format, (long long)n_valid, s.dc_partial, s.dc, c->stream));
+    int r = prebpf_launch(c, raw, (long long)n_valid, nf, s.craw, s.dc, c->stream);
     if (r) return r;
     HIPCHK(c, hipMemcpyAsync(c->ll, ll, (size_t)nf * kLinesPerFrame * 8, hipMemcpyDeviceToDevice, c->stream));
     ChromaLaunch a = chroma_launch(c, s.ctl, nf, s.craw);

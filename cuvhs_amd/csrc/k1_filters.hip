@@ -22,10 +22,50 @@
 
 namespace cuvhs {
 
-__device__ __forceinline__ double u8_to_s16(const uint8_t* cap, long long s, long long n_valid) {
-    // RawReader::convert for u8 (raw_reader.cpp:74-77); zero past the read
-    // (pipeline.cu:1539-1541)
-    return s < n_valid ? ((double)cap[s] - 128.0) * 256.0 : 0.0;
+// RawReader::convert (raw_reader.cpp:71-91): u8 (s - 128) * 256, s16 as is,
+// u16 s - 32768.
+__device__ __forceinline__ double norm_sample(const void* cap, int fmt, long long s) {
+    if (fmt == 0) return ((double)static_cast<const uint8_t*>(cap)[s] - 128.0) * 256.0;
+    if (fmt == 1) return (double)static_cast<const int16_t*>(cap)[s];
+    return (double)static_cast<const uint16_t*>(cap)[s] - 32768.0;
+}
+
+// Sample s of the chunk as the reference's chunk buffer holds it: normalised,
+// minus the read's mean under --dc-correct (raw_reader.cpp:93-100), and zero
+// past the read (pipeline.cu:1539-1541).
+__device__ __forceinline__ double raw_sample(const void* cap, int fmt, long long s, long long n_valid, double dc) {
+    return s < n_valid ? norm_sample(cap, fmt, s) - dc : 0.0;
+}
+
+constexpr int kMeanBlocks = 256;
+
+// Per-read mean for --dc-correct: fixed slices, fixed-order tree sums (the
+// reference sums sequentially on the host; the orders differ at rounding).
+__global__ __launch_bounds__(256) void k_input_sum(const void* cap, int fmt, long long n, double* partial) {
+    __shared__ double red[256];
+    const long long per = (n + kMeanBlocks - 1) / kMeanBlocks;
+    const long long a = (long long)blockIdx.x * per;
+    const long long b = a + per < n ? a + per : n;
+    double s = 0.0;
+    for (long long i = a + threadIdx.x; i < b; i += 256) s += norm_sample(cap, fmt, i);
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(256) void k_input_mean(const double* partial, long long n, double* mean) {
+    __shared__ double red[kMeanBlocks];
+    red[threadIdx.x] = partial[threadIdx.x];
+    __syncthreads();
+    for (int o = kMeanBlocks / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *mean = red[0] / (double)n;
 }
 
 // q mod n for q in (-n, 2n): every window index of these kernels lies there
@@ -44,12 +84,13 @@ __global__ __launch_bounds__(B / 16) void k1_part_a(K1ALaunch a) {
     if (qs + 1 >= kOverlap + a.spfp) return;
     const int seg0 = f * a.spfp - kOverlap;
     const int last = (int)a.total - 1;
+    const double dcv = a.dc ? *a.dc : 0.0;
     double xin[16];
 #pragma unroll
     for (int i = 0; i < 16; i++) {
         int s = seg0 + wrap_n(qs - a.lp + (int)threadIdx.x + i * NT, a.N);
         s = s < 0 ? 0 : (s > last ? last : s);
-        xin[i] = u8_to_s16(a.cap, s, a.n_valid);
+        xin[i] = raw_sample(a.cap, a.in_fmt, s, a.n_valid, dcv);
     }
     double2 v[16];
 #pragma unroll
@@ -153,11 +194,12 @@ __global__ __launch_bounds__(B / 16) void k_prebpf(PreLaunch a) {
     const int ns = b * a.V;
     if (ns >= a.spfp) return;
     const int seg0 = f * a.spfp;
+    const double dcv = a.dc ? *a.dc : 0.0;
     float xin[16];
 #pragma unroll
     for (int i = 0; i < 16; i++) {
         int n = wrap_n(ns - a.lp + (int)threadIdx.x + i * NT, a.N2);
-        xin[i] = n < a.spfp ? (float)u8_to_s16(a.cap, seg0 + n, a.n_valid) : 0.0f;
+        xin[i] = n < a.spfp ? (float)raw_sample(a.cap, a.in_fmt, seg0 + n, a.n_valid, dcv) : 0.0f;
     }
     float2 v[16];
 #pragma unroll
@@ -224,6 +266,12 @@ hipError_t run_prebpf(int B, const PreLaunch& a, int nf, hipStream_t st) {
     if (B == 8192) return launch_pre<8192>(a, nf, st);
     if (B == 16384) return launch_pre<16384>(a, nf, st);
     return hipErrorInvalidValue;
+}
+hipError_t run_input_mean(const void* cap, int in_fmt, long long n, double* partial, double* mean, hipStream_t st) {
+    if (n <= 0) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_input_sum, dim3(kMeanBlocks), dim3(256), 0, st, cap, in_fmt, n, partial);
+    hipLaunchKernelGGL(k_input_mean, dim3(1), dim3(kMeanBlocks), 0, st, partial, n, mean);
+    return hipGetLastError();
 }
 hipError_t run_fill_f64(double* p, long long n, double v, hipStream_t st) {
     if (n <= 0) return hipSuccess;

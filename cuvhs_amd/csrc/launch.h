@@ -11,7 +11,9 @@ struct K2bField;
 struct K2bAnchorOut;
 
 struct K1ALaunch {
-    const uint8_t* cap;
+    const void* cap;            // capture samples (format in_fmt)
+    int in_fmt;                 // 0 u8, 1 s16, 2 u16
+    const double* dc;           // per-read mean to subtract, or null
     long long n_valid, total;
     int spfp, N, lm, lp, V;
     double scale;
@@ -30,7 +32,9 @@ struct K1BLaunch {
     double* demod05;
 };
 struct PreLaunch {
-    const uint8_t* cap;
+    const void* cap;
+    int in_fmt;
+    const double* dc;
     long long n_valid;
     int spfp, N2, lm, lp, V;
     const float2* G;
@@ -58,6 +62,7 @@ hipError_t run_k1_part_a(int B, const K1ALaunch& a, int nf, hipStream_t st);
 hipError_t run_k1_part_b(int B, const K1BLaunch& a, int nf, hipStream_t st);
 hipError_t run_prebpf(int B, const PreLaunch& a, int nf, hipStream_t st);
 hipError_t run_fill_f64(double* p, long long n, double v, hipStream_t st);
+hipError_t run_input_mean(const void* cap, int in_fmt, long long n, double* partial, double* mean, hipStream_t st);
 hipError_t run_fft_selftest(int B, int is_f32, const void* in, void* out, const void* tw, int sign, hipStream_t st);
 int events_nwg(long long total);
 hipError_t run_events(const double* x, long long total, double thr, int* wg_count, int* wg_off, int* events,

@@ -16,6 +16,8 @@ FIELD_LINES = 263
 LINE_LEN = 910
 
 E_ARG, E_HIP, E_ALLOC, E_UNSUPPORTED, E_CAPACITY = -1, -2, -3, -4, -5
+INPUT_U8, INPUT_S16, INPUT_U16 = 0, 1, 2
+INPUT_FORMATS = {"u8": INPUT_U8, "s16": INPUT_S16, "u16": INPUT_U16}
 W_VSYNC_OVERFLOW, W_NO_VSYNC, W_EOF = 1, 2, 4
 
 _vp = ctypes.c_void_p
@@ -30,7 +32,7 @@ class CuvhsError(RuntimeError):
 
 class Config(ctypes.Structure):
     _fields_ = [("system", _i), ("sample_rate_mhz", _d), ("batch_fields", _i), ("precision", _i),
-                ("stream", _vp), ("device", _i)]
+                ("stream", _vp), ("device", _i), ("input_format", _i), ("dc_correct", _i)]
 
 
 class Format(ctypes.Structure):
@@ -182,8 +184,10 @@ def ptr(t):
 class Decoder:
     """One decode context (Pipeline, pipeline.h:28-114) on a HIP device."""
 
-    def __init__(self, sample_rate_mhz=28.0, batch_fields=64, device=0):
-        cfg = Config(0, float(sample_rate_mhz), int(batch_fields), 0, None, int(device))
+    def __init__(self, sample_rate_mhz=28.0, batch_fields=64, device=0, input_format="u8", dc_correct=False):
+        self.input_format = INPUT_FORMATS[input_format]
+        cfg = Config(0, float(sample_rate_mhz), int(batch_fields), 0, None, int(device), self.input_format,
+                     1 if dc_correct else 0)
         h = _vp()
         rc = lib().cuvhs_create(ctypes.byref(cfg), ctypes.byref(h))
         if rc != 0:
@@ -233,7 +237,9 @@ class Decoder:
         as the CLI runs it), gathering each chunk's outputs.  Returns the same
         dict as decode_capture plus the per-chunk info records."""
         import torch
+        self._check_input(d_capture)
         n = d_capture.numel()
+        esz = d_capture.element_size()
         spf = self.fmt.samples_per_field
         keys = ("luma", "chroma", "file_loc", "is_first", "phase_id", "do_count", "do_line", "do_startx", "do_endx")
         parts = {k: [] for k in keys}
@@ -244,7 +250,7 @@ class Decoder:
         while off < n and (n - off) // spf > 0:
             this_chunk = int(min(self.batch, (n - off) // spf))
             info = ChunkInfo()
-            rc = lib().cuvhs_decode_chunk(self.h, ctypes.c_void_p(base + off), n - off, off, this_chunk,
+            rc = lib().cuvhs_decode_chunk(self.h, ctypes.c_void_p(base + off * esz), n - off, off, this_chunk,
                                           ctypes.byref(info))
             check(rc, self.h, "decode_chunk")
             infos.append({"fields": info.fields, "fields_k1": info.fields_k1, "raw_offset": info.raw_offset,
@@ -270,13 +276,21 @@ class Decoder:
         out["chunks"] = infos
         return out
 
+    def _check_input(self, d_capture):
+        import torch
+        want = {INPUT_U8: torch.uint8, INPUT_S16: torch.int16, INPUT_U16: torch.uint16}[self.input_format]
+        if d_capture.dtype != want:
+            raise CuvhsError(f"capture dtype {d_capture.dtype} does not match the decoder's input format ({want})")
+
     def reset(self):
         check(lib().cuvhs_reset(self.h), self.h, "reset")
 
     def decode_capture(self, d_capture, max_fields=None):
-        """Decode a device-resident u8 capture (torch uint8 tensor on the GPU).
-        Returns a dict of device tensors trimmed to the decoded field count."""
+        """Decode a device-resident capture (torch uint8 / int16 / uint16
+        tensor on the GPU, matching the decoder's input format).  Returns a
+        dict of device tensors trimmed to the decoded field count."""
         import torch
+        self._check_input(d_capture)
         n = d_capture.numel()
         if max_fields is None:
             max_fields = int(n // self.fmt.samples_per_field) + 2

@@ -51,6 +51,10 @@ enum {
 
 enum { CUVHS_SYSTEM_NTSC = 0 };
 enum { CUVHS_PRECISION_FP64 = 0 };
+/* Capture sample formats (InputFormat, video_format.h; RawReader::convert,
+ * raw_reader.cpp:71-91): u8 -> (s - 128) * 256, s16 as is, u16 -> s - 32768.
+ * 16-bit samples are in host byte order. */
+enum { CUVHS_INPUT_U8 = 0, CUVHS_INPUT_S16 = 1, CUVHS_INPUT_U16 = 2 };
 
 typedef struct cuvhs_ctx cuvhs_ctx;
 
@@ -61,6 +65,9 @@ typedef struct cuvhs_config {
     int precision;            /* CUVHS_PRECISION_* */
     void* stream;             /* hipStream_t, NULL = a stream owned by the context */
     int device;               /* HIP device ordinal */
+    int input_format;         /* CUVHS_INPUT_* (0 = u8) */
+    int dc_correct;           /* subtract each chunk read's mean (--dc-correct,
+                                 InputConditioning, raw_reader.h:21-23) */
 } cuvhs_config;
 
 /* Derived timing constants (VideoFormat, video_format.cpp:23-104). */
@@ -112,20 +119,21 @@ typedef struct cuvhs_chunk_outputs {
 } cuvhs_chunk_outputs;
 
 /* Decode one chunk (Pipeline::process_chunk).  d_chunk points at capture
- * sample raw_offset in device memory, with `avail` samples readable from
+ * sample raw_offset in device memory (samples of the context's input
+ * format; all counts and offsets are in samples), with `avail` samples readable from
  * there (the rest of the capture, or at least max_fields * spfp); samples
  * past `avail` read as zero (pipeline.cu:1539-1541).  raw_offset is the
  * chunk's position in the capture (fileLoc, K2b cadence).  At most
  * max_fields (<= batch_fields) fields are decoded; info->next_raw_offset is
  * where the next chunk starts (pipeline.cu:1636-1651). */
-int cuvhs_decode_chunk(cuvhs_ctx* ctx, const uint8_t* d_chunk, uint64_t avail, uint64_t raw_offset,
+int cuvhs_decode_chunk(cuvhs_ctx* ctx, const void* d_chunk, uint64_t avail, uint64_t raw_offset,
                        int max_fields, cuvhs_chunk_info* info);
 int cuvhs_chunk_outputs_get(const cuvhs_ctx* ctx, cuvhs_chunk_outputs* out);
 
 /* Decode a whole device-resident capture (Pipeline::run, pipeline.cu:2626-2713)
  * into caller device buffers sized for max_out_fields fields.  Returns the
  * field count through *fields. */
-int cuvhs_decode_capture(cuvhs_ctx* ctx, const uint8_t* d_capture, uint64_t capture_len, int max_out_fields,
+int cuvhs_decode_capture(cuvhs_ctx* ctx, const void* d_capture, uint64_t capture_len, int max_out_fields,
                          uint16_t* d_luma, uint16_t* d_chroma, uint64_t* d_file_loc, int32_t* d_is_first,
                          int32_t* d_phase_id, int32_t* d_do_count, int32_t* d_do_line, int32_t* d_do_startx,
                          int32_t* d_do_endx, int* fields);
@@ -134,8 +142,9 @@ int cuvhs_decode_capture(cuvhs_ctx* ctx, const uint8_t* d_capture, uint64_t capt
  * All FP64 signal buffers hold nfields*spfp samples, laid out as the
  * reference's chunk buffers. */
 
-/* fm_demod (fm_demod.h:46-53): u8 chunk (n_valid samples, zero past it). */
-int cuvhs_fm_demod(cuvhs_ctx* ctx, const uint8_t* d_raw_u8, uint64_t n_valid, int nfields, double* d_demod,
+/* fm_demod (fm_demod.h:46-53): one chunk read of the context's input format
+ * (n_valid samples, zero past it). */
+int cuvhs_fm_demod(cuvhs_ctx* ctx, const void* d_raw, uint64_t n_valid, int nfields, double* d_demod,
                    double* d_demod05, double* d_env);
 /* discover_vsyncs (vsync_discover.h:6-11): ascending candidates, returns the
  * raw count through *count (may exceed cap). */
@@ -152,9 +161,9 @@ int cuvhs_hsync_refine(cuvhs_ctx* ctx, const double* d_demod05, int total, doubl
 /* tbc_resample (tbc_resample.h:16-27), debug counters omitted. */
 int cuvhs_tbc_resample(cuvhs_ctx* ctx, const double* d_demod, int total, const double* d_linelocs, int nfields,
                        uint16_t* d_luma);
-/* chroma_decode (chroma_decode.h:38-46): u8 chunk in, phase IDs out
+/* chroma_decode (chroma_decode.h:38-46): chunk read in, phase IDs out
  * (host array of nfields), using and updating the carried ChromaState. */
-int cuvhs_chroma(cuvhs_ctx* ctx, const uint8_t* d_raw_u8, uint64_t n_valid, const double* d_linelocs,
+int cuvhs_chroma(cuvhs_ctx* ctx, const void* d_raw, uint64_t n_valid, const double* d_linelocs,
                  int nfields, uint16_t* d_chroma, int* h_phase_ids);
 /* dropout_detect (dropout_detect.h:22-32), concealment in place. */
 int cuvhs_dropouts(cuvhs_ctx* ctx, const double* d_env, const double* d_linelocs, int nfields, uint16_t* d_luma,

@@ -1315,6 +1315,8 @@ struct or_dec {
     K2bState k2b;
     ChromaState chroma;
     bool last_parity = true;
+    int in_fmt = 0;       // 0 u8, 1 s16, 2 u16 (InputFormat, video_format.h)
+    bool dc_correct = false;
     // persistent device-like buffers
     std::vector<double> raw, demod, d05, env;
     std::vector<uint16_t> luma, chroma_out;
@@ -1322,6 +1324,33 @@ struct or_dec {
     std::vector<Meta> meta;
     std::vector<ChunkTrace> traces;
 };
+
+// RawReader::convert (raw_reader.cpp:71-100): bit-depth normalisation to the
+// s16 range, then the optional per-read DC removal (sequential mean of the
+// samples of this read).
+extern "C" int or_convert(int format, int dc_correct, const uint8_t* data, size_t n, double* out) {
+    if (format < 0 || format > 2) return -1;
+    for (size_t i = 0; i < n; i++) {
+        if (format == 0) {
+            out[i] = (static_cast<double>(data[i]) - 128.0) * 256.0;
+        } else if (format == 1) {
+            int16_t v;
+            memcpy(&v, data + 2 * i, 2);
+            out[i] = static_cast<double>(v);
+        } else {
+            uint16_t v;
+            memcpy(&v, data + 2 * i, 2);
+            out[i] = static_cast<double>(v) - 32768.0;
+        }
+    }
+    if (dc_correct && n > 0) {
+        double sum = 0.0;
+        for (size_t i = 0; i < n; i++) sum += out[i];
+        double mean = sum / static_cast<double>(n);
+        for (size_t i = 0; i < n; i++) out[i] -= mean;
+    }
+    return 0;
+}
 
 static int process_chunk(or_dec* d, const uint8_t* data, size_t total_n, size_t raw_offset, int num_fields,
                          size_t& next_raw_offset) {
@@ -1335,7 +1364,7 @@ static int process_chunk(or_dec* d, const uint8_t* data, size_t total_n, size_t 
     if (raw_offset < total_n) n_read = std::min(total_samples, total_n - raw_offset);
     if (n_read < spf) return 0;
     double* h_raw = d->raw.data();
-    for (size_t i = 0; i < n_read; i++) h_raw[i] = (static_cast<double>(data[raw_offset + i]) - 128.0) * 256.0;
+    or_convert(d->in_fmt, d->dc_correct ? 1 : 0, data + (d->in_fmt == 0 ? 1 : 2) * raw_offset, n_read, h_raw);
     for (size_t i = n_read; i < total_samples; i++) h_raw[i] = 0.0;
     int fields_loaded = (int)(n_read / spf);
     if (fields_loaded > num_fields) fields_loaded = num_fields;
@@ -1550,6 +1579,13 @@ or_dec* or_create(double mhz, int batch, int record_trace) {
 
 void or_destroy(or_dec* d) { delete d; }
 
+int or_set_input(or_dec* d, int format, int dc_correct) {
+    if (!d || format < 0 || format > 2) return -1;
+    d->in_fmt = format;
+    d->dc_correct = dc_correct != 0;
+    return 0;
+}
+
 int or_decode(or_dec* d, const uint8_t* data, size_t n) {
     size_t raw_offset = 0;
     int total_fields = 0;
@@ -1651,6 +1687,10 @@ int or_write_json_meta(double mhz, const char* path, int n, const uint64_t* file
 }
 
 int or_decode_file(const char* in_path, const char* out_base, double mhz, int batch) {
+    return or_decode_file_fmt(in_path, out_base, mhz, batch, 0, 0);
+}
+
+int or_decode_file_fmt(const char* in_path, const char* out_base, double mhz, int batch, int format, int dc_correct) {
     FILE* f = fopen(in_path, "rb");
     if (!f) return -1;
     fseek(f, 0, SEEK_END);
@@ -1661,7 +1701,11 @@ int or_decode_file(const char* in_path, const char* out_base, double mhz, int ba
     fclose(f);
     or_dec* d = or_create(mhz, batch, 0);
     if (!d) return -1;
-    or_decode(d, buf.data(), got);
+    if (or_set_input(d, format, dc_correct) != 0) {
+        or_destroy(d);
+        return -1;
+    }
+    or_decode(d, buf.data(), got / (format == 0 ? 1 : 2));
     int rc = or_write_outputs(d, out_base);
     int n = or_num_fields(d);
     or_destroy(d);

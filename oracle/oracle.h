@@ -74,7 +74,14 @@ int or_tbc_resample(double mhz, const double* demod, int total, const double* li
 typedef struct or_dec or_dec;
 or_dec* or_create(double mhz, int batch, int record_trace);
 void or_destroy(or_dec* d);
-/* Decode a whole capture held in memory (u8 samples).  Returns field count. */
+/* Input sample format (0 u8, 1 s16, 2 u16; 16-bit samples little-endian as
+ * the host stores them) and per-read DC removal (raw_reader.cpp:71-100).
+ * Default u8 without DC removal. */
+int or_set_input(or_dec* d, int format, int dc_correct);
+/* The conversion alone: n samples (n or 2n bytes) -> out[n] doubles. */
+int or_convert(int format, int dc_correct, const uint8_t* data, size_t n, double* out);
+/* Decode a whole capture held in memory: n samples (n or 2n bytes).  Returns
+ * field count. */
 int or_decode(or_dec* d, const uint8_t* data, size_t n);
 int or_num_fields(const or_dec* d);
 const uint16_t* or_luma(const or_dec* d);
@@ -100,6 +107,7 @@ int or_write_json_meta(double mhz, const char* path, int n, const uint64_t* file
 
 /* CLI-equivalent: decode file -> out_base.{tbc,_chroma.tbc,tbc.json} */
 int or_decode_file(const char* in_path, const char* out_base, double mhz, int batch);
+int or_decode_file_fmt(const char* in_path, const char* out_base, double mhz, int batch, int format, int dc_correct);
 
 #ifdef __cplusplus
 }

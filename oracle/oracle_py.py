@@ -53,16 +53,26 @@ def lib():
         L.or_fft.argtypes = [ctypes.c_int, _vp, _vp, ctypes.c_int]
         L.or_format_init.argtypes = [ctypes.c_double, _vp]
         L.or_decode_file.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_double, ctypes.c_int]
+        L.or_decode_file_fmt.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_double, ctypes.c_int,
+                                         ctypes.c_int, ctypes.c_int]
+        L.or_set_input.argtypes = [_vp, ctypes.c_int, ctypes.c_int]
+        L.or_convert.argtypes = [ctypes.c_int, ctypes.c_int, _vp, ctypes.c_size_t, _vp]
         _L = L
     return _L
 
 
-def decode(data, mhz, batch, trace=True):
+FORMATS = {np.dtype(np.uint8): 0, np.dtype(np.int16): 1, np.dtype(np.uint16): 2}
+
+
+def decode(data, mhz, batch, trace=True, dc_correct=False):
     """Decode a u8 capture; returns a dict of numpy outputs (+ per-chunk traces)."""
     L = lib()
     d = L.or_create(float(mhz), int(batch), 1 if trace else 0)
     try:
-        data = np.ascontiguousarray(data, dtype=np.uint8)
+        data = np.ascontiguousarray(data)
+        if data.dtype not in FORMATS:
+            raise ValueError(f"capture dtype {data.dtype} is not u8, s16 or u16")
+        L.or_set_input(d, FORMATS[data.dtype], 1 if dc_correct else 0)
         n = L.or_decode(d, data.ctypes.data, data.size)
         out = {"fields": n}
         tbc = n * 263 * 910
@@ -120,3 +130,13 @@ def fm_demod(raw_s16, mhz, nfields):
     env = np.zeros(n)
     L.or_fm_demod(float(mhz), raw.ctypes.data, int(nfields), dm.ctypes.data, d5.ctypes.data, env.ctypes.data)
     return dm, d5, env
+
+
+def convert(data, dc_correct=False):
+    """RawReader::convert of a u8 / s16 / u16 array (numpy) to doubles."""
+    data = np.ascontiguousarray(data)
+    out = np.zeros(data.size)
+    rc = lib().or_convert(FORMATS[data.dtype], 1 if dc_correct else 0, data.ctypes.data, data.size, out.ctypes.data)
+    if rc != 0:
+        raise ValueError("or_convert failed")
+    return out

@@ -6,7 +6,7 @@
 // compiled code.  Test infrastructure only.
 //
 // usage: ref_host format <mhz>
-//        ref_host convert <file.u8>            (prints one converted value per line)
+//        ref_host convert <file> [u8|s16|u16] [dc]   (one converted value per line)
 //        ref_host json <out_base>              (fixed metadata, see below)
 #include <cstdio>
 #include <cstdlib>
@@ -33,8 +33,15 @@ int main(int argc, char** argv) {
         return 0;
     }
     if (!strcmp(argv[1], "convert")) {
+        // ref_host convert <file> [u8|s16|u16] [dc]
+        InputFormat fmt = InputFormat::U8;
+        if (argc > 3 && !strcmp(argv[3], "s16")) fmt = InputFormat::S16;
+        if (argc > 3 && !strcmp(argv[3], "u16")) fmt = InputFormat::U16;
         RawReader r;
-        if (!r.open(argv[2], InputFormat::U8)) return 1;
+        InputConditioning cond;
+        cond.dc_correct = argc > 4 && !strcmp(argv[4], "dc");
+        r.set_conditioning(cond);
+        if (!r.open(argv[2], fmt)) return 1;
         size_t n = r.total_samples();
         std::vector<double> v(n);
         size_t got = r.read_at(v.data(), 0, n);

@@ -8,6 +8,8 @@
 * ref_format.json        VideoFormat constants printed by the reference's own
                          video_format.cpp (oracle/_ref/ref_host format)
 * ref_convert_u8.txt     RawReader::read_at conversion of all 256 u8 values
+* ref_convert_formats.npz  RawReader::read_at conversion of 4096 random u8,
+                         s16 and u16 samples, with and without --dc-correct
                          by the reference's own raw_reader.cpp
 * ref_writer_3fields.tbc.json  .tbc.json written by the reference's own
                          tbc_writer.cpp for fixed metadata (ref_host json)
@@ -116,8 +118,31 @@ def reference_files():
         f.write(text)
 
 
+def convert_formats():
+    """RawReader::read_at conversion of u8 / s16 / u16 samples, with and
+    without --dc-correct, by the reference's compiled raw_reader.cpp."""
+    if not os.path.exists(REF):
+        print("oracle/_ref/ref_host missing: conversion fixtures not regenerated")
+        return
+    rng = np.random.default_rng(16)
+    data = {"u8": rng.integers(0, 256, 4096).astype(np.uint8),
+            "s16": rng.integers(-32768, 32768, 4096).astype(np.int16),
+            "u16": rng.integers(0, 65536, 4096).astype(np.uint16)}
+    out = {}
+    for name, arr in data.items():
+        tmp = "/tmp/_conv." + name
+        arr.tofile(tmp)
+        out[name + "_in"] = arr
+        for dc in ("", "dc"):
+            args = [REF, "convert", tmp, name] + ([dc] if dc else [])
+            txt = subprocess.run(args, capture_output=True, text=True, check=True).stdout
+            out[name + ("_dc" if dc else "") + "_out"] = np.array([float(x) for x in txt.split()])
+    np.savez(os.path.join(HERE, "ref_convert_formats.npz"), **out)
+
+
 if __name__ == "__main__":
     fft_vectors()
     filter_refs()
     reference_files()
+    convert_formats()
     print("fixtures written to", HERE)

@@ -23,10 +23,9 @@ def test_help_exits_zero():
     (("--bogus", "a.u8", "o"), "Unknown option: --bogus"),
     (("--system", "SECAM", "a.u8", "o"), "Unknown system"),
     (("--system", "PAL", "a.u8", "o"), "PAL is not supported"),
-    (("a.s16", "o"), "not supported"),
-    (("--format", "u16", "a.u8", "o"), "not supported"),
+    (("--format", "s24", "a.u8", "o"), "Unknown format: s24"),
+    (("--batch", "0", "a.u8", "o"), "--batch must be >= 1"),
     (("a.u8", "o", "extra"), "Too many arguments"),
-    (("-", "o"), "stdin streaming is not supported"),
 ])
 def test_bad_arguments_exit_one(args, msg):
     r = run(*args)

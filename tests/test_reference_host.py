@@ -87,3 +87,27 @@ def test_live_reference_build_matches_fixtures(tmp_path):
     subprocess.run([REF, "json", base], check=True, capture_output=True)
     with open(base + ".tbc.json") as f, open(os.path.join(GOLDEN, "ref_writer_3fields.tbc.json")) as g:
         assert f.read() == g.read()
+
+
+@pytest.mark.parametrize("fmt", ["u8", "s16", "u16"])
+@pytest.mark.parametrize("dc", [False, True])
+def test_oracle_conversion_matches_reference(oracle, fmt, dc):
+    """RawReader::convert for every input format, with and without the
+    per-read DC removal: bit-identical to the reference's compiled code."""
+    g = np.load(os.path.join(GOLDEN, "ref_convert_formats.npz"))
+    out = oracle.convert(g[fmt + "_in"], dc_correct=dc)
+    ref = g[fmt + ("_dc" if dc else "") + "_out"]
+    assert np.array_equal(out, ref)
+
+
+def test_conversion_closed_forms(oracle):
+    """The normalisations are exact affine maps, and the DC removal leaves a
+    zero-mean sequence (to rounding)."""
+    x = np.array([0, 1, 127, 128, 255], np.uint8)
+    assert np.array_equal(oracle.convert(x), (x.astype(float) - 128.0) * 256.0)
+    y = np.array([-32768, -1, 0, 1, 32767], np.int16)
+    assert np.array_equal(oracle.convert(y), y.astype(float))
+    z = np.array([0, 1, 32768, 65535], np.uint16)
+    assert np.array_equal(oracle.convert(z), z.astype(float) - 32768.0)
+    w = oracle.convert(np.arange(1000, dtype=np.int16), dc_correct=True)
+    assert abs(w.mean()) < 1e-9 and w[0] == -499.5

@@ -111,6 +111,12 @@ std::string detect_format(const std::string& path) {
     return "u8";  // .u8, .raw and anything else (main.cpp:89-95)
 }
 
+// env_flag_enabled (pipeline.cu:37-40)
+bool env_flag(const char* name) {
+    const char* e = getenv(name);
+    return e && e[0] && strcmp(e, "0") != 0;
+}
+
 #define CHECK_HIP(x)                                                                       \
     do {                                                                                   \
         hipError_t e_ = (x);                                                               \
@@ -262,6 +268,11 @@ int main(int argc, char** argv) {
     cfg.device = a.gpu;
     cfg.input_format = in_fmt;
     cfg.dc_correct = a.dc_correct ? 1 : 0;
+    // the reference switches to K3 line location when CUVHS_K2B_DISABLE is
+    // set at all (pipeline.cu:1694)
+    cfg.line_locator = getenv("CUVHS_K2B_DISABLE") ? CUVHS_LINELOC_K3 : CUVHS_LINELOC_K2B;
+    const bool log_k3 = env_flag("CUVHS_DEBUG_K3_LOG"), log_k4 = env_flag("CUVHS_DEBUG_K4_LOG"),
+               log_k5 = env_flag("CUVHS_DEBUG_K5_LOG");
     cuvhs_ctx* ctx = nullptr;
     int rc = cuvhs_create(&cfg, &ctx);
     if (rc != 0) {
@@ -273,6 +284,11 @@ int main(int argc, char** argv) {
     fprintf(stderr, "Format: NTSC @ %.1f MHz capture rate\n  Samples/line: %d  Samples/field: %d\n", a.mhz,
             fmt.samples_per_line, fmt.samples_per_field);
 
+    if ((log_k3 || log_k4 || log_k5) && cuvhs_diagnostics(ctx, 1) != 0) {
+        fprintf(stderr, "Failed to enable diagnostics\n");
+        cuvhs_destroy(ctx);
+        return 1;
+    }
     InputSource src;
     if (!src.open(a.input, bps)) {
         fprintf(stderr, "Failed to open input: %s\n", a.input.c_str());
@@ -341,6 +357,20 @@ int main(int argc, char** argv) {
             fprintf(stderr, "Warning: no VSYNC candidates found in chunk at raw offset %llu, using nominal field spacing fallback\n",
                     (unsigned long long)raw_offset);
         if (info.fields == 0) break;
+        if (log_k3 || log_k4 || log_k5) {
+            // the reference's per-chunk debug lines (pipeline.cu:1874, 2036, 2074)
+            int32_t dg[9];
+            cuvhs_diagnostics_read(ctx, nullptr, dg);
+            if (log_k3)
+                fprintf(stderr, "  [K3 spacing] chunk raw_offset=%zu: bad_spacing=%d isolated_spacing=%d large_jump=%d\n",
+                        (size_t)raw_offset, dg[0], dg[1], dg[2]);
+            if (log_k4)
+                fprintf(stderr, "  [K4 debug] chunk raw_offset=%zu: large_delta=%d isolated_jump=%d refined_sync_like=%d\n",
+                        (size_t)raw_offset, dg[3], dg[4], dg[5]);
+            if (log_k5)
+                fprintf(stderr, "  [K5 debug] chunk raw_offset=%zu: mode=%s oob_pixels=%d bad_geom_lines=%d sync_like_pixels=%d\n",
+                        (size_t)raw_offset, "refined", dg[6], dg[7], dg[8]);
+        }
         const int n = info.fields;
         cuvhs_chunk_outputs o;
         cuvhs_chunk_outputs_get(ctx, &o);

@@ -97,6 +97,11 @@ struct ChunkCtl {
     int pad_[2];
     uint64_t raw_offset;
     uint64_t next_raw_offset;
+    // diagnostic counters of the reference's debug logs, when enabled:
+    // K3 bad/isolated spacing, large jump; K4 large delta, isolated jump,
+    // refined sync-like; K5 out-of-bounds pixels, bad rows, sync-like pixels
+    int diag[9];
+    int pad2_;
 };
 
 // State carried from chunk to chunk; only the line-location, chroma and

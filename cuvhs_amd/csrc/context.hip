@@ -103,6 +103,10 @@ struct cuvhs_ctx {
     bool prof = false;
     bool serial = false;            // profiling: every stage on the head stream, no overlap
     double stage_ms[kStages + 1] = {};
+    bool diag = false;              // K3/K4/K5 diagnostic counters
+    long long diag_total[9] = {};
+    int diag_last[9] = {};
+    double* ll_coarse = nullptr;    // line locations before K4 (diagnostics)
     uint64_t work_samples = 0;      // profiling: K1 samples demodulated
     int work_chunks = 0, work_fields = 0;
 };
@@ -325,13 +329,26 @@ int tail_stage(cuvhs_ctx* c, Slot& s, const OutCopy* out, int n) {
     mark(c, s, 5, st);
     HIPCHK(c, run_sync_pulses(s.events, s.ctl, s.offsets, f.spfp, c->k2s, c->k2l, c->k2c, s.max_fields, 0, st));
     mark(c, s, 6, st);
-    HIPCHK(c, run_k2b(s.ctl, s.ctl, c->state, 0, s.max_fields, f, s.offsets, c->k2s, c->k2l, c->k2c, c->vps, c->vpl,
-                      c->vptg, (K2bField*)c->summ, (K2bAnchorOut*)c->anchors, c->ps, c->pl, c->pc, c->pt, c->ll,
-                      c->isf_k2b, st));
+    if (c->cfg.line_locator == CUVHS_LINELOC_K3) {
+        // line_locs on the K2 pulses (pipeline.cu:1848-1856); K2b's cadence
+        // state is left untouched, as in the reference
+        HIPCHK(c, run_k3(s.ctl, 0, s.max_fields, f, c->k2s, c->k2l, c->k2c, c->pt, c->ll, c->isf_k2b, st));
+    } else {
+        HIPCHK(c, run_k2b(s.ctl, s.ctl, c->state, 0, s.max_fields, f, s.offsets, c->k2s, c->k2l, c->k2c, c->vps,
+                          c->vpl, c->vptg, (K2bField*)c->summ, (K2bAnchorOut*)c->anchors, c->ps, c->pl, c->pc, c->pt,
+                          c->ll, c->isf_k2b, st));
+    }
+    if (c->diag) {
+        HIPCHK(c, run_diag_spacing(c->ll, s.ctl, s.max_fields, f, st));
+        HIPCHK(c, hipMemcpyAsync(c->ll_coarse, c->ll, (size_t)s.max_fields * kLinesPerFrame * sizeof(double),
+                                 hipMemcpyDeviceToDevice, st));
+    }
     mark(c, s, 7, st);
     HIPCHK(c, run_hsync_refine(s.d05, c->ll, s.ctl, 0, s.max_fields, f, st));
+    if (c->diag) HIPCHK(c, run_diag_k4(s.d05, c->ll_coarse, c->ll, s.ctl, s.max_fields, f, st));
     mark(c, s, 8, st);
     HIPCHK(c, run_tbc_resample(s.demod, c->ll, c->luma, s.ctl, 0, s.max_fields, f, st));
+    if (c->diag) HIPCHK(c, run_diag_k5(s.demod, c->ll, s.ctl, s.max_fields, f, st));
     mark(c, s, 9, st);
     HIPCHK(c, hipStreamWaitEvent(st, s.pre1, 0));
     mark(c, s, 10, st);
@@ -386,6 +403,12 @@ int sync_all(cuvhs_ctx* c) {
 int finish_tail(cuvhs_ctx* c, Slot& s) {
     HIPCHK(c, hipEventSynchronize(s.tail_done));
     collect_profile(c, s);
+    if (c->diag) {
+        for (int i = 0; i < 9; i++) {
+            c->diag_last[i] = s.hctl->diag[i];
+            c->diag_total[i] += s.hctl->diag[i];
+        }
+    }
     if (s.hctl->vp_overflow) {
         c->err = "K2b validated-pulse capacity exceeded";
         return CUVHS_E_CAPACITY;
@@ -423,6 +446,7 @@ int cuvhs_create(const cuvhs_config* cfg, cuvhs_ctx** out) {
     if (cfg->system != CUVHS_SYSTEM_NTSC || cfg->precision != CUVHS_PRECISION_FP64) return CUVHS_E_UNSUPPORTED;
     if (!(cfg->sample_rate_mhz >= 16.0 && cfg->sample_rate_mhz <= 40.0)) return CUVHS_E_UNSUPPORTED;
     if (cfg->input_format < CUVHS_INPUT_U8 || cfg->input_format > CUVHS_INPUT_U16) return CUVHS_E_UNSUPPORTED;
+    if (cfg->line_locator != CUVHS_LINELOC_K2B && cfg->line_locator != CUVHS_LINELOC_K3) return CUVHS_E_UNSUPPORTED;
     if (cfg->batch_fields < 1 || cfg->batch_fields > 4096) return CUVHS_E_ARG;
     cuvhs_ctx* c = new cuvhs_ctx();
     c->cfg = *cfg;
@@ -545,6 +569,7 @@ int cuvhs_create(const cuvhs_config* cfg, cuvhs_ctx** out) {
         c->anchors = p2;
     }
     ALLOC(c->ll, Fz * kLinesPerFrame);
+    ALLOC(c->ll_coarse, Fz * kLinesPerFrame);
     ALLOC(c->isf_k2b, Fz);
     ALLOC(c->luma, Fz * kFieldLines * kLineLen);
     ALLOC(c->chroma, Fz * kFieldLines * kLineLen);
@@ -737,6 +762,35 @@ int cuvhs_profile(cuvhs_ctx* c, int enable) {
     c->work_samples = 0;
     c->work_chunks = 0;
     c->work_fields = 0;
+    return 0;
+}
+
+int cuvhs_diagnostics(cuvhs_ctx* c, int enable) {
+    if (!c) return CUVHS_E_ARG;
+    int r = sync_all(c);
+    if (r) return r;
+    c->diag = enable != 0;
+    for (int i = 0; i < 9; i++) {
+        c->diag_total[i] = 0;
+        c->diag_last[i] = 0;
+    }
+    return 0;
+}
+
+int cuvhs_diagnostics_read(const cuvhs_ctx* c, int64_t* totals, int32_t* last_chunk) {
+    if (!c) return CUVHS_E_ARG;
+    for (int i = 0; i < 9; i++) {
+        if (totals) totals[i] = c->diag_total[i];
+        if (last_chunk) last_chunk[i] = c->diag_last[i];
+    }
+    return 9;
+}
+
+int cuvhs_k3_linelocs(cuvhs_ctx* c, const int* starts, const int* lens, const int* count, int nf, int* types,
+                      double* linelocs, int* is_first) {
+    if (!c || !starts || !lens || !count || !linelocs || !is_first || nf < 1 || nf > c->F) return CUVHS_E_ARG;
+    HIPCHK(c, run_k3(c->slot[0].ctl, nf, nf, c->fmt, starts, lens, count, types, linelocs, is_first, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return 0;
 }
 

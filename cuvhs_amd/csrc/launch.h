@@ -62,6 +62,13 @@ hipError_t run_k1_part_a(int B, const K1ALaunch& a, int nf, hipStream_t st);
 hipError_t run_k1_part_b(int B, const K1BLaunch& a, int nf, hipStream_t st);
 hipError_t run_prebpf(int B, const PreLaunch& a, int nf, hipStream_t st);
 hipError_t run_fill_f64(double* p, long long n, double v, hipStream_t st);
+hipError_t run_k3(const ChunkCtl* ctl, int nf_override, int max_fields, const Fmt& fmt, const int* starts,
+                  const int* lens, const int* count, int* types, double* linelocs, int* is_first, hipStream_t st);
+hipError_t run_diag_spacing(const double* ll, ChunkCtl* ctl, int max_fields, const Fmt& fmt, hipStream_t st);
+hipError_t run_diag_k4(const double* d05, const double* before, const double* after, ChunkCtl* ctl, int max_fields,
+                       const Fmt& fmt, hipStream_t st);
+hipError_t run_diag_k5(const double* demod, const double* ll, ChunkCtl* ctl, int max_fields, const Fmt& fmt,
+                       hipStream_t st);
 hipError_t run_input_mean(const void* cap, int in_fmt, long long n, double* partial, double* mean, hipStream_t st);
 hipError_t run_fft_selftest(int B, int is_f32, const void* in, void* out, const void* tw, int sign, hipStream_t st);
 int events_nwg(long long total);

@@ -18,6 +18,9 @@ LINE_LEN = 910
 E_ARG, E_HIP, E_ALLOC, E_UNSUPPORTED, E_CAPACITY = -1, -2, -3, -4, -5
 INPUT_U8, INPUT_S16, INPUT_U16 = 0, 1, 2
 INPUT_FORMATS = {"u8": INPUT_U8, "s16": INPUT_S16, "u16": INPUT_U16}
+LINE_LOCATORS = {"k2b": 0, "k3": 1}
+DIAG_NAMES = ("k3_bad_spacing", "k3_isolated_spacing", "k3_large_jump", "k4_large_delta", "k4_isolated_jump",
+              "k4_refined_sync_like", "k5_oob_pixels", "k5_bad_geom_lines", "k5_sync_like_pixels")
 W_VSYNC_OVERFLOW, W_NO_VSYNC, W_EOF = 1, 2, 4
 
 _vp = ctypes.c_void_p
@@ -32,7 +35,7 @@ class CuvhsError(RuntimeError):
 
 class Config(ctypes.Structure):
     _fields_ = [("system", _i), ("sample_rate_mhz", _d), ("batch_fields", _i), ("precision", _i),
-                ("stream", _vp), ("device", _i), ("input_format", _i), ("dc_correct", _i)]
+                ("stream", _vp), ("device", _i), ("input_format", _i), ("dc_correct", _i), ("line_locator", _i)]
 
 
 class Format(ctypes.Structure):
@@ -73,6 +76,9 @@ _SIGS = {
     "cuvhs_sync_pulses": (_i, [_vp, _vp, _i, _vp, _i, _vp, _vp, _vp]),
     "cuvhs_linelocs": (_i, [_vp, _u64, _vp, _i, _vp, _vp, _vp, _vp, _vp, _vp]),
     "cuvhs_hsync_refine": (_i, [_vp, _vp, _i, _vp, _i]),
+    "cuvhs_k3_linelocs": (_i, [_vp, _vp, _vp, _vp, _i, _vp, _vp, _vp]),
+    "cuvhs_diagnostics": (_i, [_vp, _i]),
+    "cuvhs_diagnostics_read": (_i, [_vp, _vp, _vp]),
     "cuvhs_tbc_resample": (_i, [_vp, _vp, _i, _vp, _i, _vp]),
     "cuvhs_chroma": (_i, [_vp, _vp, _u64, _vp, _i, _vp, _vp]),
     "cuvhs_dropouts": (_i, [_vp, _vp, _vp, _i, _vp, _vp, _vp, _vp, _vp, _vp]),
@@ -184,10 +190,11 @@ def ptr(t):
 class Decoder:
     """One decode context (Pipeline, pipeline.h:28-114) on a HIP device."""
 
-    def __init__(self, sample_rate_mhz=28.0, batch_fields=64, device=0, input_format="u8", dc_correct=False):
+    def __init__(self, sample_rate_mhz=28.0, batch_fields=64, device=0, input_format="u8", dc_correct=False,
+                 line_locator="k2b"):
         self.input_format = INPUT_FORMATS[input_format]
         cfg = Config(0, float(sample_rate_mhz), int(batch_fields), 0, None, int(device), self.input_format,
-                     1 if dc_correct else 0)
+                     1 if dc_correct else 0, LINE_LOCATORS[line_locator])
         h = _vp()
         rc = lib().cuvhs_create(ctypes.byref(cfg), ctypes.byref(h))
         if rc != 0:
@@ -224,6 +231,17 @@ class Decoder:
         k = lib().cuvhs_profile_read(self.h, ms, names, n)
         return {names[i].decode(): ms[i] for i in range(k)}
 
+    def diagnostics(self, enable=True):
+        """Collect the reference's K3/K4/K5 debug counters per chunk (clears totals)."""
+        check(lib().cuvhs_diagnostics(self.h, 1 if enable else 0), self.h, "diagnostics")
+
+    def diagnostics_read(self):
+        """({name: total since enabled}, {name: last chunk})."""
+        tot = (ctypes.c_int64 * 9)()
+        last = (ctypes.c_int32 * 9)()
+        check(lib().cuvhs_diagnostics_read(self.h, tot, last), self.h, "diagnostics_read")
+        return dict(zip(DIAG_NAMES, list(tot))), dict(zip(DIAG_NAMES, list(last)))
+
     def profile_work(self):
         """(K1 samples demodulated, chunks, fields) since profiling was enabled."""
         smp, ch, fl = _u64(), _i(), _i()
@@ -255,7 +273,8 @@ class Decoder:
             check(rc, self.h, "decode_chunk")
             infos.append({"fields": info.fields, "fields_k1": info.fields_k1, "raw_offset": info.raw_offset,
                           "next_raw_offset": info.next_raw_offset, "status": info.status,
-                          "vsync_candidates": info.vsync_candidates})
+                          "vsync_candidates": info.vsync_candidates,
+                          "diag": list(self.diagnostics_read()[1].values()) if info.fields else None})
             if info.fields == 0:
                 break
             o = ChunkOutputs()

@@ -55,6 +55,10 @@ enum { CUVHS_PRECISION_FP64 = 0 };
  * raw_reader.cpp:71-91): u8 -> (s - 128) * 256, s16 as is, u16 -> s - 32768.
  * 16-bit samples are in host byte order. */
 enum { CUVHS_INPUT_U8 = 0, CUVHS_INPUT_S16 = 1, CUVHS_INPUT_U16 = 2 };
+/* Line locator: K2b (the reference's default, pipeline.cu:410-893) or K3
+ * (line_locs.cu, what the reference runs with CUVHS_K2B_DISABLE set,
+ * pipeline.cu:1694). */
+enum { CUVHS_LINELOC_K2B = 0, CUVHS_LINELOC_K3 = 1 };
 
 typedef struct cuvhs_ctx cuvhs_ctx;
 
@@ -68,6 +72,7 @@ typedef struct cuvhs_config {
     int input_format;         /* CUVHS_INPUT_* (0 = u8) */
     int dc_correct;           /* subtract each chunk read's mean (--dc-correct,
                                  InputConditioning, raw_reader.h:21-23) */
+    int line_locator;         /* CUVHS_LINELOC_* (0 = K2b) */
 } cuvhs_config;
 
 /* Derived timing constants (VideoFormat, video_format.cpp:23-104). */
@@ -156,6 +161,11 @@ int cuvhs_sync_pulses(cuvhs_ctx* ctx, const double* d_demod05, int total, const 
  * and updating the context's carried cadence state. */
 int cuvhs_linelocs(cuvhs_ctx* ctx, uint64_t raw_offset, const uint64_t* d_offsets, int nfields, int* d_starts,
                    int* d_lens, int* d_count, int* d_types, double* d_linelocs, int* d_is_first);
+/* line_locs (line_locs.h:48-56): K3 on K2 pulse lists ([nfields][800]);
+ * d_types (optional) receives the nominal width classes, d_is_first the
+ * field parity (1 first, 0 second, -1 unknown). */
+int cuvhs_k3_linelocs(cuvhs_ctx* ctx, const int* d_starts, const int* d_lens, const int* d_count, int nfields,
+                      int* d_types, double* d_linelocs, int* d_is_first);
 /* hsync_refine (hsync_refine.h:18-22), in place. */
 int cuvhs_hsync_refine(cuvhs_ctx* ctx, const double* d_demod05, int total, double* d_linelocs, int nfields);
 /* tbc_resample (tbc_resample.h:16-27), debug counters omitted. */
@@ -190,6 +200,16 @@ int cuvhs_profile_read(const cuvhs_ctx* ctx, double* ms, const char** names, int
  * demodulated by K1 (fields_k1 * spfp per chunk), chunks, and fields decoded.
  * Any pointer may be NULL.  (No counterpart in the reference.) */
 int cuvhs_profile_work(const cuvhs_ctx* ctx, uint64_t* samples, int* chunks, int* fields);
+
+/* The reference's debug-log counters (CUVHS_DEBUG_K3_LOG, _K4_LOG, _K5_LOG:
+ * line_locs.cu:585-653, hsync_refine.cu:273-349, tbc_resample.cu:75-116),
+ * collected per chunk while enabled; enabling clears the totals.  Order:
+ * K3 bad_spacing, isolated_spacing, large_jump; K4 large_delta,
+ * isolated_jump, refined_sync_like; K5 oob_pixels, bad_geom_lines,
+ * sync_like_pixels.  cuvhs_diagnostics_read fills totals since enabling and
+ * the last chunk's values (either may be NULL) and returns 9. */
+int cuvhs_diagnostics(cuvhs_ctx* ctx, int enable);
+int cuvhs_diagnostics_read(const cuvhs_ctx* ctx, int64_t* totals, int32_t* last_chunk);
 
 /* Last HIP error text for the context (or a static string). */
 const char* cuvhs_last_error(const cuvhs_ctx* ctx);

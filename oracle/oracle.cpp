@@ -1317,6 +1317,9 @@ struct or_dec {
     bool last_parity = true;
     int in_fmt = 0;       // 0 u8, 1 s16, 2 u16 (InputFormat, video_format.h)
     bool dc_correct = false;
+    int line_locator = 0; // 0 K2b (default), 1 K3 (CUVHS_K2B_DISABLE)
+    bool diag = false;    // collect the K3/K4/K5 diagnostic counters
+    std::vector<std::vector<int>> diag_counts;  // per chunk: K3 x3, K4 x3, K5 x3
     // persistent device-like buffers
     std::vector<double> raw, demod, d05, env;
     std::vector<uint16_t> luma, chroma_out;
@@ -1324,6 +1327,278 @@ struct or_dec {
     std::vector<Meta> meta;
     std::vector<ChunkTrace> traces;
 };
+
+// ---------------------------------------------------------------------------
+// K3 line location, the reference's alternative GPU path selected by
+// CUVHS_K2B_DISABLE (line_locs.cu:18-476; pipeline.cu:1694, 1848-1856).
+// ---------------------------------------------------------------------------
+struct K3Thresholds {
+    double tol, hsync_min, hsync_max, eq_min, eq_max, vsync_min, vsync_max;
+};
+
+// line_locs host entry (line_locs.cu:491-506)
+static K3Thresholds k3_thresholds(const Fmt& fmt) {
+    K3Thresholds t;
+    t.tol = 0.5e-6 * fmt.sample_rate;
+    t.hsync_min = fmt.hsync_width - t.tol;
+    t.hsync_max = fmt.hsync_width + t.tol;
+    t.eq_min = fmt.eq_pulse_width - t.tol;
+    t.eq_max = fmt.eq_pulse_width + t.tol;
+    t.vsync_min = fmt.vsync_width * 0.5;
+    t.vsync_max = fmt.vsync_width + 1.0e-6 * fmt.sample_rate;
+    return t;
+}
+
+// Phase 1: nominal width classes (k_classify_pulses, line_locs.cu:18-48)
+static int k3_nominal_type(const K3Thresholds& t, int length) {
+    double len = (double)length;
+    if (len >= t.vsync_min && len <= t.vsync_max) return PULSE_VSYNC;
+    if (len >= t.hsync_min && len <= t.hsync_max) return PULSE_HSYNC;
+    if (len >= t.eq_min && len <= t.eq_max) return PULSE_EQ1;
+    return PULSE_HSYNC;
+}
+
+// Phases 2 and 3 for one field (k_compute_linelocs, line_locs.cu:60-476).
+static void k3_field(const Fmt& fmt, const int* starts, const int* lengths, int npc, double* locs, int* is_first) {
+    const double tol_samples = 0.5e-6 * fmt.sample_rate;
+    const double hsync_nominal = fmt.hsync_width, eq_nominal = fmt.eq_pulse_width, vsync_nominal = fmt.vsync_width;
+    const int lines_per_frame = fmt.lines_per_frame;
+    const double inlinelen = (double)fmt.samples_per_line;
+    // Step 0: adaptive classification (:96-137)
+    double generous_hsync_min = hsync_nominal - 3.5 * tol_samples;
+    double generous_hsync_max = hsync_nominal + 4.0 * tol_samples;
+    double hsync_sum = 0.0;
+    int hsync_count = 0;
+    for (int i = 0; i < npc; i++) {
+        double len = (double)lengths[i];
+        if (len >= generous_hsync_min && len <= generous_hsync_max) {
+            hsync_sum += len;
+            hsync_count++;
+        }
+    }
+    double hsync_median_est = (hsync_count > 0) ? (hsync_sum / hsync_count) : hsync_nominal;
+    double hsync_offset = hsync_median_est - hsync_nominal;
+    double a_hsync_min = hsync_nominal + hsync_offset - tol_samples;
+    double a_hsync_max = hsync_nominal + hsync_offset + tol_samples;
+    double a_eq_min = eq_nominal + hsync_offset - tol_samples;
+    double a_eq_max = eq_nominal + hsync_offset + tol_samples;
+    double a_vsync_min = (vsync_nominal + hsync_offset) * 0.5;
+    double a_vsync_max = vsync_nominal + hsync_offset + 2.0 * tol_samples;
+    const int n = npc < 800 ? npc : 800;
+    std::vector<int> types((size_t)n);
+    for (int i = 0; i < n; i++) {
+        double len = (double)lengths[i];
+        if (len >= a_vsync_min && len <= a_vsync_max) types[(size_t)i] = PULSE_VSYNC;
+        else if (len >= a_hsync_min && len <= a_hsync_max) types[(size_t)i] = PULSE_HSYNC;
+        else if (len >= a_eq_min && len <= a_eq_max) types[(size_t)i] = PULSE_EQ1;
+        else types[(size_t)i] = PULSE_HSYNC;
+    }
+    // Step 1: vblank state machine (:139-196)
+    int ref_pulse_idx = -1, last_hsync_idx = -1, first_eq1_idx = -1, last_eq2_idx = -1, state = -1;
+    for (int i = 0; i < n; i++) {
+        int t = types[(size_t)i];
+        if (state == -1) {
+            if (t == PULSE_HSYNC) { state = PULSE_HSYNC; last_hsync_idx = i; }
+        } else if (state == PULSE_HSYNC) {
+            if (t == PULSE_HSYNC) last_hsync_idx = i;
+            else if (t == PULSE_EQ1) { state = PULSE_EQ1; first_eq1_idx = i; }
+            else if (t == PULSE_VSYNC) state = PULSE_VSYNC;
+        } else if (state == PULSE_EQ1) {
+            if (t == PULSE_VSYNC) state = PULSE_VSYNC;
+            else if (t == PULSE_HSYNC) { state = PULSE_HSYNC; last_hsync_idx = i; first_eq1_idx = -1; }
+        } else if (state == PULSE_VSYNC) {
+            if (t == PULSE_EQ1) { state = PULSE_EQ2; last_eq2_idx = i; }
+            else if (t == PULSE_HSYNC) { ref_pulse_idx = i; break; }
+        } else if (state == PULSE_EQ2) {
+            if (t == PULSE_EQ1) last_eq2_idx = i;
+            else if (t == PULSE_HSYNC) {
+                if (last_eq2_idx < 0) last_eq2_idx = i - 1;
+                ref_pulse_idx = i;
+                break;
+            }
+        }
+    }
+    // Step 1b: field parity (:198-229), NTSC
+    int field_parity = -1;
+    if (ref_pulse_idx >= 0 && last_hsync_idx >= 0 && first_eq1_idx >= 0) {
+        double entry_lines = (double)(starts[first_eq1_idx] - starts[last_hsync_idx]) / inlinelen;
+        field_parity = (entry_lines > 0.75) ? 1 : 0;
+    }
+    if (field_parity < 0 && ref_pulse_idx >= 0 && last_eq2_idx >= 0) {
+        double exit_lines = (double)(starts[ref_pulse_idx] - starts[last_eq2_idx]) / inlinelen;
+        field_parity = (exit_lines > 0.75) ? 1 : 0;
+    }
+    *is_first = field_parity;
+    // Step 2: mean line length from the longest HSYNC run (:231-277)
+    int best_run_start = -1, best_run_len = 0, cur_run_start = -1, cur_run_len = 0;
+    for (int i = 0; i < n; i++) {
+        if (types[(size_t)i] == PULSE_HSYNC) {
+            if (cur_run_start == -1) { cur_run_start = i; cur_run_len = 1; }
+            else cur_run_len++;
+        } else {
+            if (cur_run_len > best_run_len) { best_run_start = cur_run_start; best_run_len = cur_run_len; }
+            cur_run_start = -1;
+            cur_run_len = 0;
+        }
+    }
+    if (cur_run_len > best_run_len) { best_run_start = cur_run_start; best_run_len = cur_run_len; }
+    double meanlinelen = inlinelen;
+    if (best_run_len >= 2) {
+        double sum = 0.0;
+        int count = 0;
+        for (int i = best_run_start + 1; i < best_run_start + best_run_len; i++) {
+            double spacing = (double)(starts[i] - starts[i - 1]);
+            if (spacing >= inlinelen * 0.95 && spacing <= inlinelen * 1.05) { sum += spacing; count++; }
+        }
+        if (count > 0) meanlinelen = sum / count;
+    }
+    // Step 3: reference pulse (:279-324)
+    double ref_position, ref_line;
+    if (ref_pulse_idx >= 0) {
+        ref_position = (double)starts[ref_pulse_idx];
+        ref_line = 10.0;
+        if (ref_pulse_idx > n / 2) {
+            double corrected = ref_position - (double)fmt.samples_per_field;
+            if (corrected > 0.0) ref_position = corrected;
+        }
+    } else {
+        ref_pulse_idx = 0;
+        for (int i = 0; i < n; i++)
+            if (types[(size_t)i] == PULSE_HSYNC) { ref_pulse_idx = i; break; }
+        ref_position = (double)starts[ref_pulse_idx];
+        ref_line = 0.0;
+    }
+    // Step 4a: line markers (:346-387)
+    std::vector<int> ms, ml;
+    double tiny_hsync_max = hsync_nominal * 0.75;
+    double surrogate_gap_max = inlinelen * 0.25;
+    for (int i = 0; i < n && (int)ms.size() < 800; i++) {
+        int ti = types[(size_t)i];
+        bool surrogate_eq = (ti == PULSE_EQ1 || ti == PULSE_EQ2) && i + 1 < n && types[(size_t)i + 1] == PULSE_HSYNC &&
+                            (double)lengths[i + 1] <= tiny_hsync_max &&
+                            (double)(starts[i + 1] - starts[i]) <= surrogate_gap_max;
+        bool suppress = ti == PULSE_HSYNC && (double)lengths[i] <= tiny_hsync_max && i > 0 &&
+                        (types[(size_t)i - 1] == PULSE_EQ1 || types[(size_t)i - 1] == PULSE_EQ2) &&
+                        (double)(starts[i] - starts[i - 1]) <= surrogate_gap_max;
+        if (surrogate_eq || (ti == PULSE_HSYNC && !suppress)) {
+            ms.push_back(starts[i]);
+            ml.push_back(lengths[i]);
+        }
+    }
+    // Step 4b: collapse near-duplicate markers (:389-433)
+    std::vector<int> hs;
+    const int marker_count = (int)ms.size();
+    double duplicate_gap_max = inlinelen * 0.75;
+    int cluster_start = 0;
+    while (cluster_start < marker_count && (int)hs.size() < 800) {
+        int best_idx = cluster_start;
+        double best_score = fabs((double)ml[(size_t)cluster_start] - hsync_nominal);
+        int cluster_end = cluster_start + 1;
+        while (cluster_end < marker_count &&
+               (double)(ms[(size_t)cluster_end] - ms[(size_t)cluster_start]) < duplicate_gap_max) {
+            double score = fabs((double)ml[(size_t)cluster_end] - hsync_nominal);
+            if (score < best_score || (score == best_score && ms[(size_t)cluster_end] < ms[(size_t)best_idx])) {
+                best_idx = cluster_end;
+                best_score = score;
+            }
+            cluster_end++;
+        }
+        hs.push_back(ms[(size_t)best_idx]);
+        cluster_start = cluster_end;
+    }
+    // Step 4c: greedy nearest-marker snapping (:437-476)
+    const int num_hsyncs = (int)hs.size();
+    double max_allowed_distance = meanlinelen / 1.5;
+    int cur_hsync = 0;
+    for (int line = 0; line < lines_per_frame; line++) {
+        double expected = ref_position + meanlinelen * (line - ref_line);
+        locs[line] = expected;
+        if (cur_hsync >= num_hsyncs) continue;
+        double best_dist = max_allowed_distance;
+        int best_pos = -1;
+        int search = cur_hsync;
+        while (search < num_hsyncs) {
+            double dist = fabs((double)hs[(size_t)search] - expected);
+            if (dist <= best_dist) {
+                best_dist = dist;
+                best_pos = hs[(size_t)search];
+                cur_hsync = search;
+            }
+            if (search + 1 < num_hsyncs) {
+                double next_dist = fabs((double)hs[(size_t)search + 1] - expected);
+                if (next_dist > dist) break;
+            } else {
+                break;
+            }
+            search++;
+        }
+        if (best_pos >= 0) {
+            locs[line] = (double)best_pos;
+            cur_hsync++;
+        }
+    }
+}
+
+// Diagnostic counters of the reference (K3: line_locs.cu:585-653, K4:
+// hsync_refine.cu:273-349, K5: tbc_resample.cu:75-116), over nf fields.
+static void k3_spacing_counts(const Fmt& fmt, const double* ll, int nf, int* out) {
+    const int L = fmt.lines_per_frame, als = fmt.active_line_start;
+    const double nominal = (double)fmt.samples_per_line;
+    for (int f = 0; f < nf; f++) {
+        const double* l = ll + (size_t)f * L;
+        for (int line = als; line + 1 < L; line++) {
+            double spacing = l[line + 1] - l[line];
+            if (spacing < 0.9 * nominal || spacing > 1.1 * nominal) out[0]++;
+            if (fabs(spacing - nominal) > 0.25 * nominal) out[2]++;
+            if (line > als && line + 2 < L) {
+                double avg = 0.5 * ((l[line] - l[line - 1]) + (l[line + 2] - l[line + 1]));
+                if (fabs(spacing - avg) > 0.15 * nominal) out[1]++;
+            }
+        }
+    }
+}
+
+static void k4_counts(const Fmt& fmt, const double* d05, int total, const double* before, const double* after, int nf,
+                      int* out) {
+    const int L = fmt.lines_per_frame, als = fmt.active_line_start;
+    const double us = 1.0e-6 * fmt.sample_rate;
+    for (int f = 0; f < nf; f++) {
+        const double* b = before + (size_t)f * L;
+        const double* a = after + (size_t)f * L;
+        for (int line = als; line + 1 < L; line++) {
+            double delta = a[line] - b[line];
+            if (fabs(delta) > 1.5 * us) out[0]++;
+            if (line > als) {
+                double avg = 0.5 * ((a[line - 1] - b[line - 1]) + (a[line + 1] - b[line + 1]));
+                if (fabs(delta - avg) > 1.0 * us) out[1]++;
+            }
+            int sample = (int)(a[line] + 0.5);
+            if (sample >= 0 && sample < total && d05[sample] <= fmt.pulse_threshold_hz) out[2]++;
+        }
+    }
+}
+
+static void k5_counts(const Fmt& fmt, const double* demod, int total, const double* ll, int nf, int* out) {
+    const int L = fmt.lines_per_frame;
+    const double spl = (double)fmt.samples_per_line;
+    for (int f = 0; f < nf; f++) {
+        for (int row = 0; row < fmt.output_field_lines; row++) {
+            double ls = ll[(size_t)f * L + row], le = ll[(size_t)f * L + row + 1];
+            double len = le - ls;
+            if (len <= 0.0 || len < 0.8 * spl || len > 1.2 * spl) out[1]++;
+            for (int col = 0; col < fmt.output_line_len; col++) {
+                double coord = ls + ((double)col / (double)fmt.output_line_len) * len;
+                int ci = (int)coord;
+                double x = coord - (double)ci;
+                if (ci < 1 || ci + 2 >= total) { out[0]++; continue; }
+                double p0 = demod[ci - 1], p1 = demod[ci], p2 = demod[ci + 1], p3 = demod[ci + 2];
+                double a = p2 - p0, bb = 2.0 * p0 - 5.0 * p1 + 4.0 * p2 - p3, c = 3.0 * (p1 - p2) + p3 - p0;
+                double hz = p1 + 0.5 * x * (a + x * (bb + x * c));
+                if (col >= 134 && col < 894 && hz <= fmt.pulse_threshold_hz) out[2]++;
+            }
+        }
+    }
+}
 
 // RawReader::convert (raw_reader.cpp:71-100): bit-depth normalisation to the
 // s16 range, then the optional per-read DC removal (sequential mean of the
@@ -1417,19 +1692,38 @@ static int process_chunk(or_dec* d, const uint8_t* data, size_t total_n, size_t 
     std::vector<int> pt((size_t)nf * MAX_PULSES, 0);
     sync_pulses(fmt, d->d05.data(), offs.data(), nf, ps.data(), pl.data(), pc.data());
     if (d->trace) { tr.pc = pc; tr.ps = ps; tr.pl = pl; tr.offsets = offs; }
-    // K2b
+    // K2b, or K3 when selected (pipeline.cu:1694-1856)
     std::vector<double> ll((size_t)nf * fmt.lines_per_frame);
     std::vector<int> is_first((size_t)nf);
-    k2b_chunk(fmt, d->k2b, raw_offset, offs, nf, ps.data(), pl.data(), pc.data(), pt.data(), ll.data(),
-              is_first.data());
+    if (d->line_locator == 0) {
+        k2b_chunk(fmt, d->k2b, raw_offset, offs, nf, ps.data(), pl.data(), pc.data(), pt.data(), ll.data(),
+                  is_first.data());
+    } else {
+        K3Thresholds t3 = k3_thresholds(fmt);
+        for (int f = 0; f < nf; f++) {
+            for (int i = 0; i < pc[(size_t)f]; i++)
+                pt[(size_t)f * MAX_PULSES + i] = k3_nominal_type(t3, pl[(size_t)f * MAX_PULSES + i]);
+            k3_field(fmt, ps.data() + (size_t)f * MAX_PULSES, pl.data() + (size_t)f * MAX_PULSES, pc[(size_t)f],
+                     ll.data() + (size_t)f * fmt.lines_per_frame, &is_first[(size_t)f]);
+        }
+    }
     if (d->trace) { tr.vc = pc; tr.vs = ps; tr.vl = pl; tr.vt = pt; tr.llc = ll; tr.is_first = is_first; }
+    std::vector<int> dg(9, 0);
+    if (d->diag) k3_spacing_counts(fmt, ll.data(), nf, dg.data());
     // K4
     int total = nf * (int)spfp;
+    std::vector<double> coarse;
+    if (d->diag) coarse = ll;
     hsync_refine(fmt, d->d05.data(), ll.data(), nf, total);
     if (d->trace) tr.llr = ll;
+    if (d->diag) k4_counts(fmt, d->d05.data(), total, coarse.data(), ll.data(), nf, dg.data() + 3);
     // K5
     std::vector<uint16_t> luma((size_t)nf * tbc), chroma((size_t)nf * tbc);
     tbc_resample(fmt, d->demod.data(), ll.data(), luma.data(), nf, total);
+    if (d->diag) {
+        k5_counts(fmt, d->demod.data(), total, ll.data(), nf, dg.data() + 6);
+        d->diag_counts.push_back(dg);
+    }
     // K6 (bandpassed raw overwrites demod, pipeline.cu:2531)
     std::vector<int> phase_ids;
     chroma_decode(fmt, h_raw, ll.data(), d->demod.data(), chroma.data(), nf, total, phase_ids, &d->chroma);
@@ -1578,6 +1872,44 @@ or_dec* or_create(double mhz, int batch, int record_trace) {
 }
 
 void or_destroy(or_dec* d) { delete d; }
+
+int or_set_line_locator(or_dec* d, int mode, int diagnostics) {
+    if (!d || mode < 0 || mode > 1) return -1;
+    d->line_locator = mode;
+    d->diag = diagnostics != 0;
+    return 0;
+}
+
+int or_chunk_diagnostics(const or_dec* d, int c, int* out9) {
+    if (!d || c < 0 || c >= (int)d->diag_counts.size()) return -1;
+    for (int i = 0; i < 9; i++) out9[i] = d->diag_counts[(size_t)c][(size_t)i];
+    return 0;
+}
+
+int or_k3_linelocs(double mhz, const int* starts, const int* lens, const int* count, int nf, int* types,
+                   double* linelocs, int* is_first) {
+    Fmt fmt = make_fmt(mhz);
+    K3Thresholds t3 = k3_thresholds(fmt);
+    for (int f = 0; f < nf; f++) {
+        const int* st = starts + (size_t)f * MAX_PULSES;
+        const int* ln = lens + (size_t)f * MAX_PULSES;
+        if (types)
+            for (int i = 0; i < count[f]; i++) types[(size_t)f * MAX_PULSES + i] = k3_nominal_type(t3, ln[i]);
+        k3_field(fmt, st, ln, count[f], linelocs + (size_t)f * fmt.lines_per_frame, &is_first[f]);
+    }
+    return 0;
+}
+
+int or_diag_counts(double mhz, int which, const double* a, const double* b, const double* c, int total, int nf,
+                   int* out3) {
+    Fmt fmt = make_fmt(mhz);
+    out3[0] = out3[1] = out3[2] = 0;
+    if (which == 3) k3_spacing_counts(fmt, a, nf, out3);
+    else if (which == 4) k4_counts(fmt, a, total, b, c, nf, out3);
+    else if (which == 5) k5_counts(fmt, a, total, b, nf, out3);
+    else return -1;
+    return 0;
+}
 
 int or_set_input(or_dec* d, int format, int dc_correct) {
     if (!d || format < 0 || format > 2) return -1;

@@ -78,6 +78,24 @@ void or_destroy(or_dec* d);
  * the host stores them) and per-read DC removal (raw_reader.cpp:71-100).
  * Default u8 without DC removal. */
 int or_set_input(or_dec* d, int format, int dc_correct);
+/* Line locator: 0 = K2b (default), 1 = K3 (the reference with
+ * CUVHS_K2B_DISABLE set, line_locs.cu); diagnostics != 0 records per chunk
+ * the reference's K3/K4/K5 debug counters (line_locs.cu:585-653,
+ * hsync_refine.cu:273-349, tbc_resample.cu:75-116) in this order:
+ * K3 bad_spacing, isolated_spacing, large_jump; K4 large_delta,
+ * isolated_jump, refined_sync_like; K5 oob_pixels, bad_geom_lines,
+ * sync_like_pixels. */
+int or_set_line_locator(or_dec* d, int mode, int diagnostics);
+int or_chunk_diagnostics(const or_dec* d, int c, int* out9);
+/* One diagnostic counter set: which = 3 (a = linelocs), 4 (a = demod_05 of
+ * `total` samples, b = linelocs before K4, c = after), 5 (a = demod of
+ * `total` samples, b = linelocs); nf fields; three counts out. */
+int or_diag_counts(double mhz, int which, const double* a, const double* b, const double* c, int total, int nf,
+                   int* out3);
+/* K3 on K2 pulse lists ([nf][800] each): nominal types (line_locs.cu:18-48),
+ * line locations and field parity (1 first, 0 second, -1 unknown). */
+int or_k3_linelocs(double mhz, const int* starts, const int* lens, const int* count, int nf, int* types,
+                   double* linelocs, int* is_first);
 /* The conversion alone: n samples (n or 2n bytes) -> out[n] doubles. */
 int or_convert(int format, int dc_correct, const uint8_t* data, size_t n, double* out);
 /* Decode a whole capture held in memory: n samples (n or 2n bytes).  Returns

@@ -57,6 +57,10 @@ def lib():
                                          ctypes.c_int, ctypes.c_int]
         L.or_set_input.argtypes = [_vp, ctypes.c_int, ctypes.c_int]
         L.or_convert.argtypes = [ctypes.c_int, ctypes.c_int, _vp, ctypes.c_size_t, _vp]
+        L.or_set_line_locator.argtypes = [_vp, ctypes.c_int, ctypes.c_int]
+        L.or_chunk_diagnostics.argtypes = [_vp, ctypes.c_int, _vp]
+        L.or_diag_counts.argtypes = [ctypes.c_double, ctypes.c_int, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int, _vp]
+        L.or_k3_linelocs.argtypes = [ctypes.c_double, _vp, _vp, _vp, ctypes.c_int, _vp, _vp, _vp]
         _L = L
     return _L
 
@@ -64,7 +68,7 @@ def lib():
 FORMATS = {np.dtype(np.uint8): 0, np.dtype(np.int16): 1, np.dtype(np.uint16): 2}
 
 
-def decode(data, mhz, batch, trace=True, dc_correct=False):
+def decode(data, mhz, batch, trace=True, dc_correct=False, line_locator=0, diagnostics=False):
     """Decode a u8 capture; returns a dict of numpy outputs (+ per-chunk traces)."""
     L = lib()
     d = L.or_create(float(mhz), int(batch), 1 if trace else 0)
@@ -73,6 +77,7 @@ def decode(data, mhz, batch, trace=True, dc_correct=False):
         if data.dtype not in FORMATS:
             raise ValueError(f"capture dtype {data.dtype} is not u8, s16 or u16")
         L.or_set_input(d, FORMATS[data.dtype], 1 if dc_correct else 0)
+        L.or_set_line_locator(d, int(line_locator), 1 if diagnostics else 0)
         n = L.or_decode(d, data.ctypes.data, data.size)
         out = {"fields": n}
         tbc = n * 263 * 910
@@ -116,6 +121,14 @@ def decode(data, mhz, batch, trace=True, dc_correct=False):
                                    ll_coarse=llc.reshape(B, -1)[:nf], ll_refined=llr.reshape(B, -1)[:nf],
                                    is_first_k2b=isk[:nf]))
         out["chunks"] = chunks
+        if diagnostics:
+            diag = []
+            for c in range(L.or_num_chunks(d) if trace else 10 ** 6):
+                v = np.zeros(9, np.int32)
+                if L.or_chunk_diagnostics(d, c, v.ctypes.data) != 0:
+                    break
+                diag.append(v)
+            out["diag"] = np.array(diag).reshape(-1, 9)
         return out
     finally:
         L.or_destroy(d)
@@ -140,3 +153,33 @@ def convert(data, dc_correct=False):
     if rc != 0:
         raise ValueError("or_convert failed")
     return out
+
+
+def k3_linelocs(starts, lens, count, mhz):
+    """K3 line location on K2 pulse lists ([nf, 800] int arrays, count [nf]):
+    (nominal types, linelocs [nf, 525], is_first [nf])."""
+    starts = np.ascontiguousarray(starts, np.int32)
+    lens = np.ascontiguousarray(lens, np.int32)
+    count = np.ascontiguousarray(count, np.int32)
+    nf = count.size
+    types = np.zeros((nf, MAX_PULSES), np.int32)
+    ll = np.zeros((nf, 525))
+    isf = np.zeros(nf, np.int32)
+    lib().or_k3_linelocs(float(mhz), starts.ctypes.data, lens.ctypes.data, count.ctypes.data, nf, types.ctypes.data,
+                         ll.ctypes.data, isf.ctypes.data)
+    return types, ll, isf
+
+
+def diag_counts(which, mhz, a, b=None, c=None, nf=1):
+    """One of the reference's debug counter sets (3: K3 spacing on linelocs a;
+    4: K4 with demod_05 a, linelocs before b and after c; 5: K5 with demod a
+    and linelocs b) over nf fields."""
+    a = np.ascontiguousarray(a, np.float64)
+    b = None if b is None else np.ascontiguousarray(b, np.float64)
+    c = None if c is None else np.ascontiguousarray(c, np.float64)
+    out = np.zeros(3, np.int32)
+    rc = lib().or_diag_counts(float(mhz), int(which), a.ctypes.data, None if b is None else b.ctypes.data,
+                              None if c is None else c.ctypes.data, a.size, int(nf), out.ctypes.data)
+    if rc != 0:
+        raise ValueError("bad counter set")
+    return tuple(int(x) for x in out)

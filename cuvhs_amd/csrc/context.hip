@@ -190,6 +190,8 @@ ChromaLaunch chroma_launch(cuvhs_ctx* c, const ChunkCtl* ctl, int nf_override, c
     a.nf_override = nf_override;
     a.spfp = c->fmt.spfp;
     a.het_scale = (c->fmt.fsc + c->fmt.chroma_under) / c->fmt.output_rate;
+    a.het_dc = cos(2.0 * M_PI * a.het_scale * 64.0);
+    a.het_ds = sin(2.0 * M_PI * a.het_scale * 64.0);
     a.fft_scale = 1.0 / (double)kChromaFFT;
     a.burst_abs_ref = c->fmt.burst_abs_ref;
     a.burst_start = c->fmt.burst_start;

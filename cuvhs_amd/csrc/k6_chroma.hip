@@ -26,9 +26,10 @@ constexpr int kBurstLen = 36;      // 112 - 76 at 4 fsc
 constexpr int kSkip = 16;          // chroma_decode.cu:255
 constexpr int kPairs = (kFieldLines - 2 * kSkip - 1) / 2;  // 115
 
-// Resampled, heterodyned sample of one line (chroma_decode.cu:75-157).
-__device__ __forceinline__ double chroma_het_sample(const ChromaLaunch& a, double ls, double le, int line, int col,
-                                                   double phase_offset, long long total) {
+// Resampled sample of one line (chroma_decode.cu:75-137), before the
+// heterodyne.
+__device__ __forceinline__ double chroma_resample(const ChromaLaunch& a, double ls, double le, int col,
+                                                 long long total) {
     if (col >= kLineLen) return 0.0;
     double frac = (double)col / (double)kLineLen;
     double coord = ls + frac * (le - ls);
@@ -39,10 +40,25 @@ __device__ __forceinline__ double chroma_het_sample(const ChromaLaunch& a, doubl
     double aa = p2 - p0;
     double bb = 2.0 * p0 - 5.0 * p1 + 4.0 * p2 - p3;
     double cc = 3.0 * (p1 - p2) + p3 - p0;
-    double raw_val = p1 + 0.5 * x * (aa + x * (bb + x * cc));
-    double abs_sample = (double)(line * kLineLen + col);
-    double het = -cos(2.0 * M_PI * a.het_scale * abs_sample + phase_offset);
-    return raw_val * het;
+    return p1 + 0.5 * x * (aa + x * (bb + x * cc));
+}
+
+// Heterodyne -cos(2 pi het_scale (line * 910 + col) + phase) of
+// chroma_decode.cu:143-148 at the columns col = t + 64 r, r = 0..15, that
+// thread t owns: one sincos at col = t, then rotations by the fixed step
+// exp(i 2 pi het_scale 64) (het_dc, het_ds).  The rotation differs from a
+// direct cos by ~1e-14 relative; the reference's own FP64 argument, up to
+// 4.4e5 rad, is rounded at ~5e-11 rad.
+__device__ __forceinline__ void chroma_het(const ChromaLaunch& a, int line, double phase_offset, double* het) {
+    double s, c;
+    sincos(2.0 * M_PI * a.het_scale * (double)(line * kLineLen + (int)threadIdx.x) + phase_offset, &s, &c);
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        het[r] = -c;
+        const double cn = c * a.het_dc - s * a.het_ds;
+        s = s * a.het_dc + c * a.het_ds;
+        c = cn;
+    }
 }
 
 // One wavefront per pair of lines (la, la + 1): resample + heterodyne
@@ -78,11 +94,17 @@ __global__ __launch_bounds__(64) void k_chroma_line(ChromaLaunch a) {
         ph_b = (double)((lb * rot + 0) & 3) * (M_PI * 0.5);
     }
     double2 v[16];
+    double het[16];
+    chroma_het(a, la, ph_a, het);
 #pragma unroll
-    for (int r = 0; r < 16; r++) {
-        const int col = (int)threadIdx.x + 64 * r;
-        v[r].x = chroma_het_sample(a, ls_a, le_a, la, col, ph_a, total);
-        v[r].y = has_b ? chroma_het_sample(a, ls_b, le_b, lb, col, ph_b, total) : 0.0;
+    for (int r = 0; r < 16; r++) v[r].x = chroma_resample(a, ls_a, le_a, (int)threadIdx.x + 64 * r, total) * het[r];
+    if (has_b) {
+        chroma_het(a, lb, ph_b, het);
+#pragma unroll
+        for (int r = 0; r < 16; r++) v[r].y = chroma_resample(a, ls_b, le_b, (int)threadIdx.x + 64 * r, total) * het[r];
+    } else {
+#pragma unroll
+        for (int r = 0; r < 16; r++) v[r].y = 0.0;
     }
     fft_fwd_regs_mul<double, kChromaFFT>(buf, v, a.tw, a.mask);
     fft_lds<+1, double, kChromaFFT>(buf, a.tw);

@@ -48,6 +48,7 @@ struct ChromaLaunch {
     int nf_override;
     int spfp;
     double het_scale, fft_scale, burst_abs_ref;
+    double het_dc, het_ds;      // cos / sin of 2 pi het_scale 64 (heterodyne step of 64 columns)
     int burst_start, lo, hi;
     const int* track;
     int line0, line_end;        // lines [line0, line_end) of each field

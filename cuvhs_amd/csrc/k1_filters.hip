@@ -183,29 +183,29 @@ __global__ __launch_bounds__(B / 16) void k1_part_b(K1BLaunch a) {
 }
 
 // Chroma pre-band-pass: segment f is raw[f*spfp, (f+1)*spfp) zero-padded to N2
-// and filtered circularly (chroma_decode.cu:606-627).
+// and filtered circularly (chroma_decode.cu:606-627).  The filter is real,
+// so each workgroup convolves two adjacent V-output blocks in one complex
+// transform: block 2p in the real part, block 2p+1 in the imaginary part
+// (G is the spectrum of a real kernel, so the two never mix beyond rounding).
 template <int B>
 __global__ __launch_bounds__(B / 16) void k_prebpf(PreLaunch a) {
     extern __shared__ __align__(16) unsigned char smem[];
     float2* buf = reinterpret_cast<float2*>(smem);
     constexpr int NT = B / 16;
     const int f = blockIdx.y;
-    const int b = blockIdx.x;
-    const int ns = b * a.V;
-    if (ns >= a.spfp) return;
+    const int ns0 = 2 * (int)blockIdx.x * a.V;
+    if (ns0 >= a.spfp) return;
+    const int ns1 = ns0 + a.V;
+    const bool two = ns1 < a.spfp;
     const int seg0 = f * a.spfp;
     const double dcv = a.dc ? *a.dc : 0.0;
-    float xin[16];
-#pragma unroll
-    for (int i = 0; i < 16; i++) {
-        int n = wrap_n(ns - a.lp + (int)threadIdx.x + i * NT, a.N2);
-        xin[i] = n < a.spfp ? (float)raw_sample(a.cap, a.in_fmt, seg0 + n, a.n_valid, dcv) : 0.0f;
-    }
     float2 v[16];
 #pragma unroll
     for (int i = 0; i < 16; i++) {
-        v[i].x = xin[i];
-        v[i].y = 0.0f;
+        int n0 = wrap_n(ns0 - a.lp + (int)threadIdx.x + i * NT, a.N2);
+        int n1 = wrap_n(ns1 - a.lp + (int)threadIdx.x + i * NT, a.N2);
+        v[i].x = n0 < a.spfp ? (float)raw_sample(a.cap, a.in_fmt, seg0 + n0, a.n_valid, dcv) : 0.0f;
+        v[i].y = two && n1 < a.spfp ? (float)raw_sample(a.cap, a.in_fmt, seg0 + n1, a.n_valid, dcv) : 0.0f;
     }
     // the spectrum product stays a separate pass here: fused into the last
     // (radix-4) stage it needs more than the 128 VGPRs a 1024-thread
@@ -214,9 +214,9 @@ __global__ __launch_bounds__(B / 16) void k_prebpf(PreLaunch a) {
     spectrum_multiply<float, B>(buf, a.G);
     fft_lds<+1, float, B>(buf, a.tw);
     for (int o = threadIdx.x; o < a.V; o += NT) {
-        int n = ns + o;
-        if (n >= a.spfp) break;
-        a.out[seg0 + n] = buf[pidx(a.lp + o)].x;
+        float2 z = buf[pidx(a.lp + o)];
+        if (ns0 + o < a.spfp) a.out[seg0 + ns0 + o] = z.x;
+        if (ns1 + o < a.spfp) a.out[seg0 + ns1 + o] = z.y;
     }
 }
 
@@ -245,7 +245,7 @@ static hipError_t launch_b(const K1BLaunch& a, int nf, hipStream_t st) {
 }
 template <int B>
 static hipError_t launch_pre(const PreLaunch& a, int nf, hipStream_t st) {
-    int nb = (int)((a.spfp + a.V - 1) / a.V);
+    int nb = (int)((a.spfp + 2 * a.V - 1) / (2 * a.V));
     size_t sh = (size_t)lds_elems(B) * sizeof(float2);
     (void)hipFuncSetAttribute((const void*)k_prebpf<B>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
     hipLaunchKernelGGL(k_prebpf<B>, dim3(nb, nf), dim3(B / 16), sh, st, a);

@@ -6,6 +6,11 @@ One step decodes the whole synthetic capture (600 NTSC fields of u8 RF at
 28 MHz: SNR 25-40 dB per field, Poisson dropouts, wow/flutter; chunks of
 F = 64 fields) with the capture already resident in HBM and outputs left in
 HBM.  Prints one JSON line on stdout.
+
+The headline value is the FP64 form (the reference's arithmetic, checked
+against the oracle).  The line also carries the FP32 form (K1 transforms in
+FP32) timed the same way, with its error and decision report against the
+FP64 form (cuvhs_amd.precision); --no-forms skips it.
 """
 import argparse
 import json
@@ -55,6 +60,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-forms", action="store_true", help="skip the FP32 form and its error report")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 4 plain device copies of MIB MiB (calibrates HBM byte counters) and exit")
     args = ap.parse_args()
@@ -133,6 +139,32 @@ def main():
                 "achieved_isolated": round(alg_bytes_per_launch / iso_launch_s / 1e9, 1),
                 "avg_launch_ms_isolated": round(iso_launch_s * 1e3, 4)}
 
+    forms = None
+    if not args.no_forms:
+        from cuvhs_amd import precision
+        d32 = native.Decoder(28.0, args.batch, device=0, precision="fp32")
+        for _ in range(args.warmup):
+            d32.reset()
+            d32.decode_capture(d_cap)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            d32.reset()
+            f32_fields = d32.decode_capture(d_cap)["fields"]
+        torch.cuda.synchronize()
+        dt32 = time.perf_counter() - t0
+        d32.profile(True, serial=True)
+        d32.reset()
+        d32.decode_capture(d_cap)
+        iso32 = d32.profile_read()
+        d32.close()
+        log(f"fp32 form: {f32_fields * args.steps / dt32:.1f} fields/s; comparing with fp64 ...")
+        report = precision.compare(d_cap, 28.0, args.batch, "fp32")
+        forms = {"fp32": {"value": round(f32_fields * args.steps / dt32, 2), "unit": "fields/s",
+                          "ms_per_step": round(dt32 * 1000.0 / args.steps, 3), "dtype": "f32 (K1), f64 (rest)",
+                          "stage_ms_per_step_isolated": {k: round(v, 3) for k, v in iso32.items()},
+                          "vs_fp64": report}}
+
     cpu = None
     if not args.no_cpu_baseline:
         log("running CPU baseline ...")
@@ -160,6 +192,7 @@ def main():
         "stage_ms_per_step_isolated": {k: round(v, 3) for k, v in iso.items()},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "forms": forms,
     }
     print(json.dumps(line), flush=True)
 

@@ -2,13 +2,16 @@
 // Pipeline::run pipeline.cu:2626-2713) on top of the C ABI in include/cuvhs.h.
 //
 //   cuvhs [--system NTSC] [-f MHz] [--format u8|s16|u16] [--dc-correct]
-//         [--gpu id] [--batch F] [--overwrite] <input | -> <output_base>
+//         [--gpu id] [--batch F] [--precision fp64|fp32] [--overwrite]
+//         <input | -> <output_base>
 //
 // Writes output_base.tbc, output_base_chroma.tbc and output_base.tbc.json.
 // Exit status 0 on success, 1 on any failure.  --batch sets the chunk size F
 // (default 64); the reference derives F from free VRAM and its results
 // depend on it, so it is explicit here.  Input "-" (or a FIFO) is decoded as
-// a stream with the same results as the same bytes in a file.
+// a stream with the same results as the same bytes in a file.  --precision
+// fp32 runs the FM demodulation transforms in FP32 (default fp64, the
+// reference's arithmetic).
 #include <fcntl.h>
 #include <hip/hip_runtime.h>
 #include <strings.h>
@@ -37,6 +40,7 @@ struct Args {
     bool dc_correct = false;
     int gpu = 0;
     int batch = 64;
+    int precision = CUVHS_PRECISION_FP64;
     bool overwrite = false;
 };
 
@@ -54,6 +58,7 @@ void usage(const char* prog) {
             "  --dc-correct          Subtract per-read DC offset after normalization\n"
             "  --gpu <id>            GPU device ID (default: 0)\n"
             "  --batch <fields>      Fields per chunk (default: 64)\n"
+            "  --precision <fp64|fp32> FM demodulation arithmetic (default: fp64)\n"
             "  --overwrite           Overwrite existing output files\n"
             "  -h, --help            Show this help\n",
             prog);
@@ -84,6 +89,11 @@ bool parse(int argc, char** argv, Args& a) {
             a.gpu = atoi(argv[++i]);
         } else if (!strcmp(argv[i], "--batch") && i + 1 < argc) {
             a.batch = atoi(argv[++i]);
+        } else if (!strcmp(argv[i], "--precision") && i + 1 < argc) {
+            i++;
+            if (!strcmp(argv[i], "fp64")) a.precision = CUVHS_PRECISION_FP64;
+            else if (!strcmp(argv[i], "fp32")) a.precision = CUVHS_PRECISION_FP32;
+            else { fprintf(stderr, "Unknown precision: %s\n", argv[i]); return false; }
         } else if (!strcmp(argv[i], "--overwrite")) {
             a.overwrite = true;
         } else if (argv[i][0] == '-' && strlen(argv[i]) > 1) {
@@ -263,7 +273,7 @@ int main(int argc, char** argv) {
     cfg.system = CUVHS_SYSTEM_NTSC;
     cfg.sample_rate_mhz = a.mhz;
     cfg.batch_fields = a.batch;
-    cfg.precision = CUVHS_PRECISION_FP64;
+    cfg.precision = a.precision;
     cfg.stream = nullptr;
     cfg.device = a.gpu;
     cfg.input_format = in_fmt;

@@ -21,6 +21,7 @@
 #include "../../include/cuvhs.h"
 #include "common.h"
 #include "design.h"
+#include "fft_lds.h"
 #include "launch.h"
 
 using namespace cuvhs;
@@ -65,11 +66,12 @@ struct cuvhs_ctx {
     std::string err;
     K1Design k1;
     PreDesign pre;
-    double2* G_a = nullptr;
-    double2* G_b = nullptr;
+    int k1_f32 = 0;            // K1 transforms in FP32 (CUVHS_PRECISION_FP32)
+    void* G_a = nullptr;       // K1 block spectra and twiddles: double2, or float2 when k1_f32
+    void* G_b = nullptr;
     float2* G_p = nullptr;
-    double2* tw_a = nullptr;
-    double2* tw_b = nullptr;
+    void* tw_a = nullptr;
+    void* tw_b = nullptr;
     float2* tw_p = nullptr;
     double2* tw_1024 = nullptr;
     double* cs_tab = nullptr;       // cos/sin of the burst phase detector
@@ -147,13 +149,30 @@ std::vector<V> twiddles(int B) {
     return t;
 }
 
+// perm: store bin k at conv4096_gperm(k) (the 4096-point wave-local blocks)
 template <typename V, typename S>
-hipError_t upload_spectrum(cuvhs_ctx* c, V** dst, const std::vector<cdbl>& s) {
+hipError_t upload_spectrum(cuvhs_ctx* c, V** dst, const std::vector<cdbl>& s, bool perm = false) {
     std::vector<V> h(s.size());
-    for (size_t i = 0; i < s.size(); i++) { h[i].x = (S)s[i].real(); h[i].y = (S)s[i].imag(); }
+    for (size_t i = 0; i < s.size(); i++) {
+        size_t j = perm ? (size_t)conv4096_gperm((int)i) : i;
+        h[j].x = (S)s[i].real();
+        h[j].y = (S)s[i].imag();
+    }
     hipError_t e = dalloc(c, dst, h.size());
     if (e != hipSuccess) return e;
     return hipMemcpy(*dst, h.data(), h.size() * sizeof(V), hipMemcpyHostToDevice);
+}
+
+hipError_t upload_twiddles(cuvhs_ctx* c, void** dst, int B, int f32) {
+    hipError_t e;
+    if (f32) {
+        auto t = twiddles<float2, float>(B);
+        if ((e = dalloc(c, (float2**)dst, t.size())) != hipSuccess) return e;
+        return hipMemcpy(*dst, t.data(), t.size() * sizeof(float2), hipMemcpyHostToDevice);
+    }
+    auto t = twiddles<double2, double>(B);
+    if ((e = dalloc(c, (double2**)dst, t.size())) != hipSuccess) return e;
+    return hipMemcpy(*dst, t.data(), t.size() * sizeof(double2), hipMemcpyHostToDevice);
 }
 
 int sample_bytes(const cuvhs_ctx* c) { return c->cfg.input_format == CUVHS_INPUT_U8 ? 1 : 2; }
@@ -222,7 +241,7 @@ int k1_launch(cuvhs_ctx* c, Slot& s, const void* chunk, long long n_valid, int n
     a.tw = c->tw_a;
     a.rawdemod = c->rawdemod;
     a.env = s.env;
-    HIPCHK(c, run_k1_part_a(c->k1.rf.B, a, nfk1, st));
+    HIPCHK(c, run_k1_part_a(c->k1.rf.B, c->k1_f32, a, nfk1, st));
     mark(c, s, 1, st);
     K1BLaunch b{};
     b.rawdemod = c->rawdemod;
@@ -236,7 +255,7 @@ int k1_launch(cuvhs_ctx* c, Slot& s, const void* chunk, long long n_valid, int n
     b.tw = c->tw_b;
     b.demod = s.demod;
     b.demod05 = s.d05;
-    HIPCHK(c, run_k1_part_b(c->k1.video.B, b, nfk1, st));
+    HIPCHK(c, run_k1_part_b(c->k1.video.B, c->k1_f32, b, nfk1, st));
     // beyond the demodulated region the buffers read as zero
     HIPCHK(c, run_fill_f64(s.demod + T, c->buf_n - T, 0.0, st));
     HIPCHK(c, run_fill_f64(s.d05 + T, c->buf_n - T, 0.0, st));
@@ -445,7 +464,8 @@ int cuvhs_format_for(double mhz, cuvhs_format* o) {
 int cuvhs_create(const cuvhs_config* cfg, cuvhs_ctx** out) {
     if (!cfg || !out) return CUVHS_E_ARG;
     *out = nullptr;
-    if (cfg->system != CUVHS_SYSTEM_NTSC || cfg->precision != CUVHS_PRECISION_FP64) return CUVHS_E_UNSUPPORTED;
+    if (cfg->system != CUVHS_SYSTEM_NTSC || (cfg->precision != CUVHS_PRECISION_FP64 && cfg->precision != CUVHS_PRECISION_FP32))
+        return CUVHS_E_UNSUPPORTED;
     if (!(cfg->sample_rate_mhz >= 16.0 && cfg->sample_rate_mhz <= 40.0)) return CUVHS_E_UNSUPPORTED;
     if (cfg->input_format < CUVHS_INPUT_U8 || cfg->input_format > CUVHS_INPUT_U16) return CUVHS_E_UNSUPPORTED;
     if (cfg->line_locator != CUVHS_LINELOC_K2B && cfg->line_locator != CUVHS_LINELOC_K3) return CUVHS_E_UNSUPPORTED;
@@ -481,8 +501,11 @@ int cuvhs_create(const cuvhs_config* cfg, cuvhs_ctx** out) {
         }
     }
     try {
+        // taps below 1e-13 (FP64) or 1e-8 (FP32) of the largest are dropped:
+        // both far below the rounding of the transform they feed
         const int bk1[2] = {4096, 8192};
-        c->k1 = design_k1(f, 1e-13, bk1, 2, 1);
+        c->k1_f32 = cfg->precision == CUVHS_PRECISION_FP32;
+        c->k1 = c->k1_f32 ? design_k1(f, 1e-8, bk1, 2, 1, 8) : design_k1(f, 1e-13, bk1, 2, 1, 16);
         const int bpre[2] = {8192, 16384};
         c->pre = design_prebpf(f, 1e-7, bpre, 2);
     } catch (...) {
@@ -494,16 +517,20 @@ int cuvhs_create(const cuvhs_config* cfg, cuvhs_ctx** out) {
         hipError_t e2 = dalloc(c, &(p), (size_t)(n));  \
         if (e2 != hipSuccess) return bad(e2, #p);      \
     } while (0)
-    if ((e = upload_spectrum<double2, double>(c, &c->G_a, c->k1.rf.spectrum)) != hipSuccess) return bad(e, "G_a");
-    if ((e = upload_spectrum<double2, double>(c, &c->G_b, c->k1.video.spectrum)) != hipSuccess) return bad(e, "G_b");
+    {
+        const bool pa = c->k1.rf.B == 4096, pb = c->k1.video.B == 4096;
+        if (c->k1_f32) {
+            if ((e = upload_spectrum<float2, float>(c, (float2**)&c->G_a, c->k1.rf.spectrum, pa)) != hipSuccess) return bad(e, "G_a");
+            if ((e = upload_spectrum<float2, float>(c, (float2**)&c->G_b, c->k1.video.spectrum, pb)) != hipSuccess) return bad(e, "G_b");
+        } else {
+            if ((e = upload_spectrum<double2, double>(c, (double2**)&c->G_a, c->k1.rf.spectrum, pa)) != hipSuccess) return bad(e, "G_a");
+            if ((e = upload_spectrum<double2, double>(c, (double2**)&c->G_b, c->k1.video.spectrum, pb)) != hipSuccess) return bad(e, "G_b");
+        }
+    }
     if ((e = upload_spectrum<float2, float>(c, &c->G_p, c->pre.bpf.spectrum)) != hipSuccess) return bad(e, "G_p");
     {
-        auto ta = twiddles<double2, double>(c->k1.rf.B);
-        ALLOC(c->tw_a, ta.size());
-        if ((e = hipMemcpy(c->tw_a, ta.data(), ta.size() * sizeof(double2), hipMemcpyHostToDevice)) != hipSuccess) return bad(e, "twiddles");
-        auto tb = twiddles<double2, double>(c->k1.video.B);
-        ALLOC(c->tw_b, tb.size());
-        if ((e = hipMemcpy(c->tw_b, tb.data(), tb.size() * sizeof(double2), hipMemcpyHostToDevice)) != hipSuccess) return bad(e, "twiddles");
+        if ((e = upload_twiddles(c, &c->tw_a, c->k1.rf.B, c->k1_f32)) != hipSuccess) return bad(e, "twiddles");
+        if ((e = upload_twiddles(c, &c->tw_b, c->k1.video.B, c->k1_f32)) != hipSuccess) return bad(e, "twiddles");
         auto tp = twiddles<float2, float>(c->pre.bpf.B);
         ALLOC(c->tw_p, tp.size());
         if ((e = hipMemcpy(c->tw_p, tp.data(), tp.size() * sizeof(float2), hipMemcpyHostToDevice)) != hipSuccess) return bad(e, "twiddles");

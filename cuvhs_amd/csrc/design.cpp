@@ -136,10 +136,13 @@ void kernel_support(const std::vector<cdbl>& g, double eps, int* lm, int* lp) {
     }
 }
 
-// Block size choice: FFT work per valid output, credited when two workgroups
-// fit in one CU's 160 KB of LDS (measured on MI355X: the FP64 4096-point
-// blocks of K1 run ~1.3x more work per unit time at two workgroups per CU
-// than the 8192-point blocks at one, from better latency hiding).
+// Block size choice: FFT work per valid output, credited by how many
+// workgroups fit in one CU's 160 KB of LDS.  The transforms are latency-bound
+// (a barrier per stage), and more independent workgroups per CU hide more of
+// it: measured on MI355X, FP64 4096-point K1 blocks (two per CU) run ~1.3x
+// more work per unit time than 8192-point blocks (one per CU), and FP32
+// 4096-point blocks (four per CU) ~1.45x more than FP32 8192-point blocks
+// (two per CU) with the same waves per CU.
 BlockFilter make_block(const std::vector<cdbl>& g, double eps, const int* bch, int nb, int extra, int elem_bytes) {
     int n = (int)g.size();
     int lm = 0, lp = 0;
@@ -151,7 +154,8 @@ BlockFilter make_block(const std::vector<cdbl>& g, double eps, const int* bch, i
         int v = B - lm - lp - extra;
         if (v < 256) continue;
         double lds = (double)(B + B / 16) * (double)elem_bytes;
-        double occ = lds <= 80.0 * 1024.0 ? 1.3 : 1.0;
+        int per_cu = (int)(160.0 * 1024.0 / lds);
+        double occ = per_cu >= 4 ? 1.9 : (per_cu == 3 ? 1.6 : (per_cu == 2 ? 1.3 : 1.0));
         double cost = (double)B * log2((double)B) / (double)v / occ;
         if (cost < best) { best = cost; bf.B = B; bf.valid = v; }
     }
@@ -223,7 +227,7 @@ void host_fft(std::vector<cdbl>& x, int sign) {
     if (a != x.data()) std::copy(a, a + n, x.data());
 }
 
-K1Design design_k1(const Fmt& fmt, double eps, const int* bch, int nb, int extra_a) {
+K1Design design_k1(const Fmt& fmt, double eps, const int* bch, int nb, int extra_a, int elem_bytes) {
     K1Design d;
     d.N = next_7smooth(fmt.spfp + kOverlap);
     const int N = d.N, fb = N / 2 + 1;
@@ -239,7 +243,7 @@ K1Design design_k1(const Fmt& fmt, double eps, const int* bch, int nb, int extra
                           butter_hpf_mag(om, w_hpf, 8) * hil;
     }
     host_fft(spec, +1);  // unnormalised inverse: the analytic signal carries the N scale
-    d.rf = make_block(spec, eps, bch, nb, extra_a, 16);
+    d.rf = make_block(spec, eps, bch, nb, extra_a, elem_bytes);
 
     double bs[3], as[3];
     shelf_high(273755.82, 13.9794, fs, 0.462088186, bs, as);
@@ -259,7 +263,7 @@ K1Design design_k1(const Fmt& fmt, double eps, const int* bch, int nb, int extra
     std::vector<double> g5 = real_kernel_from_half(f05, N);
     std::vector<cdbl> gc((size_t)N);
     for (int i = 0; i < N; i++) gc[(size_t)i] = cdbl(gv[(size_t)i], g5[(size_t)i]);
-    d.video = make_block(gc, eps, bch, nb, 0, 16);
+    d.video = make_block(gc, eps, bch, nb, 0, elem_bytes);
     return d;
 }
 

@@ -254,4 +254,117 @@ __device__ __forceinline__ void fft_lds(typename C2<T>::type* buf, const typenam
     }
 }
 
+// ---- 4096-point convolution block with wave-local sub-transforms ----
+//
+// y = IFFT(FFT(x) * G) for B = 4096 = 16 x 256 on 256 threads (4 waves):
+//   X[k1 + 16 k2] = sum_n2 W256^(n2 k2) W4096^(n2 k1) sum_n1 x[256 n1 + n2] W16^(n1 k1)
+// Stage 1: thread t = n2 does the DFT16 over n1 on values it already holds
+// and the W4096^(t k1) twiddles, then stores row k1, column t.  One block
+// barrier.  Wave w then owns rows k1 = 4w..4w+3 (1024 contiguous elements)
+// and does their four 256-point transforms as 16 x 16 (lane (j, q): row
+// 4w + j, column class q) with a transpose through its own rows, needing only
+// wavefront-scope ordering.  The spectrum product happens there, in the
+// transposed bin order k = k1 + 16 c + 256 d (G is stored permuted so the
+// loads are coalesced: Gp[1024 w + 64 d + lane], see conv4096_gperm), and the
+// inverse retraces the same steps.  Two block barriers per block instead of
+// two per Stockham stage.
+
+// wavefront-scope ordering of LDS stores before later loads by other lanes
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// W4096^(e0 * {1, 2, 4, 8}) from the table.
+template <typename T>
+__device__ __forceinline__ void twiddle_base(typename C2<T>::type* wb, const typename C2<T>::type* tw, int e0) {
+#pragma unroll
+    for (int i = 0; i < 4; i++) wb[i] = tw[(e0 << i) & 4095];
+}
+
+// v[k] *= W4096^(S * e0 * k), k = 1..15 (S = -1 forward, +1 inverse), from
+// the four base powers of twiddle_base and eleven products.
+template <int S, typename T>
+__device__ __forceinline__ void twiddle_run(typename C2<T>::type* v, const typename C2<T>::type* wb) {
+    using V = typename C2<T>::type;
+    V w[16];
+    w[1] = wb[0];
+    w[2] = wb[1];
+    w[4] = wb[2];
+    w[8] = wb[3];
+    if (S > 0) {
+        w[1] = cconj(w[1]);
+        w[2] = cconj(w[2]);
+        w[4] = cconj(w[4]);
+        w[8] = cconj(w[8]);
+    }
+    w[3] = cmul(w[1], w[2]);
+    w[5] = cmul(w[4], w[1]);
+    w[6] = cmul(w[4], w[2]);
+    w[7] = cmul(w[4], w[3]);
+#pragma unroll
+    for (int r = 9; r < 16; r++) w[r] = cmul(w[8], w[r - 8]);
+#pragma unroll
+    for (int r = 1; r < 16; r++) v[r] = cmul(v[r], w[r]);
+}
+
+// Host-side bin permutation of a natural-order spectrum for conv4096.
+inline int conv4096_gperm(int k) {
+    int k1 = k & 15, c = (k >> 4) & 15, d = k >> 8;
+    return (k1 >> 2) * 1024 + d * 64 + (k1 & 3) * 16 + c;
+}
+
+// In: v[n1] = x[256 n1 + t].  Out: v[n1] = y[256 n1 + t].  buf: lds_elems(4096)
+// elements; the caller synchronises before reusing it.
+template <typename T>
+__device__ __forceinline__ void conv4096(typename C2<T>::type* buf, typename C2<T>::type* v,
+                                         const typename C2<T>::type* tw, const typename C2<T>::type* Gp) {
+    using V = typename C2<T>::type;
+    const int t = threadIdx.x;
+    const int w = t >> 6, l = t & 63, q = l & 15;
+    const int base = ((w << 2) + (l >> 4)) << 8;
+    V wb[4];
+    dft16<-1, T>(v);
+    twiddle_base<T>(wb, tw, t);
+    twiddle_run<-1, T>(v, wb);
+#pragma unroll
+    for (int r = 0; r < 16; r++) buf[pidx((r << 8) + t)] = v[r];
+    __syncthreads();
+    // 256-point transforms: n2 = 16 a + b, k2 = c + 16 d
+#pragma unroll
+    for (int a = 0; a < 16; a++) v[a] = buf[pidx(base + 16 * a + q)];          // lane q = b
+    dft16<-1, T>(v);                                                            // a -> c
+    twiddle_base<T>(wb, tw, 16 * q);
+    twiddle_run<-1, T>(v, wb);                                                  // W256^(b c)
+#pragma unroll
+    for (int c = 0; c < 16; c++) buf[pidx(base + 16 * c + q)] = v[c];
+    wave_lds_sync();
+#pragma unroll
+    for (int b = 0; b < 16; b++) v[b] = buf[pidx(base + 16 * q + b)];          // lane q = c
+    dft16<-1, T>(v);                                                            // b -> d
+    V g[16];
+#pragma unroll
+    for (int d = 0; d < 16; d++) g[d] = Gp[(w << 10) + (d << 6) + l];
+#pragma unroll
+    for (int d = 0; d < 16; d++) v[d] = cmul(v[d], g[d]);
+    dft16<+1, T>(v);                                                            // d -> b
+    twiddle_base<T>(wb, tw, 16 * q);
+    twiddle_run<+1, T>(v, wb);
+#pragma unroll
+    for (int b = 0; b < 16; b++) buf[pidx(base + 16 * q + b)] = v[b];
+    wave_lds_sync();
+#pragma unroll
+    for (int c = 0; c < 16; c++) v[c] = buf[pidx(base + 16 * c + q)];          // lane q = b
+    dft16<+1, T>(v);                                                            // c -> a
+#pragma unroll
+    for (int a = 0; a < 16; a++) buf[pidx(base + 16 * a + q)] = v[a];
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 16; r++) v[r] = buf[pidx((r << 8) + t)];
+    twiddle_base<T>(wb, tw, t);
+    twiddle_run<+1, T>(v, wb);
+    dft16<+1, T>(v);
+}
+
 }  // namespace cuvhs

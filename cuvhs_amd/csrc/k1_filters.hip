@@ -72,11 +72,26 @@ __global__ __launch_bounds__(256) void k_input_mean(const double* partial, long 
 // (the block start is within one segment and a block is shorter than n).
 __device__ __forceinline__ int wrap_n(int q, int n) { return q < 0 ? q + n : (q >= n ? q - n : q); }
 
-// Part A: analytic signal, envelope and instantaneous frequency.
-template <int B>
-__global__ __launch_bounds__(B / 16) void k1_part_a(K1ALaunch a) {
+// Workgroups per CU the register allocation must allow: four FP32
+// 4096-point blocks fit one CU's LDS (35 KB each), so their kernels are held
+// to 128 VGPRs; the FP64 ones are limited to two by LDS anyway.
+template <int B, typename T>
+constexpr int k1_min_blocks() { return (B == 4096 && sizeof(T) == 4) ? 4 : 1; }
+
+// Phase of z = x + iy in the transform's precision.
+__device__ __forceinline__ double phase_of(double y, double x) { return atan2(y, x); }
+__device__ __forceinline__ float phase_of(float y, float x) { return atan2f(y, x); }
+
+// Part A: analytic signal, envelope and instantaneous frequency.  T is the
+// transform precision: double (the reference's FP64 arithmetic) or float
+// (the FP32 form); outputs are stored as double either way.
+template <int B, typename T>
+__global__ __launch_bounds__(B / 16, (k1_min_blocks<B, T>())) void k1_part_a(K1ALaunch a) {
+    using V2 = typename C2<T>::type;
     extern __shared__ __align__(16) unsigned char smem[];
-    double2* buf = reinterpret_cast<double2*>(smem);
+    V2* buf = reinterpret_cast<V2*>(smem);
+    const V2* G = static_cast<const V2*>(a.G);
+    const V2* tw = static_cast<const V2*>(a.tw);
     constexpr int NT = B / 16;
     const int f = blockIdx.y;
     const int b = blockIdx.x;
@@ -85,36 +100,79 @@ __global__ __launch_bounds__(B / 16) void k1_part_a(K1ALaunch a) {
     const int seg0 = f * a.spfp - kOverlap;
     const int last = (int)a.total - 1;
     const double dcv = a.dc ? *a.dc : 0.0;
-    double xin[16];
+    T xin[16];
 #pragma unroll
     for (int i = 0; i < 16; i++) {
         int s = seg0 + wrap_n(qs - a.lp + (int)threadIdx.x + i * NT, a.N);
         s = s < 0 ? 0 : (s > last ? last : s);
-        xin[i] = raw_sample(a.cap, a.in_fmt, s, a.n_valid, dcv);
+        xin[i] = (T)raw_sample(a.cap, a.in_fmt, s, a.n_valid, dcv);
     }
-    double2 v[16];
+    V2 v[16];
 #pragma unroll
     for (int i = 0; i < 16; i++) {
         v[i].x = xin[i];
-        v[i].y = 0.0;
+        v[i].y = (T)0;
     }
-    fft_fwd_regs_mul<double, B>(buf, v, a.tw, a.G);
-    fft_lds<+1, double, B>(buf, a.tw);
+    const int qend = kOverlap + a.spfp;  // one past the last demod output
+    const T PI = (T)3.14159265358979323846;
+    const T TAU = (T)6.28318530717958647692;
+    if constexpr (B == 4096) {
+        // G is in conv4096's bin order; v[r] becomes a(qs + o), o = 256 r + t - lp
+        conv4096<T>(buf, v, tw, G);
+        __shared__ T edge[4][16];
+        const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+        T ang[16];
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            // only outputs 0..V are used (the rest of the block is the
+            // convolution's wrap-around)
+            int o = (r << 8) + t - a.lp;
+            ang[r] = (T)0;
+            if (o >= 0 && o <= a.V) {
+                ang[r] = phase_of(v[r].y, v[r].x);
+                int q = qs + o;
+                if (q >= kOverlap && q < qend)
+                    a.env[f * a.spfp + (q - kOverlap)] = (double)sqrt(v[r].x * v[r].x + v[r].y * v[r].y);
+            }
+        }
+        // the phase at o - 1 is held by lane - 1 of this wave, or for lane 0
+        // by lane 63 of the previous wave (of the previous row for t = 0)
+        if (lane == 63) {
+#pragma unroll
+            for (int r = 0; r < 16; r++) edge[wv][r] = ang[r];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            T prev = __shfl_up(ang[r], 1);
+            if (lane == 0) prev = wv > 0 ? edge[wv - 1][r] : (r > 0 ? edge[3][r - 1] : (T)0);
+            int o = (r << 8) + t - a.lp;
+            int q = qs + o;
+            if (o >= 1 && o <= a.V && q < qend) {
+                T delta = ang[r] - prev;
+                if (delta > PI) delta -= TAU;
+                if (delta < -PI) delta += TAU;
+                a.rawdemod[f * a.spfp + (q - kOverlap)] = (double)delta * a.scale;
+            }
+        }
+        return;
+    }
+    fft_fwd_regs_mul<T, B>(buf, v, tw, G);
+    fft_lds<+1, T, B>(buf, tw);
     // outputs o = 0..V: analytic a(qs + o) = buf[lp + o]
     constexpr int MAXO = (B + NT - 1) / NT;
-    double ang[MAXO];
-    const int qend = kOverlap + a.spfp;  // one past the last demod output
+    T ang[MAXO];
 #pragma unroll
     for (int i = 0; i < MAXO; i++) {
         int o = threadIdx.x + i * NT;
-        ang[i] = 0.0;
+        ang[i] = (T)0;
         if (o <= a.V) {
-            double2 z = buf[pidx(a.lp + o)];
-            ang[i] = atan2(z.y, z.x);
+            V2 z = buf[pidx(a.lp + o)];
+            ang[i] = phase_of(z.y, z.x);
             int q = qs + o;
             if (q >= kOverlap && q < qend) {
                 int s = f * a.spfp + (q - kOverlap);
-                a.env[s] = sqrt(z.x * z.x + z.y * z.y);
+                a.env[s] = (double)sqrt(z.x * z.x + z.y * z.y);
             }
         }
     }
@@ -125,19 +183,17 @@ __global__ __launch_bounds__(B / 16) void k1_part_a(K1ALaunch a) {
         if (o <= a.V) buf[pidx(a.lp + o)].x = ang[i];
     }
     __syncthreads();
-    const double PI = 3.14159265358979323846;
-    const double TAU = 2.0 * PI;
 #pragma unroll
     for (int i = 0; i < MAXO; i++) {
         int o = threadIdx.x + i * NT;
         if (o >= 1 && o <= a.V) {
             int q = qs + o;
             if (q < qend) {
-                double delta = ang[i] - buf[pidx(a.lp + o - 1)].x;
+                T delta = ang[i] - buf[pidx(a.lp + o - 1)].x;
                 if (delta > PI) delta -= TAU;
                 if (delta < -PI) delta += TAU;
                 int s = f * a.spfp + (q - kOverlap);
-                a.rawdemod[s] = delta * a.scale;
+                a.rawdemod[s] = (double)delta * a.scale;
             }
         }
     }
@@ -145,10 +201,13 @@ __global__ __launch_bounds__(B / 16) void k1_part_a(K1ALaunch a) {
 
 // Part B: video and sync filters in one complex transform
 // (real input, kernel g_video + i * g_sync -> real part demod, imag demod_05).
-template <int B>
-__global__ __launch_bounds__(B / 16) void k1_part_b(K1BLaunch a) {
+template <int B, typename T>
+__global__ __launch_bounds__(B / 16, (k1_min_blocks<B, T>())) void k1_part_b(K1BLaunch a) {
+    using V2 = typename C2<T>::type;
     extern __shared__ __align__(16) unsigned char smem[];
-    double2* buf = reinterpret_cast<double2*>(smem);
+    V2* buf = reinterpret_cast<V2*>(smem);
+    const V2* G = static_cast<const V2*>(a.G);
+    const V2* tw = static_cast<const V2*>(a.tw);
     constexpr int NT = B / 16;
     const int f = blockIdx.y;
     const int b = blockIdx.x;
@@ -157,28 +216,42 @@ __global__ __launch_bounds__(B / 16) void k1_part_b(K1BLaunch a) {
     if (qs >= qend) return;
     const int seg0 = f * a.spfp - kOverlap;
     const int last = (int)a.total - 1;
-    double xin[16];
+    T xin[16];
 #pragma unroll
     for (int i = 0; i < 16; i++) {
         int s = seg0 + wrap_n(qs - a.lp + (int)threadIdx.x + i * NT, a.N);
         s = s < 0 ? 0 : (s > last ? last : s);
-        xin[i] = a.rawdemod[s];
+        xin[i] = (T)a.rawdemod[s];
     }
-    double2 v[16];
+    V2 v[16];
 #pragma unroll
     for (int i = 0; i < 16; i++) {
         v[i].x = xin[i];
-        v[i].y = 0.0;
+        v[i].y = (T)0;
     }
-    fft_fwd_regs_mul<double, B>(buf, v, a.tw, a.G);
-    fft_lds<+1, double, B>(buf, a.tw);
+    if constexpr (B == 4096) {
+        conv4096<T>(buf, v, tw, G);
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            int o = (r << 8) + (int)threadIdx.x - a.lp;
+            int q = qs + o;
+            if (o >= 0 && o < a.V && q < qend) {
+                int s = f * a.spfp + (q - kOverlap);
+                a.demod[s] = (double)v[r].x;
+                a.demod05[s] = (double)v[r].y;
+            }
+        }
+        return;
+    }
+    fft_fwd_regs_mul<T, B>(buf, v, tw, G);
+    fft_lds<+1, T, B>(buf, tw);
     for (int o = threadIdx.x; o < a.V; o += NT) {
         int q = qs + o;
         if (q >= qend) break;
-        double2 z = buf[pidx(a.lp + o)];
+        V2 z = buf[pidx(a.lp + o)];
         int s = f * a.spfp + (q - kOverlap);
-        a.demod[s] = z.x;
-        a.demod05[s] = z.y;
+        a.demod[s] = (double)z.x;
+        a.demod05[s] = (double)z.y;
     }
 }
 
@@ -227,20 +300,20 @@ __global__ void k_fill_f64(double* p, long long n, double v) {
 }
 
 // ---- host launchers ----
-template <int B>
+template <int B, typename T>
 static hipError_t launch_a(const K1ALaunch& a, int nf, hipStream_t st) {
     int nb = (int)((a.spfp + a.V - 1) / a.V);
-    size_t sh = (size_t)lds_elems(B) * sizeof(double2);
-    (void)hipFuncSetAttribute((const void*)k1_part_a<B>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
-    hipLaunchKernelGGL(k1_part_a<B>, dim3(nb, nf), dim3(B / 16), sh, st, a);
+    size_t sh = (size_t)lds_elems(B) * sizeof(typename C2<T>::type);
+    (void)hipFuncSetAttribute((const void*)k1_part_a<B, T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
+    hipLaunchKernelGGL((k1_part_a<B, T>), dim3(nb, nf), dim3(B / 16), sh, st, a);
     return hipGetLastError();
 }
-template <int B>
+template <int B, typename T>
 static hipError_t launch_b(const K1BLaunch& a, int nf, hipStream_t st) {
     int nb = (int)((a.spfp + a.V - 1) / a.V);
-    size_t sh = (size_t)lds_elems(B) * sizeof(double2);
-    (void)hipFuncSetAttribute((const void*)k1_part_b<B>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
-    hipLaunchKernelGGL(k1_part_b<B>, dim3(nb, nf), dim3(B / 16), sh, st, a);
+    size_t sh = (size_t)lds_elems(B) * sizeof(typename C2<T>::type);
+    (void)hipFuncSetAttribute((const void*)k1_part_b<B, T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
+    hipLaunchKernelGGL((k1_part_b<B, T>), dim3(nb, nf), dim3(B / 16), sh, st, a);
     return hipGetLastError();
 }
 template <int B>
@@ -252,14 +325,18 @@ static hipError_t launch_pre(const PreLaunch& a, int nf, hipStream_t st) {
     return hipGetLastError();
 }
 
-hipError_t run_k1_part_a(int B, const K1ALaunch& a, int nf, hipStream_t st) {
-    if (B == 4096) return launch_a<4096>(a, nf, st);
-    if (B == 8192) return launch_a<8192>(a, nf, st);
+hipError_t run_k1_part_a(int B, int f32, const K1ALaunch& a, int nf, hipStream_t st) {
+    if (!f32 && B == 4096) return launch_a<4096, double>(a, nf, st);
+    if (!f32 && B == 8192) return launch_a<8192, double>(a, nf, st);
+    if (f32 && B == 4096) return launch_a<4096, float>(a, nf, st);
+    if (f32 && B == 8192) return launch_a<8192, float>(a, nf, st);
     return hipErrorInvalidValue;
 }
-hipError_t run_k1_part_b(int B, const K1BLaunch& a, int nf, hipStream_t st) {
-    if (B == 4096) return launch_b<4096>(a, nf, st);
-    if (B == 8192) return launch_b<8192>(a, nf, st);
+hipError_t run_k1_part_b(int B, int f32, const K1BLaunch& a, int nf, hipStream_t st) {
+    if (!f32 && B == 4096) return launch_b<4096, double>(a, nf, st);
+    if (!f32 && B == 8192) return launch_b<8192, double>(a, nf, st);
+    if (f32 && B == 4096) return launch_b<4096, float>(a, nf, st);
+    if (f32 && B == 8192) return launch_b<8192, float>(a, nf, st);
     return hipErrorInvalidValue;
 }
 hipError_t run_prebpf(int B, const PreLaunch& a, int nf, hipStream_t st) {

@@ -17,8 +17,8 @@ struct K1ALaunch {
     long long n_valid, total;
     int spfp, N, lm, lp, V;
     double scale;
-    const double2* G;
-    const double2* tw;
+    const void* G;              // block spectrum, double2 or float2 (the transform precision)
+    const void* tw;             // twiddles W_B^m, same type
     double* rawdemod;
     double* env;
 };
@@ -26,8 +26,8 @@ struct K1BLaunch {
     const double* rawdemod;
     long long total;
     int spfp, N, lm, lp, V;
-    const double2* G;
-    const double2* tw;
+    const void* G;              // as in K1ALaunch
+    const void* tw;
     double* demod;
     double* demod05;
 };
@@ -59,8 +59,9 @@ struct ChromaLaunch {
 };
 
 
-hipError_t run_k1_part_a(int B, const K1ALaunch& a, int nf, hipStream_t st);
-hipError_t run_k1_part_b(int B, const K1BLaunch& a, int nf, hipStream_t st);
+// f32: transform in FP32 (float2 spectra and twiddles) instead of FP64
+hipError_t run_k1_part_a(int B, int f32, const K1ALaunch& a, int nf, hipStream_t st);
+hipError_t run_k1_part_b(int B, int f32, const K1BLaunch& a, int nf, hipStream_t st);
 hipError_t run_prebpf(int B, const PreLaunch& a, int nf, hipStream_t st);
 hipError_t run_fill_f64(double* p, long long n, double v, hipStream_t st);
 hipError_t run_k3(const ChunkCtl* ctl, int nf_override, int max_fields, const Fmt& fmt, const int* starts,

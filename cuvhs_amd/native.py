@@ -19,6 +19,7 @@ E_ARG, E_HIP, E_ALLOC, E_UNSUPPORTED, E_CAPACITY = -1, -2, -3, -4, -5
 INPUT_U8, INPUT_S16, INPUT_U16 = 0, 1, 2
 INPUT_FORMATS = {"u8": INPUT_U8, "s16": INPUT_S16, "u16": INPUT_U16}
 LINE_LOCATORS = {"k2b": 0, "k3": 1}
+PRECISIONS = {"fp64": 0, "fp32": 1}
 DIAG_NAMES = ("k3_bad_spacing", "k3_isolated_spacing", "k3_large_jump", "k4_large_delta", "k4_isolated_jump",
               "k4_refined_sync_like", "k5_oob_pixels", "k5_bad_geom_lines", "k5_sync_like_pixels")
 W_VSYNC_OVERFLOW, W_NO_VSYNC, W_EOF = 1, 2, 4
@@ -191,9 +192,11 @@ class Decoder:
     """One decode context (Pipeline, pipeline.h:28-114) on a HIP device."""
 
     def __init__(self, sample_rate_mhz=28.0, batch_fields=64, device=0, input_format="u8", dc_correct=False,
-                 line_locator="k2b"):
+                 line_locator="k2b", precision="fp64"):
         self.input_format = INPUT_FORMATS[input_format]
-        cfg = Config(0, float(sample_rate_mhz), int(batch_fields), 0, None, int(device), self.input_format,
+        self.precision = precision
+        cfg = Config(0, float(sample_rate_mhz), int(batch_fields), PRECISIONS[precision], None, int(device),
+                     self.input_format,
                      1 if dc_correct else 0, LINE_LOCATORS[line_locator])
         h = _vp()
         rc = lib().cuvhs_create(ctypes.byref(cfg), ctypes.byref(h))

@@ -50,7 +50,11 @@ enum {
 };
 
 enum { CUVHS_SYSTEM_NTSC = 0 };
-enum { CUVHS_PRECISION_FP64 = 0 };
+/* Arithmetic of the K1 FM-demodulation transforms (fm_demod.cu, cuFFT FP64
+ * in the reference).  FP64: the reference's precision, the default.  FP32:
+ * K1's block transforms, phase and envelope in FP32 (its outputs are stored
+ * as FP64; every later stage is unchanged), the plain FP32 form. */
+enum { CUVHS_PRECISION_FP64 = 0, CUVHS_PRECISION_FP32 = 1 };
 /* Capture sample formats (InputFormat, video_format.h; RawReader::convert,
  * raw_reader.cpp:71-91): u8 -> (s - 128) * 256, s16 as is, u16 -> s - 32768.
  * 16-bit samples are in host byte order. */

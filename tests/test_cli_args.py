@@ -25,6 +25,7 @@ def test_help_exits_zero():
     (("--system", "PAL", "a.u8", "o"), "PAL is not supported"),
     (("--format", "s24", "a.u8", "o"), "Unknown format: s24"),
     (("--batch", "0", "a.u8", "o"), "--batch must be >= 1"),
+    (("--precision", "fp16", "a.u8", "o"), "Unknown precision: fp16"),
     (("a.u8", "o", "extra"), "Too many arguments"),
 ])
 def test_bad_arguments_exit_one(args, msg):

@@ -53,6 +53,8 @@ struct Slot {
     const void* chunk = nullptr;
     uint64_t n_read = 0, raw_offset = 0;
     int max_fields = 0, fk1 = 0;
+    // demod/d05/env hold K1 output in [0, k1_extent) and zeros beyond
+    long long k1_extent = 0;
 };
 
 struct cuvhs_ctx {
@@ -256,10 +258,14 @@ int k1_launch(cuvhs_ctx* c, Slot& s, const void* chunk, long long n_valid, int n
     b.demod = s.demod;
     b.demod05 = s.d05;
     HIPCHK(c, run_k1_part_b(c->k1.video.B, c->k1_f32, b, nfk1, st));
-    // beyond the demodulated region the buffers read as zero
-    HIPCHK(c, run_fill_f64(s.demod + T, c->buf_n - T, 0.0, st));
-    HIPCHK(c, run_fill_f64(s.d05 + T, c->buf_n - T, 0.0, st));
-    HIPCHK(c, run_fill_f64(s.env + T, c->buf_n - T, 0.0, st));
+    // beyond the demodulated region the buffers read as zero: only what an
+    // earlier, longer chunk left there needs clearing
+    if (s.k1_extent > T) {
+        HIPCHK(c, run_fill_f64(s.demod + T, s.k1_extent - T, 0.0, st));
+        HIPCHK(c, run_fill_f64(s.d05 + T, s.k1_extent - T, 0.0, st));
+        HIPCHK(c, run_fill_f64(s.env + T, s.k1_extent - T, 0.0, st));
+    }
+    s.k1_extent = T;
     return 0;
 }
 
@@ -334,16 +340,17 @@ int head_stage(cuvhs_ctx* c, Slot& s) {
     return 0;
 }
 
-struct OutCopy {
+struct OutArrays {
     uint16_t* luma;
     uint16_t* chroma;
     uint64_t* file_loc;
     int32_t *is_first, *phase_id, *do_count, *do_line, *do_startx, *do_endx;
 };
 
-// Tail stage on the tail stream; n is the field count the head produced, out
-// (optional) receives the chunk's outputs.
-int tail_stage(cuvhs_ctx* c, Slot& s, const OutCopy* out, int n) {
+// Tail stage on the tail stream.  out (optional): caller arrays that receive
+// the chunk's outputs in place of the context's, with room for the field
+// count the head stage produced.
+int tail_stage(cuvhs_ctx* c, Slot& s, const OutArrays* out) {
     const Fmt& f = c->fmt;
     hipStream_t st = c->serial ? c->stream : c->tail;
     HIPCHK(c, hipStreamWaitEvent(st, s.head_done, 0));
@@ -368,31 +375,31 @@ int tail_stage(cuvhs_ctx* c, Slot& s, const OutCopy* out, int n) {
     HIPCHK(c, run_hsync_refine(s.d05, c->ll, s.ctl, 0, s.max_fields, f, st));
     if (c->diag) HIPCHK(c, run_diag_k4(s.d05, c->ll_coarse, c->ll, s.ctl, s.max_fields, f, st));
     mark(c, s, 8, st);
-    HIPCHK(c, run_tbc_resample(s.demod, c->ll, c->luma, s.ctl, 0, s.max_fields, f, st));
+    // outputs go straight to the caller's arrays where given (the field count
+    // n was checked against their capacity), else to the context's
+    uint16_t* luma = out && out->luma ? out->luma : c->luma;
+    uint16_t* chroma = out && out->chroma ? out->chroma : c->chroma;
+    int* phase = out && out->phase_id ? out->phase_id : c->phase;
+    int* doc = out && out->do_count ? out->do_count : c->doc;
+    int* dol = out && out->do_line ? out->do_line : c->dol;
+    int* dos = out && out->do_startx ? out->do_startx : c->dos;
+    int* doe = out && out->do_endx ? out->do_endx : c->doe;
+    unsigned long long* floc = out && out->file_loc ? reinterpret_cast<unsigned long long*>(out->file_loc) : c->file_loc;
+    int* isf = out && out->is_first ? out->is_first : c->isf_out;
+    HIPCHK(c, run_tbc_resample(s.demod, c->ll, luma, s.ctl, 0, s.max_fields, f, st));
     if (c->diag) HIPCHK(c, run_diag_k5(s.demod, c->ll, s.ctl, s.max_fields, f, st));
     mark(c, s, 9, st);
     HIPCHK(c, hipStreamWaitEvent(st, s.pre1, 0));
     mark(c, s, 10, st);
-    HIPCHK(c, run_chroma(chroma_launch(c, s.ctl, 0, s.craw), c->state, s.max_fields, c->m0, c->m1, c->cs_tab, c->iq0, c->track,
-                         c->phase, 1, st));
+    ChromaLaunch cl = chroma_launch(c, s.ctl, 0, s.craw);
+    cl.out = chroma;
+    HIPCHK(c, run_chroma(cl, c->state, s.max_fields, c->m0, c->m1, c->cs_tab, c->iq0, c->track, phase, 1, st));
     mark(c, s, 11, st);
-    HIPCHK(c, run_dropouts(s.env, c->ll, s.ctl, 0, s.max_fields, f, c->do_partial, c->do_cls, c->luma, c->chroma,
-                           c->dol, c->dos, c->doe, c->doc, st));
+    HIPCHK(c, run_dropouts(s.env, c->ll, s.ctl, 0, s.max_fields, f, c->do_partial, c->do_cls, luma, chroma, dol, dos,
+                           doe, doc, st));
     mark(c, s, 12, st);
-    HIPCHK(c, run_field_meta(s.ctl, c->state, s.offsets, c->isf_k2b, c->file_loc, c->isf_out, st));
+    HIPCHK(c, run_field_meta(s.ctl, c->state, s.offsets, c->isf_k2b, floc, isf, st));
     mark(c, s, 13, st);
-    if (out && n > 0) {
-        const size_t tbc = (size_t)n * kFieldLines * kLineLen * 2, dsz = (size_t)n * kMaxDropouts * 4;
-        if (out->luma) HIPCHK(c, hipMemcpyAsync(out->luma, c->luma, tbc, hipMemcpyDeviceToDevice, st));
-        if (out->chroma) HIPCHK(c, hipMemcpyAsync(out->chroma, c->chroma, tbc, hipMemcpyDeviceToDevice, st));
-        if (out->file_loc) HIPCHK(c, hipMemcpyAsync(out->file_loc, c->file_loc, n * 8, hipMemcpyDeviceToDevice, st));
-        if (out->is_first) HIPCHK(c, hipMemcpyAsync(out->is_first, c->isf_out, n * 4, hipMemcpyDeviceToDevice, st));
-        if (out->phase_id) HIPCHK(c, hipMemcpyAsync(out->phase_id, c->phase, n * 4, hipMemcpyDeviceToDevice, st));
-        if (out->do_count) HIPCHK(c, hipMemcpyAsync(out->do_count, c->doc, n * 4, hipMemcpyDeviceToDevice, st));
-        if (out->do_line) HIPCHK(c, hipMemcpyAsync(out->do_line, c->dol, dsz, hipMemcpyDeviceToDevice, st));
-        if (out->do_startx) HIPCHK(c, hipMemcpyAsync(out->do_startx, c->dos, dsz, hipMemcpyDeviceToDevice, st));
-        if (out->do_endx) HIPCHK(c, hipMemcpyAsync(out->do_endx, c->doe, dsz, hipMemcpyDeviceToDevice, st));
-    }
     HIPCHK(c, hipMemcpyAsync(s.hctl, s.ctl, sizeof(ChunkCtl), hipMemcpyDeviceToHost, st));
     HIPCHK(c, hipEventRecord(s.tail_done, st));
     return 0;
@@ -684,7 +691,7 @@ int cuvhs_decode_chunk(cuvhs_ctx* c, const void* chunk, uint64_t avail, uint64_t
     info->status = h.status;
     info->next_raw_offset = h.next_raw_offset;
     if (h.nf > 0) {
-        r = tail_stage(c, s, nullptr, h.nf);
+        r = tail_stage(c, s, nullptr);
         if (r) { sync_all(c); return r; }
         r = finish_tail(c, s);
         if (r) return r;
@@ -736,7 +743,7 @@ int cuvhs_decode_capture(cuvhs_ctx* c, const void* cap, uint64_t cap_len, int ma
         warn |= s.hctl->status & ~CUVHS_W_EOF;
         if (nf == 0) break;
         if (total + nf > max_out) { rc = CUVHS_E_CAPACITY; break; }
-        OutCopy o{};
+        OutArrays o{};
         o.luma = d_luma ? d_luma + (size_t)total * tbc : nullptr;
         o.chroma = d_chroma ? d_chroma + (size_t)total * tbc : nullptr;
         o.file_loc = d_file_loc ? d_file_loc + total : nullptr;
@@ -746,7 +753,7 @@ int cuvhs_decode_capture(cuvhs_ctx* c, const void* cap, uint64_t cap_len, int ma
         o.do_line = d_do_line ? d_do_line + (size_t)total * kMaxDropouts : nullptr;
         o.do_startx = d_do_startx ? d_do_startx + (size_t)total * kMaxDropouts : nullptr;
         o.do_endx = d_do_endx ? d_do_endx + (size_t)total * kMaxDropouts : nullptr;
-        if ((rc = tail_stage(c, s, &o, nf)) != 0) break;
+        if ((rc = tail_stage(c, s, &o)) != 0) break;
         total += nf;
         // the other slot is free once the previous chunk's tail has finished
         if (pending >= 0) {

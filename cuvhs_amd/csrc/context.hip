@@ -39,6 +39,7 @@ struct Slot {
     double* env = nullptr;
     int* events = nullptr;
     int* wg_count = nullptr;
+    uint2* ev_masks = nullptr;
     int* wg_off = nullptr;
     unsigned long long* offsets = nullptr;
     ChunkCtl* ctl = nullptr;      // device
@@ -330,7 +331,7 @@ int head_stage(cuvhs_ctx* c, Slot& s) {
     if (r) return r;
     mark(c, s, 2, st);
     const long long T = (long long)s.fk1 * f.spfp;
-    HIPCHK(c, run_events(s.d05, T, f.pulse_threshold_hz, s.wg_count, s.wg_off, s.events, s.ctl, st));
+    HIPCHK(c, run_events(s.d05, T, f.pulse_threshold_hz, s.wg_count, s.wg_off, s.ev_masks, s.events, s.ctl, st));
     mark(c, s, 3, st);
     HIPCHK(c, run_vsync_cluster(s.events, s.ctl, T, f.vsync_min_w, f.vsync_max_w, s.fk1 * 20, c->cand, s.offsets,
                                 s.max_fields, f.samples_per_field, f.samples_per_line, s.fk1, 1, st));
@@ -577,6 +578,7 @@ int cuvhs_create(const cuvhs_config* cfg, cuvhs_ctx** out) {
         ALLOC(s.events, S + 2);
         ALLOC(s.wg_count, events_nwg(S) + 1);
         ALLOC(s.wg_off, events_nwg(S) + 1);
+        ALLOC(s.ev_masks, events_mask_count(S));
         ALLOC(s.offsets, Fz);
         ALLOC(s.craw, S);
         ALLOC(s.dc, 1);
@@ -872,7 +874,7 @@ int cuvhs_vsync_discover(cuvhs_ctx* c, const double* d05, int total, int* idx, i
     if (!c || !d05 || !idx || !count || total < 1 || cap < 0 || total > (long long)c->F * c->fmt.spfp) return CUVHS_E_ARG;
     const Fmt& f = c->fmt;
     Slot& s = c->slot[0];
-    HIPCHK(c, run_events(d05, total, f.pulse_threshold_hz, s.wg_count, s.wg_off, s.events, s.ctl, c->stream));
+    HIPCHK(c, run_events(d05, total, f.pulse_threshold_hz, s.wg_count, s.wg_off, s.ev_masks, s.events, s.ctl, c->stream));
     HIPCHK(c, run_vsync_cluster(s.events, s.ctl, total, f.vsync_min_w, f.vsync_max_w, cap, idx, s.offsets, 1,
                                 f.samples_per_field, f.samples_per_line, 1, 0, c->stream));
     HIPCHK(c, hipMemcpyAsync(s.hctl, s.ctl, sizeof(ChunkCtl), hipMemcpyDeviceToHost, c->stream));
@@ -886,7 +888,7 @@ int cuvhs_sync_pulses(cuvhs_ctx* c, const double* d05, int total, const uint64_t
     if (!c || !d05 || !offsets || nf < 1 || total < 1 || total > (long long)c->F * c->fmt.spfp) return CUVHS_E_ARG;
     const Fmt& f = c->fmt;
     Slot& s = c->slot[0];
-    HIPCHK(c, run_events(d05, total, f.pulse_threshold_hz, s.wg_count, s.wg_off, s.events, s.ctl, c->stream));
+    HIPCHK(c, run_events(d05, total, f.pulse_threshold_hz, s.wg_count, s.wg_off, s.ev_masks, s.events, s.ctl, c->stream));
     HIPCHK(c, run_sync_pulses(s.events, s.ctl, (const unsigned long long*)offsets, f.spfp, starts, lens, count, nf,
                               nf, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));

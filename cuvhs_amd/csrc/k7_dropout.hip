@@ -36,8 +36,12 @@ __device__ __forceinline__ int find_tbc_line(const double* ll, int als, int n_li
     return lo;
 }
 
-constexpr int kSumSlices = 16;
-constexpr int kClassSlices = 32;
+// Workgroups per field for the envelope sum and the block classes.  Only the
+// fields whose scan window is non-empty do work (a dozen or so per chunk, the
+// window quirk of dropout_detect.cu:106-114), so each is split finely enough
+// to spread over the whole GPU.
+constexpr int kSumSlices = 128;
+constexpr int kClassSlices = 128;
 
 struct DoWindow {
     int start, end;
@@ -93,7 +97,10 @@ __global__ __launch_bounds__(kDoThreads) void k_do_class(const double* __restric
     if (f >= nf) return;
     DoWindow w = do_window(linelocs, f, spfp, als);
     if (w.end <= w.start) return;
-    const double down = field_mean(partial, f, spfp) * 0.18;
+    __shared__ double s_mean;
+    if (threadIdx.x == 0) s_mean = field_mean(partial, f, spfp);
+    __syncthreads();
+    const double down = s_mean * 0.18;
     const double up = down * 1.25;
     const int nblk = min((w.end - w.start + kEnvBlock - 1) / kEnvBlock, kMaxClassBlocks);
     const int per = (nblk + kClassSlices - 1) / kClassSlices;

@@ -74,8 +74,9 @@ hipError_t run_diag_k5(const double* demod, const double* ll, ChunkCtl* ctl, int
 hipError_t run_input_mean(const void* cap, int in_fmt, long long n, double* partial, double* mean, hipStream_t st);
 hipError_t run_fft_selftest(int B, int is_f32, const void* in, void* out, const void* tw, int sign, hipStream_t st);
 int events_nwg(long long total);
-hipError_t run_events(const double* x, long long total, double thr, int* wg_count, int* wg_off, int* events,
-                      ChunkCtl* ctl, hipStream_t st);
+size_t events_mask_count(long long total);  // uint2 entries of the masks buffer
+hipError_t run_events(const double* x, long long total, double thr, int* wg_count, int* wg_off, uint2* masks,
+                      int* events, ChunkCtl* ctl, hipStream_t st);
 hipError_t run_vsync_cluster(const int* events, ChunkCtl* ctl, long long total, int min_w, int max_w, int cap,
                              int* cand, unsigned long long* offsets, int num_fields, int spf, int spl, int fields_k1,
                              int do_cluster, hipStream_t st);

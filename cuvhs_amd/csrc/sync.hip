@@ -34,18 +34,25 @@ __device__ __forceinline__ int event_at(const double* x, long long i, double thr
 }
 
 // Sample i of workgroup g, iteration k, thread t: g*kEvSpan + k*256 + t, so
-// each load instruction of a wavefront reads 64 consecutive doubles.
-__global__ __launch_bounds__(kEvThreads) void k_ev_count(const double* x, long long total, double thr, int* wg_count) {
+// each load instruction of a wavefront reads 64 consecutive doubles.  Each
+// thread's start/end events are kept as two 32-bit masks (bit k: iteration
+// k) for k_ev_write, which so never reads the signal again.
+__global__ __launch_bounds__(kEvThreads) void k_ev_count(const double* x, long long total, double thr, int* wg_count,
+                                                         uint2* masks) {
     __shared__ int s_total;
     if (threadIdx.x == 0) s_total = 0;
     __syncthreads();
     const long long base = (long long)blockIdx.x * kEvSpan + threadIdx.x;
-    int c = 0;
+    unsigned int fs = 0, fe = 0;
 #pragma unroll 8
     for (int k = 0; k < kEvPerThread; k++) {
         long long i = base + (long long)k * kEvThreads;
-        if (i < total) c += event_at(x, i, thr) != 0;
+        int e = i < total ? event_at(x, i, thr) : 0;
+        if (e == 1) fs |= 1u << k;
+        if (e == 2) fe |= 1u << k;
     }
+    masks[(size_t)blockIdx.x * kEvThreads + threadIdx.x] = make_uint2(fs, fe);
+    int c = __popc(fs | fe);
     // wavefront sum, then one LDS atomic per wavefront
     for (int off = 32; off > 0; off >>= 1) c += __shfl_down(c, off, 64);
     if ((threadIdx.x & 63) == 0) atomicAdd(&s_total, c);
@@ -81,20 +88,13 @@ __global__ __launch_bounds__(1024) void k_ev_scan(const int* wg_count, int nwg, 
 // events encoded as 2*pos + is_end, ascending.  Order inside a workgroup is
 // (iteration k, wavefront w, lane) = ascending sample index; the offsets come
 // from per-(k, w) ballot counts.
-__global__ __launch_bounds__(kEvThreads) void k_ev_write(const double* x, long long total, double thr,
-                                                         const int* wg_off, int* events) {
+__global__ __launch_bounds__(kEvThreads) void k_ev_write(const uint2* masks, const int* wg_off, int* events) {
     constexpr int kWaves = kEvThreads / 64;
     __shared__ int s_cnt[kEvPerThread][kWaves];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const long long base = (long long)blockIdx.x * kEvSpan + threadIdx.x;
-    unsigned int fs = 0, fe = 0;
-#pragma unroll 8
-    for (int k = 0; k < kEvPerThread; k++) {
-        long long i = base + (long long)k * kEvThreads;
-        int e = i < total ? event_at(x, i, thr) : 0;
-        if (e == 1) fs |= 1u << k;
-        if (e == 2) fe |= 1u << k;
-    }
+    const uint2 msk = masks[(size_t)blockIdx.x * kEvThreads + threadIdx.x];
+    const unsigned int fs = msk.x, fe = msk.y;
     const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     for (int k = 0; k < kEvPerThread; k++) {
         unsigned long long m = __ballot(((fs | fe) >> k) & 1u);
@@ -263,13 +263,15 @@ __global__ __launch_bounds__(64) void k_sync_pulses(const int* events, const Chu
 // ---- host launchers ----
 int events_nwg(long long total) { return (int)((total + kEvSpan - 1) / kEvSpan); }
 
-hipError_t run_events(const double* x, long long total, double thr, int* wg_count, int* wg_off, int* events,
-                      ChunkCtl* ctl, hipStream_t st) {
+size_t events_mask_count(long long total) { return (size_t)events_nwg(total) * kEvThreads; }
+
+hipError_t run_events(const double* x, long long total, double thr, int* wg_count, int* wg_off, uint2* masks,
+                      int* events, ChunkCtl* ctl, hipStream_t st) {
     int nwg = events_nwg(total);
     if (nwg == 0) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_ev_count, dim3(nwg), dim3(kEvThreads), 0, st, x, total, thr, wg_count);
+    hipLaunchKernelGGL(k_ev_count, dim3(nwg), dim3(kEvThreads), 0, st, x, total, thr, wg_count, masks);
     hipLaunchKernelGGL(k_ev_scan, dim3(1), dim3(1024), 0, st, wg_count, nwg, wg_off, ctl);
-    hipLaunchKernelGGL(k_ev_write, dim3(nwg), dim3(kEvThreads), 0, st, x, total, thr, wg_off, events);
+    hipLaunchKernelGGL(k_ev_write, dim3(nwg), dim3(kEvThreads), 0, st, masks, wg_off, events);
     return hipGetLastError();
 }
 

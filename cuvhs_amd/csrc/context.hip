@@ -79,6 +79,7 @@ struct cuvhs_ctx {
     double2* tw_1024 = nullptr;
     double* cs_tab = nullptr;       // cos/sin of the burst phase detector
     double2* chroma_mask = nullptr; // per-line band-pass: (1, 0) kept bins, (0, 0) others
+    float* chroma_filt = nullptr;   // band-passed chroma lines of a chunk (track 0)
     long long buf_n = 0;            // F * spfp + one segment of zero padding
     Slot slot[2];
     double* rawdemod = nullptr;     // head only
@@ -220,6 +221,7 @@ ChromaLaunch chroma_launch(cuvhs_ctx* c, const ChunkCtl* ctl, int nf_override, c
     chroma_band(c->fmt, &a.lo, &a.hi);
     a.mask = c->chroma_mask;
     a.burst = c->burst;
+    a.filt = c->chroma_filt;
     a.out = c->chroma;
     a.tw = c->tw_1024;
     return a;
@@ -612,6 +614,7 @@ int cuvhs_create(const cuvhs_config* cfg, cuvhs_ctx** out) {
     ALLOC(c->luma, Fz * kFieldLines * kLineLen);
     ALLOC(c->chroma, Fz * kFieldLines * kLineLen);
     ALLOC(c->burst, Fz * kFieldLines * 36);
+    ALLOC(c->chroma_filt, Fz * kFieldLines * kLineLen);
     ALLOC(c->m0, Fz);
     ALLOC(c->m1, Fz);
     ALLOC(c->iq0, 2 * kFieldLines);

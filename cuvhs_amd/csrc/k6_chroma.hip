@@ -9,11 +9,12 @@
 // Here the decision logic needs only per-field burst metrics for both track
 // hypotheses.  For even output lines the heterodyne phase is the same for
 // both tracks; for odd lines switching track adds pi, i.e. negates the
-// line.  The band-pass is linear, so one pass over lines 16..246 with track 0
-// gives the track-0 metric (|a + b|) and the track-1 metric (|a - b|) of
-// every field.  A single-thread kernel then replays the reference's
-// detection, retry and state logic exactly, and the output pass recomputes
-// every line with the resolved track.
+// line.  The band-pass is linear, so one pass over all lines with track 0
+// gives every band-passed line for either track up to its sign, the
+// track-0 metric (|a + b|) and the track-1 metric (|a - b|) of every field.
+// A single-thread kernel then replays the reference's detection, retry and
+// state logic exactly, and the output kernel applies the resolved track's
+// sign, the ACC gain and the u16 conversion to the stored lines.
 #include <hip/hip_runtime.h>
 
 #include "common.h"
@@ -61,19 +62,18 @@ __device__ __forceinline__ void chroma_het(const ChromaLaunch& a, int line, doub
     }
 }
 
-// One wavefront per pair of lines (la, la + 1): resample + heterodyne
-// (chroma_decode.cu:75-157), 1024-point FFT, band-pass (:163-182), inverse
-// FFT, then either burst capture or ACC + u16 (:188-231).  The two real
+// One wavefront per pair of lines (la, la + 1): resample + heterodyne with
+// track 0 (chroma_decode.cu:75-157), 1024-point FFT, band-pass (:163-182),
+// inverse FFT; the burst window (FP64) and the line (FP32, ~1e-3 LSB of the
+// u16 output) are stored for the metric and output kernels.  The two real
 // lines go in as the real and imaginary parts of one complex transform; the
 // band-pass mask is symmetric in frequency, so the band-passed line la comes
 // back as the real part and la + 1 as the imaginary part.  (The reference
 // transforms each line on its own; the results agree to rounding.)  Thread t
 // computes columns t + 64 r, which is the register layout the first FFT
 // stage takes, and the mask is applied in the last forward stage.
-template <int OUT>
 __global__ __launch_bounds__(64) void k_chroma_line(ChromaLaunch a) {
     __shared__ __align__(16) double2 buf[lds_elems(kChromaFFT)];
-    __shared__ double s_acc[2];
     const int f = blockIdx.y;
     const int nf = a.nf_override > 0 ? a.nf_override : a.ctl->nf;
     if (f >= nf) return;
@@ -83,8 +83,7 @@ __global__ __launch_bounds__(64) void k_chroma_line(ChromaLaunch a) {
     const bool has_b = lb < a.line_end;
     const long long total = (long long)nf * a.spfp;
     const double* ll = a.linelocs + (size_t)f * kLinesPerFrame;
-    const int tr = a.track ? a.track[f] : 0;
-    const int rot = tr ? 1 : 3;
+    const int rot = 3;  // track 0
     const double ls_a = ll[la], le_a = ll[la + 1];
     const double ph_a = (double)((la * rot + 0) & 3) * (M_PI * 0.5);
     double ls_b = 0.0, le_b = 0.0, ph_b = 0.0;
@@ -108,43 +107,45 @@ __global__ __launch_bounds__(64) void k_chroma_line(ChromaLaunch a) {
     }
     fft_fwd_regs_mul<double, kChromaFFT>(buf, v, a.tw, a.mask);
     fft_lds<+1, double, kChromaFFT>(buf, a.tw);
-    if (!OUT) {
-        double* da = a.burst + ((size_t)f * kFieldLines + la) * kBurstLen;
-        for (int i = threadIdx.x; i < kBurstLen; i += 64) {
-            double2 z = buf[pidx(a.burst_start + i)];
-            da[i] = z.x * a.fft_scale;
-            if (has_b) da[kBurstLen + i] = z.y * a.fft_scale;
-        }
-        return;
+    double* da = a.burst + ((size_t)f * kFieldLines + la) * kBurstLen;
+    for (int i = threadIdx.x; i < kBurstLen; i += 64) {
+        double2 z = buf[pidx(a.burst_start + i)];
+        da[i] = z.x * a.fft_scale;
+        if (has_b) da[kBurstLen + i] = z.y * a.fft_scale;
     }
-    if (threadIdx.x < 2 && (threadIdx.x == 0 || has_b)) {
-        double ssq = 0.0;
-        for (int i = 0; i < kBurstLen; i++) {
-            double2 z = buf[pidx(a.burst_start + i)];
-            double v1 = (threadIdx.x == 0 ? z.x : z.y) * a.fft_scale;
-            ssq += v1 * v1;
-        }
-        double rms = sqrt(ssq / (double)kBurstLen);
-        s_acc[threadIdx.x] = (rms > 1.0) ? (a.burst_abs_ref / rms) : 0.0;
-    }
-    __syncthreads();
-    const double acc_a = s_acc[0], acc_b = s_acc[1];
-    uint16_t* oa = a.out + ((size_t)f * kFieldLines + la) * kLineLen;
-    uint16_t* ob = oa + kLineLen;
+    float* fa = a.filt + ((size_t)f * kFieldLines + la) * kLineLen;
     for (int col = threadIdx.x; col < kLineLen; col += 64) {
         double2 z = buf[pidx(col)];
-        double va = z.x * a.fft_scale * acc_a;
-        double ov = va + 32768.0;
+        fa[col] = (float)(z.x * a.fft_scale);
+        if (has_b) fa[kLineLen + col] = (float)(z.y * a.fft_scale);
+    }
+}
+
+// ACC + u16 of every line (chroma_decode.cu:188-231) with the resolved track:
+// an odd line of track 1 is the negated track-0 line.  One workgroup per
+// (line, field).
+__global__ __launch_bounds__(256) void k_chroma_out(ChromaLaunch a) {
+    __shared__ double s_acc;
+    const int f = blockIdx.y, line = blockIdx.x;
+    const int nf = a.nf_override > 0 ? a.nf_override : a.ctl->nf;
+    if (f >= nf) return;
+    const size_t row = (size_t)f * kFieldLines + line;
+    if (threadIdx.x == 0) {
+        const double* da = a.burst + row * kBurstLen;
+        double ssq = 0.0;
+        for (int i = 0; i < kBurstLen; i++) ssq += da[i] * da[i];
+        double rms = sqrt(ssq / (double)kBurstLen);
+        s_acc = (rms > 1.0) ? (a.burst_abs_ref / rms) : 0.0;
+    }
+    __syncthreads();
+    const double acc = ((line & 1) && a.track[f]) ? -s_acc : s_acc;
+    const float* fa = a.filt + row * kLineLen;
+    uint16_t* o = a.out + row * kLineLen;
+    for (int col = threadIdx.x; col < kLineLen; col += 256) {
+        double ov = (double)fa[col] * acc + 32768.0;
         if (ov < 0.0) ov = 0.0;
         if (ov > 65535.0) ov = 65535.0;
-        oa[col] = (uint16_t)(ov + 0.5);
-        if (has_b) {
-            double vb = z.y * a.fft_scale * acc_b;
-            double ob_ = vb + 32768.0;
-            if (ob_ < 0.0) ob_ = 0.0;
-            if (ob_ > 65535.0) ob_ = 65535.0;
-            ob[col] = (uint16_t)(ob_ + 0.5);
-        }
+        o[col] = (uint16_t)(ov + 0.5);
     }
 }
 
@@ -280,21 +281,18 @@ __global__ void k_chroma_resolve(const ChunkCtl* ctl, DecodeState* dst, int nf_o
 hipError_t run_chroma(ChromaLaunch a, DecodeState* dst, int max_fields, double* m0, double* m1,
                       const double* cs_tab, double* iq0, int* out_track, int* phase_ids, int need_iq,
                       hipStream_t st) {
-    // metric pass: lines 16..246, track 0
+    // band-pass of every line with track 0; metrics from lines 16..246
     ChromaLaunch m = a;
-    m.track = nullptr;
-    m.line0 = kSkip;
-    m.line_end = kFieldLines - kSkip;
-    hipLaunchKernelGGL(k_chroma_line<0>, dim3((m.line_end - m.line0 + 1) / 2, max_fields), dim3(64), 0, st, m);
+    m.line0 = 0;
+    m.line_end = kFieldLines;
+    hipLaunchKernelGGL(k_chroma_line, dim3((m.line_end - m.line0 + 1) / 2, max_fields), dim3(64), 0, st, m);
     hipLaunchKernelGGL(k_chroma_metrics, dim3(max_fields), dim3(64), 0, st, a.burst, a.ctl,
                        a.nf_override, m0, m1, cs_tab, need_iq ? iq0 : nullptr);
     hipLaunchKernelGGL(k_chroma_resolve, dim3(1), dim3(64), 0, st, a.ctl, dst, a.nf_override, m0, m1, iq0, out_track,
                        phase_ids);
     ChromaLaunch o = a;
     o.track = out_track;
-    o.line0 = 0;
-    o.line_end = kFieldLines;
-    hipLaunchKernelGGL(k_chroma_line<1>, dim3((o.line_end - o.line0 + 1) / 2, max_fields), dim3(64), 0, st, o);
+    hipLaunchKernelGGL(k_chroma_out, dim3(kFieldLines, max_fields), dim3(256), 0, st, o);
     return hipGetLastError();
 }
 

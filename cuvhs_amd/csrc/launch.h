@@ -50,9 +50,10 @@ struct ChromaLaunch {
     double het_scale, fft_scale, burst_abs_ref;
     double het_dc, het_ds;      // cos / sin of 2 pi het_scale 64 (heterodyne step of 64 columns)
     int burst_start, lo, hi;
-    const int* track;
+    const int* track;           // resolved track per field (output kernel)
     int line0, line_end;        // lines [line0, line_end) of each field
-    double* burst;
+    double* burst;              // burst window of every line, track 0, scaled
+    float* filt;                // band-passed lines, track 0, scaled: [field][263][910]
     uint16_t* out;
     const double2* tw;
     const double2* mask;        // band-pass bins: (1, 0) kept, (0, 0) removed

@@ -153,12 +153,13 @@ std::vector<V> twiddles(int B) {
     return t;
 }
 
-// perm: store bin k at conv4096_gperm(k) (the 4096-point wave-local blocks)
+// perm: store bin k at conv4096_gperm(k) / conv16384_gperm(k) (the bin
+// order of the wave-local 4096- and 16384-point blocks)
 template <typename V, typename S>
 hipError_t upload_spectrum(cuvhs_ctx* c, V** dst, const std::vector<cdbl>& s, bool perm = false) {
     std::vector<V> h(s.size());
     for (size_t i = 0; i < s.size(); i++) {
-        size_t j = perm ? (size_t)conv4096_gperm((int)i) : i;
+        size_t j = !perm ? i : (s.size() == 4096 ? (size_t)conv4096_gperm((int)i) : (size_t)conv16384_gperm((int)i));
         h[j].x = (S)s[i].real();
         h[j].y = (S)s[i].imag();
     }
@@ -537,7 +538,8 @@ int cuvhs_create(const cuvhs_config* cfg, cuvhs_ctx** out) {
             if ((e = upload_spectrum<double2, double>(c, (double2**)&c->G_b, c->k1.video.spectrum, pb)) != hipSuccess) return bad(e, "G_b");
         }
     }
-    if ((e = upload_spectrum<float2, float>(c, &c->G_p, c->pre.bpf.spectrum)) != hipSuccess) return bad(e, "G_p");
+    if ((e = upload_spectrum<float2, float>(c, &c->G_p, c->pre.bpf.spectrum, c->pre.bpf.B == 16384)) != hipSuccess)
+        return bad(e, "G_p");
     {
         if ((e = upload_twiddles(c, &c->tw_a, c->k1.rf.B, c->k1_f32)) != hipSuccess) return bad(e, "twiddles");
         if ((e = upload_twiddles(c, &c->tw_b, c->k1.video.B, c->k1_f32)) != hipSuccess) return bad(e, "twiddles");

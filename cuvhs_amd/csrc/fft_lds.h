@@ -276,14 +276,14 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// W4096^(e0 * {1, 2, 4, 8}) from the table.
-template <typename T>
+// WB^(e0 * {1, 2, 4, 8}) from the table of WB^m, m < B (B a power of two).
+template <typename T, int B = 4096>
 __device__ __forceinline__ void twiddle_base(typename C2<T>::type* wb, const typename C2<T>::type* tw, int e0) {
 #pragma unroll
-    for (int i = 0; i < 4; i++) wb[i] = tw[(e0 << i) & 4095];
+    for (int i = 0; i < 4; i++) wb[i] = tw[(e0 << i) & (B - 1)];
 }
 
-// v[k] *= W4096^(S * e0 * k), k = 1..15 (S = -1 forward, +1 inverse), from
+// v[k] *= WB^(S * e0 * k), k = 1..15 (S = -1 forward, +1 inverse), from
 // the four base powers of twiddle_base and eleven products.
 template <int S, typename T>
 __device__ __forceinline__ void twiddle_run(typename C2<T>::type* v, const typename C2<T>::type* wb) {
@@ -363,6 +363,102 @@ __device__ __forceinline__ void conv4096(typename C2<T>::type* buf, typename C2<
 #pragma unroll
     for (int r = 0; r < 16; r++) v[r] = buf[pidx((r << 8) + t)];
     twiddle_base<T>(wb, tw, t);
+    twiddle_run<+1, T>(v, wb);
+    dft16<+1, T>(v);
+}
+
+// ---- 16384-point convolution block, same scheme ----
+//
+// 16384 = 16 x 1024 on 1024 threads (16 waves).  Stage 1 as in conv4096
+// (thread t = n2 < 1024, rows k1 of 1024), one block barrier, then wave w
+// owns row k1 = w and does its 1024-point transform as 16 x 16 x 4 with two
+// transposes through its own row:
+//   n2 = 64 a + 4 b + g,  k2 = c + 16 e1 + 256 e2  (e1 = 4 e1h + e1l)
+//   pass A, lane m = 4 b + g: DFT16 over a -> c, twiddle W1024^(m c)
+//   pass B, lane 4 c + g:     DFT16 over b -> e1, twiddle W64^(g e1)
+//   pass C, lane 4 c + e1h:   four DFT4 over g -> e2 (held as v[4 e1l + g])
+// The spectrum is applied in pass C's order (Gp[1024 w + 64 j + lane],
+// j = 4 e1l + e2, see conv16384_gperm) and the inverse retraces the passes.
+
+inline int conv16384_gperm(int k) {
+    int k1 = k & 15, k2 = k >> 4;
+    int c = k2 & 15, e1 = (k2 >> 4) & 15, e2 = k2 >> 8;
+    int lane = 4 * c + (e1 >> 2), j = 4 * (e1 & 3) + e2;
+    return k1 * 1024 + j * 64 + lane;
+}
+
+// In: v[n1] = x[1024 n1 + t].  Out: v[n1] = y[1024 n1 + t].  buf:
+// lds_elems(16384) elements.  tw: the table of W16384^m.
+template <typename T>
+__device__ __forceinline__ void conv16384(typename C2<T>::type* buf, typename C2<T>::type* v,
+                                          const typename C2<T>::type* tw, const typename C2<T>::type* Gp) {
+    using V = typename C2<T>::type;
+    constexpr int NB = 16384;
+    const int t = threadIdx.x;
+    const int w = t >> 6, l = t & 63;
+    const int row = w << 10;
+    const int hi = l >> 2, lo = l & 3;      // (c, g) in pass B, (c, e1h) in pass C
+    V wb[4];
+    dft16<-1, T>(v);
+    twiddle_base<T, NB>(wb, tw, t);
+    twiddle_run<-1, T>(v, wb);
+#pragma unroll
+    for (int r = 0; r < 16; r++) buf[pidx((r << 10) + t)] = v[r];
+    __syncthreads();
+    // pass A: lane m = l holds n2 = 64 a + m
+#pragma unroll
+    for (int a = 0; a < 16; a++) v[a] = buf[pidx(row + 64 * a + l)];
+    dft16<-1, T>(v);                                               // a -> c
+    twiddle_base<T, NB>(wb, tw, 16 * l);
+    twiddle_run<-1, T>(v, wb);                                     // W1024^(m c)
+#pragma unroll
+    for (int c = 0; c < 16; c++) buf[pidx(row + 64 * c + l)] = v[c];
+    wave_lds_sync();
+    // pass B: lane (c = hi, g = lo) holds m = 4 b + g
+#pragma unroll
+    for (int b = 0; b < 16; b++) v[b] = buf[pidx(row + 64 * hi + 4 * b + lo)];
+    dft16<-1, T>(v);                                               // b -> e1
+    twiddle_base<T, NB>(wb, tw, 256 * lo);
+    twiddle_run<-1, T>(v, wb);                                     // W64^(g e1)
+#pragma unroll
+    for (int e1 = 0; e1 < 16; e1++) buf[pidx(row + 64 * hi + 4 * e1 + lo)] = v[e1];
+    wave_lds_sync();
+    // pass C: lane (c = hi, e1h = lo) holds (e1l, g) at v[4 e1l + g]
+#pragma unroll
+    for (int j = 0; j < 16; j++) v[j] = buf[pidx(row + 64 * hi + 16 * lo + j)];
+#pragma unroll
+    for (int e = 0; e < 4; e++) dft4<-1>(v[4 * e], v[4 * e + 1], v[4 * e + 2], v[4 * e + 3]);
+    V g[16];
+#pragma unroll
+    for (int j = 0; j < 16; j++) g[j] = Gp[(w << 10) + (j << 6) + l];
+#pragma unroll
+    for (int j = 0; j < 16; j++) v[j] = cmul(v[j], g[j]);
+#pragma unroll
+    for (int e = 0; e < 4; e++) dft4<+1>(v[4 * e], v[4 * e + 1], v[4 * e + 2], v[4 * e + 3]);
+#pragma unroll
+    for (int j = 0; j < 16; j++) buf[pidx(row + 64 * hi + 16 * lo + j)] = v[j];
+    wave_lds_sync();
+    // inverse pass B: lane (c, g)
+#pragma unroll
+    for (int e1 = 0; e1 < 16; e1++) v[e1] = buf[pidx(row + 64 * hi + 4 * e1 + lo)];
+    twiddle_base<T, NB>(wb, tw, 256 * lo);
+    twiddle_run<+1, T>(v, wb);
+    dft16<+1, T>(v);                                               // e1 -> b
+#pragma unroll
+    for (int b = 0; b < 16; b++) buf[pidx(row + 64 * hi + 4 * b + lo)] = v[b];
+    wave_lds_sync();
+    // inverse pass A: lane m
+#pragma unroll
+    for (int c = 0; c < 16; c++) v[c] = buf[pidx(row + 64 * c + l)];
+    twiddle_base<T, NB>(wb, tw, 16 * l);
+    twiddle_run<+1, T>(v, wb);
+    dft16<+1, T>(v);                                               // c -> a
+#pragma unroll
+    for (int a = 0; a < 16; a++) buf[pidx(row + 64 * a + l)] = v[a];
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 16; r++) v[r] = buf[pidx((r << 10) + t)];
+    twiddle_base<T, NB>(wb, tw, t);
     twiddle_run<+1, T>(v, wb);
     dft16<+1, T>(v);
 }

@@ -280,6 +280,20 @@ __global__ __launch_bounds__(B / 16) void k_prebpf(PreLaunch a) {
         v[i].x = n0 < a.spfp ? (float)raw_sample(a.cap, a.in_fmt, seg0 + n0, a.n_valid, dcv) : 0.0f;
         v[i].y = two && n1 < a.spfp ? (float)raw_sample(a.cap, a.in_fmt, seg0 + n1, a.n_valid, dcv) : 0.0f;
     }
+    if constexpr (B == 16384) {
+        // G is in conv16384's bin order; v[r] becomes the outputs at
+        // o = 1024 r + t - lp of both blocks
+        conv16384<float>(buf, v, a.tw, a.G);
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            int o = (r << 10) + (int)threadIdx.x - a.lp;
+            if (o >= 0 && o < a.V) {
+                if (ns0 + o < a.spfp) a.out[seg0 + ns0 + o] = v[r].x;
+                if (ns1 + o < a.spfp) a.out[seg0 + ns1 + o] = v[r].y;
+            }
+        }
+        return;
+    }
     // the spectrum product stays a separate pass here: fused into the last
     // (radix-4) stage it needs more than the 128 VGPRs a 1024-thread
     // workgroup allows and spills

@@ -50,6 +50,15 @@ __device__ __forceinline__ double wv_find_rising(const double* sig, int start, i
     return -1.0;
 }
 
+// The value lane k holds, for k the same in every lane: two scalar lane reads
+// instead of an LDS permute.
+__device__ __forceinline__ double lane_value(double v, int k) {
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, k);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), k);
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
 // The n/2-th smallest of up to 64 samples, which is what the reference's
 // partial selection sort leaves at buf[n/2] (hsync_refine.cu:72-97): lane j
 // holds sample j and counts its stable rank.
@@ -60,11 +69,11 @@ __device__ __forceinline__ double wv_window_median(const double* sig, int start,
     double v = lane < n ? sig[start + lane] : 0.0;
     int r = 0;
     for (int k = 0; k < n; k++) {
-        double vk = __shfl(v, k, 64);
+        double vk = lane_value(v, k);
         r += (vk < v) || (vk == v && k < lane);
     }
     unsigned long long m = __ballot(lane < n && r == mid);
-    return __shfl(v, __ffsll((long long)m) - 1, 64);
+    return lane_value(v, __ffsll((long long)m) - 1);
 }
 
 // One wavefront per line (hsync_refine.cu:99-239).

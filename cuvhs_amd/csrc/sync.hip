@@ -120,39 +120,50 @@ __global__ __launch_bounds__(kEvThreads) void k_ev_write(const uint2* masks, con
 
 // VSYNC discovery over the event list + chunk clustering (one workgroup).
 // offsets[] and ctl->{nf, next_raw_offset, vsync_count, status} are written.
-// The candidate compaction takes 1024 events per step: wavefront ballots give
-// each candidate its rank, the 16 wavefront counts are exchanged through LDS
-// (double-buffered, one barrier per step), and the next step's events are
-// loaded before the barrier.
+// The candidate compaction takes 4096 events per step, four consecutive ones
+// per thread (events alternate start/end, so at most two candidates each):
+// two ballots give each candidate its rank in the wavefront, the 16
+// wavefront counts are exchanged through LDS (double-buffered, one barrier
+// per step), and the next step's events are loaded before the barrier.  The
+// first kCandLds candidates are also kept in LDS for the clustering loop.
+constexpr int kCandLds = 4096;
 __global__ __launch_bounds__(1024) void k_vsync_cluster(const int* events, ChunkCtl* ctl, long long total,
                                                         int min_w, int max_w, int cap, int* cand,
                                                         unsigned long long* offsets, int num_fields, int spf,
                                                         int spl, int fields_k1, int do_cluster) {
     constexpr int kWaves = 1024 / 64;
     __shared__ int wcnt[2][kWaves];
+    __shared__ int lcand[kCandLds];
     const int nev = ctl->nevents;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     int carry_all = 0;
-    int k = threadIdx.x;
-    int e0 = k < nev ? events[k] : 0;
-    int e1 = k + 1 < nev ? events[k + 1] : 0;
-    for (int base = 0, it = 0; base < nev; base += 1024, it ^= 1) {
-        int is_c = 0;
-        int pos = 0;
-        if (k < nev && (e0 & 1) == 0) {
-            pos = e0 >> 1;
-            long long end = total;
-            if (k + 1 < nev) end = e1 >> 1;  // next event is this run's end
-            long long w = end - pos;
-            if (w > max_w + 1) w = max_w + 1;
-            is_c = (w >= min_w && w <= max_w) ? 1 : 0;
+    // event k of the step, or 0 past the end; e[4] is the next group's first
+    auto load = [&](int k0, int* e) {
+#pragma unroll
+        for (int q = 0; q < 5; q++) e[q] = k0 + q < nev ? events[k0 + q] : 0;
+    };
+    int k0 = 4 * (int)threadIdx.x;
+    int e[5], nx[5];
+    load(k0, e);
+    for (int base = 0, it = 0; base < nev; base += 4096, it ^= 1) {
+        int pos[2] = {0, 0};
+        int c = 0;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int k = k0 + q;
+            if (k < nev && (e[q] & 1) == 0) {
+                const int p = e[q] >> 1;
+                long long end = total;
+                if (k + 1 < nev) end = e[q + 1] >> 1;  // next event is this run's end
+                long long w = end - p;
+                if (w > max_w + 1) w = max_w + 1;
+                if (w >= min_w && w <= max_w) pos[c++ & 1] = p;
+            }
         }
-        const int kn = k + 1024;
-        const int n0 = kn < nev ? events[kn] : 0;
-        const int n1 = kn + 1 < nev ? events[kn + 1] : 0;
-        const unsigned long long m = __ballot(is_c);
-        if (lane == 0) wcnt[it][wave] = __popcll(m);
+        load(k0 + 4096, nx);
+        const unsigned long long m0 = __ballot(c >= 1), m1 = __ballot(c >= 2);
+        if (lane == 0) wcnt[it][wave] = __popcll(m0) + __popcll(m1);
         __syncthreads();
         int woff = 0, tot = 0;
 #pragma unroll
@@ -161,12 +172,15 @@ __global__ __launch_bounds__(1024) void k_vsync_cluster(const int* events, Chunk
             woff += w < wave ? v : 0;
             tot += v;
         }
-        const int idx = carry_all + woff + __popcll(m & lt);
-        if (is_c && idx < cap) cand[idx] = pos;
+        const int idx = carry_all + woff + __popcll(m0 & lt) + __popcll(m1 & lt);
+        for (int q = 0; q < c; q++) {
+            if (idx + q < cap) cand[idx + q] = pos[q];
+            if (idx + q < kCandLds) lcand[idx + q] = pos[q];
+        }
         carry_all += tot;
-        k = kn;
-        e0 = n0;
-        e1 = n1;
+        k0 += 4096;
+#pragma unroll
+        for (int q = 0; q < 5; q++) e[q] = nx[q];
     }
     __syncthreads();
     if (threadIdx.x != 0 || !do_cluster) {
@@ -182,7 +196,7 @@ __global__ __launch_bounds__(1024) void k_vsync_cluster(const int* events, Chunk
     int n = 0;
     unsigned long long last = 0;
     for (int i = 0; i < stored; i++) {
-        int pos = cand[i];
+        int pos = i < kCandLds ? lcand[i] : cand[i];
         if (n == 0 || ((unsigned long long)pos - last > (unsigned long long)(spf / 2))) {
             unsigned long long safe = (pos > 5 * spl) ? (unsigned long long)(pos - 5 * spl) : 0ull;
             if (n < num_fields) offsets[n] = safe;

@@ -27,6 +27,10 @@
 using namespace cuvhs;
 
 constexpr int kStages = 12;
+// Chunk buffer sets in flight: the head of chunk c+1 may start once the tail
+// of chunk c+1-kSlots has finished, so with three the heads never wait for
+// the tail launched just before them.
+constexpr int kSlots = 3;
 constexpr int kMarks = 14;
 
 // Buffers and control block of one chunk in flight.  Two slots alternate so
@@ -81,7 +85,7 @@ struct cuvhs_ctx {
     double2* chroma_mask = nullptr; // per-line band-pass: (1, 0) kept bins, (0, 0) others
     float* chroma_filt = nullptr;   // band-passed chroma lines of a chunk (track 0)
     long long buf_n = 0;            // F * spfp + one segment of zero padding
-    Slot slot[2];
+    Slot slot[kSlots];
     double* rawdemod = nullptr;     // head only
     int* cand = nullptr;            // head only
     int *k2s = nullptr, *k2l = nullptr, *k2c = nullptr;
@@ -228,7 +232,9 @@ ChromaLaunch chroma_launch(cuvhs_ctx* c, const ChunkCtl* ctl, int nf_override, c
     return a;
 }
 
-int k1_launch(cuvhs_ctx* c, Slot& s, const void* chunk, long long n_valid, int nfk1, hipStream_t st) {
+// ctl (optional): the chunk's control block, initialised by K1's first workgroup
+int k1_launch(cuvhs_ctx* c, Slot& s, const void* chunk, long long n_valid, int nfk1, hipStream_t st,
+              ChunkCtl* ctl = nullptr) {
     const Fmt& f = c->fmt;
     long long T = (long long)nfk1 * f.spfp;
     K1ALaunch a{};
@@ -247,6 +253,9 @@ int k1_launch(cuvhs_ctx* c, Slot& s, const void* chunk, long long n_valid, int n
     a.tw = c->tw_a;
     a.rawdemod = c->rawdemod;
     a.env = s.env;
+    a.ctl = ctl;
+    a.ctl_raw_offset = (long long)s.raw_offset;
+    a.ctl_fields_k1 = s.fk1;
     HIPCHK(c, run_k1_part_a(c->k1.rf.B, c->k1_f32, a, nfk1, st));
     mark(c, s, 1, st);
     K1BLaunch b{};
@@ -325,12 +334,8 @@ int head_stage(cuvhs_ctx* c, Slot& s) {
     if (r) return r;
     HIPCHK(c, hipEventRecord(s.pre1, side));
     ChunkCtl* h = s.hctl;
-    memset(h, 0, sizeof(*h));
-    h->fields_k1 = s.fk1;
-    h->raw_offset = s.raw_offset;
-    HIPCHK(c, hipMemcpyAsync(s.ctl, h, sizeof(ChunkCtl), hipMemcpyHostToDevice, st));
     mark(c, s, 0, st);
-    r = k1_launch(c, s, s.chunk, (long long)s.n_read, s.fk1, st);
+    r = k1_launch(c, s, s.chunk, (long long)s.n_read, s.fk1, st, s.ctl);
     if (r) return r;
     mark(c, s, 2, st);
     const long long T = (long long)s.fk1 * f.spfp;
@@ -432,8 +437,18 @@ int sync_all(cuvhs_ctx* c) {
 }
 
 // Host wait for a slot's tail stage; collects its stage times and capacity flag.
+// Host wait for an event by polling: the decode loop's waits gate the next
+// launches, and a blocking wait's wake-up latency would leave the GPU idle
+// between the head of one chunk and the tail launched after it.
+hipError_t spin_wait(hipEvent_t ev) {
+    hipError_t e;
+    while ((e = hipEventQuery(ev)) == hipErrorNotReady) {
+    }
+    return e;
+}
+
 int finish_tail(cuvhs_ctx* c, Slot& s) {
-    HIPCHK(c, hipEventSynchronize(s.tail_done));
+    HIPCHK(c, spin_wait(s.tail_done));
     collect_profile(c, s);
     if (c->diag) {
         for (int i = 0; i < 9; i++) {
@@ -740,10 +755,12 @@ int cuvhs_decode_capture(cuvhs_ctx* c, const void* cap, uint64_t cap_len, int ma
     if (!plan(c->slot[0], 0)) return 0;
     int rc = head_stage(c, c->slot[0]);
     if (rc) { sync_all(c); return rc; }
-    int total = 0, warn = 0, cur = 0, pending = -1;
+    int total = 0, warn = 0;
+    long long chunk = 0;        // the chunk whose head is in flight
+    long long finished = 0;     // tails of chunks < finished have been collected
     while (true) {
-        Slot& s = c->slot[cur];
-        const hipError_t he = hipEventSynchronize(s.head_done);
+        Slot& s = c->slot[chunk % kSlots];
+        const hipError_t he = spin_wait(s.head_done);
         if (he != hipSuccess) { rc = fail(c, he, "head stage"); break; }
         const int nf = s.hctl->nf;
         const uint64_t next = s.hctl->next_raw_offset;
@@ -760,22 +777,22 @@ int cuvhs_decode_capture(cuvhs_ctx* c, const void* cap, uint64_t cap_len, int ma
         o.do_line = d_do_line ? d_do_line + (size_t)total * kMaxDropouts : nullptr;
         o.do_startx = d_do_startx ? d_do_startx + (size_t)total * kMaxDropouts : nullptr;
         o.do_endx = d_do_endx ? d_do_endx + (size_t)total * kMaxDropouts : nullptr;
+        // the next chunk's buffer set is free once the tail of the chunk that
+        // last used it has finished; the next head goes in before this
+        // chunk's tail, so the longer head chain starts as early as it can
+        for (; finished <= chunk + 1 - kSlots; finished++)
+            if ((rc = finish_tail(c, c->slot[finished % kSlots])) != 0) break;
+        if (rc) break;
+        const bool more = plan(c->slot[(chunk + 1) % kSlots], next);
+        if (more && (rc = head_stage(c, c->slot[(chunk + 1) % kSlots])) != 0) break;
         if ((rc = tail_stage(c, s, &o)) != 0) break;
         total += nf;
-        // the other slot is free once the previous chunk's tail has finished
-        if (pending >= 0) {
-            int p = pending;
-            pending = -1;
-            if ((rc = finish_tail(c, c->slot[p])) != 0) break;
-        }
-        pending = cur;
-        const int nxt = cur ^ 1;
-        if (!plan(c->slot[nxt], next)) break;
-        if ((rc = head_stage(c, c->slot[nxt])) != 0) break;
-        cur = nxt;
+        chunk++;
+        if (!more) break;
     }
-    if (pending >= 0) {
-        int r = finish_tail(c, c->slot[pending]);
+    // tails launched and not yet collected: chunks [finished, chunk)
+    for (; finished < chunk; finished++) {
+        int r = finish_tail(c, c->slot[finished % kSlots]);
         if (!rc) rc = r;
     }
     int r = sync_all(c);

@@ -16,6 +16,8 @@
 // (|g| < eps * max) plus rounding.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+
 #include "common.h"
 #include "fft_lds.h"
 #include "launch.h"
@@ -95,6 +97,13 @@ __global__ __launch_bounds__(B / 16, (k1_min_blocks<B, T>())) void k1_part_a(K1A
     constexpr int NT = B / 16;
     const int f = blockIdx.y;
     const int b = blockIdx.x;
+    if (a.ctl && f == 0 && b == 0 && threadIdx.x == 0) {
+        // the chunk's control block starts here: no host-to-device copy
+        ChunkCtl z{};
+        z.fields_k1 = a.ctl_fields_k1;
+        z.raw_offset = a.ctl_raw_offset;
+        *a.ctl = z;
+    }
     const int qs = (kOverlap - 1) + b * a.V;  // first analytic output
     if (qs + 1 >= kOverlap + a.spfp) return;
     const int seg0 = f * a.spfp - kOverlap;
@@ -314,11 +323,24 @@ __global__ void k_fill_f64(double* p, long long n, double v) {
 }
 
 // ---- host launchers ----
+
+// Dynamic LDS above 64 KB needs a per-kernel attribute; set it once per
+// device rather than before every launch.
+template <typename K>
+static void ensure_lds(K* kern, size_t sh) {
+    static std::atomic<unsigned> done{0};
+    int d = 0;
+    (void)hipGetDevice(&d);
+    const unsigned bit = 1u << (d & 31);
+    if (done.load(std::memory_order_relaxed) & bit) return;
+    (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
+    done.fetch_or(bit);
+}
 template <int B, typename T>
 static hipError_t launch_a(const K1ALaunch& a, int nf, hipStream_t st) {
     int nb = (int)((a.spfp + a.V - 1) / a.V);
     size_t sh = (size_t)lds_elems(B) * sizeof(typename C2<T>::type);
-    (void)hipFuncSetAttribute((const void*)k1_part_a<B, T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
+    ensure_lds(k1_part_a<B, T>, sh);
     hipLaunchKernelGGL((k1_part_a<B, T>), dim3(nb, nf), dim3(B / 16), sh, st, a);
     return hipGetLastError();
 }
@@ -326,7 +348,7 @@ template <int B, typename T>
 static hipError_t launch_b(const K1BLaunch& a, int nf, hipStream_t st) {
     int nb = (int)((a.spfp + a.V - 1) / a.V);
     size_t sh = (size_t)lds_elems(B) * sizeof(typename C2<T>::type);
-    (void)hipFuncSetAttribute((const void*)k1_part_b<B, T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
+    ensure_lds(k1_part_b<B, T>, sh);
     hipLaunchKernelGGL((k1_part_b<B, T>), dim3(nb, nf), dim3(B / 16), sh, st, a);
     return hipGetLastError();
 }
@@ -334,7 +356,7 @@ template <int B>
 static hipError_t launch_pre(const PreLaunch& a, int nf, hipStream_t st) {
     int nb = (int)((a.spfp + 2 * a.V - 1) / (2 * a.V));
     size_t sh = (size_t)lds_elems(B) * sizeof(float2);
-    (void)hipFuncSetAttribute((const void*)k_prebpf<B>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
+    ensure_lds(k_prebpf<B>, sh);
     hipLaunchKernelGGL(k_prebpf<B>, dim3(nb, nf), dim3(B / 16), sh, st, a);
     return hipGetLastError();
 }

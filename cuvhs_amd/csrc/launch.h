@@ -21,6 +21,9 @@ struct K1ALaunch {
     const void* tw;             // twiddles W_B^m, same type
     double* rawdemod;
     double* env;
+    ChunkCtl* ctl;              // if set, initialised by the first workgroup:
+    long long ctl_raw_offset;   //   zeros, fields_k1 and raw_offset
+    int ctl_fields_k1;
 };
 struct K1BLaunch {
     const double* rawdemod;

@@ -109,7 +109,16 @@ __global__ __launch_bounds__(kDoThreads) void k_do_class(const double* __restric
         int bs = w.start + b * kEnvBlock;
         int be = min(bs + kEnvBlock, w.end);
         double bsum = 0.0;
-        for (int i = bs; i < be; i++) bsum += env[i];
+        if (be - bs == kEnvBlock) {
+            // full block: all loads issued together, summed in sample order
+            double e[kEnvBlock];
+#pragma unroll
+            for (int i = 0; i < kEnvBlock; i++) e[i] = env[bs + i];
+#pragma unroll
+            for (int i = 0; i < kEnvBlock; i++) bsum += e[i];
+        } else {
+            for (int i = bs; i < be; i++) bsum += env[i];
+        }
         double bm = bsum / (double)(be - bs);
         cls[(size_t)f * kMaxClassBlocks + b] = bm <= down ? 1 : (bm >= up ? 2 : 0);
     }

@@ -264,7 +264,8 @@ class Decoder:
         spf = self.fmt.samples_per_field
         keys = ("luma", "chroma", "file_loc", "is_first", "phase_id", "do_count", "do_line", "do_startx", "do_endx")
         parts = {k: [] for k in keys}
-        torch.cuda.synchronize(d_capture.device)
+        # the capture was written on torch's stream; the decoder's streams are its own
+        torch.cuda.current_stream(d_capture.device).synchronize()
         infos = []
         off = 0
         base = d_capture.data_ptr()
@@ -327,7 +328,9 @@ class Decoder:
         dos = torch.empty_like(dol)
         doe = torch.empty_like(dol)
         nf = _i()
-        torch.cuda.synchronize(dev)
+        # the capture was written on torch's stream; the decoder's streams are
+        # its own (a device-wide sync would also wait for other decoders)
+        torch.cuda.current_stream(dev).synchronize()
         rc = lib().cuvhs_decode_capture(self.h, ptr(d_capture), n, max_fields, ptr(luma), ptr(chroma), ptr(file_loc),
                                         ptr(is_first), ptr(phase), ptr(doc), ptr(dol), ptr(dos), ptr(doe),
                                         ctypes.byref(nf))

@@ -40,16 +40,26 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(cap, fmt, batch):
+# the workloads a bench line can be taken on
+WORKLOADS = {
+    "600f_28mhz": {"config": 1, "mhz": 28.0, "fields": None,
+                   "desc": "600 synthetic NTSC fields of u8 RF at 28 MHz (SNR 25-40 dB, dropouts, wow/flutter)"},
+    "640f_40mhz_headswitch": {"config": 2, "mhz": 40.0, "fields": 640,
+                              "desc": "first 640 fields of the 6000-field 40 MHz u8 workload (same signal model, "
+                                      "0-4 us head-switch offsets in groups of 1-8 fields)"},
+}
+
+
+def cpu_baseline(cap, fmt, batch, mhz):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle_py
     sample = cap[: 2 * batch * fmt.spfp]  # two chunks
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     t = time.perf_counter()
-    ref = oracle_py.decode(sample, 28.0, batch, trace=False)
+    ref = oracle_py.decode(sample, mhz, batch, trace=False)
     dt = time.perf_counter() - t
     return {"value": ref["fields"] / dt, "unit": "fields/s", "cores": threads, "kind": "port",
-            "sample": f"first {ref['fields']} fields of the same 28 MHz capture ({sample.size} u8 samples), "
+            "sample": f"first {ref['fields']} fields of the same {mhz:g} MHz capture ({sample.size} u8 samples), "
                       f"oracle liboracle.so (FP64, own FFT), OpenMP over fields, {dt:.1f} s"}
 
 
@@ -98,6 +108,8 @@ def main():
     ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-forms", action="store_true", help="skip the FP32 form and its error report")
+    ap.add_argument("--workload", default="600f_28mhz", choices=sorted(WORKLOADS),
+                    help="the capture a bench line is taken on (default: the 600-field 28 MHz workload)")
     ap.add_argument("--serving", action="store_true",
                     help="instead: the serving loop (16 concurrent captures at 28/35.8/40 MHz, Poisson arrivals), "
                          "one JSON line with latency percentiles and aggregate fields/s per precision form")
@@ -128,11 +140,16 @@ def main():
         return
 
     t = time.perf_counter()
-    cap = native.generate(1)
+    wl = WORKLOADS[args.workload]
+    mhz = wl["mhz"]
+    if wl["fields"]:
+        cap = native.generate(wl["config"], n_samples=wl["fields"] * native.format_for(mhz).samples_per_field)
+    else:
+        cap = native.generate(wl["config"])
     log(f"generated {cap.size} samples in {time.perf_counter() - t:.1f} s")
     dev = torch.device("cuda:0")
     d_cap = torch.from_numpy(cap).to(dev)
-    dec = native.Decoder(28.0, args.batch, device=0)
+    dec = native.Decoder(mhz, args.batch, device=0)
     fmt = dec.fmt
     fields = 0
     for _ in range(args.warmup):
@@ -178,7 +195,7 @@ def main():
     iso_launch_s = iso[dom] / 1000.0 / launches
     traffic = None
     pmc_path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
-    if os.path.exists(pmc_path):
+    if os.path.exists(pmc_path) and args.workload == "600f_28mhz":  # counters taken on that workload
         with open(pmc_path) as f:
             traffic = json.load(f).get("bytes_per_launch", {}).get(dom)
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
@@ -190,7 +207,7 @@ def main():
     forms = None
     if not args.no_forms:
         from cuvhs_amd import precision
-        d32 = native.Decoder(28.0, args.batch, device=0, precision="fp32")
+        d32 = native.Decoder(mhz, args.batch, device=0, precision="fp32")
         for _ in range(args.warmup):
             d32.reset()
             d32.decode_capture(d_cap)
@@ -207,7 +224,7 @@ def main():
         iso32 = d32.profile_read()
         d32.close()
         log(f"fp32 form: {f32_fields * args.steps / dt32:.1f} fields/s; comparing with fp64 ...")
-        report = precision.compare(d_cap, 28.0, args.batch, "fp32")
+        report = precision.compare(d_cap, mhz, args.batch, "fp32")
         forms = {"fp32": {"value": round(f32_fields * args.steps / dt32, 2), "unit": "fields/s",
                           "ms_per_step": round(dt32 * 1000.0 / args.steps, 3), "dtype": "f32 (K1), f64 (rest)",
                           "stage_ms_per_step_isolated": {k: round(v, 3) for k, v in iso32.items()},
@@ -216,11 +233,11 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         log("running CPU baseline ...")
-        cpu = cpu_baseline(cap, fmt, args.batch)
+        cpu = cpu_baseline(cap, fmt, args.batch, mhz)
         log(f"cpu baseline {cpu['value']:.2f} fields/s on {cpu['cores']} threads")
 
     line = {
-        "metric": METRIC,
+        "metric": METRIC if mhz == 28.0 else METRIC.replace("28 MHz", f"{mhz:g} MHz"),
         "value": round(value, 2),
         "unit": "fields/s",
         "n_gpus": 1,
@@ -232,8 +249,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "600 synthetic NTSC fields of u8 RF at 28 MHz (SNR 25-40 dB, dropouts, wow/flutter)",
-                   "fields_per_step": fields, "batch_fields": args.batch, "sample_rate_mhz": 28.0,
+        "config": {"workload": wl["desc"],
+                   "fields_per_step": fields, "batch_fields": args.batch, "sample_rate_mhz": mhz,
                    "precision": "fp64 (chroma pre-band-pass fp32)"},
         "hbm_gbs_algorithmic": round(value * bytes_per_field / 1e9, 2),
         "stage_ms_per_step": {k: round(v, 3) for k, v in stage_ms.items()},

@@ -157,13 +157,19 @@ std::vector<V> twiddles(int B) {
     return t;
 }
 
-// perm: store bin k at conv4096_gperm(k) / conv16384_gperm(k) (the bin
-// order of the wave-local 4096- and 16384-point blocks)
+// Bin order of the wave-local convolution blocks (fft_lds.h) of size n.
+size_t block_bin(size_t n, size_t k) {
+    if (n == 4096) return (size_t)conv4096_gperm((int)k);
+    if (n == 8192) return (size_t)conv_rows_gperm<512>((int)k);
+    return (size_t)conv_rows_gperm<1024>((int)k);
+}
+
+// perm: store bin k at block_bin(size, k)
 template <typename V, typename S>
 hipError_t upload_spectrum(cuvhs_ctx* c, V** dst, const std::vector<cdbl>& s, bool perm = false) {
     std::vector<V> h(s.size());
     for (size_t i = 0; i < s.size(); i++) {
-        size_t j = !perm ? i : (s.size() == 4096 ? (size_t)conv4096_gperm((int)i) : (size_t)conv16384_gperm((int)i));
+        size_t j = perm ? block_bin(s.size(), i) : i;
         h[j].x = (S)s[i].real();
         h[j].y = (S)s[i].imag();
     }
@@ -544,7 +550,7 @@ int cuvhs_create(const cuvhs_config* cfg, cuvhs_ctx** out) {
         if (e2 != hipSuccess) return bad(e2, #p);      \
     } while (0)
     {
-        const bool pa = c->k1.rf.B == 4096, pb = c->k1.video.B == 4096;
+        const bool pa = true, pb = true;  // every K1 block size is a wave-local block
         if (c->k1_f32) {
             if ((e = upload_spectrum<float2, float>(c, (float2**)&c->G_a, c->k1.rf.spectrum, pa)) != hipSuccess) return bad(e, "G_a");
             if ((e = upload_spectrum<float2, float>(c, (float2**)&c->G_b, c->k1.video.spectrum, pb)) != hipSuccess) return bad(e, "G_b");
@@ -553,7 +559,7 @@ int cuvhs_create(const cuvhs_config* cfg, cuvhs_ctx** out) {
             if ((e = upload_spectrum<double2, double>(c, (double2**)&c->G_b, c->k1.video.spectrum, pb)) != hipSuccess) return bad(e, "G_b");
         }
     }
-    if ((e = upload_spectrum<float2, float>(c, &c->G_p, c->pre.bpf.spectrum, c->pre.bpf.B == 16384)) != hipSuccess)
+    if ((e = upload_spectrum<float2, float>(c, &c->G_p, c->pre.bpf.spectrum, true)) != hipSuccess)
         return bad(e, "G_p");
     {
         if ((e = upload_twiddles(c, &c->tw_a, c->k1.rf.B, c->k1_f32)) != hipSuccess) return bad(e, "twiddles");

@@ -367,97 +367,107 @@ __device__ __forceinline__ void conv4096(typename C2<T>::type* buf, typename C2<
     dft16<+1, T>(v);
 }
 
-// ---- 16384-point convolution block, same scheme ----
+// ---- 16 R-point convolution blocks, same scheme, R = 512 or 1024 ----
 //
-// 16384 = 16 x 1024 on 1024 threads (16 waves).  Stage 1 as in conv4096
-// (thread t = n2 < 1024, rows k1 of 1024), one block barrier, then wave w
-// owns row k1 = w and does its 1024-point transform as 16 x 16 x 4 with two
-// transposes through its own row:
-//   n2 = 64 a + 4 b + g,  k2 = c + 16 e1 + 256 e2  (e1 = 4 e1h + e1l)
-//   pass A, lane m = 4 b + g: DFT16 over a -> c, twiddle W1024^(m c)
-//   pass B, lane 4 c + g:     DFT16 over b -> e1, twiddle W64^(g e1)
-//   pass C, lane 4 c + e1h:   four DFT4 over g -> e2 (held as v[4 e1l + g])
-// The spectrum is applied in pass C's order (Gp[1024 w + 64 j + lane],
-// j = 4 e1l + e2, see conv16384_gperm) and the inverse retraces the passes.
+// B = 16 R on R threads.  Stage 1 as in conv4096 (thread t = n2 < R, rows k1
+// of R), one block barrier, then each row is transformed by L = R/16 lanes of
+// one wave (a wave owns 64/L rows) as 16 x 16 x G, G = R/256, with two
+// transposes through its own rows:
+//   n2 = L a + G b + g,  k2 = c + 16 e1 + 256 e2  (e1 = (16/G) e1h + e1l)
+//   pass A, lane m = G b + g: DFT16 over a -> c, twiddle WR^(m c)
+//   pass B, lane G c + g:     DFT16 over b -> e1, twiddle WL^(g e1)
+//   pass C, lane G c + e1h:   DFT_G over g -> e2 (held as v[G e1l + g])
+// The spectrum is applied in pass C's order (Gp[R k1 + L j + lane in row],
+// j = G e1l + e2, see conv_rows_gperm) and the inverse retraces the passes.
 
-inline int conv16384_gperm(int k) {
+template <int R>
+inline int conv_rows_gperm(int k) {
+    constexpr int G = R / 256, L = R / 16, Q = 16 / G;
     int k1 = k & 15, k2 = k >> 4;
     int c = k2 & 15, e1 = (k2 >> 4) & 15, e2 = k2 >> 8;
-    int lane = 4 * c + (e1 >> 2), j = 4 * (e1 & 3) + e2;
-    return k1 * 1024 + j * 64 + lane;
+    int lane = G * c + e1 / Q, j = G * (e1 % Q) + e2;
+    return k1 * R + j * L + lane;
 }
 
-// In: v[n1] = x[1024 n1 + t].  Out: v[n1] = y[1024 n1 + t].  buf:
-// lds_elems(16384) elements.  tw: the table of W16384^m.
-template <typename T>
-__device__ __forceinline__ void conv16384(typename C2<T>::type* buf, typename C2<T>::type* v,
+template <int G, int S, typename V>
+__device__ __forceinline__ void dft_small(V* v) {
+    if constexpr (G == 4) dft4<S>(v[0], v[1], v[2], v[3]);
+    else dft2<S>(v[0], v[1]);
+}
+
+// In: v[n1] = x[R n1 + t].  Out: v[n1] = y[R n1 + t].  buf: lds_elems(16 R)
+// elements.  tw: the table of W(16 R)^m.
+template <typename T, int R>
+__device__ __forceinline__ void conv_rows(typename C2<T>::type* buf, typename C2<T>::type* v,
                                           const typename C2<T>::type* tw, const typename C2<T>::type* Gp) {
     using V = typename C2<T>::type;
-    constexpr int NB = 16384;
+    constexpr int NB = 16 * R, G = R / 256, L = R / 16;
+    static_assert(R == 512 || R == 1024, "row length");
     const int t = threadIdx.x;
-    const int w = t >> 6, l = t & 63;
-    const int row = w << 10;
-    const int hi = l >> 2, lo = l & 3;      // (c, g) in pass B, (c, e1h) in pass C
+    const int rw = t / L;                   // row owned in the passes (L lanes per row)
+    const int l = t % L;                    // lane within the row
+    const int row = rw * R;
+    const int hi = l / G, lo = l % G;       // (c, g) in pass B, (c, e1h) in pass C
     V wb[4];
     dft16<-1, T>(v);
     twiddle_base<T, NB>(wb, tw, t);
     twiddle_run<-1, T>(v, wb);
 #pragma unroll
-    for (int r = 0; r < 16; r++) buf[pidx((r << 10) + t)] = v[r];
+    for (int r = 0; r < 16; r++) buf[pidx(r * R + t)] = v[r];
     __syncthreads();
-    // pass A: lane m = l holds n2 = 64 a + m
+    // pass A: lane m = l holds n2 = L a + m
 #pragma unroll
-    for (int a = 0; a < 16; a++) v[a] = buf[pidx(row + 64 * a + l)];
+    for (int a = 0; a < 16; a++) v[a] = buf[pidx(row + L * a + l)];
     dft16<-1, T>(v);                                               // a -> c
     twiddle_base<T, NB>(wb, tw, 16 * l);
-    twiddle_run<-1, T>(v, wb);                                     // W1024^(m c)
+    twiddle_run<-1, T>(v, wb);                                     // WR^(m c)
 #pragma unroll
-    for (int c = 0; c < 16; c++) buf[pidx(row + 64 * c + l)] = v[c];
+    for (int c = 0; c < 16; c++) buf[pidx(row + L * c + l)] = v[c];
     wave_lds_sync();
-    // pass B: lane (c = hi, g = lo) holds m = 4 b + g
+    // pass B: lane (c = hi, g = lo) holds m = G b + g
 #pragma unroll
-    for (int b = 0; b < 16; b++) v[b] = buf[pidx(row + 64 * hi + 4 * b + lo)];
+    for (int b = 0; b < 16; b++) v[b] = buf[pidx(row + L * hi + G * b + lo)];
     dft16<-1, T>(v);                                               // b -> e1
     twiddle_base<T, NB>(wb, tw, 256 * lo);
-    twiddle_run<-1, T>(v, wb);                                     // W64^(g e1)
+    twiddle_run<-1, T>(v, wb);                                     // WL^(g e1)
 #pragma unroll
-    for (int e1 = 0; e1 < 16; e1++) buf[pidx(row + 64 * hi + 4 * e1 + lo)] = v[e1];
+    for (int e1 = 0; e1 < 16; e1++) buf[pidx(row + L * hi + G * e1 + lo)] = v[e1];
     wave_lds_sync();
-    // pass C: lane (c = hi, e1h = lo) holds (e1l, g) at v[4 e1l + g]
+    // pass C: lane (c = hi, e1h = lo) holds (e1l, g) at v[G e1l + g]
 #pragma unroll
-    for (int j = 0; j < 16; j++) v[j] = buf[pidx(row + 64 * hi + 16 * lo + j)];
+    for (int j = 0; j < 16; j++) v[j] = buf[pidx(row + L * hi + 16 * lo + j)];
 #pragma unroll
-    for (int e = 0; e < 4; e++) dft4<-1>(v[4 * e], v[4 * e + 1], v[4 * e + 2], v[4 * e + 3]);
+    for (int e = 0; e < 16; e += G) dft_small<G, -1>(v + e);
     V g[16];
 #pragma unroll
-    for (int j = 0; j < 16; j++) g[j] = Gp[(w << 10) + (j << 6) + l];
+    for (int j = 0; j < 16; j++) g[j] = Gp[row + j * L + l];
 #pragma unroll
     for (int j = 0; j < 16; j++) v[j] = cmul(v[j], g[j]);
 #pragma unroll
-    for (int e = 0; e < 4; e++) dft4<+1>(v[4 * e], v[4 * e + 1], v[4 * e + 2], v[4 * e + 3]);
+    for (int e = 0; e < 16; e += G) dft_small<G, +1>(v + e);
 #pragma unroll
-    for (int j = 0; j < 16; j++) buf[pidx(row + 64 * hi + 16 * lo + j)] = v[j];
+    for (int j = 0; j < 16; j++) buf[pidx(row + L * hi + 16 * lo + j)] = v[j];
     wave_lds_sync();
     // inverse pass B: lane (c, g)
 #pragma unroll
-    for (int e1 = 0; e1 < 16; e1++) v[e1] = buf[pidx(row + 64 * hi + 4 * e1 + lo)];
+    for (int e1 = 0; e1 < 16; e1++) v[e1] = buf[pidx(row + L * hi + G * e1 + lo)];
     twiddle_base<T, NB>(wb, tw, 256 * lo);
     twiddle_run<+1, T>(v, wb);
     dft16<+1, T>(v);                                               // e1 -> b
 #pragma unroll
-    for (int b = 0; b < 16; b++) buf[pidx(row + 64 * hi + 4 * b + lo)] = v[b];
+    for (int b = 0; b < 16; b++) buf[pidx(row + L * hi + G * b + lo)] = v[b];
     wave_lds_sync();
     // inverse pass A: lane m
 #pragma unroll
-    for (int c = 0; c < 16; c++) v[c] = buf[pidx(row + 64 * c + l)];
+    for (int c = 0; c < 16; c++) v[c] = buf[pidx(row + L * c + l)];
     twiddle_base<T, NB>(wb, tw, 16 * l);
     twiddle_run<+1, T>(v, wb);
     dft16<+1, T>(v);                                               // c -> a
 #pragma unroll
-    for (int a = 0; a < 16; a++) buf[pidx(row + 64 * a + l)] = v[a];
+    for (int a = 0; a < 16; a++) buf[pidx(row + L * a + l)] = v[a];
     __syncthreads();
 #pragma unroll
-    for (int r = 0; r < 16; r++) v[r] = buf[pidx((r << 10) + t)];
+    for (int r = 0; r < 16; r++) v[r] = buf[pidx(r * R + t)];
     twiddle_base<T, NB>(wb, tw, t);
     twiddle_run<+1, T>(v, wb);
     dft16<+1, T>(v);

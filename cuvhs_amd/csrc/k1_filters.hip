@@ -125,17 +125,20 @@ __global__ __launch_bounds__(B / 16, (k1_min_blocks<B, T>())) void k1_part_a(K1A
     const int qend = kOverlap + a.spfp;  // one past the last demod output
     const T PI = (T)3.14159265358979323846;
     const T TAU = (T)6.28318530717958647692;
-    if constexpr (B == 4096) {
-        // G is in conv4096's bin order; v[r] becomes a(qs + o), o = 256 r + t - lp
-        conv4096<T>(buf, v, tw, G);
-        __shared__ T edge[4][16];
+    if constexpr (B == 4096 || B == 8192) {
+        // G is in the wave-local block's bin order; v[r] becomes a(qs + o),
+        // o = NT r + t - lp
+        if constexpr (B == 4096) conv4096<T>(buf, v, tw, G);
+        else conv_rows<T, 512>(buf, v, tw, G);
+        constexpr int NW = NT / 64;
+        __shared__ T edge[NW][16];
         const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
         T ang[16];
 #pragma unroll
         for (int r = 0; r < 16; r++) {
             // only outputs 0..V are used (the rest of the block is the
             // convolution's wrap-around)
-            int o = (r << 8) + t - a.lp;
+            int o = r * NT + t - a.lp;
             ang[r] = (T)0;
             if (o >= 0 && o <= a.V) {
                 ang[r] = phase_of(v[r].y, v[r].x);
@@ -154,8 +157,8 @@ __global__ __launch_bounds__(B / 16, (k1_min_blocks<B, T>())) void k1_part_a(K1A
 #pragma unroll
         for (int r = 0; r < 16; r++) {
             T prev = __shfl_up(ang[r], 1);
-            if (lane == 0) prev = wv > 0 ? edge[wv - 1][r] : (r > 0 ? edge[3][r - 1] : (T)0);
-            int o = (r << 8) + t - a.lp;
+            if (lane == 0) prev = wv > 0 ? edge[wv - 1][r] : (r > 0 ? edge[NW - 1][r - 1] : (T)0);
+            int o = r * NT + t - a.lp;
             int q = qs + o;
             if (o >= 1 && o <= a.V && q < qend) {
                 T delta = ang[r] - prev;
@@ -238,11 +241,12 @@ __global__ __launch_bounds__(B / 16, (k1_min_blocks<B, T>())) void k1_part_b(K1B
         v[i].x = xin[i];
         v[i].y = (T)0;
     }
-    if constexpr (B == 4096) {
-        conv4096<T>(buf, v, tw, G);
+    if constexpr (B == 4096 || B == 8192) {
+        if constexpr (B == 4096) conv4096<T>(buf, v, tw, G);
+        else conv_rows<T, 512>(buf, v, tw, G);
 #pragma unroll
         for (int r = 0; r < 16; r++) {
-            int o = (r << 8) + (int)threadIdx.x - a.lp;
+            int o = r * NT + (int)threadIdx.x - a.lp;
             int q = qs + o;
             if (o >= 0 && o < a.V && q < qend) {
                 int s = f * a.spfp + (q - kOverlap);
@@ -289,30 +293,16 @@ __global__ __launch_bounds__(B / 16) void k_prebpf(PreLaunch a) {
         v[i].x = n0 < a.spfp ? (float)raw_sample(a.cap, a.in_fmt, seg0 + n0, a.n_valid, dcv) : 0.0f;
         v[i].y = two && n1 < a.spfp ? (float)raw_sample(a.cap, a.in_fmt, seg0 + n1, a.n_valid, dcv) : 0.0f;
     }
-    if constexpr (B == 16384) {
-        // G is in conv16384's bin order; v[r] becomes the outputs at
-        // o = 1024 r + t - lp of both blocks
-        conv16384<float>(buf, v, a.tw, a.G);
+    // G is in the wave-local block's bin order; v[r] becomes the outputs at
+    // o = NT r + t - lp of both blocks
+    conv_rows<float, NT>(buf, v, a.tw, a.G);
 #pragma unroll
-        for (int r = 0; r < 16; r++) {
-            int o = (r << 10) + (int)threadIdx.x - a.lp;
-            if (o >= 0 && o < a.V) {
-                if (ns0 + o < a.spfp) a.out[seg0 + ns0 + o] = v[r].x;
-                if (ns1 + o < a.spfp) a.out[seg0 + ns1 + o] = v[r].y;
-            }
+    for (int r = 0; r < 16; r++) {
+        int o = r * NT + (int)threadIdx.x - a.lp;
+        if (o >= 0 && o < a.V) {
+            if (ns0 + o < a.spfp) a.out[seg0 + ns0 + o] = v[r].x;
+            if (ns1 + o < a.spfp) a.out[seg0 + ns1 + o] = v[r].y;
         }
-        return;
-    }
-    // the spectrum product stays a separate pass here: fused into the last
-    // (radix-4) stage it needs more than the 128 VGPRs a 1024-thread
-    // workgroup allows and spills
-    fft_fwd_regs_mul<float, B>(buf, v, a.tw, nullptr);
-    spectrum_multiply<float, B>(buf, a.G);
-    fft_lds<+1, float, B>(buf, a.tw);
-    for (int o = threadIdx.x; o < a.V; o += NT) {
-        float2 z = buf[pidx(a.lp + o)];
-        if (ns0 + o < a.spfp) a.out[seg0 + ns0 + o] = z.x;
-        if (ns1 + o < a.spfp) a.out[seg0 + ns1 + o] = z.y;
     }
 }
 

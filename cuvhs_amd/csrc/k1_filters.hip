@@ -13,7 +13,10 @@
 // B-point circular convolution of B loaded inputs with the taps of g_N kept
 // in [-lm, lp]; the loaded input indices wrap modulo N exactly like the
 // reference's window, so the only difference is the dropped taps
-// (|g| < eps * max) plus rounding.
+// (|g| < eps * max) plus rounding.  The B-point convolution itself is one of
+// the wave-local blocks of fft_lds.h (conv4096, conv_rows): a radix-16 stage
+// across the workgroup, then per-wave sub-transforms, the spectrum product
+// and the inverse, with two workgroup barriers per block.
 #include <hip/hip_runtime.h>
 
 #include <atomic>
@@ -125,7 +128,8 @@ __global__ __launch_bounds__(B / 16, (k1_min_blocks<B, T>())) void k1_part_a(K1A
     const int qend = kOverlap + a.spfp;  // one past the last demod output
     const T PI = (T)3.14159265358979323846;
     const T TAU = (T)6.28318530717958647692;
-    if constexpr (B == 4096 || B == 8192) {
+    static_assert(B == 4096 || B == 8192, "K1 blocks are wave-local 4096- or 8192-point blocks");
+    {
         // G is in the wave-local block's bin order; v[r] becomes a(qs + o),
         // o = NT r + t - lp
         if constexpr (B == 4096) conv4096<T>(buf, v, tw, G);
@@ -167,47 +171,6 @@ __global__ __launch_bounds__(B / 16, (k1_min_blocks<B, T>())) void k1_part_a(K1A
                 a.rawdemod[f * a.spfp + (q - kOverlap)] = (double)delta * a.scale;
             }
         }
-        return;
-    }
-    fft_fwd_regs_mul<T, B>(buf, v, tw, G);
-    fft_lds<+1, T, B>(buf, tw);
-    // outputs o = 0..V: analytic a(qs + o) = buf[lp + o]
-    constexpr int MAXO = (B + NT - 1) / NT;
-    T ang[MAXO];
-#pragma unroll
-    for (int i = 0; i < MAXO; i++) {
-        int o = threadIdx.x + i * NT;
-        ang[i] = (T)0;
-        if (o <= a.V) {
-            V2 z = buf[pidx(a.lp + o)];
-            ang[i] = phase_of(z.y, z.x);
-            int q = qs + o;
-            if (q >= kOverlap && q < qend) {
-                int s = f * a.spfp + (q - kOverlap);
-                a.env[s] = (double)sqrt(z.x * z.x + z.y * z.y);
-            }
-        }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int i = 0; i < MAXO; i++) {
-        int o = threadIdx.x + i * NT;
-        if (o <= a.V) buf[pidx(a.lp + o)].x = ang[i];
-    }
-    __syncthreads();
-#pragma unroll
-    for (int i = 0; i < MAXO; i++) {
-        int o = threadIdx.x + i * NT;
-        if (o >= 1 && o <= a.V) {
-            int q = qs + o;
-            if (q < qend) {
-                T delta = ang[i] - buf[pidx(a.lp + o - 1)].x;
-                if (delta > PI) delta -= TAU;
-                if (delta < -PI) delta += TAU;
-                int s = f * a.spfp + (q - kOverlap);
-                a.rawdemod[s] = (double)delta * a.scale;
-            }
-        }
     }
 }
 
@@ -241,7 +204,8 @@ __global__ __launch_bounds__(B / 16, (k1_min_blocks<B, T>())) void k1_part_b(K1B
         v[i].x = xin[i];
         v[i].y = (T)0;
     }
-    if constexpr (B == 4096 || B == 8192) {
+    static_assert(B == 4096 || B == 8192, "K1 blocks are wave-local 4096- or 8192-point blocks");
+    {
         if constexpr (B == 4096) conv4096<T>(buf, v, tw, G);
         else conv_rows<T, 512>(buf, v, tw, G);
 #pragma unroll
@@ -254,17 +218,6 @@ __global__ __launch_bounds__(B / 16, (k1_min_blocks<B, T>())) void k1_part_b(K1B
                 a.demod05[s] = (double)v[r].y;
             }
         }
-        return;
-    }
-    fft_fwd_regs_mul<T, B>(buf, v, tw, G);
-    fft_lds<+1, T, B>(buf, tw);
-    for (int o = threadIdx.x; o < a.V; o += NT) {
-        int q = qs + o;
-        if (q >= qend) break;
-        V2 z = buf[pidx(a.lp + o)];
-        int s = f * a.spfp + (q - kOverlap);
-        a.demod[s] = (double)z.x;
-        a.demod05[s] = (double)z.y;
     }
 }
 

@@ -365,13 +365,30 @@ __global__ __launch_bounds__(64) void k_k2b_field(const ChunkCtl* ctl, int nf_ov
 __global__ void k_k2b_chain(ChunkCtl* ctl, DecodeState* dst, int nf_override, const unsigned long long* offsets,
                             const K2bField* summ, const int* g_vp_start, const unsigned char* g_vp_tg,
                             K2bAnchorOut* anchors, int* is_first, int num_eq, int fl0, int fl1) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    // the field summaries and offsets are staged in LDS by the whole
+    // workgroup, so the sequential loop below waits on LDS, not on L2
+    constexpr int kStage = 64;
+    static_assert(sizeof(K2bField) % sizeof(int) == 0, "summaries are copied as words");
+    __shared__ K2bField s_summ[kStage];
+    __shared__ unsigned long long s_off[kStage];
+    if (blockIdx.x != 0) return;
     const int nf = nf_override > 0 ? nf_override : ctl->nf;
+    const int ns = nf < kStage ? nf : kStage;
+    {
+        const int* src = reinterpret_cast<const int*>(summ);
+        int* dstw = reinterpret_cast<int*>(s_summ);
+        const int words = ns * (int)(sizeof(K2bField) / sizeof(int));
+        for (int i = threadIdx.x; i < words; i += blockDim.x) dstw[i] = src[i];
+        for (int i = threadIdx.x; i < ns; i += blockDim.x) s_off[i] = offsets[i];
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
     K2bState st = dst->k2b;
     const unsigned long long raw_offset = ctl->raw_offset;
     int ovf = 0;
     for (int f = 0; f < nf; f++) {
-        const K2bField& S = summ[f];
+        const K2bField& S = f < kStage ? s_summ[f] : summ[f];
+        const unsigned long long off_f = f < kStage ? s_off[f] : offsets[f];
         ovf |= S.overflow;
         K2bAnchorOut A;
         A.valid = 0;
@@ -382,7 +399,7 @@ __global__ void k_k2b_chain(ChunkCtl* ctl, DecodeState* dst, int nf_override, co
         A.meanlinelen = mll;
         double prev_offset_lines = 0.0;
         if (st.prev_first_hsync_readloc != -1 && mll > 0.0)
-            prev_offset_lines = ((double)st.prev_first_hsync_readloc - (double)(raw_offset + offsets[f])) / mll;
+            prev_offset_lines = ((double)st.prev_first_hsync_readloc - (double)(raw_offset + off_f)) / mll;
         bool valid = false;
         bool first_field = false;
         double loc = 0.0, hsl = 0.0, new_diff = st.prev_first_hsync_diff;
@@ -457,7 +474,7 @@ __global__ void k_k2b_chain(ChunkCtl* ctl, DecodeState* dst, int nf_override, co
             is_first[f] = A.first_field;
             st.prev_first_field = A.first_field;
             st.prev_progressive_field = 0;
-            st.prev_first_hsync_readloc = (long long)(raw_offset + offsets[f]);
+            st.prev_first_hsync_readloc = (long long)(raw_offset + off_f);
             st.prev_first_hsync_loc = loc;
             st.prev_first_hsync_diff = new_diff;
         } else {
@@ -507,12 +524,19 @@ __global__ __launch_bounds__(256) void k_k2b_linelocs(const ChunkCtl* ctl, int n
         for (int line = threadIdx.x; line < kLinesPerFrame; line += 256) ll[line] = ref + mll * (double)line;
         return;
     }
-    // bitonic sort of the starts (pad with INT_MAX)
+    // bitonic sort of the starts (pad with INT_MAX), skipped when the list
+    // is already ascending (the usual case: pulses are validated in order)
+    __shared__ int s_unsorted;
     int P = 1;
     while (P < n) P <<= 1;
-    for (int i = threadIdx.x; i < P; i += 256) key[i] = i < n ? g_vp_start[(size_t)f * kVpCap + i] : 0x7fffffff;
+    if (threadIdx.x == 0) s_unsorted = 0;
     __syncthreads();
-    for (int k = 2; k <= P; k <<= 1) {
+    for (int i = threadIdx.x; i < P; i += 256) {
+        key[i] = i < n ? g_vp_start[(size_t)f * kVpCap + i] : 0x7fffffff;
+        if (i > 0 && i < n && g_vp_start[(size_t)f * kVpCap + i - 1] > key[i]) s_unsorted = 1;
+    }
+    __syncthreads();
+    for (int k = 2; k <= P && s_unsorted; k <<= 1) {
         for (int j = k >> 1; j > 0; j >>= 1) {
             for (int i = threadIdx.x; i < P; i += 256) {
                 int ixj = i ^ j;

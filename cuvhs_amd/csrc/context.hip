@@ -786,8 +786,7 @@ int cuvhs_decode_capture(cuvhs_ctx* c, const void* cap, uint64_t cap_len, int ma
         // the next chunk's buffer set is free once the tail of the chunk that
         // last used it has finished; the next head goes in before this
         // chunk's tail, so the longer head chain starts as early as it can
-        for (; finished <= chunk + 1 - kSlots; finished++)
-            if ((rc = finish_tail(c, c->slot[finished % kSlots])) != 0) break;
+        while (finished <= chunk + 1 - kSlots && rc == 0) rc = finish_tail(c, c->slot[finished++ % kSlots]);
         if (rc) break;
         const bool more = plan(c->slot[(chunk + 1) % kSlots], next);
         if (more && (rc = head_stage(c, c->slot[(chunk + 1) % kSlots])) != 0) break;

@@ -133,8 +133,20 @@ __device__ bool vblank_state_machine(const int* rs, const int* rl, int npc, int 
     return done;
 }
 
+// v of lane k (k the same in every lane)
+__device__ __forceinline__ double k2b_lane_value(double v, int k) {
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, k);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), k);
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
 // first-HSYNC anchor given the field parity (pipeline.cu:747-845, without the
-// estimate-only branch which needs the previous field)
+// estimate-only branch which needs the previous field).  Called by all 64
+// lanes of the wave with the same arguments; every lane returns the result.
+// The HSYNC offset average runs over the validated pulses in order: each
+// lane computes the terms of its pulses, and the sum is taken in pulse order
+// from the lanes' values (the reference's sequential rounding).
 __device__ AnchorVariant anchor_variant(VP vp, int nvp, const int* vbp, double mll, bool first_field,
                                         bool prev_nonpos, int num_eq, int fl0, int fl1) {
     AnchorVariant r;
@@ -195,16 +207,33 @@ __device__ AnchorVariant anchor_variant(VP vp, int nvp, const int* vbp, double m
     r.loc_rounded = loc;
     double hoff = 0.0;
     int hc = 0;
-    for (int i = 0; i < nvp; i++) {
-        int tg = vp.tg[i];
-        if ((tg & 15) != kPulseHsync || !(tg & 16)) continue;
-        double lineloc = ((double)vp.start[i] - loc) / mll + hsl;
-        int rl = (int)llround(lineloc);
-        if (rl > cur_lines) break;
-        if (rl >= hsl) {
-            hoff += loc + mll * ((double)rl - hsl) - (double)vp.start[i];
+    const int lane = threadIdx.x & 63;
+    for (int base = 0; base < nvp; base += 64) {
+        const int i = base + lane;
+        bool brk = false, take = false;
+        double term = 0.0;
+        if (i < nvp) {
+            const int tg = vp.tg[i];
+            if ((tg & 15) == kPulseHsync && (tg & 16)) {
+                const double ps = (double)vp.start[i];
+                const double lineloc = (ps - loc) / mll + hsl;
+                const int rl = (int)llround(lineloc);
+                brk = rl > cur_lines;
+                take = !brk && rl >= hsl;
+                term = loc + mll * ((double)rl - hsl) - ps;
+            }
+        }
+        // pulses after the first break are not scanned
+        const unsigned long long mb = __ballot(brk);
+        unsigned long long mt = __ballot(take);
+        if (mb) mt &= (mb & (~mb + 1ull)) - 1ull;
+        while (mt) {
+            const int j = __builtin_ctzll(mt);
+            mt &= mt - 1ull;
+            hoff += k2b_lane_value(term, j);
             hc++;
         }
+        if (mb) break;
     }
     if (hc > 0) {
         hoff /= hc;
@@ -349,10 +378,11 @@ __global__ __launch_bounds__(64) void k_k2b_field(const ChunkCtl* ctl, int nf_ov
     }
     __syncthreads();
     const int n = s_nvp;
-    if (threadIdx.x < 4) {
-        bool ff = (threadIdx.x & 1) != 0;
-        bool pn = (threadIdx.x & 2) != 0;
-        S.var[ff ? 1 : 0][pn ? 1 : 0] = anchor_variant(vp, n, s_vbp, s_mll, ff, pn, num_eq, fl0, fl1);
+    for (int var = 0; var < 4; var++) {
+        const bool ff = (var & 1) != 0;
+        const bool pn = (var & 2) != 0;
+        const AnchorVariant av = anchor_variant(vp, n, s_vbp, s_mll, ff, pn, num_eq, fl0, fl1);
+        if (threadIdx.x == 0) S.var[ff ? 1 : 0][pn ? 1 : 0] = av;
     }
     for (int i = threadIdx.x; i < n; i += 64) {
         g_vp_start[(size_t)f * kVpCap + i] = vs[i];
@@ -493,7 +523,6 @@ __global__ __launch_bounds__(256) void k_k2b_linelocs(const ChunkCtl* ctl, int n
                                                       int* out_starts, int* out_lens, int* out_count,
                                                       int* out_types, double* linelocs) {
     __shared__ int key[kVpCap];
-    __shared__ double s_ll[kLinesPerFrame];
     const int f = blockIdx.x;
     const int nf = nf_override > 0 ? nf_override : ctl->nf;
     if (f >= nf) return;
@@ -549,37 +578,75 @@ __global__ __launch_bounds__(256) void k_k2b_linelocs(const ChunkCtl* ctl, int n
             __syncthreads();
         }
     }
-    if (threadIdx.x == 0) {
-        // host_valid_pulses_to_linelocs_exactish (pipeline.cu:857-893)
-        double max_allowed = mll / 1.5;
-        int cur = 0;
-        for (int line = 0; line < kLinesPerFrame; line++) {
-            double l = A.first_hsync_loc + mll * ((double)line - A.first_hsync_line);
-            s_ll[line] = l;
-            if (cur >= n) continue;
-            double cd = fabs((double)key[cur] - l);
-            double smallest = max_allowed;
-            double loc = -1.0;
-            int si = cur;
-            while (si + 1 < n) {
-                if (cd <= smallest) {
-                    smallest = cd;
-                    cur = si;
-                    loc = (double)key[si];
+    // host_valid_pulses_to_linelocs_exactish (pipeline.cu:857-893), restated.
+    // The reference walks forward from a cursor cur while |key - l| does not
+    // increase, keeping the last index walked with distance <= mll/1.5, and
+    // never looks at index n-1.  With the keys ascending the distance is
+    // unimodal in the index, so the walk from cur ends at g = the last index
+    // of the nearest run over the whole list when g >= cur, and otherwise at
+    // the end of the run of keys equal to key[cur]; the kept index is that
+    // end clamped to n-2, if within tolerance.  Wave 0 finds g for every line
+    // by binary search (lane j: lines j + 64 k), then carries the cursor
+    // through the 525 lines reading those results from the lanes.
+    __shared__ int s_pick[kLinesPerFrame];
+    const double max_allowed = mll / 1.5;
+    if (threadIdx.x < 64) {
+        const int lane = threadIdx.x;
+        constexpr int K = (kLinesPerFrame + 63) / 64;
+        int g[K];
+        int ok[K];
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const int line = lane + 64 * k;
+            g[k] = -1;
+            ok[k] = 0;
+            if (line < kLinesPerFrame && n >= 2) {
+                const double l = A.first_hsync_loc + mll * ((double)line - A.first_hsync_line);
+                int lo = 0, hi = n;  // first index with key >= l
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if ((double)key[mid] < l) lo = mid + 1;
+                    else hi = mid;
                 }
-                double nd = fabs((double)key[si + 1] - l);
-                if (nd > cd) break;
-                cd = nd;
-                si++;
+                int p;
+                if (lo == 0) p = 0;
+                else if (lo == n) p = n - 1;
+                else p = fabs((double)key[lo] - l) <= fabs((double)key[lo - 1] - l) ? lo : lo - 1;
+                while (p + 1 < n && key[p + 1] == key[p]) p++;  // last of the nearest run
+                g[k] = p;
+                const int sl = p < n - 2 ? p : n - 2;
+                ok[k] = fabs((double)key[sl] - l) <= max_allowed;
             }
-            if (loc >= 0.0) {
-                s_ll[line] = loc;
-                cur++;
+        }
+        int cur = 0;
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            for (int j = 0; j < 64; j++) {
+                const int line = 64 * k + j;
+                if (line >= kLinesPerFrame) break;
+                const int gl = __builtin_amdgcn_readlane(g[k], j);
+                int pick = -1;
+                if (cur <= n - 2) {
+                    if (gl >= cur) {
+                        if (__builtin_amdgcn_readlane(ok[k], j)) pick = gl < n - 2 ? gl : n - 2;
+                    } else {
+                        int e = cur;
+                        while (e + 1 < n && key[e + 1] == key[e]) e++;
+                        const int sl = e < n - 2 ? e : n - 2;
+                        const double l = A.first_hsync_loc + mll * ((double)line - A.first_hsync_line);
+                        if (fabs((double)key[sl] - l) <= max_allowed) pick = sl;
+                    }
+                }
+                if (pick >= 0) cur = pick + 1;
+                if (lane == 0) s_pick[line] = pick;
             }
         }
     }
     __syncthreads();
-    for (int line = threadIdx.x; line < kLinesPerFrame; line += 256) ll[line] = s_ll[line];
+    for (int line = threadIdx.x; line < kLinesPerFrame; line += 256) {
+        const int pk = s_pick[line];
+        ll[line] = pk >= 0 ? (double)key[pk] : A.first_hsync_loc + mll * ((double)line - A.first_hsync_line);
+    }
 }
 
 // ---- host launchers ----

@@ -201,12 +201,39 @@ __device__ int ntsc_phase_id_first(int quadrant) {
     return 1;
 }
 
-// Detection, retry replay and state update (chroma_decode.cu:674-1001).
-__global__ void k_chroma_resolve(const ChunkCtl* ctl, DecodeState* dst, int nf_override, const double* m0, const double* m1,
-                                 const double* iq0, int* out_track, int* phase_ids) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// Detection, retry replay and state update (chroma_decode.cu:674-1001).  The
+// logic is sequential (one thread); for chunks of up to kResolveLds fields
+// the metrics are staged in LDS and the tracks built there, so its many
+// dependent reads and rewrites wait on LDS rather than on L2.
+constexpr int kResolveLds = 1024;
+__device__ void resolve_tracks(DecodeState* dst, int nf, const double* m0, const double* m1, const double* iq0,
+                               int* out_track, int* phase_ids);
+__global__ void k_chroma_resolve(const ChunkCtl* ctl, DecodeState* dst, int nf_override, const double* g_m0,
+                                 const double* g_m1, const double* iq0, int* g_track, int* phase_ids) {
+    __shared__ double s_m0[kResolveLds], s_m1[kResolveLds];
+    __shared__ int s_track[kResolveLds];
+    if (blockIdx.x != 0) return;
     const int nf = nf_override > 0 ? nf_override : ctl->nf;
     if (nf <= 0) return;
+    const bool staged = nf <= kResolveLds;
+    if (staged) {
+        for (int i = threadIdx.x; i < nf; i += blockDim.x) {
+            s_m0[i] = g_m0[i];
+            s_m1[i] = g_m1[i];
+        }
+    }
+    __syncthreads();
+    const double* m0 = staged ? s_m0 : g_m0;
+    const double* m1 = staged ? s_m1 : g_m1;
+    int* out_track = staged ? s_track : g_track;
+    if (threadIdx.x == 0) resolve_tracks(dst, nf, m0, m1, iq0, out_track, phase_ids);
+    __syncthreads();
+    if (staged)
+        for (int i = threadIdx.x; i < nf; i += blockDim.x) g_track[i] = s_track[i];
+}
+
+__device__ void resolve_tracks(DecodeState* dst, int nf, const double* m0, const double* m1, const double* iq0,
+                               int* out_track, int* phase_ids) {
     ChromaState cs = dst->chroma;
     const int cycle[8] = {3, 2, 1, 4, 3, 2, 1, 4};
     int det, cycle_start;

@@ -253,7 +253,7 @@ __global__ __launch_bounds__(64) void k_k2b_field(const ChunkCtl* ctl, int nf_ov
     __shared__ int rs[kMaxPulses], rl[kMaxPulses];
     __shared__ int vs[kVpCap], vlen[kVpCap];
     __shared__ unsigned char vtg[kVpCap];
-    __shared__ int s_nvp;
+    __shared__ int s_nvp, s_ovf;
     __shared__ int s_vbp[8];
     __shared__ double s_mll;
     const int f = blockIdx.x;
@@ -267,19 +267,26 @@ __global__ __launch_bounds__(64) void k_k2b_field(const ChunkCtl* ctl, int nf_ov
     __syncthreads();
     VP vp{vs, vlen, vtg};
     K2bField& S = summ[f];
+    // host_get_line_timings (pipeline.cu:450-477): the lengths are integers,
+    // so their sum is exact in any order and the wavefront can share it
+    const double tol = tol_half_us;
+    const double gmin = hsync_width - 3.5 * tol, gmax = hsync_width + 4.0 * tol;
+    double psum = 0.0;
+    int pcnt = 0;
+    for (int i = threadIdx.x; i < npc; i += 64) {
+        double len = (double)rl[i];
+        if (len >= gmin && len <= gmax) { psum += len; pcnt++; }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        psum += __shfl_down(psum, o, 64);
+        pcnt += __shfl_down(pcnt, o, 64);
+    }
     if (threadIdx.x == 0) {
         int ovf = 0;
         int n = 0;
-        // host_get_line_timings (pipeline.cu:450-477)
         Timings lt;
-        double tol = tol_half_us;
-        double gmin = hsync_width - 3.5 * tol, gmax = hsync_width + 4.0 * tol;
-        double sum = 0.0;
-        int cnt = 0;
-        for (int i = 0; i < npc; i++) {
-            double len = (double)rl[i];
-            if (len >= gmin && len <= gmax) { sum += len; cnt++; }
-        }
+        const double sum = psum;
+        const int cnt = pcnt;
         double est = cnt > 0 ? (sum / cnt) : hsync_width;
         double hoff = est - hsync_width;
         double nom = hsync_width + hoff;
@@ -289,74 +296,151 @@ __global__ __launch_bounds__(64) void k_k2b_field(const ChunkCtl* ctl, int nf_ov
         lt.eq_max = eq_width + hoff + tol;
         lt.vsync_min = (vsync_width + hoff) * 0.5;
         lt.vsync_max = vsync_width + hoff + 2.0 * tol;
-        // host_refine_pulses_exactish (pipeline.cu:611-648)
-        for (int i = 0; i < npc;) {
-            int cs = rs[i], cl = rl[i];
+        // host_refine_pulses_exactish (pipeline.cu:611-648).  The last
+        // validated pulse's type and start are kept in registers, and the
+        // next pulse is loaded while this one is processed.
+        int prev_tg = 0, prev_start = 0;
+        int cs = npc > 0 ? rs[0] : 0, cl = npc > 0 ? rl[0] : 0;
+        for (int i = 0; i < npc; i++) {
+            const int ncs = i + 1 < npc ? rs[i + 1] : 0, ncl = i + 1 < npc ? rl[i + 1] : 0;
             if (inr((double)cl, lt.hsync_min, lt.hsync_max)) {
-                bool good = n > 0 && qualitycheck(vtg[n - 1] & 15, kPulseHsync, cs - vs[n - 1], spl);
+                bool good = n > 0 && qualitycheck(prev_tg & 15, kPulseHsync, cs - prev_start, spl);
                 if (n < kVpCap) {
-                    vs[n] = cs; vlen[n] = cl; vtg[n] = (unsigned char)(kPulseHsync | (good ? 16 : 0)); n++;
+                    const unsigned char tg = (unsigned char)(kPulseHsync | (good ? 16 : 0));
+                    vs[n] = cs; vlen[n] = cl; vtg[n] = tg; n++;
+                    prev_tg = tg;
+                    prev_start = cs;
                 } else {
                     ovf = 1;
                 }
-                i++;
-            } else if (i > 2 && inr((double)cl, lt.eq_min, lt.eq_max) && n > 0 && (vtg[n - 1] & 15) == kPulseHsync) {
+            } else if (i > 2 && inr((double)cl, lt.eq_min, lt.eq_max) && n > 0 && (prev_tg & 15) == kPulseHsync) {
                 int n2 = n;
                 bool done = vblank_state_machine(rs, rl, npc, i - 2, lt, num_eq, spl, vp, n, &n2, &ovf);
-                if (done && n2 > n) n = n2;  // keep the appended sequence
-                i++;                          // advances by one either way (pipeline.cu:637-642)
-            } else {
-                i++;
-            }
-        }
-        // host_compute_meanlinelen_from_valid_pulses (pipeline.cu:479-517)
-        int lr_start = -1, lr_len = -1, cr_start = -1, cr_len = 0;
-        for (int i = 0; i < n; i++) {
-            if ((vtg[i] & 15) != kPulseHsync) {
-                if (cr_start >= 0 && cr_len > lr_len) { lr_start = cr_start; lr_len = cr_len; }
-                cr_start = -1;
-                cr_len = 0;
-            } else if (cr_start < 0) {
-                cr_start = i;
-                cr_len = 0;
-            } else {
-                cr_len++;
-            }
-        }
-        if (cr_start >= 0 && cr_len > lr_len) { lr_start = cr_start; lr_len = cr_len; }
-        double nominal = (double)spl;
-        double lsum = 0.0;
-        int lcnt = 0;
-        if (lr_start >= 1 && lr_len >= 1) {
-            for (int i = lr_start + 1; i < lr_start + lr_len; i++) {
-                double ll = (double)(vs[i] - vs[i - 1]);
-                double ratio = ll / nominal;
-                if (ratio >= 0.95 && ratio <= 1.05) { lsum += ll; lcnt++; }
-            }
-        }
-        double mll = lcnt == 0 ? nominal : lsum / (double)lcnt;
-        // vblank pulse scan (pipeline.cu:675-727)
-        double fol[4] = {-1, -1, -1, -1};
-        int vbp[8] = {-1, -1, -1, -1, -1, -1, -1, -1};
-        int last = -1, group = 0, fgroup = 0;
-        for (int i = 0; i < n; i++) {
-            if (last != -1 && (vtg[i] & 16)) {
-                if (group == 0 && vs[i] > vs[0] + fl0 * mll) { group = 4; fgroup = 2; }
-                int lt_ = vtg[last] & 15, ty = vtg[i] & 15;
-                if (lt_ == kPulseHsync && ty > kPulseHsync) {
-                    vbp[0 + group] = vs[i];
-                    fol[0 + fgroup] = round_line_loc((vs[i] - vs[last]) / mll);
-                } else if (lt_ == kPulseEq1 && ty == kPulseVsync) {
-                    vbp[1 + group] = vs[i];
-                } else if (lt_ == kPulseVsync && ty == kPulseEq2) {
-                    vbp[2 + group] = vs[i];
-                } else if (lt_ > kPulseHsync && ty == kPulseHsync) {
-                    vbp[3 + group] = vs[last];
-                    fol[1 + fgroup] = round_line_loc((vs[i] - vs[last]) / mll);
+                if (done && n2 > n) {  // keep the appended sequence
+                    n = n2;
+                    prev_tg = vtg[n - 1];
+                    prev_start = vs[n - 1];
                 }
             }
-            last = i;
+            // every branch advances by one pulse (pipeline.cu:637-642)
+            cs = ncs;
+            cl = ncl;
         }
+        s_nvp = n;
+        s_ovf = ovf;
+    }
+    __syncthreads();
+    const int n = s_nvp;
+    const int lane = threadIdx.x;
+    // host_compute_meanlinelen_from_valid_pulses (pipeline.cu:479-517): the
+    // reference keeps the first longest run of consecutive HSYNCs (its
+    // "length" counts the pulses after the first) and averages the spacings
+    // within +-5 % of nominal over its pulses 1 .. len-1.  Here: the HSYNC
+    // flags as a bitmask in LDS, each run start finds its end in the mask,
+    // and a wavefront max picks the longest (earliest on ties); the spacings
+    // are integers, so their sum is exact in any order.
+    __shared__ unsigned long long s_hmask[kVpCap / 64];
+    const int nw = (n + 63) / 64;
+    for (int w = 0; w < nw; w++) {
+        const int i = w * 64 + lane;
+        const unsigned long long m = __ballot(i < n && (vtg[i] & 15) == kPulseHsync);
+        if (lane == 0) s_hmask[w] = m;
+    }
+    __syncthreads();
+    unsigned long long best = 0;  // (len + 1) << 32 | (0x7fffffff - start); 0 = no run
+    for (int w = 0; w < nw; w++) {
+        const int i = w * 64 + lane;
+        const unsigned long long m = s_hmask[w];
+        const bool h = ((m >> lane) & 1ull) != 0;
+        const bool prev_h = lane > 0 ? ((m >> (lane - 1)) & 1ull) != 0 : (w > 0 && (s_hmask[w - 1] >> 63) != 0);
+        if (i < n && h && !prev_h) {
+            // end of this run: the first clear bit after i (or n)
+            int e = n;
+            unsigned long long rest = ~m & (lane < 63 ? (~0ull << (lane + 1)) : 0ull);
+            if (rest) e = w * 64 + __builtin_ctzll(rest);
+            else
+                for (int w2 = w + 1; w2 < nw; w2++)
+                    if (~s_hmask[w2]) { e = w2 * 64 + __builtin_ctzll(~s_hmask[w2]); break; }
+            if (e > n) e = n;
+            const unsigned long long key =
+                ((unsigned long long)(e - 1 - i + 1) << 32) | (unsigned long long)(0x7fffffff - i);
+            if (key > best) best = key;
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        const unsigned long long other = __shfl_xor(best, o, 64);
+        if (other > best) best = other;
+    }
+    const int lr_start = best ? 0x7fffffff - (int)(best & 0xffffffffull) : -1;
+    const int lr_len = best ? (int)(best >> 32) - 1 : -1;
+    const double nominal = (double)spl;
+    double lsum = 0.0;
+    int lcnt = 0;
+    if (lr_start >= 1 && lr_len >= 1) {
+        for (int i = lr_start + 1 + lane; i < lr_start + lr_len; i += 64) {
+            double ll = (double)(vs[i] - vs[i - 1]);
+            double ratio = ll / nominal;
+            if (ratio >= 0.95 && ratio <= 1.05) { lsum += ll; lcnt++; }
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        lsum += __shfl_xor(lsum, o, 64);
+        lcnt += __shfl_xor(lcnt, o, 64);
+    }
+    const double mll = lcnt == 0 ? nominal : lsum / (double)lcnt;
+    // vblank pulse scan (pipeline.cu:675-727).  Pulse i (i >= 1, good) is
+    // compared with pulse i-1; the reference switches to the second group at
+    // the first good pulse past vs[0] + fl0 * mll and lets the last match of
+    // each kind in each group win.  The kinds are mutually exclusive, so each
+    // (kind, group) is the last matching index, found by ballots.
+    __shared__ int s_jstar;
+    __shared__ int s_vb[8];
+    __shared__ double s_fo[4];
+    if (lane == 0) {
+        s_jstar = n;
+        for (int k = 0; k < 8; k++) s_vb[k] = -1;
+        for (int k = 0; k < 4; k++) s_fo[k] = -1.0;
+    }
+    __syncthreads();
+    const double gthr = vs[0] + fl0 * mll;
+    for (int w = 0; w < nw; w++) {
+        const int i = w * 64 + lane;
+        const bool c = i >= 1 && i < n && (vtg[i] & 16) && vs[i] > gthr;
+        const unsigned long long m = __ballot(c);
+        if (m) {
+            if (lane == 0) s_jstar = w * 64 + __builtin_ctzll(m);
+            break;
+        }
+    }
+    __syncthreads();
+    const int jstar = s_jstar;
+    for (int w = 0; w < nw; w++) {
+        const int i = w * 64 + lane;
+        int kind = -1;
+        if (i >= 1 && i < n && (vtg[i] & 16)) {
+            const int lt_ = vtg[i - 1] & 15, ty = vtg[i] & 15;
+            if (lt_ == kPulseHsync && ty > kPulseHsync) kind = 0;
+            else if (lt_ == kPulseEq1 && ty == kPulseVsync) kind = 1;
+            else if (lt_ == kPulseVsync && ty == kPulseEq2) kind = 2;
+            else if (lt_ > kPulseHsync && ty == kPulseHsync) kind = 3;
+        }
+        const int grp = i >= jstar ? 1 : 0;
+        for (int k = 0; k < 4; k++)
+            for (int g = 0; g < 2; g++) {
+                const unsigned long long m = __ballot(kind == k && grp == g);
+                if (m && lane == 63 - __builtin_clzll(m)) {
+                    s_vb[k + 4 * g] = k == 3 ? vs[i - 1] : vs[i];
+                    if (k == 0) s_fo[0 + 2 * g] = round_line_loc((vs[i] - vs[i - 1]) / mll);
+                    if (k == 3) s_fo[1 + 2 * g] = round_line_loc((vs[i] - vs[i - 1]) / mll);
+                }
+            }
+    }
+    __syncthreads();
+    if (lane == 0) {
+        double fol[4];
+        int vbp[8];
+        for (int k = 0; k < 4; k++) fol[k] = s_fo[k];
+        for (int k = 0; k < 8; k++) vbp[k] = s_vb[k];
         double ffl[4] = {1, 0.5, 0.5, 1}, sfl[4] = {0.5, 1, 1, 0.5};
         double ic = 0.0, id = 0.0;
         for (int i = 0; i < 4; i++) {
@@ -365,7 +449,7 @@ __global__ __launch_bounds__(64) void k_k2b_field(const ChunkCtl* ctl, int nf_ov
         }
         S.nvp = n;
         S.npulses_k2 = npc;
-        S.overflow = ovf;
+        S.overflow = s_ovf;
         S.ic = ic;
         S.id = id;
         S.id_zero = id == 0.0;
@@ -373,11 +457,9 @@ __global__ __launch_bounds__(64) void k_k2b_field(const ChunkCtl* ctl, int nf_ov
         S.meanlinelen = mll;
         S.first_k2_start = npc > 0 ? rs[0] : 0;
         for (int k = 0; k < 8; k++) { S.vblank_pulses[k] = vbp[k]; s_vbp[k] = vbp[k]; }
-        s_nvp = n;
         s_mll = mll;
     }
     __syncthreads();
-    const int n = s_nvp;
     for (int var = 0; var < 4; var++) {
         const bool ff = (var & 1) != 0;
         const bool pn = (var & 2) != 0;

@@ -70,43 +70,66 @@ __device__ __forceinline__ bool qualitycheck(int prev_type, int type, int start_
     return span >= mn && span <= mx;
 }
 
+// qualitycheck with the span start_delta / in_line_len already divided out
+__device__ __forceinline__ bool quality_span(int prev_type, int type, double span) {
+    double mn, mx;
+    if (prev_type > 0 && type > 0) { mn = 0.4; mx = 0.6; }
+    else if (prev_type == 0 && type == 0) { mn = 0.9; mx = 1.1; }
+    else { mn = 0.4; mx = 1.1; }
+    return span >= mn && span <= mx;
+}
+
+// Per-K2-pulse data of the refinement, computed across the wavefront before
+// the sequential passes: width class bits (kClsHs/Eq/Vs) and the spacing to
+// the previous K2 pulse in lines, (double)(rs[i] - rs[i-1]) / line length,
+// the span the quality check takes when pulse i-1 is the last validated one.
+constexpr unsigned char kClsHs = 1, kClsEq = 2, kClsVs = 8;
+struct PulseInfo {
+    const unsigned char* cls;
+    const double* span;
+};
+
 __device__ __forceinline__ double round_line_loc(double line_number) {
     return round(0.5 * round(line_number / 0.5) * 10.0) / 10.0;
 }
 
 // host_run_vblank_state_machine (pipeline.cu:519-609) appending to vp[n..)
 // Returns done; *n_out is the new length (entries past the cap are dropped
-// and flagged).
-__device__ bool vblank_state_machine(const int* rs, const int* rl, int npc, int start_idx, const Timings& lt,
-                                     int num_pulses, int in_line_len, VP vp, int n, int* n_out, int* ovf) {
+// and flagged).  prev_src: the K2 index the last validated pulse came from,
+// or -1; the machine keeps that pulse's type and start in registers.
+__device__ bool vblank_state_machine(const int* rs, const int* rl, PulseInfo pi, int npc, int start_idx,
+                                     int num_pulses, int in_line_len, VP vp, int n, int prev_src, int* n_out,
+                                     int* ovf) {
     bool done = false;
     double num_pulses_half = num_pulses / 2.0;
     int vsync_start = -1;
     double state_end = 0.0;
     double state_length = -1.0;
+    int state = n == 0 ? -1 : (vp.tg[n - 1] & 15);
+    int prev_start = n == 0 ? 0 : vp.start[n - 1];
     for (int i = start_idx; i < npc && i < start_idx + 26; i++) {
-        int ps = rs[i], pl = rl[i];
-        double len = (double)pl;
+        const int ps = rs[i];
+        const unsigned char c = pi.cls[i];
+        const bool hs = (c & kClsHs) != 0, eq = (c & kClsEq) != 0, vsn = (c & kClsVs) != 0;
         int type = -1;
         bool have = false;
-        int state = n == 0 ? -1 : (vp.tg[n - 1] & 15);
         if (state == -1) {
-            if (inr(len, lt.hsync_min, lt.hsync_max)) { type = kPulseHsync; have = true; }
+            if (hs) { type = kPulseHsync; have = true; }
         } else if (state == kPulseHsync) {
-            if (inr(len, lt.hsync_min, lt.hsync_max)) { type = kPulseHsync; have = true; }
-            else if (inr(len, lt.eq_min, lt.eq_max)) { type = kPulseEq1; state_length = num_pulses_half; have = true; }
-            else if (inr(len, lt.vsync_min, lt.vsync_max)) { vsync_start = n - 1; type = kPulseVsync; have = true; }
+            if (hs) { type = kPulseHsync; have = true; }
+            else if (eq) { type = kPulseEq1; state_length = num_pulses_half; have = true; }
+            else if (vsn) { vsync_start = n - 1; type = kPulseVsync; have = true; }
         } else if (state == kPulseEq1) {
-            if (inr(len, lt.eq_min, lt.eq_max)) { type = kPulseEq1; have = true; }
-            else if (inr(len, lt.vsync_min, lt.vsync_max)) { vsync_start = n - 1; type = kPulseVsync; state_length = num_pulses_half; have = true; }
-            else if (inr(len, lt.hsync_min, lt.hsync_max)) { type = kPulseHsync; have = true; }
+            if (eq) { type = kPulseEq1; have = true; }
+            else if (vsn) { vsync_start = n - 1; type = kPulseVsync; state_length = num_pulses_half; have = true; }
+            else if (hs) { type = kPulseHsync; have = true; }
         } else if (state == kPulseVsync) {
-            if (inr(len, lt.eq_min, lt.eq_max)) { type = kPulseEq2; state_length = num_pulses_half; have = true; }
-            else if (inr(len, lt.vsync_min, lt.vsync_max)) { type = kPulseVsync; have = true; }
-            else if ((double)ps > state_end && inr(len, lt.hsync_min, lt.hsync_max)) { type = kPulseHsync; have = true; }
+            if (eq) { type = kPulseEq2; state_length = num_pulses_half; have = true; }
+            else if (vsn) { type = kPulseVsync; have = true; }
+            else if ((double)ps > state_end && hs) { type = kPulseHsync; have = true; }
         } else if (state == kPulseEq2) {
-            if (inr(len, lt.eq_min, lt.eq_max)) { type = kPulseEq2; have = true; }
-            else if (inr(len, lt.hsync_min, lt.hsync_max)) { type = kPulseHsync; have = true; done = true; }
+            if (eq) { type = kPulseEq2; have = true; }
+            else if (hs) { type = kPulseHsync; have = true; done = true; }
         }
         if (have && type != state) {
             if ((double)ps < state_end) {
@@ -117,12 +140,18 @@ __device__ bool vblank_state_machine(const int* rs, const int* rl, int npc, int 
             }
         }
         if (have) {
-            bool good = n == 0 ? false : qualitycheck(vp.tg[n - 1] & 15, type, ps - vp.start[n - 1], in_line_len);
+            bool good = false;
+            if (n > 0)
+                good = prev_src == i - 1 ? quality_span(state, type, pi.span[i])
+                                         : qualitycheck(state, type, ps - prev_start, in_line_len);
             if (n < kVpCap) {
                 vp.start[n] = ps;
-                vp.len[n] = pl;
+                vp.len[n] = rl[i];
                 vp.tg[n] = (unsigned char)(type | (good ? 16 : 0));
                 n++;
+                state = type;
+                prev_start = ps;
+                prev_src = i;
             } else {
                 *ovf = 1;
             }
@@ -278,13 +307,11 @@ __global__ __launch_bounds__(64) void k_k2b_field(const ChunkCtl* ctl, int nf_ov
         if (len >= gmin && len <= gmax) { psum += len; pcnt++; }
     }
     for (int o = 32; o > 0; o >>= 1) {
-        psum += __shfl_down(psum, o, 64);
-        pcnt += __shfl_down(pcnt, o, 64);
+        psum += __shfl_xor(psum, o, 64);
+        pcnt += __shfl_xor(pcnt, o, 64);
     }
-    if (threadIdx.x == 0) {
-        int ovf = 0;
-        int n = 0;
-        Timings lt;
+    Timings lt;
+    {
         const double sum = psum;
         const int cnt = pcnt;
         double est = cnt > 0 ? (sum / cnt) : hsync_width;
@@ -296,35 +323,58 @@ __global__ __launch_bounds__(64) void k_k2b_field(const ChunkCtl* ctl, int nf_ov
         lt.eq_max = eq_width + hoff + tol;
         lt.vsync_min = (vsync_width + hoff) * 0.5;
         lt.vsync_max = vsync_width + hoff + 2.0 * tol;
+    }
+    // per-pulse width classes and spacings (PulseInfo), across the wavefront
+    __shared__ unsigned char s_cls[kMaxPulses];
+    __shared__ double s_span[kMaxPulses];
+    for (int i = threadIdx.x; i < npc; i += 64) {
+        const double len = (double)rl[i];
+        unsigned char c = 0;
+        if (inr(len, lt.hsync_min, lt.hsync_max)) c |= kClsHs;
+        if (inr(len, lt.eq_min, lt.eq_max)) c |= kClsEq;
+        if (inr(len, lt.vsync_min, lt.vsync_max)) c |= kClsVs;
+        s_cls[i] = c;
+        s_span[i] = i > 0 ? (double)(rs[i] - rs[i - 1]) / (double)spl : 0.0;
+    }
+    __syncthreads();
+    const PulseInfo pi{s_cls, s_span};
+    if (threadIdx.x == 0) {
+        int ovf = 0;
+        int n = 0;
         // host_refine_pulses_exactish (pipeline.cu:611-648).  The last
         // validated pulse's type and start are kept in registers, and the
         // next pulse is loaded while this one is processed.
-        int prev_tg = 0, prev_start = 0;
-        int cs = npc > 0 ? rs[0] : 0, cl = npc > 0 ? rl[0] : 0;
+        // prev_src: the K2 index the last validated pulse came from, when the
+        // main loop appended it (-1 after the state machine's appends)
+        int prev_tg = 0, prev_start = 0, prev_src = -1;
         for (int i = 0; i < npc; i++) {
-            const int ncs = i + 1 < npc ? rs[i + 1] : 0, ncl = i + 1 < npc ? rl[i + 1] : 0;
-            if (inr((double)cl, lt.hsync_min, lt.hsync_max)) {
-                bool good = n > 0 && qualitycheck(prev_tg & 15, kPulseHsync, cs - prev_start, spl);
+            const unsigned char c = s_cls[i];
+            if (c & kClsHs) {
+                const int cs = rs[i];
+                bool good = false;
+                if (n > 0)
+                    good = prev_src == i - 1 ? quality_span(prev_tg & 15, kPulseHsync, s_span[i])
+                                             : qualitycheck(prev_tg & 15, kPulseHsync, cs - prev_start, spl);
                 if (n < kVpCap) {
                     const unsigned char tg = (unsigned char)(kPulseHsync | (good ? 16 : 0));
-                    vs[n] = cs; vlen[n] = cl; vtg[n] = tg; n++;
+                    vs[n] = cs; vlen[n] = rl[i]; vtg[n] = tg; n++;
                     prev_tg = tg;
                     prev_start = cs;
+                    prev_src = i;
                 } else {
                     ovf = 1;
                 }
-            } else if (i > 2 && inr((double)cl, lt.eq_min, lt.eq_max) && n > 0 && (prev_tg & 15) == kPulseHsync) {
+            } else if (i > 2 && (c & kClsEq) && n > 0 && (prev_tg & 15) == kPulseHsync) {
                 int n2 = n;
-                bool done = vblank_state_machine(rs, rl, npc, i - 2, lt, num_eq, spl, vp, n, &n2, &ovf);
+                bool done = vblank_state_machine(rs, rl, pi, npc, i - 2, num_eq, spl, vp, n, prev_src, &n2, &ovf);
                 if (done && n2 > n) {  // keep the appended sequence
                     n = n2;
                     prev_tg = vtg[n - 1];
                     prev_start = vs[n - 1];
+                    prev_src = -1;
                 }
             }
             // every branch advances by one pulse (pipeline.cu:637-642)
-            cs = ncs;
-            cl = ncl;
         }
         s_nvp = n;
         s_ovf = ovf;

@@ -142,7 +142,12 @@ void kernel_support(const std::vector<cdbl>& g, double eps, int* lm, int* lp) {
 // it: measured on MI355X, FP64 4096-point K1 blocks (two per CU) run ~1.3x
 // more work per unit time than 8192-point blocks (one per CU), and FP32
 // 4096-point blocks (four per CU) ~1.45x more than FP32 8192-point blocks
-// (two per CU) with the same waves per CU.
+// (two per CU) with the same waves per CU.  The FP64 4096-point kernels now
+// exchange real and imaginary parts separately and run three per CU (limited
+// by registers); that only widens their lead, and the table keeps the
+// whole-complex figures, under which the choices are the same (4096 points
+// for K1 part A at 28-40 MHz; 4096 / 8192 / 8192 for part B at
+// 28 / 35.8 / 40 MHz).
 BlockFilter make_block(const std::vector<cdbl>& g, double eps, const int* bch, int nb, int extra, int elem_bytes) {
     int n = (int)g.size();
     int lm = 0, lp = 0;

@@ -315,11 +315,48 @@ inline int conv4096_gperm(int k) {
     return (k1 >> 2) * 1024 + d * 64 + (k1 & 3) * 16 + c;
 }
 
-// In: v[n1] = x[256 n1 + t].  Out: v[n1] = y[256 n1 + t].  buf: lds_elems(4096)
-// elements; the caller synchronises before reusing it.
-template <typename T>
-__device__ __forceinline__ void conv4096(typename C2<T>::type* buf, typename C2<T>::type* v,
-                                         const typename C2<T>::type* tw, const typename C2<T>::type* Gp) {
+// LDS exchange of the 16 values a lane holds: v[i] is stored at pidx(ws(i))
+// and v[i] reloaded from pidx(rs(i)).  Block: the exchange crosses waves
+// (workgroup barriers), else it stays within the wave.  Whole: complex
+// elements through a buffer of lds_elems(n) V; otherwise the real parts and
+// then the imaginary parts through a buffer of lds_elems(n) T, half the LDS
+// for two more barriers.  The stores may overwrite only locations whose
+// earlier loads were made by the same lane.
+template <bool Block> __device__ __forceinline__ void lds_sync() {
+    if constexpr (Block) __syncthreads();
+    else wave_lds_sync();
+}
+template <bool Block, bool Whole, typename V, typename WS, typename RS>
+__device__ __forceinline__ void lds_exchange(void* lds, V* v, WS ws, RS rs) {
+    if constexpr (Whole) {
+        V* b = static_cast<V*>(lds);
+#pragma unroll
+        for (int i = 0; i < 16; i++) b[pidx(ws(i))] = v[i];
+        lds_sync<Block>();
+#pragma unroll
+        for (int i = 0; i < 16; i++) v[i] = b[pidx(rs(i))];
+    } else {
+        using T = decltype(v[0].x);
+        T* b = static_cast<T*>(lds);
+#pragma unroll
+        for (int i = 0; i < 16; i++) b[pidx(ws(i))] = v[i].x;
+        lds_sync<Block>();
+#pragma unroll
+        for (int i = 0; i < 16; i++) v[i].x = b[pidx(rs(i))];
+        lds_sync<Block>();
+#pragma unroll
+        for (int i = 0; i < 16; i++) b[pidx(ws(i))] = v[i].y;
+        lds_sync<Block>();
+#pragma unroll
+        for (int i = 0; i < 16; i++) v[i].y = b[pidx(rs(i))];
+    }
+}
+
+// In: v[n1] = x[256 n1 + t].  Out: v[n1] = y[256 n1 + t].  lds: lds_elems(4096)
+// elements of V (Whole) or of T; the caller synchronises before reusing it.
+template <typename T, bool Whole = true>
+__device__ __forceinline__ void conv4096(void* lds, typename C2<T>::type* v, const typename C2<T>::type* tw,
+                                         const typename C2<T>::type* Gp) {
     using V = typename C2<T>::type;
     const int t = threadIdx.x;
     const int w = t >> 6, l = t & 63, q = l & 15;
@@ -328,20 +365,14 @@ __device__ __forceinline__ void conv4096(typename C2<T>::type* buf, typename C2<
     dft16<-1, T>(v);
     twiddle_base<T>(wb, tw, t);
     twiddle_run<-1, T>(v, wb);
-#pragma unroll
-    for (int r = 0; r < 16; r++) buf[pidx((r << 8) + t)] = v[r];
-    __syncthreads();
-    // 256-point transforms: n2 = 16 a + b, k2 = c + 16 d
-#pragma unroll
-    for (int a = 0; a < 16; a++) v[a] = buf[pidx(base + 16 * a + q)];          // lane q = b
+    // rows k1 to the waves; 256-point transforms: n2 = 16 a + b, k2 = c + 16 d
+    lds_exchange<true, Whole>(lds, v, [&](int r) { return (r << 8) + t; },
+                              [&](int a) { return base + 16 * a + q; });           // lane q = b
     dft16<-1, T>(v);                                                            // a -> c
     twiddle_base<T>(wb, tw, 16 * q);
     twiddle_run<-1, T>(v, wb);                                                  // W256^(b c)
-#pragma unroll
-    for (int c = 0; c < 16; c++) buf[pidx(base + 16 * c + q)] = v[c];
-    wave_lds_sync();
-#pragma unroll
-    for (int b = 0; b < 16; b++) v[b] = buf[pidx(base + 16 * q + b)];          // lane q = c
+    lds_exchange<false, Whole>(lds, v, [&](int c) { return base + 16 * c + q; },
+                               [&](int b) { return base + 16 * q + b; });          // lane q = c
     dft16<-1, T>(v);                                                            // b -> d
     V g[16];
 #pragma unroll
@@ -351,17 +382,11 @@ __device__ __forceinline__ void conv4096(typename C2<T>::type* buf, typename C2<
     dft16<+1, T>(v);                                                            // d -> b
     twiddle_base<T>(wb, tw, 16 * q);
     twiddle_run<+1, T>(v, wb);
-#pragma unroll
-    for (int b = 0; b < 16; b++) buf[pidx(base + 16 * q + b)] = v[b];
-    wave_lds_sync();
-#pragma unroll
-    for (int c = 0; c < 16; c++) v[c] = buf[pidx(base + 16 * c + q)];          // lane q = b
+    lds_exchange<false, Whole>(lds, v, [&](int b) { return base + 16 * q + b; },
+                               [&](int c) { return base + 16 * c + q; });          // lane q = b
     dft16<+1, T>(v);                                                            // c -> a
-#pragma unroll
-    for (int a = 0; a < 16; a++) buf[pidx(base + 16 * a + q)] = v[a];
-    __syncthreads();
-#pragma unroll
-    for (int r = 0; r < 16; r++) v[r] = buf[pidx((r << 8) + t)];
+    lds_exchange<true, Whole>(lds, v, [&](int a) { return base + 16 * a + q; },
+                              [&](int r) { return (r << 8) + t; });
     twiddle_base<T>(wb, tw, t);
     twiddle_run<+1, T>(v, wb);
     dft16<+1, T>(v);

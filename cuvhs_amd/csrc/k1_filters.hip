@@ -77,11 +77,19 @@ __global__ __launch_bounds__(256) void k_input_mean(const double* partial, long 
 // (the block start is within one segment and a block is shorter than n).
 __device__ __forceinline__ int wrap_n(int q, int n) { return q < 0 ? q + n : (q >= n ? q - n : q); }
 
+// FP64 4096-point blocks exchange real and imaginary parts through LDS
+// separately (35 KB instead of 70 KB per workgroup).
+template <int B, typename T>
+constexpr bool k1_whole() { return !(B == 4096 && sizeof(T) == 8); }
+template <int B, typename T>
+constexpr size_t k1_lds_bytes() { return (size_t)lds_elems(B) * (k1_whole<B, T>() ? 2 * sizeof(T) : sizeof(T)); }
+
 // Workgroups per CU the register allocation must allow: four FP32
 // 4096-point blocks fit one CU's LDS (35 KB each), so their kernels are held
-// to 128 VGPRs; the FP64 ones are limited to two by LDS anyway.
+// to 128 VGPRs; FP64 4096-point blocks (35 KB with split exchanges) to 168
+// VGPRs, three per CU.
 template <int B, typename T>
-constexpr int k1_min_blocks() { return (B == 4096 && sizeof(T) == 4) ? 4 : 1; }
+constexpr int k1_min_blocks() { return B == 4096 ? (sizeof(T) == 4 ? 4 : 3) : 1; }
 
 // Phase of z = x + iy in the transform's precision.
 __device__ __forceinline__ double phase_of(double y, double x) { return atan2(y, x); }
@@ -132,7 +140,7 @@ __global__ __launch_bounds__(B / 16, (k1_min_blocks<B, T>())) void k1_part_a(K1A
     {
         // G is in the wave-local block's bin order; v[r] becomes a(qs + o),
         // o = NT r + t - lp
-        if constexpr (B == 4096) conv4096<T>(buf, v, tw, G);
+        if constexpr (B == 4096) conv4096<T, k1_whole<B, T>()>(smem, v, tw, G);
         else conv_rows<T, 512>(buf, v, tw, G);
         constexpr int NW = NT / 64;
         __shared__ T edge[NW][16];
@@ -206,7 +214,7 @@ __global__ __launch_bounds__(B / 16, (k1_min_blocks<B, T>())) void k1_part_b(K1B
     }
     static_assert(B == 4096 || B == 8192, "K1 blocks are wave-local 4096- or 8192-point blocks");
     {
-        if constexpr (B == 4096) conv4096<T>(buf, v, tw, G);
+        if constexpr (B == 4096) conv4096<T, k1_whole<B, T>()>(smem, v, tw, G);
         else conv_rows<T, 512>(buf, v, tw, G);
 #pragma unroll
         for (int r = 0; r < 16; r++) {
@@ -282,7 +290,7 @@ static void ensure_lds(K* kern, size_t sh) {
 template <int B, typename T>
 static hipError_t launch_a(const K1ALaunch& a, int nf, hipStream_t st) {
     int nb = (int)((a.spfp + a.V - 1) / a.V);
-    size_t sh = (size_t)lds_elems(B) * sizeof(typename C2<T>::type);
+    size_t sh = k1_lds_bytes<B, T>();
     ensure_lds(k1_part_a<B, T>, sh);
     hipLaunchKernelGGL((k1_part_a<B, T>), dim3(nb, nf), dim3(B / 16), sh, st, a);
     return hipGetLastError();
@@ -290,7 +298,7 @@ static hipError_t launch_a(const K1ALaunch& a, int nf, hipStream_t st) {
 template <int B, typename T>
 static hipError_t launch_b(const K1BLaunch& a, int nf, hipStream_t st) {
     int nb = (int)((a.spfp + a.V - 1) / a.V);
-    size_t sh = (size_t)lds_elems(B) * sizeof(typename C2<T>::type);
+    size_t sh = k1_lds_bytes<B, T>();
     ensure_lds(k1_part_b<B, T>, sh);
     hipLaunchKernelGGL((k1_part_b<B, T>), dim3(nb, nf), dim3(B / 16), sh, st, a);
     return hipGetLastError();

@@ -88,6 +88,7 @@ struct cuvhs_ctx {
     Slot slot[kSlots];
     double* rawdemod = nullptr;     // head only
     int* cand = nullptr;            // head only
+    unsigned int* thr_bits = nullptr;  // head only: sync-threshold bitmask of d05
     int *k2s = nullptr, *k2l = nullptr, *k2c = nullptr;
     int *ps = nullptr, *pl = nullptr, *pc = nullptr, *pt = nullptr;
     int *vps = nullptr, *vpl = nullptr;
@@ -238,9 +239,10 @@ ChromaLaunch chroma_launch(cuvhs_ctx* c, const ChunkCtl* ctl, int nf_override, c
     return a;
 }
 
-// ctl (optional): the chunk's control block, initialised by K1's first workgroup
+// ctl (optional): the chunk's control block, initialised by K1's first
+// workgroup.  thr_bits (optional): receives the sync-threshold bitmask of d05.
 int k1_launch(cuvhs_ctx* c, Slot& s, const void* chunk, long long n_valid, int nfk1, hipStream_t st,
-              ChunkCtl* ctl = nullptr) {
+              ChunkCtl* ctl = nullptr, unsigned int* thr_bits = nullptr) {
     const Fmt& f = c->fmt;
     long long T = (long long)nfk1 * f.spfp;
     K1ALaunch a{};
@@ -262,6 +264,7 @@ int k1_launch(cuvhs_ctx* c, Slot& s, const void* chunk, long long n_valid, int n
     a.ctl = ctl;
     a.ctl_raw_offset = (long long)s.raw_offset;
     a.ctl_fields_k1 = s.fk1;
+    a.thr_bits = thr_bits;
     HIPCHK(c, run_k1_part_a(c->k1.rf.B, c->k1_f32, a, nfk1, st));
     mark(c, s, 1, st);
     K1BLaunch b{};
@@ -276,6 +279,8 @@ int k1_launch(cuvhs_ctx* c, Slot& s, const void* chunk, long long n_valid, int n
     b.tw = c->tw_b;
     b.demod = s.demod;
     b.demod05 = s.d05;
+    b.thr_bits = thr_bits;
+    b.thr = f.pulse_threshold_hz;
     HIPCHK(c, run_k1_part_b(c->k1.video.B, c->k1_f32, b, nfk1, st));
     // beyond the demodulated region the buffers read as zero: only what an
     // earlier, longer chunk left there needs clearing
@@ -341,11 +346,11 @@ int head_stage(cuvhs_ctx* c, Slot& s) {
     HIPCHK(c, hipEventRecord(s.pre1, side));
     ChunkCtl* h = s.hctl;
     mark(c, s, 0, st);
-    r = k1_launch(c, s, s.chunk, (long long)s.n_read, s.fk1, st, s.ctl);
+    r = k1_launch(c, s, s.chunk, (long long)s.n_read, s.fk1, st, s.ctl, c->thr_bits);
     if (r) return r;
     mark(c, s, 2, st);
     const long long T = (long long)s.fk1 * f.spfp;
-    HIPCHK(c, run_events(s.d05, T, f.pulse_threshold_hz, s.wg_count, s.wg_off, s.ev_masks, s.events, s.ctl, st));
+    HIPCHK(c, run_events_bits(c->thr_bits, T, s.wg_count, s.wg_off, s.ev_masks, s.events, s.ctl, st));
     mark(c, s, 3, st);
     HIPCHK(c, run_vsync_cluster(s.events, s.ctl, T, f.vsync_min_w, f.vsync_max_w, s.fk1 * 20, c->cand, s.offsets,
                                 s.max_fields, f.samples_per_field, f.samples_per_line, s.fk1, 1, st));
@@ -596,6 +601,7 @@ int cuvhs_create(const cuvhs_config* cfg, cuvhs_ctx** out) {
     c->buf_n = S + f.spfp;  // one segment of zero padding past the chunk
     ALLOC(c->rawdemod, S);
     ALLOC(c->cand, 20 * c->F);
+    ALLOC(c->thr_bits, thr_bits_words(S));
     for (Slot& s : c->slot) {
         ALLOC(s.demod, c->buf_n);
         ALLOC(s.d05, c->buf_n);
@@ -901,7 +907,8 @@ int cuvhs_vsync_discover(cuvhs_ctx* c, const double* d05, int total, int* idx, i
     if (!c || !d05 || !idx || !count || total < 1 || cap < 0 || total > (long long)c->F * c->fmt.spfp) return CUVHS_E_ARG;
     const Fmt& f = c->fmt;
     Slot& s = c->slot[0];
-    HIPCHK(c, run_events(d05, total, f.pulse_threshold_hz, s.wg_count, s.wg_off, s.ev_masks, s.events, s.ctl, c->stream));
+    HIPCHK(c, run_events(d05, total, f.pulse_threshold_hz, c->thr_bits, s.wg_count, s.wg_off, s.ev_masks, s.events,
+                         s.ctl, c->stream));
     HIPCHK(c, run_vsync_cluster(s.events, s.ctl, total, f.vsync_min_w, f.vsync_max_w, cap, idx, s.offsets, 1,
                                 f.samples_per_field, f.samples_per_line, 1, 0, c->stream));
     HIPCHK(c, hipMemcpyAsync(s.hctl, s.ctl, sizeof(ChunkCtl), hipMemcpyDeviceToHost, c->stream));
@@ -915,7 +922,8 @@ int cuvhs_sync_pulses(cuvhs_ctx* c, const double* d05, int total, const uint64_t
     if (!c || !d05 || !offsets || nf < 1 || total < 1 || total > (long long)c->F * c->fmt.spfp) return CUVHS_E_ARG;
     const Fmt& f = c->fmt;
     Slot& s = c->slot[0];
-    HIPCHK(c, run_events(d05, total, f.pulse_threshold_hz, s.wg_count, s.wg_off, s.ev_masks, s.events, s.ctl, c->stream));
+    HIPCHK(c, run_events(d05, total, f.pulse_threshold_hz, c->thr_bits, s.wg_count, s.wg_off, s.ev_masks, s.events,
+                         s.ctl, c->stream));
     HIPCHK(c, run_sync_pulses(s.events, s.ctl, (const unsigned long long*)offsets, f.spfp, starts, lens, count, nf,
                               nf, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));

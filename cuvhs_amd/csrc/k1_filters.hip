@@ -117,6 +117,14 @@ __global__ __launch_bounds__(B / 16, (k1_min_blocks<B, T>())) void k1_part_a(K1A
     }
     const int qs = (kOverlap - 1) + b * a.V;  // first analytic output
     if (qs + 1 >= kOverlap + a.spfp) return;
+    if (a.thr_bits) {
+        // clear the threshold bitmask words of this block's samples
+        // [f spfp + b V, f spfp + min((b + 1) V, spfp)) for part B's ORs
+        // (words shared with a neighbouring block are cleared by both)
+        const long long s0 = (long long)f * a.spfp + (long long)b * a.V;
+        const long long s1 = (long long)f * a.spfp + min((long long)(b + 1) * a.V, (long long)a.spfp);
+        for (long long w = (s0 >> 5) + threadIdx.x; w <= ((s1 - 1) >> 5); w += NT) a.thr_bits[w] = 0u;
+    }
     const int seg0 = f * a.spfp - kOverlap;
     const int last = (int)a.total - 1;
     const double dcv = a.dc ? *a.dc : 0.0;
@@ -224,6 +232,48 @@ __global__ __launch_bounds__(B / 16, (k1_min_blocks<B, T>())) void k1_part_b(K1B
                 int s = f * a.spfp + (q - kOverlap);
                 a.demod[s] = (double)v[r].x;
                 a.demod05[s] = (double)v[r].y;
+            }
+        }
+        if (a.thr_bits) {
+            // sync-threshold bitmask of the stored demod_05 values over the
+            // block's samples [s_lo, s_end), assembled in LDS: the
+            // wavefront's 64 consecutive samples of row r as one ballot,
+            // ORed into the (up to three) words they straddle.  Words inside
+            // the range are then stored whole; the first and last, shared
+            // with the neighbouring blocks when partly covered, are ORed
+            // into the bitmask part A cleared.
+            constexpr int kWords = B / 32 + 4;
+            __shared__ unsigned int s_bits[kWords];
+            const int lane = (int)threadIdx.x & 63;
+            const long long s_lo = (long long)f * a.spfp + (qs - kOverlap);
+            const long long s_end = (long long)f * a.spfp + (min(qs + a.V, qend) - kOverlap);
+            const long long w_lo = s_lo >> 5, w_hi = (s_end - 1) >> 5;
+            const int nw = (int)(w_hi - w_lo) + 1;
+            const int off = (int)(s_lo & 31);
+            for (int i = (int)threadIdx.x; i < kWords; i += NT) s_bits[i] = 0u;
+            __syncthreads();
+#pragma unroll
+            for (int r = 0; r < 16; r++) {
+                const int o = r * NT + (int)threadIdx.x - a.lp;
+                const int q = qs + o;
+                const bool in = o >= 0 && o < a.V && q < qend;
+                const unsigned long long m = __ballot(in && (double)v[r].y <= a.thr);
+                if (m != 0ull && lane < 3) {
+                    const int rel = off + (o - lane);  // lane 0's sample - 32 w_lo
+                    const int sh = rel & 31;
+                    unsigned int part;
+                    if (lane == 0) part = (unsigned int)(m << sh);
+                    else if (lane == 1) part = (unsigned int)((m << sh) >> 32);
+                    else part = sh ? (unsigned int)(m >> (64 - sh)) : 0u;
+                    if (part) atomicOr(&s_bits[(rel >> 5) + lane], part);
+                }
+            }
+            __syncthreads();
+            for (int i = (int)threadIdx.x; i < nw; i += NT) {
+                const unsigned int word = s_bits[i];
+                const bool partial = (i == 0 && off != 0) || (i == nw - 1 && (s_end & 31) != 0);
+                if (!partial) a.thr_bits[w_lo + i] = word;
+                else if (word) atomicOr(a.thr_bits + (w_lo + i), word);
             }
         }
     }

@@ -24,6 +24,7 @@ struct K1ALaunch {
     ChunkCtl* ctl;              // if set, initialised by the first workgroup:
     long long ctl_raw_offset;   //   zeros, fields_k1 and raw_offset
     int ctl_fields_k1;
+    unsigned int* thr_bits;     // if set, zeroed over [0, total) for part B
 };
 struct K1BLaunch {
     const double* rawdemod;
@@ -33,6 +34,8 @@ struct K1BLaunch {
     const void* tw;
     double* demod;
     double* demod05;
+    unsigned int* thr_bits;     // if set: bit s = (demod05[s] <= thr), s < total
+    double thr;
 };
 struct PreLaunch {
     const void* cap;
@@ -79,8 +82,13 @@ hipError_t run_input_mean(const void* cap, int in_fmt, long long n, double* part
 hipError_t run_fft_selftest(int B, int is_f32, const void* in, void* out, const void* tw, int sign, hipStream_t st);
 int events_nwg(long long total);
 size_t events_mask_count(long long total);  // uint2 entries of the masks buffer
-hipError_t run_events(const double* x, long long total, double thr, int* wg_count, int* wg_off, uint2* masks,
-                      int* events, ChunkCtl* ctl, hipStream_t st);
+size_t thr_bits_words(long long total);     // 32-bit words of a threshold bitmask
+// events from the threshold bitmask (bit s = x[s] <= thr), as K1 part B writes it
+hipError_t run_events_bits(const unsigned int* bits, long long total, int* wg_count, int* wg_off, uint2* masks,
+                           int* events, ChunkCtl* ctl, hipStream_t st);
+// the same from a demod_05 array: bitmask into bits, then run_events_bits
+hipError_t run_events(const double* x, long long total, double thr, unsigned int* bits, int* wg_count, int* wg_off,
+                      uint2* masks, int* events, ChunkCtl* ctl, hipStream_t st);
 hipError_t run_vsync_cluster(const int* events, ChunkCtl* ctl, long long total, int min_w, int max_w, int cap,
                              int* cand, unsigned long long* offsets, int num_fields, int spf, int spl, int fields_k1,
                              int do_cluster, hipStream_t st);

@@ -5,14 +5,18 @@
 // count, atomic append), pipeline.cu:1567-1651 (host sort + clustering),
 // sync_pulses.cu:10-82 (one thread per field scanning ~5e5 samples).
 //
-// Here the signal is scanned once: every workgroup turns 8192 samples into
-// ordered run-boundary events (start = first sample <= thr, end = first
-// sample > thr after a run) with wavefront ballots and a workgroup scan, and
-// all three consumers work on that short, sorted event list.  Both the
-// reference's serial loops are functions of these boundaries alone:
+// Here the signal is reduced once to a bitmask (bit s = demod_05[s] <= thr),
+// written by K1 part B as it produces demod_05 (or by k_thr_bits from an
+// array), and every workgroup turns 8192 bits into ordered run-boundary
+// events (start = first sample <= thr, end = first sample > thr after a run)
+// with a workgroup scan; all three consumers work on that short, sorted
+// event list.  Both the reference's serial loops are functions of these
+// boundaries alone:
 //   VSYNC width = min(end - start, max_w + 1), end = chunk end if no end;
 //   K2 keeps runs with off < start and end < off + spfp, length end - start.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 
 #include "common.h"
 #include "launch.h"
@@ -23,35 +27,44 @@ constexpr int kEvThreads = 256;
 constexpr int kEvPerThread = 32;
 constexpr int kEvSpan = kEvThreads * kEvPerThread;  // samples per workgroup
 
-// events of sample i: bit0 = start, bit1 = end
-__device__ __forceinline__ int event_at(const double* x, long long i, double thr) {
-    bool b = x[i] <= thr;
-    if (i == 0) return b ? 1 : 0;
-    bool p = x[i - 1] <= thr;
-    if (b && !p) return 1;
-    if (!b && p) return 2;
-    return 0;
+// Threshold bitmask of an array: one ballot per 64 samples (64-aligned, so
+// each wavefront writes two whole words; the buffer holds thr_bits_words).
+__global__ __launch_bounds__(256) void k_thr_bits(const double* x, long long total, double thr, unsigned int* bits) {
+    const int lane = threadIdx.x & 63;
+    const long long stride = (long long)gridDim.x * 256;
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i - lane < total; i += stride) {
+        const unsigned long long m = __ballot(i < total && x[i] <= thr);
+        if (lane == 0) {
+            bits[i >> 5] = (unsigned int)m;
+            bits[(i >> 5) + 1] = (unsigned int)(m >> 32);
+        }
+    }
 }
 
-// Sample i of workgroup g, iteration k, thread t: g*kEvSpan + k*256 + t, so
-// each load instruction of a wavefront reads 64 consecutive doubles.  Each
-// thread's start/end events are kept as two 32-bit masks (bit k: iteration
-// k) for k_ev_write, which so never reads the signal again.
-__global__ __launch_bounds__(kEvThreads) void k_ev_count(const double* x, long long total, double thr, int* wg_count,
+// Thread j owns samples 32 j .. 32 j + 31 (bitmask word j).  Start events:
+// bit set, previous sample's clear (sample 0: set); end events: bit clear,
+// previous set; none at or past total.  The (start, end) masks are kept for
+// k_ev_write, which so never reads the bitmask again.
+__global__ __launch_bounds__(kEvThreads) void k_ev_count(const unsigned int* bits, long long total, int* wg_count,
                                                          uint2* masks) {
     __shared__ int s_total;
     if (threadIdx.x == 0) s_total = 0;
     __syncthreads();
-    const long long base = (long long)blockIdx.x * kEvSpan + threadIdx.x;
+    const long long j = (long long)blockIdx.x * kEvThreads + threadIdx.x;
+    const long long s = j * 32;
     unsigned int fs = 0, fe = 0;
-#pragma unroll 8
-    for (int k = 0; k < kEvPerThread; k++) {
-        long long i = base + (long long)k * kEvThreads;
-        int e = i < total ? event_at(x, i, thr) : 0;
-        if (e == 1) fs |= 1u << k;
-        if (e == 2) fe |= 1u << k;
+    if (s < total) {
+        const unsigned int b = bits[j];
+        const unsigned int p = (b << 1) | (j > 0 ? bits[j - 1] >> 31 : 0u);  // bit k: sample s + k - 1
+        fs = b & ~p;
+        fe = ~b & p;
+        if (total - s < 32) {
+            const unsigned int valid = (1u << (unsigned int)(total - s)) - 1u;
+            fs &= valid;
+            fe &= valid;
+        }
     }
-    masks[(size_t)blockIdx.x * kEvThreads + threadIdx.x] = make_uint2(fs, fe);
+    masks[j] = make_uint2(fs, fe);
     int c = __popc(fs | fe);
     // wavefront sum, then one LDS atomic per wavefront
     for (int off = 32; off > 0; off >>= 1) c += __shfl_down(c, off, 64);
@@ -85,36 +98,31 @@ __global__ __launch_bounds__(1024) void k_ev_scan(const int* wg_count, int nwg, 
     if (threadIdx.x == 0) ctl->nevents = carry;
 }
 
-// events encoded as 2*pos + is_end, ascending.  Order inside a workgroup is
-// (iteration k, wavefront w, lane) = ascending sample index; the offsets come
-// from per-(k, w) ballot counts.
+// events encoded as 2*pos + is_end, ascending: threads own consecutive
+// samples, so a thread's slot is its workgroup's offset plus the exclusive
+// scan of the per-thread counts.
 __global__ __launch_bounds__(kEvThreads) void k_ev_write(const uint2* masks, const int* wg_off, int* events) {
     constexpr int kWaves = kEvThreads / 64;
-    __shared__ int s_cnt[kEvPerThread][kWaves];
+    __shared__ int s_wave[kWaves];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const long long base = (long long)blockIdx.x * kEvSpan + threadIdx.x;
-    const uint2 msk = masks[(size_t)blockIdx.x * kEvThreads + threadIdx.x];
-    const unsigned int fs = msk.x, fe = msk.y;
-    const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    for (int k = 0; k < kEvPerThread; k++) {
-        unsigned long long m = __ballot(((fs | fe) >> k) & 1u);
-        if (lane == 0) s_cnt[k][wave] = __popcll(m);
+    const long long j = (long long)blockIdx.x * kEvThreads + threadIdx.x;
+    const uint2 msk = masks[j];
+    unsigned int all = msk.x | msk.y;
+    const int c = __popc(all);
+    int x = c;  // inclusive scan over the wavefront
+    for (int off = 1; off < 64; off <<= 1) {
+        const int y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
     }
+    if (lane == 63) s_wave[wave] = x;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        int acc = wg_off[blockIdx.x];
-        for (int k = 0; k < kEvPerThread; k++)
-            for (int w = 0; w < kWaves; w++) { int c = s_cnt[k][w]; s_cnt[k][w] = acc; acc += c; }
-    }
-    __syncthreads();
-    for (int k = 0; k < kEvPerThread; k++) {
-        unsigned int bit = ((fs | fe) >> k) & 1u;
-        unsigned long long m = __ballot(bit);
-        if (bit) {
-            long long i = base + (long long)k * kEvThreads;
-            int pos = s_cnt[k][wave] + __popcll(m & lt);
-            events[pos] = (int)(2 * i + ((fe >> k) & 1u));
-        }
+    int pos = wg_off[blockIdx.x] + x - c;
+    for (int w = 0; w < wave; w++) pos += s_wave[w];
+    const long long s0 = j * 32;
+    while (all) {
+        const int k = __ffs(all) - 1;
+        events[pos++] = (int)(2 * (s0 + k) + ((msk.y >> k) & 1u));
+        all &= all - 1u;
     }
 }
 
@@ -279,14 +287,25 @@ int events_nwg(long long total) { return (int)((total + kEvSpan - 1) / kEvSpan);
 
 size_t events_mask_count(long long total) { return (size_t)events_nwg(total) * kEvThreads; }
 
-hipError_t run_events(const double* x, long long total, double thr, int* wg_count, int* wg_off, uint2* masks,
-                      int* events, ChunkCtl* ctl, hipStream_t st) {
+size_t thr_bits_words(long long total) { return (size_t)((total + 63) / 64) * 2; }
+
+hipError_t run_events_bits(const unsigned int* bits, long long total, int* wg_count, int* wg_off, uint2* masks,
+                           int* events, ChunkCtl* ctl, hipStream_t st) {
     int nwg = events_nwg(total);
     if (nwg == 0) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_ev_count, dim3(nwg), dim3(kEvThreads), 0, st, x, total, thr, wg_count, masks);
+    hipLaunchKernelGGL(k_ev_count, dim3(nwg), dim3(kEvThreads), 0, st, bits, total, wg_count, masks);
     hipLaunchKernelGGL(k_ev_scan, dim3(1), dim3(1024), 0, st, wg_count, nwg, wg_off, ctl);
     hipLaunchKernelGGL(k_ev_write, dim3(nwg), dim3(kEvThreads), 0, st, masks, wg_off, events);
     return hipGetLastError();
+}
+
+hipError_t run_events(const double* x, long long total, double thr, unsigned int* bits, int* wg_count, int* wg_off,
+                      uint2* masks, int* events, ChunkCtl* ctl, hipStream_t st) {
+    if (total <= 0) return hipErrorInvalidValue;
+    long long waves = (total + 63) / 64;
+    int nb = (int)std::min<long long>((waves + 3) / 4, 4096);
+    hipLaunchKernelGGL(k_thr_bits, dim3(nb), dim3(256), 0, st, x, total, thr, bits);
+    return run_events_bits(bits, total, wg_count, wg_off, masks, events, ctl, st);
 }
 
 hipError_t run_vsync_cluster(const int* events, ChunkCtl* ctl, long long total, int min_w, int max_w, int cap,

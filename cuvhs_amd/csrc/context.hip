@@ -53,6 +53,7 @@ struct Slot {
     double* dc_partial = nullptr; // its partial sums
     hipEvent_t dc_done = nullptr;
     hipEvent_t head_done = nullptr, tail_done = nullptr, pre0 = nullptr, pre1 = nullptr;
+    hipEvent_t k4_done = nullptr, aux_done = nullptr;  // fork / join of the tail's aux stream
     hipEvent_t mark[kMarks] = {};
     // host view of the chunk
     const void* chunk = nullptr;
@@ -70,6 +71,7 @@ struct cuvhs_ctx {
     bool own_stream = false;
     hipStream_t tail = nullptr;     // tail stage
     hipStream_t side = nullptr;     // chroma pre-band-pass, next to the tail's sync stages
+    hipStream_t aux = nullptr;      // tail stage: luma resample and dropout classes beside chroma
     std::string err;
     K1Design k1;
     PreDesign pre;
@@ -406,17 +408,34 @@ int tail_stage(cuvhs_ctx* c, Slot& s, const OutArrays* out) {
     int* doe = out && out->do_endx ? out->do_endx : c->doe;
     unsigned long long* floc = out && out->file_loc ? reinterpret_cast<unsigned long long*>(out->file_loc) : c->file_loc;
     int* isf = out && out->is_first ? out->is_first : c->isf_out;
-    HIPCHK(c, run_tbc_resample(s.demod, c->ll, luma, s.ctl, 0, s.max_fields, f, st));
+    // K5 and the dropout classes need only the line locations: they run on
+    // the aux stream beside chroma, and the dropout scan, which conceals into
+    // both outputs, joins them (serial profiling keeps one stream)
+    const bool fork = !c->serial && !c->diag;
+    hipStream_t ax = fork ? c->aux : st;
+    if (fork) {
+        HIPCHK(c, hipEventRecord(s.k4_done, st));
+        HIPCHK(c, hipStreamWaitEvent(ax, s.k4_done, 0));
+    }
+    HIPCHK(c, run_tbc_resample(s.demod, c->ll, luma, s.ctl, 0, s.max_fields, f, ax));
     if (c->diag) HIPCHK(c, run_diag_k5(s.demod, c->ll, s.ctl, s.max_fields, f, st));
     mark(c, s, 9, st);
+    if (fork) {
+        HIPCHK(c, run_dropout_classes(s.env, c->ll, s.ctl, 0, s.max_fields, f, c->do_partial, c->do_cls, ax));
+        HIPCHK(c, hipEventRecord(s.aux_done, ax));
+    }
     HIPCHK(c, hipStreamWaitEvent(st, s.pre1, 0));
     mark(c, s, 10, st);
     ChromaLaunch cl = chroma_launch(c, s.ctl, 0, s.craw);
     cl.out = chroma;
     HIPCHK(c, run_chroma(cl, c->state, s.max_fields, c->m0, c->m1, c->cs_tab, c->iq0, c->track, phase, 1, st));
     mark(c, s, 11, st);
-    HIPCHK(c, run_dropouts(s.env, c->ll, s.ctl, 0, s.max_fields, f, c->do_partial, c->do_cls, luma, chroma, dol, dos,
-                           doe, doc, st));
+    if (fork) {
+        HIPCHK(c, hipStreamWaitEvent(st, s.aux_done, 0));
+    } else {
+        HIPCHK(c, run_dropout_classes(s.env, c->ll, s.ctl, 0, s.max_fields, f, c->do_partial, c->do_cls, st));
+    }
+    HIPCHK(c, run_dropout_scan(c->do_cls, c->ll, s.ctl, 0, s.max_fields, f, luma, chroma, dol, dos, doe, doc, st));
     mark(c, s, 12, st);
     HIPCHK(c, run_field_meta(s.ctl, c->state, s.offsets, c->isf_k2b, floc, isf, st));
     mark(c, s, 13, st);
@@ -444,6 +463,7 @@ int sync_all(cuvhs_ctx* c) {
     HIPCHK(c, hipStreamSynchronize(c->stream));
     HIPCHK(c, hipStreamSynchronize(c->tail));
     HIPCHK(c, hipStreamSynchronize(c->side));
+    HIPCHK(c, hipStreamSynchronize(c->aux));
     return 0;
 }
 
@@ -527,8 +547,10 @@ int cuvhs_create(const cuvhs_config* cfg, cuvhs_ctx** out) {
     if (e != hipSuccess) return bad(e, "hipStreamCreate tail");
     e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking);
     if (e != hipSuccess) return bad(e, "hipStreamCreate side");
+    e = hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking);
+    if (e != hipSuccess) return bad(e, "hipStreamCreate aux");
     for (Slot& s : c->slot) {
-        for (hipEvent_t* ev : {&s.head_done, &s.tail_done, &s.dc_done}) {
+        for (hipEvent_t* ev : {&s.head_done, &s.tail_done, &s.dc_done, &s.k4_done, &s.aux_done}) {
             e = hipEventCreateWithFlags(ev, hipEventDisableTiming);
             if (e != hipSuccess) return bad(e, "hipEventCreate");
         }
@@ -670,15 +692,16 @@ int cuvhs_create(const cuvhs_config* cfg, cuvhs_ctx** out) {
 
 int cuvhs_destroy(cuvhs_ctx* c) {
     if (!c) return CUVHS_E_ARG;
-    for (hipStream_t st : {c->stream, c->tail, c->side})
+    for (hipStream_t st : {c->stream, c->tail, c->side, c->aux})
         if (st) (void)hipStreamSynchronize(st);
     for (Slot& s : c->slot) {
         for (hipEvent_t e : s.mark)
             if (e) (void)hipEventDestroy(e);
-        for (hipEvent_t e : {s.head_done, s.tail_done, s.pre0, s.pre1, s.dc_done})
+        for (hipEvent_t e : {s.head_done, s.tail_done, s.pre0, s.pre1, s.dc_done, s.k4_done, s.aux_done})
             if (e) (void)hipEventDestroy(e);
         if (s.hctl) (void)hipHostFree(s.hctl);
     }
+    if (c->aux) (void)hipStreamDestroy(c->aux);
     if (c->side) (void)hipStreamDestroy(c->side);
     if (c->tail) (void)hipStreamDestroy(c->tail);
     for (void* p : c->allocs) (void)hipFree(p);
@@ -827,6 +850,7 @@ int cuvhs_profile(cuvhs_ctx* c, int enable) {
     HIPCHK(c, hipStreamSynchronize(c->stream));
     HIPCHK(c, hipStreamSynchronize(c->tail));
     HIPCHK(c, hipStreamSynchronize(c->side));
+    HIPCHK(c, hipStreamSynchronize(c->aux));
     c->prof = enable != 0;
     c->serial = enable == 2;
     for (double& v : c->stage_ms) v = 0.0;

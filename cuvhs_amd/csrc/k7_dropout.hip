@@ -295,17 +295,30 @@ __global__ void k_field_meta(const ChunkCtl* ctl, DecodeState* dst, const unsign
 size_t dropout_class_bytes(int max_fields) { return (size_t)max_fields * kMaxClassBlocks; }
 size_t dropout_partial_count(int max_fields) { return (size_t)max_fields * kSumSlices; }
 
-hipError_t run_dropouts(const double* env, const double* ll, const ChunkCtl* ctl, int nf_override, int max_fields,
-                        const Fmt& fmt, double* partial, unsigned char* cls, uint16_t* luma, uint16_t* chroma,
-                        int* lines, int* starts, int* ends, int* count, hipStream_t st) {
+hipError_t run_dropout_classes(const double* env, const double* ll, const ChunkCtl* ctl, int nf_override,
+                               int max_fields, const Fmt& fmt, double* partial, unsigned char* cls, hipStream_t st) {
     if ((fmt.spfp + kEnvBlock - 1) / kEnvBlock > kMaxClassBlocks) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_do_sum, dim3(max_fields, kSumSlices), dim3(kDoThreads), 0, st, env, ll, ctl, nf_override,
                        fmt.spfp, fmt.active_line_start, partial);
     hipLaunchKernelGGL(k_do_class, dim3(max_fields, kClassSlices), dim3(kDoThreads), 0, st, env, ll, ctl, nf_override,
                        fmt.spfp, fmt.active_line_start, partial, cls);
+    return hipGetLastError();
+}
+
+hipError_t run_dropout_scan(const unsigned char* cls, const double* ll, const ChunkCtl* ctl, int nf_override,
+                            int max_fields, const Fmt& fmt, uint16_t* luma, uint16_t* chroma, int* lines, int* starts,
+                            int* ends, int* count, hipStream_t st) {
     hipLaunchKernelGGL(k_dropouts, dim3(max_fields), dim3(kDoThreads), 0, st, cls, ll, ctl, nf_override, fmt.spfp,
                        fmt.active_line_start, luma, chroma, lines, starts, ends, count);
     return hipGetLastError();
+}
+
+hipError_t run_dropouts(const double* env, const double* ll, const ChunkCtl* ctl, int nf_override, int max_fields,
+                        const Fmt& fmt, double* partial, unsigned char* cls, uint16_t* luma, uint16_t* chroma,
+                        int* lines, int* starts, int* ends, int* count, hipStream_t st) {
+    hipError_t e = run_dropout_classes(env, ll, ctl, nf_override, max_fields, fmt, partial, cls, st);
+    if (e != hipSuccess) return e;
+    return run_dropout_scan(cls, ll, ctl, nf_override, max_fields, fmt, luma, chroma, lines, starts, ends, count, st);
 }
 
 hipError_t run_field_meta(const ChunkCtl* ctl, DecodeState* dst, const unsigned long long* offsets,

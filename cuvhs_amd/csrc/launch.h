@@ -113,6 +113,14 @@ size_t dropout_partial_count(int max_fields);
 hipError_t run_dropouts(const double* env, const double* ll, const ChunkCtl* ctl, int nf_override, int max_fields,
                         const Fmt& fmt, double* partial, unsigned char* cls, uint16_t* luma, uint16_t* chroma,
                         int* lines, int* starts, int* ends, int* count, hipStream_t st);
+// the two halves of run_dropouts: envelope sums and block classes (need only
+// env and line locations), then the scan and concealment (luma and chroma
+// written first)
+hipError_t run_dropout_classes(const double* env, const double* ll, const ChunkCtl* ctl, int nf_override,
+                               int max_fields, const Fmt& fmt, double* partial, unsigned char* cls, hipStream_t st);
+hipError_t run_dropout_scan(const unsigned char* cls, const double* ll, const ChunkCtl* ctl, int nf_override,
+                            int max_fields, const Fmt& fmt, uint16_t* luma, uint16_t* chroma, int* lines, int* starts,
+                            int* ends, int* count, hipStream_t st);
 hipError_t run_field_meta(const ChunkCtl* ctl, DecodeState* dst, const unsigned long long* offsets,
                           const int* is_first_k2b, unsigned long long* file_loc, int* is_first_out, hipStream_t st);
 

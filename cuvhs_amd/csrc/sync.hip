@@ -128,13 +128,26 @@ __global__ __launch_bounds__(kEvThreads) void k_ev_write(const uint2* masks, con
 
 // VSYNC discovery over the event list + chunk clustering (one workgroup).
 // offsets[] and ctl->{nf, next_raw_offset, vsync_count, status} are written.
-// The candidate compaction takes 4096 events per step, four consecutive ones
-// per thread (events alternate start/end, so at most two candidates each):
-// two ballots give each candidate its rank in the wavefront, the 16
-// wavefront counts are exchanged through LDS (double-buffered, one barrier
-// per step), and the next step's events are loaded before the barrier.  The
-// first kCandLds candidates are also kept in LDS for the clustering loop.
+// The candidate compaction takes 16384 events per step, sixteen consecutive
+// ones per thread (events alternate start/end, so at most eight candidates
+// each; the seventeenth event closes the last run): a wavefront scan of the
+// per-thread counts and the 16 wavefront totals exchanged through LDS
+// (double-buffered, one barrier per step) give each candidate its slot.
+// The first kCandLds candidates are also kept in LDS for the clustering.
 constexpr int kCandLds = 4096;
+constexpr int kVsEvPerThread = 16;
+constexpr int kVsStep = 1024 * kVsEvPerThread;
+
+// pipeline.cu:1598-1610 on sorted candidates: candidate j is accepted iff
+// it lies more than spf/2 past the last accepted one's safe offset (its
+// position less 5 lines), the first one always.  Positions ascend, so the
+// candidate accepted after i is the first j with pos[j] > safe(i) + spf/2
+// (upper_bound; j > i since spf/2 > 5 lines), and the accepted set is the
+// chain 0, next(0), next(next(0)), ...
+__device__ __forceinline__ long long vs_safe(int pos, int spl) {
+    return pos > 5 * spl ? (long long)pos - 5LL * spl : 0LL;
+}
+
 __global__ __launch_bounds__(1024) void k_vsync_cluster(const int* events, ChunkCtl* ctl, long long total,
                                                         int min_w, int max_w, int cap, int* cand,
                                                         unsigned long long* offsets, int num_fields, int spf,
@@ -142,74 +155,105 @@ __global__ __launch_bounds__(1024) void k_vsync_cluster(const int* events, Chunk
     constexpr int kWaves = 1024 / 64;
     __shared__ int wcnt[2][kWaves];
     __shared__ int lcand[kCandLds];
+    __shared__ int lnext[kCandLds];
     const int nev = ctl->nevents;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     int carry_all = 0;
-    // event k of the step, or 0 past the end; e[4] is the next group's first
-    auto load = [&](int k0, int* e) {
+    for (int base = 0, it = 0; base < nev; base += kVsStep, it ^= 1) {
+        const int k0 = base + kVsEvPerThread * (int)threadIdx.x;
+        int e[kVsEvPerThread + 1];
+        if (k0 + kVsEvPerThread < nev) {
+            const int4* e4 = reinterpret_cast<const int4*>(events + k0);
 #pragma unroll
-        for (int q = 0; q < 5; q++) e[q] = k0 + q < nev ? events[k0 + q] : 0;
-    };
-    int k0 = 4 * (int)threadIdx.x;
-    int e[5], nx[5];
-    load(k0, e);
-    for (int base = 0, it = 0; base < nev; base += 4096, it ^= 1) {
-        int pos[2] = {0, 0};
-        int c = 0;
+            for (int q = 0; q < kVsEvPerThread / 4; q++) {
+                const int4 x = e4[q];
+                e[4 * q] = x.x;
+                e[4 * q + 1] = x.y;
+                e[4 * q + 2] = x.z;
+                e[4 * q + 3] = x.w;
+            }
+            e[kVsEvPerThread] = events[k0 + kVsEvPerThread];
+        } else {
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
+            for (int q = 0; q <= kVsEvPerThread; q++) e[q] = k0 + q < nev ? events[k0 + q] : 0;
+        }
+        unsigned int cm = 0;  // bit q: event k0 + q starts a VSYNC-width run
+#pragma unroll
+        for (int q = 0; q < kVsEvPerThread; q++) {
             const int k = k0 + q;
             if (k < nev && (e[q] & 1) == 0) {
-                const int p = e[q] >> 1;
+                const long long p = e[q] >> 1;
                 long long end = total;
                 if (k + 1 < nev) end = e[q + 1] >> 1;  // next event is this run's end
                 long long w = end - p;
                 if (w > max_w + 1) w = max_w + 1;
-                if (w >= min_w && w <= max_w) pos[c++ & 1] = p;
+                if (w >= min_w && w <= max_w) cm |= 1u << q;
             }
         }
-        load(k0 + 4096, nx);
-        const unsigned long long m0 = __ballot(c >= 1), m1 = __ballot(c >= 2);
-        if (lane == 0) wcnt[it][wave] = __popcll(m0) + __popcll(m1);
+        const int c = __popc(cm);
+        int x = c;  // inclusive scan over the wavefront
+        for (int off = 1; off < 64; off <<= 1) {
+            const int y = __shfl_up(x, off, 64);
+            if (lane >= off) x += y;
+        }
+        if (lane == 63) wcnt[it][wave] = x;
         __syncthreads();
         int woff = 0, tot = 0;
 #pragma unroll
         for (int w = 0; w < kWaves; w++) {
-            int v = wcnt[it][w];
+            const int v = wcnt[it][w];
             woff += w < wave ? v : 0;
             tot += v;
         }
-        const int idx = carry_all + woff + __popcll(m0 & lt) + __popcll(m1 & lt);
-        for (int q = 0; q < c; q++) {
-            if (idx + q < cap) cand[idx + q] = pos[q];
-            if (idx + q < kCandLds) lcand[idx + q] = pos[q];
+        int idx = carry_all + woff + x - c;
+#pragma unroll
+        for (int q = 0; q < kVsEvPerThread; q++) {
+            if ((cm >> q) & 1u) {
+                const int p = e[q] >> 1;
+                if (idx < cap) cand[idx] = p;
+                if (idx < kCandLds) lcand[idx] = p;
+                idx++;
+            }
         }
         carry_all += tot;
-        k0 += 4096;
-#pragma unroll
-        for (int q = 0; q < 5; q++) e[q] = nx[q];
     }
     __syncthreads();
-    if (threadIdx.x != 0 || !do_cluster) {
+    if (!do_cluster) {
         if (threadIdx.x == 0) ctl->vsync_count = carry_all;
         return;
     }
-    // pipeline.cu:1595-1651 (sequential, <= 20 * fields candidates)
-    int count = carry_all;
+    const int count = carry_all;
+    const int stored = count < cap ? count : cap;
+    const bool in_lds = stored <= kCandLds;
+    if (in_lds) {
+        const long long half = spf / 2;
+        for (int i = threadIdx.x; i < stored; i += 1024) {
+            const long long thr = vs_safe(lcand[i], spl) + half;
+            int lo = i + 1, hi = stored;  // first j in (i, stored) with lcand[j] > thr
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if ((long long)lcand[mid] > thr) hi = mid;
+                else lo = mid + 1;
+            }
+            lnext[i] = lo;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
     ctl->vsync_count = count;
     int status = 0;
-    int stored = count < cap ? count : cap;
     if (count > cap) status |= 1;
     int n = 0;
-    unsigned long long last = 0;
-    for (int i = 0; i < stored; i++) {
-        int pos = i < kCandLds ? lcand[i] : cand[i];
-        if (n == 0 || ((unsigned long long)pos - last > (unsigned long long)(spf / 2))) {
-            unsigned long long safe = (pos > 5 * spl) ? (unsigned long long)(pos - 5 * spl) : 0ull;
-            if (n < num_fields) offsets[n] = safe;
-            n++;
-            last = safe;
+    if (in_lds) {
+        for (int i = 0; i < stored && n < num_fields; i = lnext[i]) offsets[n++] = (unsigned long long)vs_safe(lcand[i], spl);
+    } else {
+        unsigned long long last = 0;
+        for (int i = 0; i < stored && n < num_fields; i++) {
+            const int pos = i < kCandLds ? lcand[i] : cand[i];
+            if (n == 0 || ((unsigned long long)pos - last > (unsigned long long)(spf / 2))) {
+                last = (unsigned long long)vs_safe(pos, spl);
+                offsets[n++] = last;
+            }
         }
     }
     if (n == 0) {

@@ -53,7 +53,7 @@ struct Slot {
     double* dc_partial = nullptr; // its partial sums
     hipEvent_t dc_done = nullptr;
     hipEvent_t head_done = nullptr, tail_done = nullptr, pre0 = nullptr, pre1 = nullptr;
-    hipEvent_t k4_done = nullptr, aux_done = nullptr;  // fork / join of the tail's aux stream
+    hipEvent_t k4_done = nullptr, aux_done = nullptr;  // fork / join of the tail's side-stream work
     hipEvent_t mark[kMarks] = {};
     // host view of the chunk
     const void* chunk = nullptr;
@@ -71,7 +71,6 @@ struct cuvhs_ctx {
     bool own_stream = false;
     hipStream_t tail = nullptr;     // tail stage
     hipStream_t side = nullptr;     // chroma pre-band-pass, next to the tail's sync stages
-    hipStream_t aux = nullptr;      // tail stage: luma resample and dropout classes beside chroma
     std::string err;
     K1Design k1;
     PreDesign pre;
@@ -328,27 +327,36 @@ bool plan_chunk(cuvhs_ctx* c, Slot& s, const void* chunk, uint64_t avail, uint64
     return true;
 }
 
-// Head stage on the head stream: K1, sync events, VSYNC clustering; the
-// chroma pre-band-pass, which needs only the capture, starts at the same time
-// on the side stream.
-int head_stage(cuvhs_ctx* c, Slot& s) {
+// The chroma pre-band-pass of a chunk, which needs only the capture, on the
+// side stream (after the head stage's input mean under --dc-correct).
+int pre_stage(cuvhs_ctx* c, Slot& s) {
+    hipStream_t side = c->serial ? c->stream : c->side;
+    if (c->cfg.dc_correct) HIPCHK(c, hipStreamWaitEvent(side, s.dc_done, 0));
+    HIPCHK(c, hipEventRecord(s.pre0, side));
+    int r = prebpf_launch(c, s.chunk, (long long)s.n_read, s.max_fields, s.craw, s.dc, side);
+    if (r) return r;
+    HIPCHK(c, hipEventRecord(s.pre1, side));
+    return 0;
+}
+
+// Head stage on the head stream: K1, sync events, VSYNC clustering; with
+// pre, the chunk's pre-band-pass goes in first on the side stream.
+int head_stage(cuvhs_ctx* c, Slot& s, bool pre = true) {
     const Fmt& f = c->fmt;
     hipStream_t st = c->stream;
-    hipStream_t side = c->serial ? c->stream : c->side;
     if (c->cfg.dc_correct) {
         // per-read mean of the chunk's samples, needed by both K1 and the
         // pre-band-pass (raw_reader.cpp:93-100)
         HIPCHK(c, run_input_mean(s.chunk, c->cfg.input_format, (long long)s.n_read, s.dc_partial, s.dc, st));
         HIPCHK(c, hipEventRecord(s.dc_done, st));
-        HIPCHK(c, hipStreamWaitEvent(side, s.dc_done, 0));
     }
-    HIPCHK(c, hipEventRecord(s.pre0, side));
-    int r = prebpf_launch(c, s.chunk, (long long)s.n_read, s.max_fields, s.craw, s.dc, side);
-    if (r) return r;
-    HIPCHK(c, hipEventRecord(s.pre1, side));
+    if (pre) {
+        int r = pre_stage(c, s);
+        if (r) return r;
+    }
     ChunkCtl* h = s.hctl;
     mark(c, s, 0, st);
-    r = k1_launch(c, s, s.chunk, (long long)s.n_read, s.fk1, st, s.ctl, c->thr_bits);
+    int r = k1_launch(c, s, s.chunk, (long long)s.n_read, s.fk1, st, s.ctl, c->thr_bits);
     if (r) return r;
     mark(c, s, 2, st);
     const long long T = (long long)s.fk1 * f.spfp;
@@ -409,10 +417,11 @@ int tail_stage(cuvhs_ctx* c, Slot& s, const OutArrays* out) {
     unsigned long long* floc = out && out->file_loc ? reinterpret_cast<unsigned long long*>(out->file_loc) : c->file_loc;
     int* isf = out && out->is_first ? out->is_first : c->isf_out;
     // K5 and the dropout classes need only the line locations: they run on
-    // the aux stream beside chroma, and the dropout scan, which conceals into
-    // both outputs, joins them (serial profiling keeps one stream)
+    // the side stream beside chroma, and the dropout scan, which conceals
+    // into both outputs, joins them (serial profiling keeps one stream).  The
+    // side stream's next pre-band-pass goes in after them (cuvhs_decode_capture).
     const bool fork = !c->serial && !c->diag;
-    hipStream_t ax = fork ? c->aux : st;
+    hipStream_t ax = fork ? c->side : st;
     if (fork) {
         HIPCHK(c, hipEventRecord(s.k4_done, st));
         HIPCHK(c, hipStreamWaitEvent(ax, s.k4_done, 0));
@@ -463,7 +472,6 @@ int sync_all(cuvhs_ctx* c) {
     HIPCHK(c, hipStreamSynchronize(c->stream));
     HIPCHK(c, hipStreamSynchronize(c->tail));
     HIPCHK(c, hipStreamSynchronize(c->side));
-    HIPCHK(c, hipStreamSynchronize(c->aux));
     return 0;
 }
 
@@ -547,8 +555,6 @@ int cuvhs_create(const cuvhs_config* cfg, cuvhs_ctx** out) {
     if (e != hipSuccess) return bad(e, "hipStreamCreate tail");
     e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking);
     if (e != hipSuccess) return bad(e, "hipStreamCreate side");
-    e = hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking);
-    if (e != hipSuccess) return bad(e, "hipStreamCreate aux");
     for (Slot& s : c->slot) {
         for (hipEvent_t* ev : {&s.head_done, &s.tail_done, &s.dc_done, &s.k4_done, &s.aux_done}) {
             e = hipEventCreateWithFlags(ev, hipEventDisableTiming);
@@ -692,7 +698,7 @@ int cuvhs_create(const cuvhs_config* cfg, cuvhs_ctx** out) {
 
 int cuvhs_destroy(cuvhs_ctx* c) {
     if (!c) return CUVHS_E_ARG;
-    for (hipStream_t st : {c->stream, c->tail, c->side, c->aux})
+    for (hipStream_t st : {c->stream, c->tail, c->side})
         if (st) (void)hipStreamSynchronize(st);
     for (Slot& s : c->slot) {
         for (hipEvent_t e : s.mark)
@@ -701,7 +707,6 @@ int cuvhs_destroy(cuvhs_ctx* c) {
             if (e) (void)hipEventDestroy(e);
         if (s.hctl) (void)hipHostFree(s.hctl);
     }
-    if (c->aux) (void)hipStreamDestroy(c->aux);
     if (c->side) (void)hipStreamDestroy(c->side);
     if (c->tail) (void)hipStreamDestroy(c->tail);
     for (void* p : c->allocs) (void)hipFree(p);
@@ -814,12 +819,17 @@ int cuvhs_decode_capture(cuvhs_ctx* c, const void* cap, uint64_t cap_len, int ma
         o.do_endx = d_do_endx ? d_do_endx + (size_t)total * kMaxDropouts : nullptr;
         // the next chunk's buffer set is free once the tail of the chunk that
         // last used it has finished; the next head goes in before this
-        // chunk's tail, so the longer head chain starts as early as it can
+        // chunk's tail, so the head chain starts as early as it can.  The
+        // next pre-band-pass goes in after this tail: it shares the side
+        // stream with the tail's forked work, and a process has few hardware
+        // queues, so the order on the side stream is the order the work is
+        // needed in.
         while (finished <= chunk + 1 - kSlots && rc == 0) rc = finish_tail(c, c->slot[finished++ % kSlots]);
         if (rc) break;
         const bool more = plan(c->slot[(chunk + 1) % kSlots], next);
-        if (more && (rc = head_stage(c, c->slot[(chunk + 1) % kSlots])) != 0) break;
+        if (more && (rc = head_stage(c, c->slot[(chunk + 1) % kSlots], false)) != 0) break;
         if ((rc = tail_stage(c, s, &o)) != 0) break;
+        if (more && (rc = pre_stage(c, c->slot[(chunk + 1) % kSlots])) != 0) break;
         total += nf;
         chunk++;
         if (!more) break;
@@ -850,7 +860,6 @@ int cuvhs_profile(cuvhs_ctx* c, int enable) {
     HIPCHK(c, hipStreamSynchronize(c->stream));
     HIPCHK(c, hipStreamSynchronize(c->tail));
     HIPCHK(c, hipStreamSynchronize(c->side));
-    HIPCHK(c, hipStreamSynchronize(c->aux));
     c->prof = enable != 0;
     c->serial = enable == 2;
     for (double& v : c->stage_ms) v = 0.0;

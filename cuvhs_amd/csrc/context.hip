@@ -1045,4 +1045,12 @@ int cuvhs_selftest_fft(int B, int is_f32, const void* d_in, void* d_out, int sig
     return e == hipSuccess ? 0 : CUVHS_E_HIP;
 }
 
+// FP64 arctangent self test (not a reference stage).
+int cuvhs_selftest_atan2(const double* d_y, const double* d_x, double* d_out, int64_t n) {
+    if (!d_y || !d_x || !d_out || n < 1) return CUVHS_E_ARG;
+    hipError_t e = run_atan2_selftest(d_y, d_x, d_out, (long long)n, nullptr);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    return e == hipSuccess ? 0 : CUVHS_E_HIP;
+}
+
 }  // extern "C"

@@ -19,8 +19,10 @@
 // and the inverse, with two workgroup barriers per block.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <atomic>
 
+#include "atan_f64.h"
 #include "common.h"
 #include "fft_lds.h"
 #include "launch.h"
@@ -91,8 +93,8 @@ constexpr size_t k1_lds_bytes() { return (size_t)lds_elems(B) * (k1_whole<B, T>(
 template <int B, typename T>
 constexpr int k1_min_blocks() { return B == 4096 ? (sizeof(T) == 4 ? 4 : 3) : 1; }
 
-// Phase of z = x + iy in the transform's precision.
-__device__ __forceinline__ double phase_of(double y, double x) { return atan2(y, x); }
+// Phase of z = x + iy in the transform's precision (atan_f64.h for FP64).
+__device__ __forceinline__ double phase_of(double y, double x) { return atan2_f64(y, x); }
 __device__ __forceinline__ float phase_of(float y, float x) { return atan2f(y, x); }
 
 // Part A: analytic signal, envelope and instantaneous frequency.  T is the
@@ -412,6 +414,19 @@ __global__ __launch_bounds__(B / 16) void k_fft_selftest(const typename C2<T>::t
         else fft_lds<+1, T, B>(buf, tw);
     }
     for (int j = threadIdx.x; j < B; j += B / 16) out[j] = buf[pidx(j)];
+}
+
+// FP64 arctangent self-test (atan_f64.h) over device arrays.
+__global__ __launch_bounds__(256) void k_atan2_selftest(const double* y, const double* x, double* out, long long n) {
+    const long long stride = (long long)gridDim.x * 256;
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) out[i] = atan2_f64(y[i], x[i]);
+}
+
+hipError_t run_atan2_selftest(const double* y, const double* x, double* out, long long n, hipStream_t st) {
+    if (n <= 0) return hipErrorInvalidValue;
+    const int nb = (int)std::min<long long>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_atan2_selftest, dim3(nb), dim3(256), 0, st, y, x, out, n);
+    return hipGetLastError();
 }
 
 template <int B, typename T>

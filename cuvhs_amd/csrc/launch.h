@@ -80,6 +80,7 @@ hipError_t run_diag_k5(const double* demod, const double* ll, ChunkCtl* ctl, int
                        hipStream_t st);
 hipError_t run_input_mean(const void* cap, int in_fmt, long long n, double* partial, double* mean, hipStream_t st);
 hipError_t run_fft_selftest(int B, int is_f32, const void* in, void* out, const void* tw, int sign, hipStream_t st);
+hipError_t run_atan2_selftest(const double* y, const double* x, double* out, long long n, hipStream_t st);
 int events_nwg(long long total);
 size_t events_mask_count(long long total);  // uint2 entries of the masks buffer
 size_t thr_bits_words(long long total);     // 32-bit words of a threshold bitmask

@@ -189,6 +189,11 @@ int cuvhs_dropouts(cuvhs_ctx* ctx, const double* d_env, const double* d_linelocs
  * forward transform fed from registers, as the blocked convolutions use it. */
 int cuvhs_selftest_fft(int B, int is_f32, const void* d_in, void* d_out, int sign);
 
+/* Self-test of the FP64 arctangent of K1's phase (no counterpart in the
+ * reference; replaces the atan2 of fm_demod.cu:203):
+ * d_out[i] = atan2(d_y[i], d_x[i]) for i < n, device arrays of doubles. */
+int cuvhs_selftest_atan2(const double* d_y, const double* d_x, double* d_out, int64_t n);
+
 /* Per-stage device time (HIP events on the stream each stage runs on)
  * accumulated over chunk decodes while enabled; enabling clears the totals.
  * enable = 1 keeps the overlapped schedule (stages that share the GPU with

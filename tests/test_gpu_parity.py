@@ -50,6 +50,31 @@ def test_lds_fft_vs_numpy(torch, N, B, f32):
         assert np.abs(y - ref).max() / np.abs(ref).max() < tol
 
 
+def test_atan2_f64_vs_host(torch, N):
+    """K1's FP64 arctangent (atan_f64.h) against numpy's: within 4 ulp of
+    the result everywhere, signed zeros and axes exact; inputs spread over
+    nine decades, near the tan(pi/8) reduction threshold and the diagonal."""
+    rng = np.random.default_rng(7)
+    n = 1 << 20
+    y = rng.standard_normal(n) * 10.0 ** rng.uniform(-3, 6, n)
+    x = rng.standard_normal(n) * 10.0 ** rng.uniform(-3, 6, n)
+    k = n // 4
+    x[:k] = np.abs(x[:k])
+    y[:k] = x[:k] * 0.41421356237309503 * (1 + rng.uniform(-1e-12, 1e-12, k))
+    x[k:2 * k] = y[k:2 * k] * (1 + rng.uniform(-1e-9, 1e-9, k))
+    sy = np.array([0.0, -0.0, 0.0, -0.0, 1.0, -1.0, 0.0, 0.0, 1.0, 1.0, -1.0, 3.0])
+    sx = np.array([0.0, 0.0, -0.0, -0.0, 0.0, 0.0, 1.0, -1.0, 1.0, -1.0, -1.0, -3.0])
+    y = np.concatenate([y, sy])
+    x = np.concatenate([x, sx])
+    out = torch.empty(y.size, dtype=torch.float64, device="cuda:0")
+    assert N.lib().cuvhs_selftest_atan2(N.ptr(dev(torch, y)), N.ptr(dev(torch, x)), N.ptr(out), y.size) == 0
+    got = out.cpu().numpy()
+    want = np.arctan2(y, x)
+    ulp = np.abs(got - want) / np.spacing(np.abs(want))
+    assert ulp[:n].max() <= 4.0
+    assert np.array_equal(got[n:], want[n:]) and np.array_equal(np.signbit(got[n:]), np.signbit(want[n:]))
+
+
 def test_fm_demod_vs_oracle(torch, N, oracle, cap0):
     nf = 4
     dec = N.Decoder(MHZ, 64)

@@ -380,7 +380,9 @@ struct OutArrays {
 // Tail stage on the tail stream.  out (optional): caller arrays that receive
 // the chunk's outputs in place of the context's, with room for the field
 // count the head stage produced.
-int tail_stage(cuvhs_ctx* c, Slot& s, const OutArrays* out) {
+// fork: the stream that takes K5 and the dropout classes beside chroma (the
+// tail stream itself in serial profiling and diagnostics).
+int tail_stage(cuvhs_ctx* c, Slot& s, const OutArrays* out, hipStream_t fork_stream) {
     const Fmt& f = c->fmt;
     hipStream_t st = c->serial ? c->stream : c->tail;
     HIPCHK(c, hipStreamWaitEvent(st, s.head_done, 0));
@@ -417,11 +419,10 @@ int tail_stage(cuvhs_ctx* c, Slot& s, const OutArrays* out) {
     unsigned long long* floc = out && out->file_loc ? reinterpret_cast<unsigned long long*>(out->file_loc) : c->file_loc;
     int* isf = out && out->is_first ? out->is_first : c->isf_out;
     // K5 and the dropout classes need only the line locations: they run on
-    // the side stream beside chroma, and the dropout scan, which conceals
-    // into both outputs, joins them (serial profiling keeps one stream).  The
-    // side stream's next pre-band-pass goes in after them (cuvhs_decode_capture).
-    const bool fork = !c->serial && !c->diag;
-    hipStream_t ax = fork ? c->side : st;
+    // the fork stream beside chroma, and the dropout scan, which conceals
+    // into both outputs, joins them
+    const bool fork = !c->serial && !c->diag && fork_stream != st;
+    hipStream_t ax = fork ? fork_stream : st;
     if (fork) {
         HIPCHK(c, hipEventRecord(s.k4_done, st));
         HIPCHK(c, hipStreamWaitEvent(ax, s.k4_done, 0));
@@ -753,7 +754,7 @@ int cuvhs_decode_chunk(cuvhs_ctx* c, const void* chunk, uint64_t avail, uint64_t
     info->status = h.status;
     info->next_raw_offset = h.next_raw_offset;
     if (h.nf > 0) {
-        r = tail_stage(c, s, nullptr);
+        r = tail_stage(c, s, nullptr, c->side);
         if (r) { sync_all(c); return r; }
         r = finish_tail(c, s);
         if (r) return r;
@@ -818,18 +819,17 @@ int cuvhs_decode_capture(cuvhs_ctx* c, const void* cap, uint64_t cap_len, int ma
         o.do_startx = d_do_startx ? d_do_startx + (size_t)total * kMaxDropouts : nullptr;
         o.do_endx = d_do_endx ? d_do_endx + (size_t)total * kMaxDropouts : nullptr;
         // the next chunk's buffer set is free once the tail of the chunk that
-        // last used it has finished; the next head goes in before this
-        // chunk's tail, so the head chain starts as early as it can.  The
-        // next pre-band-pass goes in after this tail: it shares the side
-        // stream with the tail's forked work, and a process has few hardware
-        // queues, so the order on the side stream is the order the work is
-        // needed in.
+        // last used it has finished; the next head (with its pre-band-pass on
+        // the side stream) goes in before this chunk's tail, so the head
+        // chain starts as early as it can.  The tail's forked work goes on
+        // the head stream behind that head: a process has few hardware
+        // queues, and there it delays neither the pre-band-pass the next
+        // tail waits for nor this tail.
         while (finished <= chunk + 1 - kSlots && rc == 0) rc = finish_tail(c, c->slot[finished++ % kSlots]);
         if (rc) break;
         const bool more = plan(c->slot[(chunk + 1) % kSlots], next);
-        if (more && (rc = head_stage(c, c->slot[(chunk + 1) % kSlots], false)) != 0) break;
-        if ((rc = tail_stage(c, s, &o)) != 0) break;
-        if (more && (rc = pre_stage(c, c->slot[(chunk + 1) % kSlots])) != 0) break;
+        if (more && (rc = head_stage(c, c->slot[(chunk + 1) % kSlots])) != 0) break;
+        if ((rc = tail_stage(c, s, &o, c->stream)) != 0) break;
         total += nf;
         chunk++;
         if (!more) break;

@@ -392,6 +392,57 @@ __device__ __forceinline__ void conv4096(void* lds, typename C2<T>::type* v, con
     dft16<+1, T>(v);
 }
 
+// ---- 1024-point filter block on one wavefront ----
+//
+// y = IFFT(M(FFT(x))) for 1024 = 16 x 64 on 64 lanes, M applied bin by bin:
+//   X[k1 + 16 k2] = sum_n2 W64^(n2 k2) W1024^(n2 k1) sum_n1 x[64 n1 + n2] W16^(n1 k1)
+// Lane l first does the DFT16 over n1 of the inputs it holds (x[64 n1 + l])
+// and the W1024^(l k1) twiddles.  Row k1 (64 points) then belongs to lanes
+// 4 k1 .. 4 k1 + 3 (lane l = 4 j + q: row j, column class q) and is
+// transformed as n2 = 4 a + b, k2 = c + 16 d: a DFT16 over a (b = q), the
+// W64^(b c) twiddles, a transpose through the row so that lane (j, q) holds
+// c = 4 q + i, b = 0..3 at v[4 i + b], and DFT4s over b.  v[4 i + d] is then
+// bin k = j + 16 (4 q + i) + 256 d, where M is applied, and the inverse
+// retraces the steps.  Every exchange stays inside the wavefront (no
+// workgroup barrier).  In: v[n1] = x[64 n1 + l].  Out: v[n1] = 1024 y[64 n1 + l].
+// lds: lds_elems(1024) elements of V (Whole) or of T, owned by this wavefront.
+template <typename T, bool Whole, typename M>
+__device__ __forceinline__ void filter1024_wave(void* lds, typename C2<T>::type* v, const typename C2<T>::type* tw,
+                                                M apply) {
+    using V = typename C2<T>::type;
+    const int l = (int)threadIdx.x & 63;
+    const int j = l >> 2, q = l & 3;
+    V wb[4];
+    dft16<-1, T>(v);                                                     // n1 -> k1
+    twiddle_base<T, 1024>(wb, tw, l);
+    twiddle_run<-1, T>(v, wb);                                           // W1024^(l k1)
+    lds_exchange<false, Whole>(lds, v, [&](int k1) { return 64 * k1 + l; },
+                               [&](int a) { return 64 * j + 4 * a + q; });    // row j, b = q
+    dft16<-1, T>(v);                                                     // a -> c
+    twiddle_base<T, 1024>(wb, tw, 16 * q);
+    twiddle_run<-1, T>(v, wb);                                           // W64^(b c)
+    lds_exchange<false, Whole>(lds, v, [&](int c) { return 64 * j + 4 * c + q; },
+                               [&](int m) { return 64 * j + 16 * q + m; });   // c = 4 q + i, b
+#pragma unroll
+    for (int i = 0; i < 4; i++) dft4<-1>(v[4 * i], v[4 * i + 1], v[4 * i + 2], v[4 * i + 3]);  // b -> d
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int d = 0; d < 4; d++) v[4 * i + d] = apply(v[4 * i + d], j + 16 * (4 * q + i) + 256 * d);
+#pragma unroll
+    for (int i = 0; i < 4; i++) dft4<+1>(v[4 * i], v[4 * i + 1], v[4 * i + 2], v[4 * i + 3]);  // d -> b
+    lds_exchange<false, Whole>(lds, v, [&](int m) { return 64 * j + 16 * q + m; },
+                               [&](int c) { return 64 * j + 4 * c + q; });
+    twiddle_base<T, 1024>(wb, tw, 16 * q);
+    twiddle_run<+1, T>(v, wb);
+    dft16<+1, T>(v);                                                     // c -> a
+    lds_exchange<false, Whole>(lds, v, [&](int a) { return 64 * j + 4 * a + q; },
+                               [&](int k1) { return 64 * k1 + l; });
+    twiddle_base<T, 1024>(wb, tw, l);
+    twiddle_run<+1, T>(v, wb);
+    dft16<+1, T>(v);                                                     // k1 -> n1
+}
+
 // ---- 16 R-point convolution blocks, same scheme, R = 512 or 1024 ----
 //
 // B = 16 R on R threads.  Stage 1 as in conv4096 (thread t = n2 < R, rows k1

@@ -67,13 +67,14 @@ __device__ __forceinline__ void chroma_het(const ChromaLaunch& a, int line, doub
 // inverse FFT; the burst window (FP64) and the line (FP32, ~1e-3 LSB of the
 // u16 output) are stored for the metric and output kernels.  The two real
 // lines go in as the real and imaginary parts of one complex transform; the
-// band-pass mask is symmetric in frequency, so the band-passed line la comes
-// back as the real part and la + 1 as the imaginary part.  (The reference
+// band-pass keeps bins by |frequency|, so the band-passed line la comes back
+// as the real part and la + 1 as the imaginary part.  (The reference
 // transforms each line on its own; the results agree to rounding.)  Thread t
-// computes columns t + 64 r, which is the register layout the first FFT
-// stage takes, and the mask is applied in the last forward stage.
-__global__ __launch_bounds__(64) void k_chroma_line(ChromaLaunch a) {
-    __shared__ __align__(16) double2 buf[lds_elems(kChromaFFT)];
+// computes columns t + 64 r, the register layout filter1024_wave takes and
+// returns, so the lines never pass through LDS outside the transform.
+constexpr bool kChromaWhole = false;  // real and imaginary parts exchanged in turn
+__global__ __launch_bounds__(64, 3) void k_chroma_line(ChromaLaunch a) {
+    __shared__ __align__(16) double buf[lds_elems(kChromaFFT) * (kChromaWhole ? 2 : 1)];
     const int f = blockIdx.y;
     const int nf = a.nf_override > 0 ? a.nf_override : a.ctl->nf;
     if (f >= nf) return;
@@ -105,19 +106,30 @@ __global__ __launch_bounds__(64) void k_chroma_line(ChromaLaunch a) {
 #pragma unroll
         for (int r = 0; r < 16; r++) v[r].y = 0.0;
     }
-    fft_fwd_regs_mul<double, kChromaFFT>(buf, v, a.tw, a.mask);
-    fft_lds<+1, double, kChromaFFT>(buf, a.tw);
+    // band-pass: bins with lo <= |k| <= hi kept (chroma_decode.cu:163-182)
+    const int lo = a.lo, hi = a.hi;
+    filter1024_wave<double, kChromaWhole>(buf, v, a.tw, [&](double2 z, int k) {
+        const int kk = k <= kChromaFFT / 2 ? k : kChromaFFT - k;
+        if (kk < lo || kk > hi) {
+            z.x = 0.0;
+            z.y = 0.0;
+        }
+        return z;
+    });
     double* da = a.burst + ((size_t)f * kFieldLines + la) * kBurstLen;
-    for (int i = threadIdx.x; i < kBurstLen; i += 64) {
-        double2 z = buf[pidx(a.burst_start + i)];
-        da[i] = z.x * a.fft_scale;
-        if (has_b) da[kBurstLen + i] = z.y * a.fft_scale;
-    }
     float* fa = a.filt + ((size_t)f * kFieldLines + la) * kLineLen;
-    for (int col = threadIdx.x; col < kLineLen; col += 64) {
-        double2 z = buf[pidx(col)];
-        fa[col] = (float)(z.x * a.fft_scale);
-        if (has_b) fa[kLineLen + col] = (float)(z.y * a.fft_scale);
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        const int col = (int)threadIdx.x + 64 * r;
+        if (col < kLineLen) {
+            fa[col] = (float)(v[r].x * a.fft_scale);
+            if (has_b) fa[kLineLen + col] = (float)(v[r].y * a.fft_scale);
+        }
+        const int bi = col - a.burst_start;
+        if (bi >= 0 && bi < kBurstLen) {
+            da[bi] = v[r].x * a.fft_scale;
+            if (has_b) da[kBurstLen + bi] = v[r].y * a.fft_scale;
+        }
     }
 }
 

@@ -114,3 +114,52 @@ def test_block_layout_is_circular_convolution(n):
     if n == 4096:
         g2 = g.reshape(16, 256).T.reshape(-1)
         assert np.abs(conv4096_emulated(x, g2) - ref).max() > 1e-3 * np.abs(ref).max()
+
+
+def filter1024_emulated(x, keep):
+    """fft_lds.h filter1024_wave: lane l holds x[64 n1 + l]; rows of 64 to
+    lanes 4 j .. 4 j + 3; bin k = j + 16 (4 q + i) + 256 d at v[4 i + d]."""
+    NB = 1024
+    W = lambda e: np.exp(-2j * np.pi * (np.asarray(e) % NB) / NB)
+    buf = np.zeros(NB, complex)
+    l = np.arange(64)
+    j, q = l >> 2, l & 3
+    v = dft(np.stack([x[64 * r + l] for r in range(16)], 1), -1) * W(np.outer(l, np.arange(16)))
+    for k1 in range(16):
+        buf[64 * k1 + l] = v[:, k1]
+    v = dft(np.stack([buf[64 * j + 4 * a + q] for a in range(16)], 1), -1) * W(16 * np.outer(q, np.arange(16)))
+    for c in range(16):
+        buf[64 * j + 4 * c + q] = v[:, c]
+    v = np.stack([buf[64 * j + 16 * q + m] for m in range(16)], 1)
+    for i in range(4):
+        v[:, 4 * i:4 * i + 4] = dft(v[:, 4 * i:4 * i + 4], -1)
+    for i in range(4):
+        for d in range(4):
+            k = j + 16 * (4 * q + i) + 256 * d
+            v[:, 4 * i + d] *= keep[k]
+    for i in range(4):
+        v[:, 4 * i:4 * i + 4] = dft(v[:, 4 * i:4 * i + 4], +1)
+    for m in range(16):
+        buf[64 * j + 16 * q + m] = v[:, m]
+    v = np.stack([buf[64 * j + 4 * c + q] for c in range(16)], 1)
+    v = dft(v * np.conj(W(16 * np.outer(q, np.arange(16)))), +1)
+    for a in range(16):
+        buf[64 * j + 4 * a + q] = v[:, a]
+    v = np.stack([buf[64 * k1 + l] for k1 in range(16)], 1)
+    v = dft(v * np.conj(W(np.outer(l, np.arange(16)))), +1)
+    y = np.zeros(NB, complex)
+    for r in range(16):
+        y[64 * r + l] = v[:, r]
+    return y
+
+
+def test_filter1024_wave_layout():
+    rng = np.random.default_rng(1024)
+    x = rng.standard_normal(1024) + 1j * rng.standard_normal(1024)
+    kk = np.minimum(np.arange(1024), 1024 - np.arange(1024))
+    keep = ((kk >= 180) & (kk <= 300)).astype(float)  # a chroma-like band by |k|
+    ref = np.fft.ifft(np.fft.fft(x) * keep) * 1024
+    assert np.abs(filter1024_emulated(x, keep) - ref).max() < 1e-11 * np.abs(ref).max()
+    g = rng.standard_normal(1024) + 1j * rng.standard_normal(1024)  # any per-bin map
+    ref = np.fft.ifft(np.fft.fft(x) * g) * 1024
+    assert np.abs(filter1024_emulated(x, g) - ref).max() < 1e-11 * np.abs(ref).max()

@@ -14,6 +14,7 @@ FP64 form (cuvhs_amd.precision); --no-forms skips it.
 """
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -23,6 +24,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "NTSC fields/s at 28 MHz (MFMA mixed vs plain FP32, 1 GPU) + achieved HBM GB/s"
 PEAK_HBM_GBS = 8000.0
+PEAK_FP64_TFLOPS = 78.6   # MI355X FP64, vector and matrix
 
 # Algorithmic bytes per unit of each stage (the stage's compulsory reads and
 # writes of its own inputs/outputs, counted once).
@@ -204,6 +206,24 @@ def main():
                 "achieved_isolated": round(alg_bytes_per_launch / iso_launch_s / 1e9, 1),
                 "avg_launch_ms_isolated": round(iso_launch_s * 1e3, 4)}
 
+    # the same kernel against the FP64 arithmetic roof: its convolution blocks
+    # as algorithmic flops, two B-point complex FFTs (5 B log2 B each) and the
+    # B-point spectrum product (6 B) per block of V outputs, over the blocks
+    # of one launch (the epilogue's arctangent, square root and difference are
+    # not counted); peak 78.6 TFLOP/s FP64
+    roofline_fp64 = None
+    if dom in ("k1_analytic", "k1_video"):
+        ba, va, bb, vb = dec.k1_blocks()
+        B, V = (ba, va) if dom == "k1_analytic" else (bb, vb)
+        blocks = (k1_samples / fmt.spfp) * -(-fmt.spfp // V) / launches
+        flops = blocks * (2 * 5 * B * math.log2(B) + 6 * B)
+        roofline_fp64 = {"bound": "fp64", "kernel": dom, "block": B, "valid_per_block": V,
+                         "alg_flops_per_launch": int(flops), "unit": "TFLOP/s", "peak": PEAK_FP64_TFLOPS,
+                         "achieved": round(flops / avg_launch_s / 1e12, 2),
+                         "frac": round(flops / avg_launch_s / 1e12 / PEAK_FP64_TFLOPS, 4),
+                         "achieved_isolated": round(flops / iso_launch_s / 1e12, 2),
+                         "frac_isolated": round(flops / iso_launch_s / 1e12 / PEAK_FP64_TFLOPS, 4)}
+
     forms = None
     if not args.no_forms:
         from cuvhs_amd import precision
@@ -256,6 +276,7 @@ def main():
         "stage_ms_per_step": {k: round(v, 3) for k, v in stage_ms.items()},
         "stage_ms_per_step_isolated": {k: round(v, 3) for k, v in iso.items()},
         "roofline": roofline,
+        "roofline_fp64": roofline_fp64,
         "cpu_baseline": cpu,
         "forms": forms,
     }

@@ -906,6 +906,15 @@ int cuvhs_profile_work(const cuvhs_ctx* c, uint64_t* samples, int* chunks, int* 
     return 0;
 }
 
+int cuvhs_k1_blocks(const cuvhs_ctx* c, int* block_a, int* valid_a, int* block_b, int* valid_b) {
+    if (!c) return CUVHS_E_ARG;
+    if (block_a) *block_a = c->k1.rf.B;
+    if (valid_a) *valid_a = c->k1.rf.valid;
+    if (block_b) *block_b = c->k1.video.B;
+    if (valid_b) *valid_b = c->k1.video.valid;
+    return 0;
+}
+
 int cuvhs_profile_read(const cuvhs_ctx* c, double* ms, const char** names, int n) {
     if (!c || n < 0) return CUVHS_E_ARG;
     for (int i = 0; i < n && i <= kStages; i++) {

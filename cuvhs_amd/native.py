@@ -89,6 +89,7 @@ _SIGS = {
     "cuvhs_profile": (_i, [_vp, _i]),
     "cuvhs_profile_read": (_i, [_vp, _vp, _vp, _i]),
     "cuvhs_profile_work": (_i, [_vp, ctypes.POINTER(_u64), ctypes.POINTER(_i), ctypes.POINTER(_i)]),
+    "cuvhs_k1_blocks": (_i, [_vp, ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_i)]),
 }
 
 _lib = None
@@ -252,6 +253,12 @@ class Decoder:
         check(lib().cuvhs_profile_work(self.h, ctypes.byref(smp), ctypes.byref(ch), ctypes.byref(fl)), self.h,
               "profile_work")
         return smp.value, ch.value, fl.value
+
+    def k1_blocks(self):
+        """K1 block shapes: (block_a, valid_a, block_b, valid_b)."""
+        v = [_i() for _ in range(4)]
+        check(lib().cuvhs_k1_blocks(self.h, *[ctypes.byref(x) for x in v]), self.h, "k1_blocks")
+        return tuple(x.value for x in v)
 
     def decode_chunked(self, d_capture):
         """Decode a device-resident u8 capture chunk by chunk through

@@ -209,6 +209,11 @@ int cuvhs_profile_read(const cuvhs_ctx* ctx, double* ms, const char** names, int
  * demodulated by K1 (fields_k1 * spfp per chunk), chunks, and fields decoded.
  * Any pointer may be NULL.  (No counterpart in the reference.) */
 int cuvhs_profile_work(const cuvhs_ctx* ctx, uint64_t* samples, int* chunks, int* fields);
+/* K1's block shapes as designed for this context: block size (points) and
+ * valid outputs per block of part A (analytic) and part B (video and sync
+ * filters).  Any pointer may be NULL.  (No counterpart in the reference,
+ * which transforms whole segments, fm_demod.cu:449-539.) */
+int cuvhs_k1_blocks(const cuvhs_ctx* ctx, int* block_a, int* valid_a, int* block_b, int* valid_b);
 
 /* The reference's debug-log counters (CUVHS_DEBUG_K3_LOG, _K4_LOG, _K5_LOG:
  * line_locs.cu:585-653, hsync_refine.cu:273-349, tbc_resample.cu:75-116),

@@ -19,6 +19,13 @@ import os
 import sys
 import time
 
+# Hardware queues for this process (read when HIP initialises, before torch
+# touches the GPU).  The decoder runs four streams besides torch's; with
+# HIP's default of four queues two of them share one, and a context then
+# keeps its forked tail work on the head stream (cuvhs_create reads the
+# same variable).  Eight gives every stream its own queue.
+os.environ["GPU_MAX_HW_QUEUES"] = "8"
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 

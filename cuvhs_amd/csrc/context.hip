@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -71,6 +72,8 @@ struct cuvhs_ctx {
     bool own_stream = false;
     hipStream_t tail = nullptr;     // tail stage
     hipStream_t side = nullptr;     // chroma pre-band-pass, next to the tail's sync stages
+    hipStream_t aux = nullptr;      // the tail's forked work, when the process has a queue for it
+    bool aux_queue = false;         // more hardware queues than the default four
     std::string err;
     K1Design k1;
     PreDesign pre;
@@ -203,6 +206,15 @@ void reset_state(DecodeState* h) {
     h->k2b.prev_first_hsync_diff = -1.0;
     h->chroma.valid = 0;
     h->last_parity = 1;
+}
+
+// Hardware queues HIP gives this process (GPU_MAX_HW_QUEUES, default 4).
+// Streams beyond them share queues, and work on a shared queue runs in
+// submission order whatever its stream.
+int hw_queues() {
+    const char* v = getenv("GPU_MAX_HW_QUEUES");
+    const int n = v ? atoi(v) : 0;
+    return n > 0 ? n : 4;
 }
 
 void mark(cuvhs_ctx* c, Slot& s, int i, hipStream_t st) {
@@ -473,6 +485,7 @@ int sync_all(cuvhs_ctx* c) {
     HIPCHK(c, hipStreamSynchronize(c->stream));
     HIPCHK(c, hipStreamSynchronize(c->tail));
     HIPCHK(c, hipStreamSynchronize(c->side));
+    HIPCHK(c, hipStreamSynchronize(c->aux));
     return 0;
 }
 
@@ -556,6 +569,9 @@ int cuvhs_create(const cuvhs_config* cfg, cuvhs_ctx** out) {
     if (e != hipSuccess) return bad(e, "hipStreamCreate tail");
     e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking);
     if (e != hipSuccess) return bad(e, "hipStreamCreate side");
+    e = hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking);
+    if (e != hipSuccess) return bad(e, "hipStreamCreate aux");
+    c->aux_queue = hw_queues() >= 5;
     for (Slot& s : c->slot) {
         for (hipEvent_t* ev : {&s.head_done, &s.tail_done, &s.dc_done, &s.k4_done, &s.aux_done}) {
             e = hipEventCreateWithFlags(ev, hipEventDisableTiming);
@@ -699,7 +715,7 @@ int cuvhs_create(const cuvhs_config* cfg, cuvhs_ctx** out) {
 
 int cuvhs_destroy(cuvhs_ctx* c) {
     if (!c) return CUVHS_E_ARG;
-    for (hipStream_t st : {c->stream, c->tail, c->side})
+    for (hipStream_t st : {c->stream, c->tail, c->side, c->aux})
         if (st) (void)hipStreamSynchronize(st);
     for (Slot& s : c->slot) {
         for (hipEvent_t e : s.mark)
@@ -708,6 +724,7 @@ int cuvhs_destroy(cuvhs_ctx* c) {
             if (e) (void)hipEventDestroy(e);
         if (s.hctl) (void)hipHostFree(s.hctl);
     }
+    if (c->aux) (void)hipStreamDestroy(c->aux);
     if (c->side) (void)hipStreamDestroy(c->side);
     if (c->tail) (void)hipStreamDestroy(c->tail);
     for (void* p : c->allocs) (void)hipFree(p);
@@ -822,14 +839,14 @@ int cuvhs_decode_capture(cuvhs_ctx* c, const void* cap, uint64_t cap_len, int ma
         // last used it has finished; the next head (with its pre-band-pass on
         // the side stream) goes in before this chunk's tail, so the head
         // chain starts as early as it can.  The tail's forked work goes on
-        // the head stream behind that head: a process has few hardware
-        // queues, and there it delays neither the pre-band-pass the next
-        // tail waits for nor this tail.
+        // the aux stream when the process has a hardware queue for it, else
+        // on the head stream behind that head (a queue shared with the side
+        // stream would hold it behind the next pre-band-pass).
         while (finished <= chunk + 1 - kSlots && rc == 0) rc = finish_tail(c, c->slot[finished++ % kSlots]);
         if (rc) break;
         const bool more = plan(c->slot[(chunk + 1) % kSlots], next);
         if (more && (rc = head_stage(c, c->slot[(chunk + 1) % kSlots])) != 0) break;
-        if ((rc = tail_stage(c, s, &o, c->stream)) != 0) break;
+        if ((rc = tail_stage(c, s, &o, c->aux_queue ? c->aux : c->stream)) != 0) break;
         total += nf;
         chunk++;
         if (!more) break;
@@ -860,6 +877,7 @@ int cuvhs_profile(cuvhs_ctx* c, int enable) {
     HIPCHK(c, hipStreamSynchronize(c->stream));
     HIPCHK(c, hipStreamSynchronize(c->tail));
     HIPCHK(c, hipStreamSynchronize(c->side));
+    HIPCHK(c, hipStreamSynchronize(c->aux));
     c->prof = enable != 0;
     c->serial = enable == 2;
     for (double& v : c->stage_ms) v = 0.0;

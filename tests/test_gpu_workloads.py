@@ -147,3 +147,32 @@ def test_full_prefix_matches_oracle(torch, N, full, cap1_prefix, ref1_prefix):
     refc = {k: (v[:128] if isinstance(v, np.ndarray) else v) for k, v in ref.items()}
     refc["fields"] = 128
     compare(cut, refc)
+
+
+def _digest(out):
+    import hashlib
+    h = hashlib.sha256()
+    for k in ("luma", "chroma", "file_loc", "is_first", "phase_id", "do_count", "do_line", "do_startx", "do_endx"):
+        h.update(np.ascontiguousarray(out[k].cpu().numpy()).tobytes())
+    return out["fields"], h.hexdigest()
+
+
+def test_aux_queue_schedule_same_outputs(torch, N, monkeypatch):
+    """With more hardware queues than HIP's default four the decoder forks
+    the tail's K5 and dropout classes onto its own stream instead of the
+    head stream (cuvhs_create reads GPU_MAX_HW_QUEUES).  The outputs must
+    not depend on the schedule: 200 fields of the 28 MHz workload (four
+    chunks) decoded by a context created with the variable at 8 and by one
+    created without it.  (HIP read its own queue count at start-up, so here
+    the extra stream may share a queue: this checks the dependencies, not
+    the overlap.)"""
+    cap = dev(torch, N.generate(1, n_samples=200 * 468140))
+    monkeypatch.delenv("GPU_MAX_HW_QUEUES", raising=False)
+    dec = N.Decoder(28.0, 64)
+    want = _digest(dec.decode_capture(cap))
+    dec.close()
+    monkeypatch.setenv("GPU_MAX_HW_QUEUES", "8")
+    dec = N.Decoder(28.0, 64)
+    got = _digest(dec.decode_capture(cap))
+    dec.close()
+    assert got == want

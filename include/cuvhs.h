@@ -13,10 +13,12 @@
  * Stage entry points enqueue on the context's stream and synchronise it.
  * Chunk decodes run a chunk's head (K1, sync events, VSYNC clustering) on
  * the context's stream and its tail (pulses through metadata) on a second
- * stream owned by the context, with the chroma pre-band-pass on a third, so
- * that consecutive chunks overlap; they return once all three are idle.  All
- * streams have the default priority.  A context is not thread-safe; distinct
- * contexts are independent.
+ * stream owned by the context, with the chroma pre-band-pass on a third and,
+ * when the process has more than four hardware queues (GPU_MAX_HW_QUEUES,
+ * read at cuvhs_create), part of the tail on a fourth, so that consecutive
+ * chunks overlap; they return once all are idle.  All streams have the
+ * default priority.  A context is not thread-safe; distinct contexts are
+ * independent.
  */
 #ifndef CUVHS_H
 #define CUVHS_H

@@ -362,10 +362,6 @@ int head_stage(cuvhs_ctx* c, Slot& s, bool pre = true) {
         HIPCHK(c, run_input_mean(s.chunk, c->cfg.input_format, (long long)s.n_read, s.dc_partial, s.dc, st));
         HIPCHK(c, hipEventRecord(s.dc_done, st));
     }
-    if (pre) {
-        int r = pre_stage(c, s);
-        if (r) return r;
-    }
     ChunkCtl* h = s.hctl;
     mark(c, s, 0, st);
     int r = k1_launch(c, s, s.chunk, (long long)s.n_read, s.fk1, st, s.ctl, c->thr_bits);
@@ -374,11 +370,17 @@ int head_stage(cuvhs_ctx* c, Slot& s, bool pre = true) {
     const long long T = (long long)s.fk1 * f.spfp;
     HIPCHK(c, run_events_bits(c->thr_bits, T, s.wg_count, s.wg_off, s.ev_masks, s.events, s.ctl, st));
     mark(c, s, 3, st);
+    // the kernel also writes the control block to the pinned host copy
     HIPCHK(c, run_vsync_cluster(s.events, s.ctl, T, f.vsync_min_w, f.vsync_max_w, s.fk1 * 20, c->cand, s.offsets,
-                                s.max_fields, f.samples_per_field, f.samples_per_line, s.fk1, 1, st));
+                                s.max_fields, f.samples_per_field, f.samples_per_line, s.fk1, 1, h, st));
     mark(c, s, 4, st);
-    HIPCHK(c, hipMemcpyAsync(h, s.ctl, sizeof(ChunkCtl), hipMemcpyDeviceToHost, st));
     HIPCHK(c, hipEventRecord(s.head_done, st));
+    // the pre-band-pass goes in after the head's launches: the head chain is
+    // what the next chunk waits for
+    if (pre) {
+        int r = pre_stage(c, s);
+        if (r) return r;
+    }
     return 0;
 }
 
@@ -970,7 +972,7 @@ int cuvhs_vsync_discover(cuvhs_ctx* c, const double* d05, int total, int* idx, i
     HIPCHK(c, run_events(d05, total, f.pulse_threshold_hz, c->thr_bits, s.wg_count, s.wg_off, s.ev_masks, s.events,
                          s.ctl, c->stream));
     HIPCHK(c, run_vsync_cluster(s.events, s.ctl, total, f.vsync_min_w, f.vsync_max_w, cap, idx, s.offsets, 1,
-                                f.samples_per_field, f.samples_per_line, 1, 0, c->stream));
+                                f.samples_per_field, f.samples_per_line, 1, 0, nullptr, c->stream));
     HIPCHK(c, hipMemcpyAsync(s.hctl, s.ctl, sizeof(ChunkCtl), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     *count = s.hctl->vsync_count;

@@ -90,9 +90,10 @@ hipError_t run_events_bits(const unsigned int* bits, long long total, int* wg_co
 // the same from a demod_05 array: bitmask into bits, then run_events_bits
 hipError_t run_events(const double* x, long long total, double thr, unsigned int* bits, int* wg_count, int* wg_off,
                       uint2* masks, int* events, ChunkCtl* ctl, hipStream_t st);
+// host_out (optional): pinned host copy of the finished control block
 hipError_t run_vsync_cluster(const int* events, ChunkCtl* ctl, long long total, int min_w, int max_w, int cap,
                              int* cand, unsigned long long* offsets, int num_fields, int spf, int spl, int fields_k1,
-                             int do_cluster, hipStream_t st);
+                             int do_cluster, ChunkCtl* host_out, hipStream_t st);
 hipError_t run_sync_pulses(const int* events, const ChunkCtl* ctl, const unsigned long long* offsets, int spfp,
                            int* starts, int* lens, int* count, int max_fields, int nf_override, hipStream_t st);
 hipError_t run_k2b(const ChunkCtl* ctl_dev, ChunkCtl* ctl_mut, DecodeState* dst, int nf_override, int max_fields, const Fmt& fmt,

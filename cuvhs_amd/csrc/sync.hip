@@ -148,10 +148,14 @@ __device__ __forceinline__ long long vs_safe(int pos, int spl) {
     return pos > 5 * spl ? (long long)pos - 5LL * spl : 0LL;
 }
 
+// host_out (optional): pinned host memory that receives the finished control
+// block (visible to the host once the kernel has completed), in place of a
+// separate device-to-host copy.
 __global__ __launch_bounds__(1024) void k_vsync_cluster(const int* events, ChunkCtl* ctl, long long total,
                                                         int min_w, int max_w, int cap, int* cand,
                                                         unsigned long long* offsets, int num_fields, int spf,
-                                                        int spl, int fields_k1, int do_cluster) {
+                                                        int spl, int fields_k1, int do_cluster,
+                                                        ChunkCtl* host_out) {
     constexpr int kWaves = 1024 / 64;
     __shared__ int wcnt[2][kWaves];
     __shared__ int lcand[kCandLds];
@@ -219,7 +223,10 @@ __global__ __launch_bounds__(1024) void k_vsync_cluster(const int* events, Chunk
     }
     __syncthreads();
     if (!do_cluster) {
-        if (threadIdx.x == 0) ctl->vsync_count = carry_all;
+        if (threadIdx.x == 0) {
+            ctl->vsync_count = carry_all;
+            if (host_out) *host_out = *ctl;
+        }
         return;
     }
     const int count = carry_all;
@@ -269,6 +276,7 @@ __global__ __launch_bounds__(1024) void k_vsync_cluster(const int* events, Chunk
     ctl->next_raw_offset = predicted > margin ? ctl->raw_offset + predicted - margin : ctl->raw_offset + predicted;
     ctl->nf = n;
     ctl->status |= status;
+    if (host_out) *host_out = *ctl;
 }
 
 // K2: one wavefront per field.  Events alternate start/end, so after an
@@ -354,9 +362,9 @@ hipError_t run_events(const double* x, long long total, double thr, unsigned int
 
 hipError_t run_vsync_cluster(const int* events, ChunkCtl* ctl, long long total, int min_w, int max_w, int cap,
                              int* cand, unsigned long long* offsets, int num_fields, int spf, int spl, int fields_k1,
-                             int do_cluster, hipStream_t st) {
+                             int do_cluster, ChunkCtl* host_out, hipStream_t st) {
     hipLaunchKernelGGL(k_vsync_cluster, dim3(1), dim3(1024), 0, st, events, ctl, total, min_w, max_w, cap, cand,
-                       offsets, num_fields, spf, spl, fields_k1, do_cluster);
+                       offsets, num_fields, spf, spl, fields_k1, do_cluster, host_out);
     return hipGetLastError();
 }
 

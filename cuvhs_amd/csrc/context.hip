@@ -520,6 +520,25 @@ int finish_tail(cuvhs_ctx* c, Slot& s) {
 
 }  // namespace
 
+template <typename V, typename T>
+static int block_selftest(int B, int is_f32, const void* d_x, const void* d_g, void* d_y) {
+    std::vector<V> g((size_t)B), gp((size_t)B);
+    if (hipMemcpy(g.data(), d_g, (size_t)B * sizeof(V), hipMemcpyDeviceToHost) != hipSuccess) return CUVHS_E_HIP;
+    for (size_t k = 0; k < (size_t)B; k++) gp[B == 1024 ? k : block_bin((size_t)B, k)] = g[k];
+    auto t = twiddles<V, T>(B);
+    void* tw = nullptr;
+    void* dg = nullptr;
+    if (hipMalloc(&tw, (size_t)B * sizeof(V)) != hipSuccess) return CUVHS_E_ALLOC;
+    if (hipMalloc(&dg, (size_t)B * sizeof(V)) != hipSuccess) { (void)hipFree(tw); return CUVHS_E_ALLOC; }
+    hipError_t e = hipMemcpy(tw, t.data(), (size_t)B * sizeof(V), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dg, gp.data(), (size_t)B * sizeof(V), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = run_block_selftest(B, is_f32, d_x, dg, d_y, tw, nullptr);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    (void)hipFree(dg);
+    (void)hipFree(tw);
+    return e == hipSuccess ? 0 : (e == hipErrorInvalidValue ? CUVHS_E_ARG : CUVHS_E_HIP);
+}
+
 extern "C" {
 
 int cuvhs_format_for(double mhz, cuvhs_format* o) {
@@ -1055,23 +1074,16 @@ int cuvhs_dropouts(cuvhs_ctx* c, const double* env, const double* ll, int nf, ui
     return 0;
 }
 
-// FFT building-block self test (not a reference stage): B-point LDS FFT of
-// device data with the context-independent twiddle table.
-int cuvhs_selftest_fft(int B, int is_f32, const void* d_in, void* d_out, int sign) {
-    void* tw = nullptr;
-    size_t es = is_f32 ? sizeof(float2) : sizeof(double2);
-    if (hipMalloc(&tw, (size_t)B * es) != hipSuccess) return CUVHS_E_ALLOC;
-    if (is_f32) {
-        auto t = twiddles<float2, float>(B);
-        if (hipMemcpy(tw, t.data(), t.size() * es, hipMemcpyHostToDevice) != hipSuccess) { (void)hipFree(tw); return CUVHS_E_HIP; }
-    } else {
-        auto t = twiddles<double2, double>(B);
-        if (hipMemcpy(tw, t.data(), t.size() * es, hipMemcpyHostToDevice) != hipSuccess) { (void)hipFree(tw); return CUVHS_E_HIP; }
-    }
-    hipError_t e = run_fft_selftest(B, is_f32, d_in, d_out, tw, sign, nullptr);
-    if (e == hipSuccess) e = hipDeviceSynchronize();
-    (void)hipFree(tw);
-    return e == hipSuccess ? 0 : CUVHS_E_HIP;
+// Convolution-block self test (not a reference stage): one block of the
+// kind the decoder runs, with the spectrum given in natural bin order and
+// permuted here exactly as the context uploads its filters.
+int cuvhs_selftest_block(int B, int is_f32, const void* d_x, const void* d_g, void* d_y) {
+    if (!d_x || !d_g || !d_y) return CUVHS_E_ARG;
+    // the shapes the decoder runs; anything else is refused before any copy
+    const bool ok = is_f32 ? (B == 4096 || B == 8192 || B == 16384) : (B == 1024 || B == 4096 || B == 8192);
+    if (!ok) return CUVHS_E_ARG;
+    return is_f32 ? block_selftest<float2, float>(B, 1, d_x, d_g, d_y)
+                  : block_selftest<double2, double>(B, 0, d_x, d_g, d_y);
 }
 
 // FP64 arctangent self test (not a reference stage).

@@ -1,17 +1,22 @@
-// LDS-resident Stockham FFT for one workgroup (gfx950, wave64).
+// Convolution blocks in LDS for gfx950 (wave64): y = IFFT(FFT(x) G) over
+// one block, with the 16 complex values a thread holds in registers.
 //
-// B points of complex T (double2 / float2) held in LDS; B/16 threads each own
-// one radix-16 butterfly per stage (16 complex values in VGPRs), so a stage
-// is: load 16 values -> barrier -> twiddle + DFT16 in registers -> store to
-// the Stockham position -> barrier.  The buffer is reused in place because
-// every load of a stage completes before any store of that stage.
-// Sizes: 1024 = 16*16*4, 4096 = 16^3, 8192 = 16^3*2, 16384 = 16^3*4.
-// LDS layout is padded by one element per 16 (element p lives at p + p/16):
-// the first stage stores at a stride of 16 elements, which without padding
-// puts every lane of a wavefront on the same LDS banks.
-// Twiddles come from a table of W_B^m = exp(-2 pi i m / B) in global memory:
-// four loads per butterfly (w1, w2, w4, w8), the other powers by at most
-// three complex products.
+// * conv4096: 4096 = 16 x 256 on 256 threads (4 waves): a radix-16 stage
+//   across the workgroup on the values each thread loaded itself, one
+//   cross-wave exchange, then per-wave 256-point transforms (16 x 16) with
+//   wave-local transposes; the spectrum is applied in that transposed bin
+//   order, and the inverse retraces the steps.
+// * conv_rows<R>: 16 R points on R threads (R = 512 or 1024), the same
+//   scheme with 16 rows of R points transformed as 16 x 16 x R/256.
+// * filter1024_wave: 1024 = 16 x 64 on one wavefront, a per-bin function
+//   applied in the middle (the chroma line band-pass).
+//
+// Exchanges go through LDS either as whole complex elements or as real and
+// then imaginary parts (half the LDS for two more barriers); the LDS layout
+// is padded by one element per 16 (element p lives at p + p/16), which keeps
+// the strided exchange patterns off shared banks.  Twiddles come from a table
+// of W_B^m = exp(-2 pi i m / B) in global memory: four loads per run (w1, w2,
+// w4, w8), the other powers by complex products.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -103,155 +108,6 @@ template <int S, typename T> __device__ __forceinline__ void dft16(typename C2<T
     }
 #pragma unroll
     for (int i = 0; i < 16; i++) v[i] = out[i];
-}
-
-// One Stockham stage with radix R over B points, ns = product of previous radices.
-// Each of the NT threads handles 16/R butterflies (16 values).  With G set
-// (last stage of a forward transform, whose outputs are in natural order)
-// every output bin is multiplied by G[bin] before it is stored.
-template <int S, typename T, int B, int R, int NT>
-__device__ __forceinline__ void fft_stage(typename C2<T>::type* buf, const typename C2<T>::type* tw, int ns,
-                                          const typename C2<T>::type* G = nullptr) {
-    using V = typename C2<T>::type;
-    constexpr int NB = B / R;          // butterflies in the stage
-    constexpr int PER = NB / NT;       // butterflies per thread
-    static_assert(PER * R == 16, "each thread owns 16 values");
-    V v[16];
-    const int t = threadIdx.x;
-    // base twiddles W_B^{e}, e = step * {1, 2, 4, 8}, loaded before the
-    // stage's LDS reads so their latency overlaps the barrier
-    V wb[PER][4];
-    if (ns > 1) {
-#pragma unroll
-        for (int p = 0; p < PER; p++) {
-            int step = ((t + p * NT) % ns) * (B / (ns * R));
-#pragma unroll
-            for (int q = 0; q < 4; q++) {
-                if ((1 << q) < R) {
-                    V x = tw[(step << q) & (B - 1)];
-                    wb[p][q] = S < 0 ? x : cconj(x);
-                }
-            }
-        }
-    }
-#pragma unroll
-    for (int p = 0; p < PER; p++) {
-        int j = t + p * NT;
-#pragma unroll
-        for (int r = 0; r < R; r++) v[p * R + r] = buf[pidx(j + r * NB)];
-    }
-    __syncthreads();
-#pragma unroll
-    for (int p = 0; p < PER; p++) {
-        V* u = v + p * R;
-        if (ns > 1) {
-            // twiddles W_{ns*R}^{r*k} = W_B^{r*k*B/(ns*R)}, k = j mod ns
-            V w[16];
-            w[0].x = (T)1; w[0].y = (T)0;
-            if (R >= 2) w[1] = wb[p][0];
-            if (R >= 4) { w[2] = wb[p][1]; w[3] = cmul(w[1], w[2]); }
-            if (R >= 8) { w[4] = wb[p][2]; w[5] = cmul(w[4], w[1]); w[6] = cmul(w[4], w[2]); w[7] = cmul(w[4], w[3]); }
-            if (R >= 16) {
-                w[8] = wb[p][3];
-#pragma unroll
-                for (int r = 9; r < 16; r++) w[r] = cmul(w[8], w[r - 8]);
-            }
-#pragma unroll
-            for (int r = 1; r < R; r++) u[r] = cmul(u[r], w[r]);
-        }
-        if (R == 16) dft16<S, T>(u);
-        else if (R == 4) dft4<S>(u[0], u[1], u[2], u[3]);
-        else if (R == 2) dft2<S>(u[0], u[1]);
-    }
-    if (G) {
-        V g[16];
-#pragma unroll
-        for (int p = 0; p < PER; p++) {
-            int j = t + p * NT;
-            int idx = (j / ns) * ns * R + j % ns;
-#pragma unroll
-            for (int r = 0; r < R; r++) g[p * R + r] = G[idx + r * ns];
-        }
-#pragma unroll
-        for (int i = 0; i < 16; i++) v[i] = cmul(v[i], g[i]);
-    }
-#pragma unroll
-    for (int p = 0; p < PER; p++) {
-        int j = t + p * NT;
-        int k = j % ns;
-        int idx = (j / ns) * ns * R + k;
-#pragma unroll
-        for (int r = 0; r < R; r++) buf[pidx(idx + r * ns)] = v[p * R + r];
-    }
-    __syncthreads();
-}
-
-// First stage (ns = 1, radix 16) on values the calling thread already holds:
-// thread t owns inputs t + r * B/16, r = 0..15, which is exactly what the
-// stage would load from LDS, so the input never needs a round trip through
-// LDS.  The stage output is stored and synchronised as usual.
-template <int S, typename T, int B>
-__device__ __forceinline__ void fft_first_stage_regs(typename C2<T>::type* buf, typename C2<T>::type* v) {
-    dft16<S, T>(v);
-    const int idx = (int)threadIdx.x * 16;
-#pragma unroll
-    for (int r = 0; r < 16; r++) buf[pidx(idx + r)] = v[r];
-    __syncthreads();
-}
-
-// Forward transform of register-held input (layout of fft_first_stage_regs)
-// followed by the pointwise product with the spectrum G (natural bin order),
-// fused into the last stage's stores.  Result: buf[k] = FFT(x)[k] * G[k].
-template <typename T, int B>
-__device__ __forceinline__ void fft_fwd_regs_mul(typename C2<T>::type* buf, typename C2<T>::type* v,
-                                                 const typename C2<T>::type* tw, const typename C2<T>::type* G) {
-    constexpr int NT = B / 16;
-    static_assert(B == 1024 || B == 4096 || B == 8192 || B == 16384, "unsupported FFT size");
-    fft_first_stage_regs<-1, T, B>(buf, v);
-    if (B == 1024) {
-        fft_stage<-1, T, B, 16, NT>(buf, tw, 16);
-        fft_stage<-1, T, B, 4, NT>(buf, tw, 256, G);
-    } else if (B == 4096) {
-        fft_stage<-1, T, B, 16, NT>(buf, tw, 16);
-        fft_stage<-1, T, B, 16, NT>(buf, tw, 256, G);
-    } else {
-        fft_stage<-1, T, B, 16, NT>(buf, tw, 16);
-        fft_stage<-1, T, B, 16, NT>(buf, tw, 256);
-        if (B == 8192) fft_stage<-1, T, B, 2, NT>(buf, tw, 4096, G);
-        if (B == 16384) fft_stage<-1, T, B, 4, NT>(buf, tw, 4096, G);
-    }
-}
-
-// buf[k] *= G[k] for all B bins (G in global memory, loads issued together).
-template <typename T, int B>
-__device__ __forceinline__ void spectrum_multiply(typename C2<T>::type* buf, const typename C2<T>::type* G) {
-    using V = typename C2<T>::type;
-    constexpr int NT = B / 16;
-    V g[16];
-#pragma unroll
-    for (int i = 0; i < 16; i++) g[i] = G[threadIdx.x + i * NT];
-#pragma unroll
-    for (int i = 0; i < 16; i++) {
-        int j = pidx(threadIdx.x + i * NT);
-        buf[j] = cmul(buf[j], g[i]);
-    }
-    __syncthreads();
-}
-
-// Full transform; the caller has written buf and synchronised.
-template <int S, typename T, int B>
-__device__ __forceinline__ void fft_lds(typename C2<T>::type* buf, const typename C2<T>::type* tw) {
-    constexpr int NT = B / 16;
-    static_assert(B == 1024 || B == 4096 || B == 8192 || B == 16384, "unsupported FFT size");
-    fft_stage<S, T, B, 16, NT>(buf, tw, 1);
-    fft_stage<S, T, B, 16, NT>(buf, tw, 16);
-    if (B == 1024) {
-        fft_stage<S, T, B, 4, NT>(buf, tw, 256);
-    } else {
-        fft_stage<S, T, B, 16, NT>(buf, tw, 256);
-        if (B == 8192) fft_stage<S, T, B, 2, NT>(buf, tw, 4096);
-        if (B == 16384) fft_stage<S, T, B, 4, NT>(buf, tw, 4096);
-    }
 }
 
 // ---- 4096-point convolution block with wave-local sub-transforms ----

@@ -395,25 +395,56 @@ hipError_t run_fill_f64(double* p, long long n, double v, hipStream_t st) {
     return hipGetLastError();
 }
 
-// FFT self-test: one LDS transform of B points (parity of the FFT building block).
+// Self-test of the convolution blocks the decoder runs: y = B IFFT(FFT(x) G)
+// for one block, in the register layout the kernels use (v[r] = x[NT r + t]),
+// with G in the block's bin order.  B = 4096: conv4096 (FP64 with split
+// exchanges as in K1, FP32 whole); 8192 / 16384: conv_rows<512 / 1024>;
+// 1024: filter1024_wave (one wavefront, FP64, split exchanges, G applied per
+// bin as in the chroma kernel).
 template <int B, typename T>
-__global__ __launch_bounds__(B / 16) void k_fft_selftest(const typename C2<T>::type* in, typename C2<T>::type* out,
-                                                         const typename C2<T>::type* tw, int sign) {
+__global__ __launch_bounds__(B == 1024 ? 64 : B / 16) void k_block_selftest(const typename C2<T>::type* x,
+                                                                          const typename C2<T>::type* gp,
+                                                                          typename C2<T>::type* y,
+                                                                          const typename C2<T>::type* tw) {
     using V = typename C2<T>::type;
     extern __shared__ __align__(16) unsigned char smem[];
-    V* buf = reinterpret_cast<V*>(smem);
-    if (sign == -2) {
-        // forward transform fed from registers (the blocked-convolution path)
-        V v[16];
-        for (int r = 0; r < 16; r++) v[r] = in[threadIdx.x + r * (B / 16)];
-        fft_fwd_regs_mul<T, B>(buf, v, tw, nullptr);
+    constexpr int NT = B == 1024 ? 64 : B / 16;
+    V v[16];
+#pragma unroll
+    for (int r = 0; r < 16; r++) v[r] = x[NT * r + threadIdx.x];
+    if constexpr (B == 1024) {
+        filter1024_wave<T, false>(smem, v, tw, [&](V z, int k) { return cmul(z, gp[k]); });
+    } else if constexpr (B == 4096) {
+        conv4096<T, k1_whole<B, T>()>(smem, v, tw, gp);
     } else {
-        for (int j = threadIdx.x; j < B; j += B / 16) buf[pidx(j)] = in[j];
-        __syncthreads();
-        if (sign < 0) fft_lds<-1, T, B>(buf, tw);
-        else fft_lds<+1, T, B>(buf, tw);
+        conv_rows<T, B / 16>(reinterpret_cast<V*>(smem), v, tw, gp);
     }
-    for (int j = threadIdx.x; j < B; j += B / 16) out[j] = buf[pidx(j)];
+#pragma unroll
+    for (int r = 0; r < 16; r++) y[NT * r + threadIdx.x] = v[r];
+}
+
+template <int B, typename T>
+static hipError_t block_st_launch(const void* x, const void* gp, void* y, const void* tw, hipStream_t st) {
+    using V = typename C2<T>::type;
+    size_t sh = (size_t)lds_elems(B) * (B == 1024 || (B == 4096 && sizeof(T) == 8) ? sizeof(T) : sizeof(V));
+    (void)hipFuncSetAttribute((const void*)k_block_selftest<B, T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
+    hipLaunchKernelGGL((k_block_selftest<B, T>), dim3(1), dim3(B == 1024 ? 64 : B / 16), sh, st, (const V*)x,
+                       (const V*)gp, (V*)y, (const V*)tw);
+    return hipGetLastError();
+}
+
+hipError_t run_block_selftest(int B, int is_f32, const void* x, const void* gp, void* y, const void* tw,
+                              hipStream_t st) {
+    if (!is_f32) {
+        if (B == 1024) return block_st_launch<1024, double>(x, gp, y, tw, st);
+        if (B == 4096) return block_st_launch<4096, double>(x, gp, y, tw, st);
+        if (B == 8192) return block_st_launch<8192, double>(x, gp, y, tw, st);
+    } else {
+        if (B == 4096) return block_st_launch<4096, float>(x, gp, y, tw, st);
+        if (B == 8192) return block_st_launch<8192, float>(x, gp, y, tw, st);
+        if (B == 16384) return block_st_launch<16384, float>(x, gp, y, tw, st);
+    }
+    return hipErrorInvalidValue;
 }
 
 // FP64 arctangent self-test (atan_f64.h) over device arrays.
@@ -427,29 +458,6 @@ hipError_t run_atan2_selftest(const double* y, const double* x, double* out, lon
     const int nb = (int)std::min<long long>((n + 255) / 256, 4096);
     hipLaunchKernelGGL(k_atan2_selftest, dim3(nb), dim3(256), 0, st, y, x, out, n);
     return hipGetLastError();
-}
-
-template <int B, typename T>
-static hipError_t st_launch(const void* in, void* out, const void* tw, int sign, hipStream_t st) {
-    using V = typename C2<T>::type;
-    size_t sh = (size_t)lds_elems(B) * sizeof(V);
-    (void)hipFuncSetAttribute((const void*)k_fft_selftest<B, T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
-    hipLaunchKernelGGL((k_fft_selftest<B, T>), dim3(1), dim3(B / 16), sh, st, (const V*)in, (V*)out, (const V*)tw,
-                       sign);
-    return hipGetLastError();
-}
-
-hipError_t run_fft_selftest(int B, int is_f32, const void* in, void* out, const void* tw, int sign, hipStream_t st) {
-    if (!is_f32) {
-        if (B == 1024) return st_launch<1024, double>(in, out, tw, sign, st);
-        if (B == 4096) return st_launch<4096, double>(in, out, tw, sign, st);
-        if (B == 8192) return st_launch<8192, double>(in, out, tw, sign, st);
-    } else {
-        if (B == 1024) return st_launch<1024, float>(in, out, tw, sign, st);
-        if (B == 8192) return st_launch<8192, float>(in, out, tw, sign, st);
-        if (B == 16384) return st_launch<16384, float>(in, out, tw, sign, st);
-    }
-    return hipErrorInvalidValue;
 }
 
 }  // namespace cuvhs

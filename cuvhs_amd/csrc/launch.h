@@ -79,7 +79,9 @@ hipError_t run_diag_k4(const double* d05, const double* before, const double* af
 hipError_t run_diag_k5(const double* demod, const double* ll, ChunkCtl* ctl, int max_fields, const Fmt& fmt,
                        hipStream_t st);
 hipError_t run_input_mean(const void* cap, int in_fmt, long long n, double* partial, double* mean, hipStream_t st);
-hipError_t run_fft_selftest(int B, int is_f32, const void* in, void* out, const void* tw, int sign, hipStream_t st);
+// one convolution block of the kinds the decoder runs (gp in the block's bin order)
+hipError_t run_block_selftest(int B, int is_f32, const void* x, const void* gp, void* y, const void* tw,
+                              hipStream_t st);
 hipError_t run_atan2_selftest(const double* y, const double* x, double* out, long long n, hipStream_t st);
 int events_nwg(long long total);
 size_t events_mask_count(long long total);  // uint2 entries of the masks buffer

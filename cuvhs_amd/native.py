@@ -83,7 +83,7 @@ _SIGS = {
     "cuvhs_tbc_resample": (_i, [_vp, _vp, _i, _vp, _i, _vp]),
     "cuvhs_chroma": (_i, [_vp, _vp, _u64, _vp, _i, _vp, _vp]),
     "cuvhs_dropouts": (_i, [_vp, _vp, _vp, _i, _vp, _vp, _vp, _vp, _vp, _vp]),
-    "cuvhs_selftest_fft": (_i, [_i, _i, _vp, _vp, _i]),
+    "cuvhs_selftest_block": (_i, [_i, _i, _vp, _vp, _vp]),
     "cuvhs_selftest_atan2": (_i, [_vp, _vp, _vp, ctypes.c_int64]),
     "cuvhs_last_error": (ctypes.c_char_p, [_vp]),
     "cuvhs_profile": (_i, [_vp, _i]),

@@ -185,11 +185,14 @@ int cuvhs_chroma(cuvhs_ctx* ctx, const void* d_raw, uint64_t n_valid, const doub
 int cuvhs_dropouts(cuvhs_ctx* ctx, const double* d_env, const double* d_linelocs, int nfields, uint16_t* d_luma,
                    uint16_t* d_chroma, int* d_lines, int* d_starts, int* d_ends, int* d_count);
 
-/* Self-test of the LDS FFT building block (no counterpart in the reference):
- * one B-point transform (B in 1024/4096/8192 FP64, 1024/8192/16384 FP32) of
- * device data, sign -1 forward / +1 inverse, unnormalised; sign -2 is the
- * forward transform fed from registers, as the blocked convolutions use it. */
-int cuvhs_selftest_fft(int B, int is_f32, const void* d_in, void* d_out, int sign);
+/* Self-test of the convolution blocks the decoder runs (no counterpart in
+ * the reference, whose transforms are cuFFT's, fm_demod.cu:449-539):
+ * d_y = B * IFFT(FFT(d_x) * d_g) for one B-point block of complex values
+ * (double2, or float2 with is_f32), d_g in natural bin order.  B = 4096
+ * (K1 blocks, FP64 or FP32), 8192 (FP64 or FP32), 16384 (FP32, the chroma
+ * pre-band-pass), 1024 (FP64, the chroma line filter); other shapes return
+ * CUVHS_E_ARG. */
+int cuvhs_selftest_block(int B, int is_f32, const void* d_x, const void* d_g, void* d_y);
 
 /* Self-test of the FP64 arctangent of K1's phase (no counterpart in the
  * reference; replaces the atan2 of fm_demod.cu:203):

@@ -35,19 +35,27 @@ def dev(torch, a):
     return torch.from_numpy(np.ascontiguousarray(a)).to("cuda:0")
 
 
-@pytest.mark.parametrize("B,f32", [(1024, 0), (4096, 0), (8192, 0), (1024, 1), (8192, 1), (16384, 1)])
-def test_lds_fft_vs_numpy(torch, N, B, f32):
+@pytest.mark.parametrize("B,f32", [(1024, 0), (4096, 0), (8192, 0), (4096, 1), (8192, 1), (16384, 1)])
+def test_conv_blocks_vs_numpy(torch, N, B, f32):
+    """The convolution blocks the decoder runs (K1's 4096/8192-point blocks,
+    the pre-band-pass's 16384-point FP32 block, the chroma line's 1024-point
+    FP64 filter), each on random complex data and a random spectrum given in
+    natural bin order (permuted by the library as for its filters), against
+    B * ifft(fft(x) * g)."""
     rng = np.random.default_rng(B + f32)
     x = rng.standard_normal(B) + 1j * rng.standard_normal(B)
+    g = rng.standard_normal(B) + 1j * rng.standard_normal(B)
     dt, rt = (np.complex64, np.float32) if f32 else (np.complex128, np.float64)
-    xin = dev(torch, x.astype(dt).view(rt))
-    out = torch.empty_like(xin)
-    for sign in (-1, 1, -2):  # -2: forward transform fed from registers
-        assert N.lib().cuvhs_selftest_fft(B, f32, N.ptr(xin), N.ptr(out), sign) == 0
-        y = out.cpu().numpy().view(dt)
-        ref = np.fft.fft(x) if sign < 0 else np.fft.ifft(x) * B
-        tol = 1e-6 if f32 else 2e-15
-        assert np.abs(y - ref).max() / np.abs(ref).max() < tol
+    dx = dev(torch, x.astype(dt).view(rt))
+    dg = dev(torch, g.astype(dt).view(rt))
+    out = torch.empty_like(dx)
+    assert N.lib().cuvhs_selftest_block(B, f32, N.ptr(dx), N.ptr(dg), N.ptr(out)) == 0
+    y = out.cpu().numpy().view(dt)
+    ref = np.fft.ifft(np.fft.fft(x) * g) * B
+    tol = 2e-6 if f32 else 1e-13
+    assert np.abs(y - ref).max() / np.abs(ref).max() < tol
+    # unsupported shapes are refused, not run
+    assert N.lib().cuvhs_selftest_block(2048, f32, N.ptr(dx), N.ptr(dg), N.ptr(out)) == N.E_ARG
 
 
 def test_atan2_f64_vs_host(torch, N):

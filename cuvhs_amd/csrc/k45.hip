@@ -59,21 +59,34 @@ __device__ __forceinline__ double lane_value(double v, int k) {
     return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
 }
 
-// The n/2-th smallest of up to 64 samples, which is what the reference's
-// partial selection sort leaves at buf[n/2] (hsync_refine.cu:72-97): lane j
-// holds sample j and counts its stable rank.
-__device__ __forceinline__ double wv_window_median(const double* sig, int start, int count, int total, int lane) {
+// wavefront-scope ordering of LDS stores before later loads by other lanes
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// The n/2-th smallest of up to 64 samples (hsync_refine.cu:72-97): lane j
+// holds sample j and counts its stable rank against the window read back
+// from the wavefront's LDS slots as broadcasts, two samples per read.
+__device__ __forceinline__ double wv_window_median(const double* sig, int start, int count, int total, int lane,
+                                                   double* wl) {
     if (start < 0 || start + count > total || count <= 0) return 0.0;
     const int n = count < 64 ? count : 64;
     const int mid = n / 2;
     double v = lane < n ? sig[start + lane] : 0.0;
+    wl[lane] = v;
+    wave_sync();
     int r = 0;
-    for (int k = 0; k < n; k++) {
-        double vk = lane_value(v, k);
-        r += (vk < v) || (vk == v && k < lane);
+    for (int k = 0; k < n; k += 2) {
+        const double2 p = *reinterpret_cast<const double2*>(wl + k);
+        r += (p.x < v) || (p.x == v && k < lane);
+        if (k + 1 < n) r += (p.y < v) || (p.y == v && k + 1 < lane);
     }
     unsigned long long m = __ballot(lane < n && r == mid);
-    return lane_value(v, __ffsll((long long)m) - 1);
+    const double res = lane_value(v, __ffsll((long long)m) - 1);
+    wave_sync();  // the reads finish before the wavefront's next window is stored
+    return res;
 }
 
 // One wavefront per line (hsync_refine.cu:99-239).
@@ -81,8 +94,10 @@ __global__ __launch_bounds__(256) void k_hsync_refine(const double* __restrict__
                                                       const ChunkCtl* ctl, int nf_override, int total_per_field,
                                                       double thr, double one_usec_samples, int als, double hw,
                                                       double right_edge_offset) {
+    __shared__ __align__(16) double s_win[4][64];
     const int nf = nf_override > 0 ? nf_override : ctl->nf;
     const int lane = threadIdx.x & 63;
+    double* wl = s_win[threadIdx.x >> 6];
     const int idx = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (idx >= nf * kLinesPerFrame) return;
     const int line = idx % kLinesPerFrame;
@@ -97,9 +112,9 @@ __global__ __launch_bounds__(256) void k_hsync_refine(const double* __restrict__
     if (zc >= 0.0) {
         int zci = (int)(zc + 0.5);
         double sync_level = wv_window_median(d05, zci + (int)(1.0 * one_usec_samples), (int)(1.5 * one_usec_samples),
-                                             total, lane);
+                                             total, lane, wl);
         double porch_level = wv_window_median(d05, zci + (int)(8.0 * one_usec_samples), (int)(1.0 * one_usec_samples),
-                                              total, lane);
+                                              total, lane, wl);
         if (porch_level > sync_level) {
             double mid = (sync_level + porch_level) / 2.0;
             int rs = (int)(original - one_usec * 2);
@@ -116,9 +131,9 @@ __global__ __launch_bounds__(256) void k_hsync_refine(const double* __restrict__
         double zc_fr = rcross - hw;
         int zfi = (int)(zc_fr + 0.5);
         double rsync = wv_window_median(d05, zfi + (int)(1.0 * one_usec_samples), (int)(1.5 * one_usec_samples),
-                                        total, lane);
+                                        total, lane, wl);
         double rporch = wv_window_median(d05, zfi + (int)(hw + 1.0 * one_usec_samples), (int)(1.0 * one_usec_samples),
-                                         total, lane);
+                                         total, lane, wl);
         if (rporch > rsync) {
             double rmid = (rsync + rporch) / 2.0;
             double zc2r = wv_find_rising(d05, rs0, rc0, rmid, total, lane);

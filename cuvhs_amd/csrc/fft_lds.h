@@ -20,6 +20,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "wave.h"
+
 namespace cuvhs {
 
 // padded LDS index, and the LDS elements a B-point buffer needs
@@ -125,12 +127,6 @@ template <int S, typename T> __device__ __forceinline__ void dft16(typename C2<T
 // inverse retraces the same steps.  Two block barriers per block instead of
 // two per Stockham stage.
 
-// wavefront-scope ordering of LDS stores before later loads by other lanes
-__device__ __forceinline__ void wave_lds_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 // WB^(e0 * {1, 2, 4, 8}) from the table of WB^m, m < B (B a power of two).
 template <typename T, int B = 4096>
@@ -180,7 +176,7 @@ inline int conv4096_gperm(int k) {
 // earlier loads were made by the same lane.
 template <bool Block> __device__ __forceinline__ void lds_sync() {
     if constexpr (Block) __syncthreads();
-    else wave_lds_sync();
+    else wave_sync();
 }
 template <bool Block, bool Whole, typename V, typename WS, typename RS>
 __device__ __forceinline__ void lds_exchange(void* lds, V* v, WS ws, RS rs) {
@@ -355,7 +351,7 @@ __device__ __forceinline__ void conv_rows(typename C2<T>::type* buf, typename C2
     twiddle_run<-1, T>(v, wb);                                     // WR^(m c)
 #pragma unroll
     for (int c = 0; c < 16; c++) buf[pidx(row + L * c + l)] = v[c];
-    wave_lds_sync();
+    wave_sync();
     // pass B: lane (c = hi, g = lo) holds m = G b + g
 #pragma unroll
     for (int b = 0; b < 16; b++) v[b] = buf[pidx(row + L * hi + G * b + lo)];
@@ -364,7 +360,7 @@ __device__ __forceinline__ void conv_rows(typename C2<T>::type* buf, typename C2
     twiddle_run<-1, T>(v, wb);                                     // WL^(g e1)
 #pragma unroll
     for (int e1 = 0; e1 < 16; e1++) buf[pidx(row + L * hi + G * e1 + lo)] = v[e1];
-    wave_lds_sync();
+    wave_sync();
     // pass C: lane (c = hi, e1h = lo) holds (e1l, g) at v[G e1l + g]
 #pragma unroll
     for (int j = 0; j < 16; j++) v[j] = buf[pidx(row + L * hi + 16 * lo + j)];
@@ -379,7 +375,7 @@ __device__ __forceinline__ void conv_rows(typename C2<T>::type* buf, typename C2
     for (int e = 0; e < 16; e += G) dft_small<G, +1>(v + e);
 #pragma unroll
     for (int j = 0; j < 16; j++) buf[pidx(row + L * hi + 16 * lo + j)] = v[j];
-    wave_lds_sync();
+    wave_sync();
     // inverse pass B: lane (c, g)
 #pragma unroll
     for (int e1 = 0; e1 < 16; e1++) v[e1] = buf[pidx(row + L * hi + G * e1 + lo)];
@@ -388,7 +384,7 @@ __device__ __forceinline__ void conv_rows(typename C2<T>::type* buf, typename C2
     dft16<+1, T>(v);                                               // e1 -> b
 #pragma unroll
     for (int b = 0; b < 16; b++) buf[pidx(row + L * hi + G * b + lo)] = v[b];
-    wave_lds_sync();
+    wave_sync();
     // inverse pass A: lane m
 #pragma unroll
     for (int c = 0; c < 16; c++) v[c] = buf[pidx(row + L * c + l)];

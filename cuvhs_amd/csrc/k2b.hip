@@ -18,6 +18,7 @@
 
 #include "common.h"
 #include "launch.h"
+#include "wave.h"
 
 namespace cuvhs {
 
@@ -162,14 +163,6 @@ __device__ bool vblank_state_machine(const int* rs, const int* rl, PulseInfo pi,
     return done;
 }
 
-// v of lane k (k the same in every lane)
-__device__ __forceinline__ double k2b_lane_value(double v, int k) {
-    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
-    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, k);
-    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), k);
-    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
-}
-
 // first-HSYNC anchor given the field parity (pipeline.cu:747-845, without the
 // estimate-only branch which needs the previous field).  Called by all 64
 // lanes of the wave with the same arguments; every lane returns the result.
@@ -259,7 +252,7 @@ __device__ AnchorVariant anchor_variant(VP vp, int nvp, const int* vbp, double m
         while (mt) {
             const int j = __builtin_ctzll(mt);
             mt &= mt - 1ull;
-            hoff += k2b_lane_value(term, j);
+            hoff += lane_value(term, j);
             hc++;
         }
         if (mb) break;

@@ -8,6 +8,7 @@
 
 #include "common.h"
 #include "launch.h"
+#include "wave.h"
 
 namespace cuvhs {
 
@@ -48,22 +49,6 @@ __device__ __forceinline__ double wv_find_rising(const double* sig, int start, i
         }
     }
     return -1.0;
-}
-
-// The value lane k holds, for k the same in every lane: two scalar lane reads
-// instead of an LDS permute.
-__device__ __forceinline__ double lane_value(double v, int k) {
-    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
-    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, k);
-    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), k);
-    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
-}
-
-// wavefront-scope ordering of LDS stores before later loads by other lanes
-__device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 // The n/2-th smallest of up to 64 samples (hsync_refine.cu:72-97): lane j

@@ -20,6 +20,7 @@
 #include "common.h"
 #include "fft_lds.h"
 #include "launch.h"
+#include "wave.h"
 
 namespace cuvhs {
 
@@ -137,19 +138,24 @@ __global__ __launch_bounds__(64, 3) void k_chroma_line(ChromaLaunch a) {
 // an odd line of track 1 is the negated track-0 line.  One workgroup per
 // (line, field).
 __global__ __launch_bounds__(256) void k_chroma_out(ChromaLaunch a) {
-    __shared__ double s_acc;
     const int f = blockIdx.y, line = blockIdx.x;
     const int nf = a.nf_override > 0 ? a.nf_override : a.ctl->nf;
     if (f >= nf) return;
     const size_t row = (size_t)f * kFieldLines + line;
-    if (threadIdx.x == 0) {
-        const double* da = a.burst + row * kBurstLen;
-        double ssq = 0.0;
-        for (int i = 0; i < kBurstLen; i++) ssq += da[i] * da[i];
-        double rms = sqrt(ssq / (double)kBurstLen);
-        s_acc = (rms > 1.0) ? (a.burst_abs_ref / rms) : 0.0;
+    // burst RMS (chroma_decode.cu:205-221, whose atomic sum has no fixed
+    // order; here, as in the oracle, sample order): every wavefront loads the
+    // 36 burst samples at once and sums their squares through lane reads, so
+    // no lane waits on a workgroup barrier
+    const int lane = (int)threadIdx.x & 63;
+    const double* da = a.burst + row * kBurstLen;
+    const double bv = lane < kBurstLen ? da[lane] : 0.0;
+    double ssq = 0.0;
+    for (int i = 0; i < kBurstLen; i++) {
+        const double d = lane_value(bv, i);
+        ssq += d * d;
     }
-    __syncthreads();
+    const double rms = sqrt(ssq / (double)kBurstLen);
+    const double s_acc = (rms > 1.0) ? (a.burst_abs_ref / rms) : 0.0;
     const double acc = ((line & 1) && a.track[f]) ? -s_acc : s_acc;
     const float* fa = a.filt + row * kLineLen;
     uint16_t* o = a.out + row * kLineLen;

@@ -154,7 +154,11 @@ __global__ __launch_bounds__(B / 16, (k1_min_blocks<B, T>())) void k1_part_a(K1A
         else conv_rows<T, 512>(buf, v, tw, G);
         constexpr int NW = NT / 64;
         __shared__ T edge[NW][16];
-        const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+        // the thread index anew: indices derived from it before the
+        // transform are not kept live across it
+        int t = (int)threadIdx.x;
+        asm volatile("" : "+v"(t));
+        const int lane = t & 63, wv = t >> 6;
         T ang[16];
 #pragma unroll
         for (int r = 0; r < 16; r++) {

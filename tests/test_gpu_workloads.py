@@ -176,3 +176,22 @@ def test_aux_queue_schedule_same_outputs(torch, N, monkeypatch):
     got = _digest(dec.decode_capture(cap))
     dec.close()
     assert got == want
+
+
+def test_aux_queue_schedule_u16_dc_correct(torch, N, monkeypatch):
+    """The same with u16 input under --dc-correct in chunks of 16 fields:
+    heads planned on the device address the capture in 2-byte samples and
+    take each chunk's mean over the samples their plan reads."""
+    fmt = N.format_for(28.0)
+    cap = N.generate(1, n_samples=70 * fmt.samples_per_field)
+    s16 = np.clip(np.round((cap.astype(np.float64) - 128.0) * 200.0 - 2500.0), -32768, 32767).astype(np.int32)
+    u16 = dev(torch, (s16 + 32768).astype(np.uint16))
+    monkeypatch.delenv("GPU_MAX_HW_QUEUES", raising=False)
+    dec = N.Decoder(28.0, 16, input_format="u16", dc_correct=True)
+    want = _digest(dec.decode_capture(u16))
+    dec.close()
+    monkeypatch.setenv("GPU_MAX_HW_QUEUES", "8")
+    dec = N.Decoder(28.0, 16, input_format="u16", dc_correct=True)
+    got = _digest(dec.decode_capture(u16))
+    dec.close()
+    assert got == want and want[0] >= 48

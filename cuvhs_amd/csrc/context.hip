@@ -774,7 +774,7 @@ const char* cuvhs_last_error(const cuvhs_ctx* c) { return c ? c->err.c_str() : "
 
 int cuvhs_decode_chunk(cuvhs_ctx* c, const void* chunk, uint64_t avail, uint64_t raw_offset, int max_fields,
                        cuvhs_chunk_info* info) {
-    if (!c || !chunk || !info || max_fields < 1 || max_fields > c->F) return CUVHS_E_ARG;
+    if (!c || (!chunk && avail) || !info || max_fields < 1 || max_fields > c->F) return CUVHS_E_ARG;
     memset(info, 0, sizeof(*info));
     info->raw_offset = raw_offset;
     Slot& s = c->slot[0];
@@ -818,7 +818,7 @@ int cuvhs_decode_capture(cuvhs_ctx* c, const void* cap, uint64_t cap_len, int ma
                          uint16_t* d_chroma, uint64_t* d_file_loc, int32_t* d_is_first, int32_t* d_phase_id,
                          int32_t* d_do_count, int32_t* d_do_line, int32_t* d_do_startx, int32_t* d_do_endx,
                          int* fields) {
-    if (!c || !cap || !fields || max_out < 0) return CUVHS_E_ARG;
+    if (!c || (!cap && cap_len) || !fields || max_out < 0) return CUVHS_E_ARG;
     *fields = 0;
     const uint64_t spf = (uint64_t)c->fmt.samples_per_field;
     const size_t tbc = (size_t)kFieldLines * kLineLen;

@@ -136,14 +136,18 @@ typedef struct cuvhs_chunk_outputs {
  * past `avail` read as zero (pipeline.cu:1539-1541).  raw_offset is the
  * chunk's position in the capture (fileLoc, K2b cadence).  At most
  * max_fields (<= batch_fields) fields are decoded; info->next_raw_offset is
- * where the next chunk starts (pipeline.cu:1636-1651). */
+ * where the next chunk starts (pipeline.cu:1636-1651).  Fewer than one field
+ * of samples (avail == 0 with a null pointer included) decodes nothing and
+ * returns CUVHS_W_EOF in info->status. */
 int cuvhs_decode_chunk(cuvhs_ctx* ctx, const void* d_chunk, uint64_t avail, uint64_t raw_offset,
                        int max_fields, cuvhs_chunk_info* info);
 int cuvhs_chunk_outputs_get(const cuvhs_ctx* ctx, cuvhs_chunk_outputs* out);
 
 /* Decode a whole device-resident capture (Pipeline::run, pipeline.cu:2626-2713)
  * into caller device buffers sized for max_out_fields fields.  Returns the
- * field count through *fields. */
+ * field count through *fields: 0 for a capture shorter than one field
+ * (capture_len == 0 with a null pointer included).  More fields than
+ * max_out_fields: CUVHS_E_CAPACITY. */
 int cuvhs_decode_capture(cuvhs_ctx* ctx, const void* d_capture, uint64_t capture_len, int max_out_fields,
                          uint16_t* d_luma, uint16_t* d_chroma, uint64_t* d_file_loc, int32_t* d_is_first,
                          int32_t* d_phase_id, int32_t* d_do_count, int32_t* d_do_line, int32_t* d_do_startx,

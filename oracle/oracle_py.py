@@ -81,8 +81,12 @@ def decode(data, mhz, batch, trace=True, dc_correct=False, line_locator=0, diagn
         n = L.or_decode(d, data.ctypes.data, data.size)
         out = {"fields": n}
         tbc = n * 263 * 910
-        out["luma"] = np.ctypeslib.as_array(ctypes.cast(L.or_luma(d), ctypes.POINTER(ctypes.c_uint16)), (max(tbc, 1),))[:tbc].reshape(n, 263, 910).copy()
-        out["chroma"] = np.ctypeslib.as_array(ctypes.cast(L.or_chroma(d), ctypes.POINTER(ctypes.c_uint16)), (max(tbc, 1),))[:tbc].reshape(n, 263, 910).copy()
+        for key, fn in (("luma", L.or_luma), ("chroma", L.or_chroma)):
+            if n == 0:  # no field was produced: the plane buffers were never allocated
+                out[key] = np.zeros((0, 263, 910), np.uint16)
+            else:
+                ptr = ctypes.cast(fn(d), ctypes.POINTER(ctypes.c_uint16))
+                out[key] = np.ctypeslib.as_array(ptr, (tbc,)).reshape(n, 263, 910).copy()
         fl = np.zeros(n, np.uint64)
         isf = np.zeros(n, np.int32)
         ph = np.zeros(n, np.int32)

@@ -1,0 +1,178 @@
+"""Capture lengths and contents at the edges of the chunk loop
+(pipeline.cu:2626-2713 sizes each chunk from the whole fields left; a read
+shorter than one field ends the decode, pipeline.cu:1535): empty, one sample
+short of a field, exactly one field, one field and a fraction, two and a
+half fields, and a silent capture that takes the nominal-spacing fallback
+(pipeline.cu:1614-1626).  Each is decoded by the whole-capture entry point
+and the per-chunk entry point and compared with the oracle; the CLI writes
+the same files as the oracle's file decode.
+
+A field that reaches past the end of the capture demodulates zeros there
+(the chunk buffer is zero past the read, pipeline.cu:1539-1541): the
+band-passed signal falls to the transform's rounding floor (an envelope
+near 1e-6 against 1e10 inside the capture) and the instantaneous frequency
+is the phase of rounding noise, which depends on the transform's summation
+order.  Those pixels are not determined by the capture, so the u16 planes
+are compared on the rows whose span, plus two lines for the hsync search
+windows and the chroma filter, lies inside the read; the integer outputs
+are compared on every field.  The silent capture is exact arithmetic end to
+end (all-zero samples, no sync), so its outputs are compared exactly."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+
+pytestmark = pytest.mark.gpu
+
+MHZ = 28.0
+SPF = 468140
+W_NO_VSYNC = 2
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch as t
+    assert t.cuda.is_available()
+    return t
+
+
+@pytest.fixture(scope="module")
+def N():
+    from cuvhs_amd import native
+    native.lib()
+    return native
+
+
+@pytest.fixture(scope="module")
+def dec(N):
+    d = N.Decoder(MHZ, 64)
+    yield d
+    d.close()
+
+
+SPL = 1780
+LPF = 525
+
+
+def rows_inside(ref, n_read):
+    """[fields, 263] mask of the output rows whose samples (the row's span
+    from the oracle's refined line locations, plus two lines) lie inside
+    the first chunk's read.  Every case here decodes in one chunk at raw
+    offset 0, so chunk positions are capture positions."""
+    c = ref["chunks"][0]
+    n = ref["fields"]
+    ll = np.asarray(c["ll_refined"]).reshape(-1)[:n * LPF].reshape(n, LPF)
+    return ll[:, 1:264] + 2 * SPL < n_read
+
+
+def compare(out, ref, exact=False, rows=None):
+    n = out["fields"]
+    assert n == ref["fields"]
+    for k in ("file_loc", "is_first", "phase_id", "do_count"):
+        g = out[k].cpu().numpy() if out[k] is not None else np.zeros(0)
+        assert np.array_equal(g.astype(np.int64), np.asarray(ref[k]).astype(np.int64)), k
+    dc = ref["do_count"]
+    for k in ("do_line", "do_startx", "do_endx"):
+        if n == 0:
+            continue
+        g = out[k].cpu().numpy()
+        for i in range(n):
+            assert np.array_equal(g[i, :dc[i]], ref[k][i, :dc[i]]), (k, i)
+    for k in ("luma", "chroma"):
+        if n == 0:
+            assert out[k] is None or out[k].numel() == 0
+            continue
+        d = np.abs(out[k].cpu().numpy().astype(np.int64) - ref[k].astype(np.int64))
+        if rows is not None:
+            d = d[rows]
+        assert int(d.max()) <= (0 if exact else 1), (k, int(d.max()), int((d > 1).sum()))
+
+
+def silent(n):
+    return np.full(n, 128, np.uint8)
+
+
+CASES = {
+    "empty": lambda c: c[:0],
+    "one_short": lambda c: c[:SPF - 1],
+    "one_field": lambda c: c[:SPF],
+    "one_and_a_bit": lambda c: c[:SPF + 100],
+    "two_and_a_half": lambda c: c[:int(2.5 * SPF)],
+}
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_capture_lengths_vs_oracle(torch, N, dec, oracle, cap0, case):
+    cap = np.ascontiguousarray(CASES[case](cap0))
+    print(f"case {case}: {cap.size} samples", flush=True)
+    ref = oracle.decode(cap, MHZ, 64, trace=True)
+    want = {"empty": 0, "one_short": 0, "one_field": 1, "one_and_a_bit": 1, "two_and_a_half": 2}[case]
+    assert ref["fields"] == want
+    rows = None
+    if want:
+        rows = rows_inside(ref, cap.size)
+        # the fields start in the first 40 lines, so well over 200 rows of each lie inside
+        assert (rows.sum(axis=1) > 200).all()
+    d = torch.from_numpy(cap).to("cuda:0")
+    dec.reset()
+    out = dec.decode_capture(d)
+    assert out["status"] >= 0
+    compare(out, ref, rows=rows)
+    dec.reset()
+    outc = dec.decode_chunked(d)
+    compare(outc, ref, rows=rows)
+    if want == 0:
+        assert outc["chunks"] == []
+
+
+def test_silent_capture_nominal_fallback(torch, N, dec, oracle):
+    cap = silent(5 * SPF + 777)
+    ref = oracle.decode(cap, MHZ, 64, trace=True)
+    assert ref["fields"] == 5
+    assert [int(x) for x in ref["file_loc"]] == [k * SPF for k in range(5)]
+    d = torch.from_numpy(cap).to("cuda:0")
+    dec.reset()
+    out = dec.decode_capture(d)
+    assert out["status"] & W_NO_VSYNC
+    compare(out, ref, exact=True)
+    dec.reset()
+    outc = dec.decode_chunked(d)
+    assert [c["status"] & W_NO_VSYNC for c in outc["chunks"]] == [W_NO_VSYNC]
+    assert outc["chunks"][0]["vsync_candidates"] == 0
+    assert outc["chunks"][0]["next_raw_offset"] == ref["chunks"][0]["next_offset"]
+    compare(outc, ref, exact=True)
+
+
+def test_output_capacity_too_small(torch, N, dec, cap0):
+    d = torch.from_numpy(cap0).to("cuda:0")
+    dec.reset()
+    with pytest.raises(N.CuvhsError, match="decode_capture"):
+        dec.decode_capture(d, max_fields=3)
+    # the context stays usable after the refused call
+    dec.reset()
+    out = dec.decode_capture(d)
+    assert out["fields"] == 4
+
+
+@pytest.mark.parametrize("case", ["one_short", "silent"])
+def test_cli_edge_files_match_oracle(tmp_path, oracle, cap0, case):
+    exe = os.path.join(ROOT, "cuvhs_amd", "lib", "cuvhs")
+    cap = cap0[:SPF - 1] if case == "one_short" else silent(2 * SPF + 5)
+    src = str(tmp_path / "cap.u8")
+    cap.tofile(src)
+    print(f"cli case {case}: {cap.size} samples", flush=True)
+    r = subprocess.run([exe, src, str(tmp_path / "g")], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    nf = oracle.lib().or_decode_file(src.encode(), str(tmp_path / "o").encode(), MHZ, 64)
+    assert nf == (0 if case == "one_short" else 2)
+    if case == "silent":
+        assert "no VSYNC candidates found in chunk at raw offset 0" in r.stderr
+    with open(tmp_path / "g.tbc.json") as a, open(tmp_path / "o.tbc.json") as b:
+        assert a.read() == b.read()
+    for s in (".tbc", "_chroma.tbc"):
+        a = np.fromfile(str(tmp_path / ("g" + s)), np.uint16)
+        b = np.fromfile(str(tmp_path / ("o" + s)), np.uint16)
+        assert a.size == b.size == nf * 263 * 910 and np.array_equal(a, b)

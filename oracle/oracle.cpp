@@ -1300,6 +1300,7 @@ void dropout_detect(const Fmt& fmt, const double* env, const double* linelocs, u
 struct ChunkTrace {
     size_t raw_offset = 0, next_offset = 0;
     int fields_k1 = 0, nf = 0;
+    int candidates = 0;                   // VSYNC candidates found, before the cap
     std::vector<size_t> offsets;
     std::vector<int> pc, ps, pl;          // K2 output
     std::vector<int> vc, vs, vl, vt;      // K2b output
@@ -1659,6 +1660,7 @@ static int process_chunk(or_dec* d, const uint8_t* data, size_t total_n, size_t 
     int cap = fields_loaded * 20;
     std::vector<int> cand;
     int num_candidates = vsync_discover(fmt, d->d05.data(), total_chunk, cand, cap);
+    tr.candidates = num_candidates;
     if (num_candidates > cap)
         fprintf(stderr, "Warning: VSYNC candidate buffer overflow (%d > %d), truncating chunk results\n",
                 num_candidates, cap);
@@ -1961,6 +1963,11 @@ int or_field_dropouts(const or_dec* d, int field, int* lines, int* startx, int* 
 }
 
 int or_num_chunks(const or_dec* d) { return (int)d->traces.size(); }
+
+int or_trace_candidates(const or_dec* d, int c) {
+    if (c < 0 || c >= (int)d->traces.size()) return -1;
+    return d->traces[(size_t)c].candidates;
+}
 
 int or_trace_chunk(const or_dec* d, int c, uint64_t* raw_offset, int* fields_k1, uint64_t* next_offset,
                    uint64_t* offsets, int* pulse_count, int* pulse_starts, int* pulse_lens, int* vpulse_count,

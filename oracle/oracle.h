@@ -117,6 +117,8 @@ int or_trace_chunk(const or_dec* d, int c, uint64_t* raw_offset, int* fields_k1,
                    int* vpulse_lens, int* vpulse_types, double* ll_coarse, double* ll_refined,
                    int* is_first_k2b);
 int or_num_chunks(const or_dec* d);
+/* VSYNC candidates chunk c found (may exceed the cap of 20 per field); -1 if no such chunk */
+int or_trace_candidates(const or_dec* d, int c);
 
 /* .tbc.json from explicit metadata (do_* arrays concatenated over fields) */
 int or_write_json_meta(double mhz, const char* path, int n, const uint64_t* file_loc, const int* is_first,

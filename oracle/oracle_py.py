@@ -36,6 +36,7 @@ def lib():
         L.or_decode.argtypes = [_vp, _vp, ctypes.c_size_t]
         L.or_num_fields.argtypes = [_vp]
         L.or_num_chunks.argtypes = [_vp]
+        L.or_trace_candidates.argtypes = [_vp, ctypes.c_int]
         L.or_luma.restype = _vp
         L.or_luma.argtypes = [_vp]
         L.or_chroma.restype = _vp
@@ -123,7 +124,7 @@ def decode(data, mhz, batch, trace=True, dc_correct=False, line_locator=0, diagn
                                    k2_lens=pl.reshape(B, -1)[:nf], vp_count=vc[:nf], vp_starts=vs.reshape(B, -1)[:nf],
                                    vp_lens=vl.reshape(B, -1)[:nf], vp_types=vt.reshape(B, -1)[:nf],
                                    ll_coarse=llc.reshape(B, -1)[:nf], ll_refined=llr.reshape(B, -1)[:nf],
-                                   is_first_k2b=isk[:nf]))
+                                   is_first_k2b=isk[:nf], candidates=L.or_trace_candidates(d, c)))
         out["chunks"] = chunks
         if diagnostics:
             diag = []

@@ -29,6 +29,8 @@ pytestmark = pytest.mark.gpu
 
 MHZ = 28.0
 SPF = 468140
+SPFP = 503740
+W_VSYNC_OVERFLOW = 1
 W_NO_VSYNC = 2
 
 
@@ -57,15 +59,17 @@ SPL = 1780
 LPF = 525
 
 
-def rows_inside(ref, n_read):
+def rows_inside(ref, n, batch=64, spfp=503740):
     """[fields, 263] mask of the output rows whose samples (the row's span
     from the oracle's refined line locations, plus two lines) lie inside
-    the first chunk's read.  Every case here decodes in one chunk at raw
-    offset 0, so chunk positions are capture positions."""
-    c = ref["chunks"][0]
-    n = ref["fields"]
-    ll = np.asarray(c["ll_refined"]).reshape(-1)[:n * LPF].reshape(n, LPF)
-    return ll[:, 1:264] + 2 * SPL < n_read
+    their chunk's read.  Line locations are chunk positions; a chunk reads
+    min(batch * spfp, n - raw_offset) samples."""
+    masks = []
+    for c in ref["chunks"]:
+        n_read = min(batch * spfp, n - c["raw_offset"])
+        ll = np.asarray(c["ll_refined"]).reshape(c["nf"], LPF)
+        masks.append(ll[:, 1:264] + 2 * SPL < n_read)
+    return np.concatenate(masks)
 
 
 def compare(out, ref, exact=False, rows=None):
@@ -124,6 +128,12 @@ def test_capture_lengths_vs_oracle(torch, N, dec, oracle, cap0, case):
     dec.reset()
     outc = dec.decode_chunked(d)
     compare(outc, ref, rows=rows)
+    # candidate counts where the chunk's K1 segments lie inside the read (past
+    # it, VSYNC-width runs of the rounding noise count too: one field of this
+    # capture has 6 inside and, in the oracle, 3 more past the end)
+    for g, o in zip(outc["chunks"], ref["chunks"]):
+        if o["fields_k1"] * SPFP <= cap.size - o["raw_offset"]:
+            assert g["vsync_candidates"] == o["candidates"]
     if want == 0:
         assert outc["chunks"] == []
 
@@ -144,6 +154,82 @@ def test_silent_capture_nominal_fallback(torch, N, dec, oracle):
     assert outc["chunks"][0]["vsync_candidates"] == 0
     assert outc["chunks"][0]["next_raw_offset"] == ref["chunks"][0]["next_offset"]
     compare(outc, ref, exact=True)
+
+
+def square_fm(n, colour_under):
+    """u8 RF: FM alternating 800 samples at the sync tip (-40 IRE, a
+    VSYNC-width run below the pulse threshold) with 1200 at 44 IRE, plus,
+    if given, a colour-under carrier (40 fH) of that amplitude in u8 steps."""
+    from cuvhs_amd import native
+    fmt = native.format_for(MHZ)
+    fs = fmt.sample_rate
+    k = np.arange(n)
+    f = np.where((k % 2000) < 800, fmt.ire0 - 40 * fmt.hz_ire, fmt.ire0 + 44 * fmt.hz_ire)
+    rf = 128 + 90 * np.cos(np.cumsum(2 * np.pi * f / fs))
+    if colour_under:
+        rf = rf + colour_under * np.cos(2 * np.pi * 40 * 15734.264 / fs * k)
+    return np.round(rf).astype(np.uint8)
+
+
+def decode_both(dec, torch, cap):
+    d = torch.from_numpy(cap).to("cuda:0")
+    dec.reset()
+    out = dec.decode_capture(d)
+    dec.reset()
+    return out, dec.decode_chunked(d)
+
+
+def test_vsync_candidate_overflow(torch, N, dec, oracle):
+    """More VSYNC-width runs than the candidate cap (20 per field,
+    pipeline.cu:1578-1590), square-wave FM with a colour-under carrier, two
+    padded segments long so that no chunk's K1 segments reach past its read
+    (every candidate is then a run of the capture): the first chunk finds
+    504 candidates for a cap of 40, keeps the first 40 in ascending order
+    and clusters them into one field; the second chunk (one field)
+    overflows again."""
+    n = 2 * SPFP
+    cap = square_fm(n, 20)
+    ref = oracle.decode(cap, MHZ, 64, trace=True)
+    want_cands = [c["candidates"] for c in ref["chunks"]]
+    assert want_cands == [504, 251] and ref["fields"] == 2
+    assert all(c["fields_k1"] * SPFP <= n - c["raw_offset"] for c in ref["chunks"])
+    rows = rows_inside(ref, n)
+    assert (rows.sum(axis=1) > 200).all()
+    out, outc = decode_both(dec, torch, cap)
+    assert out["status"] & W_VSYNC_OVERFLOW
+    compare(out, ref, rows=rows)
+    assert [c["vsync_candidates"] for c in outc["chunks"]] == want_cands
+    assert all(c["status"] & W_VSYNC_OVERFLOW for c in outc["chunks"])
+    assert [c["raw_offset"] for c in outc["chunks"]] == [c["raw_offset"] for c in ref["chunks"]]
+    assert [c["fields_k1"] for c in outc["chunks"]] == [c["fields_k1"] for c in ref["chunks"]]
+    compare(outc, ref, rows=rows)
+
+
+def test_chroma_band_empty(torch, N, dec, oracle):
+    """The same square-wave FM without a colour-under carrier: the chroma
+    band (60 kHz - 1.2 MHz) holds only the u8 quantisation, its RMS 1/443 of
+    the input's (1/4.6 with the carrier, about 1/7 for the synthetic
+    captures).  The FP64 form runs the chroma pre-band-pass in FP32 with
+    taps truncated at 1e-7 of the largest; both errors are proportional to
+    the input's RMS, and the burst gain scales the band-passed signal to a
+    fixed burst level, so in u16 steps the error grows with that ratio:
+    here up to 7 steps (2 to 7 on about 2% of the pixels, on the lines
+    whose burst window holds an FM step).  Integer outputs and luma stay
+    exact / within 1 step.  The bound is asserted so that a change in it
+    shows."""
+    n = 2 * SPFP
+    cap = square_fm(n, 0)
+    ref = oracle.decode(cap, MHZ, 64, trace=True)
+    assert [c["candidates"] for c in ref["chunks"]] == [504, 251]
+    rows = rows_inside(ref, n)
+    out, _ = decode_both(dec, torch, cap)
+    assert out["fields"] == ref["fields"]
+    for k in ("file_loc", "is_first", "phase_id", "do_count"):
+        assert np.array_equal(out[k].cpu().numpy().astype(np.int64), np.asarray(ref[k]).astype(np.int64)), k
+    d = np.abs(out["luma"].cpu().numpy().astype(np.int64) - ref["luma"].astype(np.int64))[rows]
+    assert int(d.max()) <= 1
+    d = np.abs(out["chroma"].cpu().numpy().astype(np.int64) - ref["chroma"].astype(np.int64))[rows]
+    assert int(d.max()) <= 8 and (d > 1).mean() < 0.05, (int(d.max()), float((d > 1).mean()))
 
 
 def test_output_capacity_too_small(torch, N, dec, cap0):

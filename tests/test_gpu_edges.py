@@ -232,6 +232,18 @@ def test_chroma_band_empty(torch, N, dec, oracle):
     assert int(d.max()) <= 8 and (d > 1).mean() < 0.05, (int(d.max()), float((d > 1).mean()))
 
 
+def test_placed_dropouts(torch, N, dec, oracle, cap0, ref0):
+    """Capture 0 with RF dips placed inside and outside the K7 scan windows
+    (the oracle's entries for it are pinned in test_oracle_dropouts)."""
+    from test_oracle_dropouts import dipped_capture
+    cap, _ = dipped_capture(cap0, ref0["chunks"][0]["ll_refined"])
+    ref = oracle.decode(cap, MHZ, 64, trace=True)
+    assert list(ref["do_count"][:3]) == [0, 1, 1]
+    out, outc = decode_both(dec, torch, cap)
+    compare(out, ref)
+    compare(outc, ref)
+
+
 def test_output_capacity_too_small(torch, N, dec, cap0):
     d = torch.from_numpy(cap0).to("cuda:0")
     dec.reset()

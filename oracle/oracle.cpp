@@ -12,6 +12,17 @@
 //     copy, dropout_detect.cu:274-290).
 //   * per-field mean envelope and burst sums are summed sequentially
 //     (reference: parallel reductions in an unspecified order).
+//
+// How each part is pinned (tests/): format, convert and writer against the
+// reference's own host sources built in oracle/_ref; FFT against numpy and
+// brute-force DFTs; filter design against scipy.signal; demod, VSYNC, K2,
+// K4, K5 by closed forms and brute force; K7 by RF dips placed at known
+// samples.  Pinned by invariants only: k2b_chunk (clean-input line
+// locations on pulse starts, fallback spacing), the VSYNC clustering (field
+// offsets one field period apart), the chroma track trial (right track's
+// metric far below, choice follows the recorded field).  Parity unpinned:
+// K2b's vblank state machine and first-HSYNC voting beyond those
+// invariants, and the chroma retry and phase-ID selection in chroma_decode.
 #include "oracle.h"
 
 #include <algorithm>
@@ -952,6 +963,7 @@ struct ChromaState {
     int current_track = 0;
     double good_metric_threshold = 0.0;
     int cycle_start = 0;
+    double trial_metric[2] = {0.0, 0.0};   // field-0 trial of the first chunk, per track (test access)
 };
 
 struct NTSCPhaseEntry { int is_first, quadrant, delta, phase_id, offset; };
@@ -1127,6 +1139,7 @@ void chroma_decode(const Fmt& fmt, const double* raw, const double* linelocs, do
         }
         detected_track = (metric[1] < metric[0]) ? 1 : 0;
         thr = metric[detected_track] * 4.0;
+        if (state) { state->trial_metric[0] = metric[0]; state->trial_metric[1] = metric[1]; }
         tr[0] = detected_track;
         chroma_lines(fmt, cc, plan, craw, linelocs, total, 0, 1, tr, buf);
         double ph0 = burst_phase(buf.data(), fmt, cc);
@@ -1963,6 +1976,14 @@ int or_field_dropouts(const or_dec* d, int field, int* lines, int* startx, int* 
 }
 
 int or_num_chunks(const or_dec* d) { return (int)d->traces.size(); }
+
+int or_chroma_trial(const or_dec* d, double* metrics, int* track) {
+    if (!d->chroma.valid) return -1;
+    metrics[0] = d->chroma.trial_metric[0];
+    metrics[1] = d->chroma.trial_metric[1];
+    *track = metrics[1] < metrics[0] ? 1 : 0;
+    return 0;
+}
 
 int or_trace_candidates(const or_dec* d, int c) {
     if (c < 0 || c >= (int)d->traces.size()) return -1;

@@ -119,6 +119,9 @@ int or_trace_chunk(const or_dec* d, int c, uint64_t* raw_offset, int* fields_k1,
 int or_num_chunks(const or_dec* d);
 /* VSYNC candidates chunk c found (may exceed the cap of 20 per field); -1 if no such chunk */
 int or_trace_candidates(const or_dec* d, int c);
+/* burst metrics of the field-0 track trial (track 0, track 1) and the track
+ * chosen; -1 before any chroma was decoded */
+int or_chroma_trial(const or_dec* d, double* metrics, int* track);
 
 /* .tbc.json from explicit metadata (do_* arrays concatenated over fields) */
 int or_write_json_meta(double mhz, const char* path, int n, const uint64_t* file_loc, const int* is_first,

@@ -37,6 +37,7 @@ def lib():
         L.or_num_fields.argtypes = [_vp]
         L.or_num_chunks.argtypes = [_vp]
         L.or_trace_candidates.argtypes = [_vp, ctypes.c_int]
+        L.or_chroma_trial.argtypes = [_vp, _vp, _vp]
         L.or_luma.restype = _vp
         L.or_luma.argtypes = [_vp]
         L.or_chroma.restype = _vp
@@ -126,6 +127,10 @@ def decode(data, mhz, batch, trace=True, dc_correct=False, line_locator=0, diagn
                                    ll_coarse=llc.reshape(B, -1)[:nf], ll_refined=llr.reshape(B, -1)[:nf],
                                    is_first_k2b=isk[:nf], candidates=L.or_trace_candidates(d, c)))
         out["chunks"] = chunks
+        m = np.zeros(2)
+        tk = np.zeros(1, np.int32)
+        if L.or_chroma_trial(d, m.ctypes.data, tk.ctypes.data) == 0:
+            out["track_trial"] = {"metrics": m, "track": int(tk[0])}
         if diagnostics:
             diag = []
             for c in range(L.or_num_chunks(d) if trace else 10 ** 6):

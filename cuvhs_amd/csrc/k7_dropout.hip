@@ -259,14 +259,20 @@ __global__ __launch_bounds__(kDoThreads) void k_dropouts(const unsigned char* __
         do_ends[o] = s_ent[3 * i + 2];
     }
     if (t == 0) do_count[f] = ne;
-    // concealment, column-owned threads, entries in order
+    // concealment, entries in order: column col belongs to thread
+    // col % kDoThreads in every entry, so a row written by one entry and read
+    // by a later one (row l concealed, then row l + 1 copied from it) is
+    // handled by one thread in program order.  Starting each entry at sx + t
+    // instead would hand a column to different threads in entries with
+    // different sx, a race with no barrier between the entries.
     uint16_t* lu = luma + (size_t)f * kFieldLines * kLineLen;
     uint16_t* ch = chroma + (size_t)f * kFieldLines * kLineLen;
     for (int d = 0; d < ne; d++) {
         int line = s_ent[3 * d], sx = s_ent[3 * d + 1], ex = s_ent[3 * d + 2];
         int src = (line > 0) ? line - 1 : line + 1;
         if (src >= kFieldLines) continue;
-        for (int col = sx + t; col < ex; col += kDoThreads) {
+        for (int col = t; col < ex; col += kDoThreads) {
+            if (col < sx) continue;
             lu[line * kLineLen + col] = lu[src * kLineLen + col];
             ch[line * kLineLen + col] = ch[src * kLineLen + col];
         }

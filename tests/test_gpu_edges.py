@@ -244,6 +244,21 @@ def test_placed_dropouts(torch, N, dec, oracle, cap0, ref0):
     compare(outc, ref)
 
 
+@pytest.mark.parametrize("start", [1, 2])
+def test_capture_starting_on_either_track(torch, N, dec, oracle, start):
+    """A capture that starts `start` recorded fields into the tape: the
+    chroma track trial picks track start % 2 (the other heterodyne rotation
+    for odd starts), as the oracle does."""
+    g = N.generate(1, n_samples=(start + 4) * SPF)
+    cap = np.ascontiguousarray(g[start * SPF:])
+    ref = oracle.decode(cap, MHZ, 64, trace=True)
+    assert ref["track_trial"]["track"] == start % 2 and ref["fields"] == 4
+    rows = rows_inside(ref, cap.size)
+    out, outc = decode_both(dec, torch, cap)
+    compare(out, ref, rows=rows)
+    compare(outc, ref, rows=rows)
+
+
 def test_output_capacity_too_small(torch, N, dec, cap0):
     d = torch.from_numpy(cap0).to("cuda:0")
     dec.reset()

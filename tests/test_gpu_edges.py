@@ -59,7 +59,7 @@ SPL = 1780
 LPF = 525
 
 
-def rows_inside(ref, n, batch=64, spfp=503740):
+def rows_inside(ref, n, batch=64, spfp=503740, spl=SPL):
     """[fields, 263] mask of the output rows whose samples (the row's span
     from the oracle's refined line locations, plus two lines) lie inside
     their chunk's read.  Line locations are chunk positions; a chunk reads
@@ -68,7 +68,7 @@ def rows_inside(ref, n, batch=64, spfp=503740):
     for c in ref["chunks"]:
         n_read = min(batch * spfp, n - c["raw_offset"])
         ll = np.asarray(c["ll_refined"]).reshape(c["nf"], LPF)
-        masks.append(ll[:, 1:264] + 2 * SPL < n_read)
+        masks.append(ll[:, 1:264] + 2 * spl < n_read)
     return np.concatenate(masks)
 
 
@@ -136,6 +136,37 @@ def test_capture_lengths_vs_oracle(torch, N, dec, oracle, cap0, case):
             assert g["vsync_candidates"] == o["candidates"]
     if want == 0:
         assert outc["chunks"] == []
+
+
+@pytest.mark.parametrize("mhz,fmt_name,dc,fields", [(40.0, "u8", False, 1.5), (28.0, "u16", True, 1.2),
+                                                     (28.0, "s16", False, 2.5)])
+def test_short_captures_other_rates_and_formats(torch, N, oracle, mhz, fmt_name, dc, fields):
+    """Partial-field captures through the 40 MHz decoder (8192-point K1
+    blocks) and the s16 / u16 input paths (u16 with the per-read mean
+    removed, raw_reader.cpp:93-100)."""
+    f = N.format_for(mhz)
+    n = int(fields * f.samples_per_field)
+    g = N.generate(2 if mhz == 40.0 else 1, sample_rate_mhz=mhz, n_samples=n + f.samples_per_field)[:n]
+    if fmt_name == "u16":
+        cap = (g.astype(np.uint16) << 8) | np.uint16(0x55)
+    elif fmt_name == "s16":
+        cap = ((g.astype(np.int16) - 128) * 256).astype(np.int16)
+    else:
+        cap = np.ascontiguousarray(g)
+    print(f"{mhz} MHz {fmt_name} dc={dc}: {n} samples", flush=True)
+    ref = oracle.decode(cap, mhz, 64, trace=True, dc_correct=dc)
+    assert ref["fields"] == int(fields)
+    rows = rows_inside(ref, n, spfp=f.spfp, spl=f.samples_per_line)
+    assert (rows.sum(axis=1) > 200).all()
+    d = N.Decoder(mhz, 64, input_format=fmt_name, dc_correct=dc)
+    try:
+        dd = torch.from_numpy(cap).to("cuda:0")
+        out = d.decode_capture(dd)
+        compare(out, ref, rows=rows)
+        d.reset()
+        compare(d.decode_chunked(dd), ref, rows=rows)
+    finally:
+        d.close()
 
 
 def test_silent_capture_nominal_fallback(torch, N, dec, oracle):

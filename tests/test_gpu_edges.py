@@ -320,3 +320,11 @@ def test_cli_edge_files_match_oracle(tmp_path, oracle, cap0, case):
         a = np.fromfile(str(tmp_path / ("g" + s)), np.uint16)
         b = np.fromfile(str(tmp_path / ("o" + s)), np.uint16)
         assert a.size == b.size == nf * 263 * 910 and np.array_equal(a, b)
+    # the same capture streamed on stdin (length unknown to the reader)
+    with open(src, "rb") as f:
+        r = subprocess.run([exe, "-", str(tmp_path / "s")], stdin=f, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "streaming" in r.stderr
+    for s in (".tbc", "_chroma.tbc", ".tbc.json"):
+        with open(tmp_path / ("g" + s), "rb") as a, open(tmp_path / ("s" + s), "rb") as b:
+            assert a.read() == b.read(), s

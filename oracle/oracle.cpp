@@ -20,9 +20,11 @@
 // samples.  Pinned by invariants only: k2b_chunk (clean-input line
 // locations on pulse starts, fallback spacing), the VSYNC clustering (field
 // offsets one field period apart), the chroma track trial (right track's
-// metric far below, choice follows the recorded field).  Parity unpinned:
-// K2b's vblank state machine and first-HSYNC voting beyond those
-// invariants, and the chroma retry and phase-ID selection in chroma_decode.
+// metric far below, choice follows the recorded field), the phase-ID cycle
+// (3, 2, 1, 4 field after field, across chunks).  Parity unpinned: K2b's
+// vblank state machine and first-HSYNC voting beyond those invariants, and
+// in chroma_decode the retry and the first field's phase ID from the burst
+// phase.
 #include "oracle.h"
 
 #include <algorithm>

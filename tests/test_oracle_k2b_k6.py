@@ -59,3 +59,19 @@ def test_chroma_track_follows_the_recording(oracle, start):
     track = r["track_trial"]["track"]
     assert track == start % 2
     assert m[1 - track] > 5.0 * m[track], m
+
+
+def test_phase_ids_follow_the_cycle_across_chunks(oracle):
+    """Phase IDs step through the 8-field cycle 3, 2, 1, 4, 3, 2, 1, 4
+    (chroma_decode.cu:671-672, 761) field after field, also across chunk
+    boundaries, where the cycle position is carried in the chroma state
+    (chroma_decode.cu:999-1000):
+    70 fields in chunks of 16 (five chunks)."""
+    from cuvhs_amd import native
+    cap = native.generate(1, n_samples=70 * SPF)
+    r = oracle.decode(cap, 28.0, 16, trace=True)
+    assert len(r["chunks"]) == 5
+    ids = [int(x) for x in r["phase_id"]]
+    assert len(ids) == r["fields"] >= 65
+    order = {3: 2, 2: 1, 1: 4, 4: 3}   # successor in the cycle
+    assert all(order[a] == b for a, b in zip(ids, ids[1:])), ids

@@ -21,10 +21,11 @@
 // locations on pulse starts, fallback spacing), the VSYNC clustering (field
 // offsets one field period apart), the chroma track trial (right track's
 // metric far below, choice follows the recorded field), the phase-ID cycle
-// (3, 2, 1, 4 field after field, across chunks).  Parity unpinned: K2b's
-// vblank state machine and first-HSYNC voting beyond those invariants, and
-// in chroma_decode the retry and the first field's phase ID from the burst
-// phase.
+// (3, 2, 1, 4 field after field, across chunks), the vblank state machine,
+// first-HSYNC anchor and field order on clean vertical intervals.  Parity
+// unpinned: K2b on degraded vertical intervals (vote confidences, the
+// cross-field anchor estimate), and in chroma_decode the retry and the
+// first field's phase ID from the burst phase.
 #include "oracle.h"
 
 #include <algorithm>

@@ -75,3 +75,29 @@ def test_phase_ids_follow_the_cycle_across_chunks(oracle):
     assert len(ids) == r["fields"] >= 65
     order = {3: 2, 2: 1, 1: 4, 4: 3}   # successor in the cycle
     assert all(order[a] == b for a, b in zip(ids, ids[1:])), ids
+
+
+@pytest.mark.parametrize("start", [0, 1])
+def test_k2b_vblank_anchor_and_field_order(oracle, start):
+    """The vertical interval of a clean capture (the generator writes 6
+    equalising, 6 broad and 6 equalising pulses at half-line spacing) comes
+    out of K2b's vblank state machine (pipeline.cu:519-609) as 6 EQ1, 6 VSYNC
+    and 6 EQ2 pulses;
+    line location 10 is the first HSYNC after them (the first-HSYNC anchor,
+    pipeline.cu:650-855); K2b's field order matches the generator's field
+    type (even recorded fields start their lines on whole-line origins),
+    also when the capture starts one recorded field later."""
+    from cuvhs_amd import native
+    g = native.generate(0, snr_lo_db=60.0, snr_hi_db=60.0, dropouts_per_field=0.0, n_samples=(start + 4) * SPF)
+    r = oracle.decode(np.ascontiguousarray(g[start * SPF:]), 28.0, 64, trace=True)
+    c = r["chunks"][0]
+    assert c["nf"] >= 3
+    for f in range(c["nf"]):
+        n = c["vp_count"][f]
+        ty = [int(x) for x in c["vp_types"][f][:n]]
+        st = c["vp_starts"][f][:n]
+        i = ty.index(1)                               # first EQ1
+        assert ty[i:i + 18] == [1] * 6 + [2] * 6 + [3] * 6, ty[:30]
+        h = i + 18 + ty[i + 18:].index(0)             # first HSYNC after the interval
+        assert c["ll_coarse"][f][10] == st[h]
+        assert c["is_first_k2b"][f] == int((start + f) % 2 == 0)

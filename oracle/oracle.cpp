@@ -22,10 +22,11 @@
 // offsets one field period apart), the chroma track trial (right track's
 // metric far below, choice follows the recorded field), the phase-ID cycle
 // (3, 2, 1, 4 field after field, across chunks), the vblank state machine,
-// first-HSYNC anchor and field order on clean vertical intervals.  Parity
-// unpinned: K2b on degraded vertical intervals (vote confidences, the
-// cross-field anchor estimate), and in chroma_decode the retry and the
-// first field's phase ID from the burst phase.
+// first-HSYNC anchor and field order on clean vertical intervals, the
+// chroma retry (track re-acquired after a field cut out of the tape).
+// Parity unpinned: K2b on degraded vertical intervals (vote confidences,
+// the cross-field anchor estimate), and in chroma_decode the first field's
+// phase ID from the burst phase.
 #include "oracle.h"
 
 #include <algorithm>

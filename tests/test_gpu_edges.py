@@ -290,6 +290,19 @@ def test_capture_starting_on_either_track(torch, N, dec, oracle, start):
     compare(outc, ref, rows=rows)
 
 
+def test_capture_with_a_dropped_field(torch, N, dec, oracle):
+    """One recorded field cut out of the tape: the chroma track flips at the
+    first field after the cut (the retry pass), on the GPU as in the oracle."""
+    g = N.generate(1, n_samples=14 * SPF)
+    cap = np.ascontiguousarray(np.concatenate([g[:6 * SPF], g[7 * SPF:]]))
+    ref = oracle.decode(cap, MHZ, 64, trace=True)
+    assert ref["fields"] == 13
+    rows = rows_inside(ref, cap.size)
+    out, outc = decode_both(dec, torch, cap)
+    compare(out, ref, rows=rows)
+    compare(outc, ref, rows=rows)
+
+
 def test_output_capacity_too_small(torch, N, dec, cap0):
     d = torch.from_numpy(cap0).to("cuda:0")
     dec.reset()

@@ -101,3 +101,24 @@ def test_k2b_vblank_anchor_and_field_order(oracle, start):
         h = i + 18 + ty[i + 18:].index(0)             # first HSYNC after the interval
         assert c["ll_coarse"][f][10] == st[h]
         assert c["is_first_k2b"][f] == int((start + f) % 2 == 0)
+
+
+def test_chroma_track_reacquired_after_a_dropped_field(oracle):
+    """A capture with one recorded field cut out (samples 6 spf to 7 spf of
+    the tape) breaks the head alternation: the field after the cut is read
+    by the same head as the one before it.  The per-sub-chunk retry
+    (chroma_decode.cu:774-801, 900-975) flips the track at the first bad
+    field, so each recorded field after the cut decodes as it does without
+    the cut: the same fileLoc (less one field period), identical luma, and
+    chroma within 1 LSB on over 98% of the pixels (the circular pre-band-pass
+    of the chunk's segments sees different neighbours at segment edges).  On
+    the wrong track every other line's chroma would be inverted."""
+    from cuvhs_amd import native
+    g = native.generate(1, n_samples=14 * SPF)
+    plain = oracle.decode(np.ascontiguousarray(g[:13 * SPF]), 28.0, 64, trace=False)
+    cut = oracle.decode(np.ascontiguousarray(np.concatenate([g[:6 * SPF], g[7 * SPF:]])), 28.0, 64, trace=False)
+    for j in range(6, 11):   # recorded field j + 1 (the last field is cut by the capture end)
+        assert int(cut["file_loc"][j]) + SPF == int(plain["file_loc"][j + 1])
+        assert np.array_equal(cut["luma"][j], plain["luma"][j + 1]) or cut["do_count"][j] != plain["do_count"][j + 1]
+        d = np.abs(cut["chroma"][j].astype(np.int64) - plain["chroma"][j + 1].astype(np.int64))
+        assert (d > 1).mean() < 0.02, (j, float((d > 1).mean()))

@@ -1,11 +1,18 @@
-"""Capture lengths and contents at the edges of the chunk loop
-(pipeline.cu:2626-2713 sizes each chunk from the whole fields left; a read
-shorter than one field ends the decode, pipeline.cu:1535): empty, one sample
-short of a field, exactly one field, one field and a fraction, two and a
-half fields, and a silent capture that takes the nominal-spacing fallback
-(pipeline.cu:1614-1626).  Each is decoded by the whole-capture entry point
-and the per-chunk entry point and compared with the oracle; the CLI writes
-the same files as the oracle's file decode.
+"""Captures at the edges of the chunk loop and of the decoder's soft paths,
+each decoded by the whole-capture and the per-chunk entry points and
+compared with the oracle:
+
+* lengths (pipeline.cu:2626-2713 sizes each chunk from the whole fields
+  left; a read shorter than one field ends the decode, pipeline.cu:1535):
+  empty, one sample short of a field, exactly one field, one field and a
+  fraction, two and a half fields, also at 40 MHz and for s16 / u16 input;
+* a silent capture (the nominal-spacing fallback, pipeline.cu:1614-1626);
+* a VSYNC candidate overflow, and a capture with nothing in the chroma band;
+* RF dips placed inside and outside the dropout scan windows;
+* captures starting on either chroma track, and one with a recorded field
+  cut out (the track retry);
+* an output buffer too small, and the CLI on short and silent files, from a
+  file and from stdin.
 
 A field that reaches past the end of the capture demodulates zeros there
 (the chunk buffer is zero past the read, pipeline.cu:1539-1541): the

@@ -32,7 +32,7 @@ struct K1Design {
     BlockFilter video;            // real video kernel + i * real sync kernel
 };
 
-// fm_demod.cu:326-395 with spf = spfp (pipeline.cu:1455).
+// fm_demod.cu:273-374 with spf = spfp (pipeline.cu:1455).
 // eps: relative tap magnitude below which the kernel is truncated.
 // elem_bytes: bytes per complex LDS element of the transform (16 FP64, 8 FP32).
 K1Design design_k1(const Fmt& fmt, double eps, const int* b_choices, int nb_choices, int extra_a, int elem_bytes);

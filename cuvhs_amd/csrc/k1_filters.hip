@@ -1,14 +1,14 @@
 // K1 FM demodulation and the chroma capture-rate pre-band-pass as blocked
 // overlap-save convolutions in LDS.
 //
-// Reference: fm_demod.cu:449-539 (whole-segment D2Z -> filter -> Z2Z/Z2D over
+// Reference: fm_demod.cu:396-486 (whole-segment D2Z -> filter -> Z2Z/Z2D over
 // N = next_7smooth(spfp + 2048) points per field segment) and
 // chroma_decode.cu:594-633 (per-segment circular pre-band-pass over
 // N2 = next_7smooth(spfp)).
 //
 // Equivalence: the reference computes, for segment f and window position q,
 //   y_f(q) = sum_{m} g_N[m] * x_f((q - m) mod N)
-// with x_f(p) = src[clamp(f*spfp - 2048 + p, 0, T-1)] (fm_demod.cu:186-206).
+// with x_f(p) = src[clamp(f*spfp - 2048 + p, 0, T-1)] (fm_demod.cu:133-153).
 // Each workgroup here computes V consecutive outputs of one segment as a
 // B-point circular convolution of B loaded inputs with the taps of g_N kept
 // in [-lm, lp]; the loaded input indices wrap modulo N exactly like the

@@ -93,7 +93,7 @@ typedef struct cuvhs_format {
 } cuvhs_format;
 
 /* Create/destroy.  Replaces Pipeline::allocate_buffers + FMDemodState::init
- * (pipeline.cu:1383-1475, fm_demod.cu:326-427). */
+ * (pipeline.cu:1383-1475, fm_demod.cu:273-374). */
 int cuvhs_create(const cuvhs_config* cfg, cuvhs_ctx** out);
 int cuvhs_destroy(cuvhs_ctx* ctx);
 int cuvhs_get_format(const cuvhs_ctx* ctx, cuvhs_format* out);
@@ -190,7 +190,7 @@ int cuvhs_dropouts(cuvhs_ctx* ctx, const double* d_env, const double* d_linelocs
                    uint16_t* d_chroma, int* d_lines, int* d_starts, int* d_ends, int* d_count);
 
 /* Self-test of the convolution blocks the decoder runs (no counterpart in
- * the reference, whose transforms are cuFFT's, fm_demod.cu:449-539):
+ * the reference, whose transforms are cuFFT's, fm_demod.cu:396-486):
  * d_y = B * IFFT(FFT(d_x) * d_g) for one B-point block of complex values
  * (double2, or float2 with is_f32), d_g in natural bin order.  B = 4096
  * (K1 blocks, FP64 or FP32), 8192 (FP64 or FP32), 16384 (FP32, the chroma
@@ -221,7 +221,7 @@ int cuvhs_profile_work(const cuvhs_ctx* ctx, uint64_t* samples, int* chunks, int
 /* K1's block shapes as designed for this context: block size (points) and
  * valid outputs per block of part A (analytic) and part B (video and sync
  * filters).  Any pointer may be NULL.  (No counterpart in the reference,
- * which transforms whole segments, fm_demod.cu:449-539.) */
+ * which transforms whole segments, fm_demod.cu:396-486.) */
 int cuvhs_k1_blocks(const cuvhs_ctx* ctx, int* block_a, int* valid_a, int* block_b, int* valid_b);
 
 /* The reference's debug-log counters (CUVHS_DEBUG_K3_LOG, _K4_LOG, _K5_LOG:

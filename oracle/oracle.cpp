@@ -14,19 +14,18 @@
 //     (reference: parallel reductions in an unspecified order).
 //
 // How each part is pinned (tests/): format, convert and writer against the
-// reference's own host sources built in oracle/_ref; FFT against numpy and
-// brute-force DFTs; filter design against scipy.signal; demod, VSYNC, K2,
+// reference's own host sources built in oracle/_ref; K2b line location
+// (pipeline.cu:133-148, 410-893: validated pulses, good flags, anchors, line
+// locations, is_first) and the chroma burst metric, burst phase and NTSC
+// phase table (chroma_decode.cu:329-450) bit for bit against the reference's
+// own host functions, compiled as they lie (oracle/ref_k2b_driver.cpp), on
+// seven degraded captures (tests/test_reference_k2b.py); FFT against numpy
+// and brute-force DFTs; filter design against scipy.signal; demod, VSYNC, K2,
 // K4, K5 by closed forms and brute force; K7 by RF dips placed at known
-// samples.  Pinned by invariants only: k2b_chunk (clean-input line
-// locations on pulse starts, fallback spacing), the VSYNC clustering (field
-// offsets one field period apart), the chroma track trial (right track's
-// metric far below, choice follows the recorded field), the phase-ID cycle
-// (3, 2, 1, 4 field after field, across chunks), the vblank state machine,
-// first-HSYNC anchor and field order on clean vertical intervals, the
-// chroma retry (track re-acquired after a field cut out of the tape).
-// Parity unpinned: K2b on degraded vertical intervals (vote confidences,
-// the cross-field anchor estimate), and in chroma_decode the first field's
-// phase ID from the burst phase.
+// samples.  Pinned by invariants: the VSYNC clustering (field offsets one
+// field period apart), the phase-ID cycle (3, 2, 1, 4 field after field,
+// across chunks), the chroma retry (track re-acquired after a field cut out
+// of the tape).
 #include "oracle.h"
 
 #include <algorithm>
@@ -200,7 +199,7 @@ struct K1Filters {
     std::vector<cd> fv, fv05;
 };
 
-// FMDemodState::init (fm_demod.cu:326-395), with spf = spfp (pipeline.cu:1455)
+// FMDemodState::init (fm_demod.cu:273-374), with spf = spfp (pipeline.cu:1455)
 K1Filters make_k1_filters(const Fmt& fmt) {
     K1Filters K;
     K.N = orc::next_7smooth(fmt.spfp + OVERLAP);
@@ -258,7 +257,7 @@ std::vector<double> make_chroma_prebpf(const Fmt& fmt, int field_stride, int* n_
 }
 
 // ---------------------------------------------------------------------------
-// K1 fm_demod (fm_demod.cu:449-539 with kernels :186-310)
+// K1 fm_demod (fm_demod.cu:396-486 with kernels :133-257)
 // raw: nfields*spfp samples; outputs likewise
 // ---------------------------------------------------------------------------
 void fm_demod(const Fmt& fmt, const K1Filters& K, const double* raw, int nf, double* demod,
@@ -270,7 +269,7 @@ void fm_demod(const Fmt& fmt, const K1Filters& K, const double* raw, int nf, dou
     const double scale = fmt.sample_rate / TAU;
     std::vector<double> rawdemod((size_t)total_src);
 
-    // k_copy_and_pad (fm_demod.cu:186-206)
+    // k_copy_and_pad (fm_demod.cu:133-153)
     auto window = [&](const double* src, int f, std::vector<double>& w) {
         for (int p = 0; p < N; p++) {
             long long s = (long long)f * spf + p - OVERLAP;
@@ -779,8 +778,12 @@ struct K2bState {
 };
 
 // K2b driver (pipeline.cu:1696-1791)
+// good (optional): the validated pulses' good flags [nf][800]; anchor
+// (optional): per field valid, line0loc, first_hsync_loc, first_hsync_line,
+// next_field, first_field, progressive_field, prev_hsync_diff, meanlinelen.
 void k2b_chunk(const Fmt& fmt, K2bState& st, size_t raw_offset, const std::vector<size_t>& offs, int nf,
-               int* starts, int* lens, int* count, int* types, double* linelocs, int* is_first) {
+               int* starts, int* lens, int* count, int* types, double* linelocs, int* is_first,
+               unsigned char* good = nullptr, double* anchor = nullptr) {
     const int LPF = fmt.lines_per_frame;
     for (int field = 0; field < nf; field++) {
         int before = std::max(0, std::min(count[field], MAX_PULSES));
@@ -799,11 +802,25 @@ void k2b_chunk(const Fmt& fmt, K2bState& st, size_t raw_offset, const std::vecto
             st_f[i] = vp[(size_t)i].pulse.start;
             ln_f[i] = vp[(size_t)i].pulse.len;
             types[(size_t)field * MAX_PULSES + i] = vp[(size_t)i].type;
+            if (good) good[(size_t)field * MAX_PULSES + i] = vp[(size_t)i].good ? 1 : 0;
         }
         for (int i = after; i < MAX_PULSES; i++) {
             st_f[i] = 0;
             ln_f[i] = 0;
             types[(size_t)field * MAX_PULSES + i] = PULSE_HSYNC;
+            if (good) good[(size_t)field * MAX_PULSES + i] = 0;
+        }
+        if (anchor) {
+            double* a = anchor + (size_t)field * 9;
+            a[0] = an.valid ? 1.0 : 0.0;
+            a[1] = an.line0loc;
+            a[2] = an.first_hsync_loc;
+            a[3] = an.first_hsync_line;
+            a[4] = an.next_field;
+            a[5] = (double)an.first_field;
+            a[6] = (double)an.progressive_field;
+            a[7] = an.prev_hsync_diff;
+            a[8] = mll;
         }
         count[field] = after;
         double* ll = linelocs + (size_t)field * LPF;
@@ -968,6 +985,8 @@ struct ChromaState {
     double good_metric_threshold = 0.0;
     int cycle_start = 0;
     double trial_metric[2] = {0.0, 0.0};   // field-0 trial of the first chunk, per track (test access)
+    bool keep_trial = false;                // keep the trial's band-passed lines (test access)
+    std::vector<double> trial_buf[3];       // track 0, track 1, detected track (re-run for the phase)
 };
 
 struct NTSCPhaseEntry { int is_first, quadrant, delta, phase_id, offset; };
@@ -1140,12 +1159,14 @@ void chroma_decode(const Fmt& fmt, const double* raw, const double* linelocs, do
             tr[0] = t;
             chroma_lines(fmt, cc, plan, craw, linelocs, total, 0, 1, tr, buf);
             metric[t] = burst_metric(buf.data(), fmt, cc);
+            if (state && state->keep_trial) state->trial_buf[t] = buf;
         }
         detected_track = (metric[1] < metric[0]) ? 1 : 0;
         thr = metric[detected_track] * 4.0;
         if (state) { state->trial_metric[0] = metric[0]; state->trial_metric[1] = metric[1]; }
         tr[0] = detected_track;
         chroma_lines(fmt, cc, plan, craw, linelocs, total, 0, 1, tr, buf);
+        if (state && state->keep_trial) state->trial_buf[2] = buf;
         double ph0 = burst_phase(buf.data(), fmt, cc);
         int q0 = ((int)(ph0 / 90.0 + 0.5)) % 4;
         int pid0 = lookup_ntsc_phase_id(1, q0, -1);
@@ -1879,6 +1900,7 @@ or_dec* or_create(double mhz, int batch, int record_trace) {
     d->fmt = make_fmt(mhz);
     d->batch = batch;
     d->trace = record_trace != 0;
+    d->chroma.keep_trial = d->trace;
     d->k1 = make_k1_filters(d->fmt);
     size_t n = (size_t)batch * d->fmt.spfp;
     // one segment of zero padding: a K2 window may start in the last segment
@@ -1987,6 +2009,48 @@ int or_chroma_trial(const or_dec* d, double* metrics, int* track) {
     metrics[1] = d->chroma.trial_metric[1];
     *track = metrics[1] < metrics[0] ? 1 : 0;
     return 0;
+}
+
+int or_chroma_trial_lines(const or_dec* d, int which, double* out) {
+    if (which < 0 || which > 2) return -1;
+    const std::vector<double>& b = d->chroma.trial_buf[which];
+    if (b.empty()) return -1;
+    if (out) std::copy(b.begin(), b.end(), out);
+    return (int)b.size();
+}
+
+int or_k2b_chunk(double mhz, or_k2b_state* st, uint64_t raw_offset, const uint64_t* offsets, int nf, int* starts,
+                 int* lens, int* count, int* types, unsigned char* good, double* linelocs, int* is_first,
+                 double* anchor) {
+    if (!st || nf < 0 || !(mhz > 0.0)) return -1;
+    Fmt fmt = make_fmt(mhz);
+    K2bState k;
+    k.prev_first_field = st->prev_first_field;
+    k.prev_progressive_field = st->prev_progressive_field;
+    k.prev_first_hsync_readloc = st->prev_first_hsync_readloc;
+    k.prev_first_hsync_loc = st->prev_first_hsync_loc;
+    k.prev_first_hsync_diff = st->prev_first_hsync_diff;
+    std::vector<size_t> offs((size_t)nf);
+    for (int i = 0; i < nf; i++) offs[(size_t)i] = (size_t)offsets[i];
+    k2b_chunk(fmt, k, (size_t)raw_offset, offs, nf, starts, lens, count, types, linelocs, is_first, good, anchor);
+    st->prev_first_field = k.prev_first_field;
+    st->prev_progressive_field = k.prev_progressive_field;
+    st->prev_first_hsync_readloc = k.prev_first_hsync_readloc;
+    st->prev_first_hsync_loc = k.prev_first_hsync_loc;
+    st->prev_first_hsync_diff = k.prev_first_hsync_diff;
+    return 0;
+}
+
+int or_lookup_ntsc_phase(int is_first, int quadrant, int delta) { return lookup_ntsc_phase_id(is_first, quadrant, delta); }
+
+double or_burst_metric(double mhz, const double* lines) {
+    Fmt fmt = make_fmt(mhz);
+    return burst_metric(lines, fmt, chroma_consts(fmt));
+}
+
+double or_burst_phase(double mhz, const double* lines) {
+    Fmt fmt = make_fmt(mhz);
+    return burst_phase(lines, fmt, chroma_consts(fmt));
 }
 
 int or_trace_candidates(const or_dec* d, int c) {

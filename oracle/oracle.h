@@ -10,11 +10,13 @@
 // Parity pins (see tests/): FFT vs numpy.fft + brute-force DFT; filter
 // design vs scipy.signal; VideoFormat known answers; u8 conversion
 // exhaustive; K2/VSYNC vs brute-force run scans; K4 vs closed-form linear
-// edges; K5 vs closed-form Catmull-Rom on polynomials; JSON vs golden text.
-// Parity unpinned (restatement only, reviewed line by line against the
-// reference): K2b line location (pipeline.cu:410-893), the chunk/VSYNC
-// clustering heuristics (pipeline.cu:1595-1651), chroma track detection,
-// retry and phase-ID selection (chroma_decode.cu:674-1001).
+// edges; K5 vs closed-form Catmull-Rom on polynomials; JSON vs golden text;
+// K2b line location (pipeline.cu:133-148, 410-893) and the chroma burst
+// metrics, burst phase and NTSC phase table (chroma_decode.cu:329-450) bit
+// for bit against the reference's own host code compiled from its sources
+// (oracle/ref_k2b_driver.cpp, vectors in tests/golden/ref_*.npz).  Pinned
+// by invariants: the chunk/VSYNC clustering (pipeline.cu:1595-1651), the
+// chroma retry (chroma_decode.cu:774-975).
 #pragma once
 #include <stddef.h>
 #include <stdint.h>
@@ -122,6 +124,36 @@ int or_trace_candidates(const or_dec* d, int c);
 /* burst metrics of the field-0 track trial (track 0, track 1) and the track
  * chosen; -1 before any chroma was decoded */
 int or_chroma_trial(const or_dec* d, double* metrics, int* track);
+
+/* The field-0 track trial of the first chunk's chroma (record_trace=1):
+ * which = 0 / 1: band-passed lines [263][1024] (unnormalised inverse
+ * transforms) with track 0 / 1; 2: re-run with the detected track, from
+ * which the burst phase is measured (chroma_decode.cu:688-731).  Returns the
+ * element count, -1 if none was kept. */
+int or_chroma_trial_lines(const or_dec* d, int which, double* out);
+
+/* K2b on one chunk's K2 pulses, stand-alone (pipeline.cu:1717-1778), with
+ * the cross-field cadence state (pipeline.h:92-97) passed in and out.
+ * starts/lens/count ([nf][800]) are replaced by the validated pulses; good
+ * [nf][800] gets their good flags, anchor [nf][9] per field: valid,
+ * line0loc, first_hsync_loc, first_hsync_line, next_field, first_field,
+ * progressive_field, prev_hsync_diff, meanlinelen (good, anchor optional). */
+typedef struct or_k2b_state {
+    int prev_first_field;
+    int prev_progressive_field;
+    long long prev_first_hsync_readloc;
+    double prev_first_hsync_loc;
+    double prev_first_hsync_diff;
+} or_k2b_state;
+int or_k2b_chunk(double mhz, or_k2b_state* st, uint64_t raw_offset, const uint64_t* offsets, int nf, int* starts,
+                 int* lens, int* count, int* types, unsigned char* good, double* linelocs, int* is_first,
+                 double* anchor);
+/* lookup_ntsc_phase (chroma_decode.cu:359-381): the phase ID */
+int or_lookup_ntsc_phase(int is_first, int quadrant, int delta);
+/* measure_burst_cancellation / measure_burst_phase (chroma_decode.cu:383-450)
+ * on one field's band-passed lines [263][1024] */
+double or_burst_metric(double mhz, const double* lines);
+double or_burst_phase(double mhz, const double* lines);
 
 /* .tbc.json from explicit metadata (do_* arrays concatenated over fields) */
 int or_write_json_meta(double mhz, const char* path, int n, const uint64_t* file_loc, const int* is_first,

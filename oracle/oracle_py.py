@@ -63,6 +63,13 @@ def lib():
         L.or_chunk_diagnostics.argtypes = [_vp, ctypes.c_int, _vp]
         L.or_diag_counts.argtypes = [ctypes.c_double, ctypes.c_int, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int, _vp]
         L.or_k3_linelocs.argtypes = [ctypes.c_double, _vp, _vp, _vp, ctypes.c_int, _vp, _vp, _vp]
+        L.or_chroma_trial_lines.argtypes = [_vp, ctypes.c_int, _vp]
+        L.or_k2b_chunk.argtypes = [ctypes.c_double, _vp, ctypes.c_uint64, _vp, ctypes.c_int] + [_vp] * 8
+        L.or_lookup_ntsc_phase.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.or_burst_metric.argtypes = [ctypes.c_double, _vp]
+        L.or_burst_metric.restype = ctypes.c_double
+        L.or_burst_phase.argtypes = [ctypes.c_double, _vp]
+        L.or_burst_phase.restype = ctypes.c_double
         _L = L
     return _L
 
@@ -131,6 +138,14 @@ def decode(data, mhz, batch, trace=True, dc_correct=False, line_locator=0, diagn
         tk = np.zeros(1, np.int32)
         if L.or_chroma_trial(d, m.ctypes.data, tk.ctypes.data) == 0:
             out["track_trial"] = {"metrics": m, "track": int(tk[0])}
+            if trace:
+                lines = []
+                for which in range(3):
+                    buf = np.zeros(263 * 1024)
+                    if L.or_chroma_trial_lines(d, which, buf.ctypes.data) == buf.size:
+                        lines.append(buf.reshape(263, 1024))
+                if len(lines) == 3:
+                    out["track_trial"]["lines"] = lines
         if diagnostics:
             diag = []
             for c in range(L.or_num_chunks(d) if trace else 10 ** 6):
@@ -193,3 +208,98 @@ def diag_counts(which, mhz, a, b=None, c=None, nf=1):
     if rc != 0:
         raise ValueError("bad counter set")
     return tuple(int(x) for x in out)
+
+
+class K2bState(ctypes.Structure):
+    """Cross-field K2b cadence state (pipeline.h:92-97), initial values of
+    Pipeline's members."""
+    _fields_ = [("prev_first_field", ctypes.c_int), ("prev_progressive_field", ctypes.c_int),
+                ("prev_first_hsync_readloc", ctypes.c_longlong), ("prev_first_hsync_loc", ctypes.c_double),
+                ("prev_first_hsync_diff", ctypes.c_double)]
+
+    @classmethod
+    def initial(cls):
+        return cls(-1, -1, -1, -1.0, -1.0)
+
+
+def k2b_chunk_with(fn, mhz, state, raw_offset, offsets, count, starts, lens):
+    """Run one chunk of K2b through fn (or_k2b_chunk, or the reference's
+    ref_k2b_chunk with the same signature) on K2 pulses count[nf],
+    starts/lens[nf][<=800]; returns the validated pulses, types, good flags,
+    line locations, is_first and the anchor values per field."""
+    nf = len(offsets)
+    st = np.zeros((nf, MAX_PULSES), np.int32)
+    ln = np.zeros((nf, MAX_PULSES), np.int32)
+    st[:, :starts.shape[1]] = starts
+    ln[:, :lens.shape[1]] = lens
+    cnt = np.ascontiguousarray(count, np.int32).copy()
+    offs = np.ascontiguousarray(offsets, np.uint64)
+    types = np.zeros((nf, MAX_PULSES), np.int32)
+    good = np.zeros((nf, MAX_PULSES), np.uint8)
+    ll = np.zeros((nf, LPF))
+    isf = np.zeros(nf, np.int32)
+    anchor = np.zeros((nf, 9))
+    rc = fn(ctypes.c_double(float(mhz)), ctypes.byref(state), ctypes.c_uint64(int(raw_offset)), offs.ctypes.data,
+            nf, st.ctypes.data, ln.ctypes.data, cnt.ctypes.data, types.ctypes.data, good.ctypes.data,
+            ll.ctypes.data, isf.ctypes.data, anchor.ctypes.data)
+    if rc != 0:
+        raise RuntimeError(f"k2b chunk failed ({rc})")
+    return dict(count=cnt, starts=st, lens=ln, types=types, good=good, linelocs=ll, is_first=isf, anchor=anchor)
+
+
+def k2b_chunk(mhz, state, raw_offset, offsets, count, starts, lens):
+    """The oracle's K2b (pipeline.cu:1717-1778 restated) on one chunk."""
+    return k2b_chunk_with(lib().or_k2b_chunk, mhz, state, raw_offset, offsets, count, starts, lens)
+
+
+def lookup_ntsc_phase(is_first, quadrant, delta):
+    return lib().or_lookup_ntsc_phase(int(is_first), int(quadrant), int(delta))
+
+
+def burst_metric(mhz, lines):
+    lines = np.ascontiguousarray(lines, np.float64)
+    return lib().or_burst_metric(float(mhz), lines.ctypes.data)
+
+
+def burst_phase(mhz, lines):
+    lines = np.ascontiguousarray(lines, np.float64)
+    return lib().or_burst_phase(float(mhz), lines.ctypes.data)
+
+
+_R = None
+
+
+def ref_k2b_lib():
+    """The reference's own host K2b and chroma helpers, compiled by the
+    Makefile target `refk2b` into oracle/_ref/libref_k2b.so when the reference
+    sources are mounted; None when that library is absent."""
+    global _R
+    if _R is None:
+        path = os.path.join(HERE, "_ref", "libref_k2b.so")
+        if not os.path.exists(path):
+            return None
+        R = ctypes.CDLL(path)
+        R.ref_k2b_chunk.argtypes = [ctypes.c_double, _vp, ctypes.c_uint64, _vp, ctypes.c_int] + [_vp] * 8
+        R.ref_lookup_ntsc_phase.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp]
+        R.ref_burst_cancellation.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                             ctypes.c_int, ctypes.c_double]
+        R.ref_burst_cancellation.restype = ctypes.c_double
+        R.ref_burst_phase.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                      ctypes.c_double, ctypes.c_double, ctypes.c_double]
+        R.ref_burst_phase.restype = ctypes.c_double
+        R.ref_chroma_params.argtypes = [ctypes.c_double, _vp]
+        _R = R
+    return _R
+
+
+def ref_chroma_trial(R, mhz, lines):
+    """The reference's measure_burst_cancellation on the track-0 and track-1
+    trial lines and measure_burst_phase on the detected-track lines
+    (chroma_decode.cu:688-731), with its own format constants."""
+    p = np.zeros(6)
+    R.ref_chroma_params(float(mhz), p.ctypes.data)
+    fsc, rate, L, rows, bs, be = p
+    args = (int(rows), 1024, int(L), int(bs), int(be), 1.0 / 1024)
+    m = [R.ref_burst_cancellation(np.ascontiguousarray(lines[t]).ctypes.data, *args) for t in (0, 1)]
+    ph = R.ref_burst_phase(np.ascontiguousarray(lines[2]).ctypes.data, *args, fsc, rate)
+    return m[0], m[1], ph

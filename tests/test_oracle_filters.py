@@ -1,4 +1,4 @@
-"""Oracle filter design (fm_demod.cu:326-395, chroma_decode.cu:568-587)
+"""Oracle filter design (fm_demod.cu:273-374, chroma_decode.cu:568-587)
 against scipy.signal and closed-form invariants."""
 import os
 

@@ -19,13 +19,6 @@ import os
 import sys
 import time
 
-# Hardware queues for this process (read when HIP initialises, before torch
-# touches the GPU).  The decoder runs four streams besides torch's; with
-# HIP's default of four queues two of them share one, and a context then
-# keeps its forked tail work on the head stream (cuvhs_create reads the
-# same variable).  Eight gives every stream its own queue.
-os.environ["GPU_MAX_HW_QUEUES"] = "8"
-
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -33,16 +26,36 @@ METRIC = "NTSC fields/s at 28 MHz (MFMA mixed vs plain FP32, 1 GPU) + achieved H
 PEAK_HBM_GBS = 8000.0
 PEAK_FP64_TFLOPS = 78.6   # MI355X FP64, vector and matrix
 
-# Algorithmic bytes per unit of each stage (the stage's compulsory reads and
-# writes of its own inputs/outputs, counted once).
-STAGE_BYTES = {
-    # per capture sample: u8 in, analytic envelope (f64) + instantaneous frequency (f64) out
-    "k1_analytic": ("sample", 1 + 8 + 8),
-    # per capture sample: instantaneous frequency in, demod + demod_05 out (f64)
-    "k1_video": ("sample", 8 + 8 + 8),
-    # per capture sample: u8 in, band-passed f32 out
-    "k6_prebpf": ("sample", 1 + 4),
-}
+LINE_RATE = 15734.264      # NTSC line rate (video_format.cpp)
+TBC_BYTES = 2 * 2 * 910 * 263  # luma + chroma u16 per field
+
+# The stages a roofline line can be taken on (their stage names in the
+# decoder's profile); the dominant one is the largest serialised time.
+ROOFLINE_STAGES = ("k1_analytic", "k1_video", "k6_prebpf", "k6_chroma")
+
+
+# Compulsory HBM bytes of one field at the benchmark's sample rates: its
+# unique u8 input (one field period of samples) plus the two u16 TBC planes,
+# as the benchmark's measurement plan tabulates them.
+ALG_BYTES_PER_FIELD = {28.0: 1424455, 35.8: 1553968, 40.0: 1624655}
+
+
+def alg_bytes_per_field(mhz):
+    """Compulsory bytes of one field (the table above; for other rates one
+    field period of u8 samples, fs * 262.5 / line rate, plus the planes)."""
+    return ALG_BYTES_PER_FIELD.get(round(mhz, 1), round(mhz * 1e6 * 262.5 / LINE_RATE) + TBC_BYTES)
+
+
+def fft_flops(n):
+    """5 N log2 N for a complex transform of N points."""
+    return 5.0 * n * math.log2(n)
+
+
+def k1_fft_flops_per_segment(stage, n):
+    """The reference's K1 transforms of one segment in the FFT formulation
+    (a real transform counted as half a complex one): part A one D2Z and one
+    Z2Z, part B one D2Z and two Z2D (fm_demod.cu:396-486)."""
+    return (0.5 + 1.0) * fft_flops(n) if stage == "k1_analytic" else (0.5 + 0.5 + 0.5) * fft_flops(n)
 
 
 def log(*a):
@@ -59,17 +72,91 @@ WORKLOADS = {
 }
 
 
-def cpu_baseline(cap, fmt, batch, mhz):
+def _oracle():
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle_py
-    sample = cap[: 2 * batch * fmt.spfp]  # two chunks
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return oracle_py
+
+
+def cpu_decode(cap, mhz, batch, threads):
+    """One oracle decode of cap with the given OpenMP thread count: fields/s
+    and per-stage ms per field."""
+    O = _oracle()
+    used = O.set_threads(threads)
     t = time.perf_counter()
-    ref = oracle_py.decode(sample, mhz, batch, trace=False)
+    ref = O.decode(cap, mhz, batch, trace=False)
     dt = time.perf_counter() - t
-    return {"value": ref["fields"] / dt, "unit": "fields/s", "cores": threads, "kind": "port",
-            "sample": f"first {ref['fields']} fields of the same {mhz:g} MHz capture ({sample.size} u8 samples), "
-                      f"oracle liboracle.so (FP64, own FFT), OpenMP over fields, {dt:.1f} s"}
+    n = max(ref["fields"], 1)
+    return {"fields": ref["fields"], "seconds": round(dt, 2), "fields_per_s": round(ref["fields"] / dt, 3),
+            "threads": used, "stage_ms_per_field": {k: round(v * 1e3 / n, 2) for k, v in ref["stage_seconds"].items()}}
+
+
+def host_threads():
+    return int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+
+
+def cpu_baseline(cap, fmt, batch, mhz):
+    """The oracle (the reference's stage logic restated in C++, FP64) on the
+    host cores: the first two chunks of the workload with every host thread,
+    and the 4-field capture with one thread, each with per-stage ms/field."""
+    from cuvhs_amd import native
+    threads = host_threads()
+    sample = cap[: 2 * batch * fmt.spfp]
+    multi = cpu_decode(sample, mhz, batch, threads)
+    one = cpu_decode(native.generate(0), 28.0, batch, 1)
+    _oracle().set_threads(threads)
+    return {"value": multi["fields_per_s"], "unit": "fields/s", "cores": multi["threads"], "kind": "port",
+            "sample": f"first {multi['fields']} fields of the same {mhz:g} MHz capture ({sample.size} u8 samples), "
+                      f"oracle liboracle.so (FP64, own FFT), OpenMP over fields, {multi['seconds']} s",
+            "stage_ms_per_field": multi["stage_ms_per_field"],
+            "one_thread": {"workload": "4-field 28 MHz capture", **one}}
+
+
+def cpu_baseline_full(args):
+    """The full CPU baseline set: the 4-field and the 600-field 28 MHz
+    captures in full and the first 640 fields of the 40 MHz head-switch
+    capture, each with every host thread, and the 4-field capture with one
+    thread; one JSON line."""
+    from cuvhs_amd import native
+    threads = host_threads()
+    runs = {}
+    c0 = native.generate(0)
+    runs["4f_28mhz"] = cpu_decode(c0, 28.0, args.batch, threads)
+    runs["4f_28mhz_1thread"] = cpu_decode(c0, 28.0, args.batch, 1)
+    log(f"4 fields: {runs['4f_28mhz']}")
+    c1 = native.generate(1)
+    runs["600f_28mhz"] = cpu_decode(c1, 28.0, args.batch, threads)
+    log(f"600 fields: {runs['600f_28mhz']}")
+    del c1
+    c2 = native.generate(2, n_samples=640 * native.format_for(40.0).samples_per_field)
+    runs["640f_40mhz_headswitch"] = cpu_decode(c2, 40.0, args.batch, threads)
+    log(f"640 fields at 40 MHz: {runs['640f_40mhz_headswitch']}")
+    line = {"metric": "CPU baseline: oracle (reference stage logic, FP64) NTSC fields/s on the host cores",
+            "unit": "fields/s", "threads": threads, "kind": "port", "batch_fields": args.batch, "runs": runs}
+    print(json.dumps(line), flush=True)
+
+
+def cli_end_to_end(cap, mhz, batch):
+    """The `cuvhs` CLI on the capture written to a file: disk read, host to
+    device copies, decode, device to host copies and .tbc/.tbc.json writes."""
+    import subprocess
+    import tempfile
+    exe = os.path.join(ROOT, "cuvhs_amd", "lib", "cuvhs")
+    with tempfile.TemporaryDirectory() as tmp:
+        inp = os.path.join(tmp, "in.u8")
+        cap.tofile(inp)
+        out = os.path.join(tmp, "out")
+        runs = []
+        for _ in range(2):   # the first run pays for kernel loading
+            t = time.perf_counter()
+            r = subprocess.run([exe, "-f", f"{mhz:g}", "--batch", str(batch), "--overwrite", inp, out],
+                               stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)
+            runs.append(time.perf_counter() - t)
+            if r.returncode != 0:
+                return {"error": r.stderr[-300:]}
+        nf = json.load(open(out + ".tbc.json"))["videoParameters"]["numberOfSequentialFields"]
+    return {"fields": nf, "seconds": round(runs[-1], 3), "fields_per_s": round(nf / runs[-1], 1),
+            "includes": "process start, file read, PCIe copies, decode, file writes"}
 
 
 def serving_main(args):
@@ -122,6 +209,10 @@ def main():
     ap.add_argument("--serving", action="store_true",
                     help="instead: the serving loop (16 concurrent captures at 28/35.8/40 MHz, Poisson arrivals), "
                          "one JSON line with latency percentiles and aggregate fields/s per precision form")
+    ap.add_argument("--cpu-baseline-full", action="store_true",
+                    help="instead: the CPU baseline on the 4-field and 600-field 28 MHz captures and 640 fields of "
+                         "the 40 MHz capture with every host thread, and the 4-field capture with one thread")
+    ap.add_argument("--no-cli", action="store_true", help="skip the CLI end-to-end figure")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 4 plain device copies of MIB MiB (calibrates HBM byte counters) and exit")
     args = ap.parse_args()
@@ -129,7 +220,10 @@ def main():
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the decode path shards by "
               "capture segment, see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
-    import numpy as np
+    if args.cpu_baseline_full:
+        cpu_baseline_full(args)
+        return
+    import numpy as np  # noqa: F401
     import torch
     from cuvhs_amd import native
 
@@ -188,43 +282,52 @@ def main():
     dec.profile(False)
     ms_per_step = dt * 1000.0 / args.steps
     value = fields * args.steps / dt
-    bytes_per_field = cap.size / fields + 2 * 2 * 910 * 263  # u8 in + luma/chroma u16 out
+    bytes_per_field = alg_bytes_per_field(mhz)
     log(f"{fields} fields/step, {ms_per_step:.2f} ms/step, {value:.1f} fields/s")
     stage_ms = dict(prof)
     log("per-step stage ms (overlapped): " + ", ".join(f"{k}={v:.3f}" for k, v in stage_ms.items()))
     log("per-step stage ms (serialised): " + ", ".join(f"{k}={v:.3f}" for k, v in iso.items()))
 
-    # dominant kernel (largest serialised time) among those with a byte count
-    dom = max((k for k in STAGE_BYTES if k in iso), key=lambda k: iso[k])
-    unit, per_unit = STAGE_BYTES[dom]
+    # The dominant kernel (largest serialised stage time among the
+    # transform stages) against the HBM roof, in the units of the metric:
+    # the compulsory bytes of the fields one launch decodes (unique u8 input
+    # plus the luma and chroma planes, alg_bytes_per_field) over its average
+    # launch time, from HIP events on the stream it runs on.
+    dom = max((k for k in ROOFLINE_STAGES if k in iso), key=lambda k: iso[k])
     launches = chunks                      # one launch per chunk of the profiled decode
-    alg_bytes_per_launch = per_unit * k1_samples / launches
+    bpf = alg_bytes_per_field(mhz)
+    fields_per_launch = fields / launches
+    alg_bytes_per_launch = bpf * fields_per_launch
     avg_launch_s = stage_ms[dom] / 1000.0 / launches
-    achieved = alg_bytes_per_launch / avg_launch_s / 1e9
     iso_launch_s = iso[dom] / 1000.0 / launches
-    traffic = None
+    achieved = alg_bytes_per_launch / avg_launch_s / 1e9
+    traffic = traffic_ratio = None
     pmc_path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
     if os.path.exists(pmc_path) and args.workload == "600f_28mhz":  # counters taken on that workload
         with open(pmc_path) as f:
-            traffic = json.load(f).get("bytes_per_launch", {}).get(dom)
+            pmc = json.load(f)
+        traffic = pmc.get("bytes_per_launch", {}).get(dom)
+        if pmc.get("bytes_per_decode"):
+            traffic_ratio = round(pmc["bytes_per_decode"] / (bpf * fields), 2)
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                 "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
+                "alg_bytes_per_field": bpf, "fields_per_launch": round(fields_per_launch, 2),
                 "alg_bytes_per_launch": int(alg_bytes_per_launch), "avg_launch_ms": round(avg_launch_s * 1e3, 4),
                 "achieved_isolated": round(alg_bytes_per_launch / iso_launch_s / 1e9, 1),
-                "avg_launch_ms_isolated": round(iso_launch_s * 1e3, 4)}
+                "avg_launch_ms_isolated": round(iso_launch_s * 1e3, 4),
+                "decode_traffic_ratio": traffic_ratio}
 
-    # the same kernel against the FP64 arithmetic roof: its convolution blocks
-    # as algorithmic flops, two B-point complex FFTs (5 B log2 B each) and the
-    # B-point spectrum product (6 B) per block of V outputs, over the blocks
-    # of one launch (the epilogue's arctangent, square root and difference are
-    # not counted); peak 78.6 TFLOP/s FP64
+    # the same kernel against the FP64 arithmetic roof, in the reference's
+    # FFT formulation (5 N log2 N per complex transform of a segment's
+    # N-point window, half for a real one): part A one D2Z and one Z2Z, part
+    # B one D2Z and two Z2D per segment (fm_demod.cu:396-486)
     roofline_fp64 = None
     if dom in ("k1_analytic", "k1_video"):
-        ba, va, bb, vb = dec.k1_blocks()
-        B, V = (ba, va) if dom == "k1_analytic" else (bb, vb)
-        blocks = (k1_samples / fmt.spfp) * -(-fmt.spfp // V) / launches
-        flops = blocks * (2 * 5 * B * math.log2(B) + 6 * B)
-        roofline_fp64 = {"bound": "fp64", "kernel": dom, "block": B, "valid_per_block": V,
+        from cuvhs_amd import native as _n
+        segs = k1_samples / fmt.spfp / launches
+        n_fft = _n.k1_fft_size(mhz)
+        flops = segs * k1_fft_flops_per_segment(dom, n_fft)
+        roofline_fp64 = {"bound": "fp64", "kernel": dom, "fft_points": n_fft, "segments_per_launch": round(segs, 2),
                          "alg_flops_per_launch": int(flops), "unit": "TFLOP/s", "peak": PEAK_FP64_TFLOPS,
                          "achieved": round(flops / avg_launch_s / 1e12, 2),
                          "frac": round(flops / avg_launch_s / 1e12 / PEAK_FP64_TFLOPS, 4),
@@ -257,6 +360,11 @@ def main():
                           "stage_ms_per_step_isolated": {k: round(v, 3) for k, v in iso32.items()},
                           "vs_fp64": report}}
 
+    cli = None
+    if not args.no_cli:
+        cli = cli_end_to_end(cap, mhz, args.batch)
+        log(f"cli end to end: {cli}")
+
     cpu = None
     if not args.no_cpu_baseline:
         log("running CPU baseline ...")
@@ -278,13 +386,15 @@ def main():
         "data": "synthetic",
         "config": {"workload": wl["desc"],
                    "fields_per_step": fields, "batch_fields": args.batch, "sample_rate_mhz": mhz,
-                   "precision": "fp64 (chroma pre-band-pass fp32)"},
+                   "precision": "fp64 (chroma pre-band-pass fp32)",
+                   "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4"))},
         "hbm_gbs_algorithmic": round(value * bytes_per_field / 1e9, 2),
         "stage_ms_per_step": {k: round(v, 3) for k, v in stage_ms.items()},
         "stage_ms_per_step_isolated": {k: round(v, 3) for k, v in iso.items()},
         "roofline": roofline,
         "roofline_fp64": roofline_fp64,
         "cpu_baseline": cpu,
+        "cli_end_to_end": cli,
         "forms": forms,
     }
     print(json.dumps(line), flush=True)

@@ -56,7 +56,7 @@ void usage(const char* prog) {
             "  -f <MHz>              Sample rate in MHz (default: 28)\n"
             "  --format <u8|s16|u16> Input sample format (default: from extension)\n"
             "  --dc-correct          Subtract per-read DC offset after normalization\n"
-            "  --gpu <id>            GPU device ID (default: 0)\n"
+            "  --gpu <id>            GPU device ID (default: 0, -1 = most memory)\n"
             "  --batch <fields>      Fields per chunk (default: 64)\n"
             "  --precision <fp64|fp32> FM demodulation arithmetic (default: fp64)\n"
             "  --overwrite           Overwrite existing output files\n"

@@ -11,12 +11,16 @@
 // head's control block (next chunk offset, field count).
 #include <hip/hip_runtime.h>
 
+#include <sched.h>
+
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/cuvhs.h"
@@ -491,13 +495,20 @@ int sync_all(cuvhs_ctx* c) {
     return 0;
 }
 
-// Host wait for a slot's tail stage; collects its stage times and capacity flag.
 // Host wait for an event by polling: the decode loop's waits gate the next
 // launches, and a blocking wait's wake-up latency would leave the GPU idle
-// between the head of one chunk and the tail launched after it.
+// between the head of one chunk and the tail launched after it.  The poll
+// spins for the first 50 us, then yields the core between polls, and after
+// 2 ms sleeps 50 us between polls, so a long wait does not hold a host core.
 hipError_t spin_wait(hipEvent_t ev) {
+    using clk = std::chrono::steady_clock;
+    const auto t0 = clk::now();
     hipError_t e;
     while ((e = hipEventQuery(ev)) == hipErrorNotReady) {
+        const auto dt = clk::now() - t0;
+        if (dt < std::chrono::microseconds(50)) continue;
+        if (dt < std::chrono::milliseconds(2)) sched_yield();
+        else std::this_thread::sleep_for(std::chrono::microseconds(50));
     }
     return e;
 }
@@ -577,7 +588,21 @@ int cuvhs_create(const cuvhs_config* cfg, cuvhs_ctx** out) {
     const Fmt& f = c->fmt;
     if ((long long)c->F * f.spfp > 0x3fffffffLL) { delete c; return CUVHS_E_CAPACITY; }
     auto bad = [&](hipError_t e, const char* w) { fail(c, e, w); cuvhs_destroy(c); return CUVHS_E_HIP; };
-    hipError_t e = hipSetDevice(cfg->device);
+    if (cfg->device < 0) {
+        // GPUDevice::init(-1) (device.cu:13-24): the device with the most memory
+        int n = 0;
+        hipError_t ce = hipGetDeviceCount(&n);
+        if (ce != hipSuccess || n < 1) return bad(ce != hipSuccess ? ce : hipErrorNoDevice, "hipGetDeviceCount");
+        size_t best = 0;
+        for (int i = 0; i < n; i++) {
+            size_t mem = 0;
+            if (hipDeviceTotalMem(&mem, i) == hipSuccess && mem > best) {
+                best = mem;
+                c->cfg.device = i;
+            }
+        }
+    }
+    hipError_t e = hipSetDevice(c->cfg.device);
     if (e != hipSuccess) return bad(e, "hipSetDevice");
     if (cfg->stream) {
         c->stream = (hipStream_t)cfg->stream;

@@ -170,6 +170,24 @@ def _hip_d2d(dst, src, nbytes):
         raise CuvhsError(f"hipMemcpy failed ({rc})")
 
 
+def next_7smooth(n):
+    """Smallest 2^a 3^b 5^c 7^d >= n (next_fft_size, fm_demod.cu:22-33)."""
+    while True:
+        m = n
+        for p in (2, 3, 5, 7):
+            while m % p == 0:
+                m //= p
+        if m == 1:
+            return n
+        n += 1
+
+
+def k1_fft_size(mhz):
+    """The reference's K1 transform length: a padded field window plus the
+    2048-sample overlap, rounded up to 7-smooth (fm_demod.cu:273-280)."""
+    return next_7smooth(int(format_for(mhz).spfp) + 2048)
+
+
 def format_for(mhz):
     f = Format()
     rc = lib().cuvhs_format_for(float(mhz), ctypes.byref(f))
@@ -316,6 +334,16 @@ class Decoder:
     def reset(self):
         check(lib().cuvhs_reset(self.h), self.h, "reset")
 
+    def default_capacity(self, n):
+        """Output fields a capture of n samples can yield: one per field
+        period, plus one per chunk (a chunk whose VSYNC search fails advances
+        by F field periods less 10 lines, pipeline.cu:1614-1651), plus 1%
+        for a tape running fast, plus one."""
+        spf = int(self.fmt.samples_per_field)
+        step = max(1, self.batch * spf - 10 * int(self.fmt.samples_per_line))
+        per_period = n // spf
+        return int(per_period + -(-n // step) + per_period // 100 + 1)
+
     def decode_capture(self, d_capture, max_fields=None):
         """Decode a device-resident capture (torch uint8 / int16 / uint16
         tensor on the GPU, matching the decoder's input format).  Returns a
@@ -324,7 +352,7 @@ class Decoder:
         self._check_input(d_capture)
         n = d_capture.numel()
         if max_fields is None:
-            max_fields = int(n // self.fmt.samples_per_field) + 2
+            max_fields = self.default_capacity(n)
         dev = d_capture.device
         luma = torch.empty((max_fields, FIELD_LINES, LINE_LEN), dtype=torch.uint16, device=dev)
         chroma = torch.empty_like(luma)

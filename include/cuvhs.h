@@ -74,7 +74,9 @@ typedef struct cuvhs_config {
     int batch_fields;         /* F: fields per chunk; results depend on F */
     int precision;            /* CUVHS_PRECISION_* */
     void* stream;             /* hipStream_t, NULL = a stream owned by the context */
-    int device;               /* HIP device ordinal */
+    int device;               /* HIP device ordinal; -1 = the device with the
+                                 most memory (GPUDevice::init(-1),
+                                 device.cu:13-24) */
     int input_format;         /* CUVHS_INPUT_* (0 = u8) */
     int dc_correct;           /* subtract each chunk read's mean (--dc-correct,
                                  InputConditioning, raw_reader.h:21-23) */

@@ -29,6 +29,7 @@
 #include "oracle.h"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <complex>
 #include <cstdio>
@@ -37,6 +38,10 @@
 #include <vector>
 
 #include "fft.h"
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 using orc::cd;
 using orc::FFTPlan;
@@ -1335,6 +1340,21 @@ void dropout_detect(const Fmt& fmt, const double* env, const double* linelocs, u
 // ===========================================================================
 // Whole-capture decoder (pipeline.cu:1525-2713)
 // ===========================================================================
+// Wall time of each stage of a chunk, accumulated per decode (the CPU
+// baseline's per-stage figures): convert, K1, VSYNC + clustering, K2, K2b,
+// K4, K5, K6, K7.
+constexpr int kOrStages = 9;
+struct StageClock {
+    double* acc;
+    std::chrono::steady_clock::time_point t = std::chrono::steady_clock::now();
+    explicit StageClock(double* a) : acc(a) {}
+    void lap(int i) {
+        auto n = std::chrono::steady_clock::now();
+        acc[i] += std::chrono::duration<double>(n - t).count();
+        t = n;
+    }
+};
+
 struct ChunkTrace {
     size_t raw_offset = 0, next_offset = 0;
     int fields_k1 = 0, nf = 0;
@@ -1358,6 +1378,7 @@ struct or_dec {
     bool dc_correct = false;
     int line_locator = 0; // 0 K2b (default), 1 K3 (CUVHS_K2B_DISABLE)
     bool diag = false;    // collect the K3/K4/K5 diagnostic counters
+    double stage_s[kOrStages] = {};
     std::vector<std::vector<int>> diag_counts;  // per chunk: K3 x3, K4 x3, K5 x3
     // persistent device-like buffers
     std::vector<double> raw, demod, d05, env;
@@ -1678,8 +1699,10 @@ static int process_chunk(or_dec* d, const uint8_t* data, size_t total_n, size_t 
     if (raw_offset < total_n) n_read = std::min(total_samples, total_n - raw_offset);
     if (n_read < spf) return 0;
     double* h_raw = d->raw.data();
+    StageClock clk(d->stage_s);
     or_convert(d->in_fmt, d->dc_correct ? 1 : 0, data + (d->in_fmt == 0 ? 1 : 2) * raw_offset, n_read, h_raw);
     for (size_t i = n_read; i < total_samples; i++) h_raw[i] = 0.0;
+    clk.lap(0);
     int fields_loaded = (int)(n_read / spf);
     if (fields_loaded > num_fields) fields_loaded = num_fields;
     if (fields_loaded == 0) return 0;
@@ -1692,6 +1715,7 @@ static int process_chunk(or_dec* d, const uint8_t* data, size_t total_n, size_t 
     std::fill(d->d05.begin(), d->d05.end(), 0.0);
     std::fill(d->env.begin(), d->env.end(), 0.0);
     fm_demod(fmt, d->k1, h_raw, fields_loaded, d->demod.data(), d->d05.data(), d->env.data());
+    clk.lap(1);
 
     // VSYNC discovery + clustering (pipeline.cu:1567-1651)
     int total_chunk = fields_loaded * (int)spfp;
@@ -1726,11 +1750,13 @@ static int process_chunk(or_dec* d, const uint8_t* data, size_t total_n, size_t 
     size_t margin = 10 * (size_t)fmt.samples_per_line;
     next_raw_offset = predicted > margin ? raw_offset + predicted - margin : raw_offset + predicted;
     const int nf = fields_loaded;
+    clk.lap(2);
 
     // K2
     std::vector<int> ps((size_t)nf * MAX_PULSES, 0), pl((size_t)nf * MAX_PULSES, 0), pc((size_t)nf, 0);
     std::vector<int> pt((size_t)nf * MAX_PULSES, 0);
     sync_pulses(fmt, d->d05.data(), offs.data(), nf, ps.data(), pl.data(), pc.data());
+    clk.lap(3);
     if (d->trace) { tr.pc = pc; tr.ps = ps; tr.pl = pl; tr.offsets = offs; }
     // K2b, or K3 when selected (pipeline.cu:1694-1856)
     std::vector<double> ll((size_t)nf * fmt.lines_per_frame);
@@ -1748,6 +1774,7 @@ static int process_chunk(or_dec* d, const uint8_t* data, size_t total_n, size_t 
         }
     }
     if (d->trace) { tr.vc = pc; tr.vs = ps; tr.vl = pl; tr.vt = pt; tr.llc = ll; tr.is_first = is_first; }
+    clk.lap(4);
     std::vector<int> dg(9, 0);
     if (d->diag) k3_spacing_counts(fmt, ll.data(), nf, dg.data());
     // K4
@@ -1755,11 +1782,13 @@ static int process_chunk(or_dec* d, const uint8_t* data, size_t total_n, size_t 
     std::vector<double> coarse;
     if (d->diag) coarse = ll;
     hsync_refine(fmt, d->d05.data(), ll.data(), nf, total);
+    clk.lap(5);
     if (d->trace) tr.llr = ll;
     if (d->diag) k4_counts(fmt, d->d05.data(), total, coarse.data(), ll.data(), nf, dg.data() + 3);
     // K5
     std::vector<uint16_t> luma((size_t)nf * tbc), chroma((size_t)nf * tbc);
     tbc_resample(fmt, d->demod.data(), ll.data(), luma.data(), nf, total);
+    clk.lap(6);
     if (d->diag) {
         k5_counts(fmt, d->demod.data(), total, ll.data(), nf, dg.data() + 6);
         d->diag_counts.push_back(dg);
@@ -1767,10 +1796,12 @@ static int process_chunk(or_dec* d, const uint8_t* data, size_t total_n, size_t 
     // K6 (bandpassed raw overwrites demod, pipeline.cu:2531)
     std::vector<int> phase_ids;
     chroma_decode(fmt, h_raw, ll.data(), d->demod.data(), chroma.data(), nf, total, phase_ids, &d->chroma);
+    clk.lap(7);
     // K7
     std::vector<int> dl((size_t)nf * MAX_DO), dsx((size_t)nf * MAX_DO), dex((size_t)nf * MAX_DO), dc((size_t)nf);
     dropout_detect(fmt, d->env.data(), ll.data(), luma.data(), chroma.data(), dl.data(), dsx.data(), dex.data(),
                    dc.data(), nf);
+    clk.lap(8);
     // output assembly (pipeline.cu:2575-2610)
     for (int i = 0; i < nf; i++) {
         or_dec::Meta m;
@@ -2051,6 +2082,21 @@ double or_burst_metric(double mhz, const double* lines) {
 double or_burst_phase(double mhz, const double* lines) {
     Fmt fmt = make_fmt(mhz);
     return burst_phase(lines, fmt, chroma_consts(fmt));
+}
+
+int or_stage_seconds(const or_dec* d, double* out) {
+    for (int i = 0; i < kOrStages; i++) out[i] = d->stage_s[i];
+    return kOrStages;
+}
+
+int or_set_threads(int n) {
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
+#else
+    (void)n;
+    return 1;
+#endif
 }
 
 int or_trace_candidates(const or_dec* d, int c) {

@@ -155,6 +155,12 @@ int or_lookup_ntsc_phase(int is_first, int quadrant, int delta);
 double or_burst_metric(double mhz, const double* lines);
 double or_burst_phase(double mhz, const double* lines);
 
+/* Seconds spent per stage over the decodes of d (9 values: conversion, K1,
+ * VSYNC discovery + clustering, K2, K2b or K3, K4, K5, K6, K7); returns 9. */
+int or_stage_seconds(const or_dec* d, double* out);
+/* OpenMP threads for later calls (n > 0); returns the current maximum. */
+int or_set_threads(int n);
+
 /* .tbc.json from explicit metadata (do_* arrays concatenated over fields) */
 int or_write_json_meta(double mhz, const char* path, int n, const uint64_t* file_loc, const int* is_first,
                        const int* phase_id, const int* do_count, const int* do_line, const int* do_startx,

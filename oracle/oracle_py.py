@@ -64,6 +64,8 @@ def lib():
         L.or_diag_counts.argtypes = [ctypes.c_double, ctypes.c_int, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int, _vp]
         L.or_k3_linelocs.argtypes = [ctypes.c_double, _vp, _vp, _vp, ctypes.c_int, _vp, _vp, _vp]
         L.or_chroma_trial_lines.argtypes = [_vp, ctypes.c_int, _vp]
+        L.or_stage_seconds.argtypes = [_vp, _vp]
+        L.or_set_threads.argtypes = [ctypes.c_int]
         L.or_k2b_chunk.argtypes = [ctypes.c_double, _vp, ctypes.c_uint64, _vp, ctypes.c_int] + [_vp] * 8
         L.or_lookup_ntsc_phase.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.or_burst_metric.argtypes = [ctypes.c_double, _vp]
@@ -72,6 +74,15 @@ def lib():
         L.or_burst_phase.restype = ctypes.c_double
         _L = L
     return _L
+
+
+STAGES = ("convert", "k1_fm_demod", "vsync_cluster", "k2_pulses", "k2b_linelocs", "k4_hsync_refine",
+          "k5_tbc_resample", "k6_chroma", "k7_dropouts")
+
+
+def set_threads(n):
+    """OpenMP threads of the oracle's parallel loops; returns the count in effect."""
+    return lib().or_set_threads(int(n))
 
 
 FORMATS = {np.dtype(np.uint8): 0, np.dtype(np.int16): 1, np.dtype(np.uint16): 2}
@@ -89,6 +100,9 @@ def decode(data, mhz, batch, trace=True, dc_correct=False, line_locator=0, diagn
         L.or_set_line_locator(d, int(line_locator), 1 if diagnostics else 0)
         n = L.or_decode(d, data.ctypes.data, data.size)
         out = {"fields": n}
+        st = np.zeros(9)
+        L.or_stage_seconds(d, st.ctypes.data)
+        out["stage_seconds"] = dict(zip(STAGES, st.tolist()))
         tbc = n * 263 * 910
         for key, fn in (("luma", L.or_luma), ("chroma", L.or_chroma)):
             if n == 0:  # no field was produced: the plane buffers were never allocated

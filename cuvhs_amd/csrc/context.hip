@@ -84,10 +84,11 @@ struct cuvhs_ctx {
     int k1_f32 = 0;            // K1 transforms in FP32 (CUVHS_PRECISION_FP32)
     void* G_a = nullptr;       // K1 block spectra and twiddles: double2, or float2 when k1_f32
     void* G_b = nullptr;
-    float2* G_p = nullptr;
+    void* G_p = nullptr;       // pre-band-pass spectrum: double2, or float2 when pre_f32
     void* tw_a = nullptr;
     void* tw_b = nullptr;
-    float2* tw_p = nullptr;
+    void* tw_p = nullptr;
+    int pre_f32 = 0;           // chroma pre-band-pass transforms in FP32 (CUVHS_PRECISION_FP32)
     double2* tw_1024 = nullptr;
     double* cs_tab = nullptr;       // cos/sin of the burst phase detector
     double2* chroma_mask = nullptr; // per-line band-pass: (1, 0) kept bins, (0, 0) others
@@ -325,7 +326,7 @@ int prebpf_launch(cuvhs_ctx* c, const void* chunk, long long n_valid, int nf_max
     p.G = c->G_p;
     p.tw = c->tw_p;
     p.out = out;
-    HIPCHK(c, run_prebpf(c->pre.bpf.B, p, nf_max, st));
+    HIPCHK(c, run_prebpf(c->pre.bpf.B, c->pre_f32, p, nf_max, st));
     return 0;
 }
 
@@ -635,7 +636,11 @@ int cuvhs_create(const cuvhs_config* cfg, cuvhs_ctx** out) {
         c->k1_f32 = cfg->precision == CUVHS_PRECISION_FP32;
         c->k1 = c->k1_f32 ? design_k1(f, 1e-8, bk1, 2, 1, 8) : design_k1(f, 1e-13, bk1, 2, 1, 16);
         const int bpre[2] = {8192, 16384};
-        c->pre = design_prebpf(f, 1e-7, bpre, 2);
+        // the pre-band-pass: FP64 with taps to 1e-9 of the largest (its
+        // truncation is then ~1e-6 of the chroma band's level even when the
+        // band holds only quantisation noise), or FP32 with taps to 1e-7
+        c->pre_f32 = cfg->precision == CUVHS_PRECISION_FP32;
+        c->pre = c->pre_f32 ? design_prebpf(f, 1e-7, bpre, 2) : design_prebpf(f, 1e-9, bpre, 2);
     } catch (...) {
         cuvhs_destroy(c);
         return CUVHS_E_UNSUPPORTED;
@@ -655,14 +660,13 @@ int cuvhs_create(const cuvhs_config* cfg, cuvhs_ctx** out) {
             if ((e = upload_spectrum<double2, double>(c, (double2**)&c->G_b, c->k1.video.spectrum, pb)) != hipSuccess) return bad(e, "G_b");
         }
     }
-    if ((e = upload_spectrum<float2, float>(c, &c->G_p, c->pre.bpf.spectrum, true)) != hipSuccess)
-        return bad(e, "G_p");
+    if (c->pre_f32) e = upload_spectrum<float2, float>(c, (float2**)&c->G_p, c->pre.bpf.spectrum, true);
+    else e = upload_spectrum<double2, double>(c, (double2**)&c->G_p, c->pre.bpf.spectrum, true);
+    if (e != hipSuccess) return bad(e, "G_p");
     {
         if ((e = upload_twiddles(c, &c->tw_a, c->k1.rf.B, c->k1_f32)) != hipSuccess) return bad(e, "twiddles");
         if ((e = upload_twiddles(c, &c->tw_b, c->k1.video.B, c->k1_f32)) != hipSuccess) return bad(e, "twiddles");
-        auto tp = twiddles<float2, float>(c->pre.bpf.B);
-        ALLOC(c->tw_p, tp.size());
-        if ((e = hipMemcpy(c->tw_p, tp.data(), tp.size() * sizeof(float2), hipMemcpyHostToDevice)) != hipSuccess) return bad(e, "twiddles");
+        if ((e = upload_twiddles(c, &c->tw_p, c->pre.bpf.B, c->pre_f32)) != hipSuccess) return bad(e, "twiddles");
         auto t1 = twiddles<double2, double>(kChromaFFT);
         ALLOC(c->tw_1024, t1.size());
         if ((e = hipMemcpy(c->tw_1024, t1.data(), t1.size() * sizeof(double2), hipMemcpyHostToDevice)) != hipSuccess) return bad(e, "twiddles");
@@ -1105,7 +1109,7 @@ int cuvhs_dropouts(cuvhs_ctx* c, const double* env, const double* ll, int nf, ui
 int cuvhs_selftest_block(int B, int is_f32, const void* d_x, const void* d_g, void* d_y) {
     if (!d_x || !d_g || !d_y) return CUVHS_E_ARG;
     // the shapes the decoder runs; anything else is refused before any copy
-    const bool ok = is_f32 ? (B == 4096 || B == 8192 || B == 16384) : (B == 1024 || B == 4096 || B == 8192);
+    const bool ok = B == 4096 || B == 8192 || B == 16384 || (!is_f32 && B == 1024);
     if (!ok) return CUVHS_E_ARG;
     return is_f32 ? block_selftest<float2, float>(B, 1, d_x, d_g, d_y)
                   : block_selftest<double2, double>(B, 0, d_x, d_g, d_y);

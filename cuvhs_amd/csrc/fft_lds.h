@@ -7,7 +7,9 @@
 //   wave-local transposes; the spectrum is applied in that transposed bin
 //   order, and the inverse retraces the steps.
 // * conv_rows<R>: 16 R points on R threads (R = 512 or 1024), the same
-//   scheme with 16 rows of R points transformed as 16 x 16 x R/256.
+//   scheme with 16 rows of R points transformed as 16 x 16 x R/256 (a wave
+//   owns 64 / (R / 16) rows, so only the first and last exchange cross
+//   waves).
 // * filter1024_wave: 1024 = 16 x 64 on one wavefront, a per-bin function
 //   applied in the middle (the chroma line band-pass).
 //
@@ -151,14 +153,16 @@ __device__ __forceinline__ void twiddle_run(typename C2<T>::type* v, const typen
         w[4] = cconj(w[4]);
         w[8] = cconj(w[8]);
     }
+    // each power is applied as it is formed, so at most eight are live
     w[3] = cmul(w[1], w[2]);
     w[5] = cmul(w[4], w[1]);
     w[6] = cmul(w[4], w[2]);
     w[7] = cmul(w[4], w[3]);
 #pragma unroll
-    for (int r = 9; r < 16; r++) w[r] = cmul(w[8], w[r - 8]);
+    for (int r = 1; r < 8; r++) v[r] = cmul(v[r], w[r]);
+    v[8] = cmul(v[8], w[8]);
 #pragma unroll
-    for (int r = 1; r < 16; r++) v[r] = cmul(v[r], w[r]);
+    for (int r = 9; r < 16; r++) v[r] = cmul(v[r], cmul(w[8], w[r - 8]));
 }
 
 // Host-side bin permutation of a natural-order spectrum for conv4096.
@@ -178,29 +182,34 @@ template <bool Block> __device__ __forceinline__ void lds_sync() {
     if constexpr (Block) __syncthreads();
     else wave_sync();
 }
-template <bool Block, bool Whole, typename V, typename WS, typename RS>
+// Padded: ws / rs return padded indices already (for patterns whose padding
+// the caller can write as a base plus constants, so the compiler keeps one
+// base register and immediate offsets instead of sixteen addresses).
+template <bool Block, bool Whole, bool Padded = false, typename V, typename WS, typename RS>
 __device__ __forceinline__ void lds_exchange(void* lds, V* v, WS ws, RS rs) {
+    auto pw = [&](int i) { return Padded ? ws(i) : pidx(ws(i)); };
+    auto pr = [&](int i) { return Padded ? rs(i) : pidx(rs(i)); };
     if constexpr (Whole) {
         V* b = static_cast<V*>(lds);
 #pragma unroll
-        for (int i = 0; i < 16; i++) b[pidx(ws(i))] = v[i];
+        for (int i = 0; i < 16; i++) b[pw(i)] = v[i];
         lds_sync<Block>();
 #pragma unroll
-        for (int i = 0; i < 16; i++) v[i] = b[pidx(rs(i))];
+        for (int i = 0; i < 16; i++) v[i] = b[pr(i)];
     } else {
         using T = decltype(v[0].x);
         T* b = static_cast<T*>(lds);
 #pragma unroll
-        for (int i = 0; i < 16; i++) b[pidx(ws(i))] = v[i].x;
+        for (int i = 0; i < 16; i++) b[pw(i)] = v[i].x;
         lds_sync<Block>();
 #pragma unroll
-        for (int i = 0; i < 16; i++) v[i].x = b[pidx(rs(i))];
+        for (int i = 0; i < 16; i++) v[i].x = b[pr(i)];
         lds_sync<Block>();
 #pragma unroll
-        for (int i = 0; i < 16; i++) b[pidx(ws(i))] = v[i].y;
+        for (int i = 0; i < 16; i++) b[pw(i)] = v[i].y;
         lds_sync<Block>();
 #pragma unroll
-        for (int i = 0; i < 16; i++) v[i].y = b[pidx(rs(i))];
+        for (int i = 0; i < 16; i++) v[i].y = b[pr(i)];
     }
 }
 
@@ -213,33 +222,51 @@ __device__ __forceinline__ void conv4096(void* lds, typename C2<T>::type* v, con
     const int t = threadIdx.x;
     const int w = t >> 6, l = t & 63, q = l & 15;
     const int base = ((w << 2) + (l >> 4)) << 8;
+    // padded LDS indices as a lane base plus compile-time constants (base is
+    // a multiple of 256): (r << 8) + t -> 272 r + pidx(t);
+    // base + 16 a + q -> pb + 17 a; base + 16 q + b -> pq + b
+    const int pt = pidx(t);
+    const int pb = base + (base >> 4) + q;
+    const int pq = base + (base >> 4) + 17 * q;
+    // FP64: the inverse stages reload their twiddles and the spectrum is
+    // loaded four bins at a time (indices opaque to the compiler, memory
+    // clobbers), so forward-stage values are not held across the transform
+    auto fresh = [](int e) {
+        if constexpr (sizeof(T) == 8) asm volatile("" : "+v"(e));
+        return e;
+    };
     V wb[4];
     dft16<-1, T>(v);
     twiddle_base<T>(wb, tw, t);
     twiddle_run<-1, T>(v, wb);
     // rows k1 to the waves; 256-point transforms: n2 = 16 a + b, k2 = c + 16 d
-    lds_exchange<true, Whole>(lds, v, [&](int r) { return (r << 8) + t; },
-                              [&](int a) { return base + 16 * a + q; });           // lane q = b
+    lds_exchange<true, Whole, true>(lds, v, [&](int r) { return 272 * r + pt; },
+                                    [&](int a) { return pb + 17 * a; });          // lane q = b
     dft16<-1, T>(v);                                                            // a -> c
     twiddle_base<T>(wb, tw, 16 * q);
     twiddle_run<-1, T>(v, wb);                                                  // W256^(b c)
-    lds_exchange<false, Whole>(lds, v, [&](int c) { return base + 16 * c + q; },
-                               [&](int b) { return base + 16 * q + b; });          // lane q = c
+    lds_exchange<false, Whole, true>(lds, v, [&](int c) { return pb + 17 * c; },
+                                     [&](int b) { return pq + b; });              // lane q = c
     dft16<-1, T>(v);                                                            // b -> d
-    V g[16];
 #pragma unroll
-    for (int d = 0; d < 16; d++) g[d] = Gp[(w << 10) + (d << 6) + l];
+    for (int d0 = 0; d0 < 16; d0 += 4) {
+        V g[4];
 #pragma unroll
-    for (int d = 0; d < 16; d++) v[d] = cmul(v[d], g[d]);
+        for (int d = 0; d < 4; d++) g[d] = Gp[(w << 10) + ((d0 + d) << 6) + l];
+#pragma unroll
+        for (int d = 0; d < 4; d++) v[d0 + d] = cmul(v[d0 + d], g[d]);
+        if constexpr (sizeof(T) == 8) asm volatile("" ::: "memory");
+    }
     dft16<+1, T>(v);                                                            // d -> b
-    twiddle_base<T>(wb, tw, 16 * q);
+    twiddle_base<T>(wb, tw, fresh(16 * q));
     twiddle_run<+1, T>(v, wb);
-    lds_exchange<false, Whole>(lds, v, [&](int b) { return base + 16 * q + b; },
-                               [&](int c) { return base + 16 * c + q; });          // lane q = b
+    lds_exchange<false, Whole, true>(lds, v, [&](int b) { return pq + b; },
+                                     [&](int c) { return pb + 17 * c; });         // lane q = b
     dft16<+1, T>(v);                                                            // c -> a
-    lds_exchange<true, Whole>(lds, v, [&](int a) { return base + 16 * a + q; },
-                              [&](int r) { return (r << 8) + t; });
-    twiddle_base<T>(wb, tw, t);
+    const int pt2 = fresh(pt);
+    lds_exchange<true, Whole, true>(lds, v, [&](int a) { return pb + 17 * a; },
+                                    [&](int r) { return 272 * r + pt2; });
+    twiddle_base<T>(wb, tw, fresh(t));
     twiddle_run<+1, T>(v, wb);
     dft16<+1, T>(v);
 }
@@ -323,11 +350,12 @@ __device__ __forceinline__ void dft_small(V* v) {
     else dft2<S>(v[0], v[1]);
 }
 
-// In: v[n1] = x[R n1 + t].  Out: v[n1] = y[R n1 + t].  buf: lds_elems(16 R)
-// elements.  tw: the table of W(16 R)^m.
-template <typename T, int R>
-__device__ __forceinline__ void conv_rows(typename C2<T>::type* buf, typename C2<T>::type* v,
-                                          const typename C2<T>::type* tw, const typename C2<T>::type* Gp) {
+// In: v[n1] = x[R n1 + t].  Out: v[n1] = y[R n1 + t].  lds: lds_elems(16 R)
+// elements of V (Whole) or of T (real and imaginary parts exchanged in turn,
+// half the LDS for more barriers).  tw: the table of W(16 R)^m.
+template <typename T, int R, bool Whole = true>
+__device__ __forceinline__ void conv_rows(void* lds, typename C2<T>::type* v, const typename C2<T>::type* tw,
+                                          const typename C2<T>::type* Gp) {
     using V = typename C2<T>::type;
     constexpr int NB = 16 * R, G = R / 256, L = R / 16;
     static_assert(R == 512 || R == 1024, "row length");
@@ -336,67 +364,71 @@ __device__ __forceinline__ void conv_rows(typename C2<T>::type* buf, typename C2
     const int l = t % L;                    // lane within the row
     const int row = rw * R;
     const int hi = l / G, lo = l % G;       // (c, g) in pass B, (c, e1h) in pass C
+    // In the FP64 1024-row block (128 registers at 1024 threads) the inverse
+    // stages reload their twiddles: indices made opaque to the compiler keep
+    // it from holding the forward stages' values across the whole transform.
+    auto fresh = [](int e) {
+        if constexpr (sizeof(T) == 8 && R == 1024) asm volatile("" : "+v"(e));
+        return e;
+    };
     V wb[4];
     dft16<-1, T>(v);
     twiddle_base<T, NB>(wb, tw, t);
     twiddle_run<-1, T>(v, wb);
-#pragma unroll
-    for (int r = 0; r < 16; r++) buf[pidx(r * R + t)] = v[r];
-    __syncthreads();
-    // pass A: lane m = l holds n2 = L a + m
-#pragma unroll
-    for (int a = 0; a < 16; a++) v[a] = buf[pidx(row + L * a + l)];
+    // Padded LDS indices, written as a lane base plus a compile-time
+    // constant (pidx(A + B) = pidx(A) + pidx(B) for A a multiple of 16, and
+    // row, L and 16 lo are): the compiler then keeps one base register per
+    // pattern and immediate offsets.
+    constexpr int RP = R + R / 16, LP = L + L / 16;
+    const int pt = pidx(t);                              // r R + t  ->  r RP + pt
+    const int prow = row + row / 16;
+    const int pa = prow + pidx(l);                       // pass A: row + L a + l
+    const int pb = prow + LP * hi + lo;                  // pass B: row + L hi + G b + lo
+    const int pc = prow + LP * hi + 17 * lo;             // pass C: row + L hi + 16 lo + j
+    // rows k1 to their waves; pass A: lane m = l holds n2 = L a + m
+    lds_exchange<true, Whole, true>(lds, v, [&](int r) { return r * RP + pt; }, [&](int a) { return pa + LP * a; });
     dft16<-1, T>(v);                                               // a -> c
     twiddle_base<T, NB>(wb, tw, 16 * l);
     twiddle_run<-1, T>(v, wb);                                     // WR^(m c)
-#pragma unroll
-    for (int c = 0; c < 16; c++) buf[pidx(row + L * c + l)] = v[c];
-    wave_sync();
     // pass B: lane (c = hi, g = lo) holds m = G b + g
-#pragma unroll
-    for (int b = 0; b < 16; b++) v[b] = buf[pidx(row + L * hi + G * b + lo)];
+    lds_exchange<false, Whole, true>(lds, v, [&](int c) { return pa + LP * c; },
+                                     [&](int b) { return pb + pidx(G * b); });
     dft16<-1, T>(v);                                               // b -> e1
     twiddle_base<T, NB>(wb, tw, 256 * lo);
     twiddle_run<-1, T>(v, wb);                                     // WL^(g e1)
-#pragma unroll
-    for (int e1 = 0; e1 < 16; e1++) buf[pidx(row + L * hi + G * e1 + lo)] = v[e1];
-    wave_sync();
     // pass C: lane (c = hi, e1h = lo) holds (e1l, g) at v[G e1l + g]
-#pragma unroll
-    for (int j = 0; j < 16; j++) v[j] = buf[pidx(row + L * hi + 16 * lo + j)];
+    lds_exchange<false, Whole, true>(lds, v, [&](int e1) { return pb + pidx(G * e1); },
+                                     [&](int j) { return pc + j; });
 #pragma unroll
     for (int e = 0; e < 16; e += G) dft_small<G, -1>(v + e);
-    V g[16];
+    // the spectrum in groups of four loads (the compiler would otherwise hold
+    // all sixteen beside the data: 64 more registers in FP64)
 #pragma unroll
-    for (int j = 0; j < 16; j++) g[j] = Gp[row + j * L + l];
+    for (int j0 = 0; j0 < 16; j0 += 4) {
+        V g[4];
 #pragma unroll
-    for (int j = 0; j < 16; j++) v[j] = cmul(v[j], g[j]);
+        for (int j = 0; j < 4; j++) g[j] = Gp[row + (j0 + j) * L + l];
+#pragma unroll
+        for (int j = 0; j < 4; j++) v[j0 + j] = cmul(v[j0 + j], g[j]);
+        if constexpr (sizeof(T) == 8) asm volatile("" ::: "memory");
+    }
 #pragma unroll
     for (int e = 0; e < 16; e += G) dft_small<G, +1>(v + e);
-#pragma unroll
-    for (int j = 0; j < 16; j++) buf[pidx(row + L * hi + 16 * lo + j)] = v[j];
-    wave_sync();
     // inverse pass B: lane (c, g)
-#pragma unroll
-    for (int e1 = 0; e1 < 16; e1++) v[e1] = buf[pidx(row + L * hi + G * e1 + lo)];
-    twiddle_base<T, NB>(wb, tw, 256 * lo);
+    lds_exchange<false, Whole, true>(lds, v, [&](int j) { return pc + j; },
+                                     [&](int e1) { return pb + pidx(G * e1); });
+    twiddle_base<T, NB>(wb, tw, fresh(256 * lo));
     twiddle_run<+1, T>(v, wb);
     dft16<+1, T>(v);                                               // e1 -> b
-#pragma unroll
-    for (int b = 0; b < 16; b++) buf[pidx(row + L * hi + G * b + lo)] = v[b];
-    wave_sync();
     // inverse pass A: lane m
-#pragma unroll
-    for (int c = 0; c < 16; c++) v[c] = buf[pidx(row + L * c + l)];
-    twiddle_base<T, NB>(wb, tw, 16 * l);
+    lds_exchange<false, Whole, true>(lds, v, [&](int b) { return pb + pidx(G * b); },
+                                     [&](int c) { return pa + LP * c; });
+    twiddle_base<T, NB>(wb, tw, fresh(16 * l));
     twiddle_run<+1, T>(v, wb);
     dft16<+1, T>(v);                                               // c -> a
-#pragma unroll
-    for (int a = 0; a < 16; a++) buf[pidx(row + L * a + l)] = v[a];
-    __syncthreads();
-#pragma unroll
-    for (int r = 0; r < 16; r++) v[r] = buf[pidx(r * R + t)];
-    twiddle_base<T, NB>(wb, tw, t);
+    const int pt2 = fresh(pt);
+    lds_exchange<true, Whole, true>(lds, v, [&](int a) { return pa + LP * a; }, [&](int r) { return r * RP + pt2; });
+    twiddle_base<T, NB>(wb, tw, fresh(t));
     twiddle_run<+1, T>(v, wb);
     dft16<+1, T>(v);
 }

@@ -290,10 +290,15 @@ __global__ __launch_bounds__(B / 16, (k1_min_blocks<B, T>())) void k1_part_b(K1B
 // so each workgroup convolves two adjacent V-output blocks in one complex
 // transform: block 2p in the real part, block 2p+1 in the imaginary part
 // (G is the spectrum of a real kernel, so the two never mix beyond rounding).
-template <int B>
+// T is the transform precision: double in the FP64 form (the reference's
+// FP64 cuFFT band-pass; real and imaginary parts exchanged through LDS in
+// turn, 139 KB for 16384 points), float in the FP32 form.  The output is
+// stored as float either way: it holds only the 60 kHz - 1.2 MHz band, so
+// its FP32 rounding (6e-8 of the band's own level) is far below a chroma LSB.
+template <int B, typename T>
 __global__ __launch_bounds__(B / 16) void k_prebpf(PreLaunch a) {
+    using V2 = typename C2<T>::type;
     extern __shared__ __align__(16) unsigned char smem[];
-    float2* buf = reinterpret_cast<float2*>(smem);
     constexpr int NT = B / 16;
     const int f = blockIdx.y;
     const int ns0 = 2 * (int)blockIdx.x * a.V;
@@ -302,23 +307,23 @@ __global__ __launch_bounds__(B / 16) void k_prebpf(PreLaunch a) {
     const bool two = ns1 < a.spfp;
     const int seg0 = f * a.spfp;
     const double dcv = a.dc ? *a.dc : 0.0;
-    float2 v[16];
+    V2 v[16];
 #pragma unroll
     for (int i = 0; i < 16; i++) {
         int n0 = wrap_n(ns0 - a.lp + (int)threadIdx.x + i * NT, a.N2);
         int n1 = wrap_n(ns1 - a.lp + (int)threadIdx.x + i * NT, a.N2);
-        v[i].x = n0 < a.spfp ? (float)raw_sample(a.cap, a.in_fmt, seg0 + n0, a.n_valid, dcv) : 0.0f;
-        v[i].y = two && n1 < a.spfp ? (float)raw_sample(a.cap, a.in_fmt, seg0 + n1, a.n_valid, dcv) : 0.0f;
+        v[i].x = n0 < a.spfp ? (T)raw_sample(a.cap, a.in_fmt, seg0 + n0, a.n_valid, dcv) : (T)0;
+        v[i].y = two && n1 < a.spfp ? (T)raw_sample(a.cap, a.in_fmt, seg0 + n1, a.n_valid, dcv) : (T)0;
     }
     // G is in the wave-local block's bin order; v[r] becomes the outputs at
     // o = NT r + t - lp of both blocks
-    conv_rows<float, NT>(buf, v, a.tw, a.G);
+    conv_rows<T, NT, sizeof(T) == 4>(smem, v, static_cast<const V2*>(a.tw), static_cast<const V2*>(a.G));
 #pragma unroll
     for (int r = 0; r < 16; r++) {
         int o = r * NT + (int)threadIdx.x - a.lp;
         if (o >= 0 && o < a.V) {
-            if (ns0 + o < a.spfp) a.out[seg0 + ns0 + o] = v[r].x;
-            if (ns1 + o < a.spfp) a.out[seg0 + ns1 + o] = v[r].y;
+            if (ns0 + o < a.spfp) a.out[seg0 + ns0 + o] = (float)v[r].x;
+            if (ns1 + o < a.spfp) a.out[seg0 + ns1 + o] = (float)v[r].y;
         }
     }
 }
@@ -359,12 +364,13 @@ static hipError_t launch_b(const K1BLaunch& a, int nf, hipStream_t st) {
     hipLaunchKernelGGL((k1_part_b<B, T>), dim3(nb, nf), dim3(B / 16), sh, st, a);
     return hipGetLastError();
 }
-template <int B>
+template <int B, typename T>
 static hipError_t launch_pre(const PreLaunch& a, int nf, hipStream_t st) {
     int nb = (int)((a.spfp + 2 * a.V - 1) / (2 * a.V));
-    size_t sh = (size_t)lds_elems(B) * sizeof(float2);
-    ensure_lds(k_prebpf<B>, sh);
-    hipLaunchKernelGGL(k_prebpf<B>, dim3(nb, nf), dim3(B / 16), sh, st, a);
+    // whole complex elements in FP32, real and imaginary parts in turn in FP64
+    size_t sh = (size_t)lds_elems(B) * (sizeof(T) == 4 ? 2 * sizeof(T) : sizeof(T));
+    ensure_lds(k_prebpf<B, T>, sh);
+    hipLaunchKernelGGL((k_prebpf<B, T>), dim3(nb, nf), dim3(B / 16), sh, st, a);
     return hipGetLastError();
 }
 
@@ -382,9 +388,11 @@ hipError_t run_k1_part_b(int B, int f32, const K1BLaunch& a, int nf, hipStream_t
     if (f32 && B == 8192) return launch_b<8192, float>(a, nf, st);
     return hipErrorInvalidValue;
 }
-hipError_t run_prebpf(int B, const PreLaunch& a, int nf, hipStream_t st) {
-    if (B == 8192) return launch_pre<8192>(a, nf, st);
-    if (B == 16384) return launch_pre<16384>(a, nf, st);
+hipError_t run_prebpf(int B, int f32, const PreLaunch& a, int nf, hipStream_t st) {
+    if (f32 && B == 8192) return launch_pre<8192, float>(a, nf, st);
+    if (f32 && B == 16384) return launch_pre<16384, float>(a, nf, st);
+    if (!f32 && B == 8192) return launch_pre<8192, double>(a, nf, st);
+    if (!f32 && B == 16384) return launch_pre<16384, double>(a, nf, st);
     return hipErrorInvalidValue;
 }
 hipError_t run_input_mean(const void* cap, int in_fmt, long long n, double* partial, double* mean, hipStream_t st) {
@@ -399,10 +407,16 @@ hipError_t run_fill_f64(double* p, long long n, double v, hipStream_t st) {
     return hipGetLastError();
 }
 
+// LDS layout of the self-tested blocks as the decoder runs them: FP64 1024-,
+// 4096- and 16384-point blocks exchange real and imaginary parts in turn.
+template <int B, typename T>
+constexpr bool selftest_whole() { return !(sizeof(T) == 8 && (B == 1024 || B == 4096 || B == 16384)); }
+
 // Self-test of the convolution blocks the decoder runs: y = B IFFT(FFT(x) G)
 // for one block, in the register layout the kernels use (v[r] = x[NT r + t]),
 // with G in the block's bin order.  B = 4096: conv4096 (FP64 with split
-// exchanges as in K1, FP32 whole); 8192 / 16384: conv_rows<512 / 1024>;
+// exchanges as in K1, FP32 whole); 8192 / 16384: conv_rows<512 / 1024>
+// (FP64 16384 with split exchanges as in the pre-band-pass);
 // 1024: filter1024_wave (one wavefront, FP64, split exchanges, G applied per
 // bin as in the chroma kernel).
 template <int B, typename T>
@@ -421,7 +435,7 @@ __global__ __launch_bounds__(B == 1024 ? 64 : B / 16) void k_block_selftest(cons
     } else if constexpr (B == 4096) {
         conv4096<T, k1_whole<B, T>()>(smem, v, tw, gp);
     } else {
-        conv_rows<T, B / 16>(reinterpret_cast<V*>(smem), v, tw, gp);
+        conv_rows<T, B / 16, selftest_whole<B, T>()>(smem, v, tw, gp);
     }
 #pragma unroll
     for (int r = 0; r < 16; r++) y[NT * r + threadIdx.x] = v[r];
@@ -430,7 +444,7 @@ __global__ __launch_bounds__(B == 1024 ? 64 : B / 16) void k_block_selftest(cons
 template <int B, typename T>
 static hipError_t block_st_launch(const void* x, const void* gp, void* y, const void* tw, hipStream_t st) {
     using V = typename C2<T>::type;
-    size_t sh = (size_t)lds_elems(B) * (B == 1024 || (B == 4096 && sizeof(T) == 8) ? sizeof(T) : sizeof(V));
+    size_t sh = (size_t)lds_elems(B) * (selftest_whole<B, T>() ? sizeof(V) : sizeof(T));
     (void)hipFuncSetAttribute((const void*)k_block_selftest<B, T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
     hipLaunchKernelGGL((k_block_selftest<B, T>), dim3(1), dim3(B == 1024 ? 64 : B / 16), sh, st, (const V*)x,
                        (const V*)gp, (V*)y, (const V*)tw);
@@ -443,6 +457,7 @@ hipError_t run_block_selftest(int B, int is_f32, const void* x, const void* gp, 
         if (B == 1024) return block_st_launch<1024, double>(x, gp, y, tw, st);
         if (B == 4096) return block_st_launch<4096, double>(x, gp, y, tw, st);
         if (B == 8192) return block_st_launch<8192, double>(x, gp, y, tw, st);
+        if (B == 16384) return block_st_launch<16384, double>(x, gp, y, tw, st);
     } else {
         if (B == 4096) return block_st_launch<4096, float>(x, gp, y, tw, st);
         if (B == 8192) return block_st_launch<8192, float>(x, gp, y, tw, st);

@@ -43,8 +43,8 @@ struct PreLaunch {
     const double* dc;
     long long n_valid;
     int spfp, N2, lm, lp, V;
-    const float2* G;
-    const float2* tw;
+    const void* G;      // block spectrum: double2 (FP64 form) or float2 (FP32 form)
+    const void* tw;     // twiddles of the same type
     float* out;
 };
 struct ChromaLaunch {
@@ -69,7 +69,7 @@ struct ChromaLaunch {
 // f32: transform in FP32 (float2 spectra and twiddles) instead of FP64
 hipError_t run_k1_part_a(int B, int f32, const K1ALaunch& a, int nf, hipStream_t st);
 hipError_t run_k1_part_b(int B, int f32, const K1BLaunch& a, int nf, hipStream_t st);
-hipError_t run_prebpf(int B, const PreLaunch& a, int nf, hipStream_t st);
+hipError_t run_prebpf(int B, int f32, const PreLaunch& a, int nf, hipStream_t st);
 hipError_t run_fill_f64(double* p, long long n, double v, hipStream_t st);
 hipError_t run_k3(const ChunkCtl* ctl, int nf_override, int max_fields, const Fmt& fmt, const int* starts,
                   const int* lens, const int* count, int* types, double* linelocs, int* is_first, hipStream_t st);

@@ -195,9 +195,9 @@ int cuvhs_dropouts(cuvhs_ctx* ctx, const double* d_env, const double* d_linelocs
  * the reference, whose transforms are cuFFT's, fm_demod.cu:396-486):
  * d_y = B * IFFT(FFT(d_x) * d_g) for one B-point block of complex values
  * (double2, or float2 with is_f32), d_g in natural bin order.  B = 4096
- * (K1 blocks, FP64 or FP32), 8192 (FP64 or FP32), 16384 (FP32, the chroma
- * pre-band-pass), 1024 (FP64, the chroma line filter); other shapes return
- * CUVHS_E_ARG. */
+ * (K1 blocks, FP64 or FP32), 8192 (FP64 or FP32), 16384 (the chroma
+ * pre-band-pass, FP64 or FP32), 1024 (FP64, the chroma line filter); other
+ * shapes return CUVHS_E_ARG. */
 int cuvhs_selftest_block(int B, int is_f32, const void* d_x, const void* d_g, void* d_y);
 
 /* Self-test of the FP64 arctangent of K1's phase (no counterpart in the

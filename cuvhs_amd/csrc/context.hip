@@ -19,6 +19,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -530,6 +531,52 @@ int finish_tail(cuvhs_ctx* c, Slot& s) {
     return 0;
 }
 
+// ---- scratch of the allocation-free dropout entry point ----
+//
+// cuvhs_dropouts_noalloc takes its device scratch (the per-slice envelope
+// sums and the 16-sample block classes of K7, the buffers a context
+// allocates for cuvhs_dropouts) from this file-scope array instead of
+// hipMalloc: kNoAllocMaxFields fields, 5.9 MB, held on every device the
+// library runs on until the process ends.  Host code never uses the array's
+// name as an address: the device address is taken once per device with
+// hipGetSymbolAddress (with that device current) and kept in
+// g_noalloc_scratch.  Every call on a device shares its array, so calls that
+// overlap on one device race; the kernels write every scratch element they
+// read in the same call, so nothing needs clearing between calls.
+constexpr int kNoAllocMaxFields = 128;
+constexpr size_t kNoAllocBytes = 5898240;  // 128 fields x (45,056 class bytes + 128 partial sums x 8 B)
+constexpr int kNoAllocDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+std::mutex g_noalloc_mu;
+void* g_noalloc_scratch[kNoAllocDevices] = {};
+std::string g_free_err;  // last error of a context-free call (cuvhs_last_error(NULL))
+
+int noalloc_fail(int rc, const std::string& what) {
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    g_free_err = what;
+    return rc;
+}
+
+// The scratch array's device address on the current device.
+int noalloc_scratch(void** out) {
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return noalloc_fail(CUVHS_E_HIP, std::string("hipGetDevice: ") + hipGetErrorString(e));
+    if (dev < 0 || dev >= kNoAllocDevices) return noalloc_fail(CUVHS_E_UNSUPPORTED, "device ordinal beyond 63");
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (!g_noalloc_scratch[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            g_free_err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+            return CUVHS_E_HIP;
+        }
+        g_noalloc_scratch[dev] = p;
+    }
+    *out = g_noalloc_scratch[dev];
+    return 0;
+}
+
 }  // namespace
 
 template <typename V, typename T>
@@ -799,7 +846,11 @@ int cuvhs_reset(cuvhs_ctx* c) {
     return 0;
 }
 
-const char* cuvhs_last_error(const cuvhs_ctx* c) { return c ? c->err.c_str() : "no context"; }
+const char* cuvhs_last_error(const cuvhs_ctx* c) {
+    if (c) return c->err.c_str();
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    return g_free_err.empty() ? "no context" : g_free_err.c_str();
+}
 
 int cuvhs_decode_chunk(cuvhs_ctx* c, const void* chunk, uint64_t avail, uint64_t raw_offset, int max_fields,
                        cuvhs_chunk_info* info) {
@@ -1100,6 +1151,34 @@ int cuvhs_dropouts(cuvhs_ctx* c, const double* env, const double* ll, int nf, ui
     HIPCHK(c, run_dropouts(env, ll, c->slot[0].ctl, nf, nf, c->fmt, c->do_partial, c->do_cls, luma, chroma, lines,
                            starts, ends, count, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+// dropout_detect on caller buffers without a context and without any
+// allocation: the scratch comes from the file-scope array above.  Enqueues on
+// `stream` (NULL: the default stream of the current device) and returns.
+int cuvhs_dropouts_noalloc(double sample_rate_mhz, const double* env, const double* ll, int nf, uint16_t* luma,
+                           uint16_t* chroma, int* lines, int* starts, int* ends, int* count, void* stream) {
+    if (!env || !ll || !luma || !chroma || !lines || !starts || !ends || !count || nf < 1)
+        return noalloc_fail(CUVHS_E_ARG, "cuvhs_dropouts_noalloc: null buffer or no field");
+    if (!(sample_rate_mhz >= 16.0 && sample_rate_mhz <= 40.0))
+        return noalloc_fail(CUVHS_E_UNSUPPORTED, "cuvhs_dropouts_noalloc: sample rate outside 16-40 MHz");
+    if (nf > kNoAllocMaxFields)
+        return noalloc_fail(CUVHS_E_CAPACITY, "cuvhs_dropouts_noalloc: " + std::to_string(nf) +
+                                                  " fields exceed the scratch limit of 128 fields per call");
+    const size_t cls_bytes = dropout_class_bytes(kNoAllocMaxFields);
+    if (cls_bytes + dropout_partial_count(kNoAllocMaxFields) * sizeof(double) > kNoAllocBytes)
+        return noalloc_fail(CUVHS_E_UNSUPPORTED, "cuvhs_dropouts_noalloc: scratch layout larger than the array");
+    void* base = nullptr;
+    int rc = noalloc_scratch(&base);
+    if (rc) return rc;
+    unsigned char* cls = static_cast<unsigned char*>(base);
+    double* partial = reinterpret_cast<double*>(cls + cls_bytes);
+    const Fmt f = make_fmt(sample_rate_mhz);
+    hipError_t e = run_dropouts(env, ll, nullptr, nf, nf, f, partial, cls, luma, chroma, lines, starts, ends, count,
+                                static_cast<hipStream_t>(stream));
+    if (e != hipSuccess)
+        return noalloc_fail(CUVHS_E_HIP, std::string("cuvhs_dropouts_noalloc: ") + hipGetErrorString(e));
     return 0;
 }
 

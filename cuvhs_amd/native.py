@@ -83,6 +83,7 @@ _SIGS = {
     "cuvhs_tbc_resample": (_i, [_vp, _vp, _i, _vp, _i, _vp]),
     "cuvhs_chroma": (_i, [_vp, _vp, _u64, _vp, _i, _vp, _vp]),
     "cuvhs_dropouts": (_i, [_vp, _vp, _vp, _i, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "cuvhs_dropouts_noalloc": (_i, [_d, _vp, _vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "cuvhs_selftest_block": (_i, [_i, _i, _vp, _vp, _vp]),
     "cuvhs_selftest_atan2": (_i, [_vp, _vp, _vp, ctypes.c_int64]),
     "cuvhs_last_error": (ctypes.c_char_p, [_vp]),

@@ -191,6 +191,20 @@ int cuvhs_chroma(cuvhs_ctx* ctx, const void* d_raw, uint64_t n_valid, const doub
 int cuvhs_dropouts(cuvhs_ctx* ctx, const double* d_env, const double* d_linelocs, int nfields, uint16_t* d_luma,
                    uint16_t* d_chroma, int* d_lines, int* d_starts, int* d_ends, int* d_count);
 
+/* dropout_detect without a context and without allocating (no counterpart
+ * in the reference: for callers that cannot allocate device memory, such as
+ * code under HIP graph capture or tight serving loops).  Same inputs,
+ * outputs and results as cuvhs_dropouts for a capture at sample_rate_mhz;
+ * the kernels' scratch is a file-scope device array of 5.9 MB (held on each
+ * device until the process ends) that holds at most 128 fields: more return
+ * CUVHS_E_CAPACITY, with the reason in cuvhs_last_error(NULL).  Enqueues on
+ * `stream` (hipStream_t, NULL = the current device's default stream) and
+ * returns without waiting.  Calls that overlap on one device share the
+ * scratch and race; serialise them (one stream, or events). */
+int cuvhs_dropouts_noalloc(double sample_rate_mhz, const double* d_env, const double* d_linelocs, int nfields,
+                           uint16_t* d_luma, uint16_t* d_chroma, int* d_lines, int* d_starts, int* d_ends,
+                           int* d_count, void* stream);
+
 /* Self-test of the convolution blocks the decoder runs (no counterpart in
  * the reference, whose transforms are cuFFT's, fm_demod.cu:396-486):
  * d_y = B * IFFT(FFT(d_x) * d_g) for one B-point block of complex values
@@ -236,7 +250,8 @@ int cuvhs_k1_blocks(const cuvhs_ctx* ctx, int* block_a, int* valid_a, int* block
 int cuvhs_diagnostics(cuvhs_ctx* ctx, int enable);
 int cuvhs_diagnostics_read(const cuvhs_ctx* ctx, int64_t* totals, int32_t* last_chunk);
 
-/* Last HIP error text for the context (or a static string). */
+/* Last error text for the context; with NULL, of the last failed
+ * context-free call (cuvhs_dropouts_noalloc). */
 const char* cuvhs_last_error(const cuvhs_ctx* ctx);
 
 #ifdef __cplusplus

@@ -7,10 +7,15 @@ One step decodes the whole synthetic capture (600 NTSC fields of u8 RF at
 F = 64 fields) with the capture already resident in HBM and outputs left in
 HBM.  Prints one JSON line on stdout.
 
-The headline value is the FP64 form (the reference's arithmetic, checked
-against the oracle).  The line also carries the FP32 form (K1 transforms in
-FP32) timed the same way, with its error and decision report against the
-FP64 form (cuvhs_amd.precision); --no-forms skips it.
+The headline value is the MFMA mixed form (--form, default mixed-fp16: K1
+part A on the matrix cores with f16 hi/lo taps, part B in FP32, an FP64
+guard band that keeps every sync-threshold decision the FP64 form's).  The
+line also carries the other forms timed the same way -- fp64 (the
+reference's arithmetic, the one checked against the oracle), fp32 (K1
+transforms in FP32) and mixed-bf16 -- each reduced form with its error and
+decision report against the FP64 form (cuvhs_amd.precision), the mixed
+form's MFMA utilisation and its speed over the plain FP32 form;
+--no-forms skips them.
 """
 import argparse
 import json
@@ -25,6 +30,11 @@ sys.path.insert(0, ROOT)
 METRIC = "NTSC fields/s at 28 MHz (MFMA mixed vs plain FP32, 1 GPU) + achieved HBM GB/s"
 PEAK_HBM_GBS = 8000.0
 PEAK_FP64_TFLOPS = 78.6   # MI355X FP64, vector and matrix
+PEAK_F16_TFLOPS = 2500.0  # MI355X dense f16 / bf16 matrix
+FORMS = ("fp64", "fp32", "mixed-fp16", "mixed-bf16")
+DTYPES = {"fp64": "f64", "fp32": "f32 (K1), f64 (rest)",
+          "mixed-fp16": "f16 hi/lo MFMA (K1 part A), f32 (K1 part B), f64 (guard band, rest)",
+          "mixed-bf16": "bf16 hi/lo MFMA (K1 part A), f32 (K1 part B), f64 (guard band, rest)"}
 
 LINE_RATE = 15734.264      # NTSC line rate (video_format.cpp)
 TBC_BYTES = 2 * 2 * 910 * 263  # luma + chroma u16 per field
@@ -136,7 +146,7 @@ def cpu_baseline_full(args):
     print(json.dumps(line), flush=True)
 
 
-def cli_end_to_end(cap, mhz, batch):
+def cli_end_to_end(cap, mhz, batch, form):
     """The `cuvhs` CLI on the capture written to a file: disk read, host to
     device copies, decode, device to host copies and .tbc/.tbc.json writes."""
     import subprocess
@@ -149,13 +159,13 @@ def cli_end_to_end(cap, mhz, batch):
         runs = []
         for _ in range(2):   # the first run pays for kernel loading
             t = time.perf_counter()
-            r = subprocess.run([exe, "-f", f"{mhz:g}", "--batch", str(batch), "--overwrite", inp, out],
+            r = subprocess.run([exe, "-f", f"{mhz:g}", "--batch", str(batch), "--precision", form, "--overwrite", inp, out],
                                stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)
             runs.append(time.perf_counter() - t)
             if r.returncode != 0:
                 return {"error": r.stderr[-300:]}
         nf = json.load(open(out + ".tbc.json"))["videoParameters"]["numberOfSequentialFields"]
-    return {"fields": nf, "seconds": round(runs[-1], 3), "fields_per_s": round(nf / runs[-1], 1),
+    return {"fields": nf, "precision": form, "seconds": round(runs[-1], 3), "fields_per_s": round(nf / runs[-1], 1),
             "includes": "process start, file read, PCIe copies, decode, file writes"}
 
 
@@ -175,7 +185,7 @@ def serving_main(args):
     reqs = serving.make_requests(16, 4.0, 30, pool_fields, pool_fields)
     res = {}
     burst = [dict(r, arrival=0.0) for r in reqs]
-    for form in ("fp64", "fp32"):
+    for form in ("mixed-fp16", "fp32", "fp64"):
         # one untimed pass warms the contexts' kernels, then the timed runs:
         # Poisson arrivals, and all 16 at once (the saturated aggregate rate)
         serving.run(d_pool, reqs[:2], precision=form)
@@ -184,7 +194,8 @@ def serving_main(args):
         log(f"{form}: {res[form]}")
     line = {
         "metric": "serving: per-field latency p50/p99 and aggregate NTSC fields/s, 16 concurrent captures",
-        "value": res["fp64"]["aggregate_fields_per_s"],
+        "value": res["mixed-fp16"]["aggregate_fields_per_s"],
+        "mixed_vs_fp32": round(res["mixed-fp16"]["aggregate_fields_per_s"] / res["fp32"]["aggregate_fields_per_s"], 3),
         "unit": "fields/s",
         "n_gpus": 1,
         "higher_is_better": True,
@@ -196,6 +207,41 @@ def serving_main(args):
     print(json.dumps(line), flush=True)
 
 
+def guard_blocks(dec, d_cap):
+    """FP64 guard-band blocks (part A, part B) one decode recomputes."""
+    ch = dec.decode_chunked(d_cap)["chunks"]
+    dec.reset()
+    return [sum(c["guard_blocks"][0] for c in ch), sum(c["guard_blocks"][1] for c in ch)]
+
+
+def time_form(native, torch, d_cap, mhz, args, form):
+    """One precision form on the same capture and schedule as the headline:
+    warmup, K timed decodes, one serialised decode for stage times."""
+    d = native.Decoder(mhz, args.batch, device=0, precision=form)
+    n = 0
+    for _ in range(args.warmup):
+        d.reset()
+        d.decode_capture(d_cap)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        d.reset()
+        n = d.decode_capture(d_cap)["fields"]
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    d.profile(True, serial=True)
+    d.reset()
+    d.decode_capture(d_cap)
+    iso = d.profile_read()
+    d.profile(False)
+    r = {"value": round(n * args.steps / dt, 2), "unit": "fields/s", "ms_per_step": round(dt * 1000.0 / args.steps, 3),
+         "dtype": DTYPES[form], "stage_ms_per_step_isolated": {k: round(v, 3) for k, v in iso.items()},
+         "guard_blocks_per_step": guard_blocks(d, d_cap) if form.startswith("mixed") else None,
+         "matrix_flops_per_segment": d.k1_matrix_flops()}
+    d.close()
+    return r
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -203,7 +249,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-forms", action="store_true", help="skip the FP32 form and its error report")
+    ap.add_argument("--no-forms", action="store_true", help="skip the other precision forms and their reports")
+    ap.add_argument("--form", default="mixed-fp16", choices=FORMS, help="precision form of the headline value")
     ap.add_argument("--workload", default="600f_28mhz", choices=sorted(WORKLOADS),
                     help="the capture a bench line is taken on (default: the 600-field 28 MHz workload)")
     ap.add_argument("--serving", action="store_true",
@@ -252,8 +299,9 @@ def main():
     log(f"generated {cap.size} samples in {time.perf_counter() - t:.1f} s")
     dev = torch.device("cuda:0")
     d_cap = torch.from_numpy(cap).to(dev)
-    dec = native.Decoder(mhz, args.batch, device=0)
+    dec = native.Decoder(mhz, args.batch, device=0, precision=args.form)
     fmt = dec.fmt
+    matrix_flops_per_seg = dec.k1_matrix_flops()
     fields = 0
     for _ in range(args.warmup):
         dec.reset()
@@ -280,6 +328,8 @@ def main():
     dec.decode_capture(d_cap)
     iso = dec.profile_read()
     dec.profile(False)
+    guard = guard_blocks(dec, d_cap)
+    dec.close()
     ms_per_step = dt * 1000.0 / args.steps
     value = fields * args.steps / dt
     bytes_per_field = alg_bytes_per_field(mhz)
@@ -322,7 +372,7 @@ def main():
     # N-point window, half for a real one): part A one D2Z and one Z2Z, part
     # B one D2Z and two Z2D per segment (fm_demod.cu:396-486)
     roofline_fp64 = None
-    if dom in ("k1_analytic", "k1_video"):
+    if dom in ("k1_analytic", "k1_video") and args.form == "fp64":
         from cuvhs_amd import native as _n
         segs = k1_samples / fmt.spfp / launches
         n_fft = _n.k1_fft_size(mhz)
@@ -334,35 +384,40 @@ def main():
                          "achieved_isolated": round(flops / iso_launch_s / 1e12, 2),
                          "frac_isolated": round(flops / iso_launch_s / 1e12 / PEAK_FP64_TFLOPS, 4)}
 
+    # the matrix cores' share of K1 part A in the mixed forms: executed MFMA
+    # flops (every hi/lo product, padding steps included) over the part's
+    # serialised stage time, against the dense f16 / bf16 peak
+    def mfma_use(per_seg, iso_ms):
+        if not per_seg or "k1_analytic" not in iso_ms:
+            return None
+        fl = per_seg * k1_samples / fmt.spfp
+        tf = fl / (iso_ms["k1_analytic"] / 1000.0) / 1e12
+        return {"kernel": "k1_part_a_mfma", "flops_per_segment": per_seg, "flops_per_step": fl,
+                "achieved_tflops": round(tf, 1), "peak_tflops": PEAK_F16_TFLOPS, "frac": round(tf / PEAK_F16_TFLOPS, 4),
+                "k1_analytic_ms_isolated": round(iso_ms["k1_analytic"], 3)}
+
+    from cuvhs_amd import precision
+    head_report = None
+    if args.form != "fp64" and not args.no_forms:
+        log(f"{args.form}: comparing with fp64 ...")
+        head_report = precision.compare(d_cap, mhz, args.batch, args.form)
     forms = None
     if not args.no_forms:
-        from cuvhs_amd import precision
-        d32 = native.Decoder(mhz, args.batch, device=0, precision="fp32")
-        for _ in range(args.warmup):
-            d32.reset()
-            d32.decode_capture(d_cap)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            d32.reset()
-            f32_fields = d32.decode_capture(d_cap)["fields"]
-        torch.cuda.synchronize()
-        dt32 = time.perf_counter() - t0
-        d32.profile(True, serial=True)
-        d32.reset()
-        d32.decode_capture(d_cap)
-        iso32 = d32.profile_read()
-        d32.close()
-        log(f"fp32 form: {f32_fields * args.steps / dt32:.1f} fields/s; comparing with fp64 ...")
-        report = precision.compare(d_cap, mhz, args.batch, "fp32")
-        forms = {"fp32": {"value": round(f32_fields * args.steps / dt32, 2), "unit": "fields/s",
-                          "ms_per_step": round(dt32 * 1000.0 / args.steps, 3), "dtype": "f32 (K1), f64 (rest)",
-                          "stage_ms_per_step_isolated": {k: round(v, 3) for k, v in iso32.items()},
-                          "vs_fp64": report}}
+        forms = {}
+        for form in FORMS:
+            if form == args.form:
+                continue
+            r = time_form(native, torch, d_cap, mhz, args, form)
+            log(f"{form} form: {r['value']:.1f} fields/s")
+            r["mfma"] = mfma_use(r.pop("matrix_flops_per_segment"), r["stage_ms_per_step_isolated"])
+            if form != "fp64":
+                log(f"{form}: comparing with fp64 ...")
+                r["vs_fp64"] = precision.compare(d_cap, mhz, args.batch, form)
+            forms[form] = r
 
     cli = None
     if not args.no_cli:
-        cli = cli_end_to_end(cap, mhz, args.batch)
+        cli = cli_end_to_end(cap, mhz, args.batch, args.form)
         log(f"cli end to end: {cli}")
 
     cpu = None
@@ -382,17 +437,22 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": DTYPES[args.form],
         "data": "synthetic",
         "config": {"workload": wl["desc"],
                    "fields_per_step": fields, "batch_fields": args.batch, "sample_rate_mhz": mhz,
-                   "precision": "fp64 (chroma pre-band-pass fp32)",
+                   "precision": args.form + ("" if args.form == "fp64" else " (chroma pre-band-pass fp32)"),
                    "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4"))},
         "hbm_gbs_algorithmic": round(value * bytes_per_field / 1e9, 2),
         "stage_ms_per_step": {k: round(v, 3) for k, v in stage_ms.items()},
         "stage_ms_per_step_isolated": {k: round(v, 3) for k, v in iso.items()},
         "roofline": roofline,
         "roofline_fp64": roofline_fp64,
+        "mfma": mfma_use(matrix_flops_per_seg, iso),
+        "guard_blocks_per_step": guard,
+        "vs_fp64": head_report,
+        "mixed_vs_fp32": (round(value / forms["fp32"]["value"], 3)
+                          if forms and "fp32" in forms and args.form.startswith("mixed") else None),
         "cpu_baseline": cpu,
         "cli_end_to_end": cli,
         "forms": forms,

@@ -13,7 +13,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "lib")
 
-HIP_SOURCES = ["k1_filters", "sync", "k2b", "k3", "k45", "k6_chroma", "k7_dropout", "context"]
+HIP_SOURCES = ["k1_filters", "k1_mfma", "guard", "sync", "k2b", "k3", "k45", "k6_chroma", "k7_dropout", "context"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wno-unused-result"]

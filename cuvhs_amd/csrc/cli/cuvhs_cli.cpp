@@ -2,7 +2,7 @@
 // Pipeline::run pipeline.cu:2626-2713) on top of the C ABI in include/cuvhs.h.
 //
 //   cuvhs [--system NTSC] [-f MHz] [--format u8|s16|u16] [--dc-correct]
-//         [--gpu id] [--batch F] [--precision fp64|fp32] [--overwrite]
+//         [--gpu id] [--batch F] [--precision fp64|fp32|mixed-fp16|mixed-bf16] [--overwrite]
 //         <input | -> <output_base>
 //
 // Writes output_base.tbc, output_base_chroma.tbc and output_base.tbc.json.
@@ -58,7 +58,8 @@ void usage(const char* prog) {
             "  --dc-correct          Subtract per-read DC offset after normalization\n"
             "  --gpu <id>            GPU device ID (default: 0, -1 = most memory)\n"
             "  --batch <fields>      Fields per chunk (default: 64)\n"
-            "  --precision <fp64|fp32> FM demodulation arithmetic (default: fp64)\n"
+            "  --precision <fp64|fp32|mixed-fp16|mixed-bf16>\n"
+            "                        decode arithmetic (default: fp64, the reference's)\n"
             "  --overwrite           Overwrite existing output files\n"
             "  -h, --help            Show this help\n",
             prog);
@@ -93,6 +94,8 @@ bool parse(int argc, char** argv, Args& a) {
             i++;
             if (!strcmp(argv[i], "fp64")) a.precision = CUVHS_PRECISION_FP64;
             else if (!strcmp(argv[i], "fp32")) a.precision = CUVHS_PRECISION_FP32;
+            else if (!strcmp(argv[i], "mixed-fp16")) a.precision = CUVHS_PRECISION_MIXED_F16;
+            else if (!strcmp(argv[i], "mixed-bf16")) a.precision = CUVHS_PRECISION_MIXED_BF16;
             else { fprintf(stderr, "Unknown precision: %s\n", argv[i]); return false; }
         } else if (!strcmp(argv[i], "--overwrite")) {
             a.overwrite = true;

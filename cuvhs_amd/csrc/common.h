@@ -94,14 +94,14 @@ struct ChunkCtl {
     int status;               // warning bits
     int vp_overflow;          // K2b capacity exceeded in some field
     int nevents;              // threshold-crossing events found
-    int pad_[2];
+    int guard_a, guard_b;     // FP64 guard-band blocks recomputed (K1 part A, part B)
     uint64_t raw_offset;
     uint64_t next_raw_offset;
     // diagnostic counters of the reference's debug logs, when enabled:
     // K3 bad/isolated spacing, large jump; K4 large delta, isolated jump,
     // refined sync-like; K5 out-of-bounds pixels, bad rows, sync-like pixels
     int diag[9];
-    int pad2_;
+    int guard_lines;          // lines K4 refined again on FP64 demod_05 (guard band)
 };
 
 // State carried from chunk to chunk; only the line-location, chroma and

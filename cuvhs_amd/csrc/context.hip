@@ -32,12 +32,26 @@
 
 using namespace cuvhs;
 
-constexpr int kStages = 12;
+constexpr int kStages = 13;
 // Chunk buffer sets in flight: the head of chunk c+1 may start once the tail
 // of chunk c+1-kSlots has finished, so with three the heads never wait for
 // the tail launched just before them.
 constexpr int kSlots = 3;
-constexpr int kMarks = 14;
+constexpr int kMarks = 15;
+// Guard band of the mixed forms.  A part-B block of the FP64 design is
+// recomputed in FP64 when a demod_05 output of the FP32 part B in its 256-
+// sample neighbourhood lies within delta Hz of the sync threshold, or when an
+// envelope tile in its input is below rho times the largest there (dropouts
+// and the chunk's zero-filled end, where the phase error grows as the
+// envelope falls).  Measured without the guard on the 600-field 28 MHz
+// capture (3.0e8 samples) against the FP64 form: the f16 form's demod_05
+// error is at most 1.55 Hz at every envelope ratio; the bf16 form's is at
+// most 4.0 Hz where the ratio is 0.5 or more and 28.7 Hz below.  delta is
+// about ten times the largest error where the block is not recomputed.
+constexpr double kGuardDeltaF16 = 16.0;
+constexpr float kGuardRhoF16 = 0.1f;
+constexpr double kGuardDeltaBf16 = 32.0;
+constexpr float kGuardRhoBf16 = 0.5f;
 
 // Buffers and control block of one chunk in flight.  Two slots alternate so
 // that the head stage of chunk c+1 (K1, sync events, VSYNC clustering) can
@@ -82,7 +96,24 @@ struct cuvhs_ctx {
     std::string err;
     K1Design k1;
     PreDesign pre;
-    int k1_f32 = 0;            // K1 transforms in FP32 (CUVHS_PRECISION_FP32)
+    int k1_f32 = 0;            // K1's FFT blocks in FP32 (FP32 form; part B of the mixed forms)
+    int mixed = 0, mixed_bf16 = 0;  // mixed forms: K1 part A on the matrix cores (f16 or bf16)
+    int sig_f32 = 0;           // demod, envelope (and rawdemod) stored as float
+    int d05_f32 = 0;           // demod_05 stored as float (FP32 form)
+    // mixed forms: part A tap operands, and the FP64 guard band
+    uint4* m_taps = nullptr;
+    int m_nc = 0, m_nx = 0, m_nblk = 0;
+    double m_scale = 0.0;
+    K1Design k1g;              // the FP64 design the guard band recomputes with
+    void *G_ga = nullptr, *G_gb = nullptr, *tw_ga = nullptr, *tw_gb = nullptr;
+    unsigned char* g_flag = nullptr;   // 256-sample flags of part-B outputs near the threshold
+    float2* g_env = nullptr;           // envelope range per tile of 1024 outputs
+    unsigned char *g_markA = nullptr, *g_markB = nullptr;
+    int2 *g_listA = nullptr, *g_listB = nullptr;
+    int* g_counts = nullptr;
+    double* g_raw64 = nullptr;         // FP64 rawdemod of the listed part-A blocks
+    double g_delta = 0.0;              // threshold band (Hz) of the FP32 part B
+    float g_rho = 0.0f;                // envelope ratio that marks an ill-conditioned phase
     void* G_a = nullptr;       // K1 block spectra and twiddles: double2, or float2 when k1_f32
     void* G_b = nullptr;
     void* G_p = nullptr;       // pre-band-pass spectrum: double2, or float2 when pre_f32
@@ -260,34 +291,65 @@ ChromaLaunch chroma_launch(cuvhs_ctx* c, const ChunkCtl* ctl, int nf_override, c
 
 // ctl (optional): the chunk's control block, initialised by K1's first
 // workgroup.  thr_bits (optional): receives the sync-threshold bitmask of d05.
+// Outputs go to the slot's demod / d05 / env in the form's storage types.
 int k1_launch(cuvhs_ctx* c, Slot& s, const void* chunk, long long n_valid, int nfk1, hipStream_t st,
               ChunkCtl* ctl = nullptr, unsigned int* thr_bits = nullptr) {
     const Fmt& f = c->fmt;
     long long T = (long long)nfk1 * f.spfp;
-    K1ALaunch a{};
-    a.cap = chunk;
-    a.in_fmt = c->cfg.input_format;
-    a.dc = c->cfg.dc_correct ? s.dc : nullptr;
-    a.n_valid = n_valid;
-    a.total = T;
-    a.spfp = f.spfp;
-    a.N = c->k1.N;
-    a.lm = c->k1.rf.lm;
-    a.lp = c->k1.rf.lp;
-    a.V = c->k1.rf.valid;
-    a.scale = f.sample_rate / (2.0 * 3.14159265358979323846);
-    a.G = c->G_a;
-    a.tw = c->tw_a;
-    a.rawdemod = c->rawdemod;
-    a.env = s.env;
-    a.ctl = ctl;
-    a.ctl_raw_offset = (long long)s.raw_offset;
-    a.ctl_fields_k1 = s.fk1;
-    a.thr_bits = thr_bits;
-    HIPCHK(c, run_k1_part_a(c->k1.rf.B, c->k1_f32, a, nfk1, st));
+    const double fs_tau = f.sample_rate / (2.0 * 3.14159265358979323846);
+    if (c->mixed) {
+        // part A on the matrix cores: FP32 rawdemod and envelope, and the
+        // envelope of each tile of 1024 outputs for the guard band
+        K1MLaunch m{};
+        m.cap = chunk;
+        m.n_valid = n_valid;
+        m.total = T;
+        m.spfp = f.spfp;
+        m.N = c->k1.N;
+        m.lp = c->k1.rf.lp;
+        m.nc = c->m_nc;
+        m.nx = c->m_nx;
+        m.a_lds_offset = mfma_a_lds_offset(c->m_nx);
+        m.scale = (float)c->m_scale;
+        m.fs_over_tau = (float)fs_tau;
+        m.A = c->m_taps;
+        m.rawdemod = reinterpret_cast<float*>(c->rawdemod);
+        m.env = reinterpret_cast<float*>(s.env);
+        m.ctl = ctl;
+        m.ctl_raw_offset = (long long)s.raw_offset;
+        m.ctl_fields_k1 = s.fk1;
+        m.thr_bits = thr_bits;
+        m.blk_env = c->g_env;
+        m.nblk = c->m_nblk;
+        HIPCHK(c, run_k1_part_a_mfma(c->mixed_bf16, m, nfk1, st));
+    } else {
+        K1ALaunch a{};
+        a.cap = chunk;
+        a.in_fmt = c->cfg.input_format;
+        a.dc = c->cfg.dc_correct ? s.dc : nullptr;
+        a.n_valid = n_valid;
+        a.total = T;
+        a.spfp = f.spfp;
+        a.N = c->k1.N;
+        a.lm = c->k1.rf.lm;
+        a.lp = c->k1.rf.lp;
+        a.V = c->k1.rf.valid;
+        a.scale = fs_tau;
+        a.G = c->G_a;
+        a.tw = c->tw_a;
+        a.rawdemod = c->rawdemod;
+        a.env = s.env;
+        a.out_f32 = c->sig_f32;
+        a.ctl = ctl;
+        a.ctl_raw_offset = (long long)s.raw_offset;
+        a.ctl_fields_k1 = s.fk1;
+        a.thr_bits = thr_bits;
+        HIPCHK(c, run_k1_part_a(c->k1.rf.B, c->k1_f32, a, nfk1, st));
+    }
     mark(c, s, 1, st);
     K1BLaunch b{};
     b.rawdemod = c->rawdemod;
+    b.in_f32 = c->sig_f32;
     b.total = T;
     b.spfp = f.spfp;
     b.N = c->k1.N;
@@ -297,16 +359,90 @@ int k1_launch(cuvhs_ctx* c, Slot& s, const void* chunk, long long n_valid, int n
     b.G = c->G_b;
     b.tw = c->tw_b;
     b.demod = s.demod;
+    b.demod_f32 = c->sig_f32;
     b.demod05 = s.d05;
+    b.d05_f32 = c->d05_f32;
     b.thr_bits = thr_bits;
     b.thr = f.pulse_threshold_hz;
+    if (c->mixed) {
+        b.gflag = c->g_flag;
+        b.gdelta = c->g_delta;
+    }
     HIPCHK(c, run_k1_part_b(c->k1.video.B, c->k1_f32, b, nfk1, st));
+    mark(c, s, 2, st);
+    if (c->mixed) {
+        // FP64 guard band: the blocks whose sync decisions the reduced
+        // precision could change, recomputed with the FP64 design
+        GuardLaunch g{};
+        g.gflag = c->g_flag;
+        g.nsub = ((T - 1) >> kGuardSubShift) + 1;
+        g.blk_env = c->g_env;
+        g.ntile = c->m_nblk * 16;
+        g.mvalid = mfma_valid_per_block();
+        g.rho = c->g_rho;
+        g.spfp = f.spfp;
+        g.nseg = nfk1;
+        g.N = c->k1g.N;
+        g.total = T;
+        g.VA = c->k1g.rf.valid;
+        g.nbA = (f.spfp + g.VA - 1) / g.VA;
+        g.VB = c->k1g.video.valid;
+        g.nbB = (f.spfp + g.VB - 1) / g.VB;
+        g.BB = c->k1g.video.B;
+        g.lpB = c->k1g.video.lp;
+        g.markA = c->g_markA;
+        g.markB = c->g_markB;
+        g.listA = c->g_listA;
+        g.listB = c->g_listB;
+        g.counts = c->g_counts;
+        g.ctl = ctl;
+        HIPCHK(c, run_guard_lists(g, st));
+        K1ALaunch a{};
+        a.cap = chunk;
+        a.in_fmt = c->cfg.input_format;
+        a.n_valid = n_valid;
+        a.total = T;
+        a.spfp = f.spfp;
+        a.N = c->k1g.N;
+        a.lm = c->k1g.rf.lm;
+        a.lp = c->k1g.rf.lp;
+        a.V = c->k1g.rf.valid;
+        a.scale = fs_tau;
+        a.G = c->G_ga;
+        a.tw = c->tw_ga;
+        a.rawdemod = c->g_raw64;
+        a.blist = c->g_listA;
+        a.bcount = c->g_counts;
+        HIPCHK(c, run_k1_part_a(c->k1g.rf.B, 0, a, nfk1, st));
+        K1BLaunch gb{};
+        gb.rawdemod = c->g_raw64;
+        gb.total = T;
+        gb.spfp = f.spfp;
+        gb.N = c->k1g.N;
+        gb.lm = c->k1g.video.lm;
+        gb.lp = c->k1g.video.lp;
+        gb.V = c->k1g.video.valid;
+        gb.G = c->G_gb;
+        gb.tw = c->tw_gb;
+        gb.demod = s.demod;
+        gb.demod_f32 = c->sig_f32;
+        gb.demod05 = s.d05;
+        gb.d05_f32 = c->d05_f32;
+        gb.thr_bits = thr_bits;
+        gb.thr = f.pulse_threshold_hz;
+        gb.blist = c->g_listB;
+        gb.bcount = c->g_counts + 1;
+        HIPCHK(c, run_k1_part_b(c->k1g.video.B, 0, gb, nfk1, st));
+    }
+    mark(c, s, 3, st);
     // beyond the demodulated region the buffers read as zero: only what an
     // earlier, longer chunk left there needs clearing
     if (s.k1_extent > T) {
-        HIPCHK(c, run_fill_f64(s.demod + T, s.k1_extent - T, 0.0, st));
-        HIPCHK(c, run_fill_f64(s.d05 + T, s.k1_extent - T, 0.0, st));
-        HIPCHK(c, run_fill_f64(s.env + T, s.k1_extent - T, 0.0, st));
+        const long long n = s.k1_extent - T;
+        const int ws = c->sig_f32 ? 1 : 2, wd = c->d05_f32 ? 1 : 2;   // 32-bit words per sample
+        HIPCHK(c, run_fill_zero(reinterpret_cast<unsigned int*>(s.demod) + T * ws, n * ws, st));
+        HIPCHK(c, run_fill_zero(reinterpret_cast<unsigned int*>(s.d05) + T * wd, n * wd, st));
+        HIPCHK(c, run_fill_zero(reinterpret_cast<unsigned int*>(s.env) + T * ws, n * ws, st));
     }
     s.k1_extent = T;
     return 0;
@@ -372,14 +508,13 @@ int head_stage(cuvhs_ctx* c, Slot& s, bool pre = true) {
     mark(c, s, 0, st);
     int r = k1_launch(c, s, s.chunk, (long long)s.n_read, s.fk1, st, s.ctl, c->thr_bits);
     if (r) return r;
-    mark(c, s, 2, st);
     const long long T = (long long)s.fk1 * f.spfp;
     HIPCHK(c, run_events_bits(c->thr_bits, T, s.wg_count, s.wg_off, s.ev_masks, s.events, s.ctl, st));
-    mark(c, s, 3, st);
+    mark(c, s, 4, st);
     // the kernel also writes the control block to the pinned host copy
     HIPCHK(c, run_vsync_cluster(s.events, s.ctl, T, f.vsync_min_w, f.vsync_max_w, s.fk1 * 20, c->cand, s.offsets,
                                 s.max_fields, f.samples_per_field, f.samples_per_line, s.fk1, 1, h, st));
-    mark(c, s, 4, st);
+    mark(c, s, 5, st);
     HIPCHK(c, hipEventRecord(s.head_done, st));
     // the pre-band-pass goes in after the head's launches: the head chain is
     // what the next chunk waits for
@@ -406,9 +541,9 @@ int tail_stage(cuvhs_ctx* c, Slot& s, const OutArrays* out, hipStream_t fork_str
     const Fmt& f = c->fmt;
     hipStream_t st = c->serial ? c->stream : c->tail;
     HIPCHK(c, hipStreamWaitEvent(st, s.head_done, 0));
-    mark(c, s, 5, st);
-    HIPCHK(c, run_sync_pulses(s.events, s.ctl, s.offsets, f.spfp, c->k2s, c->k2l, c->k2c, s.max_fields, 0, st));
     mark(c, s, 6, st);
+    HIPCHK(c, run_sync_pulses(s.events, s.ctl, s.offsets, f.spfp, c->k2s, c->k2l, c->k2c, s.max_fields, 0, st));
+    mark(c, s, 7, st);
     if (c->cfg.line_locator == CUVHS_LINELOC_K3) {
         // line_locs on the K2 pulses (pipeline.cu:1848-1856); K2b's cadence
         // state is left untouched, as in the reference
@@ -423,10 +558,10 @@ int tail_stage(cuvhs_ctx* c, Slot& s, const OutArrays* out, hipStream_t fork_str
         HIPCHK(c, hipMemcpyAsync(c->ll_coarse, c->ll, (size_t)s.max_fields * kLinesPerFrame * sizeof(double),
                                  hipMemcpyDeviceToDevice, st));
     }
-    mark(c, s, 7, st);
-    HIPCHK(c, run_hsync_refine(s.d05, c->ll, s.ctl, 0, s.max_fields, f, st));
-    if (c->diag) HIPCHK(c, run_diag_k4(s.d05, c->ll_coarse, c->ll, s.ctl, s.max_fields, f, st));
     mark(c, s, 8, st);
+    HIPCHK(c, run_hsync_refine(s.d05, c->d05_f32, c->ll, s.ctl, 0, s.max_fields, f, st));
+    if (c->diag) HIPCHK(c, run_diag_k4(s.d05, c->ll_coarse, c->ll, s.ctl, s.max_fields, f, st));
+    mark(c, s, 9, st);
     // outputs go straight to the caller's arrays where given (the field count
     // n was checked against their capacity), else to the context's
     uint16_t* luma = out && out->luma ? out->luma : c->luma;
@@ -447,28 +582,30 @@ int tail_stage(cuvhs_ctx* c, Slot& s, const OutArrays* out, hipStream_t fork_str
         HIPCHK(c, hipEventRecord(s.k4_done, st));
         HIPCHK(c, hipStreamWaitEvent(ax, s.k4_done, 0));
     }
-    HIPCHK(c, run_tbc_resample(s.demod, c->ll, luma, s.ctl, 0, s.max_fields, f, ax));
+    HIPCHK(c, run_tbc_resample(s.demod, c->sig_f32, c->ll, luma, s.ctl, 0, s.max_fields, f, ax));
     if (c->diag) HIPCHK(c, run_diag_k5(s.demod, c->ll, s.ctl, s.max_fields, f, st));
-    mark(c, s, 9, st);
+    mark(c, s, 10, st);
     if (fork) {
-        HIPCHK(c, run_dropout_classes(s.env, c->ll, s.ctl, 0, s.max_fields, f, c->do_partial, c->do_cls, ax));
+        HIPCHK(c, run_dropout_classes(s.env, c->sig_f32, c->ll, s.ctl, 0, s.max_fields, f, c->do_partial, c->do_cls,
+                                      ax));
         HIPCHK(c, hipEventRecord(s.aux_done, ax));
     }
     HIPCHK(c, hipStreamWaitEvent(st, s.pre1, 0));
-    mark(c, s, 10, st);
+    mark(c, s, 11, st);
     ChromaLaunch cl = chroma_launch(c, s.ctl, 0, s.craw);
     cl.out = chroma;
     HIPCHK(c, run_chroma(cl, c->state, s.max_fields, c->m0, c->m1, c->cs_tab, c->iq0, c->track, phase, 1, st));
-    mark(c, s, 11, st);
+    mark(c, s, 12, st);
     if (fork) {
         HIPCHK(c, hipStreamWaitEvent(st, s.aux_done, 0));
     } else {
-        HIPCHK(c, run_dropout_classes(s.env, c->ll, s.ctl, 0, s.max_fields, f, c->do_partial, c->do_cls, st));
+        HIPCHK(c, run_dropout_classes(s.env, c->sig_f32, c->ll, s.ctl, 0, s.max_fields, f, c->do_partial, c->do_cls,
+                                      st));
     }
     HIPCHK(c, run_dropout_scan(c->do_cls, c->ll, s.ctl, 0, s.max_fields, f, luma, chroma, dol, dos, doe, doc, st));
-    mark(c, s, 12, st);
-    HIPCHK(c, run_field_meta(s.ctl, c->state, s.offsets, c->isf_k2b, floc, isf, st));
     mark(c, s, 13, st);
+    HIPCHK(c, run_field_meta(s.ctl, c->state, s.offsets, c->isf_k2b, floc, isf, st));
+    mark(c, s, 14, st);
     HIPCHK(c, hipMemcpyAsync(s.hctl, s.ctl, sizeof(ChunkCtl), hipMemcpyDeviceToHost, st));
     HIPCHK(c, hipEventRecord(s.tail_done, st));
     return 0;
@@ -481,7 +618,7 @@ void collect_profile(cuvhs_ctx* c, Slot& s) {
     c->work_chunks++;
     c->work_fields += s.hctl->nf;
     for (int i = 0; i < kStages; i++) {
-        int a = i < 4 ? i : i + 1;
+        int a = i < 5 ? i : i + 1;   // marks 0-5 head, 6-14 tail
         float ms = 0.0f;
         if (hipEventElapsedTime(&ms, s.mark[a], s.mark[a + 1]) == hipSuccess) c->stage_ms[i] += ms;
     }
@@ -623,7 +760,12 @@ int cuvhs_format_for(double mhz, cuvhs_format* o) {
 int cuvhs_create(const cuvhs_config* cfg, cuvhs_ctx** out) {
     if (!cfg || !out) return CUVHS_E_ARG;
     *out = nullptr;
-    if (cfg->system != CUVHS_SYSTEM_NTSC || (cfg->precision != CUVHS_PRECISION_FP64 && cfg->precision != CUVHS_PRECISION_FP32))
+    if (cfg->system != CUVHS_SYSTEM_NTSC || cfg->precision < CUVHS_PRECISION_FP64 ||
+        cfg->precision > CUVHS_PRECISION_MIXED_BF16)
+        return CUVHS_E_UNSUPPORTED;
+    // the matrix-core part A takes u8 samples, exact in f16 / bf16 after
+    // removing 128 (a per-read DC or 16-bit samples would need splitting too)
+    if (cfg->precision >= CUVHS_PRECISION_MIXED_F16 && (cfg->input_format != CUVHS_INPUT_U8 || cfg->dc_correct))
         return CUVHS_E_UNSUPPORTED;
     if (!(cfg->sample_rate_mhz >= 16.0 && cfg->sample_rate_mhz <= 40.0)) return CUVHS_E_UNSUPPORTED;
     if (cfg->input_format < CUVHS_INPUT_U8 || cfg->input_format > CUVHS_INPUT_U16) return CUVHS_E_UNSUPPORTED;
@@ -680,13 +822,24 @@ int cuvhs_create(const cuvhs_config* cfg, cuvhs_ctx** out) {
         // taps below 1e-13 (FP64) or 1e-8 (FP32) of the largest are dropped:
         // both far below the rounding of the transform they feed
         const int bk1[2] = {4096, 8192};
-        c->k1_f32 = cfg->precision == CUVHS_PRECISION_FP32;
+        c->mixed = cfg->precision >= CUVHS_PRECISION_MIXED_F16;
+        c->mixed_bf16 = cfg->precision == CUVHS_PRECISION_MIXED_BF16;
+        c->k1_f32 = cfg->precision != CUVHS_PRECISION_FP64;
+        c->sig_f32 = cfg->precision != CUVHS_PRECISION_FP64;
+        c->d05_f32 = cfg->precision == CUVHS_PRECISION_FP32;
         c->k1 = c->k1_f32 ? design_k1(f, 1e-8, bk1, 2, 1, 8) : design_k1(f, 1e-13, bk1, 2, 1, 16);
+        if (c->mixed) {
+            c->k1g = design_k1(f, 1e-13, bk1, 2, 1, 16);
+            // sync-threshold band of the FP32 part B and the envelope ratio
+            // below which a tile's phase is treated as ill-conditioned
+            c->g_delta = c->mixed_bf16 ? kGuardDeltaBf16 : kGuardDeltaF16;
+            c->g_rho = c->mixed_bf16 ? kGuardRhoBf16 : kGuardRhoF16;
+        }
         const int bpre[2] = {8192, 16384};
         // the pre-band-pass: FP64 with taps to 1e-9 of the largest (its
         // truncation is then ~1e-6 of the chroma band's level even when the
         // band holds only quantisation noise), or FP32 with taps to 1e-7
-        c->pre_f32 = cfg->precision == CUVHS_PRECISION_FP32;
+        c->pre_f32 = cfg->precision != CUVHS_PRECISION_FP64;
         c->pre = c->pre_f32 ? design_prebpf(f, 1e-7, bpre, 2) : design_prebpf(f, 1e-9, bpre, 2);
     } catch (...) {
         cuvhs_destroy(c);
@@ -706,6 +859,24 @@ int cuvhs_create(const cuvhs_config* cfg, cuvhs_ctx** out) {
             if ((e = upload_spectrum<double2, double>(c, (double2**)&c->G_a, c->k1.rf.spectrum, pa)) != hipSuccess) return bad(e, "G_a");
             if ((e = upload_spectrum<double2, double>(c, (double2**)&c->G_b, c->k1.video.spectrum, pb)) != hipSuccess) return bad(e, "G_b");
         }
+    }
+    if (c->mixed) {
+        if ((e = upload_spectrum<double2, double>(c, (double2**)&c->G_ga, c->k1g.rf.spectrum, true)) != hipSuccess)
+            return bad(e, "G_ga");
+        if ((e = upload_spectrum<double2, double>(c, (double2**)&c->G_gb, c->k1g.video.spectrum, true)) != hipSuccess)
+            return bad(e, "G_gb");
+        if ((e = upload_twiddles(c, &c->tw_ga, c->k1g.rf.B, 0)) != hipSuccess) return bad(e, "twiddles");
+        if ((e = upload_twiddles(c, &c->tw_gb, c->k1g.video.B, 0)) != hipSuccess) return bad(e, "twiddles");
+        std::vector<uint4> taps;
+        c->m_nc = build_mfma_taps(c->k1.rf.taps, c->k1.rf.lm, c->k1.rf.lp, c->mixed_bf16, taps, &c->m_scale);
+        c->m_scale *= 256.0;   // u8 sample minus 128, times 256 (raw_reader.cpp:71-76)
+        c->m_nx = mfma_input_elems(c->m_nc);
+        c->m_nblk = mfma_blocks_per_segment(f.spfp);
+        // the window never wraps past the segment's FFT length for kept outputs
+        if (c->k1.rf.lm > c->k1.N - f.spfp - kOverlap) return bad(hipErrorInvalidValue, "MFMA part A span");
+        if ((e = dalloc(c, &c->m_taps, taps.size())) != hipSuccess) return bad(e, "taps");
+        if ((e = hipMemcpy(c->m_taps, taps.data(), taps.size() * sizeof(uint4), hipMemcpyHostToDevice)) != hipSuccess)
+            return bad(e, "taps");
     }
     if (c->pre_f32) e = upload_spectrum<float2, float>(c, (float2**)&c->G_p, c->pre.bpf.spectrum, true);
     else e = upload_spectrum<double2, double>(c, (double2**)&c->G_p, c->pre.bpf.spectrum, true);
@@ -742,6 +913,18 @@ int cuvhs_create(const cuvhs_config* cfg, cuvhs_ctx** out) {
     const size_t Fz = (size_t)c->F;
     c->buf_n = S + f.spfp;  // one segment of zero padding past the chunk
     ALLOC(c->rawdemod, S);
+    if (c->mixed) {
+        const int nbA = (f.spfp + c->k1g.rf.valid - 1) / c->k1g.rf.valid;
+        const int nbB = (f.spfp + c->k1g.video.valid - 1) / c->k1g.video.valid;
+        ALLOC(c->g_flag, ((S - 1) >> kGuardSubShift) + 1);   // zeroed here, then by each compaction
+        ALLOC(c->g_env, Fz * c->m_nblk * 16);
+        ALLOC(c->g_markA, Fz * nbA);
+        ALLOC(c->g_markB, Fz * nbB);
+        ALLOC(c->g_listA, Fz * nbA);
+        ALLOC(c->g_listB, Fz * nbB);
+        ALLOC(c->g_counts, 4);
+        ALLOC(c->g_raw64, S);
+    }
     ALLOC(c->cand, 20 * c->F);
     ALLOC(c->thr_bits, thr_bits_words(S));
     for (Slot& s : c->slot) {
@@ -871,6 +1054,8 @@ int cuvhs_decode_chunk(cuvhs_ctx* c, const void* chunk, uint64_t avail, uint64_t
     info->vsync_candidates = h.vsync_count;
     info->status = h.status;
     info->next_raw_offset = h.next_raw_offset;
+    info->guard_blocks_a = h.guard_a;
+    info->guard_blocks_b = h.guard_b;
     if (h.nf > 0) {
         r = tail_stage(c, s, nullptr, c->side);
         if (r) { sync_all(c); return r; }
@@ -965,7 +1150,7 @@ int cuvhs_decode_capture(cuvhs_ctx* c, const void* cap, uint64_t cap_len, int ma
 }
 
 static const char* kStageNames[kStages + 1] = {
-    "k1_analytic", "k1_video", "sync_events", "vsync_cluster", "k2_pulses", "k2b_linelocs",
+    "k1_analytic", "k1_video", "k1_guard", "sync_events", "vsync_cluster", "k2_pulses", "k2b_linelocs",
     "k4_hsync_refine", "k5_tbc_resample", "k6_prebpf_wait", "k6_chroma", "k7_dropouts", "field_meta",
     "k6_prebpf"};
 
@@ -990,6 +1175,8 @@ int cuvhs_profile(cuvhs_ctx* c, int enable) {
 
 int cuvhs_diagnostics(cuvhs_ctx* c, int enable) {
     if (!c) return CUVHS_E_ARG;
+    // the counters read the FP64 signals of the reference's arithmetic
+    if (enable && c->cfg.precision != CUVHS_PRECISION_FP64) return CUVHS_E_UNSUPPORTED;
     int r = sync_all(c);
     if (r) return r;
     c->diag = enable != 0;
@@ -1025,6 +1212,14 @@ int cuvhs_profile_work(const cuvhs_ctx* c, uint64_t* samples, int* chunks, int* 
     return 0;
 }
 
+int cuvhs_k1_matrix_flops(const cuvhs_ctx* c, double* per_segment) {
+    if (!c || !per_segment) return CUVHS_E_ARG;
+    // workgroups per segment x 4 waves x steps x 16 products (4 tiles x
+    // real/imaginary x hi/lo) x 2 x 32 x 32 x 16
+    *per_segment = c->mixed ? (double)mfma_blocks_per_segment(c->fmt.spfp) * 4.0 * c->m_nc * 16.0 * 32768.0 : 0.0;
+    return 0;
+}
+
 int cuvhs_k1_blocks(const cuvhs_ctx* c, int* block_a, int* valid_a, int* block_b, int* valid_b) {
     if (!c) return CUVHS_E_ARG;
     if (block_a) *block_a = c->k1.rf.B;
@@ -1053,13 +1248,15 @@ int cuvhs_fm_demod(cuvhs_ctx* c, const void* raw, uint64_t n_valid, int nf, doub
         HIPCHK(c, run_input_mean(raw, c->cfg.input_format, (long long)n_valid, s.dc_partial, s.dc, c->stream));
     const bool prof = c->prof;
     c->prof = false;
-    int r = k1_launch(c, s, raw, (long long)n_valid, nf, c->stream);
+    // the mixed forms' guard band needs the threshold bitmask (its FP64
+    // blocks rewrite their bits); the context's bitmask buffer serves
+    int r = k1_launch(c, s, raw, (long long)n_valid, nf, c->stream, nullptr, c->mixed ? c->thr_bits : nullptr);
     c->prof = prof;
     if (r) return r;
-    size_t n = (size_t)nf * c->fmt.spfp * sizeof(double);
-    HIPCHK(c, hipMemcpyAsync(demod, s.demod, n, hipMemcpyDeviceToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(d05, s.d05, n, hipMemcpyDeviceToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(env, s.env, n, hipMemcpyDeviceToDevice, c->stream));
+    const long long n = (long long)nf * c->fmt.spfp;
+    HIPCHK(c, run_to_f64(s.demod, c->sig_f32, demod, n, c->stream));
+    HIPCHK(c, run_to_f64(s.d05, c->d05_f32, d05, n, c->stream));
+    HIPCHK(c, run_to_f64(s.env, c->sig_f32, env, n, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return 0;
 }
@@ -1116,14 +1313,14 @@ int cuvhs_linelocs(cuvhs_ctx* c, uint64_t raw_offset, const uint64_t* offsets, i
 
 int cuvhs_hsync_refine(cuvhs_ctx* c, const double* d05, int total, double* ll, int nf) {
     if (!c || !d05 || !ll || nf < 1 || total != nf * c->fmt.spfp) return CUVHS_E_ARG;
-    HIPCHK(c, run_hsync_refine(d05, ll, c->slot[0].ctl, nf, nf, c->fmt, c->stream));
+    HIPCHK(c, run_hsync_refine(d05, 0, ll, c->slot[0].ctl, nf, nf, c->fmt, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return 0;
 }
 
 int cuvhs_tbc_resample(cuvhs_ctx* c, const double* demod, int total, const double* ll, int nf, uint16_t* luma) {
     if (!c || !demod || !ll || !luma || nf < 1 || total != nf * c->fmt.spfp) return CUVHS_E_ARG;
-    HIPCHK(c, run_tbc_resample(demod, ll, luma, c->slot[0].ctl, nf, nf, c->fmt, c->stream));
+    HIPCHK(c, run_tbc_resample(demod, 0, ll, luma, c->slot[0].ctl, nf, nf, c->fmt, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return 0;
 }

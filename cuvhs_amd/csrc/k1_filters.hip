@@ -98,19 +98,27 @@ constexpr int k1_min_blocks() { return B == 4096 ? (sizeof(T) == 4 ? 4 : 3) : 1;
 __device__ __forceinline__ double phase_of(double y, double x) { return atan2_f64(y, x); }
 __device__ __forceinline__ float phase_of(float y, float x) { return atan2f(y, x); }
 
-// Part A: analytic signal, envelope and instantaneous frequency.  T is the
-// transform precision: double (the reference's FP64 arithmetic) or float
-// (the FP32 form); outputs are stored as double either way.
+// Stores of K1 outputs in the storage type the launch selects.
+__device__ __forceinline__ void store_sig(void* p, int f32, long long i, double v) {
+    if (f32) static_cast<float*>(p)[i] = (float)v;
+    else static_cast<double*>(p)[i] = v;
+}
+__device__ __forceinline__ double load_sig(const void* p, int f32, long long i) {
+    return f32 ? (double)static_cast<const float*>(p)[i] : static_cast<const double*>(p)[i];
+}
+
+// Part A: analytic signal, envelope and instantaneous frequency of block b of
+// segment f.  T is the transform precision: double (the reference's FP64
+// arithmetic) or float (the FP32 form); outputs are stored as the launch
+// selects (double or float).
 template <int B, typename T>
-__global__ __launch_bounds__(B / 16, (k1_min_blocks<B, T>())) void k1_part_a(K1ALaunch a) {
+__device__ __forceinline__ void k1_part_a_block(const K1ALaunch& a, int f, int b) {
     using V2 = typename C2<T>::type;
     extern __shared__ __align__(16) unsigned char smem[];
     V2* buf = reinterpret_cast<V2*>(smem);
     const V2* G = static_cast<const V2*>(a.G);
     const V2* tw = static_cast<const V2*>(a.tw);
     constexpr int NT = B / 16;
-    const int f = blockIdx.y;
-    const int b = blockIdx.x;
     if (a.ctl && f == 0 && b == 0 && threadIdx.x == 0) {
         // the chunk's control block starts here: no host-to-device copy
         ChunkCtl z{};
@@ -170,8 +178,9 @@ __global__ __launch_bounds__(B / 16, (k1_min_blocks<B, T>())) void k1_part_a(K1A
             if (o >= 0 && o <= a.V) {
                 ang[r] = phase_of(v[r].y, v[r].x);
                 int q = qs + o;
-                if (q >= kOverlap && q < qend)
-                    a.env[f * a.spfp + (q - kOverlap)] = (double)sqrt(v[r].x * v[r].x + v[r].y * v[r].y);
+                if (a.env && q >= kOverlap && q < qend)
+                    store_sig(a.env, a.out_f32, (long long)f * a.spfp + (q - kOverlap),
+                              (double)sqrt(v[r].x * v[r].x + v[r].y * v[r].y));
             }
         }
         // the phase at o - 1 is held by lane - 1 of this wave, or for lane 0
@@ -191,24 +200,40 @@ __global__ __launch_bounds__(B / 16, (k1_min_blocks<B, T>())) void k1_part_a(K1A
                 T delta = ang[r] - prev;
                 if (delta > PI) delta -= TAU;
                 if (delta < -PI) delta += TAU;
-                a.rawdemod[f * a.spfp + (q - kOverlap)] = (double)delta * a.scale;
+                store_sig(a.rawdemod, a.out_f32, (long long)f * a.spfp + (q - kOverlap), (double)delta * a.scale);
             }
         }
+    }
+}
+
+template <int B, typename T>
+__global__ __launch_bounds__(B / 16, (k1_min_blocks<B, T>())) void k1_part_a(K1ALaunch a) {
+    k1_part_a_block<B, T>(a, blockIdx.y, blockIdx.x);
+}
+
+// Block-list mode (the guard band's FP64 recompute): every gridDim.x-th
+// listed block.  A kernel of its own, so the loop's hoisted values do not
+// weigh on the registers of the full launch.
+template <int B, typename T>
+__global__ __launch_bounds__(B / 16) void k1_part_a_list(K1ALaunch a) {
+    const int n = *a.bcount;
+    for (int i = blockIdx.x; i < n; i += gridDim.x) {
+        const int2 fb = a.blist[i];
+        k1_part_a_block<B, T>(a, fb.x, fb.y);
+        __syncthreads();
     }
 }
 
 // Part B: video and sync filters in one complex transform
 // (real input, kernel g_video + i * g_sync -> real part demod, imag demod_05).
 template <int B, typename T>
-__global__ __launch_bounds__(B / 16, (k1_min_blocks<B, T>())) void k1_part_b(K1BLaunch a) {
+__device__ __forceinline__ void k1_part_b_block(const K1BLaunch& a, int f, int b) {
     using V2 = typename C2<T>::type;
     extern __shared__ __align__(16) unsigned char smem[];
     V2* buf = reinterpret_cast<V2*>(smem);
     const V2* G = static_cast<const V2*>(a.G);
     const V2* tw = static_cast<const V2*>(a.tw);
     constexpr int NT = B / 16;
-    const int f = blockIdx.y;
-    const int b = blockIdx.x;
     const int qs = kOverlap + b * a.V;
     const int qend = kOverlap + a.spfp;
     if (qs >= qend) return;
@@ -219,7 +244,7 @@ __global__ __launch_bounds__(B / 16, (k1_min_blocks<B, T>())) void k1_part_b(K1B
     for (int i = 0; i < 16; i++) {
         int s = seg0 + wrap_n(qs - a.lp + (int)threadIdx.x + i * NT, a.N);
         s = s < 0 ? 0 : (s > last ? last : s);
-        xin[i] = (T)a.rawdemod[s];
+        xin[i] = (T)load_sig(a.rawdemod, a.in_f32, s);
     }
     V2 v[16];
 #pragma unroll
@@ -236,9 +261,12 @@ __global__ __launch_bounds__(B / 16, (k1_min_blocks<B, T>())) void k1_part_b(K1B
             int o = r * NT + (int)threadIdx.x - a.lp;
             int q = qs + o;
             if (o >= 0 && o < a.V && q < qend) {
-                int s = f * a.spfp + (q - kOverlap);
-                a.demod[s] = (double)v[r].x;
-                a.demod05[s] = (double)v[r].y;
+                long long s = (long long)f * a.spfp + (q - kOverlap);
+                store_sig(a.demod, a.demod_f32, s, (double)v[r].x);
+                store_sig(a.demod05, a.d05_f32, s, (double)v[r].y);
+                // the guard band's flag of the output's 256 samples: close
+                // enough to the sync threshold that its decision needs FP64
+                if (a.gflag && fabs((double)v[r].y - a.thr) < a.gdelta) a.gflag[s >> kGuardSubShift] = 1;
             }
         }
         if (a.thr_bits) {
@@ -248,7 +276,9 @@ __global__ __launch_bounds__(B / 16, (k1_min_blocks<B, T>())) void k1_part_b(K1B
             // ORed into the (up to three) words they straddle.  Words inside
             // the range are then stored whole; the first and last, shared
             // with the neighbouring blocks when partly covered, are ORed
-            // into the bitmask part A cleared.
+            // into the bitmask part A cleared (in block-list mode, where the
+            // words hold earlier bits, the block's own bits are cleared
+            // first and the neighbours' kept).
             constexpr int kWords = B / 32 + 4;
             __shared__ unsigned int s_bits[kWords];
             const int lane = (int)threadIdx.x & 63;
@@ -257,6 +287,7 @@ __global__ __launch_bounds__(B / 16, (k1_min_blocks<B, T>())) void k1_part_b(K1B
             const long long w_lo = s_lo >> 5, w_hi = (s_end - 1) >> 5;
             const int nw = (int)(w_hi - w_lo) + 1;
             const int off = (int)(s_lo & 31);
+            __syncthreads();
             for (int i = (int)threadIdx.x; i < kWords; i += NT) s_bits[i] = 0u;
             __syncthreads();
 #pragma unroll
@@ -278,11 +309,38 @@ __global__ __launch_bounds__(B / 16, (k1_min_blocks<B, T>())) void k1_part_b(K1B
             __syncthreads();
             for (int i = (int)threadIdx.x; i < nw; i += NT) {
                 const unsigned int word = s_bits[i];
-                const bool partial = (i == 0 && off != 0) || (i == nw - 1 && (s_end & 31) != 0);
-                if (!partial) a.thr_bits[w_lo + i] = word;
-                else if (word) atomicOr(a.thr_bits + (w_lo + i), word);
+                const int lo_bit = i == 0 ? off : 0;
+                const int hi_bit = (i == nw - 1 && (s_end & 31) != 0) ? (int)(s_end & 31) : 32;
+                const bool partial = lo_bit != 0 || hi_bit != 32;
+                if (!partial) {
+                    a.thr_bits[w_lo + i] = word;
+                } else if (a.blist) {
+                    const unsigned int own = (hi_bit == 32 ? ~0u : ((1u << hi_bit) - 1u)) & ~((1u << lo_bit) - 1u);
+                    atomicAnd(a.thr_bits + (w_lo + i), ~own);
+                    if (word) atomicOr(a.thr_bits + (w_lo + i), word & own);
+                } else if (word) {
+                    atomicOr(a.thr_bits + (w_lo + i), word);
+                }
             }
         }
+    }
+}
+
+template <int B, typename T>
+__global__ __launch_bounds__(B / 16, (k1_min_blocks<B, T>())) void k1_part_b(K1BLaunch a) {
+    k1_part_b_block<B, T>(a, blockIdx.y, blockIdx.x);
+}
+
+// Block-list mode (the guard band's FP64 recompute): every gridDim.x-th
+// listed block.  A kernel of its own, so the loop's hoisted values do not
+// weigh on the registers of the full launch.
+template <int B, typename T>
+__global__ __launch_bounds__(B / 16) void k1_part_b_list(K1BLaunch a) {
+    const int n = *a.bcount;
+    for (int i = blockIdx.x; i < n; i += gridDim.x) {
+        const int2 fb = a.blist[i];
+        k1_part_b_block<B, T>(a, fb.x, fb.y);
+        __syncthreads();
     }
 }
 
@@ -329,6 +387,12 @@ __global__ __launch_bounds__(B / 16) void k_prebpf(PreLaunch a) {
     }
 }
 
+__global__ void k_fill_zero(unsigned int* p, long long n) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long st = (long long)gridDim.x * blockDim.x;
+    for (; i < n; i += st) p[i] = 0u;
+}
+
 __global__ void k_fill_f64(double* p, long long n, double v) {
     long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     long long st = (long long)gridDim.x * blockDim.x;
@@ -349,20 +413,44 @@ static void ensure_lds(K* kern, size_t sh) {
     (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
     done.fetch_or(bit);
 }
+// Grid of a block-list launch: the workgroups the device holds at once
+// (each loops over the list), kListGrid if the query fails.
+template <typename K>
+static int list_grid(K kern, int threads, size_t sh) {
+    int dev = 0, ncu = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, threads, sh) != hipSuccess || ncu < 1 ||
+        per_cu < 1)
+        return kListGrid;
+    return ncu * per_cu;
+}
 template <int B, typename T>
 static hipError_t launch_a(const K1ALaunch& a, int nf, hipStream_t st) {
     int nb = (int)((a.spfp + a.V - 1) / a.V);
     size_t sh = k1_lds_bytes<B, T>();
-    ensure_lds(k1_part_a<B, T>, sh);
-    hipLaunchKernelGGL((k1_part_a<B, T>), dim3(nb, nf), dim3(B / 16), sh, st, a);
+    if (a.blist) {
+        ensure_lds(k1_part_a_list<B, T>, sh);
+        const int grid = list_grid(k1_part_a_list<B, T>, B / 16, sh);
+        hipLaunchKernelGGL((k1_part_a_list<B, T>), dim3(grid), dim3(B / 16), sh, st, a);
+    } else {
+        ensure_lds(k1_part_a<B, T>, sh);
+        hipLaunchKernelGGL((k1_part_a<B, T>), dim3(nb, nf), dim3(B / 16), sh, st, a);
+    }
     return hipGetLastError();
 }
 template <int B, typename T>
 static hipError_t launch_b(const K1BLaunch& a, int nf, hipStream_t st) {
     int nb = (int)((a.spfp + a.V - 1) / a.V);
     size_t sh = k1_lds_bytes<B, T>();
-    ensure_lds(k1_part_b<B, T>, sh);
-    hipLaunchKernelGGL((k1_part_b<B, T>), dim3(nb, nf), dim3(B / 16), sh, st, a);
+    if (a.blist) {
+        ensure_lds(k1_part_b_list<B, T>, sh);
+        const int grid = list_grid(k1_part_b_list<B, T>, B / 16, sh);
+        hipLaunchKernelGGL((k1_part_b_list<B, T>), dim3(grid), dim3(B / 16), sh, st, a);
+    } else {
+        ensure_lds(k1_part_b<B, T>, sh);
+        hipLaunchKernelGGL((k1_part_b<B, T>), dim3(nb, nf), dim3(B / 16), sh, st, a);
+    }
     return hipGetLastError();
 }
 template <int B, typename T>
@@ -402,6 +490,25 @@ hipError_t run_input_mean(const void* cap, int in_fmt, long long n, double* part
     hipLaunchKernelGGL(k_input_mean, dim3(1), dim3(kMeanBlocks), 0, st, partial, n, mean);
     return hipGetLastError();
 }
+__global__ void k_to_f64(const float* src, double* dst, long long n) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long st = (long long)gridDim.x * blockDim.x;
+    for (; i < n; i += st) dst[i] = (double)src[i];
+}
+
+hipError_t run_to_f64(const void* src, int f32, double* dst, long long n, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    if (!f32) return hipMemcpyAsync(dst, src, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, st);
+    hipLaunchKernelGGL(k_to_f64, dim3(1024), dim3(256), 0, st, static_cast<const float*>(src), dst, n);
+    return hipGetLastError();
+}
+
+hipError_t run_fill_zero(void* p, long long words, hipStream_t st) {
+    if (words <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_zero, dim3(1024), dim3(256), 0, st, static_cast<unsigned int*>(p), words);
+    return hipGetLastError();
+}
+
 hipError_t run_fill_f64(double* p, long long n, double v, hipStream_t st) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_fill_f64, dim3(1024), dim3(256), 0, st, p, n, v);

@@ -14,17 +14,18 @@ namespace cuvhs {
 
 // Wavefront helpers: all 64 lanes of a wavefront cooperate on one line; every
 // lane returns the same value.
-__device__ __forceinline__ double wv_find_falling(const double* sig, int start, int count, double target, int total,
+template <typename S>
+__device__ __forceinline__ double wv_find_falling(const S* sig, int start, int count, double target, int total,
                                                   int lane) {
     if (start < 0 || start + count > total || count < 2) return -1.0;
     for (int b = start + 1; b < start + count; b += 64) {
         int i = b + lane;
         bool c = false;
-        if (i < start + count) c = sig[i - 1] >= target && sig[i] < target;
+        if (i < start + count) c = (double)sig[i - 1] >= target && (double)sig[i] < target;
         unsigned long long m = __ballot(c);
         if (m) {
             int f = b + __ffsll((long long)m) - 1;   // first crossing (hsync_refine.cu:29-39)
-            double a = sig[f - 1] - target, bb = sig[f] - target;
+            double a = (double)sig[f - 1] - target, bb = (double)sig[f] - target;
             double frac = (a - bb != 0.0) ? (a / (a - bb)) : 0.0;
             return (double)(f - 1) + frac;
         }
@@ -32,18 +33,19 @@ __device__ __forceinline__ double wv_find_falling(const double* sig, int start, 
     return -1.0;
 }
 
-__device__ __forceinline__ double wv_find_rising(const double* sig, int start, int count, double target, int total,
+template <typename S>
+__device__ __forceinline__ double wv_find_rising(const S* sig, int start, int count, double target, int total,
                                                  int lane) {
     if (start < 0 || start + count > total || count < 2) return -1.0;
-    if (sig[start] > target) return -1.0;
+    if ((double)sig[start] > target) return -1.0;
     for (int b = start + 1; b < start + count; b += 64) {
         int i = b + lane;
         bool c = false;
-        if (i < start + count) c = sig[i - 1] < target && sig[i] >= target;
+        if (i < start + count) c = (double)sig[i - 1] < target && (double)sig[i] >= target;
         unsigned long long m = __ballot(c);
         if (m) {
             int f = b + __ffsll((long long)m) - 1;   // hsync_refine.cu:57-67
-            double a = target - sig[f - 1], bb = target - sig[f];
+            double a = target - (double)sig[f - 1], bb = target - (double)sig[f];
             double frac = (a + bb != 0.0) ? (a / (a - bb)) : 0.0;
             return (double)(f - 1) + frac;
         }
@@ -54,12 +56,13 @@ __device__ __forceinline__ double wv_find_rising(const double* sig, int start, i
 // The n/2-th smallest of up to 64 samples (hsync_refine.cu:72-97): lane j
 // holds sample j and counts its stable rank against the window read back
 // from the wavefront's LDS slots as broadcasts, two samples per read.
-__device__ __forceinline__ double wv_window_median(const double* sig, int start, int count, int total, int lane,
+template <typename S>
+__device__ __forceinline__ double wv_window_median(const S* sig, int start, int count, int total, int lane,
                                                    double* wl) {
     if (start < 0 || start + count > total || count <= 0) return 0.0;
     const int n = count < 64 ? count : 64;
     const int mid = n / 2;
-    double v = lane < n ? sig[start + lane] : 0.0;
+    double v = lane < n ? (double)sig[start + lane] : 0.0;
     wl[lane] = v;
     wave_sync();
     int r = 0;
@@ -74,8 +77,11 @@ __device__ __forceinline__ double wv_window_median(const double* sig, int start,
     return res;
 }
 
-// One wavefront per line (hsync_refine.cu:99-239).
-__global__ __launch_bounds__(256) void k_hsync_refine(const double* __restrict__ d05, double* __restrict__ ll,
+// One wavefront per line (hsync_refine.cu:99-239).  S: storage type of
+// demod_05 (double, or float in the reduced-precision forms); the crossings,
+// levels and positions are computed in FP64.
+template <typename S>
+__global__ __launch_bounds__(256) void k_hsync_refine(const S* __restrict__ d05, double* __restrict__ ll,
                                                       const ChunkCtl* ctl, int nf_override, int total_per_field,
                                                       double thr, double one_usec_samples, int als, double hw,
                                                       double right_edge_offset) {
@@ -136,8 +142,11 @@ __global__ __launch_bounds__(256) void k_hsync_refine(const double* __restrict__
 }
 
 // One thread per output pixel; consecutive threads walk a TBC row, so the
-// four demod gathers of neighbouring pixels overlap in cache.
-__global__ __launch_bounds__(256) void k_tbc_resample(const double* __restrict__ demod, const double* __restrict__ ll,
+// four demod gathers of neighbouring pixels overlap in cache.  S: storage
+// and interpolation type of demod (double, or float in the reduced-precision
+// forms); the sampling coordinate is FP64 (positions reach 46 M samples).
+template <typename S>
+__global__ __launch_bounds__(256) void k_tbc_resample(const S* __restrict__ demod, const double* __restrict__ ll,
                                                       uint16_t* __restrict__ luma, const ChunkCtl* ctl,
                                                       int nf_override, int total_per_field, double ire0,
                                                       double hz_ire, double vsync_ire, double out_zero,
@@ -163,33 +172,47 @@ __global__ __launch_bounds__(256) void k_tbc_resample(const double* __restrict__
         luma[idx] = (uint16_t)out_zero;
         return;
     }
-    double p0 = demod[ci - 1], p1 = demod[ci], p2 = demod[ci + 1], p3 = demod[ci + 2];
-    double a = p2 - p0;
-    double b = 2.0 * p0 - 5.0 * p1 + 4.0 * p2 - p3;
-    double c = 3.0 * (p1 - p2) + p3 - p0;
-    double hz = p1 + 0.5 * x * (a + x * (b + x * c));
-    double ire = (hz - ire0) / hz_ire - vsync_ire;
-    double v = ire * out_scale + out_zero;
-    if (v < 0.0) v = 0.0;
-    if (v > 65535.0) v = 65535.0;
-    luma[idx] = (uint16_t)(v + 0.5);
+    const S p0 = demod[ci - 1], p1 = demod[ci], p2 = demod[ci + 1], p3 = demod[ci + 2];
+    const S xs = (S)x;
+    const S a = p2 - p0;
+    const S b = (S)2 * p0 - (S)5 * p1 + (S)4 * p2 - p3;
+    const S c = (S)3 * (p1 - p2) + p3 - p0;
+    const S hz = p1 + (S)0.5 * xs * (a + xs * (b + xs * c));
+    const S ire = (hz - (S)ire0) / (S)hz_ire - (S)vsync_ire;
+    S v = ire * (S)out_scale + (S)out_zero;
+    if (v < (S)0) v = (S)0;
+    if (v > (S)65535) v = (S)65535;
+    luma[idx] = (uint16_t)(v + (S)0.5);
 }
 
-hipError_t run_hsync_refine(const double* d05, double* ll, const ChunkCtl* ctl, int nf_override, int max_fields,
-                            const Fmt& fmt, hipStream_t st) {
+hipError_t run_hsync_refine(const void* d05, int f32, double* ll, const ChunkCtl* ctl, int nf_override,
+                            int max_fields, const Fmt& fmt, hipStream_t st) {
     int n = max_fields * kLinesPerFrame;
     double ous = 1.0e-6 * fmt.sample_rate;
     double reo = 2.25 * ((fmt.sample_rate / 1.0e6) / 40.0);
-    hipLaunchKernelGGL(k_hsync_refine, dim3((n + 3) / 4), dim3(256), 0, st, d05, ll, ctl, nf_override, fmt.spfp,
-                       fmt.pulse_threshold_hz, ous, fmt.active_line_start, fmt.hsync_width, reo);
+    if (f32)
+        hipLaunchKernelGGL(k_hsync_refine<float>, dim3((n + 3) / 4), dim3(256), 0, st, static_cast<const float*>(d05),
+                           ll, ctl, nf_override, fmt.spfp, fmt.pulse_threshold_hz, ous, fmt.active_line_start,
+                           fmt.hsync_width, reo);
+    else
+        hipLaunchKernelGGL(k_hsync_refine<double>, dim3((n + 3) / 4), dim3(256), 0, st,
+                           static_cast<const double*>(d05), ll, ctl, nf_override, fmt.spfp, fmt.pulse_threshold_hz,
+                           ous, fmt.active_line_start, fmt.hsync_width, reo);
     return hipGetLastError();
 }
 
-hipError_t run_tbc_resample(const double* demod, const double* ll, uint16_t* luma, const ChunkCtl* ctl,
+hipError_t run_tbc_resample(const void* demod, int f32, const double* ll, uint16_t* luma, const ChunkCtl* ctl,
                             int nf_override, int max_fields, const Fmt& fmt, hipStream_t st) {
     long long n = (long long)max_fields * kFieldLines * kLineLen;
-    hipLaunchKernelGGL(k_tbc_resample, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, demod, ll, luma, ctl,
-                       nf_override, fmt.spfp, fmt.ire0, fmt.hz_ire, fmt.vsync_ire, fmt.output_zero, fmt.output_scale);
+    const dim3 grid((unsigned)((n + 255) / 256));
+    if (f32)
+        hipLaunchKernelGGL(k_tbc_resample<float>, grid, dim3(256), 0, st, static_cast<const float*>(demod), ll, luma,
+                           ctl, nf_override, fmt.spfp, fmt.ire0, fmt.hz_ire, fmt.vsync_ire, fmt.output_zero,
+                           fmt.output_scale);
+    else
+        hipLaunchKernelGGL(k_tbc_resample<double>, grid, dim3(256), 0, st, static_cast<const double*>(demod), ll,
+                           luma, ctl, nf_override, fmt.spfp, fmt.ire0, fmt.hz_ire, fmt.vsync_ire, fmt.output_zero,
+                           fmt.output_scale);
     return hipGetLastError();
 }
 

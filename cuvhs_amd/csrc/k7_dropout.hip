@@ -59,7 +59,10 @@ __device__ __forceinline__ DoWindow do_window(const double* linelocs, int f, int
 
 // Partial sums of the field's envelope over [f*spfp, (f+1)*spfp), only for
 // fields with a non-empty scan window (the mean is unused otherwise).
-__global__ __launch_bounds__(kDoThreads) void k_do_sum(const double* __restrict__ env, const double* __restrict__ linelocs,
+// S: storage type of the envelope (double, or float in the reduced-precision
+// forms); sums are FP64.
+template <typename S>
+__global__ __launch_bounds__(kDoThreads) void k_do_sum(const S* __restrict__ env, const double* __restrict__ linelocs,
                                                         const ChunkCtl* ctl, int nf_override, int spfp, int als,
                                                         double* partial) {
     __shared__ double red[kDoThreads];
@@ -70,9 +73,9 @@ __global__ __launch_bounds__(kDoThreads) void k_do_sum(const double* __restrict_
     if (w.end <= w.start) return;
     const int L = (spfp + kSumSlices - 1) / kSumSlices;
     const int a = sl * L, b = min(spfp, a + L);
-    const double* fe = env + (size_t)f * spfp;
+    const S* fe = env + (size_t)f * spfp;
     double s = 0.0;
-    for (int i = a + t; i < b; i += kDoThreads) s += fe[i];
+    for (int i = a + t; i < b; i += kDoThreads) s += (double)fe[i];
     red[t] = s;
     __syncthreads();
     for (int o = kDoThreads / 2; o > 0; o >>= 1) {
@@ -89,7 +92,8 @@ __device__ __forceinline__ double field_mean(const double* partial, int f, int s
 }
 
 // 16-sample block means vs the hysteresis thresholds: 1 = <= down, 2 = >= up.
-__global__ __launch_bounds__(kDoThreads) void k_do_class(const double* __restrict__ env, const double* __restrict__ linelocs,
+template <typename S>
+__global__ __launch_bounds__(kDoThreads) void k_do_class(const S* __restrict__ env, const double* __restrict__ linelocs,
                                                           const ChunkCtl* ctl, int nf_override, int spfp, int als,
                                                           const double* partial, unsigned char* cls) {
     const int f = blockIdx.x, sl = blockIdx.y;
@@ -113,11 +117,11 @@ __global__ __launch_bounds__(kDoThreads) void k_do_class(const double* __restric
             // full block: all loads issued together, summed in sample order
             double e[kEnvBlock];
 #pragma unroll
-            for (int i = 0; i < kEnvBlock; i++) e[i] = env[bs + i];
+            for (int i = 0; i < kEnvBlock; i++) e[i] = (double)env[bs + i];
 #pragma unroll
             for (int i = 0; i < kEnvBlock; i++) bsum += e[i];
         } else {
-            for (int i = bs; i < be; i++) bsum += env[i];
+            for (int i = bs; i < be; i++) bsum += (double)env[i];
         }
         double bm = bsum / (double)(be - bs);
         cls[(size_t)f * kMaxClassBlocks + b] = bm <= down ? 1 : (bm >= up ? 2 : 0);
@@ -301,13 +305,23 @@ __global__ void k_field_meta(const ChunkCtl* ctl, DecodeState* dst, const unsign
 size_t dropout_class_bytes(int max_fields) { return (size_t)max_fields * kMaxClassBlocks; }
 size_t dropout_partial_count(int max_fields) { return (size_t)max_fields * kSumSlices; }
 
-hipError_t run_dropout_classes(const double* env, const double* ll, const ChunkCtl* ctl, int nf_override,
+hipError_t run_dropout_classes(const void* env, int f32, const double* ll, const ChunkCtl* ctl, int nf_override,
                                int max_fields, const Fmt& fmt, double* partial, unsigned char* cls, hipStream_t st) {
     if ((fmt.spfp + kEnvBlock - 1) / kEnvBlock > kMaxClassBlocks) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_do_sum, dim3(max_fields, kSumSlices), dim3(kDoThreads), 0, st, env, ll, ctl, nf_override,
-                       fmt.spfp, fmt.active_line_start, partial);
-    hipLaunchKernelGGL(k_do_class, dim3(max_fields, kClassSlices), dim3(kDoThreads), 0, st, env, ll, ctl, nf_override,
-                       fmt.spfp, fmt.active_line_start, partial, cls);
+    const dim3 gs(max_fields, kSumSlices), gc(max_fields, kClassSlices);
+    if (f32) {
+        const float* e = static_cast<const float*>(env);
+        hipLaunchKernelGGL(k_do_sum<float>, gs, dim3(kDoThreads), 0, st, e, ll, ctl, nf_override, fmt.spfp,
+                           fmt.active_line_start, partial);
+        hipLaunchKernelGGL(k_do_class<float>, gc, dim3(kDoThreads), 0, st, e, ll, ctl, nf_override, fmt.spfp,
+                           fmt.active_line_start, partial, cls);
+    } else {
+        const double* e = static_cast<const double*>(env);
+        hipLaunchKernelGGL(k_do_sum<double>, gs, dim3(kDoThreads), 0, st, e, ll, ctl, nf_override, fmt.spfp,
+                           fmt.active_line_start, partial);
+        hipLaunchKernelGGL(k_do_class<double>, gc, dim3(kDoThreads), 0, st, e, ll, ctl, nf_override, fmt.spfp,
+                           fmt.active_line_start, partial, cls);
+    }
     return hipGetLastError();
 }
 
@@ -322,7 +336,7 @@ hipError_t run_dropout_scan(const unsigned char* cls, const double* ll, const Ch
 hipError_t run_dropouts(const double* env, const double* ll, const ChunkCtl* ctl, int nf_override, int max_fields,
                         const Fmt& fmt, double* partial, unsigned char* cls, uint16_t* luma, uint16_t* chroma,
                         int* lines, int* starts, int* ends, int* count, hipStream_t st) {
-    hipError_t e = run_dropout_classes(env, ll, ctl, nf_override, max_fields, fmt, partial, cls, st);
+    hipError_t e = run_dropout_classes(env, 0, ll, ctl, nf_override, max_fields, fmt, partial, cls, st);
     if (e != hipSuccess) return e;
     return run_dropout_scan(cls, ll, ctl, nf_override, max_fields, fmt, luma, chroma, lines, starts, ends, count, st);
 }

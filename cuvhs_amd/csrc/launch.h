@@ -3,6 +3,9 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <complex>
+#include <vector>
+
 #include "common.h"
 
 namespace cuvhs {
@@ -19,23 +22,75 @@ struct K1ALaunch {
     double scale;
     const void* G;              // block spectrum, double2 or float2 (the transform precision)
     const void* tw;             // twiddles W_B^m, same type
-    double* rawdemod;
-    double* env;
+    void* rawdemod;             // double, or float with out_f32
+    void* env;                  // same type as rawdemod; may be null (not stored)
+    int out_f32;
     ChunkCtl* ctl;              // if set, initialised by the first workgroup:
     long long ctl_raw_offset;   //   zeros, fields_k1 and raw_offset
     int ctl_fields_k1;
     unsigned int* thr_bits;     // if set, zeroed over [0, total) for part B
+    const int2* blist;          // if set: only the blocks (segment, block) blist[i], i < *bcount
+    const int* bcount;
+};
+// K1 part A on the matrix cores (mixed-precision forms, u8 input): see
+// k1_mfma.hip.  Outputs are FP32.
+struct K1MLaunch {
+    const void* cap;            // u8 capture samples
+    long long n_valid, total;
+    int spfp, N, lp;
+    int nc, nx;                 // steps of 16 taps (whole groups), staged input elements
+    int a_lds_offset;           // byte offset of the tap operands in LDS (mfma_a_lds_offset)
+    float scale;                // 256 (u8 scale) / tap scale
+    float fs_over_tau;          // sample rate / 2 pi
+    const uint4* A;             // tap operands [nc][4][64] x 16 B
+    float* rawdemod;
+    float* env;
+    ChunkCtl* ctl;              // as in K1ALaunch
+    long long ctl_raw_offset;
+    int ctl_fields_k1;
+    unsigned int* thr_bits;
+    float2* blk_env;            // if set: (min, max) envelope of each tile of 1024
+    int nblk;                   //   outputs, [field][nblk blocks][16 tiles]
 };
 struct K1BLaunch {
-    const double* rawdemod;
+    const void* rawdemod;       // double, or float with in_f32
+    int in_f32;
     long long total;
     int spfp, N, lm, lp, V;
     const void* G;              // as in K1ALaunch
     const void* tw;
-    double* demod;
-    double* demod05;
+    void* demod;                // double, or float with demod_f32
+    int demod_f32;
+    void* demod05;              // double, or float with d05_f32
+    int d05_f32;
     unsigned int* thr_bits;     // if set: bit s = (demod05[s] <= thr), s < total
     double thr;
+    const int2* blist;          // as in K1ALaunch; the block's bitmask bits are
+    const int* bcount;          // rewritten, its neighbours' bits kept
+    unsigned char* gflag;       // if set: byte s >> kGuardSubShift of the chunk
+    double gdelta;              // set to 1 when output s lies within gdelta of
+                                // thr (never cleared here: the guard's
+                                // compaction clears the flags it read)
+};
+// The guard band's block selection (guard.hip).
+// Samples per guard flag of the reduced-precision part B: 256.
+constexpr int kGuardSubShift = 8;
+struct GuardLaunch {
+    unsigned char* gflag;       // part-B threshold flags, one per 256 samples of
+    long long nsub;             //   the chunk (nsub of them), cleared once read
+    const float2* blk_env;      // MFMA part-A envelope tiles [segment][ntile], or null
+    int ntile, mvalid;          // tiles per segment; samples per MFMA block
+    float rho;                  // a tile below rho x the largest around flags the block
+    int spfp, nseg, N;
+    long long total;
+    int VA, nbA;                // FP64 part A: outputs per block, blocks per segment
+    int VB, nbB, BB, lpB;       // FP64 part B: outputs, blocks, block points, causal taps
+    unsigned char* markA;       // [segment][nbA], [segment][nbB]: cleared by the compaction
+    unsigned char* markB;
+    int2* listA;                // listed blocks (segment, block)
+    int2* listB;
+    int* counts;                // [0] part A, [1] part B
+    ChunkCtl* ctl;              // if set: receives the counts (guard_a, guard_b)
 };
 struct PreLaunch {
     const void* cap;
@@ -70,7 +125,26 @@ struct ChromaLaunch {
 hipError_t run_k1_part_a(int B, int f32, const K1ALaunch& a, int nf, hipStream_t st);
 hipError_t run_k1_part_b(int B, int f32, const K1BLaunch& a, int nf, hipStream_t st);
 hipError_t run_prebpf(int B, int f32, const PreLaunch& a, int nf, hipStream_t st);
+// K1 part A on the matrix cores (bf16: bf16 operands, else f16)
+hipError_t run_k1_part_a_mfma(int bf16, const K1MLaunch& a, int nf, hipStream_t st);
+int mfma_blocks_per_segment(int spfp);
+int mfma_valid_per_block();
+int mfma_input_elems(int nc);
+int mfma_a_lds_offset(int nx);
+// A operands of the MFMA part A for an impulse response taps[m + lm], m =
+// -lm..lp; returns the step count, *scale the tap scale to undo
+int build_mfma_taps(const std::vector<std::complex<double>>& taps, int lm, int lp, int bf16, std::vector<uint4>& out,
+                    double* scale);
+// n samples of a float (f32) or double signal into a double array
+hipError_t run_to_f64(const void* src, int f32, double* dst, long long n, hipStream_t st);
+// the guard band's FP64 block lists (marks, compaction, counts)
+hipError_t run_guard_lists(const GuardLaunch& g, hipStream_t st);
 hipError_t run_fill_f64(double* p, long long n, double v, hipStream_t st);
+// zero n 32-bit words (a fill kernel, not a memset: safe under graph capture)
+hipError_t run_fill_zero(void* p, long long words, hipStream_t st);
+// grid size of the block-list launches when the device's residency cannot be
+// queried (each workgroup loops over the list)
+constexpr int kListGrid = 1024;
 hipError_t run_k3(const ChunkCtl* ctl, int nf_override, int max_fields, const Fmt& fmt, const int* starts,
                   const int* lens, const int* count, int* types, double* linelocs, int* is_first, hipStream_t st);
 hipError_t run_diag_spacing(const double* ll, ChunkCtl* ctl, int max_fields, const Fmt& fmt, hipStream_t st);
@@ -105,9 +179,10 @@ hipError_t run_k2b(const ChunkCtl* ctl_dev, ChunkCtl* ctl_mut, DecodeState* dst,
                    hipStream_t st);
 size_t k2b_field_summary_bytes();
 size_t k2b_anchor_bytes();
-hipError_t run_hsync_refine(const double* d05, double* ll, const ChunkCtl* ctl, int nf_override, int max_fields,
-                            const Fmt& fmt, hipStream_t st);
-hipError_t run_tbc_resample(const double* demod, const double* ll, uint16_t* luma, const ChunkCtl* ctl,
+// f32: the signal is stored as float (reduced-precision forms), else double
+hipError_t run_hsync_refine(const void* d05, int f32, double* ll, const ChunkCtl* ctl, int nf_override,
+                            int max_fields, const Fmt& fmt, hipStream_t st);
+hipError_t run_tbc_resample(const void* demod, int f32, const double* ll, uint16_t* luma, const ChunkCtl* ctl,
                             int nf_override, int max_fields, const Fmt& fmt, hipStream_t st);
 hipError_t run_chroma(ChromaLaunch a, DecodeState* dst, int max_fields, double* m0, double* m1,
                       const double* cs_tab, double* iq0, int* out_track, int* phase_ids, int need_iq,
@@ -120,7 +195,7 @@ hipError_t run_dropouts(const double* env, const double* ll, const ChunkCtl* ctl
 // the two halves of run_dropouts: envelope sums and block classes (need only
 // env and line locations), then the scan and concealment (luma and chroma
 // written first)
-hipError_t run_dropout_classes(const double* env, const double* ll, const ChunkCtl* ctl, int nf_override,
+hipError_t run_dropout_classes(const void* env, int f32, const double* ll, const ChunkCtl* ctl, int nf_override,
                                int max_fields, const Fmt& fmt, double* partial, unsigned char* cls, hipStream_t st);
 hipError_t run_dropout_scan(const unsigned char* cls, const double* ll, const ChunkCtl* ctl, int nf_override,
                             int max_fields, const Fmt& fmt, uint16_t* luma, uint16_t* chroma, int* lines, int* starts,

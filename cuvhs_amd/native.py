@@ -19,7 +19,7 @@ E_ARG, E_HIP, E_ALLOC, E_UNSUPPORTED, E_CAPACITY = -1, -2, -3, -4, -5
 INPUT_U8, INPUT_S16, INPUT_U16 = 0, 1, 2
 INPUT_FORMATS = {"u8": INPUT_U8, "s16": INPUT_S16, "u16": INPUT_U16}
 LINE_LOCATORS = {"k2b": 0, "k3": 1}
-PRECISIONS = {"fp64": 0, "fp32": 1}
+PRECISIONS = {"fp64": 0, "fp32": 1, "mixed-fp16": 2, "mixed-bf16": 3}
 DIAG_NAMES = ("k3_bad_spacing", "k3_isolated_spacing", "k3_large_jump", "k4_large_delta", "k4_isolated_jump",
               "k4_refined_sync_like", "k5_oob_pixels", "k5_bad_geom_lines", "k5_sync_like_pixels")
 W_VSYNC_OVERFLOW, W_NO_VSYNC, W_EOF = 1, 2, 4
@@ -48,7 +48,7 @@ class Format(ctypes.Structure):
 
 class ChunkInfo(ctypes.Structure):
     _fields_ = [("fields", _i), ("fields_k1", _i), ("vsync_candidates", _i), ("status", _i),
-                ("raw_offset", _u64), ("next_raw_offset", _u64)]
+                ("raw_offset", _u64), ("next_raw_offset", _u64), ("guard_blocks_a", _i), ("guard_blocks_b", _i)]
 
 
 class ChunkOutputs(ctypes.Structure):
@@ -90,6 +90,7 @@ _SIGS = {
     "cuvhs_profile": (_i, [_vp, _i]),
     "cuvhs_profile_read": (_i, [_vp, _vp, _vp, _i]),
     "cuvhs_profile_work": (_i, [_vp, ctypes.POINTER(_u64), ctypes.POINTER(_i), ctypes.POINTER(_i)]),
+    "cuvhs_k1_matrix_flops": (_i, [_vp, ctypes.POINTER(_d)]),
     "cuvhs_k1_blocks": (_i, [_vp, ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_i)]),
 }
 
@@ -273,6 +274,12 @@ class Decoder:
               "profile_work")
         return smp.value, ch.value, fl.value
 
+    def k1_matrix_flops(self):
+        """Matrix-core flops of K1 part A per segment (mixed forms; else 0)."""
+        v = _d()
+        check(lib().cuvhs_k1_matrix_flops(self.h, ctypes.byref(v)), self.h, "k1_matrix_flops")
+        return v.value
+
     def k1_blocks(self):
         """K1 block shapes: (block_a, valid_a, block_b, valid_b)."""
         v = [_i() for _ in range(4)]
@@ -305,6 +312,7 @@ class Decoder:
             infos.append({"fields": info.fields, "fields_k1": info.fields_k1, "raw_offset": info.raw_offset,
                           "next_raw_offset": info.next_raw_offset, "status": info.status,
                           "vsync_candidates": info.vsync_candidates,
+                          "guard_blocks": (info.guard_blocks_a, info.guard_blocks_b),
                           "diag": list(self.diagnostics_read()[1].values()) if info.fields else None})
             if info.fields == 0:
                 break

@@ -3,8 +3,12 @@ form on the same capture.
 
 The FP64 form reproduces the reference's arithmetic (cuFFT FP64 in
 fm_demod.cu:396-486) and is the one checked against the oracle.  A reduced
-form (CUVHS_PRECISION_FP32: the K1 transforms, phase and envelope in FP32)
-changes only K1; this module measures what that does downstream:
+form changes only K1 (and the chroma pre-band-pass, FP32 in every reduced
+form): CUVHS_PRECISION_FP32 runs the K1 transforms, phase and envelope in
+FP32; the mixed forms (CUVHS_PRECISION_MIXED_F16 / _BF16) run K1 part A on
+the matrix cores and part B in FP32, with an FP64 guard band that recomputes
+every block whose sync-threshold decisions the reduced arithmetic could
+change.  This module measures what that does downstream:
 
 * K1 outputs chunk by chunk (demod, demod_05, envelope): max and RMS
   absolute error over the samples the capture covers, up to TAIL_GUARD

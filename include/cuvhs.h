@@ -52,11 +52,21 @@ enum {
 };
 
 enum { CUVHS_SYSTEM_NTSC = 0 };
-/* Arithmetic of the K1 FM-demodulation transforms (fm_demod.cu, cuFFT FP64
- * in the reference).  FP64: the reference's precision, the default.  FP32:
- * K1's block transforms, phase and envelope in FP32 (its outputs are stored
- * as FP64; every later stage is unchanged), the plain FP32 form. */
-enum { CUVHS_PRECISION_FP64 = 0, CUVHS_PRECISION_FP32 = 1 };
+/* Arithmetic of the decode (the reference computes in FP64 with cuFFT,
+ * fm_demod.cu:396-486, chroma_decode.cu:568-635).
+ * FP64: the reference's precision, the default.
+ * FP32: the plain FP32 form: K1's transforms, phase and envelope, the chroma
+ *   pre-band-pass and K5 in FP32, demod / demod_05 / envelope stored as
+ *   float; FP64 only in line locations and K4's positions.
+ * MIXED_F16 / MIXED_BF16: K1's RF band-pass and Hilbert filter as Toeplitz
+ *   products on the matrix cores (u8 samples exact in f16 / bf16, taps split
+ *   hi + lo, FP32 accumulation), K1's video filters and the rest as the FP32
+ *   form except demod_05, kept in FP64; an FP64 guard band recomputes the K1
+ *   blocks whose sync-threshold decisions the reduced precision could change
+ *   (outputs near the threshold, envelope dips).  u8 input without
+ *   --dc-correct only (else CUVHS_E_UNSUPPORTED). */
+enum { CUVHS_PRECISION_FP64 = 0, CUVHS_PRECISION_FP32 = 1, CUVHS_PRECISION_MIXED_F16 = 2,
+       CUVHS_PRECISION_MIXED_BF16 = 3 };
 /* Capture sample formats (InputFormat, video_format.h; RawReader::convert,
  * raw_reader.cpp:71-91): u8 -> (s - 128) * 256, s16 as is, u16 -> s - 32768.
  * 16-bit samples are in host byte order. */
@@ -112,6 +122,8 @@ typedef struct cuvhs_chunk_info {
     int status;               /* warning bits for this chunk */
     uint64_t raw_offset;      /* chunk start in the capture (samples) */
     uint64_t next_raw_offset; /* start of the next chunk */
+    int guard_blocks_a;       /* mixed forms: K1 part-A / part-B blocks the FP64 */
+    int guard_blocks_b;       /*   guard band recomputed (0 in the other forms) */
 } cuvhs_chunk_info;
 
 /* Device pointers to the last chunk's outputs, owned by the context and valid
@@ -160,7 +172,8 @@ int cuvhs_decode_capture(cuvhs_ctx* ctx, const void* d_capture, uint64_t capture
  * reference's chunk buffers. */
 
 /* fm_demod (fm_demod.h:46-53): one chunk read of the context's input format
- * (n_valid samples, zero past it). */
+ * (n_valid samples, zero past it), in the context's precision form (outputs
+ * widened to FP64). */
 int cuvhs_fm_demod(cuvhs_ctx* ctx, const void* d_raw, uint64_t n_valid, int nfields, double* d_demod,
                    double* d_demod05, double* d_env);
 /* discover_vsyncs (vsync_discover.h:6-11): ascending candidates, returns the
@@ -239,6 +252,11 @@ int cuvhs_profile_work(const cuvhs_ctx* ctx, uint64_t* samples, int* chunks, int
  * filters).  Any pointer may be NULL.  (No counterpart in the reference,
  * which transforms whole segments, fm_demod.cu:396-486.) */
 int cuvhs_k1_blocks(const cuvhs_ctx* ctx, int* block_a, int* valid_a, int* block_b, int* valid_b);
+/* Floating-point operations the matrix cores execute in K1 part A per
+ * demodulated segment (mixed forms: every 32x32x16 product of the hi/lo tap
+ * operands, padding steps included; 0 in the FFT forms), for the MFMA
+ * utilisation figure.  (No counterpart in the reference.) */
+int cuvhs_k1_matrix_flops(const cuvhs_ctx* ctx, double* per_segment);
 
 /* The reference's debug-log counters (CUVHS_DEBUG_K3_LOG, _K4_LOG, _K5_LOG:
  * line_locs.cu:585-653, hsync_refine.cu:273-349, tbc_resample.cu:75-116),
@@ -246,7 +264,8 @@ int cuvhs_k1_blocks(const cuvhs_ctx* ctx, int* block_a, int* valid_a, int* block
  * K3 bad_spacing, isolated_spacing, large_jump; K4 large_delta,
  * isolated_jump, refined_sync_like; K5 oob_pixels, bad_geom_lines,
  * sync_like_pixels.  cuvhs_diagnostics_read fills totals since enabling and
- * the last chunk's values (either may be NULL) and returns 9. */
+ * the last chunk's values (either may be NULL) and returns 9.  FP64 form
+ * only (CUVHS_E_UNSUPPORTED otherwise). */
 int cuvhs_diagnostics(cuvhs_ctx* ctx, int enable);
 int cuvhs_diagnostics_read(const cuvhs_ctx* ctx, int64_t* totals, int32_t* last_chunk);
 

@@ -171,13 +171,13 @@ def cli_end_to_end(cap, mhz, batch, form):
 
 def serving_main(args):
     """Serving form: 16 requests at 4 req/s (Poisson), sample rates uniform in
-    {28, 35.8, 40} MHz, lengths log-uniform in 30..600 fields, sliced from one
-    600-field synthetic capture per rate resident in HBM; 4 workers with one
-    context per rate each, 64 GB output budget."""
+    {28, 35.8, 40} MHz, lengths log-uniform in 30..3000 fields, sliced from
+    one 3000-field synthetic capture per rate resident in HBM (5.2 GB); 4
+    workers with one context per rate each, 64 GB output budget."""
     import torch
     from cuvhs_amd import serving
     t = time.perf_counter()
-    pool_fields = 600
+    pool_fields = 3000
     pool = serving.make_pool(pool_fields)
     log(f"generated the capture pool in {time.perf_counter() - t:.1f} s")
     d_pool = {m: torch.from_numpy(a).to("cuda:0") for m, a in pool.items()}
@@ -200,7 +200,7 @@ def serving_main(args):
         "n_gpus": 1,
         "higher_is_better": True,
         "data": "synthetic",
-        "config": {"workload": "16 requests, Poisson 4 req/s, fs in {28, 35.8, 40} MHz, 30..600 fields log-uniform",
+        "config": {"workload": "16 requests, Poisson 4 req/s, fs in {28, 35.8, 40} MHz, 30..3000 fields log-uniform",
                    "workers": 4, "batch_fields": 64, "hbm_budget_gb": 64},
         "forms": res,
     }

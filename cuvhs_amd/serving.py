@@ -56,9 +56,10 @@ def make_requests(n, rate_hz, min_fields, max_fields, pool_fields, seed=4):
     return reqs
 
 
-def run(d_pool, reqs, precision="fp64", workers=4, batch=64, hbm_budget_gb=64.0, device=0):
+def run(d_pool, reqs, precision="fp64", workers=4, batch=64, hbm_budget_gb=64.0, device=0, keep_outputs=False):
     """Serve reqs (arrival times in seconds from now) against the device pool
-    {mhz: uint8 cuda tensor}; returns the per-request records."""
+    {mhz: uint8 cuda tensor}; returns the per-request records (with the
+    decode's output tensors under "out" when keep_outputs)."""
     ctx = [{m: N.Decoder(m, batch, device=device, precision=precision) for m in RATES} for _ in range(workers)]
     budget = hbm_budget_gb * 1e9
     lock = threading.Condition()
@@ -88,6 +89,8 @@ def run(d_pool, reqs, precision="fp64", workers=4, batch=64, hbm_budget_gb=64.0,
                 out = dec.decode_capture(cap, max_fields=r["fields"] + 2)
                 r["end"] = time.perf_counter() - t0
                 r["decoded"] = out["fields"]
+                if keep_outputs:
+                    r["out"] = out
                 done.append(r)
             except Exception as e:  # reported after the run
                 errors.append((r["id"], repr(e)))

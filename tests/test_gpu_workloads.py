@@ -1,8 +1,8 @@
 """GPU decodes of the benchmark workloads beyond the 28 MHz prefix: the
-40 MHz head-switch capture, another chunk size, and the full 600-field
-capture through size-independent properties (the oracle needs about a
-minute of CPU for 600 fields, so at full size the GPU result is tied to the
-oracle through its first two chunks instead)."""
+whole 600-field 28 MHz capture against the oracle (FP64 and mixed forms),
+the 40 MHz head-switch capture (a 130-field prefix against the oracle, and
+all 6000 fields through size-independent properties, tied to the prefix), a
+35.8 MHz capture against the oracle, and another chunk size."""
 import numpy as np
 import pytest
 
@@ -42,19 +42,64 @@ def compare(out, ref):
         assert int(d.max()) <= 1, (k, int(d.max()), int((d > 1).sum()))
 
 
-def test_decode_40mhz_headswitch_prefix(torch, N, oracle):
+@pytest.fixture(scope="module")
+def cap2_prefix(N, oracle):
+    """130 fields of the 6000-field 40 MHz capture and the oracle's decode."""
+    fmt = N.format_for(40.0)
+    cap = N.generate(2, n_samples=130 * fmt.samples_per_field)
+    return cap, oracle.decode(cap, 40.0, 64, trace=True)
+
+
+def test_decode_40mhz_headswitch_prefix(torch, N, cap2_prefix):
     """130 fields of the 6000-field 40 MHz capture (head-switch offsets of
     0-4 us in field groups, SNR 25-40 dB, dropouts, wow/flutter): two chunks
     of 64 plus a short one, 8192-point video blocks."""
-    fmt = N.format_for(40.0)
-    cap = N.generate(2, n_samples=130 * fmt.samples_per_field)
-    ref = oracle.decode(cap, 40.0, 64, trace=True)
+    cap, ref = cap2_prefix
     assert ref["fields"] >= 128
     assert ref["do_count"].sum() > 0
     dec = N.Decoder(40.0, 64)
     out = dec.decode_capture(dev(torch, cap))
     compare(out, ref)
     dec.close()
+
+
+def test_decode_35_8mhz_matches_oracle(torch, N, oracle):
+    """130 fields at 35.8 MHz (the serving form's middle rate: 7-smooth K1
+    length, its own filter designs), the 600-field signal model with its own
+    seed, against the oracle."""
+    fmt = N.format_for(35.8)
+    cap = N.generate(1, sample_rate_mhz=35.8, n_samples=130 * fmt.samples_per_field, seed=358)
+    ref = oracle.decode(cap, 35.8, 64, trace=False)
+    assert ref["fields"] >= 128 and ref["do_count"].sum() > 0
+    dec = N.Decoder(35.8, 64)
+    out = dec.decode_capture(dev(torch, cap))
+    compare(out, ref)
+    dec.close()
+
+
+def test_full_6000_field_40mhz_properties(torch, N, cap2_prefix):
+    """All 6000 fields of the 40 MHz head-switch capture (4.0e9 u8 samples,
+    94 chunks): field schedule, metadata sequences and output ranges, and
+    its first two chunks equal to the oracle's decode of the 130-field
+    prefix (both read the same 64-field windows)."""
+    cap = N.generate(2)
+    assert cap.size == 6000 * N.format_for(40.0).samples_per_field
+    d = dev(torch, cap)
+    del cap
+    dec = N.Decoder(40.0, 64)
+    out = dec.decode_capture(d)
+    spf = dec.fmt.samples_per_field
+    dec.close()
+    del d
+    # the generator writes 6000 fields of 263 lines: 6011 NTSC fields of
+    # 262.5 lines of signal (the 600-field capture decodes to 601)
+    check_properties(out, spf, 6005, 6012)
+    _, ref = cap2_prefix
+    cut = {k: (v[:128] if hasattr(v, "__getitem__") and k != "fields" else v) for k, v in out.items()}
+    cut["fields"] = 128
+    refc = {k: (v[:128] if isinstance(v, np.ndarray) else v) for k, v in ref.items()}
+    refc["fields"] = 128
+    compare(cut, refc)
 
 
 def test_decode_batch16_chunk_driver(torch, N, oracle):
@@ -87,13 +132,12 @@ def full(torch, N):
     dec.close()
 
 
-def test_full_600_field_properties(torch, N, full):
-    """The bench workload at full size: field schedule, metadata sequences
-    and output ranges that hold for any correct decode."""
-    cap, dec, out = full
-    spf = dec.fmt.samples_per_field
+def check_properties(out, spf, n_lo, n_hi):
+    """Field schedule, metadata sequences and output ranges that hold for any
+    correct decode of the synthetic captures."""
+    import torch
     n = out["fields"]
-    assert 598 <= n <= 602
+    assert n_lo <= n <= n_hi, n
     fl = out["file_loc"].cpu().numpy().astype(np.int64)
     gaps = np.diff(fl)
     assert (gaps > 0.95 * spf).all() and (gaps < 1.05 * spf).all()   # one field per VSYNC
@@ -116,6 +160,13 @@ def test_full_600_field_properties(torch, N, full):
     assert 10000.0 < mean < 51200.0, mean
 
 
+def test_full_600_field_properties(torch, N, full):
+    """The bench workload at full size: field schedule, metadata sequences
+    and output ranges that hold for any correct decode."""
+    cap, dec, out = full
+    check_properties(out, dec.fmt.samples_per_field, 598, 602)
+
+
 def test_full_decode_deterministic(torch, N, full):
     """A second decode of the full capture after reset is bit-identical."""
     cap, dec, out = full
@@ -133,20 +184,41 @@ def test_full_decode_deterministic(torch, N, full):
             assert np.array_equal(a[i, :dc[i]], b[i, :dc[i]]), (k, i)
 
 
-def test_full_prefix_matches_oracle(torch, N, full, cap1_prefix, ref1_prefix):
-    """The first two chunks (128 fields) of the full decode equal the
-    oracle's decode of the 140-field prefix of the same capture: both chunks
-    read their whole 64 * spfp window in either case, so they are the same
-    computation."""
-    cap, dec, out = full
-    assert np.array_equal(cap[:cap1_prefix.size], cap1_prefix)
-    ref = ref1_prefix
-    assert [c["nf"] for c in ref["chunks"]][:2] == [64, 64]
-    cut = {k: (v[:128] if hasattr(v, "__getitem__") and k != "fields" else v) for k, v in out.items()}
-    cut["fields"] = 128
-    refc = {k: (v[:128] if isinstance(v, np.ndarray) else v) for k, v in ref.items()}
-    refc["fields"] = 128
-    compare(cut, refc)
+@pytest.fixture(scope="module")
+def ref_full(oracle, full):
+    """The oracle's decode of the whole 600-field capture (about half a
+    minute on 16 host threads)."""
+    return oracle.decode(full[0], 28.0, 64, trace=False)
+
+
+def test_full_matches_oracle(torch, N, full, ref_full):
+    """All 601 fields of the bench workload equal the oracle's decode: field
+    decisions and dropout lists exactly, TBC samples within 1 LSB."""
+    compare(full[2], ref_full)
+
+
+def test_full_mixed_form_matches_oracle(torch, N, full, ref_full):
+    """The mixed f16 form (the bench headline) on the whole capture: field
+    decisions and dropout lists equal the oracle's; TBC samples within 3 LSB
+    (the FP64 form is within 1 of the oracle, the mixed form within 1 luma
+    and 2 chroma of the FP64 form), over 1 LSB at no more than 1e-6 of them."""
+    dec = N.Decoder(28.0, 64, precision="mixed-fp16")
+    out = dec.decode_capture(dev(torch, full[0]))
+    dec.close()
+    ref = ref_full
+    n = out["fields"]
+    assert n == ref["fields"]
+    for k in ("file_loc", "is_first", "phase_id", "do_count"):
+        assert np.array_equal(out[k].cpu().numpy().astype(np.int64), np.asarray(ref[k]).astype(np.int64)), k
+    dc = ref["do_count"]
+    for k in ("do_line", "do_startx", "do_endx"):
+        g = out[k].cpu().numpy()
+        for i in range(n):
+            assert np.array_equal(g[i, :dc[i]], ref[k][i, :dc[i]]), (k, i)
+    for k in ("luma", "chroma"):
+        d = np.abs(out[k].cpu().numpy().astype(np.int64) - ref[k].astype(np.int64))
+        print(k, int(d.max()), int((d > 1).sum()))
+        assert int(d.max()) <= 3 and (d > 1).sum() <= 1e-6 * d.size, (k, int(d.max()), int((d > 1).sum()))
 
 
 def _digest(out):

@@ -37,7 +37,7 @@ constexpr int kStages = 13;
 // of chunk c+1-kSlots has finished, so with three the heads never wait for
 // the tail launched just before them.
 constexpr int kSlots = 3;
-constexpr int kMarks = 15;
+constexpr int kMarks = 18;   // 0-5 head, 6-14 tail, 15-17 the early K2 / K2b-field work
 // Guard band of the mixed forms.  A part-B block of the FP64 design is
 // recomputed in FP64 when a demod_05 output of the FP32 part B in its 256-
 // sample neighbourhood lies within delta Hz of the sync threshold, or when an
@@ -74,6 +74,14 @@ struct Slot {
     hipEvent_t dc_done = nullptr;
     hipEvent_t head_done = nullptr, tail_done = nullptr, pre0 = nullptr, pre1 = nullptr;
     hipEvent_t k4_done = nullptr, aux_done = nullptr;  // fork / join of the tail's side-stream work
+    hipEvent_t k2_done = nullptr;  // K2 pulses and K2b's per-field work (side stream)
+    // K2 pulses and K2b's per-field results: written on the side stream as
+    // soon as the chunk's head is done, while the tail of the chunk before
+    // still reads its own
+    int *k2s = nullptr, *k2l = nullptr, *k2c = nullptr;
+    int *vps = nullptr, *vpl = nullptr;
+    unsigned char* vptg = nullptr;
+    void* summ = nullptr;
     hipEvent_t mark[kMarks] = {};
     // host view of the chunk
     const void* chunk = nullptr;
@@ -130,11 +138,7 @@ struct cuvhs_ctx {
     double* rawdemod = nullptr;     // head only
     int* cand = nullptr;            // head only
     unsigned int* thr_bits = nullptr;  // head only: sync-threshold bitmask of d05
-    int *k2s = nullptr, *k2l = nullptr, *k2c = nullptr;
     int *ps = nullptr, *pl = nullptr, *pc = nullptr, *pt = nullptr;
-    int *vps = nullptr, *vpl = nullptr;
-    unsigned char* vptg = nullptr;
-    void* summ = nullptr;
     void* anchors = nullptr;
     double* ll = nullptr;
     int* isf_k2b = nullptr;
@@ -493,8 +497,30 @@ int pre_stage(cuvhs_ctx* c, Slot& s) {
     return 0;
 }
 
+// K2 pulses and K2b's per-field work (the vblank state machine, mean line
+// length, vblank pulses and first-HSYNC anchors: everything that does not
+// depend on the field before) on the side stream, once the chunk's head is
+// done.  They need only the head's outputs, and their few wavefronts
+// (latency-bound, one per field) run beside the previous chunk's tail
+// instead of in it; the tail waits for k2_done.
+int pulse_stage(cuvhs_ctx* c, Slot& s) {
+    const Fmt& f = c->fmt;
+    hipStream_t side = c->serial ? c->stream : c->side;
+    HIPCHK(c, hipStreamWaitEvent(side, s.head_done, 0));
+    mark(c, s, 15, side);
+    HIPCHK(c, run_sync_pulses(s.events, s.ctl, s.offsets, f.spfp, s.k2s, s.k2l, s.k2c, s.max_fields, 0, side));
+    mark(c, s, 16, side);
+    if (c->cfg.line_locator != CUVHS_LINELOC_K3)
+        HIPCHK(c, run_k2b_fields(s.ctl, 0, s.max_fields, f, s.k2s, s.k2l, s.k2c, s.vps, s.vpl, s.vptg,
+                                 (K2bField*)s.summ, side));
+    mark(c, s, 17, side);
+    HIPCHK(c, hipEventRecord(s.k2_done, side));
+    return 0;
+}
+
 // Head stage on the head stream: K1, sync events, VSYNC clustering; with
-// pre, the chunk's pre-band-pass goes in first on the side stream.
+// pre, the chunk's pre-band-pass goes in first on the side stream; then the
+// chunk's pulse stage on the side stream.
 int head_stage(cuvhs_ctx* c, Slot& s, bool pre = true) {
     const Fmt& f = c->fmt;
     hipStream_t st = c->stream;
@@ -522,7 +548,7 @@ int head_stage(cuvhs_ctx* c, Slot& s, bool pre = true) {
         int r = pre_stage(c, s);
         if (r) return r;
     }
-    return 0;
+    return pulse_stage(c, s);
 }
 
 struct OutArrays {
@@ -542,16 +568,17 @@ int tail_stage(cuvhs_ctx* c, Slot& s, const OutArrays* out, hipStream_t fork_str
     hipStream_t st = c->serial ? c->stream : c->tail;
     HIPCHK(c, hipStreamWaitEvent(st, s.head_done, 0));
     mark(c, s, 6, st);
-    HIPCHK(c, run_sync_pulses(s.events, s.ctl, s.offsets, f.spfp, c->k2s, c->k2l, c->k2c, s.max_fields, 0, st));
+    // the chunk's K2 pulses and K2b per-field results (pulse_stage)
+    HIPCHK(c, hipStreamWaitEvent(st, s.k2_done, 0));
     mark(c, s, 7, st);
     if (c->cfg.line_locator == CUVHS_LINELOC_K3) {
         // line_locs on the K2 pulses (pipeline.cu:1848-1856); K2b's cadence
         // state is left untouched, as in the reference
-        HIPCHK(c, run_k3(s.ctl, 0, s.max_fields, f, c->k2s, c->k2l, c->k2c, c->pt, c->ll, c->isf_k2b, st));
+        HIPCHK(c, run_k3(s.ctl, 0, s.max_fields, f, s.k2s, s.k2l, s.k2c, c->pt, c->ll, c->isf_k2b, st));
     } else {
-        HIPCHK(c, run_k2b(s.ctl, s.ctl, c->state, 0, s.max_fields, f, s.offsets, c->k2s, c->k2l, c->k2c, c->vps,
-                          c->vpl, c->vptg, (K2bField*)c->summ, (K2bAnchorOut*)c->anchors, c->ps, c->pl, c->pc, c->pt,
-                          c->ll, c->isf_k2b, st));
+        HIPCHK(c, run_k2b_cadence(s.ctl, s.ctl, c->state, 0, s.max_fields, f, s.offsets, s.vps, s.vpl, s.vptg,
+                                  (const K2bField*)s.summ, (K2bAnchorOut*)c->anchors, c->ps, c->pl, c->pc, c->pt,
+                                  c->ll, c->isf_k2b, st));
     }
     if (c->diag) {
         HIPCHK(c, run_diag_spacing(c->ll, s.ctl, s.max_fields, f, st));
@@ -617,10 +644,20 @@ void collect_profile(cuvhs_ctx* c, Slot& s) {
     c->work_samples += (uint64_t)s.fk1 * (uint64_t)c->fmt.spfp;
     c->work_chunks++;
     c->work_fields += s.hctl->nf;
-    for (int i = 0; i < kStages; i++) {
-        int a = i < 5 ? i : i + 1;   // marks 0-5 head, 6-14 tail
+    auto span = [&](int a, int b) {
         float ms = 0.0f;
-        if (hipEventElapsedTime(&ms, s.mark[a], s.mark[a + 1]) == hipSuccess) c->stage_ms[i] += ms;
+        return hipEventElapsedTime(&ms, s.mark[a], s.mark[b]) == hipSuccess ? (double)ms : 0.0;
+    };
+    for (int i = 0; i < kStages; i++) {
+        // marks 0-5 head, 6-14 tail, 15-17 the pulse stage: k2_pulses is
+        // 15-16, k2b_linelocs the per-field part 16-17 plus the tail's
+        // cadence and line locations 7-8 (6-7 is the tail's wait for them)
+        if (i == 5) c->stage_ms[i] += span(15, 16);
+        else if (i == 6) c->stage_ms[i] += span(16, 17) + span(7, 8);
+        else {
+            const int a = i < 5 ? i : i + 1;
+            c->stage_ms[i] += span(a, a + 1);
+        }
     }
     float pre = 0.0f;
     if (hipEventElapsedTime(&pre, s.pre0, s.pre1) == hipSuccess) c->stage_ms[kStages] += pre;
@@ -809,7 +846,7 @@ int cuvhs_create(const cuvhs_config* cfg, cuvhs_ctx** out) {
     if (e != hipSuccess) return bad(e, "hipStreamCreate aux");
     c->aux_queue = hw_queues() >= 5;
     for (Slot& s : c->slot) {
-        for (hipEvent_t* ev : {&s.head_done, &s.tail_done, &s.dc_done, &s.k4_done, &s.aux_done}) {
+        for (hipEvent_t* ev : {&s.head_done, &s.tail_done, &s.dc_done, &s.k4_done, &s.aux_done, &s.k2_done}) {
             e = hipEventCreateWithFlags(ev, hipEventDisableTiming);
             if (e != hipSuccess) return bad(e, "hipEventCreate");
         }
@@ -944,20 +981,22 @@ int cuvhs_create(const cuvhs_config* cfg, cuvhs_ctx** out) {
         if (e != hipSuccess) return bad(e, "hipHostMalloc");
         memset(s.hctl, 0, sizeof(ChunkCtl));
     }
-    ALLOC(c->k2s, Fz * kMaxPulses);
-    ALLOC(c->k2l, Fz * kMaxPulses);
-    ALLOC(c->k2c, Fz);
+    for (Slot& s : c->slot) {
+        ALLOC(s.k2s, Fz * kMaxPulses);
+        ALLOC(s.k2l, Fz * kMaxPulses);
+        ALLOC(s.k2c, Fz);
+        ALLOC(s.vps, Fz * kVpCap);
+        ALLOC(s.vpl, Fz * kVpCap);
+        ALLOC(s.vptg, Fz * kVpCap);
+        unsigned char* p1 = nullptr;
+        ALLOC(p1, Fz * k2b_field_summary_bytes());
+        s.summ = p1;
+    }
     ALLOC(c->ps, Fz * kMaxPulses);
     ALLOC(c->pl, Fz * kMaxPulses);
     ALLOC(c->pc, Fz);
     ALLOC(c->pt, Fz * kMaxPulses);
-    ALLOC(c->vps, Fz * kVpCap);
-    ALLOC(c->vpl, Fz * kVpCap);
-    ALLOC(c->vptg, Fz * kVpCap);
     {
-        unsigned char* p1 = nullptr;
-        ALLOC(p1, Fz * k2b_field_summary_bytes());
-        c->summ = p1;
         unsigned char* p2 = nullptr;
         ALLOC(p2, Fz * k2b_anchor_bytes());
         c->anchors = p2;
@@ -1000,7 +1039,7 @@ int cuvhs_destroy(cuvhs_ctx* c) {
     for (Slot& s : c->slot) {
         for (hipEvent_t e : s.mark)
             if (e) (void)hipEventDestroy(e);
-        for (hipEvent_t e : {s.head_done, s.tail_done, s.pre0, s.pre1, s.dc_done, s.k4_done, s.aux_done})
+        for (hipEvent_t e : {s.head_done, s.tail_done, s.pre0, s.pre1, s.dc_done, s.k4_done, s.aux_done, s.k2_done})
             if (e) (void)hipEventDestroy(e);
         if (s.hctl) (void)hipHostFree(s.hctl);
     }
@@ -1296,11 +1335,11 @@ int cuvhs_linelocs(cuvhs_ctx* c, uint64_t raw_offset, const uint64_t* offsets, i
     s.hctl->raw_offset = raw_offset;
     s.hctl->nf = nf;
     HIPCHK(c, hipMemcpyAsync(s.ctl, s.hctl, sizeof(ChunkCtl), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->k2s, starts, (size_t)nf * kMaxPulses * 4, hipMemcpyDeviceToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->k2l, lens, (size_t)nf * kMaxPulses * 4, hipMemcpyDeviceToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->k2c, count, (size_t)nf * 4, hipMemcpyDeviceToDevice, c->stream));
-    HIPCHK(c, run_k2b(s.ctl, s.ctl, c->state, nf, nf, c->fmt, (const unsigned long long*)offsets, c->k2s, c->k2l,
-                      c->k2c, c->vps, c->vpl, c->vptg, (K2bField*)c->summ, (K2bAnchorOut*)c->anchors, starts, lens,
+    HIPCHK(c, hipMemcpyAsync(s.k2s, starts, (size_t)nf * kMaxPulses * 4, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(s.k2l, lens, (size_t)nf * kMaxPulses * 4, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(s.k2c, count, (size_t)nf * 4, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, run_k2b(s.ctl, s.ctl, c->state, nf, nf, c->fmt, (const unsigned long long*)offsets, s.k2s, s.k2l,
+                      s.k2c, s.vps, s.vpl, s.vptg, (K2bField*)s.summ, (K2bAnchorOut*)c->anchors, starts, lens,
                       count, types, linelocs, is_first, c->stream));
     HIPCHK(c, hipMemcpyAsync(s.hctl, s.ctl, sizeof(ChunkCtl), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));

@@ -81,13 +81,21 @@ __device__ __forceinline__ bool quality_span(int prev_type, int type, double spa
 }
 
 // Per-K2-pulse data of the refinement, computed across the wavefront before
-// the sequential passes: width class bits (kClsHs/Eq/Vs) and the spacing to
-// the previous K2 pulse in lines, (double)(rs[i] - rs[i-1]) / line length,
-// the span the quality check takes when pulse i-1 is the last validated one.
+// the sequential passes: width class bits (kClsHs/Eq/Vs) and, from the
+// spacing to the previous K2 pulse in lines, (double)(rs[i] - rs[i-1]) /
+// line length (the span the quality check takes when pulse i-1 is the last
+// validated one), quality_span's result for the three type pairings: both
+// types > 0 ([0.4, 0.6]), both HSYNC ([0.9, 1.1]), mixed ([0.4, 1.1]).  The
+// sequential passes then branch on bits, not on FP64 compares.
 constexpr unsigned char kClsHs = 1, kClsEq = 2, kClsVs = 8;
+constexpr unsigned char kQPp = 16, kQHh = 32, kQMx = 64;
+__device__ __forceinline__ bool quality_bits(int prev_type, int type, int c) {
+    if (prev_type > 0 && type > 0) return (c & kQPp) != 0;
+    if (prev_type == 0 && type == 0) return (c & kQHh) != 0;
+    return (c & kQMx) != 0;
+}
 struct PulseInfo {
-    const unsigned char* cls;
-    const double* span;
+    const unsigned char* cls;   // class and quality bits
 };
 
 __device__ __forceinline__ double round_line_loc(double line_number) {
@@ -96,21 +104,35 @@ __device__ __forceinline__ double round_line_loc(double line_number) {
 
 // host_run_vblank_state_machine (pipeline.cu:519-609) appending to vp[n..)
 // Returns done; *n_out is the new length (entries past the cap are dropped
-// and flagged).  prev_src: the K2 index the last validated pulse came from,
-// or -1; the machine keeps that pulse's type and start in registers.
+// and flagged).  Run by the whole wavefront with uniform values: the pulse
+// data are broadcast reads of LDS, lane 0 stores the appended pulses, and
+// the last validated pulse's type and start come in (prev_type, prev_start,
+// n > 0) and go out (*last_type, *last_start) in registers, so nothing the
+// machine stores is read back.  prev_src: the K2 index the last validated
+// pulse came from, or -1.
 __device__ bool vblank_state_machine(const int* rs, const int* rl, PulseInfo pi, int npc, int start_idx,
-                                     int num_pulses, int in_line_len, VP vp, int n, int prev_src, int* n_out,
-                                     int* ovf) {
+                                     int num_pulses, int in_line_len, VP vp, int n, int prev_src, int prev_type,
+                                     int prev_start, int* n_out, int* ovf, int* last_tg, int* last_start) {
+    const int lane = threadIdx.x & 63;
+    const bool writer = lane == 0;
     bool done = false;
     double num_pulses_half = num_pulses / 2.0;
     int vsync_start = -1;
     double state_end = 0.0;
     double state_length = -1.0;
-    int state = n == 0 ? -1 : (vp.tg[n - 1] & 15);
-    int prev_start = n == 0 ? 0 : vp.start[n - 1];
+    int state = prev_type;
+    int tg_out = -1;
+    // the machine reads at most 26 pulses from start_idx: lane k holds
+    // pulse start_idx + k, read back by readlane in the loop
+    const int kk = start_idx + lane;
+    const int w_rs = kk < npc ? rs[kk] : 0, w_rl = kk < npc ? rl[kk] : 0, w_cls = kk < npc ? (int)pi.cls[kk] : 0;
+    // state_end (a double) against integer starts: ps > state_end iff
+    // ps > floor(state_end), ps < state_end iff ps < ceil(state_end)
+    int end_fl = 0, end_cl = 0;
     for (int i = start_idx; i < npc && i < start_idx + 26; i++) {
-        const int ps = rs[i];
-        const unsigned char c = pi.cls[i];
+        const int jj = i - start_idx;
+        const int ps = __builtin_amdgcn_readlane(w_rs, jj);
+        const int c = __builtin_amdgcn_readlane(w_cls, jj);
         const bool hs = (c & kClsHs) != 0, eq = (c & kClsEq) != 0, vsn = (c & kClsVs) != 0;
         int type = -1;
         bool have = false;
@@ -127,39 +149,48 @@ __device__ bool vblank_state_machine(const int* rs, const int* rl, PulseInfo pi,
         } else if (state == kPulseVsync) {
             if (eq) { type = kPulseEq2; state_length = num_pulses_half; have = true; }
             else if (vsn) { type = kPulseVsync; have = true; }
-            else if ((double)ps > state_end && hs) { type = kPulseHsync; have = true; }
+            else if (ps > end_fl && hs) { type = kPulseHsync; have = true; }
         } else if (state == kPulseEq2) {
             if (eq) { type = kPulseEq2; have = true; }
             else if (hs) { type = kPulseHsync; have = true; done = true; }
         }
         if (have && type != state) {
-            if ((double)ps < state_end) {
+            if (ps < end_cl) {
                 have = false;
             } else if (state_length > 0.0) {
                 state_end = (double)ps + ((state_length - 0.1) * (double)in_line_len);
                 state_length = -1.0;
+                end_fl = (int)floor(state_end);
+                end_cl = (int)ceil(state_end);
             }
         }
         if (have) {
             bool good = false;
             if (n > 0)
-                good = prev_src == i - 1 ? quality_span(state, type, pi.span[i])
+                good = prev_src == i - 1 ? quality_bits(state, type, c)
                                          : qualitycheck(state, type, ps - prev_start, in_line_len);
             if (n < kVpCap) {
-                vp.start[n] = ps;
-                vp.len[n] = rl[i];
-                vp.tg[n] = (unsigned char)(type | (good ? 16 : 0));
+                const int tg = type | (good ? 16 : 0);
+                const int len = __builtin_amdgcn_readlane(w_rl, jj);
+                if (writer) {
+                    vp.start[n] = ps;
+                    vp.len[n] = len;
+                    vp.tg[n] = (unsigned char)tg;
+                }
                 n++;
                 state = type;
                 prev_start = ps;
                 prev_src = i;
+                tg_out = tg;
             } else {
                 *ovf = 1;
             }
         }
-        if (done && vsync_start >= 0) { *n_out = n; return true; }
+        if (done && vsync_start >= 0) break;
     }
     *n_out = n;
+    *last_tg = tg_out;
+    *last_start = prev_start;
     return done;
 }
 
@@ -245,15 +276,18 @@ __device__ AnchorVariant anchor_variant(VP vp, int nvp, const int* vbp, double m
                 term = loc + mll * ((double)rl - hsl) - ps;
             }
         }
-        // pulses after the first break are not scanned
+        // pulses after the first break are not scanned.  The sum runs in
+        // pulse order over the lanes' terms, the pulses not taken adding
+        // +0.0 (which leaves every sum unchanged but the sign of a zero one,
+        // and loc - (+-0) / hc is the same either way)
         const unsigned long long mb = __ballot(brk);
         unsigned long long mt = __ballot(take);
         if (mb) mt &= (mb & (~mb + 1ull)) - 1ull;
-        while (mt) {
-            const int j = __builtin_ctzll(mt);
-            mt &= mt - 1ull;
-            hoff += lane_value(term, j);
-            hc++;
+        if (mt) {
+            const double tv = ((mt >> lane) & 1ull) ? term : 0.0;
+#pragma unroll
+            for (int j = 0; j < 64; j++) hoff += lane_value(tv, j);
+            hc += __builtin_popcountll(mt);
         }
         if (mb) break;
     }
@@ -317,40 +351,58 @@ __global__ __launch_bounds__(64) void k_k2b_field(const ChunkCtl* ctl, int nf_ov
         lt.vsync_min = (vsync_width + hoff) * 0.5;
         lt.vsync_max = vsync_width + hoff + 2.0 * tol;
     }
-    // per-pulse width classes and spacings (PulseInfo), across the wavefront
+    // per-pulse width classes and quality bits (PulseInfo), across the wavefront
     __shared__ unsigned char s_cls[kMaxPulses];
-    __shared__ double s_span[kMaxPulses];
     for (int i = threadIdx.x; i < npc; i += 64) {
         const double len = (double)rl[i];
         unsigned char c = 0;
         if (inr(len, lt.hsync_min, lt.hsync_max)) c |= kClsHs;
         if (inr(len, lt.eq_min, lt.eq_max)) c |= kClsEq;
         if (inr(len, lt.vsync_min, lt.vsync_max)) c |= kClsVs;
+        const double span = i > 0 ? (double)(rs[i] - rs[i - 1]) / (double)spl : 0.0;
+        if (quality_span(1, 1, span)) c |= kQPp;
+        if (quality_span(0, 0, span)) c |= kQHh;
+        if (quality_span(0, 1, span)) c |= kQMx;
         s_cls[i] = c;
-        s_span[i] = i > 0 ? (double)(rs[i] - rs[i - 1]) / (double)spl : 0.0;
     }
     __syncthreads();
-    const PulseInfo pi{s_cls, s_span};
-    if (threadIdx.x == 0) {
-        int ovf = 0;
-        int n = 0;
-        // host_refine_pulses_exactish (pipeline.cu:611-648).  The last
-        // validated pulse's type and start are kept in registers, and the
-        // next pulse is loaded while this one is processed.
+    const PulseInfo pi{s_cls};
+    {
+        // host_refine_pulses_exactish (pipeline.cu:611-648), run by the whole
+        // wavefront with uniform values: pulse i's class, start, length and
+        // span come from lane i mod 64 of registers loaded once per 64
+        // pulses (readlane, no LDS latency in the loop); lane 0 stores the
+        // validated pulses; the last one's type and start stay in registers.
         // prev_src: the K2 index the last validated pulse came from, when the
         // main loop appended it (-1 after the state machine's appends)
+        const int lane = threadIdx.x & 63;
+        int ovf = 0;
+        int n = 0;
         int prev_tg = 0, prev_start = 0, prev_src = -1;
+        int w_cls = 0, w_rs = 0, w_rl = 0;
         for (int i = 0; i < npc; i++) {
-            const unsigned char c = s_cls[i];
+            const int j = i & 63;
+            if (j == 0) {
+                const int k = i + lane;
+                w_cls = k < npc ? (int)s_cls[k] : 0;
+                w_rs = k < npc ? rs[k] : 0;
+                w_rl = k < npc ? rl[k] : 0;
+            }
+            const int c = __builtin_amdgcn_readlane(w_cls, j);
             if (c & kClsHs) {
-                const int cs = rs[i];
+                const int cs = __builtin_amdgcn_readlane(w_rs, j);
                 bool good = false;
                 if (n > 0)
-                    good = prev_src == i - 1 ? quality_span(prev_tg & 15, kPulseHsync, s_span[i])
+                    good = prev_src == i - 1 ? quality_bits(prev_tg & 15, kPulseHsync, c)
                                              : qualitycheck(prev_tg & 15, kPulseHsync, cs - prev_start, spl);
                 if (n < kVpCap) {
-                    const unsigned char tg = (unsigned char)(kPulseHsync | (good ? 16 : 0));
-                    vs[n] = cs; vlen[n] = rl[i]; vtg[n] = tg; n++;
+                    const int tg = kPulseHsync | (good ? 16 : 0);
+                    if (lane == 0) {
+                        vs[n] = cs;
+                        vlen[n] = __builtin_amdgcn_readlane(w_rl, j);
+                        vtg[n] = (unsigned char)tg;
+                    }
+                    n++;
                     prev_tg = tg;
                     prev_start = cs;
                     prev_src = i;
@@ -358,19 +410,22 @@ __global__ __launch_bounds__(64) void k_k2b_field(const ChunkCtl* ctl, int nf_ov
                     ovf = 1;
                 }
             } else if (i > 2 && (c & kClsEq) && n > 0 && (prev_tg & 15) == kPulseHsync) {
-                int n2 = n;
-                bool done = vblank_state_machine(rs, rl, pi, npc, i - 2, num_eq, spl, vp, n, prev_src, &n2, &ovf);
+                int n2 = n, ltg = -1, lstart = 0;
+                bool done = vblank_state_machine(rs, rl, pi, npc, i - 2, num_eq, spl, vp, n, prev_src, prev_tg & 15,
+                                                 prev_start, &n2, &ovf, &ltg, &lstart);
                 if (done && n2 > n) {  // keep the appended sequence
                     n = n2;
-                    prev_tg = vtg[n - 1];
-                    prev_start = vs[n - 1];
+                    prev_tg = ltg;
+                    prev_start = lstart;
                     prev_src = -1;
                 }
             }
             // every branch advances by one pulse (pipeline.cu:637-642)
         }
-        s_nvp = n;
-        s_ovf = ovf;
+        if (lane == 0) {
+            s_nvp = n;
+            s_ovf = ovf;
+        }
     }
     __syncthreads();
     const int n = s_nvp;
@@ -775,21 +830,39 @@ __global__ __launch_bounds__(256) void k_k2b_linelocs(const ChunkCtl* ctl, int n
 }
 
 // ---- host launchers ----
-hipError_t run_k2b(const ChunkCtl* ctl_dev, ChunkCtl* ctl_mut, DecodeState* dst, int nf_override, int max_fields, const Fmt& fmt,
-                   const unsigned long long* offsets, const int* k2_starts, const int* k2_lens, const int* k2_count,
-                   int* vp_start, int* vp_len, unsigned char* vp_tg, K2bField* summ, K2bAnchorOut* anchors,
-                   int* out_starts, int* out_lens, int* out_count, int* out_types, double* linelocs, int* is_first,
-                   hipStream_t st) {
+hipError_t run_k2b_fields(const ChunkCtl* ctl_dev, int nf_override, int max_fields, const Fmt& fmt,
+                          const int* k2_starts, const int* k2_lens, const int* k2_count, int* vp_start, int* vp_len,
+                          unsigned char* vp_tg, K2bField* summ, hipStream_t st) {
     double tol = fmt.sample_rate * 0.5e-6;
     hipLaunchKernelGGL(k_k2b_field, dim3(max_fields), dim3(64), 0, st, ctl_dev, nf_override, k2_starts, k2_lens,
                        k2_count, tol, fmt.hsync_width, fmt.eq_pulse_width, fmt.vsync_width, fmt.samples_per_line,
                        fmt.num_eq_pulses, fmt.field_lines_first, fmt.field_lines_second, vp_start, vp_len, vp_tg,
                        summ);
+    return hipGetLastError();
+}
+
+hipError_t run_k2b_cadence(const ChunkCtl* ctl_dev, ChunkCtl* ctl_mut, DecodeState* dst, int nf_override,
+                           int max_fields, const Fmt& fmt, const unsigned long long* offsets, const int* vp_start,
+                           const int* vp_len, const unsigned char* vp_tg, const K2bField* summ, K2bAnchorOut* anchors,
+                           int* out_starts, int* out_lens, int* out_count, int* out_types, double* linelocs,
+                           int* is_first, hipStream_t st) {
     hipLaunchKernelGGL(k_k2b_chain, dim3(1), dim3(64), 0, st, ctl_mut, dst, nf_override, offsets, summ, vp_start, vp_tg,
                        anchors, is_first, fmt.num_eq_pulses, fmt.field_lines_first, fmt.field_lines_second);
     hipLaunchKernelGGL(k_k2b_linelocs, dim3(max_fields), dim3(256), 0, st, ctl_dev, nf_override, summ, anchors,
                        vp_start, vp_len, vp_tg, out_starts, out_lens, out_count, out_types, linelocs);
     return hipGetLastError();
+}
+
+hipError_t run_k2b(const ChunkCtl* ctl_dev, ChunkCtl* ctl_mut, DecodeState* dst, int nf_override, int max_fields, const Fmt& fmt,
+                   const unsigned long long* offsets, const int* k2_starts, const int* k2_lens, const int* k2_count,
+                   int* vp_start, int* vp_len, unsigned char* vp_tg, K2bField* summ, K2bAnchorOut* anchors,
+                   int* out_starts, int* out_lens, int* out_count, int* out_types, double* linelocs, int* is_first,
+                   hipStream_t st) {
+    hipError_t e = run_k2b_fields(ctl_dev, nf_override, max_fields, fmt, k2_starts, k2_lens, k2_count, vp_start,
+                                  vp_len, vp_tg, summ, st);
+    if (e != hipSuccess) return e;
+    return run_k2b_cadence(ctl_dev, ctl_mut, dst, nf_override, max_fields, fmt, offsets, vp_start, vp_len, vp_tg, summ,
+                           anchors, out_starts, out_lens, out_count, out_types, linelocs, is_first, st);
 }
 
 size_t k2b_field_summary_bytes() { return sizeof(K2bField); }

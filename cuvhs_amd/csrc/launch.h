@@ -177,6 +177,16 @@ hipError_t run_k2b(const ChunkCtl* ctl_dev, ChunkCtl* ctl_mut, DecodeState* dst,
                    int* vp_start, int* vp_len, unsigned char* vp_tg, K2bField* summ, K2bAnchorOut* anchors,
                    int* out_starts, int* out_lens, int* out_count, int* out_types, double* linelocs, int* is_first,
                    hipStream_t st);
+// the two halves of run_k2b: everything per field (needs only the chunk's
+// K2 pulses), then the cross-field cadence and the line locations
+hipError_t run_k2b_fields(const ChunkCtl* ctl_dev, int nf_override, int max_fields, const Fmt& fmt,
+                          const int* k2_starts, const int* k2_lens, const int* k2_count, int* vp_start, int* vp_len,
+                          unsigned char* vp_tg, K2bField* summ, hipStream_t st);
+hipError_t run_k2b_cadence(const ChunkCtl* ctl_dev, ChunkCtl* ctl_mut, DecodeState* dst, int nf_override,
+                           int max_fields, const Fmt& fmt, const unsigned long long* offsets, const int* vp_start,
+                           const int* vp_len, const unsigned char* vp_tg, const K2bField* summ, K2bAnchorOut* anchors,
+                           int* out_starts, int* out_lens, int* out_count, int* out_types, double* linelocs,
+                           int* is_first, hipStream_t st);
 size_t k2b_field_summary_bytes();
 size_t k2b_anchor_bytes();
 // f32: the signal is stored as float (reduced-precision forms), else double

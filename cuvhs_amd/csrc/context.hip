@@ -29,6 +29,7 @@
 #include "design.h"
 #include "fft_lds.h"
 #include "launch.h"
+#include "tuning_gfx950.h"
 
 using namespace cuvhs;
 
@@ -41,14 +42,21 @@ constexpr int kMarks = 18;   // 0-5 head, 6-14 tail, 15-17 the early K2 / K2b-fi
 // Guard band of the mixed forms.  A part-B block of the FP64 design is
 // recomputed in FP64 when a demod_05 output of the FP32 part B in its 256-
 // sample neighbourhood lies within delta Hz of the sync threshold, or when an
-// envelope tile in its input is below rho times the largest there (dropouts
-// and the chunk's zero-filled end, where the phase error grows as the
-// envelope falls).  Measured without the guard on the 600-field 28 MHz
-// capture (3.0e8 samples) against the FP64 form: the f16 form's demod_05
-// error is at most 1.55 Hz at every envelope ratio; the bf16 form's is at
-// most 4.0 Hz where the ratio is 0.5 or more and 28.7 Hz below.  delta is
-// about ten times the largest error where the block is not recomputed.
-constexpr double kGuardDeltaF16 = 16.0;
+// envelope tile in its input is below rho times the largest there.  Measured
+// without the guard on the 600-field 28 MHz capture (3.0e8 samples, binned
+// by the envelope over the largest envelope within two tiles of 1024): the
+// f16 form's demod_05 error is at most 1.55 Hz in every bin, 1.02 Hz among
+// the samples below 0.005 (dropouts); the bf16 form's is at most 1.55 Hz
+// there but 28.7 Hz next to dropouts, where part B's filter spreads the
+// larger bf16 phase error of the dip (16-bit taps) onto full-envelope
+// samples.  delta: 8 Hz for f16 (five times its largest error), 32 Hz for
+// bf16 (eight times its 4.0 Hz outside the recomputed blocks).  rho keeps
+// dropouts in FP64 in both: a dip flattens the sync edges K4 interpolates,
+// so even f16's 1.5 Hz moves a refined line start there enough to change
+// chroma (a 3.58 MHz carrier) by up to 3 LSB (71 samples over 1 LSB in 140
+// fields with rho = 0.002, none with 0.1); it also covers the zero-filled
+// end of a capture, where the phase is ill-conditioned in any precision.
+constexpr double kGuardDeltaF16 = 8.0;
 constexpr float kGuardRhoF16 = 0.1f;
 constexpr double kGuardDeltaBf16 = 32.0;
 constexpr float kGuardRhoBf16 = 0.5f;
@@ -130,6 +138,9 @@ struct cuvhs_ctx {
     void* tw_p = nullptr;
     int pre_f32 = 0;           // chroma pre-band-pass transforms in FP32 (CUVHS_PRECISION_FP32)
     double2* tw_1024 = nullptr;
+    float2* tw_1024f = nullptr;     // the chroma line filter's twiddles in FP32 (chroma_f32)
+    int chroma_f32 = 0;             // chroma line filter in FP32 (the reduced-precision forms)
+    int block_source = 2;           // block sizes from: 0 the config, 1 the tuned table, 2 the cost model
     double* cs_tab = nullptr;       // cos/sin of the burst phase detector
     double2* chroma_mask = nullptr; // per-line band-pass: (1, 0) kept bins, (0, 0) others
     float* chroma_filt = nullptr;   // band-passed chroma lines of a chunk (track 0)
@@ -236,6 +247,14 @@ hipError_t upload_twiddles(cuvhs_ctx* c, void** dst, int B, int f32) {
     return hipMemcpy(*dst, t.data(), t.size() * sizeof(double2), hipMemcpyHostToDevice);
 }
 
+// The autotuned block sizes for a sample rate and precision form, if the
+// table has a row for them.
+const TunedBlocks* tuned_blocks(double mhz, int precision) {
+    for (const TunedBlocks& t : kTunedBlocks)
+        if (t.precision == precision && std::fabs(t.mhz - mhz) <= 0.05) return &t;
+    return nullptr;
+}
+
 int sample_bytes(const cuvhs_ctx* c) { return c->cfg.input_format == CUVHS_INPUT_U8 ? 1 : 2; }
 
 void reset_state(DecodeState* h) {
@@ -290,6 +309,8 @@ ChromaLaunch chroma_launch(cuvhs_ctx* c, const ChunkCtl* ctl, int nf_override, c
     a.filt = c->chroma_filt;
     a.out = c->chroma;
     a.tw = c->tw_1024;
+    a.tw_f = c->tw_1024f;
+    a.f32 = c->chroma_f32;
     return a;
 }
 
@@ -808,6 +829,10 @@ int cuvhs_create(const cuvhs_config* cfg, cuvhs_ctx** out) {
     if (cfg->input_format < CUVHS_INPUT_U8 || cfg->input_format > CUVHS_INPUT_U16) return CUVHS_E_UNSUPPORTED;
     if (cfg->line_locator != CUVHS_LINELOC_K2B && cfg->line_locator != CUVHS_LINELOC_K3) return CUVHS_E_UNSUPPORTED;
     if (cfg->batch_fields < 1 || cfg->batch_fields > 4096) return CUVHS_E_ARG;
+    if ((cfg->block_k1a && cfg->block_k1a != 4096 && cfg->block_k1a != 8192) ||
+        (cfg->block_k1b && cfg->block_k1b != 4096 && cfg->block_k1b != 8192) ||
+        (cfg->block_pre && cfg->block_pre != 8192 && cfg->block_pre != 16384))
+        return CUVHS_E_ARG;
     cuvhs_ctx* c = new cuvhs_ctx();
     c->cfg = *cfg;
     c->fmt = make_fmt(cfg->sample_rate_mhz);
@@ -864,9 +889,19 @@ int cuvhs_create(const cuvhs_config* cfg, cuvhs_ctx** out) {
         c->k1_f32 = cfg->precision != CUVHS_PRECISION_FP64;
         c->sig_f32 = cfg->precision != CUVHS_PRECISION_FP64;
         c->d05_f32 = cfg->precision == CUVHS_PRECISION_FP32;
-        c->k1 = c->k1_f32 ? design_k1(f, 1e-8, bk1, 2, 1, 8) : design_k1(f, 1e-13, bk1, 2, 1, 16);
+        // block sizes: the caller's, else the autotuned table's, else the
+        // cost model's choice among the supported ones
+        const TunedBlocks* tb = tuned_blocks(cfg->sample_rate_mhz, cfg->precision);
+        const int ua = cfg->block_k1a ? cfg->block_k1a : (tb ? tb->k1a : 0);
+        const int ub = cfg->block_k1b ? cfg->block_k1b : (tb ? tb->k1b : 0);
+        const int up = cfg->block_pre ? cfg->block_pre : (tb ? tb->pre : 0);
+        c->block_source = (cfg->block_k1a || cfg->block_k1b || cfg->block_pre) ? 0 : (tb ? 1 : 2);
+        const int* cha = ua ? &ua : bk1;
+        const int* chb = ub ? &ub : bk1;
+        const int na = ua ? 1 : 2, nb = ub ? 1 : 2;
+        c->k1 = c->k1_f32 ? design_k1(f, 1e-8, cha, na, chb, nb, 1, 8) : design_k1(f, 1e-13, cha, na, chb, nb, 1, 16);
         if (c->mixed) {
-            c->k1g = design_k1(f, 1e-13, bk1, 2, 1, 16);
+            c->k1g = design_k1(f, 1e-13, bk1, 2, bk1, 2, 1, 16);
             // sync-threshold band of the FP32 part B and the envelope ratio
             // below which a tile's phase is treated as ill-conditioned
             c->g_delta = c->mixed_bf16 ? kGuardDeltaBf16 : kGuardDeltaF16;
@@ -877,10 +912,16 @@ int cuvhs_create(const cuvhs_config* cfg, cuvhs_ctx** out) {
         // truncation is then ~1e-6 of the chroma band's level even when the
         // band holds only quantisation noise), or FP32 with taps to 1e-7
         c->pre_f32 = cfg->precision != CUVHS_PRECISION_FP64;
-        c->pre = c->pre_f32 ? design_prebpf(f, 1e-7, bpre, 2) : design_prebpf(f, 1e-9, bpre, 2);
+        c->chroma_f32 = cfg->precision != CUVHS_PRECISION_FP64;
+        const int* chp = up ? &up : bpre;
+        const int np = up ? 1 : 2;
+        c->pre = c->pre_f32 ? design_prebpf(f, 1e-7, chp, np) : design_prebpf(f, 1e-9, chp, np);
     } catch (...) {
+        // no supported block size fits a filter: a size the caller forced,
+        // or a sample rate whose filters are too long
+        const bool forced = cfg->block_k1a || cfg->block_k1b || cfg->block_pre;
         cuvhs_destroy(c);
-        return CUVHS_E_UNSUPPORTED;
+        return forced ? CUVHS_E_ARG : CUVHS_E_UNSUPPORTED;
     }
 #define ALLOC(p, n)                                    \
     do {                                               \
@@ -925,6 +966,12 @@ int cuvhs_create(const cuvhs_config* cfg, cuvhs_ctx** out) {
         auto t1 = twiddles<double2, double>(kChromaFFT);
         ALLOC(c->tw_1024, t1.size());
         if ((e = hipMemcpy(c->tw_1024, t1.data(), t1.size() * sizeof(double2), hipMemcpyHostToDevice)) != hipSuccess) return bad(e, "twiddles");
+        if (c->chroma_f32) {
+            auto t2 = twiddles<float2, float>(kChromaFFT);
+            ALLOC(c->tw_1024f, t2.size());
+            if ((e = hipMemcpy(c->tw_1024f, t2.data(), t2.size() * sizeof(float2), hipMemcpyHostToDevice)) != hipSuccess)
+                return bad(e, "twiddles");
+        }
         // burst I/Q reference (chroma_decode.cu:435-437), computed on the host
         std::vector<double> cs(2 * 36);
         for (int i = 0; i < 36; i++) {
@@ -1268,6 +1315,14 @@ int cuvhs_k1_blocks(const cuvhs_ctx* c, int* block_a, int* valid_a, int* block_b
     return 0;
 }
 
+int cuvhs_prebpf_block(const cuvhs_ctx* c, int* block, int* valid, int* source) {
+    if (!c) return CUVHS_E_ARG;
+    if (block) *block = c->pre.bpf.B;
+    if (valid) *valid = c->pre.bpf.valid;
+    if (source) *source = c->block_source;
+    return 0;
+}
+
 int cuvhs_profile_read(const cuvhs_ctx* c, double* ms, const char** names, int n) {
     if (!c || n < 0) return CUVHS_E_ARG;
     for (int i = 0; i < n && i <= kStages; i++) {
@@ -1424,7 +1479,7 @@ int cuvhs_dropouts_noalloc(double sample_rate_mhz, const double* env, const doub
 int cuvhs_selftest_block(int B, int is_f32, const void* d_x, const void* d_g, void* d_y) {
     if (!d_x || !d_g || !d_y) return CUVHS_E_ARG;
     // the shapes the decoder runs; anything else is refused before any copy
-    const bool ok = B == 4096 || B == 8192 || B == 16384 || (!is_f32 && B == 1024);
+    const bool ok = B == 1024 || B == 4096 || B == 8192 || B == 16384;
     if (!ok) return CUVHS_E_ARG;
     return is_f32 ? block_selftest<float2, float>(B, 1, d_x, d_g, d_y)
                   : block_selftest<double2, double>(B, 0, d_x, d_g, d_y);

@@ -232,7 +232,8 @@ void host_fft(std::vector<cdbl>& x, int sign) {
     if (a != x.data()) std::copy(a, a + n, x.data());
 }
 
-K1Design design_k1(const Fmt& fmt, double eps, const int* bch, int nb, int extra_a, int elem_bytes) {
+K1Design design_k1(const Fmt& fmt, double eps, const int* ach, int na, const int* bch, int nb, int extra_a,
+                   int elem_bytes) {
     K1Design d;
     d.N = next_7smooth(fmt.spfp + kOverlap);
     const int N = d.N, fb = N / 2 + 1;
@@ -248,7 +249,7 @@ K1Design design_k1(const Fmt& fmt, double eps, const int* bch, int nb, int extra
                           butter_hpf_mag(om, w_hpf, 8) * hil;
     }
     host_fft(spec, +1);  // unnormalised inverse: the analytic signal carries the N scale
-    d.rf = make_block(spec, eps, bch, nb, extra_a, elem_bytes);
+    d.rf = make_block(spec, eps, ach, na, extra_a, elem_bytes);
 
     double bs[3], as[3];
     shelf_high(273755.82, 13.9794, fs, 0.462088186, bs, as);

@@ -35,7 +35,10 @@ struct K1Design {
 // fm_demod.cu:273-374 with spf = spfp (pipeline.cu:1455).
 // eps: relative tap magnitude below which the kernel is truncated.
 // elem_bytes: bytes per complex LDS element of the transform (16 FP64, 8 FP32).
-K1Design design_k1(const Fmt& fmt, double eps, const int* b_choices, int nb_choices, int extra_a, int elem_bytes);
+// a_choices / b_choices: the block sizes part A / part B may take (the
+// cheapest under the cost model, or the only one given).
+K1Design design_k1(const Fmt& fmt, double eps, const int* a_choices, int na, const int* b_choices, int nb,
+                   int extra_a, int elem_bytes);
 
 // chroma_decode.cu:568-587: |H|^2/N pre-band-pass at the capture rate, as a
 // real kernel on the N2 = next_7smooth(spfp) grid.

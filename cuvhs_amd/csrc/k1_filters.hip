@@ -213,7 +213,8 @@ __global__ __launch_bounds__(B / 16, (k1_min_blocks<B, T>())) void k1_part_a(K1A
 
 // Block-list mode (the guard band's FP64 recompute): every gridDim.x-th
 // listed block.  A kernel of its own, so the loop's hoisted values do not
-// weigh on the registers of the full launch.
+// weigh on the registers of the full launch.  (Bounding it to the full
+// launch's three workgroups per CU spills 43 VGPRs of the FP64 loop.)
 template <int B, typename T>
 __global__ __launch_bounds__(B / 16) void k1_part_a_list(K1ALaunch a) {
     const int n = *a.bcount;
@@ -525,8 +526,8 @@ constexpr bool selftest_whole() { return !(sizeof(T) == 8 && (B == 1024 || B == 
 // with G in the block's bin order.  B = 4096: conv4096 (FP64 with split
 // exchanges as in K1, FP32 whole); 8192 / 16384: conv_rows<512 / 1024>
 // (FP64 16384 with split exchanges as in the pre-band-pass);
-// 1024: filter1024_wave (one wavefront, FP64, split exchanges, G applied per
-// bin as in the chroma kernel).
+// 1024: filter1024_wave (one wavefront, G applied per bin as in the chroma
+// kernel: FP64 with split exchanges, or FP32 whole as in the reduced forms).
 template <int B, typename T>
 __global__ __launch_bounds__(B == 1024 ? 64 : B / 16) void k_block_selftest(const typename C2<T>::type* x,
                                                                           const typename C2<T>::type* gp,
@@ -539,7 +540,7 @@ __global__ __launch_bounds__(B == 1024 ? 64 : B / 16) void k_block_selftest(cons
 #pragma unroll
     for (int r = 0; r < 16; r++) v[r] = x[NT * r + threadIdx.x];
     if constexpr (B == 1024) {
-        filter1024_wave<T, false>(smem, v, tw, [&](V z, int k) { return cmul(z, gp[k]); });
+        filter1024_wave<T, selftest_whole<B, T>()>(smem, v, tw, [&](V z, int k) { return cmul(z, gp[k]); });
     } else if constexpr (B == 4096) {
         conv4096<T, k1_whole<B, T>()>(smem, v, tw, gp);
     } else {
@@ -567,6 +568,7 @@ hipError_t run_block_selftest(int B, int is_f32, const void* x, const void* gp, 
         if (B == 8192) return block_st_launch<8192, double>(x, gp, y, tw, st);
         if (B == 16384) return block_st_launch<16384, double>(x, gp, y, tw, st);
     } else {
+        if (B == 1024) return block_st_launch<1024, float>(x, gp, y, tw, st);
         if (B == 4096) return block_st_launch<4096, float>(x, gp, y, tw, st);
         if (B == 8192) return block_st_launch<8192, float>(x, gp, y, tw, st);
         if (B == 16384) return block_st_launch<16384, float>(x, gp, y, tw, st);

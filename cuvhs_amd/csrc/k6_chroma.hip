@@ -45,22 +45,20 @@ __device__ __forceinline__ double chroma_resample(const ChromaLaunch& a, double 
     return p1 + 0.5 * x * (aa + x * (bb + x * cc));
 }
 
-// Heterodyne -cos(2 pi het_scale (line * 910 + col) + phase) of
-// chroma_decode.cu:143-148 at the columns col = t + 64 r, r = 0..15, that
-// thread t owns: one sincos at col = t, then rotations by the fixed step
-// exp(i 2 pi het_scale 64) (het_dc, het_ds).  The rotation differs from a
-// direct cos by ~1e-14 relative; the reference's own FP64 argument, up to
-// 4.4e5 rad, is rounded at ~5e-11 rad.
-__device__ __forceinline__ void chroma_het(const ChromaLaunch& a, int line, double phase_offset, double* het) {
-    double s, c;
-    sincos(2.0 * M_PI * a.het_scale * (double)(line * kLineLen + (int)threadIdx.x) + phase_offset, &s, &c);
-#pragma unroll
-    for (int r = 0; r < 16; r++) {
-        het[r] = -c;
-        const double cn = c * a.het_dc - s * a.het_ds;
-        s = s * a.het_dc + c * a.het_ds;
-        c = cn;
-    }
+// Start of the heterodyne -cos(2 pi het_scale (line * 910 + col) + phase) of
+// chroma_decode.cu:143-148 at column col = t, the first of the columns
+// t + 64 r, r = 0..15, that thread t owns; het_step moves (s, c) on to the
+// next of them by the fixed rotation exp(i 2 pi het_scale 64) (het_dc,
+// het_ds).  The rotation differs from a direct cos by ~1e-14 relative; the
+// reference's own FP64 argument, up to 4.4e5 rad, is rounded at ~5e-11 rad.
+__device__ __forceinline__ void het_start(const ChromaLaunch& a, int line, double phase_offset, double* s,
+                                          double* c) {
+    sincos(2.0 * M_PI * a.het_scale * (double)(line * kLineLen + (int)threadIdx.x) + phase_offset, s, c);
+}
+__device__ __forceinline__ void het_step(const ChromaLaunch& a, double* s, double* c) {
+    const double cn = *c * a.het_dc - *s * a.het_ds;
+    *s = *s * a.het_dc + *c * a.het_ds;
+    *c = cn;
 }
 
 // One wavefront per pair of lines (la, la + 1): resample + heterodyne with
@@ -73,9 +71,15 @@ __device__ __forceinline__ void chroma_het(const ChromaLaunch& a, int line, doub
 // transforms each line on its own; the results agree to rounding.)  Thread t
 // computes columns t + 64 r, the register layout filter1024_wave takes and
 // returns, so the lines never pass through LDS outside the transform.
-constexpr bool kChromaWhole = false;  // real and imaginary parts exchanged in turn
-__global__ __launch_bounds__(64, 3) void k_chroma_line(ChromaLaunch a) {
-    __shared__ __align__(16) double buf[lds_elems(kChromaFFT) * (kChromaWhole ? 2 : 1)];
+// T: the transform's type.  The resampling coordinate and the heterodyne
+// stay FP64 in both (positions reach 46 M samples, the heterodyne argument
+// 4.4e5 rad); T = float is the reduced-precision forms' line filter, with
+// whole complex elements in LDS (the same 8.7 KB as the FP64 filter's real
+// and imaginary parts in turn).
+template <typename T>
+__device__ __forceinline__ void chroma_line(const ChromaLaunch& a, void* buf, const typename C2<T>::type* tw) {
+    using V = typename C2<T>::type;
+    constexpr bool whole = sizeof(T) == 4;
     const int f = blockIdx.y;
     const int nf = a.nf_override > 0 ? a.nf_override : a.ctl->nf;
     if (f >= nf) return;
@@ -86,52 +90,66 @@ __global__ __launch_bounds__(64, 3) void k_chroma_line(ChromaLaunch a) {
     const long long total = (long long)nf * a.spfp;
     const double* ll = a.linelocs + (size_t)f * kLinesPerFrame;
     const int rot = 3;  // track 0
-    const double ls_a = ll[la], le_a = ll[la + 1];
-    const double ph_a = (double)((la * rot + 0) & 3) * (M_PI * 0.5);
-    double ls_b = 0.0, le_b = 0.0, ph_b = 0.0;
-    if (has_b) {
-        ls_b = ll[lb];
-        le_b = ll[lb + 1];
-        ph_b = (double)((lb * rot + 0) & 3) * (M_PI * 0.5);
+    V v[16];
+    {
+        const double ls = ll[la], le = ll[la + 1];
+        double s, c;
+        het_start(a, la, (double)((la * rot + 0) & 3) * (M_PI * 0.5), &s, &c);
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            v[r].x = (T)(chroma_resample(a, ls, le, (int)threadIdx.x + 64 * r, total) * -c);
+            het_step(a, &s, &c);
+        }
     }
-    double2 v[16];
-    double het[16];
-    chroma_het(a, la, ph_a, het);
-#pragma unroll
-    for (int r = 0; r < 16; r++) v[r].x = chroma_resample(a, ls_a, le_a, (int)threadIdx.x + 64 * r, total) * het[r];
     if (has_b) {
-        chroma_het(a, lb, ph_b, het);
+        const double ls = ll[lb], le = ll[lb + 1];
+        double s, c;
+        het_start(a, lb, (double)((lb * rot + 0) & 3) * (M_PI * 0.5), &s, &c);
 #pragma unroll
-        for (int r = 0; r < 16; r++) v[r].y = chroma_resample(a, ls_b, le_b, (int)threadIdx.x + 64 * r, total) * het[r];
+        for (int r = 0; r < 16; r++) {
+            v[r].y = (T)(chroma_resample(a, ls, le, (int)threadIdx.x + 64 * r, total) * -c);
+            het_step(a, &s, &c);
+        }
     } else {
 #pragma unroll
-        for (int r = 0; r < 16; r++) v[r].y = 0.0;
+        for (int r = 0; r < 16; r++) v[r].y = (T)0;
     }
     // band-pass: bins with lo <= |k| <= hi kept (chroma_decode.cu:163-182)
     const int lo = a.lo, hi = a.hi;
-    filter1024_wave<double, kChromaWhole>(buf, v, a.tw, [&](double2 z, int k) {
+    filter1024_wave<T, whole>(buf, v, tw, [&](V z, int k) {
         const int kk = k <= kChromaFFT / 2 ? k : kChromaFFT - k;
         if (kk < lo || kk > hi) {
-            z.x = 0.0;
-            z.y = 0.0;
+            z.x = (T)0;
+            z.y = (T)0;
         }
         return z;
     });
+    const T scale = (T)a.fft_scale;
     double* da = a.burst + ((size_t)f * kFieldLines + la) * kBurstLen;
     float* fa = a.filt + ((size_t)f * kFieldLines + la) * kLineLen;
 #pragma unroll
     for (int r = 0; r < 16; r++) {
         const int col = (int)threadIdx.x + 64 * r;
+        const T xa = v[r].x * scale, xb = v[r].y * scale;
         if (col < kLineLen) {
-            fa[col] = (float)(v[r].x * a.fft_scale);
-            if (has_b) fa[kLineLen + col] = (float)(v[r].y * a.fft_scale);
+            fa[col] = (float)xa;
+            if (has_b) fa[kLineLen + col] = (float)xb;
         }
         const int bi = col - a.burst_start;
         if (bi >= 0 && bi < kBurstLen) {
-            da[bi] = v[r].x * a.fft_scale;
-            if (has_b) da[kBurstLen + bi] = v[r].y * a.fft_scale;
+            da[bi] = (double)xa;
+            if (has_b) da[kBurstLen + bi] = (double)xb;
         }
     }
+}
+
+__global__ __launch_bounds__(64, 3) void k_chroma_line(ChromaLaunch a) {
+    __shared__ __align__(16) double buf[lds_elems(kChromaFFT)];
+    chroma_line<double>(a, buf, a.tw);
+}
+__global__ __launch_bounds__(64, 4) void k_chroma_line_f32(ChromaLaunch a) {
+    __shared__ __align__(16) float2 buf[lds_elems(kChromaFFT)];
+    chroma_line<float>(a, buf, a.tw_f);
 }
 
 // ACC + u16 of every line (chroma_decode.cu:188-231) with the resolved track:
@@ -330,7 +348,9 @@ hipError_t run_chroma(ChromaLaunch a, DecodeState* dst, int max_fields, double* 
     ChromaLaunch m = a;
     m.line0 = 0;
     m.line_end = kFieldLines;
-    hipLaunchKernelGGL(k_chroma_line, dim3((m.line_end - m.line0 + 1) / 2, max_fields), dim3(64), 0, st, m);
+    const dim3 lg((m.line_end - m.line0 + 1) / 2, max_fields);
+    if (a.f32) hipLaunchKernelGGL(k_chroma_line_f32, lg, dim3(64), 0, st, m);
+    else hipLaunchKernelGGL(k_chroma_line, lg, dim3(64), 0, st, m);
     hipLaunchKernelGGL(k_chroma_metrics, dim3(max_fields), dim3(64), 0, st, a.burst, a.ctl,
                        a.nf_override, m0, m1, cs_tab, need_iq ? iq0 : nullptr);
     hipLaunchKernelGGL(k_chroma_resolve, dim3(1), dim3(64), 0, st, a.ctl, dst, a.nf_override, m0, m1, iq0, out_track,

@@ -117,7 +117,9 @@ struct ChromaLaunch {
     float* filt;                // band-passed lines, track 0, scaled: [field][263][910]
     uint16_t* out;
     const double2* tw;
+    const float2* tw_f;         // the same twiddles in FP32 (f32)
     const double2* mask;        // band-pass bins: (1, 0) kept, (0, 0) removed
+    int f32;                    // line filter in FP32 (the reduced-precision forms)
 };
 
 

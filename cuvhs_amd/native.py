@@ -36,7 +36,8 @@ class CuvhsError(RuntimeError):
 
 class Config(ctypes.Structure):
     _fields_ = [("system", _i), ("sample_rate_mhz", _d), ("batch_fields", _i), ("precision", _i),
-                ("stream", _vp), ("device", _i), ("input_format", _i), ("dc_correct", _i), ("line_locator", _i)]
+                ("stream", _vp), ("device", _i), ("input_format", _i), ("dc_correct", _i), ("line_locator", _i),
+                ("block_k1a", _i), ("block_k1b", _i), ("block_pre", _i)]
 
 
 class Format(ctypes.Structure):
@@ -92,6 +93,7 @@ _SIGS = {
     "cuvhs_profile_work": (_i, [_vp, ctypes.POINTER(_u64), ctypes.POINTER(_i), ctypes.POINTER(_i)]),
     "cuvhs_k1_matrix_flops": (_i, [_vp, ctypes.POINTER(_d)]),
     "cuvhs_k1_blocks": (_i, [_vp, ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_i)]),
+    "cuvhs_prebpf_block": (_i, [_vp, ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_i)]),
 }
 
 _lib = None
@@ -214,12 +216,15 @@ class Decoder:
     """One decode context (Pipeline, pipeline.h:28-114) on a HIP device."""
 
     def __init__(self, sample_rate_mhz=28.0, batch_fields=64, device=0, input_format="u8", dc_correct=False,
-                 line_locator="k2b", precision="fp64"):
+                 line_locator="k2b", precision="fp64", blocks=None):
+        """blocks: optional (k1a, k1b, pre) block sizes, 0 for the tuned
+        or modelled choice (cuvhs_config.block_*)."""
         self.input_format = INPUT_FORMATS[input_format]
         self.precision = precision
+        ba, bb, bp = blocks or (0, 0, 0)
         cfg = Config(0, float(sample_rate_mhz), int(batch_fields), PRECISIONS[precision], None, int(device),
-                     self.input_format,
-                     1 if dc_correct else 0, LINE_LOCATORS[line_locator])
+                     self.input_format, 1 if dc_correct else 0, LINE_LOCATORS[line_locator], int(ba), int(bb),
+                     int(bp))
         h = _vp()
         rc = lib().cuvhs_create(ctypes.byref(cfg), ctypes.byref(h))
         if rc != 0:
@@ -284,6 +289,14 @@ class Decoder:
         """K1 block shapes: (block_a, valid_a, block_b, valid_b)."""
         v = [_i() for _ in range(4)]
         check(lib().cuvhs_k1_blocks(self.h, *[ctypes.byref(x) for x in v]), self.h, "k1_blocks")
+        return tuple(x.value for x in v)
+
+    def prebpf_block(self):
+        """Chroma pre-band-pass block shape and where the context's block
+        sizes came from: (block, valid, source), source 0 config, 1 tuned
+        table, 2 cost model."""
+        v = [_i() for _ in range(3)]
+        check(lib().cuvhs_prebpf_block(self.h, *[ctypes.byref(x) for x in v]), self.h, "prebpf_block")
         return tuple(x.value for x in v)
 
     def decode_chunked(self, d_capture):

@@ -56,8 +56,9 @@ enum { CUVHS_SYSTEM_NTSC = 0 };
  * fm_demod.cu:396-486, chroma_decode.cu:568-635).
  * FP64: the reference's precision, the default.
  * FP32: the plain FP32 form: K1's transforms, phase and envelope, the chroma
- *   pre-band-pass and K5 in FP32, demod / demod_05 / envelope stored as
- *   float; FP64 only in line locations and K4's positions.
+ *   pre-band-pass, K5 and the chroma line filter in FP32, demod / demod_05 /
+ *   envelope stored as float; FP64 only in line locations and K4's
+ *   positions (and the heterodyne and resampling coordinates).
  * MIXED_F16 / MIXED_BF16: K1's RF band-pass and Hilbert filter as Toeplitz
  *   products on the matrix cores (u8 samples exact in f16 / bf16, taps split
  *   hi + lo, FP32 accumulation), K1's video filters and the rest as the FP32
@@ -91,6 +92,16 @@ typedef struct cuvhs_config {
     int dc_correct;           /* subtract each chunk read's mean (--dc-correct,
                                  InputConditioning, raw_reader.h:21-23) */
     int line_locator;         /* CUVHS_LINELOC_* (0 = K2b) */
+    /* Block sizes of the overlap-save convolutions (no counterpart in the
+     * reference, whose transforms are one cuFFT per field segment): K1 part
+     * A and part B (4096 or 8192 points; part A is ignored by the mixed
+     * forms, whose part A is a matrix-core product) and the chroma
+     * pre-band-pass (8192 or 16384).  0 = the size the autotuner measured
+     * fastest on this device for this sample rate and precision
+     * (cuvhs_amd/autotune.py, table cuvhs_amd/csrc/tuning_gfx950.h), or, for
+     * a rate or form it did not measure, the library's cost model.  A size
+     * the filter's support does not fit is refused (CUVHS_E_ARG). */
+    int block_k1a, block_k1b, block_pre;
 } cuvhs_config;
 
 /* Derived timing constants (VideoFormat, video_format.cpp:23-104). */
@@ -223,8 +234,8 @@ int cuvhs_dropouts_noalloc(double sample_rate_mhz, const double* d_env, const do
  * d_y = B * IFFT(FFT(d_x) * d_g) for one B-point block of complex values
  * (double2, or float2 with is_f32), d_g in natural bin order.  B = 4096
  * (K1 blocks, FP64 or FP32), 8192 (FP64 or FP32), 16384 (the chroma
- * pre-band-pass, FP64 or FP32), 1024 (FP64, the chroma line filter); other
- * shapes return CUVHS_E_ARG. */
+ * pre-band-pass, FP64 or FP32), 1024 (the chroma line filter, FP64 or
+ * FP32); other shapes return CUVHS_E_ARG. */
 int cuvhs_selftest_block(int B, int is_f32, const void* d_x, const void* d_g, void* d_y);
 
 /* Self-test of the FP64 arctangent of K1's phase (no counterpart in the
@@ -252,6 +263,10 @@ int cuvhs_profile_work(const cuvhs_ctx* ctx, uint64_t* samples, int* chunks, int
  * filters).  Any pointer may be NULL.  (No counterpart in the reference,
  * which transforms whole segments, fm_demod.cu:396-486.) */
 int cuvhs_k1_blocks(const cuvhs_ctx* ctx, int* block_a, int* valid_a, int* block_b, int* valid_b);
+/* Block size and outputs per block of the chroma pre-band-pass, and where
+ * the context's three block sizes came from: 0 = the caller's config, 1 =
+ * the autotuned table, 2 = the cost model (no counterpart in the reference). */
+int cuvhs_prebpf_block(const cuvhs_ctx* ctx, int* block, int* valid, int* source);
 /* Floating-point operations the matrix cores execute in K1 part A per
  * demodulated segment (mixed forms: every 32x32x16 product of the hi/lo tap
  * operands, padding steps included; 0 in the FFT forms), for the MFMA

@@ -28,6 +28,11 @@ def test_create_rejects_unsupported_configs():
         assert native.lib().cuvhs_create(ctypes.byref(cfg), ctypes.byref(h)) == native.E_UNSUPPORTED
     assert native.lib().cuvhs_create(ctypes.byref(native.Config(0, 28.0, 0, 0, None, 0)), ctypes.byref(h)) == native.E_ARG
     assert native.lib().cuvhs_create(None, ctypes.byref(h)) == native.E_ARG
+    # block sizes outside the supported ones (K1 4096 / 8192, pre-band-pass
+    # 8192 / 16384) are refused before any device work
+    for blocks in ((2048, 0, 0), (0, 16384, 0), (0, 0, 4096), (4097, 4096, 16384)):
+        cfg = native.Config(0, 28.0, 64, 0, None, 0, 0, 0, 0, *blocks)
+        assert native.lib().cuvhs_create(ctypes.byref(cfg), ctypes.byref(h)) == native.E_ARG, blocks
 
 
 def test_null_context_calls_fail_cleanly():

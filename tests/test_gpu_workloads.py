@@ -267,3 +267,18 @@ def test_aux_queue_schedule_u16_dc_correct(torch, N, monkeypatch):
     got = _digest(dec.decode_capture(u16))
     dec.close()
     assert got == want and want[0] >= 48
+
+
+@pytest.mark.parametrize("blocks", [(8192, 8192, 8192), (4096, 4096, 16384)])
+def test_block_sizes_keep_parity(torch, N, cap1_prefix, ref1_prefix, blocks):
+    """Any block size the filters fit (the autotuner's candidates, forced
+    through cuvhs_config.block_*) decodes the 140-field prefix of the
+    600-field workload in the FP64 form within 1 LSB of the oracle with
+    identical decisions: the block size changes which taps are truncated and
+    the rounding order, nothing the decisions see."""
+    dec = N.Decoder(28.0, 64, blocks=blocks)
+    ka, _, kb, _ = dec.k1_blocks()
+    kp, _, src = dec.prebpf_block()
+    assert (ka, kb, kp) == blocks and src == 0
+    compare(dec.decode_capture(dev(torch, cap1_prefix)), ref1_prefix)
+    dec.close()

@@ -356,9 +356,10 @@ def main():
     if os.path.exists(pmc_path) and args.workload == "600f_28mhz":  # counters taken on that workload
         with open(pmc_path) as f:
             pmc = json.load(f)
-        traffic = pmc.get("bytes_per_launch", {}).get(dom)
-        if pmc.get("bytes_per_decode"):
-            traffic_ratio = round(pmc["bytes_per_decode"] / (bpf * fields), 2)
+        if pmc.get("form") == args.form:   # and on one precision form
+            traffic = pmc.get("bytes_per_launch", {}).get(dom)
+            if pmc.get("bytes_per_decode"):
+                traffic_ratio = round(pmc["bytes_per_decode"] / (bpf * fields), 2)
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                 "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
                 "alg_bytes_per_field": bpf, "fields_per_launch": round(fields_per_launch, 2),

@@ -185,12 +185,13 @@ def serving_main(args):
     reqs = serving.make_requests(16, 4.0, 30, pool_fields, pool_fields)
     res = {}
     burst = [dict(r, arrival=0.0) for r in reqs]
+    g = bool(args.graphs)
     for form in ("mixed-fp16", "fp32", "fp64"):
         # one untimed pass warms the contexts' kernels, then the timed runs:
         # Poisson arrivals, and all 16 at once (the saturated aggregate rate)
-        serving.run(d_pool, reqs[:2], precision=form)
-        res[form] = serving.summarize(serving.run(d_pool, reqs, precision=form))
-        res[form]["all_at_once"] = serving.summarize(serving.run(d_pool, burst, precision=form))
+        serving.run(d_pool, reqs[:2], precision=form, graphs=g)
+        res[form] = serving.summarize(serving.run(d_pool, reqs, precision=form, graphs=g))
+        res[form]["all_at_once"] = serving.summarize(serving.run(d_pool, burst, precision=form, graphs=g))
         log(f"{form}: {res[form]}")
     line = {
         "metric": "serving: per-field latency p50/p99 and aggregate NTSC fields/s, 16 concurrent captures",
@@ -201,7 +202,7 @@ def serving_main(args):
         "higher_is_better": True,
         "data": "synthetic",
         "config": {"workload": "16 requests, Poisson 4 req/s, fs in {28, 35.8, 40} MHz, 30..3000 fields log-uniform",
-                   "workers": 4, "batch_fields": 64, "hbm_budget_gb": 64},
+                   "workers": 4, "batch_fields": 64, "hbm_budget_gb": 64, "stage_graphs": g},
         "forms": res,
     }
     print(json.dumps(line), flush=True)
@@ -217,7 +218,7 @@ def guard_blocks(dec, d_cap):
 def time_form(native, torch, d_cap, mhz, args, form):
     """One precision form on the same capture and schedule as the headline:
     warmup, K timed decodes, one serialised decode for stage times."""
-    d = native.Decoder(mhz, args.batch, device=0, precision=form)
+    d = native.Decoder(mhz, args.batch, device=0, precision=form, graphs=bool(args.graphs))
     n = 0
     for _ in range(args.warmup):
         d.reset()
@@ -260,6 +261,9 @@ def main():
                     help="instead: the CPU baseline on the 4-field and 600-field 28 MHz captures and 640 fields of "
                          "the 40 MHz capture with every host thread, and the 4-field capture with one thread")
     ap.add_argument("--no-cli", action="store_true", help="skip the CLI end-to-end figure")
+    ap.add_argument("--graphs", type=int, default=0, choices=(0, 1),
+                    help="1: the decode replays captured HIP graphs of its stages (cuvhs_config.graphs); "
+                         "0: it launches the kernels one by one")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 4 plain device copies of MIB MiB (calibrates HBM byte counters) and exit")
     args = ap.parse_args()
@@ -299,7 +303,7 @@ def main():
     log(f"generated {cap.size} samples in {time.perf_counter() - t:.1f} s")
     dev = torch.device("cuda:0")
     d_cap = torch.from_numpy(cap).to(dev)
-    dec = native.Decoder(mhz, args.batch, device=0, precision=args.form)
+    dec = native.Decoder(mhz, args.batch, device=0, precision=args.form, graphs=bool(args.graphs))
     fmt = dec.fmt
     matrix_flops_per_seg = dec.k1_matrix_flops()
     fields = 0
@@ -443,7 +447,8 @@ def main():
         "config": {"workload": wl["desc"],
                    "fields_per_step": fields, "batch_fields": args.batch, "sample_rate_mhz": mhz,
                    "precision": args.form + ("" if args.form == "fp64" else " (chroma pre-band-pass fp32)"),
-                   "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4"))},
+                   "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4")),
+                   "stage_graphs": bool(args.graphs)},
         "hbm_gbs_algorithmic": round(value * bytes_per_field / 1e9, 2),
         "stage_ms_per_step": {k: round(v, 3) for k, v in stage_ms.items()},
         "stage_ms_per_step_isolated": {k: round(v, 3) for k, v in iso.items()},

@@ -38,6 +38,12 @@ constexpr int kStages = 13;
 // of chunk c+1-kSlots has finished, so with three the heads never wait for
 // the tail launched just before them.
 constexpr int kSlots = 3;
+// Stage graphs of a slot (graph mode): the per-read mean (--dc-correct), the
+// head (K1, guard band, events, VSYNC), the pre-band-pass, the pulse stage
+// (K2, K2b per field), and the tail in four pieces between its cross-stream
+// waits (line locations and K4; K5 and the dropout classes on the fork
+// stream; chroma; the dropout scan, metadata and the control block's copy).
+enum { kGMean, kGHead, kGPre, kGPulse, kGTail1, kGFork, kGTail2, kGTail3, kGraphKinds };
 constexpr int kMarks = 18;   // 0-5 head, 6-14 tail, 15-17 the early K2 / K2b-field work
 // Guard band of the mixed forms.  A part-B block of the FP64 design is
 // recomputed in FP64 when a demod_05 output of the FP32 part B in its 256-
@@ -97,6 +103,13 @@ struct Slot {
     int max_fields = 0, fk1 = 0;
     // demod/d05/env hold K1 output in [0, k1_extent) and zeros beyond
     long long k1_extent = 0;
+    // graph mode of cuvhs_decode_capture: the chunk's plan (host copy, filled
+    // when the chunk is planned, and its device copy, written ahead of the
+    // chunk's head) and the slot's stage graphs, captured at first use and
+    // replayed for every chunk
+    ChunkPlan hplan{};
+    ChunkPlan* plan = nullptr;
+    hipGraphExec_t graph[kGraphKinds] = {};
 };
 
 struct cuvhs_ctx {
@@ -141,6 +154,7 @@ struct cuvhs_ctx {
     float2* tw_1024f = nullptr;     // the chroma line filter's twiddles in FP32 (chroma_f32)
     int chroma_f32 = 0;             // chroma line filter in FP32 (the reduced-precision forms)
     int block_source = 2;           // block sizes from: 0 the config, 1 the tuned table, 2 the cost model
+    int graphs = 0;                 // cuvhs_decode_capture replays captured stage graphs (cuvhs_config.graphs)
     double* cs_tab = nullptr;       // cos/sin of the burst phase detector
     double2* chroma_mask = nullptr; // per-line band-pass: (1, 0) kept bins, (0, 0) others
     float* chroma_filt = nullptr;   // band-passed chroma lines of a chunk (track 0)
@@ -317,8 +331,11 @@ ChromaLaunch chroma_launch(cuvhs_ctx* c, const ChunkCtl* ctl, int nf_override, c
 // ctl (optional): the chunk's control block, initialised by K1's first
 // workgroup.  thr_bits (optional): receives the sync-threshold bitmask of d05.
 // Outputs go to the slot's demod / d05 / env in the form's storage types.
+// plan (graph mode): the kernels read the chunk's values from it; nfk1 is then
+// the batch (the grids' size) and K1's outputs past the chunk are cleared on
+// the device from the plan's extent.
 int k1_launch(cuvhs_ctx* c, Slot& s, const void* chunk, long long n_valid, int nfk1, hipStream_t st,
-              ChunkCtl* ctl = nullptr, unsigned int* thr_bits = nullptr) {
+              ChunkCtl* ctl = nullptr, unsigned int* thr_bits = nullptr, const ChunkPlan* plan = nullptr) {
     const Fmt& f = c->fmt;
     long long T = (long long)nfk1 * f.spfp;
     const double fs_tau = f.sample_rate / (2.0 * 3.14159265358979323846);
@@ -346,6 +363,7 @@ int k1_launch(cuvhs_ctx* c, Slot& s, const void* chunk, long long n_valid, int n
         m.thr_bits = thr_bits;
         m.blk_env = c->g_env;
         m.nblk = c->m_nblk;
+        m.plan = plan;
         HIPCHK(c, run_k1_part_a_mfma(c->mixed_bf16, m, nfk1, st));
     } else {
         K1ALaunch a{};
@@ -369,6 +387,7 @@ int k1_launch(cuvhs_ctx* c, Slot& s, const void* chunk, long long n_valid, int n
         a.ctl_raw_offset = (long long)s.raw_offset;
         a.ctl_fields_k1 = s.fk1;
         a.thr_bits = thr_bits;
+        a.plan = plan;
         HIPCHK(c, run_k1_part_a(c->k1.rf.B, c->k1_f32, a, nfk1, st));
     }
     mark(c, s, 1, st);
@@ -389,6 +408,7 @@ int k1_launch(cuvhs_ctx* c, Slot& s, const void* chunk, long long n_valid, int n
     b.d05_f32 = c->d05_f32;
     b.thr_bits = thr_bits;
     b.thr = f.pulse_threshold_hz;
+    b.plan = plan;
     if (c->mixed) {
         b.gflag = c->g_flag;
         b.gdelta = c->g_delta;
@@ -421,6 +441,7 @@ int k1_launch(cuvhs_ctx* c, Slot& s, const void* chunk, long long n_valid, int n
         g.listB = c->g_listB;
         g.counts = c->g_counts;
         g.ctl = ctl;
+        g.plan = plan;
         HIPCHK(c, run_guard_lists(g, st));
         K1ALaunch a{};
         a.cap = chunk;
@@ -438,6 +459,7 @@ int k1_launch(cuvhs_ctx* c, Slot& s, const void* chunk, long long n_valid, int n
         a.rawdemod = c->g_raw64;
         a.blist = c->g_listA;
         a.bcount = c->g_counts;
+        a.plan = plan;
         HIPCHK(c, run_k1_part_a(c->k1g.rf.B, 0, a, nfk1, st));
         K1BLaunch gb{};
         gb.rawdemod = c->g_raw64;
@@ -457,14 +479,19 @@ int k1_launch(cuvhs_ctx* c, Slot& s, const void* chunk, long long n_valid, int n
         gb.thr = f.pulse_threshold_hz;
         gb.blist = c->g_listB;
         gb.bcount = c->g_counts + 1;
+        gb.plan = plan;
         HIPCHK(c, run_k1_part_b(c->k1g.video.B, 0, gb, nfk1, st));
     }
     mark(c, s, 3, st);
     // beyond the demodulated region the buffers read as zero: only what an
     // earlier, longer chunk left there needs clearing
+    const int ws = c->sig_f32 ? 1 : 2, wd = c->d05_f32 ? 1 : 2;   // 32-bit words per sample
+    if (plan) {
+        HIPCHK(c, run_clear_past(s.demod, ws, s.d05, wd, s.env, ws, plan, st));
+        return 0;
+    }
     if (s.k1_extent > T) {
         const long long n = s.k1_extent - T;
-        const int ws = c->sig_f32 ? 1 : 2, wd = c->d05_f32 ? 1 : 2;   // 32-bit words per sample
         HIPCHK(c, run_fill_zero(reinterpret_cast<unsigned int*>(s.demod) + T * ws, n * ws, st));
         HIPCHK(c, run_fill_zero(reinterpret_cast<unsigned int*>(s.d05) + T * wd, n * wd, st));
         HIPCHK(c, run_fill_zero(reinterpret_cast<unsigned int*>(s.env) + T * ws, n * ws, st));
@@ -474,7 +501,7 @@ int k1_launch(cuvhs_ctx* c, Slot& s, const void* chunk, long long n_valid, int n
 }
 
 int prebpf_launch(cuvhs_ctx* c, const void* chunk, long long n_valid, int nf_max, float* out, const double* dc,
-                  hipStream_t st) {
+                  hipStream_t st, const ChunkPlan* plan = nullptr) {
     PreLaunch p{};
     p.cap = chunk;
     p.in_fmt = c->cfg.input_format;
@@ -488,6 +515,7 @@ int prebpf_launch(cuvhs_ctx* c, const void* chunk, long long n_valid, int nf_max
     p.G = c->G_p;
     p.tw = c->tw_p;
     p.out = out;
+    p.plan = plan;
     HIPCHK(c, run_prebpf(c->pre.bpf.B, c->pre_f32, p, nf_max, st));
     return 0;
 }
@@ -726,6 +754,204 @@ int finish_tail(cuvhs_ctx* c, Slot& s) {
     return 0;
 }
 
+// ---- graph mode of cuvhs_decode_capture ----
+//
+// Each stage's kernel sequence is captured once per slot, on the one stream
+// it runs on and between the cross-stream waits (which stay outside the
+// graphs, as in the launch-by-launch schedule), and replayed for every chunk
+// the slot takes.  Every varying value comes from the slot's device plan: the
+// host fills it when it plans the chunk and a one-thread kernel writes it to
+// the device (as its argument) ahead of the chunk's head, after the tail of
+// the chunk that last used the slot has finished.  The grids are sized for
+// the batch, and the kernels return past the chunk's fields.
+//
+// The graphs are captured by cuvhs_create, one capture at a time in the
+// process (g_capture_mutex) and in relaxed mode: HIP's stream capture is not
+// isolated between threads (measured: a capture in one thread made another
+// thread's hipStreamSynchronize and synchronous hipMemcpy fail with "not
+// permitted when stream is capturing"), so no capture may overlap a decode.
+std::mutex g_capture_mutex;
+
+// The kernel sequence of each stage graph, as launched on st.
+int seq_mean(cuvhs_ctx* c, Slot& s, hipStream_t st) {
+    const long long S = (long long)c->F * c->fmt.spfp;
+    HIPCHK(c, run_input_mean(nullptr, c->cfg.input_format, S, s.dc_partial, s.dc, st, s.plan));
+    return 0;
+}
+int seq_head(cuvhs_ctx* c, Slot& s, hipStream_t st) {
+    const Fmt& f = c->fmt;
+    const int F = c->F;
+    const long long S = (long long)F * f.spfp;   // the batch: the grids' size
+    const int r = k1_launch(c, s, nullptr, S, F, st, s.ctl, c->thr_bits, s.plan);
+    if (r) return r;
+    HIPCHK(c, run_events_bits(c->thr_bits, S, s.wg_count, s.wg_off, s.ev_masks, s.events, s.ctl, st, s.plan));
+    HIPCHK(c, run_vsync_cluster(s.events, s.ctl, S, f.vsync_min_w, f.vsync_max_w, F * 20, c->cand, s.offsets, F,
+                                f.samples_per_field, f.samples_per_line, F, 1, nullptr, st, s.plan));
+    return 0;
+}
+int seq_pre(cuvhs_ctx* c, Slot& s, hipStream_t st) {
+    return prebpf_launch(c, nullptr, (long long)c->F * c->fmt.spfp, c->F, s.craw, s.dc, st, s.plan);
+}
+int seq_pulse(cuvhs_ctx* c, Slot& s, hipStream_t st) {
+    const Fmt& f = c->fmt;
+    HIPCHK(c, run_sync_pulses(s.events, s.ctl, s.offsets, f.spfp, s.k2s, s.k2l, s.k2c, c->F, 0, st));
+    if (c->cfg.line_locator != CUVHS_LINELOC_K3)
+        HIPCHK(c, run_k2b_fields(s.ctl, 0, c->F, f, s.k2s, s.k2l, s.k2c, s.vps, s.vpl, s.vptg, (K2bField*)s.summ, st));
+    return 0;
+}
+int seq_tail1(cuvhs_ctx* c, Slot& s, hipStream_t st) {
+    const Fmt& f = c->fmt;
+    if (c->cfg.line_locator == CUVHS_LINELOC_K3) {
+        HIPCHK(c, run_k3(s.ctl, 0, c->F, f, s.k2s, s.k2l, s.k2c, c->pt, c->ll, c->isf_k2b, st));
+    } else {
+        HIPCHK(c, run_k2b_cadence(s.ctl, s.ctl, c->state, 0, c->F, f, s.offsets, s.vps, s.vpl, s.vptg,
+                                  (const K2bField*)s.summ, (K2bAnchorOut*)c->anchors, c->ps, c->pl, c->pc, c->pt,
+                                  c->ll, c->isf_k2b, st));
+    }
+    HIPCHK(c, run_hsync_refine(s.d05, c->d05_f32, c->ll, s.ctl, 0, c->F, f, st));
+    return 0;
+}
+int seq_fork(cuvhs_ctx* c, Slot& s, hipStream_t st) {
+    HIPCHK(c, run_tbc_resample(s.demod, c->sig_f32, c->ll, nullptr, s.ctl, 0, c->F, c->fmt, st, s.plan));
+    HIPCHK(c, run_dropout_classes(s.env, c->sig_f32, c->ll, s.ctl, 0, c->F, c->fmt, c->do_partial, c->do_cls, st));
+    return 0;
+}
+int seq_tail2(cuvhs_ctx* c, Slot& s, hipStream_t st) {
+    ChromaLaunch cl = chroma_launch(c, s.ctl, 0, s.craw);
+    cl.plan = s.plan;
+    HIPCHK(c, run_chroma(cl, c->state, c->F, c->m0, c->m1, c->cs_tab, c->iq0, c->track, c->phase, 1, st));
+    return 0;
+}
+int seq_tail3(cuvhs_ctx* c, Slot& s, hipStream_t st) {
+    HIPCHK(c, run_dropout_scan(c->do_cls, c->ll, s.ctl, 0, c->F, c->fmt, c->luma, c->chroma, c->dol, c->dos, c->doe,
+                               c->doc, st, s.plan));
+    HIPCHK(c, run_field_meta(s.ctl, c->state, s.offsets, c->isf_k2b, c->file_loc, c->isf_out, st, s.plan));
+    HIPCHK(c, hipMemcpyAsync(s.hctl, s.ctl, sizeof(ChunkCtl), hipMemcpyDeviceToHost, st));
+    return 0;
+}
+
+// The fork stream of cuvhs_decode_capture's tails: the aux stream when the
+// process has a hardware queue for it, else the head stream.
+hipStream_t fork_stream(const cuvhs_ctx* c) { return c->aux_queue ? c->aux : c->stream; }
+
+// Capture seq's launches on st into *g.
+int capture_graph(cuvhs_ctx* c, Slot& s, hipGraphExec_t* g, hipStream_t st,
+                  int (*seq)(cuvhs_ctx*, Slot&, hipStream_t)) {
+    hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed);
+    if (e != hipSuccess) return fail(c, e, "hipStreamBeginCapture");
+    const int r = seq(c, s, st);
+    hipGraph_t graph = nullptr;
+    e = hipStreamEndCapture(st, &graph);
+    if (r != 0) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return r;
+    }
+    if (e != hipSuccess) return fail(c, e, "hipStreamEndCapture");
+    e = hipGraphInstantiate(g, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+        *g = nullptr;
+        return fail(c, e, "hipGraphInstantiate");
+    }
+    return 0;
+}
+
+// Every slot's stage graphs (cuvhs_create, graph mode).
+int capture_graphs(cuvhs_ctx* c) {
+    std::lock_guard<std::mutex> lock(g_capture_mutex);
+    for (Slot& s : c->slot) {
+        int r = 0;
+        if (c->cfg.dc_correct && (r = capture_graph(c, s, &s.graph[kGMean], c->stream, seq_mean)) != 0) return r;
+        if ((r = capture_graph(c, s, &s.graph[kGHead], c->stream, seq_head)) != 0) return r;
+        if ((r = capture_graph(c, s, &s.graph[kGPre], c->side, seq_pre)) != 0) return r;
+        if ((r = capture_graph(c, s, &s.graph[kGPulse], c->side, seq_pulse)) != 0) return r;
+        if ((r = capture_graph(c, s, &s.graph[kGTail1], c->tail, seq_tail1)) != 0) return r;
+        if ((r = capture_graph(c, s, &s.graph[kGFork], fork_stream(c), seq_fork)) != 0) return r;
+        if ((r = capture_graph(c, s, &s.graph[kGTail2], c->tail, seq_tail2)) != 0) return r;
+        if ((r = capture_graph(c, s, &s.graph[kGTail3], c->tail, seq_tail3)) != 0) return r;
+    }
+    return 0;
+}
+
+// The outputs of cuvhs_decode_capture: the caller's arrays, where given.
+struct OutBases {
+    uint16_t *luma, *chroma;
+    uint64_t* file_loc;
+    int32_t *is_first, *phase_id, *do_count, *do_line, *do_startx, *do_endx;
+};
+
+// The planned chunk of slot s into its host plan: its read, K1's extent
+// (and what an earlier, longer chunk left past it), its VSYNC candidate cap,
+// and where its fields go (out_fields fields precede it in the outputs).
+void fill_plan(cuvhs_ctx* c, Slot& s, const OutBases& ob, long long out_fields) {
+    ChunkPlan& p = s.hplan;
+    const long long T = (long long)s.fk1 * c->fmt.spfp;
+    const size_t tbc = (size_t)kFieldLines * kLineLen;
+    p.chunk = s.chunk;
+    p.n_read = (long long)s.n_read;
+    p.total = T;
+    p.clear_end = std::max(s.k1_extent, T);
+    s.k1_extent = T;
+    p.raw_offset = s.raw_offset;
+    p.fields_k1 = s.fk1;
+    p.max_fields = s.max_fields;
+    p.vsync_cap = s.fk1 * 20;
+    p.luma = ob.luma ? ob.luma + (size_t)out_fields * tbc : c->luma;
+    p.chroma = ob.chroma ? ob.chroma + (size_t)out_fields * tbc : c->chroma;
+    p.file_loc = ob.file_loc ? reinterpret_cast<unsigned long long*>(ob.file_loc) + out_fields : c->file_loc;
+    p.is_first = ob.is_first ? ob.is_first + out_fields : c->isf_out;
+    p.phase_id = ob.phase_id ? ob.phase_id + out_fields : c->phase;
+    p.do_count = ob.do_count ? ob.do_count + out_fields : c->doc;
+    p.do_line = ob.do_line ? ob.do_line + (size_t)out_fields * kMaxDropouts : c->dol;
+    p.do_startx = ob.do_startx ? ob.do_startx + (size_t)out_fields * kMaxDropouts : c->dos;
+    p.do_endx = ob.do_endx ? ob.do_endx + (size_t)out_fields * kMaxDropouts : c->doe;
+}
+
+// Head stage in graph mode: the plan to the device, the per-read mean, the
+// head (K1, guard band, events, VSYNC clustering) on the head stream, then
+// the pre-band-pass and the pulse stage on the side stream.
+int head_stage_graph(cuvhs_ctx* c, Slot& s) {
+    hipStream_t st = c->stream, side = c->side;
+    HIPCHK(c, run_put_plan(s.plan, s.hplan, st));
+    if (c->cfg.dc_correct) HIPCHK(c, hipGraphLaunch(s.graph[kGMean], st));
+    HIPCHK(c, hipEventRecord(s.dc_done, st));   // the plan (and the mean) are on the device
+    HIPCHK(c, hipGraphLaunch(s.graph[kGHead], st));
+    // the control block (field count, next chunk offset) to the host by a
+    // copy: a graph's kernel nodes need not make their stores to host memory
+    // visible when they end, as a kernel launched on its own does
+    HIPCHK(c, hipMemcpyAsync(s.hctl, s.ctl, sizeof(ChunkCtl), hipMemcpyDeviceToHost, st));
+    HIPCHK(c, hipEventRecord(s.head_done, st));
+    HIPCHK(c, hipStreamWaitEvent(side, s.dc_done, 0));
+    HIPCHK(c, hipEventRecord(s.pre0, side));
+    HIPCHK(c, hipGraphLaunch(s.graph[kGPre], side));
+    HIPCHK(c, hipEventRecord(s.pre1, side));
+    HIPCHK(c, hipStreamWaitEvent(side, s.head_done, 0));
+    HIPCHK(c, hipGraphLaunch(s.graph[kGPulse], side));
+    HIPCHK(c, hipEventRecord(s.k2_done, side));
+    return 0;
+}
+
+// Tail stage in graph mode, with the launch-by-launch schedule's waits:
+// line locations and K4; K5 and the dropout classes on the fork stream;
+// chroma once the pre-band-pass is done; the dropout scan (joining the fork),
+// metadata and the control block's copy to the host.
+int tail_stage_graph(cuvhs_ctx* c, Slot& s) {
+    hipStream_t st = c->tail, fork = fork_stream(c);
+    HIPCHK(c, hipStreamWaitEvent(st, s.head_done, 0));
+    HIPCHK(c, hipStreamWaitEvent(st, s.k2_done, 0));
+    HIPCHK(c, hipGraphLaunch(s.graph[kGTail1], st));
+    HIPCHK(c, hipEventRecord(s.k4_done, st));
+    HIPCHK(c, hipStreamWaitEvent(fork, s.k4_done, 0));
+    HIPCHK(c, hipGraphLaunch(s.graph[kGFork], fork));
+    HIPCHK(c, hipEventRecord(s.aux_done, fork));
+    HIPCHK(c, hipStreamWaitEvent(st, s.pre1, 0));
+    HIPCHK(c, hipGraphLaunch(s.graph[kGTail2], st));
+    HIPCHK(c, hipStreamWaitEvent(st, s.aux_done, 0));
+    HIPCHK(c, hipGraphLaunch(s.graph[kGTail3], st));
+    HIPCHK(c, hipEventRecord(s.tail_done, st));
+    return 0;
+}
+
 // ---- scratch of the allocation-free dropout entry point ----
 //
 // cuvhs_dropouts_noalloc takes its device scratch (the per-slice envelope
@@ -835,6 +1061,7 @@ int cuvhs_create(const cuvhs_config* cfg, cuvhs_ctx** out) {
         return CUVHS_E_ARG;
     cuvhs_ctx* c = new cuvhs_ctx();
     c->cfg = *cfg;
+    c->graphs = cfg->graphs != 0;
     c->fmt = make_fmt(cfg->sample_rate_mhz);
     c->F = cfg->batch_fields;
     const Fmt& f = c->fmt;
@@ -1027,6 +1254,7 @@ int cuvhs_create(const cuvhs_config* cfg, cuvhs_ctx** out) {
         e = hipHostMalloc((void**)&s.hctl, sizeof(ChunkCtl), hipHostMallocDefault);
         if (e != hipSuccess) return bad(e, "hipHostMalloc");
         memset(s.hctl, 0, sizeof(ChunkCtl));
+        ALLOC(s.plan, 1);
     }
     for (Slot& s : c->slot) {
         ALLOC(s.k2s, Fz * kMaxPulses);
@@ -1075,6 +1303,13 @@ int cuvhs_create(const cuvhs_config* cfg, cuvhs_ctx** out) {
     reset_state(c->hstate);
     e = hipMemcpy(c->state, c->hstate, sizeof(DecodeState), hipMemcpyHostToDevice);
     if (e != hipSuccess) return bad(e, "state upload");
+    if (c->graphs) {
+        const int r = capture_graphs(c);
+        if (r != 0) {
+            cuvhs_destroy(c);
+            return r;
+        }
+    }
     *out = c;
     return 0;
 }
@@ -1089,6 +1324,8 @@ int cuvhs_destroy(cuvhs_ctx* c) {
         for (hipEvent_t e : {s.head_done, s.tail_done, s.pre0, s.pre1, s.dc_done, s.k4_done, s.aux_done, s.k2_done})
             if (e) (void)hipEventDestroy(e);
         if (s.hctl) (void)hipHostFree(s.hctl);
+        for (hipGraphExec_t g : s.graph)
+            if (g) (void)hipGraphExecDestroy(g);
     }
     if (c->aux) (void)hipStreamDestroy(c->aux);
     if (c->side) (void)hipStreamDestroy(c->side);
@@ -1173,17 +1410,27 @@ int cuvhs_decode_capture(cuvhs_ctx* c, const void* cap, uint64_t cap_len, int ma
     *fields = 0;
     const uint64_t spf = (uint64_t)c->fmt.samples_per_field;
     const size_t tbc = (size_t)kFieldLines * kLineLen;
-    // chunk sizing of Pipeline::run (pipeline.cu:2651-2658)
-    auto plan = [&](Slot& s, uint64_t off) -> bool {
+    // the stages replay their captured graphs unless profiling or diagnostics
+    // need the launch-by-launch schedule
+    const bool graphs = c->graphs && !c->prof && !c->diag && !c->serial;
+    const OutBases ob{d_luma, d_chroma, d_file_loc, d_is_first, d_phase_id, d_do_count, d_do_line, d_do_startx,
+                      d_do_endx};
+    // chunk sizing of Pipeline::run (pipeline.cu:2651-2658); out_fields:
+    // the fields of the chunks before it
+    auto plan = [&](Slot& s, uint64_t off, long long out_fields) -> bool {
         if (off >= cap_len) return false;
         uint64_t remaining = (cap_len - off) / spf;
         if (remaining == 0) return false;
         int this_chunk = (int)std::min<uint64_t>((uint64_t)c->F, remaining);
-        return plan_chunk(c, s, static_cast<const uint8_t*>(cap) + off * sample_bytes(c), cap_len - off, off,
-                          this_chunk);
+        if (!plan_chunk(c, s, static_cast<const uint8_t*>(cap) + off * sample_bytes(c), cap_len - off, off,
+                        this_chunk))
+            return false;
+        if (graphs) fill_plan(c, s, ob, out_fields);
+        return true;
     };
-    if (!plan(c->slot[0], 0)) return 0;
-    int rc = head_stage(c, c->slot[0]);
+    auto head = [&](Slot& s) { return graphs ? head_stage_graph(c, s) : head_stage(c, s); };
+    if (!plan(c->slot[0], 0, 0)) return 0;
+    int rc = head(c->slot[0]);
     if (rc) { sync_all(c); return rc; }
     int total = 0, warn = 0;
     long long chunk = 0;        // the chunk whose head is in flight
@@ -1216,9 +1463,9 @@ int cuvhs_decode_capture(cuvhs_ctx* c, const void* cap, uint64_t cap_len, int ma
         // stream would hold it behind the next pre-band-pass).
         while (finished <= chunk + 1 - kSlots && rc == 0) rc = finish_tail(c, c->slot[finished++ % kSlots]);
         if (rc) break;
-        const bool more = plan(c->slot[(chunk + 1) % kSlots], next);
-        if (more && (rc = head_stage(c, c->slot[(chunk + 1) % kSlots])) != 0) break;
-        if ((rc = tail_stage(c, s, &o, c->aux_queue ? c->aux : c->stream)) != 0) break;
+        const bool more = plan(c->slot[(chunk + 1) % kSlots], next, (long long)total + nf);
+        if (more && (rc = head(c->slot[(chunk + 1) % kSlots])) != 0) break;
+        if ((rc = graphs ? tail_stage_graph(c, s) : tail_stage(c, s, &o, fork_stream(c))) != 0) break;
         total += nf;
         chunk++;
         if (!more) break;

@@ -60,7 +60,15 @@ __device__ __forceinline__ SampleSpan window_span(int f, int spfp, int N, int q_
     return sp;
 }
 
+// The chunk's values from a replayed graph's plan (GuardLaunch::plan).
+__device__ __forceinline__ void plan_guard(GuardLaunch& g) {
+    g.total = g.plan->total;
+    g.nseg = g.plan->fields_k1;
+    g.nsub = ((g.total - 1) >> kGuardSubShift) + 1;
+}
+
 __global__ __launch_bounds__(256) void k_guard_mark(GuardLaunch g) {
+    if (g.plan) plan_guard(g);
     const long long idx = (long long)blockIdx.x * 256 + threadIdx.x;
     if (idx == 0) {
         g.counts[0] = 0;
@@ -117,6 +125,7 @@ __global__ __launch_bounds__(256) void k_guard_mark(GuardLaunch g) {
 }
 
 __global__ __launch_bounds__(256) void k_guard_compact(GuardLaunch g) {
+    if (g.plan) plan_guard(g);
     const long long idx = (long long)blockIdx.x * 256 + threadIdx.x;
     if (idx < g.nsub) g.gflag[idx] = 0;   // read by the marking pass; clear for the next chunk
     if (idx < (long long)g.nseg * g.nbA && g.markA[idx]) {

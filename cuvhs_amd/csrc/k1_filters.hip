@@ -48,8 +48,13 @@ constexpr int kMeanBlocks = 256;
 
 // Per-read mean for --dc-correct: fixed slices, fixed-order tree sums (the
 // reference sums sequentially on the host; the orders differ at rounding).
-__global__ __launch_bounds__(256) void k_input_sum(const void* cap, int fmt, long long n, double* partial) {
+__global__ __launch_bounds__(256) void k_input_sum(const void* cap, int fmt, long long n, double* partial,
+                                                   const ChunkPlan* plan) {
     __shared__ double red[256];
+    if (plan) {
+        cap = plan->chunk;
+        n = plan->n_read;
+    }
     const long long per = (n + kMeanBlocks - 1) / kMeanBlocks;
     const long long a = (long long)blockIdx.x * per;
     const long long b = a + per < n ? a + per : n;
@@ -64,8 +69,10 @@ __global__ __launch_bounds__(256) void k_input_sum(const void* cap, int fmt, lon
     if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
 }
 
-__global__ __launch_bounds__(256) void k_input_mean(const double* partial, long long n, double* mean) {
+__global__ __launch_bounds__(256) void k_input_mean(const double* partial, long long n, double* mean,
+                                                    const ChunkPlan* plan) {
     __shared__ double red[kMeanBlocks];
+    if (plan) n = plan->n_read;
     red[threadIdx.x] = partial[threadIdx.x];
     __syncthreads();
     for (int o = kMeanBlocks / 2; o > 0; o >>= 1) {
@@ -206,8 +213,22 @@ __device__ __forceinline__ void k1_part_a_block(const K1ALaunch& a, int f, int b
     }
 }
 
+// The chunk's values from a replayed graph's plan (K1ALaunch::plan).
+__device__ __forceinline__ void plan_chunk_a(K1ALaunch& a) {
+    const ChunkPlan& p = *a.plan;
+    a.cap = p.chunk;
+    a.n_valid = p.n_read;
+    a.total = p.total;
+    a.ctl_raw_offset = (long long)p.raw_offset;
+    a.ctl_fields_k1 = p.fields_k1;
+}
+
 template <int B, typename T>
 __global__ __launch_bounds__(B / 16, (k1_min_blocks<B, T>())) void k1_part_a(K1ALaunch a) {
+    if (a.plan) {
+        if ((int)blockIdx.y >= a.plan->fields_k1) return;
+        plan_chunk_a(a);
+    }
     k1_part_a_block<B, T>(a, blockIdx.y, blockIdx.x);
 }
 
@@ -217,6 +238,7 @@ __global__ __launch_bounds__(B / 16, (k1_min_blocks<B, T>())) void k1_part_a(K1A
 // launch's three workgroups per CU spills 43 VGPRs of the FP64 loop.)
 template <int B, typename T>
 __global__ __launch_bounds__(B / 16) void k1_part_a_list(K1ALaunch a) {
+    if (a.plan) plan_chunk_a(a);
     const int n = *a.bcount;
     for (int i = blockIdx.x; i < n; i += gridDim.x) {
         const int2 fb = a.blist[i];
@@ -329,6 +351,10 @@ __device__ __forceinline__ void k1_part_b_block(const K1BLaunch& a, int f, int b
 
 template <int B, typename T>
 __global__ __launch_bounds__(B / 16, (k1_min_blocks<B, T>())) void k1_part_b(K1BLaunch a) {
+    if (a.plan) {
+        if ((int)blockIdx.y >= a.plan->fields_k1) return;
+        a.total = a.plan->total;
+    }
     k1_part_b_block<B, T>(a, blockIdx.y, blockIdx.x);
 }
 
@@ -337,6 +363,7 @@ __global__ __launch_bounds__(B / 16, (k1_min_blocks<B, T>())) void k1_part_b(K1B
 // weigh on the registers of the full launch.
 template <int B, typename T>
 __global__ __launch_bounds__(B / 16) void k1_part_b_list(K1BLaunch a) {
+    if (a.plan) a.total = a.plan->total;
     const int n = *a.bcount;
     for (int i = blockIdx.x; i < n; i += gridDim.x) {
         const int2 fb = a.blist[i];
@@ -361,6 +388,11 @@ __global__ __launch_bounds__(B / 16) void k_prebpf(PreLaunch a) {
     extern __shared__ __align__(16) unsigned char smem[];
     constexpr int NT = B / 16;
     const int f = blockIdx.y;
+    if (a.plan) {
+        if (f >= a.plan->max_fields) return;
+        a.cap = a.plan->chunk;
+        a.n_valid = a.plan->n_read;
+    }
     const int ns0 = 2 * (int)blockIdx.x * a.V;
     if (ns0 >= a.spfp) return;
     const int ns1 = ns0 + a.V;
@@ -392,6 +424,26 @@ __global__ void k_fill_zero(unsigned int* p, long long n) {
     long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     long long st = (long long)gridDim.x * blockDim.x;
     for (; i < n; i += st) p[i] = 0u;
+}
+
+// K1 outputs of [plan total, plan clear_end) to zero in three buffers of
+// w0 / w1 / w2 32-bit words per sample (what an earlier, longer chunk left
+// there); nothing to do for most chunks.
+__global__ __launch_bounds__(256) void k_clear_past(unsigned int* p0, int w0, unsigned int* p1, int w1,
+                                                    unsigned int* p2, int w2, const ChunkPlan* plan) {
+    const long long t0 = plan->total, t1 = plan->clear_end;
+    if (t1 <= t0) return;
+    const long long st = (long long)gridDim.x * blockDim.x;
+    const long long i0 = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    for (long long i = t0 * w0 + i0; i < t1 * w0; i += st) p0[i] = 0u;
+    for (long long i = t0 * w1 + i0; i < t1 * w1; i += st) p1[i] = 0u;
+    for (long long i = t0 * w2 + i0; i < t1 * w2; i += st) p2[i] = 0u;
+}
+
+// The chunk's plan to the device (a kernel argument carries it: cheaper on
+// the head stream than a copy from pinned memory).
+__global__ void k_put_plan(ChunkPlan* dst, ChunkPlan v) {
+    if (threadIdx.x == 0) *dst = v;
 }
 
 __global__ void k_fill_f64(double* p, long long n, double v) {
@@ -485,10 +537,11 @@ hipError_t run_prebpf(int B, int f32, const PreLaunch& a, int nf, hipStream_t st
     if (!f32 && B == 16384) return launch_pre<16384, double>(a, nf, st);
     return hipErrorInvalidValue;
 }
-hipError_t run_input_mean(const void* cap, int in_fmt, long long n, double* partial, double* mean, hipStream_t st) {
+hipError_t run_input_mean(const void* cap, int in_fmt, long long n, double* partial, double* mean, hipStream_t st,
+                          const ChunkPlan* plan) {
     if (n <= 0) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_input_sum, dim3(kMeanBlocks), dim3(256), 0, st, cap, in_fmt, n, partial);
-    hipLaunchKernelGGL(k_input_mean, dim3(1), dim3(kMeanBlocks), 0, st, partial, n, mean);
+    hipLaunchKernelGGL(k_input_sum, dim3(kMeanBlocks), dim3(256), 0, st, cap, in_fmt, n, partial, plan);
+    hipLaunchKernelGGL(k_input_mean, dim3(1), dim3(kMeanBlocks), 0, st, partial, n, mean, plan);
     return hipGetLastError();
 }
 __global__ void k_to_f64(const float* src, double* dst, long long n) {
@@ -507,6 +560,18 @@ hipError_t run_to_f64(const void* src, int f32, double* dst, long long n, hipStr
 hipError_t run_fill_zero(void* p, long long words, hipStream_t st) {
     if (words <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_fill_zero, dim3(1024), dim3(256), 0, st, static_cast<unsigned int*>(p), words);
+    return hipGetLastError();
+}
+
+hipError_t run_clear_past(void* p0, int w0, void* p1, int w1, void* p2, int w2, const ChunkPlan* plan,
+                          hipStream_t st) {
+    hipLaunchKernelGGL(k_clear_past, dim3(512), dim3(256), 0, st, static_cast<unsigned int*>(p0), w0,
+                       static_cast<unsigned int*>(p1), w1, static_cast<unsigned int*>(p2), w2, plan);
+    return hipGetLastError();
+}
+
+hipError_t run_put_plan(ChunkPlan* dst, const ChunkPlan& v, hipStream_t st) {
+    hipLaunchKernelGGL(k_put_plan, dim3(1), dim3(64), 0, st, dst, v);
     return hipGetLastError();
 }
 

@@ -99,6 +99,15 @@ __global__ __launch_bounds__(256, 2) __attribute__((amdgpu_waves_per_eu(2, 2))) 
     __shared__ float s_edge[kMWaves];
     const int f = blockIdx.y, b = blockIdx.x;
     const int t = threadIdx.x;
+    if (a.plan) {   // a replayed graph: the chunk's values from its plan
+        const ChunkPlan& p = *a.plan;
+        if (f >= p.fields_k1) return;
+        a.cap = p.chunk;
+        a.n_valid = p.n_read;
+        a.total = p.total;
+        a.ctl_raw_offset = (long long)p.raw_offset;
+        a.ctl_fields_k1 = p.fields_k1;
+    }
     if (a.ctl && f == 0 && b == 0 && t == 0) {
         ChunkCtl z{};
         z.fields_k1 = a.ctl_fields_k1;

@@ -150,8 +150,9 @@ __global__ __launch_bounds__(256) void k_tbc_resample(const S* __restrict__ demo
                                                       uint16_t* __restrict__ luma, const ChunkCtl* ctl,
                                                       int nf_override, int total_per_field, double ire0,
                                                       double hz_ire, double vsync_ire, double out_zero,
-                                                      double out_scale) {
+                                                      double out_scale, const ChunkPlan* plan) {
     const int nf = nf_override > 0 ? nf_override : ctl->nf;
+    if (plan) luma = plan->luma;   // a replayed graph: the chunk's place in the outputs
     const int ppf = kFieldLines * kLineLen;
     int idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= nf * ppf) return;
@@ -202,17 +203,17 @@ hipError_t run_hsync_refine(const void* d05, int f32, double* ll, const ChunkCtl
 }
 
 hipError_t run_tbc_resample(const void* demod, int f32, const double* ll, uint16_t* luma, const ChunkCtl* ctl,
-                            int nf_override, int max_fields, const Fmt& fmt, hipStream_t st) {
+                            int nf_override, int max_fields, const Fmt& fmt, hipStream_t st, const ChunkPlan* plan) {
     long long n = (long long)max_fields * kFieldLines * kLineLen;
     const dim3 grid((unsigned)((n + 255) / 256));
     if (f32)
         hipLaunchKernelGGL(k_tbc_resample<float>, grid, dim3(256), 0, st, static_cast<const float*>(demod), ll, luma,
                            ctl, nf_override, fmt.spfp, fmt.ire0, fmt.hz_ire, fmt.vsync_ire, fmt.output_zero,
-                           fmt.output_scale);
+                           fmt.output_scale, plan);
     else
         hipLaunchKernelGGL(k_tbc_resample<double>, grid, dim3(256), 0, st, static_cast<const double*>(demod), ll,
                            luma, ctl, nf_override, fmt.spfp, fmt.ire0, fmt.hz_ire, fmt.vsync_ire, fmt.output_zero,
-                           fmt.output_scale);
+                           fmt.output_scale, plan);
     return hipGetLastError();
 }
 

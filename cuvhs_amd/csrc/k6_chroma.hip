@@ -176,7 +176,7 @@ __global__ __launch_bounds__(256) void k_chroma_out(ChromaLaunch a) {
     const double s_acc = (rms > 1.0) ? (a.burst_abs_ref / rms) : 0.0;
     const double acc = ((line & 1) && a.track[f]) ? -s_acc : s_acc;
     const float* fa = a.filt + row * kLineLen;
-    uint16_t* o = a.out + row * kLineLen;
+    uint16_t* o = (a.plan ? a.plan->chroma : a.out) + row * kLineLen;
     for (int col = threadIdx.x; col < kLineLen; col += 256) {
         double ov = (double)fa[col] * acc + 32768.0;
         if (ov < 0.0) ov = 0.0;
@@ -245,8 +245,10 @@ constexpr int kResolveLds = 1024;
 __device__ void resolve_tracks(DecodeState* dst, int nf, const double* m0, const double* m1, const double* iq0,
                                int* out_track, int* phase_ids);
 __global__ void k_chroma_resolve(const ChunkCtl* ctl, DecodeState* dst, int nf_override, const double* g_m0,
-                                 const double* g_m1, const double* iq0, int* g_track, int* phase_ids) {
+                                 const double* g_m1, const double* iq0, int* g_track, int* phase_ids,
+                                 const ChunkPlan* plan) {
     __shared__ double s_m0[kResolveLds], s_m1[kResolveLds];
+    if (plan) phase_ids = plan->phase_id;   // a replayed graph: the chunk's place in the outputs
     __shared__ int s_track[kResolveLds];
     if (blockIdx.x != 0) return;
     const int nf = nf_override > 0 ? nf_override : ctl->nf;
@@ -354,7 +356,7 @@ hipError_t run_chroma(ChromaLaunch a, DecodeState* dst, int max_fields, double* 
     hipLaunchKernelGGL(k_chroma_metrics, dim3(max_fields), dim3(64), 0, st, a.burst, a.ctl,
                        a.nf_override, m0, m1, cs_tab, need_iq ? iq0 : nullptr);
     hipLaunchKernelGGL(k_chroma_resolve, dim3(1), dim3(64), 0, st, a.ctl, dst, a.nf_override, m0, m1, iq0, out_track,
-                       phase_ids);
+                       phase_ids, a.plan);
     ChromaLaunch o = a;
     o.track = out_track;
     hipLaunchKernelGGL(k_chroma_out, dim3(kFieldLines, max_fields), dim3(256), 0, st, o);

@@ -132,8 +132,17 @@ __global__ __launch_bounds__(kDoThreads) void k_dropouts(const unsigned char* __
                                                           const double* __restrict__ linelocs,
                                                           const ChunkCtl* ctl, int nf_override, int spfp, int als,
                                                           uint16_t* luma, uint16_t* chroma, int* do_lines,
-                                                          int* do_starts, int* do_ends, int* do_count) {
+                                                          int* do_starts, int* do_ends, int* do_count,
+                                                          const ChunkPlan* plan) {
     __shared__ unsigned char cls[kMaxClassBlocks];
+    if (plan) {   // a replayed graph: the chunk's place in the output arrays
+        luma = plan->luma;
+        chroma = plan->chroma;
+        do_lines = plan->do_line;
+        do_starts = plan->do_startx;
+        do_ends = plan->do_endx;
+        do_count = plan->do_count;
+    }
     __shared__ int ev[kMaxEvents];
     __shared__ int s_last[kDoThreads];
     __shared__ int s_cnt[kDoThreads];
@@ -286,8 +295,13 @@ __global__ __launch_bounds__(kDoThreads) void k_dropouts(const unsigned char* __
 // Output metadata: fileLoc and the alternating field parity
 // (pipeline.cu:2577-2593), phase IDs passed through.
 __global__ void k_field_meta(const ChunkCtl* ctl, DecodeState* dst, const unsigned long long* offsets,
-                             const int* is_first_k2b, unsigned long long* file_loc, int* is_first_out) {
+                             const int* is_first_k2b, unsigned long long* file_loc, int* is_first_out,
+                             const ChunkPlan* plan) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (plan) {
+        file_loc = plan->file_loc;
+        is_first_out = plan->is_first;
+    }
     const int nf = ctl->nf;
     int last = dst->last_parity;
     for (int i = 0; i < nf; i++) {
@@ -327,9 +341,9 @@ hipError_t run_dropout_classes(const void* env, int f32, const double* ll, const
 
 hipError_t run_dropout_scan(const unsigned char* cls, const double* ll, const ChunkCtl* ctl, int nf_override,
                             int max_fields, const Fmt& fmt, uint16_t* luma, uint16_t* chroma, int* lines, int* starts,
-                            int* ends, int* count, hipStream_t st) {
+                            int* ends, int* count, hipStream_t st, const ChunkPlan* plan) {
     hipLaunchKernelGGL(k_dropouts, dim3(max_fields), dim3(kDoThreads), 0, st, cls, ll, ctl, nf_override, fmt.spfp,
-                       fmt.active_line_start, luma, chroma, lines, starts, ends, count);
+                       fmt.active_line_start, luma, chroma, lines, starts, ends, count, plan);
     return hipGetLastError();
 }
 
@@ -342,9 +356,10 @@ hipError_t run_dropouts(const double* env, const double* ll, const ChunkCtl* ctl
 }
 
 hipError_t run_field_meta(const ChunkCtl* ctl, DecodeState* dst, const unsigned long long* offsets,
-                          const int* is_first_k2b, unsigned long long* file_loc, int* is_first_out, hipStream_t st) {
+                          const int* is_first_k2b, unsigned long long* file_loc, int* is_first_out, hipStream_t st,
+                          const ChunkPlan* plan) {
     hipLaunchKernelGGL(k_field_meta, dim3(1), dim3(64), 0, st, ctl, dst, offsets, is_first_k2b, file_loc,
-                       is_first_out);
+                       is_first_out, plan);
     return hipGetLastError();
 }
 

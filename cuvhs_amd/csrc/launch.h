@@ -13,6 +13,30 @@ namespace cuvhs {
 struct K2bField;
 struct K2bAnchorOut;
 
+// Per-chunk values of the stage graphs of cuvhs_decode_capture, which are
+// captured once per buffer slot and replayed for every chunk that slot
+// takes: the host fills them in, a kernel writes them to device memory ahead
+// of the chunk's head stage, and each kernel given a plan reads them in
+// place of the values it was captured with (grids are then sized for the
+// context's batch, and workgroups past the chunk's fields return).  Kernels
+// launched directly get no plan.
+struct ChunkPlan {
+    const void* chunk;              // the chunk's first sample in the capture
+    long long n_read;               // samples read from it (zero past them)
+    long long total;                // fields_k1 * spfp: K1's outputs
+    long long clear_end;            // K1 outputs of [total, clear_end) are cleared (a longer chunk's)
+    unsigned long long raw_offset;  // the chunk's first sample's index in the capture
+    int fields_k1;                  // fields K1 demodulates
+    int max_fields;                 // fields the chunk may hold
+    int vsync_cap;                  // VSYNC candidates kept
+    // the chunk's first field in the output arrays (the caller's, or the
+    // context's when the caller passed none)
+    uint16_t* luma;
+    uint16_t* chroma;
+    unsigned long long* file_loc;
+    int *is_first, *phase_id, *do_count, *do_line, *do_startx, *do_endx;
+};
+
 struct K1ALaunch {
     const void* cap;            // capture samples (format in_fmt)
     int in_fmt;                 // 0 u8, 1 s16, 2 u16
@@ -31,6 +55,7 @@ struct K1ALaunch {
     unsigned int* thr_bits;     // if set, zeroed over [0, total) for part B
     const int2* blist;          // if set: only the blocks (segment, block) blist[i], i < *bcount
     const int* bcount;
+    const ChunkPlan* plan;      // if set: cap, n_valid, total, ctl_* from the plan
 };
 // K1 part A on the matrix cores (mixed-precision forms, u8 input): see
 // k1_mfma.hip.  Outputs are FP32.
@@ -51,6 +76,7 @@ struct K1MLaunch {
     unsigned int* thr_bits;
     float2* blk_env;            // if set: (min, max) envelope of each tile of 1024
     int nblk;                   //   outputs, [field][nblk blocks][16 tiles]
+    const ChunkPlan* plan;      // as in K1ALaunch
 };
 struct K1BLaunch {
     const void* rawdemod;       // double, or float with in_f32
@@ -71,6 +97,7 @@ struct K1BLaunch {
     double gdelta;              // set to 1 when output s lies within gdelta of
                                 // thr (never cleared here: the guard's
                                 // compaction clears the flags it read)
+    const ChunkPlan* plan;      // if set: total (and the fields) from the plan
 };
 // The guard band's block selection (guard.hip).
 // Samples per guard flag of the reduced-precision part B: 256.
@@ -91,6 +118,7 @@ struct GuardLaunch {
     int2* listB;
     int* counts;                // [0] part A, [1] part B
     ChunkCtl* ctl;              // if set: receives the counts (guard_a, guard_b)
+    const ChunkPlan* plan;      // if set: nsub, nseg, total from the plan
 };
 struct PreLaunch {
     const void* cap;
@@ -101,6 +129,7 @@ struct PreLaunch {
     const void* G;      // block spectrum: double2 (FP64 form) or float2 (FP32 form)
     const void* tw;     // twiddles of the same type
     float* out;
+    const ChunkPlan* plan;  // if set: cap, n_valid and the field count from the plan
 };
 struct ChromaLaunch {
     const float* craw;
@@ -120,6 +149,7 @@ struct ChromaLaunch {
     const float2* tw_f;         // the same twiddles in FP32 (f32)
     const double2* mask;        // band-pass bins: (1, 0) kept, (0, 0) removed
     int f32;                    // line filter in FP32 (the reduced-precision forms)
+    const ChunkPlan* plan;      // if set: out (and the phase IDs) from the plan
 };
 
 
@@ -154,7 +184,14 @@ hipError_t run_diag_k4(const double* d05, const double* before, const double* af
                        const Fmt& fmt, hipStream_t st);
 hipError_t run_diag_k5(const double* demod, const double* ll, ChunkCtl* ctl, int max_fields, const Fmt& fmt,
                        hipStream_t st);
-hipError_t run_input_mean(const void* cap, int in_fmt, long long n, double* partial, double* mean, hipStream_t st);
+hipError_t run_input_mean(const void* cap, int in_fmt, long long n, double* partial, double* mean, hipStream_t st,
+                          const ChunkPlan* plan = nullptr);
+// [plan total, plan clear_end) of three K1 output buffers with w0 / w1 / w2
+// 32-bit words per sample to zero
+hipError_t run_clear_past(void* p0, int w0, void* p1, int w1, void* p2, int w2, const ChunkPlan* plan,
+                          hipStream_t st);
+// a chunk's plan into device memory, by value
+hipError_t run_put_plan(ChunkPlan* dst, const ChunkPlan& v, hipStream_t st);
 // one convolution block of the kinds the decoder runs (gp in the block's bin order)
 hipError_t run_block_selftest(int B, int is_f32, const void* x, const void* gp, void* y, const void* tw,
                               hipStream_t st);
@@ -164,14 +201,14 @@ size_t events_mask_count(long long total);  // uint2 entries of the masks buffer
 size_t thr_bits_words(long long total);     // 32-bit words of a threshold bitmask
 // events from the threshold bitmask (bit s = x[s] <= thr), as K1 part B writes it
 hipError_t run_events_bits(const unsigned int* bits, long long total, int* wg_count, int* wg_off, uint2* masks,
-                           int* events, ChunkCtl* ctl, hipStream_t st);
+                           int* events, ChunkCtl* ctl, hipStream_t st, const ChunkPlan* plan = nullptr);
 // the same from a demod_05 array: bitmask into bits, then run_events_bits
 hipError_t run_events(const double* x, long long total, double thr, unsigned int* bits, int* wg_count, int* wg_off,
                       uint2* masks, int* events, ChunkCtl* ctl, hipStream_t st);
 // host_out (optional): pinned host copy of the finished control block
 hipError_t run_vsync_cluster(const int* events, ChunkCtl* ctl, long long total, int min_w, int max_w, int cap,
                              int* cand, unsigned long long* offsets, int num_fields, int spf, int spl, int fields_k1,
-                             int do_cluster, ChunkCtl* host_out, hipStream_t st);
+                             int do_cluster, ChunkCtl* host_out, hipStream_t st, const ChunkPlan* plan = nullptr);
 hipError_t run_sync_pulses(const int* events, const ChunkCtl* ctl, const unsigned long long* offsets, int spfp,
                            int* starts, int* lens, int* count, int max_fields, int nf_override, hipStream_t st);
 hipError_t run_k2b(const ChunkCtl* ctl_dev, ChunkCtl* ctl_mut, DecodeState* dst, int nf_override, int max_fields, const Fmt& fmt,
@@ -195,7 +232,8 @@ size_t k2b_anchor_bytes();
 hipError_t run_hsync_refine(const void* d05, int f32, double* ll, const ChunkCtl* ctl, int nf_override,
                             int max_fields, const Fmt& fmt, hipStream_t st);
 hipError_t run_tbc_resample(const void* demod, int f32, const double* ll, uint16_t* luma, const ChunkCtl* ctl,
-                            int nf_override, int max_fields, const Fmt& fmt, hipStream_t st);
+                            int nf_override, int max_fields, const Fmt& fmt, hipStream_t st,
+                            const ChunkPlan* plan = nullptr);
 hipError_t run_chroma(ChromaLaunch a, DecodeState* dst, int max_fields, double* m0, double* m1,
                       const double* cs_tab, double* iq0, int* out_track, int* phase_ids, int need_iq,
                       hipStream_t st);
@@ -211,9 +249,10 @@ hipError_t run_dropout_classes(const void* env, int f32, const double* ll, const
                                int max_fields, const Fmt& fmt, double* partial, unsigned char* cls, hipStream_t st);
 hipError_t run_dropout_scan(const unsigned char* cls, const double* ll, const ChunkCtl* ctl, int nf_override,
                             int max_fields, const Fmt& fmt, uint16_t* luma, uint16_t* chroma, int* lines, int* starts,
-                            int* ends, int* count, hipStream_t st);
+                            int* ends, int* count, hipStream_t st, const ChunkPlan* plan = nullptr);
 hipError_t run_field_meta(const ChunkCtl* ctl, DecodeState* dst, const unsigned long long* offsets,
-                          const int* is_first_k2b, unsigned long long* file_loc, int* is_first_out, hipStream_t st);
+                          const int* is_first_k2b, unsigned long long* file_loc, int* is_first_out, hipStream_t st,
+                          const ChunkPlan* plan = nullptr);
 
 // launch parameter blocks (layouts shared with the kernel files)
 }  // namespace cuvhs

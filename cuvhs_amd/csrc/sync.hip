@@ -46,8 +46,9 @@ __global__ __launch_bounds__(256) void k_thr_bits(const double* x, long long tot
 // previous set; none at or past total.  The (start, end) masks are kept for
 // k_ev_write, which so never reads the bitmask again.
 __global__ __launch_bounds__(kEvThreads) void k_ev_count(const unsigned int* bits, long long total, int* wg_count,
-                                                         uint2* masks) {
+                                                         uint2* masks, const ChunkPlan* plan) {
     __shared__ int s_total;
+    if (plan) total = plan->total;   // a replayed graph: sized for the batch
     if (threadIdx.x == 0) s_total = 0;
     __syncthreads();
     const long long j = (long long)blockIdx.x * kEvThreads + threadIdx.x;
@@ -155,8 +156,14 @@ __global__ __launch_bounds__(1024) void k_vsync_cluster(const int* events, Chunk
                                                         int min_w, int max_w, int cap, int* cand,
                                                         unsigned long long* offsets, int num_fields, int spf,
                                                         int spl, int fields_k1, int do_cluster,
-                                                        ChunkCtl* host_out) {
+                                                        ChunkCtl* host_out, const ChunkPlan* plan) {
     constexpr int kWaves = 1024 / 64;
+    if (plan) {   // a replayed graph: the chunk's values from its plan
+        total = plan->total;
+        cap = plan->vsync_cap;
+        num_fields = plan->max_fields;
+        fields_k1 = plan->fields_k1;
+    }
     __shared__ int wcnt[2][kWaves];
     __shared__ int lcand[kCandLds];
     __shared__ int lnext[kCandLds];
@@ -341,11 +348,13 @@ size_t events_mask_count(long long total) { return (size_t)events_nwg(total) * k
 
 size_t thr_bits_words(long long total) { return (size_t)((total + 63) / 64) * 2; }
 
+// plan: total is the batch's (the grids' size); the kernels count the
+// chunk's samples from the plan
 hipError_t run_events_bits(const unsigned int* bits, long long total, int* wg_count, int* wg_off, uint2* masks,
-                           int* events, ChunkCtl* ctl, hipStream_t st) {
+                           int* events, ChunkCtl* ctl, hipStream_t st, const ChunkPlan* plan) {
     int nwg = events_nwg(total);
     if (nwg == 0) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_ev_count, dim3(nwg), dim3(kEvThreads), 0, st, bits, total, wg_count, masks);
+    hipLaunchKernelGGL(k_ev_count, dim3(nwg), dim3(kEvThreads), 0, st, bits, total, wg_count, masks, plan);
     hipLaunchKernelGGL(k_ev_scan, dim3(1), dim3(1024), 0, st, wg_count, nwg, wg_off, ctl);
     hipLaunchKernelGGL(k_ev_write, dim3(nwg), dim3(kEvThreads), 0, st, masks, wg_off, events);
     return hipGetLastError();
@@ -362,9 +371,9 @@ hipError_t run_events(const double* x, long long total, double thr, unsigned int
 
 hipError_t run_vsync_cluster(const int* events, ChunkCtl* ctl, long long total, int min_w, int max_w, int cap,
                              int* cand, unsigned long long* offsets, int num_fields, int spf, int spl, int fields_k1,
-                             int do_cluster, ChunkCtl* host_out, hipStream_t st) {
+                             int do_cluster, ChunkCtl* host_out, hipStream_t st, const ChunkPlan* plan) {
     hipLaunchKernelGGL(k_vsync_cluster, dim3(1), dim3(1024), 0, st, events, ctl, total, min_w, max_w, cap, cand,
-                       offsets, num_fields, spf, spl, fields_k1, do_cluster, host_out);
+                       offsets, num_fields, spf, spl, fields_k1, do_cluster, host_out, plan);
     return hipGetLastError();
 }
 

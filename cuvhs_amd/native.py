@@ -37,7 +37,7 @@ class CuvhsError(RuntimeError):
 class Config(ctypes.Structure):
     _fields_ = [("system", _i), ("sample_rate_mhz", _d), ("batch_fields", _i), ("precision", _i),
                 ("stream", _vp), ("device", _i), ("input_format", _i), ("dc_correct", _i), ("line_locator", _i),
-                ("block_k1a", _i), ("block_k1b", _i), ("block_pre", _i)]
+                ("block_k1a", _i), ("block_k1b", _i), ("block_pre", _i), ("graphs", _i)]
 
 
 class Format(ctypes.Structure):
@@ -216,15 +216,16 @@ class Decoder:
     """One decode context (Pipeline, pipeline.h:28-114) on a HIP device."""
 
     def __init__(self, sample_rate_mhz=28.0, batch_fields=64, device=0, input_format="u8", dc_correct=False,
-                 line_locator="k2b", precision="fp64", blocks=None):
+                 line_locator="k2b", precision="fp64", blocks=None, graphs=False):
         """blocks: optional (k1a, k1b, pre) block sizes, 0 for the tuned
-        or modelled choice (cuvhs_config.block_*)."""
+        or modelled choice (cuvhs_config.block_*).  graphs: decode_capture
+        replays captured stage graphs (cuvhs_config.graphs)."""
         self.input_format = INPUT_FORMATS[input_format]
         self.precision = precision
         ba, bb, bp = blocks or (0, 0, 0)
         cfg = Config(0, float(sample_rate_mhz), int(batch_fields), PRECISIONS[precision], None, int(device),
                      self.input_format, 1 if dc_correct else 0, LINE_LOCATORS[line_locator], int(ba), int(bb),
-                     int(bp))
+                     int(bp), 1 if graphs else 0)
         h = _vp()
         rc = lib().cuvhs_create(ctypes.byref(cfg), ctypes.byref(h))
         if rc != 0:

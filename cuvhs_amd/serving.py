@@ -56,11 +56,13 @@ def make_requests(n, rate_hz, min_fields, max_fields, pool_fields, seed=4):
     return reqs
 
 
-def run(d_pool, reqs, precision="fp64", workers=4, batch=64, hbm_budget_gb=64.0, device=0, keep_outputs=False):
+def run(d_pool, reqs, precision="fp64", workers=4, batch=64, hbm_budget_gb=64.0, device=0, keep_outputs=False,
+        graphs=False):
     """Serve reqs (arrival times in seconds from now) against the device pool
     {mhz: uint8 cuda tensor}; returns the per-request records (with the
     decode's output tensors under "out" when keep_outputs)."""
-    ctx = [{m: N.Decoder(m, batch, device=device, precision=precision) for m in RATES} for _ in range(workers)]
+    ctx = [{m: N.Decoder(m, batch, device=device, precision=precision, graphs=graphs) for m in RATES}
+           for _ in range(workers)]
     budget = hbm_budget_gb * 1e9
     lock = threading.Condition()
     in_use = [0.0]
@@ -73,20 +75,24 @@ def run(d_pool, reqs, precision="fp64", workers=4, batch=64, hbm_budget_gb=64.0,
             r = q.get()
             if r is None:
                 return
-            need = output_bytes(r["fields"] + 2)
+            dec = ctx[w][r["mhz"]]
+            spf = dec.fmt.samples_per_field
+            # output room: the decoder's bound for the capture's length (a
+            # capture yields more fields than field periods when the tape runs
+            # fast, e.g. 2693 from 2689 at 40 MHz with head-switch offsets)
+            cap_fields = dec.default_capacity(r["fields"] * spf)
+            need = output_bytes(cap_fields)
             with lock:
                 while in_use[0] + need > budget:
                     lock.wait()
                 in_use[0] += need
             try:
-                dec = ctx[w][r["mhz"]]
-                spf = dec.fmt.samples_per_field
                 a = r["start_field"] * spf
                 cap = d_pool[r["mhz"]][a: a + r["fields"] * spf]
                 r["start"] = time.perf_counter() - t0
                 dec.reset()
                 # returns once the decoder's own streams are idle
-                out = dec.decode_capture(cap, max_fields=r["fields"] + 2)
+                out = dec.decode_capture(cap, max_fields=cap_fields)
                 r["end"] = time.perf_counter() - t0
                 r["decoded"] = out["fields"]
                 if keep_outputs:

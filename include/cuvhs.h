@@ -102,6 +102,15 @@ typedef struct cuvhs_config {
      * a rate or form it did not measure, the library's cost model.  A size
      * the filter's support does not fit is refused (CUVHS_E_ARG). */
     int block_k1a, block_k1b, block_pre;
+    /* 1: cuvhs_decode_capture replays HIP graphs of its stages, captured by
+     * cuvhs_create once per buffer slot, with each chunk's values read on the
+     * device from a per-slot plan (no counterpart in the reference's host
+     * loop, pipeline.cu:2626-2713); 0: it launches the kernels one by one.
+     * Results are identical.  Profiling and diagnostics always launch one by
+     * one.  cuvhs_create captures one context at a time in the process; a
+     * capture must not overlap another thread's HIP calls, so contexts with
+     * graphs are created before decoding starts in other threads. */
+    int graphs;
 } cuvhs_config;
 
 /* Derived timing constants (VideoFormat, video_format.cpp:23-104). */

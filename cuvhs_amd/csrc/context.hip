@@ -155,6 +155,12 @@ struct cuvhs_ctx {
     int chroma_f32 = 0;             // chroma line filter in FP32 (the reduced-precision forms)
     int block_source = 2;           // block sizes from: 0 the config, 1 the tuned table, 2 the cost model
     int graphs = 0;                 // cuvhs_decode_capture replays captured stage graphs (cuvhs_config.graphs)
+    // the dropout sums and classes on the tail stream (after chroma) instead
+    // of forked with K5: in the mixed forms, whose head stream, where the
+    // fork lands with four hardware queues, also carries the guard band
+    // (measured on the 600-field decode: mixed-fp16 70.1k / 69.9k against
+    // 68.2k / 68.4k fields/s forked; FP32 72.1k against 73.0k / 72.3k)
+    bool classes_on_tail = false;
     double* cs_tab = nullptr;       // cos/sin of the burst phase detector
     double2* chroma_mask = nullptr; // per-line band-pass: (1, 0) kept bins, (0, 0) others
     float* chroma_filt = nullptr;   // band-passed chroma lines of a chunk (track 0)
@@ -662,8 +668,9 @@ int tail_stage(cuvhs_ctx* c, Slot& s, const OutArrays* out, hipStream_t fork_str
     if (c->diag) HIPCHK(c, run_diag_k5(s.demod, c->ll, s.ctl, s.max_fields, f, st));
     mark(c, s, 10, st);
     if (fork) {
-        HIPCHK(c, run_dropout_classes(s.env, c->sig_f32, c->ll, s.ctl, 0, s.max_fields, f, c->do_partial, c->do_cls,
-                                      ax));
+        if (!c->classes_on_tail)
+            HIPCHK(c, run_dropout_classes(s.env, c->sig_f32, c->ll, s.ctl, 0, s.max_fields, f, c->do_partial,
+                                          c->do_cls, ax));
         HIPCHK(c, hipEventRecord(s.aux_done, ax));
     }
     HIPCHK(c, hipStreamWaitEvent(st, s.pre1, 0));
@@ -672,12 +679,10 @@ int tail_stage(cuvhs_ctx* c, Slot& s, const OutArrays* out, hipStream_t fork_str
     cl.out = chroma;
     HIPCHK(c, run_chroma(cl, c->state, s.max_fields, c->m0, c->m1, c->cs_tab, c->iq0, c->track, phase, 1, st));
     mark(c, s, 12, st);
-    if (fork) {
-        HIPCHK(c, hipStreamWaitEvent(st, s.aux_done, 0));
-    } else {
+    if (!fork || c->classes_on_tail)
         HIPCHK(c, run_dropout_classes(s.env, c->sig_f32, c->ll, s.ctl, 0, s.max_fields, f, c->do_partial, c->do_cls,
                                       st));
-    }
+    if (fork) HIPCHK(c, hipStreamWaitEvent(st, s.aux_done, 0));
     HIPCHK(c, run_dropout_scan(c->do_cls, c->ll, s.ctl, 0, s.max_fields, f, luma, chroma, dol, dos, doe, doc, st));
     mark(c, s, 13, st);
     HIPCHK(c, run_field_meta(s.ctl, c->state, s.offsets, c->isf_k2b, floc, isf, st));
@@ -813,7 +818,9 @@ int seq_tail1(cuvhs_ctx* c, Slot& s, hipStream_t st) {
 }
 int seq_fork(cuvhs_ctx* c, Slot& s, hipStream_t st) {
     HIPCHK(c, run_tbc_resample(s.demod, c->sig_f32, c->ll, nullptr, s.ctl, 0, c->F, c->fmt, st, s.plan));
-    HIPCHK(c, run_dropout_classes(s.env, c->sig_f32, c->ll, s.ctl, 0, c->F, c->fmt, c->do_partial, c->do_cls, st));
+    if (!c->classes_on_tail)
+        HIPCHK(c, run_dropout_classes(s.env, c->sig_f32, c->ll, s.ctl, 0, c->F, c->fmt, c->do_partial, c->do_cls,
+                                      st));
     return 0;
 }
 int seq_tail2(cuvhs_ctx* c, Slot& s, hipStream_t st) {
@@ -823,6 +830,9 @@ int seq_tail2(cuvhs_ctx* c, Slot& s, hipStream_t st) {
     return 0;
 }
 int seq_tail3(cuvhs_ctx* c, Slot& s, hipStream_t st) {
+    if (c->classes_on_tail)
+        HIPCHK(c, run_dropout_classes(s.env, c->sig_f32, c->ll, s.ctl, 0, c->F, c->fmt, c->do_partial, c->do_cls,
+                                      st));
     HIPCHK(c, run_dropout_scan(c->do_cls, c->ll, s.ctl, 0, c->F, c->fmt, c->luma, c->chroma, c->dol, c->dos, c->doe,
                                c->doc, st, s.plan));
     HIPCHK(c, run_field_meta(s.ctl, c->state, s.offsets, c->isf_k2b, c->file_loc, c->isf_out, st, s.plan));
@@ -932,9 +942,10 @@ int head_stage_graph(cuvhs_ctx* c, Slot& s) {
 }
 
 // Tail stage in graph mode, with the launch-by-launch schedule's waits:
-// line locations and K4; K5 and the dropout classes on the fork stream;
-// chroma once the pre-band-pass is done; the dropout scan (joining the fork),
-// metadata and the control block's copy to the host.
+// line locations and K4; K5 (with the dropout classes outside the mixed
+// forms) on the fork stream; chroma once the pre-band-pass is done; the
+// dropout scan (joining the fork), metadata and the control block's copy to
+// the host.
 int tail_stage_graph(cuvhs_ctx* c, Slot& s) {
     hipStream_t st = c->tail, fork = fork_stream(c);
     HIPCHK(c, hipStreamWaitEvent(st, s.head_done, 0));
@@ -1113,6 +1124,7 @@ int cuvhs_create(const cuvhs_config* cfg, cuvhs_ctx** out) {
         const int bk1[2] = {4096, 8192};
         c->mixed = cfg->precision >= CUVHS_PRECISION_MIXED_F16;
         c->mixed_bf16 = cfg->precision == CUVHS_PRECISION_MIXED_BF16;
+        c->classes_on_tail = c->mixed;
         c->k1_f32 = cfg->precision != CUVHS_PRECISION_FP64;
         c->sig_f32 = cfg->precision != CUVHS_PRECISION_FP64;
         c->d05_f32 = cfg->precision == CUVHS_PRECISION_FP32;

@@ -86,20 +86,21 @@ __global__ __launch_bounds__(256) void k_input_mean(const double* partial, long 
 // (the block start is within one segment and a block is shorter than n).
 __device__ __forceinline__ int wrap_n(int q, int n) { return q < 0 ? q + n : (q >= n ? q - n : q); }
 
-// FP64 4096-point blocks exchange real and imaginary parts through LDS
-// separately (35 KB instead of 70 KB per workgroup).
+// 4096-point blocks exchange real and imaginary parts through LDS
+// separately: 35 KB instead of 70 KB per FP64 workgroup, 17 KB instead of 35
+// KB per FP32 one (five FP32 workgroups per CU instead of four; the
+// 600-field decode's FP32 form 73.5k against 72.4k fields/s).
 template <int B, typename T>
-constexpr bool k1_whole() { return !(B == 4096 && sizeof(T) == 8); }
+constexpr bool k1_whole() { return B != 4096; }
 template <int B, typename T>
 constexpr size_t k1_lds_bytes() { return (size_t)lds_elems(B) * (k1_whole<B, T>() ? 2 * sizeof(T) : sizeof(T)); }
 
-// Workgroups per CU the register allocation must allow: four FP32
-// 4096-point blocks fit one CU's LDS (35 KB each), so their kernels are held
-// to 128 VGPRs; FP64 4096-point blocks (35 KB with split exchanges) to 168
-// VGPRs, three per CU (held to 128 VGPRs for four per CU they spill and run
-// slower: part B 3.03 against 2.79 ms per 600 fields).
+// Workgroups per CU the register allocation must allow: five FP32
+// 4096-point blocks (17 KB of LDS each) at 102 VGPRs; FP64 4096-point blocks
+// (35 KB) to 168 VGPRs, three per CU (held to 128 VGPRs for four per CU they
+// spill and run slower: part B 3.03 against 2.79 ms per 600 fields).
 template <int B, typename T>
-constexpr int k1_min_blocks() { return B == 4096 ? (sizeof(T) == 4 ? 4 : 3) : 1; }
+constexpr int k1_min_blocks() { return B == 4096 ? (sizeof(T) == 4 ? 5 : 3) : 1; }
 
 // Phase of z = x + iy in the transform's precision (atan_f64.h for FP64).
 __device__ __forceinline__ double phase_of(double y, double x) { return atan2_f64(y, x); }
@@ -232,19 +233,20 @@ __global__ __launch_bounds__(B / 16, (k1_min_blocks<B, T>())) void k1_part_a(K1A
     k1_part_a_block<B, T>(a, blockIdx.y, blockIdx.x);
 }
 
-// Block-list mode (the guard band's FP64 recompute): every gridDim.x-th
-// listed block.  A kernel of its own, so the loop's hoisted values do not
-// weigh on the registers of the full launch.  (Bounding it to the full
-// launch's three workgroups per CU spills 43 VGPRs of the FP64 loop.)
+// Block-list mode (the guard band's FP64 recompute): workgroup i takes the
+// listed block i; the grid covers the longest list (every block of the
+// chunk) and workgroups past the list's length return.  One block per
+// workgroup keeps the full launch's registers and occupancy (a persistent
+// loop over the list took 216 VGPRs, two workgroups per CU, and two rounds
+// of block latency; measured 0.9% / 1.6% more fields/s mixed-fp16 / bf16
+// this way).
 template <int B, typename T>
-__global__ __launch_bounds__(B / 16) void k1_part_a_list(K1ALaunch a) {
+__global__ __launch_bounds__(B / 16, (k1_min_blocks<B, T>())) void k1_part_a_list(K1ALaunch a) {
     if (a.plan) plan_chunk_a(a);
-    const int n = *a.bcount;
-    for (int i = blockIdx.x; i < n; i += gridDim.x) {
-        const int2 fb = a.blist[i];
-        k1_part_a_block<B, T>(a, fb.x, fb.y);
-        __syncthreads();
-    }
+    const int i = blockIdx.x;
+    if (i >= *a.bcount) return;
+    const int2 fb = a.blist[i];
+    k1_part_a_block<B, T>(a, fb.x, fb.y);
 }
 
 // Part B: video and sync filters in one complex transform
@@ -358,18 +360,15 @@ __global__ __launch_bounds__(B / 16, (k1_min_blocks<B, T>())) void k1_part_b(K1B
     k1_part_b_block<B, T>(a, blockIdx.y, blockIdx.x);
 }
 
-// Block-list mode (the guard band's FP64 recompute): every gridDim.x-th
-// listed block.  A kernel of its own, so the loop's hoisted values do not
-// weigh on the registers of the full launch.
+// Block-list mode of part B: workgroup i takes the listed block i (as
+// k1_part_a_list).
 template <int B, typename T>
-__global__ __launch_bounds__(B / 16) void k1_part_b_list(K1BLaunch a) {
+__global__ __launch_bounds__(B / 16, (k1_min_blocks<B, T>())) void k1_part_b_list(K1BLaunch a) {
     if (a.plan) a.total = a.plan->total;
-    const int n = *a.bcount;
-    for (int i = blockIdx.x; i < n; i += gridDim.x) {
-        const int2 fb = a.blist[i];
-        k1_part_b_block<B, T>(a, fb.x, fb.y);
-        __syncthreads();
-    }
+    const int i = blockIdx.x;
+    if (i >= *a.bcount) return;
+    const int2 fb = a.blist[i];
+    k1_part_b_block<B, T>(a, fb.x, fb.y);
 }
 
 // Chroma pre-band-pass: segment f is raw[f*spfp, (f+1)*spfp) zero-padded to N2
@@ -453,7 +452,6 @@ __global__ void k_fill_f64(double* p, long long n, double v) {
 }
 
 // ---- host launchers ----
-
 // Dynamic LDS above 64 KB needs a per-kernel attribute; set it once per
 // device rather than before every launch.
 template <typename K>
@@ -466,26 +464,13 @@ static void ensure_lds(K* kern, size_t sh) {
     (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
     done.fetch_or(bit);
 }
-// Grid of a block-list launch: the workgroups the device holds at once
-// (each loops over the list), kListGrid if the query fails.
-template <typename K>
-static int list_grid(K kern, int threads, size_t sh) {
-    int dev = 0, ncu = 0, per_cu = 0;
-    if (hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, threads, sh) != hipSuccess || ncu < 1 ||
-        per_cu < 1)
-        return kListGrid;
-    return ncu * per_cu;
-}
 template <int B, typename T>
 static hipError_t launch_a(const K1ALaunch& a, int nf, hipStream_t st) {
     int nb = (int)((a.spfp + a.V - 1) / a.V);
     size_t sh = k1_lds_bytes<B, T>();
     if (a.blist) {
         ensure_lds(k1_part_a_list<B, T>, sh);
-        const int grid = list_grid(k1_part_a_list<B, T>, B / 16, sh);
-        hipLaunchKernelGGL((k1_part_a_list<B, T>), dim3(grid), dim3(B / 16), sh, st, a);
+        hipLaunchKernelGGL((k1_part_a_list<B, T>), dim3(nb * nf), dim3(B / 16), sh, st, a);
     } else {
         ensure_lds(k1_part_a<B, T>, sh);
         hipLaunchKernelGGL((k1_part_a<B, T>), dim3(nb, nf), dim3(B / 16), sh, st, a);
@@ -498,8 +483,7 @@ static hipError_t launch_b(const K1BLaunch& a, int nf, hipStream_t st) {
     size_t sh = k1_lds_bytes<B, T>();
     if (a.blist) {
         ensure_lds(k1_part_b_list<B, T>, sh);
-        const int grid = list_grid(k1_part_b_list<B, T>, B / 16, sh);
-        hipLaunchKernelGGL((k1_part_b_list<B, T>), dim3(grid), dim3(B / 16), sh, st, a);
+        hipLaunchKernelGGL((k1_part_b_list<B, T>), dim3(nb * nf), dim3(B / 16), sh, st, a);
     } else {
         ensure_lds(k1_part_b<B, T>, sh);
         hipLaunchKernelGGL((k1_part_b<B, T>), dim3(nb, nf), dim3(B / 16), sh, st, a);
@@ -581,10 +565,13 @@ hipError_t run_fill_f64(double* p, long long n, double v, hipStream_t st) {
     return hipGetLastError();
 }
 
-// LDS layout of the self-tested blocks as the decoder runs them: FP64 1024-,
-// 4096- and 16384-point blocks exchange real and imaginary parts in turn.
+// LDS layout of the self-tested blocks as the decoder runs them: 4096-point
+// blocks as K1's, FP64 1024- and 16384-point blocks exchange real and
+// imaginary parts in turn.
 template <int B, typename T>
-constexpr bool selftest_whole() { return !(sizeof(T) == 8 && (B == 1024 || B == 4096 || B == 16384)); }
+constexpr bool selftest_whole() {
+    return B == 4096 ? k1_whole<B, T>() : !(sizeof(T) == 8 && (B == 1024 || B == 16384));
+}
 
 // Self-test of the convolution blocks the decoder runs: y = B IFFT(FFT(x) G)
 // for one block, in the register layout the kernels use (v[r] = x[NT r + t]),

@@ -174,9 +174,6 @@ hipError_t run_guard_lists(const GuardLaunch& g, hipStream_t st);
 hipError_t run_fill_f64(double* p, long long n, double v, hipStream_t st);
 // zero n 32-bit words (a fill kernel, not a memset: safe under graph capture)
 hipError_t run_fill_zero(void* p, long long words, hipStream_t st);
-// grid size of the block-list launches when the device's residency cannot be
-// queried (each workgroup loops over the list)
-constexpr int kListGrid = 1024;
 hipError_t run_k3(const ChunkCtl* ctl, int nf_override, int max_fields, const Fmt& fmt, const int* starts,
                   const int* lens, const int* count, int* types, double* linelocs, int* is_first, hipStream_t st);
 hipError_t run_diag_spacing(const double* ll, ChunkCtl* ctl, int max_fields, const Fmt& fmt, hipStream_t st);

@@ -38,13 +38,13 @@ def build(verbose=False, jobs=8):
         objs.append(o)
         cmds.append([HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, s + ".hip"), "-o", o])
     design_o = os.path.join(LIB, "design.o")
-    cmds.append(["g++", "-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-c",
+    cmds.append(["g++", "-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fopenmp", "-c",
                  os.path.join(CSRC, "design.cpp"), "-o", design_o])
     objs.append(design_o)
     with ThreadPoolExecutor(max_workers=jobs) as ex:
         list(ex.map(lambda c: _run(c, verbose), cmds))
     so = os.path.join(LIB, "libcuvhs.so")
-    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", so + ".tmp"] + objs, verbose)
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", so + ".tmp"] + objs + ["-lgomp"], verbose)
     os.replace(so + ".tmp", so)
     gen = os.path.join(LIB, "librfgen.so")
     _run(["g++", "-O2", "-std=c++17", "-fPIC", "-fopenmp", "-shared", "-o", gen + ".tmp",

@@ -13,7 +13,11 @@
 
 #include <algorithm>
 #include <cmath>
+#include <map>
+#include <memory>
+#include <mutex>
 #include <stdexcept>
+#include <utility>
 
 namespace cuvhs {
 
@@ -107,17 +111,24 @@ std::vector<int> factorize(int n) {
     return f;
 }
 
+// Every loop below over bins, taps or transform columns computes each element
+// on its own, so they run in parallel (OpenMP, the host's cores) with the
+// same results as run serially.
+
 // Hermitian extension of a half spectrum (DC/Nyquist imaginary parts
 // ignored, the cuFFT C2R convention) followed by the unnormalised inverse:
 // the real circular kernel the reference's Z2D applies.
 std::vector<double> real_kernel_from_half(const std::vector<cdbl>& half, int n) {
     std::vector<cdbl> full((size_t)n);
+#pragma omp parallel for schedule(static)
     for (int k = 0; k <= n / 2; k++) full[(size_t)k] = half[(size_t)k];
+#pragma omp parallel for schedule(static)
     for (int k = n / 2 + 1; k < n; k++) full[(size_t)k] = std::conj(half[(size_t)(n - k)]);
     full[0] = cdbl(full[0].real(), 0.0);
     if (n % 2 == 0) full[(size_t)n / 2] = cdbl(full[(size_t)n / 2].real(), 0.0);
     host_fft(full, +1);
     std::vector<double> g((size_t)n);
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < n; i++) g[(size_t)i] = full[(size_t)i].real();
     return g;
 }
@@ -195,22 +206,37 @@ int next_7smooth(int n) {
     return n;
 }
 
+// Twiddles exp(sign 2 pi i k / n), k < n, rounded from long double; kept
+// for the sizes already used (a context's designs reuse one size).
+static const std::vector<cdbl>& twiddle_table(int n, int sign) {
+    static std::mutex mu;
+    static std::map<std::pair<int, int>, std::vector<cdbl>> cache;
+    std::lock_guard<std::mutex> lock(mu);
+    std::vector<cdbl>& tw = cache[{n, sign}];
+    if (tw.empty()) {
+        tw.resize((size_t)n);
+        const long double two_pi = 6.283185307179586476925286766559L;
+#pragma omp parallel for schedule(static)
+        for (int k = 0; k < n; k++) {
+            long double a = two_pi * (long double)k / (long double)n;
+            tw[(size_t)k] = cdbl((double)cosl(a), (double)(sign * sinl(a)));
+        }
+    }
+    return tw;
+}
+
 void host_fft(std::vector<cdbl>& x, int sign) {
     const int n = (int)x.size();
     if (n <= 1) return;
     std::vector<int> fac = factorize(n);
-    std::vector<cdbl> tw((size_t)n);
-    const long double two_pi = 6.283185307179586476925286766559L;
-    for (int k = 0; k < n; k++) {
-        long double a = two_pi * (long double)k / (long double)n;
-        tw[(size_t)k] = cdbl((double)cosl(a), (double)(sign * sinl(a)));
-    }
+    const std::vector<cdbl>& tw = twiddle_table(n, sign);
     std::vector<cdbl> y((size_t)n);
     cdbl* a = x.data();
     cdbl* b = y.data();
     long long ns = 1;
     for (int p : fac) {
         int m = n / p;
+#pragma omp parallel for schedule(static)
         for (int j = 0; j < m; j++) {
             long long k = j % ns;
             cdbl v[7], o[7];
@@ -232,16 +258,24 @@ void host_fft(std::vector<cdbl>& x, int sign) {
     if (a != x.data()) std::copy(a, a + n, x.data());
 }
 
-K1Design design_k1(const Fmt& fmt, double eps, const int* ach, int na, const int* bch, int nb, int extra_a,
-                   int elem_bytes) {
-    K1Design d;
-    d.N = next_7smooth(fmt.spfp + kOverlap);
-    const int N = d.N, fb = N / 2 + 1;
-    const double fs = fmt.sample_rate, ny = fs / 2.0;
+namespace {
+
+// The two circular kernels of K1 on the N-point grid, which depend only on
+// the sample rate and N: the analytic RF band-pass and video + i * sync.
+struct K1Kernels {
+    std::vector<cdbl> rf, video;
+};
+
+K1Kernels k1_kernels(double fs, int N) {
+    const int fb = N / 2 + 1;
+    const double ny = fs / 2.0;
     const double w_lo = PI * 1300000.0 / ny, w_hi = PI * 7500000.0 / ny;
     const double w_lpf = PI * 8000000.0 / ny, w_hpf = PI * 1000000.0 / ny;
+    K1Kernels kk;
     // analytic RF response: bins 0..N/2 kept, Hilbert weights (1,2,...,2,1)
-    std::vector<cdbl> spec((size_t)N, cdbl(0.0, 0.0));
+    std::vector<cdbl>& spec = kk.rf;
+    spec.assign((size_t)N, cdbl(0.0, 0.0));
+#pragma omp parallel for schedule(static)
     for (int k = 0; k < fb; k++) {
         double om = PI * (double)k / (double)(fb - 1);
         double hil = (k == 0 || k == fb - 1) ? 1.0 : 2.0;
@@ -249,13 +283,13 @@ K1Design design_k1(const Fmt& fmt, double eps, const int* ach, int na, const int
                           butter_hpf_mag(om, w_hpf, 8) * hil;
     }
     host_fft(spec, +1);  // unnormalised inverse: the analytic signal carries the N scale
-    d.rf = make_block(spec, eps, ach, na, extra_a, elem_bytes);
 
     double bs[3], as[3];
     shelf_high(273755.82, 13.9794, fs, 0.462088186, bs, as);
     std::vector<double> sync = hamming_lowpass(65, 500000.0 / ny);
     std::vector<cdbl> fv((size_t)fb), f05((size_t)fb);
     double inv_n = 1.0 / (double)N;
+#pragma omp parallel for schedule(static)
     for (int k = 0; k < fb; k++) {
         double om = PI * (double)k / (double)(fb - 1);
         double f_hz = (double)k * ny / (double)(fb - 1);
@@ -267,9 +301,36 @@ K1Design design_k1(const Fmt& fmt, double eps, const int* ach, int na, const int
     }
     std::vector<double> gv = real_kernel_from_half(fv, N);
     std::vector<double> g5 = real_kernel_from_half(f05, N);
-    std::vector<cdbl> gc((size_t)N);
-    for (int i = 0; i < N; i++) gc[(size_t)i] = cdbl(gv[(size_t)i], g5[(size_t)i]);
-    d.video = make_block(gc, eps, bch, nb, 0, elem_bytes);
+    kk.video.resize((size_t)N);
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < N; i++) kk.video[(size_t)i] = cdbl(gv[(size_t)i], g5[(size_t)i]);
+    return kk;
+}
+
+// The kernels of the last few (rate, N) pairs designed: a mixed-precision
+// context designs K1 twice (its own blocks and the FP64 guard band's) and a
+// server creates many contexts at one rate, all from the same kernels.
+std::shared_ptr<const K1Kernels> cached_k1_kernels(double fs, int N) {
+    static std::mutex mu;
+    static std::map<std::pair<double, int>, std::shared_ptr<const K1Kernels>> cache;
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = cache.find({fs, N});
+    if (it != cache.end()) return it->second;
+    if (cache.size() >= 4) cache.clear();
+    auto kk = std::make_shared<const K1Kernels>(k1_kernels(fs, N));
+    cache[{fs, N}] = kk;
+    return kk;
+}
+
+}  // namespace
+
+K1Design design_k1(const Fmt& fmt, double eps, const int* ach, int na, const int* bch, int nb, int extra_a,
+                   int elem_bytes) {
+    K1Design d;
+    d.N = next_7smooth(fmt.spfp + kOverlap);
+    std::shared_ptr<const K1Kernels> kk = cached_k1_kernels(fmt.sample_rate, d.N);
+    d.rf = make_block(kk->rf, eps, ach, na, extra_a, elem_bytes);
+    d.video = make_block(kk->video, eps, bch, nb, 0, elem_bytes);
     return d;
 }
 
@@ -281,6 +342,7 @@ PreDesign design_prebpf(const Fmt& fmt, double eps, const int* bch, int nb) {
     double om_hi = 2.0 * M_PI * 1200000.0 / fmt.sample_rate;
     double norm = 1.0 / (double)N2;
     std::vector<cdbl> half((size_t)fb);
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < fb; i++) {
         double om = M_PI * (double)i / (double)(fb - 1);
         double mag = butter_bpf_mag(om, om_lo, om_hi, 4);

@@ -1,7 +1,7 @@
 // `cuvhs` command line: the reference's drop-in surface (src/main.cpp:24-165,
 // Pipeline::run pipeline.cu:2626-2713) on top of the C ABI in include/cuvhs.h.
 //
-//   cuvhs [--system NTSC] [-f MHz] [--format u8|s16|u16] [--dc-correct]
+//   cuvhs [--system NTSC|PAL] [-f MHz] [--format u8|s16|u16] [--dc-correct]
 //         [--gpu id] [--batch F] [--precision fp64|fp32|mixed-fp16|mixed-bf16] [--overwrite]
 //         <input | -> <output_base>
 //
@@ -52,7 +52,7 @@ void usage(const char* prog) {
             "  input_file       Raw RF capture (.u8/.raw, .s16, .u16), or - for stdin\n"
             "  output_base      Output prefix (creates .tbc, _chroma.tbc, .tbc.json)\n\n"
             "Options:\n"
-            "  --system <NTSC>       Video system (default: NTSC)\n"
+            "  --system <NTSC|PAL>   Video system (default: NTSC)\n"
             "  -f <MHz>              Sample rate in MHz (default: 28)\n"
             "  --format <u8|s16|u16> Input sample format (default: from extension)\n"
             "  --dc-correct          Subtract per-read DC offset after normalization\n"
@@ -267,13 +267,12 @@ int main(int argc, char** argv) {
     Args a;
     if (!parse(argc, argv, a)) return 1;
     if (a.format.empty()) a.format = detect_format(a.input);
-    if (a.pal) { fprintf(stderr, "PAL is not supported by this decoder\n"); return 1; }
     if (a.batch < 1) { fprintf(stderr, "--batch must be >= 1\n"); return 1; }
     const int in_fmt = a.format == "s16" ? CUVHS_INPUT_S16 : (a.format == "u16" ? CUVHS_INPUT_U16 : CUVHS_INPUT_U8);
     const int bps = in_fmt == CUVHS_INPUT_U8 ? 1 : 2;
 
     cuvhs_config cfg{};
-    cfg.system = CUVHS_SYSTEM_NTSC;
+    cfg.system = a.pal ? CUVHS_SYSTEM_PAL : CUVHS_SYSTEM_NTSC;
     cfg.sample_rate_mhz = a.mhz;
     cfg.batch_fields = a.batch;
     cfg.precision = a.precision;
@@ -294,8 +293,8 @@ int main(int argc, char** argv) {
     }
     cuvhs_format fmt;
     cuvhs_get_format(ctx, &fmt);
-    fprintf(stderr, "Format: NTSC @ %.1f MHz capture rate\n  Samples/line: %d  Samples/field: %d\n", a.mhz,
-            fmt.samples_per_line, fmt.samples_per_field);
+    fprintf(stderr, "Format: %s @ %.1f MHz capture rate\n  Samples/line: %d  Samples/field: %d\n",
+            a.pal ? "PAL" : "NTSC", a.mhz, fmt.samples_per_line, fmt.samples_per_field);
 
     if ((log_k3 || log_k4 || log_k5) && cuvhs_diagnostics(ctx, 1) != 0) {
         fprintf(stderr, "Failed to enable diagnostics\n");
@@ -317,6 +316,7 @@ int main(int argc, char** argv) {
                 src.total() * (double)bps / (1024.0 * 1024.0 * 1024.0), dc_note);
 
     cuvhs_cli::TbcWriter writer;
+    writer.set_format(fmt);
     if (!writer.open(a.output, a.overwrite)) {
         fprintf(stderr, "Failed to open output: %s\n", a.output.c_str());
         cuvhs_destroy(ctx);
@@ -324,7 +324,7 @@ int main(int argc, char** argv) {
     }
 
     const size_t chunk_cap = (size_t)a.batch * (size_t)fmt.spfp;
-    const size_t tbc = (size_t)CUVHS_FIELD_LINES * CUVHS_LINE_LEN;
+    const size_t tbc = (size_t)fmt.field_lines * fmt.line_len;
     uint8_t* h_chunk = nullptr;
     uint8_t* d_chunk = nullptr;
     CHECK_HIP(hipHostMalloc((void**)&h_chunk, chunk_cap * bps, hipHostMallocDefault));

@@ -9,7 +9,6 @@
 namespace cuvhs_cli {
 
 namespace {
-const int kW = 910, kH = 263;
 bool exists(const std::string& p) {
     struct stat st;
     return stat(p.c_str(), &st) == 0;
@@ -40,7 +39,9 @@ bool TbcWriter::open(const std::string& base, bool overwrite) {
 }
 
 bool TbcWriter::write_fields(const uint16_t* luma, const uint16_t* chroma, int n) {
-    size_t cnt = (size_t)n * kW * kH;
+    cuvhs_format f = fmt_;
+    if (!has_fmt_) cuvhs_format_for(28.0, &f);
+    size_t cnt = (size_t)n * f.line_len * f.field_lines;
     return fwrite(luma, sizeof(uint16_t), cnt, luma_) == cnt && fwrite(chroma, sizeof(uint16_t), cnt, chroma_) == cnt;
 }
 
@@ -48,27 +49,27 @@ bool TbcWriter::write_json() const {
     std::string tmp = json_ + ".tmp";
     FILE* fp = fopen(tmp.c_str(), "w");
     if (!fp) { perror(tmp.c_str()); return false; }
-    const double output_scale = (51200.0 - 1024.0) / (100.0 - (-40.0));
-    const double black16b = (0.0 - (-40.0)) * output_scale + 1024.0;
-    const double white16b = (100.0 - (-40.0)) * output_scale + 1024.0;
-    const double output_rate = 4.0 * 3579545.0;
-    const int burst_start = (int)(5.3 * 1e-6 * output_rate + 0.5);
-    const int burst_end = (int)(7.8 * 1e-6 * output_rate + 0.5);
+    // the levels, rates and geometry come from the decoder's format (the
+    // sample rate does not enter them); the active-video columns are the
+    // reference's per-system constants (tbc_writer.cpp:106-107)
+    cuvhs_format f = fmt_;
+    if (!has_fmt_) cuvhs_format_for(28.0, &f);
+    const bool pal = f.system == CUVHS_SYSTEM_PAL;
     const int n = (int)meta_.size();
     fprintf(fp, "{\n  \"videoParameters\": {\n");
-    fprintf(fp, "    \"system\": \"NTSC\",\n");
+    fprintf(fp, "    \"system\": \"%s\",\n", pal ? "PAL" : "NTSC");
     fprintf(fp, "    \"isSubcarrierLocked\": false,\n");
-    fprintf(fp, "    \"isSourcePal\": false,\n");
+    fprintf(fp, "    \"isSourcePal\": %s,\n", pal ? "true" : "false");
     fprintf(fp, "    \"numberOfSequentialFields\": %d,\n", n);
-    fprintf(fp, "    \"black16bIre\": %.1f,\n", black16b);
-    fprintf(fp, "    \"white16bIre\": %.1f,\n", white16b);
-    fprintf(fp, "    \"sampleRate\": %.0f,\n", output_rate);
-    fprintf(fp, "    \"fieldWidth\": %d,\n", kW);
-    fprintf(fp, "    \"fieldHeight\": %d,\n", kH);
-    fprintf(fp, "    \"colourBurstStart\": %d,\n", burst_start);
-    fprintf(fp, "    \"colourBurstEnd\": %d,\n", burst_end);
-    fprintf(fp, "    \"activeVideoStart\": %d,\n", 134);
-    fprintf(fp, "    \"activeVideoEnd\": %d,\n", 894);
+    fprintf(fp, "    \"black16bIre\": %.1f,\n", f.black16b);
+    fprintf(fp, "    \"white16bIre\": %.1f,\n", f.white16b);
+    fprintf(fp, "    \"sampleRate\": %.0f,\n", f.output_rate);
+    fprintf(fp, "    \"fieldWidth\": %d,\n", f.line_len);
+    fprintf(fp, "    \"fieldHeight\": %d,\n", f.field_lines);
+    fprintf(fp, "    \"colourBurstStart\": %d,\n", f.burst_start);
+    fprintf(fp, "    \"colourBurstEnd\": %d,\n", f.burst_end);
+    fprintf(fp, "    \"activeVideoStart\": %d,\n", pal ? 185 : 134);
+    fprintf(fp, "    \"activeVideoEnd\": %d,\n", pal ? 1107 : 894);
     fprintf(fp, "    \"tapeFormat\": \"VHS\",\n");
     fprintf(fp, "    \"isMapped\": false\n");
     fprintf(fp, "  },\n  \"fields\": [\n");

@@ -6,6 +6,8 @@
 #include <string>
 #include <vector>
 
+#include "../../../include/cuvhs.h"
+
 namespace cuvhs_cli {
 
 struct FieldMeta {
@@ -18,6 +20,8 @@ struct FieldMeta {
 class TbcWriter {
   public:
     ~TbcWriter();
+    // The decoder's format (system, field geometry, levels); NTSC until set.
+    void set_format(const cuvhs_format& f) { fmt_ = f; has_fmt_ = true; }
     // Refuses existing outputs unless overwrite (tbc_writer.cpp:16-31).
     bool open(const std::string& base, bool overwrite);
     bool write_fields(const uint16_t* luma, const uint16_t* chroma, int nfields);
@@ -28,6 +32,8 @@ class TbcWriter {
     int fields() const { return (int)meta_.size(); }
 
   private:
+    cuvhs_format fmt_{};
+    bool has_fmt_ = false;
     FILE* luma_ = nullptr;
     FILE* chroma_ = nullptr;
     std::string json_;

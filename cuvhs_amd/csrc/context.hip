@@ -152,6 +152,8 @@ struct cuvhs_ctx {
     int pre_f32 = 0;           // chroma pre-band-pass transforms in FP32 (CUVHS_PRECISION_FP32)
     double2* tw_1024 = nullptr;
     float2* tw_1024f = nullptr;     // the chroma line filter's twiddles in FP32 (chroma_f32)
+    double2* tw_2048 = nullptr;     // PAL's 2048-point line transform: W2048^m
+    float2* tw_2048f = nullptr;
     int chroma_f32 = 0;             // chroma line filter in FP32 (the reduced-precision forms)
     int block_source = 2;           // block sizes from: 0 the config, 1 the tuned table, 2 the cost model
     int graphs = 0;                 // cuvhs_decode_capture replays captured stage graphs (cuvhs_config.graphs)
@@ -303,8 +305,8 @@ void mark(cuvhs_ctx* c, Slot& s, int i, hipStream_t st) {
 
 // Per-line chroma band-pass bins (chroma_decode.cu:538-545).
 void chroma_band(const Fmt& f, int* lo, int* hi) {
-    int fsc_bin = (int)(f.fsc / f.output_rate * kChromaFFT + 0.5);
-    int bw = (int)(500000.0 / f.output_rate * kChromaFFT + 0.5);
+    int fsc_bin = (int)(f.fsc / f.output_rate * f.chroma_fft + 0.5);
+    int bw = (int)(500000.0 / f.output_rate * f.chroma_fft + 0.5);
     if (bw < 10) bw = 10;
     *lo = fsc_bin - bw;
     *hi = fsc_bin + bw;
@@ -320,9 +322,14 @@ ChromaLaunch chroma_launch(cuvhs_ctx* c, const ChunkCtl* ctl, int nf_override, c
     a.het_scale = (c->fmt.fsc + c->fmt.chroma_under) / c->fmt.output_rate;
     a.het_dc = cos(2.0 * M_PI * a.het_scale * 64.0);
     a.het_ds = sin(2.0 * M_PI * a.het_scale * 64.0);
-    a.fft_scale = 1.0 / (double)kChromaFFT;
+    a.fft_scale = 1.0 / (double)c->fmt.chroma_fft;
     a.burst_abs_ref = c->fmt.burst_abs_ref;
     a.burst_start = c->fmt.burst_start;
+    a.burst_len = c->fmt.burst_end - c->fmt.burst_start;
+    a.lpf = c->fmt.lines_per_frame;
+    a.field_lines = c->fmt.field_lines;
+    a.line_len = c->fmt.line_len;
+    a.nfft = c->fmt.chroma_fft;
     chroma_band(c->fmt, &a.lo, &a.hi);
     a.mask = c->chroma_mask;
     a.burst = c->burst;
@@ -330,6 +337,8 @@ ChromaLaunch chroma_launch(cuvhs_ctx* c, const ChunkCtl* ctl, int nf_override, c
     a.out = c->chroma;
     a.tw = c->tw_1024;
     a.tw_f = c->tw_1024f;
+    a.tw2 = c->tw_2048;
+    a.tw2_f = c->tw_2048f;
     a.f32 = c->chroma_f32;
     return a;
 }
@@ -637,7 +646,7 @@ int tail_stage(cuvhs_ctx* c, Slot& s, const OutArrays* out, hipStream_t fork_str
     }
     if (c->diag) {
         HIPCHK(c, run_diag_spacing(c->ll, s.ctl, s.max_fields, f, st));
-        HIPCHK(c, hipMemcpyAsync(c->ll_coarse, c->ll, (size_t)s.max_fields * kLinesPerFrame * sizeof(double),
+        HIPCHK(c, hipMemcpyAsync(c->ll_coarse, c->ll, (size_t)s.max_fields * f.lines_per_frame * sizeof(double),
                                  hipMemcpyDeviceToDevice, st));
     }
     mark(c, s, 8, st);
@@ -896,7 +905,7 @@ struct OutBases {
 void fill_plan(cuvhs_ctx* c, Slot& s, const OutBases& ob, long long out_fields) {
     ChunkPlan& p = s.hplan;
     const long long T = (long long)s.fk1 * c->fmt.spfp;
-    const size_t tbc = (size_t)kFieldLines * kLineLen;
+    const size_t tbc = (size_t)c->fmt.field_lines * c->fmt.line_len;
     p.chunk = s.chunk;
     p.n_read = (long long)s.n_read;
     p.total = T;
@@ -1032,9 +1041,9 @@ static int block_selftest(int B, int is_f32, const void* d_x, const void* d_g, v
 
 extern "C" {
 
-int cuvhs_format_for(double mhz, cuvhs_format* o) {
-    if (!o || !(mhz > 0.0)) return CUVHS_E_ARG;
-    Fmt f = make_fmt(mhz);
+int cuvhs_format_for_system(int system, double mhz, cuvhs_format* o) {
+    if (!o || !(mhz > 0.0) || (system != CUVHS_SYSTEM_NTSC && system != CUVHS_SYSTEM_PAL)) return CUVHS_E_ARG;
+    Fmt f = make_fmt(mhz, system == CUVHS_SYSTEM_PAL ? kSystemPal : kSystemNtsc);
     o->sample_rate = f.sample_rate;
     o->samples_per_line = f.samples_per_line;
     o->samples_per_field = f.samples_per_field;
@@ -1049,13 +1058,20 @@ int cuvhs_format_for(double mhz, cuvhs_format* o) {
     o->white16b = (100.0 - f.vsync_ire) * f.output_scale + f.output_zero;
     o->burst_start = f.burst_start;
     o->burst_end = f.burst_end;
+    o->system = system;
+    o->lines_per_frame = f.lines_per_frame;
+    o->field_lines = f.field_lines;
+    o->line_len = f.line_len;
+    o->fsc = f.fsc;
     return 0;
 }
+
+int cuvhs_format_for(double mhz, cuvhs_format* o) { return cuvhs_format_for_system(CUVHS_SYSTEM_NTSC, mhz, o); }
 
 int cuvhs_create(const cuvhs_config* cfg, cuvhs_ctx** out) {
     if (!cfg || !out) return CUVHS_E_ARG;
     *out = nullptr;
-    if (cfg->system != CUVHS_SYSTEM_NTSC || cfg->precision < CUVHS_PRECISION_FP64 ||
+    if ((cfg->system != CUVHS_SYSTEM_NTSC && cfg->system != CUVHS_SYSTEM_PAL) || cfg->precision < CUVHS_PRECISION_FP64 ||
         cfg->precision > CUVHS_PRECISION_MIXED_BF16)
         return CUVHS_E_UNSUPPORTED;
     // the matrix-core part A takes u8 samples, exact in f16 / bf16 after
@@ -1073,7 +1089,7 @@ int cuvhs_create(const cuvhs_config* cfg, cuvhs_ctx** out) {
     cuvhs_ctx* c = new cuvhs_ctx();
     c->cfg = *cfg;
     c->graphs = cfg->graphs != 0;
-    c->fmt = make_fmt(cfg->sample_rate_mhz);
+    c->fmt = make_fmt(cfg->sample_rate_mhz, cfg->system == CUVHS_SYSTEM_PAL ? kSystemPal : kSystemNtsc);
     c->F = cfg->batch_fields;
     const Fmt& f = c->fmt;
     if ((long long)c->F * f.spfp > 0x3fffffffLL) { delete c; return CUVHS_E_CAPACITY; }
@@ -1202,18 +1218,32 @@ int cuvhs_create(const cuvhs_config* cfg, cuvhs_ctx** out) {
         if ((e = upload_twiddles(c, &c->tw_a, c->k1.rf.B, c->k1_f32)) != hipSuccess) return bad(e, "twiddles");
         if ((e = upload_twiddles(c, &c->tw_b, c->k1.video.B, c->k1_f32)) != hipSuccess) return bad(e, "twiddles");
         if ((e = upload_twiddles(c, &c->tw_p, c->pre.bpf.B, c->pre_f32)) != hipSuccess) return bad(e, "twiddles");
-        auto t1 = twiddles<double2, double>(kChromaFFT);
+        auto t1 = twiddles<double2, double>(1024);
         ALLOC(c->tw_1024, t1.size());
         if ((e = hipMemcpy(c->tw_1024, t1.data(), t1.size() * sizeof(double2), hipMemcpyHostToDevice)) != hipSuccess) return bad(e, "twiddles");
         if (c->chroma_f32) {
-            auto t2 = twiddles<float2, float>(kChromaFFT);
+            auto t2 = twiddles<float2, float>(1024);
             ALLOC(c->tw_1024f, t2.size());
             if ((e = hipMemcpy(c->tw_1024f, t2.data(), t2.size() * sizeof(float2), hipMemcpyHostToDevice)) != hipSuccess)
                 return bad(e, "twiddles");
         }
+        if (f.chroma_fft == 2048) {
+            auto t3 = twiddles<double2, double>(2048);
+            ALLOC(c->tw_2048, t3.size());
+            if ((e = hipMemcpy(c->tw_2048, t3.data(), t3.size() * sizeof(double2), hipMemcpyHostToDevice)) != hipSuccess)
+                return bad(e, "twiddles");
+            if (c->chroma_f32) {
+                auto t4 = twiddles<float2, float>(2048);
+                ALLOC(c->tw_2048f, t4.size());
+                if ((e = hipMemcpy(c->tw_2048f, t4.data(), t4.size() * sizeof(float2), hipMemcpyHostToDevice)) !=
+                    hipSuccess)
+                    return bad(e, "twiddles");
+            }
+        }
         // burst I/Q reference (chroma_decode.cu:435-437), computed on the host
-        std::vector<double> cs(2 * 36);
-        for (int i = 0; i < 36; i++) {
+        const int blen = f.burst_end - f.burst_start;
+        std::vector<double> cs((size_t)2 * blen);
+        for (int i = 0; i < blen; i++) {
             double t = 2.0 * M_PI * f.fsc / f.output_rate * (double)(f.burst_start + i);
             cs[(size_t)2 * i] = cos(t);
             cs[(size_t)2 * i + 1] = sin(t);
@@ -1223,9 +1253,9 @@ int cuvhs_create(const cuvhs_config* cfg, cuvhs_ctx** out) {
         // band-pass bins kept by the per-line chroma filter (chroma_decode.cu:163-182)
         int lo = 0, hi = 0;
         chroma_band(f, &lo, &hi);
-        std::vector<double2> mk((size_t)kChromaFFT);
-        for (int k = 0; k < kChromaFFT; k++) {
-            int kk = k <= kChromaFFT / 2 ? k : kChromaFFT - k;
+        std::vector<double2> mk((size_t)f.chroma_fft);
+        for (int k = 0; k < f.chroma_fft; k++) {
+            int kk = k <= f.chroma_fft / 2 ? k : f.chroma_fft - k;
             mk[(size_t)k].x = (kk >= lo && kk <= hi) ? 1.0 : 0.0;
             mk[(size_t)k].y = 0.0;
         }
@@ -1288,16 +1318,16 @@ int cuvhs_create(const cuvhs_config* cfg, cuvhs_ctx** out) {
         ALLOC(p2, Fz * k2b_anchor_bytes());
         c->anchors = p2;
     }
-    ALLOC(c->ll, Fz * kLinesPerFrame);
-    ALLOC(c->ll_coarse, Fz * kLinesPerFrame);
+    ALLOC(c->ll, Fz * f.lines_per_frame);
+    ALLOC(c->ll_coarse, Fz * f.lines_per_frame);
     ALLOC(c->isf_k2b, Fz);
-    ALLOC(c->luma, Fz * kFieldLines * kLineLen);
-    ALLOC(c->chroma, Fz * kFieldLines * kLineLen);
-    ALLOC(c->burst, Fz * kFieldLines * 36);
-    ALLOC(c->chroma_filt, Fz * kFieldLines * kLineLen);
+    ALLOC(c->luma, Fz * f.field_lines * f.line_len);
+    ALLOC(c->chroma, Fz * f.field_lines * f.line_len);
+    ALLOC(c->burst, Fz * f.field_lines * (f.burst_end - f.burst_start));
+    ALLOC(c->chroma_filt, Fz * f.field_lines * f.line_len);
     ALLOC(c->m0, Fz);
     ALLOC(c->m1, Fz);
-    ALLOC(c->iq0, 2 * kFieldLines);
+    ALLOC(c->iq0, 2 * f.field_lines);
     ALLOC(c->track, Fz);
     ALLOC(c->phase, Fz);
     ALLOC(c->dol, Fz * kMaxDropouts);
@@ -1351,7 +1381,7 @@ int cuvhs_destroy(cuvhs_ctx* c) {
 
 int cuvhs_get_format(const cuvhs_ctx* c, cuvhs_format* out) {
     if (!c) return CUVHS_E_ARG;
-    return cuvhs_format_for(c->cfg.sample_rate_mhz, out);
+    return cuvhs_format_for_system(c->cfg.system, c->cfg.sample_rate_mhz, out);
 }
 
 
@@ -1421,7 +1451,7 @@ int cuvhs_decode_capture(cuvhs_ctx* c, const void* cap, uint64_t cap_len, int ma
     if (!c || (!cap && cap_len) || !fields || max_out < 0) return CUVHS_E_ARG;
     *fields = 0;
     const uint64_t spf = (uint64_t)c->fmt.samples_per_field;
-    const size_t tbc = (size_t)kFieldLines * kLineLen;
+    const size_t tbc = (size_t)c->fmt.field_lines * c->fmt.line_len;
     // the stages replay their captured graphs unless profiling or diagnostics
     // need the launch-by-launch schedule
     const bool graphs = c->graphs && !c->prof && !c->diag && !c->serial;
@@ -1686,7 +1716,7 @@ int cuvhs_chroma(cuvhs_ctx* c, const void* raw, uint64_t n_valid, const double* 
         HIPCHK(c, run_input_mean(raw, c->cfg.input_format, (long long)n_valid, s.dc_partial, s.dc, c->stream));
     int r = prebpf_launch(c, raw, (long long)n_valid, nf, s.craw, s.dc, c->stream);
     if (r) return r;
-    HIPCHK(c, hipMemcpyAsync(c->ll, ll, (size_t)nf * kLinesPerFrame * 8, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->ll, ll, (size_t)nf * c->fmt.lines_per_frame * 8, hipMemcpyDeviceToDevice, c->stream));
     ChromaLaunch a = chroma_launch(c, s.ctl, nf, s.craw);
     a.out = out;
     HIPCHK(c, run_chroma(a, c->state, nf, c->m0, c->m1, c->cs_tab, c->iq0, c->track, c->phase, 1, c->stream));

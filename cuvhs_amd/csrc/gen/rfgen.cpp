@@ -1,12 +1,18 @@
-// Synthetic VHS NTSC RF generator (u8 captures).
+// Synthetic VHS RF generator (u8 captures), NTSC or PAL.
 //
 // Produces deterministic captures for tests and benchmarks: NTSC 525/59.94
 // sync structure (6 EQ, 6 broad, 6 EQ pulses at half-line spacing, 4.7 us
-// HSYNC), active lines in bands of horizontal ramps and 75% colour bars, the
-// luma FM-modulated with sync tip at 3.4 MHz and peak white at 4.4 MHz, the
-// colour-under QAM at 40 fH carrying the VHS per-line +-90 degree rotation of
-// an alternating head track, Gaussian noise at a stated SNR, RF-envelope
-// dropouts, wow/flutter time warp and head-switch timing steps.
+// HSYNC) or PAL 625/50 (5 EQ, 5 broad, 5 EQ), active lines in bands of
+// horizontal ramps and 75% colour bars, the luma FM-modulated on the
+// system's VHS deviation (NTSC: sync tip 3.4 MHz, peak white 4.4 MHz; PAL:
+// 3.8 and 4.8 MHz), the colour-under QAM (NTSC 40 fH, PAL 40 fH + 1953 Hz)
+// carrying the VHS per-line +-90 degree rotation of an alternating head
+// track, Gaussian noise at a stated SNR, RF-envelope dropouts, wow/flutter
+// time warp and head-switch timing steps.  The line structure of the two
+// fields follows what the decoder's field-order logic expects (the spacing
+// between the last HSYNC and the first EQ pulse, and between the last EQ
+// pulse and the first HSYNC: NTSC first field 1 and 0.5 lines, second 0.5
+// and 1; PAL first field 0.5 and 0.5, second 1 and 1).
 //
 // Determinism: the output depends only on the parameters, never on the
 // thread count.  Work is split into fixed 64 Ki-sample blocks; the FM phase
@@ -21,14 +27,75 @@
 namespace {
 
 const double kPi = 3.14159265358979323846;
-const double kLineRate = 15734.264;                  // Hz
-const double kH = 1.0 / kLineRate;                   // line period
-const double kFieldLines = 262.5;
-const double kFscGen = 227.5 * kLineRate;            // line-locked subcarrier
-const double kHzIre = 1e6 / 140.0;
-const double kIre0 = 4.4e6 - kHzIre * 100.0;
-const double kCuPerSample = 629370.6293706293 / (4.0 * 3579545.0);  // colour-under, cycles/TBC sample
 const int kBlock = 1 << 16;
+
+// The line structure and levels of a system.
+struct Sys {
+    double line_rate;     // Hz
+    double H;             // line period
+    double field_lines;   // 262.5 / 312.5
+    double fsc_gen;       // line-locked subcarrier: 227.5 / 283.75 fH
+    double tbc_len;       // TBC samples per line: 4 fsc_gen / fH
+    double hz_ire, ire0, sync_ire;
+    double cu_per_sample; // colour-under cycles per TBC sample (decoder's subcarrier)
+    double vblank_end;    // lines of half-line pulses from the first pre-EQ: 9 / 7.5
+    double broad0, broad1;// broad pulses between these positions (lines)
+    double eq_w, broad_w; // pulse widths (s)
+    double burst0, burst1;// burst gate (s from the HSYNC edge)
+    double act0, act1;    // active picture (s)
+    double act_w;         // its width (s)
+    int first_active;     // first line of the picture bands
+    bool pal;
+};
+
+Sys make_sys(int system) {
+    const double us = 1e-6;
+    Sys y;
+    y.pal = system == 1;
+    if (y.pal) {
+        y.line_rate = 15625.0;
+        y.field_lines = 312.5;
+        y.fsc_gen = 283.75 * y.line_rate;
+        y.tbc_len = 1135.0;
+        y.sync_ire = -0.3 * (100.0 / 0.7);
+        y.hz_ire = 1e6 / (100.0 - y.sync_ire);
+        y.ire0 = 4.8e6 - y.hz_ire * 100.0;
+        y.cu_per_sample = (625.0 * 25.0 * 40.0 + 1953.0) / (4.0 * 4433618.75);
+        y.vblank_end = 7.5;
+        y.broad0 = 2.5;
+        y.broad1 = 5.0;
+        y.eq_w = 2.35 * us;
+        y.broad_w = 27.3 * us;
+        y.burst0 = 5.6 * us;
+        y.burst1 = 7.85 * us;
+        y.act0 = 10.5 * us;
+        y.act1 = 62.5 * us;
+        y.act_w = 52.0 * us;
+        y.first_active = 23;
+    } else {
+        y.line_rate = 15734.264;
+        y.field_lines = 262.5;
+        y.fsc_gen = 227.5 * y.line_rate;
+        y.tbc_len = 910.0;
+        y.sync_ire = -40.0;
+        y.hz_ire = 1e6 / 140.0;
+        y.ire0 = 4.4e6 - y.hz_ire * 100.0;
+        y.cu_per_sample = 629370.6293706293 / (4.0 * 3579545.0);
+        y.vblank_end = 9.0;
+        y.broad0 = 3.0;
+        y.broad1 = 6.0;
+        y.eq_w = 2.3 * us;
+        y.broad_w = 27.1 * us;
+        y.burst0 = 5.3 * us;
+        y.burst1 = 7.8 * us;
+        y.act0 = 10.9 * us;
+        y.act1 = 62.0 * us;
+        y.act_w = 51.1 * us;
+        y.first_active = 20;
+    }
+    y.H = 1.0 / y.line_rate;
+    return y;
+}
 
 struct Rng {
     uint64_t s;
@@ -69,6 +136,7 @@ struct FieldParams {
 
 struct Model {
     cuvhs_gen_params p;
+    Sys y;
     double fs;
     double t_lead;       // ideal time of field 0's first pre-EQ pulse
     int nfield;          // fields with parameters: index k+1 for k in [-1, nfield-2]
@@ -106,37 +174,42 @@ struct Sample {
 
 // Signal at ideal time tau (already warped); field bookkeeping included.
 Sample eval(const Model& m, double tau) {
-    const double fl = kFieldLines * kH;
+    const Sys& y = m.y;
+    const double kH = y.H;
+    const double fl = y.field_lines * kH;
     long k = (long)floor((tau - m.t_lead) / fl);
     const FieldParams& f = m.field(k);
     double t = tau - f.hs_offset;
     k = (long)floor((t - m.t_lead) / fl);
     double fstart = m.t_lead + (double)k * fl;
     double v = (t - fstart) / kH;  // position in lines from the first pre-EQ
-    bool type1 = (k % 2 + 2) % 2 == 0;
+    bool type1 = (k % 2 + 2) % 2 == 0;   // the first field of the frame
+    // HSYNCs on whole lines from the first pre-EQ pulse (NTSC first field,
+    // PAL second field) or half a line later
+    const bool whole = type1 != y.pal;
     const double us = 1e-6;
     const double w = 0.14 * us;    // edge width
     double ire = 0.0;              // blanking
     double ca = 0.0, cph = 0.0;    // chroma amplitude (IRE) and phase (rad, from +U)
-    if (v < 9.0) {
+    if (v < y.vblank_end) {
         // vertical interval: half-line spaced pulses
         double hv = floor(v * 2.0) * 0.5;
         double x = (v - hv) * kH;  // seconds into the half line
-        bool broad = (v >= 3.0 && v < 6.0);
-        double pw = broad ? 27.1 * us : 2.3 * us;
-        ire = -40.0 * pulse(x, 0.0, pw, w);
+        bool broad = (v >= y.broad0 && v < y.broad1);
+        double pw = broad ? y.broad_w : y.eq_w;
+        ire = y.sync_ire * pulse(x, 0.0, pw, w);
     } else {
-        double line_origin = type1 ? floor(v) : floor(v - 0.5) + 0.5;
+        double line_origin = whole ? floor(v) : floor(v - 0.5) + 0.5;
         double x = (v - line_origin) * kH;
         int line_in_field = (int)floor(line_origin);
-        ire = -40.0 * pulse(x, 0.0, 4.7 * us, w);
-        // burst 5.3 .. 7.8 us along -U (180 degrees)
-        double bg = pulse(x, 5.3 * us, 7.8 * us, 0.3 * us);
+        ire = y.sync_ire * pulse(x, 0.0, 4.7 * us, w);
+        // burst along -U (180 degrees)
+        double bg = pulse(x, y.burst0, y.burst1, 0.3 * us);
         ca = 20.0 * bg;
         cph = kPi;
-        if (line_in_field >= 20 && x > 10.9 * us && x < 62.0 * us) {
-            double pos = (x - 10.9 * us) / (51.1 * us);   // 0..1 across active video
-            double gate = pulse(x, 10.9 * us, 62.0 * us, w);
+        if (line_in_field >= y.first_active && x > y.act0 && x < y.act1) {
+            double pos = (x - y.act0) / y.act_w;   // 0..1 across active video
+            double gate = pulse(x, y.act0, y.act1, w);
             int band = (line_in_field / 16) & 1;
             if (band == 0) {
                 ire += gate * 100.0 * pos;
@@ -154,16 +227,21 @@ Sample eval(const Model& m, double tau) {
     }
     Sample s;
     s.ire = ire;
-    // colour-under (decoder line grid: line 0 starts 1 H (type 1) or 0.5 H
-    // (type 2) before the first pre-EQ pulse)
-    double tau0 = fstart - (type1 ? 1.0 : 0.5) * kH;
-    double u = 4.0 * kFscGen * (t - tau0);       // TBC-sample coordinate
-    long r = (long)floor(u / 910.0);
+    // colour-under (decoder line grid: line 0 starts one entry spacing, 1 H
+    // or 0.5 H, before the first pre-EQ pulse)
+    double tau0 = fstart - (whole ? 1.0 : 0.5) * kH;
+    double u = 4.0 * y.fsc_gen * (t - tau0);     // TBC-sample coordinate
+    long r = (long)floor(u / y.tbc_len);
     int track = (int)((k % 2 + 2) % 2);
     int rot = track ? 1 : 3;
-    double phi_r = (double)(((r * rot) % 4 + 4) % 4) * (kPi * 0.5);
-    double psi = cph + 2.0 * kPi * fmod(kFscGen * tau0, 1.0);
-    double ucyc = fmod(kCuPerSample * u, 1.0);
+    // PAL's 1135-sample lines advance the subcarrier by 3/4 cycle per line
+    // (NTSC's 910: 1/2); a further quarter cycle per line on PAL keeps the
+    // burst alternating by half a cycle from line to line, the relation the
+    // decoder's track detection measures (chroma_decode.cu:383-411)
+    const long pal_q = y.pal ? r : 0;
+    double phi_r = (double)((((r * rot) + pal_q) % 4 + 4) % 4) * (kPi * 0.5);
+    double psi = cph + 2.0 * kPi * fmod(y.fsc_gen * tau0, 1.0);
+    double ucyc = fmod(y.cu_per_sample * u, 1.0);
     s.cu = ca * cos(2.0 * kPi * ucyc + phi_r + kPi - psi);
     return s;
 }
@@ -175,7 +253,7 @@ double tau_of(const Model& m, size_t s) {
 }
 
 double gain_at(const Model& m, double tau) {
-    const double fl = kFieldLines * kH;
+    const double fl = m.y.field_lines * m.y.H;
     long k = (long)floor((tau - m.t_lead) / fl);
     double g = 1.0;
     for (long kk = k - 1; kk <= k; kk++) {
@@ -192,6 +270,8 @@ double gain_at(const Model& m, double tau) {
 Model build_model(const cuvhs_gen_params* p) {
     Model m;
     m.p = *p;
+    m.y = make_sys(p->system);
+    const double kH = m.y.H, kFieldLines = m.y.field_lines;
     m.fs = p->sample_rate_mhz * 1e6;
     m.t_lead = p->lead_lines * kH;
     double dur = (double)p->n_samples / m.fs;
@@ -283,11 +363,11 @@ void gen_block(const Model& m, size_t b, double phase0, uint8_t* out, double* dp
     double ph = phase0;
     Rng nr(mix(m.p.seed, 0xABCDEF00ull + b));
     for (size_t i = 0; i < n; i++) {
-        double f_inst = kIre0 + kHzIre * y[i + kWarm];
+        double f_inst = m.y.ire0 + m.y.hz_ire * y[i + kWarm];
         if (out) {
             double t = tau[i];
             double g = gain_at(m, t);
-            const double fl = kFieldLines * kH;
+            const double fl = m.y.field_lines * m.y.H;
             long k = (long)floor((t - m.t_lead) / fl);
             double snr = m.field(k).snr_db;
             double sigma = sqrt(m.lsb_carrier * m.lsb_carrier * 0.5 / pow(10.0, snr / 10.0));
@@ -307,8 +387,9 @@ void gen_block(const Model& m, size_t b, double phase0, uint8_t* out, double* dp
 
 extern "C" {
 
-void cuvhs_gen_defaults(cuvhs_gen_params* p, int config) {
+void cuvhs_gen_defaults_system(cuvhs_gen_params* p, int config, int system) {
     memset(p, 0, sizeof(*p));
+    p->system = system == 1 ? 1 : 0;
     p->sample_rate_mhz = 28.0;
     p->lead_lines = 20.0;
     p->snr_lo_db = 35.0;
@@ -319,7 +400,7 @@ void cuvhs_gen_defaults(cuvhs_gen_params* p, int config) {
     p->wow_f_lo_hz = 0.5;
     p->wow_f_hi_hz = 10.0;
     p->headswitch_group_max = 8;
-    double spf_lines = 263.0;
+    double spf_lines = p->system ? 313.0 : 263.0;
     int nfields = 4;
     if (config == 0) {
         p->seed = 0;
@@ -336,12 +417,14 @@ void cuvhs_gen_defaults(cuvhs_gen_params* p, int config) {
             nfields = 6000;
         }
     }
-    int spl = (int)round(p->sample_rate_mhz * 1e6 / kLineRate);
+    int spl = (int)round(p->sample_rate_mhz * 1e6 / make_sys(p->system).line_rate);
     p->n_samples = (size_t)nfields * (size_t)(spl * (int)spf_lines);
 }
 
+void cuvhs_gen_defaults(cuvhs_gen_params* p, int config) { cuvhs_gen_defaults_system(p, config, 0); }
+
 int cuvhs_gen(const cuvhs_gen_params* p, uint8_t* out) {
-    if (!p || !out || !(p->sample_rate_mhz > 0.0) || p->n_samples == 0) return -1;
+    if (!p || !out || !(p->sample_rate_mhz > 0.0) || p->n_samples == 0 || p->system < 0 || p->system > 1) return -1;
     Model m = build_model(p);
     size_t nb = (p->n_samples + kBlock - 1) / kBlock;
     std::vector<double> dph(nb, 0.0);

@@ -1,4 +1,4 @@
-// Synthetic VHS NTSC RF capture generator (host C ABI).
+// Synthetic VHS RF capture generator, NTSC or PAL (host C ABI).
 #pragma once
 #include <stddef.h>
 #include <stdint.h>
@@ -20,12 +20,16 @@ typedef struct cuvhs_gen_params {
     double wow_f_lo_hz, wow_f_hi_hz;
     double headswitch_max_us;   /* head-switch step ~ U(0, max) per field group; 0 disables */
     int headswitch_group_max;   /* group length ~ U{1..max} fields */
+    int system;                 /* 0 NTSC, 1 PAL */
 } cuvhs_gen_params;
 
 /* config: 0 = 4 fields @28 MHz, 35 dB, seed 0, exactly 4*spf samples;
  *         1 = 600 fields @28 MHz, SNR U(25,40), dropouts, wow, seed 1;
  *         2 = 6000 fields @40 MHz, config-1 model + head-switch steps, seed 2 */
-void cuvhs_gen_defaults(cuvhs_gen_params* p, int config);
+void cuvhs_gen_defaults(cuvhs_gen_params* p, int config);   /* NTSC */
+/* the same settings on a system's line structure and levels (0 NTSC, 1 PAL;
+ * n_samples counts that system's fields) */
+void cuvhs_gen_defaults_system(cuvhs_gen_params* p, int config, int system);
 int cuvhs_gen(const cuvhs_gen_params* p, uint8_t* out);
 
 #ifdef __cplusplus

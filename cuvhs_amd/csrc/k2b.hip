@@ -194,6 +194,18 @@ __device__ bool vblank_state_machine(const int* rs, const int* rl, PulseInfo pi,
     return done;
 }
 
+// Line spacings around the vblanks of a first and of a second field: entry
+// and exit of the field's own vblank, then of the next one
+// (pipeline.cu:707-717; NTSC and PAL differ).
+__device__ __forceinline__ void field_order_lengths(bool pal, double ffl[4], double sfl[4]) {
+    const double n_f[4] = {1, 0.5, 0.5, 1}, n_s[4] = {0.5, 1, 1, 0.5};
+    const double p_f[4] = {0.5, 0.5, 1, 1}, p_s[4] = {1, 1, 0.5, 0.5};
+    for (int i = 0; i < 4; i++) {
+        ffl[i] = pal ? p_f[i] : n_f[i];
+        sfl[i] = pal ? p_s[i] : n_s[i];
+    }
+}
+
 // first-HSYNC anchor given the field parity (pipeline.cu:747-845, without the
 // estimate-only branch which needs the previous field).  Called by all 64
 // lanes of the wave with the same arguments; every lane returns the result.
@@ -201,14 +213,14 @@ __device__ bool vblank_state_machine(const int* rs, const int* rl, PulseInfo pi,
 // lane computes the terms of its pulses, and the sum is taken in pulse order
 // from the lanes' values (the reference's sequential rounding).
 __device__ AnchorVariant anchor_variant(VP vp, int nvp, const int* vbp, double mll, bool first_field,
-                                        bool prev_nonpos, int num_eq, int fl0, int fl1) {
+                                        bool prev_nonpos, int num_eq, int fl0, int fl1, bool pal) {
     AnchorVariant r;
     r.count = 0;
     r.loc = 0.0;
     r.loc_rounded = 0.0;
     const double TOL = 0.5;
-    double ffl[4] = {1, 0.5, 0.5, 1};
-    double sfl[4] = {0.5, 1, 1, 0.5};
+    double ffl[4], sfl[4];
+    field_order_lengths(pal, ffl, sfl);
     double* cfl = first_field ? ffl : sfl;
     double vsl = num_eq / 2.0;
     double cur_lines = first_field ? fl0 : fl1;
@@ -304,7 +316,7 @@ __device__ AnchorVariant anchor_variant(VP vp, int nvp, const int* vbp, double m
 __global__ __launch_bounds__(64) void k_k2b_field(const ChunkCtl* ctl, int nf_override, const int* k2_starts,
                                                   const int* k2_lens, const int* k2_count, double tol_half_us,
                                                   double hsync_width, double eq_width, double vsync_width,
-                                                  int spl, int num_eq, int fl0, int fl1, int* g_vp_start,
+                                                  int spl, int num_eq, int fl0, int fl1, int pal, int* g_vp_start,
                                                   int* g_vp_len, unsigned char* g_vp_tg, K2bField* summ) {
     __shared__ int rs[kMaxPulses], rl[kMaxPulses];
     __shared__ int vs[kVpCap], vlen[kVpCap];
@@ -539,7 +551,8 @@ __global__ __launch_bounds__(64) void k_k2b_field(const ChunkCtl* ctl, int nf_ov
         int vbp[8];
         for (int k = 0; k < 4; k++) fol[k] = s_fo[k];
         for (int k = 0; k < 8; k++) vbp[k] = s_vb[k];
-        double ffl[4] = {1, 0.5, 0.5, 1}, sfl[4] = {0.5, 1, 1, 0.5};
+        double ffl[4], sfl[4];
+        field_order_lengths(pal != 0, ffl, sfl);
         double ic = 0.0, id = 0.0;
         for (int i = 0; i < 4; i++) {
             if (fol[i] == ffl[i]) { ic += 1.0; id += 1.0; }
@@ -561,7 +574,7 @@ __global__ __launch_bounds__(64) void k_k2b_field(const ChunkCtl* ctl, int nf_ov
     for (int var = 0; var < 4; var++) {
         const bool ff = (var & 1) != 0;
         const bool pn = (var & 2) != 0;
-        const AnchorVariant av = anchor_variant(vp, n, s_vbp, s_mll, ff, pn, num_eq, fl0, fl1);
+        const AnchorVariant av = anchor_variant(vp, n, s_vbp, s_mll, ff, pn, num_eq, fl0, fl1, pal != 0);
         if (threadIdx.x == 0) S.var[ff ? 1 : 0][pn ? 1 : 0] = av;
     }
     for (int i = threadIdx.x; i < n; i += 64) {
@@ -574,7 +587,7 @@ __global__ __launch_bounds__(64) void k_k2b_field(const ChunkCtl* ctl, int nf_ov
 // Cross-field cadence (pipeline.cu:1717-1772, :729-745, :811-827), one thread.
 __global__ void k_k2b_chain(ChunkCtl* ctl, DecodeState* dst, int nf_override, const unsigned long long* offsets,
                             const K2bField* summ, const int* g_vp_start, const unsigned char* g_vp_tg,
-                            K2bAnchorOut* anchors, int* is_first, int num_eq, int fl0, int fl1) {
+                            K2bAnchorOut* anchors, int* is_first, int num_eq, int fl0, int fl1, int pal) {
     // the field summaries and offsets are staged in LDS by the whole
     // workgroup, so the sequential loop below waits on LDS, not on L2
     constexpr int kStage = 64;
@@ -625,7 +638,8 @@ __global__ void k_k2b_chain(ChunkCtl* ctl, DecodeState* dst, int nf_override, co
                 if (ffc >= conf && ffc > sfc) first_field = true;
                 else if (sfc >= conf && sfc > ffc) first_field = false;
             }
-            double ffl[4] = {1, 0.5, 0.5, 1}, sfl[4] = {0.5, 1, 1, 0.5};
+            double ffl[4], sfl[4];
+            field_order_lengths(pal != 0, ffl, sfl);
             double* cfl = first_field ? ffl : sfl;
             double vsl = num_eq / 2.0;
             hsl = 0.0 + cfl[0] + vsl + vsl + vsl - 0.5 + cfl[1];
@@ -637,7 +651,10 @@ __global__ void k_k2b_chain(ChunkCtl* ctl, DecodeState* dst, int nf_override, co
                 hsl = v3 + cfl[1];
             }
             double cur_lines = first_field ? fl0 : fl1;
-            double prev_lines = first_field ? fl1 : fl0;
+            // the estimate steps one field back from the previous anchor:
+            // the previous field's lines in NTSC, this field's in PAL
+            // (pipeline.cu:811)
+            double prev_lines = pal ? cur_lines : (first_field ? fl1 : fl0);
             bool prev_nonpos = st.prev_first_hsync_loc <= 0.0;
             const AnchorVariant& V = S.var[first_field ? 1 : 0][prev_nonpos ? 1 : 0];
             int est_loc = (int)llround((prev_offset_lines + prev_lines + st.prev_first_hsync_loc / mll) * mll);
@@ -701,7 +718,7 @@ __global__ __launch_bounds__(256) void k_k2b_linelocs(const ChunkCtl* ctl, int n
                                                       const K2bAnchorOut* anchors, const int* g_vp_start,
                                                       const int* g_vp_len, const unsigned char* g_vp_tg,
                                                       int* out_starts, int* out_lens, int* out_count,
-                                                      int* out_types, double* linelocs) {
+                                                      int* out_types, double* linelocs, int lpf) {
     __shared__ int key[kVpCap];
     const int f = blockIdx.x;
     const int nf = nf_override > 0 ? nf_override : ctl->nf;
@@ -725,12 +742,12 @@ __global__ __launch_bounds__(256) void k_k2b_linelocs(const ChunkCtl* ctl, int n
         }
     }
     if (threadIdx.x == 0) out_count[f] = after;
-    double* ll = linelocs + (size_t)f * kLinesPerFrame;
+    double* ll = linelocs + (size_t)f * lpf;
     const double mll = S.meanlinelen;
     if (!A.valid) {
         double ref = (double)S.first_k2_start;
         if (S.npulses_k2 <= 0) ref = 0.0;
-        for (int line = threadIdx.x; line < kLinesPerFrame; line += 256) ll[line] = ref + mll * (double)line;
+        for (int line = threadIdx.x; line < lpf; line += 256) ll[line] = ref + mll * (double)line;
         return;
     }
     // bitonic sort of the starts (pad with INT_MAX), skipped when the list
@@ -767,12 +784,12 @@ __global__ __launch_bounds__(256) void k_k2b_linelocs(const ChunkCtl* ctl, int n
     // the end of the run of keys equal to key[cur]; the kept index is that
     // end clamped to n-2, if within tolerance.  Wave 0 finds g for every line
     // by binary search (lane j: lines j + 64 k), then carries the cursor
-    // through the 525 lines reading those results from the lanes.
-    __shared__ int s_pick[kLinesPerFrame];
+    // through the frame's lines reading those results from the lanes.
+    __shared__ int s_pick[kMaxLinesPerFrame];
     const double max_allowed = mll / 1.5;
     if (threadIdx.x < 64) {
         const int lane = threadIdx.x;
-        constexpr int K = (kLinesPerFrame + 63) / 64;
+        constexpr int K = (kMaxLinesPerFrame + 63) / 64;
         int g[K];
         int ok[K];
 #pragma unroll
@@ -780,7 +797,7 @@ __global__ __launch_bounds__(256) void k_k2b_linelocs(const ChunkCtl* ctl, int n
             const int line = lane + 64 * k;
             g[k] = -1;
             ok[k] = 0;
-            if (line < kLinesPerFrame && n >= 2) {
+            if (line < lpf && n >= 2) {
                 const double l = A.first_hsync_loc + mll * ((double)line - A.first_hsync_line);
                 int lo = 0, hi = n;  // first index with key >= l
                 while (lo < hi) {
@@ -803,7 +820,7 @@ __global__ __launch_bounds__(256) void k_k2b_linelocs(const ChunkCtl* ctl, int n
         for (int k = 0; k < K; k++) {
             for (int j = 0; j < 64; j++) {
                 const int line = 64 * k + j;
-                if (line >= kLinesPerFrame) break;
+                if (line >= lpf) break;
                 const int gl = __builtin_amdgcn_readlane(g[k], j);
                 int pick = -1;
                 if (cur <= n - 2) {
@@ -823,7 +840,7 @@ __global__ __launch_bounds__(256) void k_k2b_linelocs(const ChunkCtl* ctl, int n
         }
     }
     __syncthreads();
-    for (int line = threadIdx.x; line < kLinesPerFrame; line += 256) {
+    for (int line = threadIdx.x; line < lpf; line += 256) {
         const int pk = s_pick[line];
         ll[line] = pk >= 0 ? (double)key[pk] : A.first_hsync_loc + mll * ((double)line - A.first_hsync_line);
     }
@@ -836,8 +853,8 @@ hipError_t run_k2b_fields(const ChunkCtl* ctl_dev, int nf_override, int max_fiel
     double tol = fmt.sample_rate * 0.5e-6;
     hipLaunchKernelGGL(k_k2b_field, dim3(max_fields), dim3(64), 0, st, ctl_dev, nf_override, k2_starts, k2_lens,
                        k2_count, tol, fmt.hsync_width, fmt.eq_pulse_width, fmt.vsync_width, fmt.samples_per_line,
-                       fmt.num_eq_pulses, fmt.field_lines_first, fmt.field_lines_second, vp_start, vp_len, vp_tg,
-                       summ);
+                       fmt.num_eq_pulses, fmt.field_lines_first, fmt.field_lines_second, fmt.system == kSystemPal,
+                       vp_start, vp_len, vp_tg, summ);
     return hipGetLastError();
 }
 
@@ -847,9 +864,11 @@ hipError_t run_k2b_cadence(const ChunkCtl* ctl_dev, ChunkCtl* ctl_mut, DecodeSta
                            int* out_starts, int* out_lens, int* out_count, int* out_types, double* linelocs,
                            int* is_first, hipStream_t st) {
     hipLaunchKernelGGL(k_k2b_chain, dim3(1), dim3(64), 0, st, ctl_mut, dst, nf_override, offsets, summ, vp_start, vp_tg,
-                       anchors, is_first, fmt.num_eq_pulses, fmt.field_lines_first, fmt.field_lines_second);
+                       anchors, is_first, fmt.num_eq_pulses, fmt.field_lines_first, fmt.field_lines_second,
+                       fmt.system == kSystemPal);
     hipLaunchKernelGGL(k_k2b_linelocs, dim3(max_fields), dim3(256), 0, st, ctl_dev, nf_override, summ, anchors,
-                       vp_start, vp_len, vp_tg, out_starts, out_lens, out_count, out_types, linelocs);
+                       vp_start, vp_len, vp_tg, out_starts, out_lens, out_count, out_types, linelocs,
+                       fmt.lines_per_frame);
     return hipGetLastError();
 }
 

@@ -19,6 +19,9 @@ namespace cuvhs {
 struct K3Params {
     double tol, hsync_nom, eq_nom, vsync_nom, inlinelen, spf;
     double hsync_min, hsync_max, eq_min, eq_max, vsync_min, vsync_max;  // nominal classes
+    int lpf, field_lines, line_len;     // line locations per frame, output lines and samples
+    int ntsc;                           // field parity rule (line_locs.cu:219-235)
+    int avs, ave;                       // active output columns (tbc_resample.cu:156-157)
 };
 
 K3Params k3_params(const Fmt& f) {
@@ -35,6 +38,12 @@ K3Params k3_params(const Fmt& f) {
     p.eq_max = f.eq_pulse_width + p.tol;
     p.vsync_min = f.vsync_width * 0.5;
     p.vsync_max = f.vsync_width + 1.0e-6 * f.sample_rate;
+    p.lpf = f.lines_per_frame;
+    p.field_lines = f.field_lines;
+    p.line_len = f.line_len;
+    p.ntsc = f.system == kSystemNtsc;
+    p.avs = f.active_video_start;
+    p.ave = f.active_video_end;
     return p;
 }
 
@@ -57,7 +66,7 @@ __global__ void k_k3_classify(const int* lens, const int* count, const ChunkCtl*
                                                      p.vsync_max, p.hsync_min, p.hsync_max, p.eq_min, p.eq_max);
 }
 
-// Phases 2 and 3 (k_compute_linelocs, line_locs.cu:60-476), NTSC.
+// Phases 2 and 3 (k_compute_linelocs, line_locs.cu:60-476).
 __global__ __launch_bounds__(64) void k_k3_field(const int* starts_g, const int* lens_g, const int* count,
                                                  const ChunkCtl* ctl, int nf_override, K3Params p, double* linelocs,
                                                  int* is_first) {
@@ -133,15 +142,16 @@ __global__ __launch_bounds__(64) void k_k3_field(const int* starts_g, const int*
             }
         }
     }
-    // Step 1b: field parity (:198-229)
+    // Step 1b: field parity (:198-235): a long entry or exit spacing is a
+    // first field in NTSC, a second field in PAL
     int field_parity = -1;
     if (ref_pulse_idx >= 0 && last_hsync_idx >= 0 && first_eq1_idx >= 0) {
         double entry_lines = (double)(st[first_eq1_idx] - st[last_hsync_idx]) / inlinelen;
-        field_parity = (entry_lines > 0.75) ? 1 : 0;
+        field_parity = ((entry_lines > 0.75) == (p.ntsc != 0)) ? 1 : 0;
     }
     if (field_parity < 0 && ref_pulse_idx >= 0 && last_eq2_idx >= 0) {
         double exit_lines = (double)(st[ref_pulse_idx] - st[last_eq2_idx]) / inlinelen;
-        field_parity = (exit_lines > 0.75) ? 1 : 0;
+        field_parity = ((exit_lines > 0.75) == (p.ntsc != 0)) ? 1 : 0;
     }
     is_first[f] = field_parity;
     // Step 2: mean line length from the longest HSYNC run (:231-277)
@@ -222,10 +232,10 @@ __global__ __launch_bounds__(64) void k_k3_field(const int* starts_g, const int*
         cluster_start = cluster_end;
     }
     // Step 4c: greedy nearest-marker snapping (:437-476)
-    double* locs = linelocs + (size_t)f * kLinesPerFrame;
+    double* locs = linelocs + (size_t)f * p.lpf;
     const double max_allowed_distance = meanlinelen / 1.5;
     int cur_hsync = 0;
-    for (int line = 0; line < kLinesPerFrame; line++) {
+    for (int line = 0; line < p.lpf; line++) {
         double expected = ref_position + meanlinelen * (line - ref_line);
         double loc = expected;
         if (cur_hsync < num_hsyncs) {
@@ -260,18 +270,18 @@ __global__ __launch_bounds__(64) void k_k3_field(const int* starts_g, const int*
 
 // K3 spacing counters (k_line_locs_debug_analyze, line_locs.cu:585-626):
 // diag[0] bad spacing, diag[1] isolated spacing, diag[2] large jump.
-__global__ void k_diag_spacing(const double* ll, ChunkCtl* ctl, int als, double nominal) {
+__global__ void k_diag_spacing(const double* ll, ChunkCtl* ctl, int als, double nominal, int lpf) {
     const int nf = ctl->nf;
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= nf * (kLinesPerFrame - 1)) return;
-    const int field = idx / (kLinesPerFrame - 1);
-    const int line = idx % (kLinesPerFrame - 1);
-    if (line < als || line + 1 >= kLinesPerFrame) return;
-    const double* l = ll + (size_t)field * kLinesPerFrame;
+    if (idx >= nf * (lpf - 1)) return;
+    const int field = idx / (lpf - 1);
+    const int line = idx % (lpf - 1);
+    if (line < als || line + 1 >= lpf) return;
+    const double* l = ll + (size_t)field * lpf;
     const double spacing = l[line + 1] - l[line];
     if (spacing < 0.9 * nominal || spacing > 1.1 * nominal) atomicAdd(&ctl->diag[0], 1);
     if (fabs(spacing - nominal) > 0.25 * nominal) atomicAdd(&ctl->diag[2], 1);
-    if (line > als && line + 2 < kLinesPerFrame) {
+    if (line > als && line + 2 < lpf) {
         const double prev_spacing = l[line] - l[line - 1];
         const double next_spacing = l[line + 2] - l[line + 1];
         const double neighbor_avg = 0.5 * (prev_spacing + next_spacing);
@@ -282,18 +292,18 @@ __global__ void k_diag_spacing(const double* ll, ChunkCtl* ctl, int als, double 
 // K4 counters (k_hsync_refine_debug_analyze, hsync_refine.cu:273-316):
 // diag[3] large delta, diag[4] isolated jump, diag[5] refined sync-like.
 __global__ void k_diag_k4(const double* d05, const double* before, const double* after, ChunkCtl* ctl, int spfp,
-                          int als, double thr, double large, double isolated) {
+                          int als, double thr, double large, double isolated, int lpf) {
     const int nf = ctl->nf;
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= nf * kLinesPerFrame) return;
-    const int field = idx / kLinesPerFrame;
-    const int line = idx % kLinesPerFrame;
-    if (line < als || line + 1 >= kLinesPerFrame) return;
-    const int base = field * kLinesPerFrame;
+    if (idx >= nf * lpf) return;
+    const int field = idx / lpf;
+    const int line = idx % lpf;
+    if (line < als || line + 1 >= lpf) return;
+    const int base = field * lpf;
     const double after_loc = after[base + line];
     const double delta = after_loc - before[base + line];
     if (fabs(delta) > large) atomicAdd(&ctl->diag[3], 1);
-    if (line > als && line + 1 < kLinesPerFrame) {
+    if (line > als && line + 1 < lpf) {
         const double prev_delta = after[base + line - 1] - before[base + line - 1];
         const double next_delta = after[base + line + 1] - before[base + line + 1];
         const double neighbor_avg = 0.5 * (prev_delta + next_delta);
@@ -307,22 +317,22 @@ __global__ void k_diag_k4(const double* d05, const double* before, const double*
 // of bounds, diag[7] rows of suspicious length, diag[8] active-picture pixels
 // at or below the sync threshold.  One wavefront count per atomic.
 __global__ __launch_bounds__(256) void k_diag_k5(const double* demod, const double* ll, ChunkCtl* ctl, int spfp,
-                                                 double spl, double thr) {
+                                                 double spl, double thr, K3Params p) {
     const int nf = ctl->nf;
-    const int ppf = kFieldLines * kLineLen;
+    const int ppf = p.field_lines * p.line_len;
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
     bool oob = false, geom = false, sync_like = false;
     if (idx < nf * ppf) {
         const int total = nf * spfp;
         const int field = idx / ppf;
         const int rem = idx - field * ppf;
-        const int row = rem / kLineLen;
-        const int col = rem - row * kLineLen;
-        const double ls = ll[field * kLinesPerFrame + row];
-        const double le = ll[field * kLinesPerFrame + row + 1];
+        const int row = rem / p.line_len;
+        const int col = rem - row * p.line_len;
+        const double ls = ll[field * p.lpf + row];
+        const double le = ll[field * p.lpf + row + 1];
         const double len = le - ls;
         geom = col == 0 && (len <= 0.0 || len < 0.8 * spl || len > 1.2 * spl);
-        const double frac = (double)col / (double)kLineLen;
+        const double frac = (double)col / (double)p.line_len;
         const double coord = ls + frac * len;
         const int ci = (int)coord;
         const double x = coord - (double)ci;
@@ -334,7 +344,7 @@ __global__ __launch_bounds__(256) void k_diag_k5(const double* demod, const doub
             const double b = 2.0 * p0 - 5.0 * p1 + 4.0 * p2 - p3;
             const double c = 3.0 * (p1 - p2) + p3 - p0;
             const double hz = p1 + 0.5 * x * (a + x * (b + x * c));
-            sync_like = col >= 134 && col < 894 && hz <= thr;
+            sync_like = col >= p.avs && col < p.ave && hz <= thr;
         }
     }
     const int c_oob = __popcll(__ballot(oob)), c_geom = __popcll(__ballot(geom)), c_sync = __popcll(__ballot(sync_like));
@@ -359,26 +369,26 @@ hipError_t run_k3(const ChunkCtl* ctl, int nf_override, int max_fields, const Fm
 }
 
 hipError_t run_diag_spacing(const double* ll, ChunkCtl* ctl, int max_fields, const Fmt& fmt, hipStream_t st) {
-    const int n = max_fields * (kLinesPerFrame - 1);
+    const int n = max_fields * (fmt.lines_per_frame - 1);
     hipLaunchKernelGGL(k_diag_spacing, dim3((n + 255) / 256), dim3(256), 0, st, ll, ctl, fmt.active_line_start,
-                       (double)fmt.samples_per_line);
+                       (double)fmt.samples_per_line, fmt.lines_per_frame);
     return hipGetLastError();
 }
 
 hipError_t run_diag_k4(const double* d05, const double* before, const double* after, ChunkCtl* ctl, int max_fields,
                        const Fmt& fmt, hipStream_t st) {
-    const int n = max_fields * kLinesPerFrame;
+    const int n = max_fields * fmt.lines_per_frame;
     const double us = 1.0e-6 * fmt.sample_rate;
     hipLaunchKernelGGL(k_diag_k4, dim3((n + 255) / 256), dim3(256), 0, st, d05, before, after, ctl, fmt.spfp,
-                       fmt.active_line_start, fmt.pulse_threshold_hz, 1.5 * us, 1.0 * us);
+                       fmt.active_line_start, fmt.pulse_threshold_hz, 1.5 * us, 1.0 * us, fmt.lines_per_frame);
     return hipGetLastError();
 }
 
 hipError_t run_diag_k5(const double* demod, const double* ll, ChunkCtl* ctl, int max_fields, const Fmt& fmt,
                        hipStream_t st) {
-    const long long n = (long long)max_fields * kFieldLines * kLineLen;
+    const long long n = (long long)max_fields * fmt.field_lines * fmt.line_len;
     hipLaunchKernelGGL(k_diag_k5, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, demod, ll, ctl, fmt.spfp,
-                       (double)fmt.samples_per_line, fmt.pulse_threshold_hz);
+                       (double)fmt.samples_per_line, fmt.pulse_threshold_hz, k3_params(fmt));
     return hipGetLastError();
 }
 

@@ -84,14 +84,14 @@ template <typename S>
 __global__ __launch_bounds__(256) void k_hsync_refine(const S* __restrict__ d05, double* __restrict__ ll,
                                                       const ChunkCtl* ctl, int nf_override, int total_per_field,
                                                       double thr, double one_usec_samples, int als, double hw,
-                                                      double right_edge_offset) {
+                                                      double right_edge_offset, int lpf) {
     __shared__ __align__(16) double s_win[4][64];
     const int nf = nf_override > 0 ? nf_override : ctl->nf;
     const int lane = threadIdx.x & 63;
     double* wl = s_win[threadIdx.x >> 6];
     const int idx = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (idx >= nf * kLinesPerFrame) return;
-    const int line = idx % kLinesPerFrame;
+    if (idx >= nf * lpf) return;
+    const int line = idx % lpf;
     if (line < als) return;
     const int total = nf * total_per_field;
     const double original = ll[idx];
@@ -150,22 +150,23 @@ __global__ __launch_bounds__(256) void k_tbc_resample(const S* __restrict__ demo
                                                       uint16_t* __restrict__ luma, const ChunkCtl* ctl,
                                                       int nf_override, int total_per_field, double ire0,
                                                       double hz_ire, double vsync_ire, double out_zero,
-                                                      double out_scale, const ChunkPlan* plan) {
+                                                      double out_scale, const ChunkPlan* plan, int lpf,
+                                                      int field_lines, int line_len) {
     const int nf = nf_override > 0 ? nf_override : ctl->nf;
     if (plan) luma = plan->luma;   // a replayed graph: the chunk's place in the outputs
-    const int ppf = kFieldLines * kLineLen;
+    const int ppf = field_lines * line_len;
     int idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= nf * ppf) return;
     const int total = nf * total_per_field;
     int field = idx / ppf;
     int rem = idx - field * ppf;
-    int row = rem / kLineLen;
-    int col = rem - row * kLineLen;
-    // row + 1 < 525 always holds for 263 rows
-    double ls = ll[field * kLinesPerFrame + row];
-    double le = ll[field * kLinesPerFrame + row + 1];
+    int row = rem / line_len;
+    int col = rem - row * line_len;
+    // row + 1 < lines per frame always holds (263 of 525 rows, 313 of 625)
+    double ls = ll[field * lpf + row];
+    double le = ll[field * lpf + row + 1];
     double len = le - ls;
-    double frac = (double)col / (double)kLineLen;
+    double frac = (double)col / (double)line_len;
     double coord = ls + frac * len;
     int ci = (int)coord;
     double x = coord - (double)ci;
@@ -188,32 +189,32 @@ __global__ __launch_bounds__(256) void k_tbc_resample(const S* __restrict__ demo
 
 hipError_t run_hsync_refine(const void* d05, int f32, double* ll, const ChunkCtl* ctl, int nf_override,
                             int max_fields, const Fmt& fmt, hipStream_t st) {
-    int n = max_fields * kLinesPerFrame;
+    int n = max_fields * fmt.lines_per_frame;
     double ous = 1.0e-6 * fmt.sample_rate;
     double reo = 2.25 * ((fmt.sample_rate / 1.0e6) / 40.0);
     if (f32)
         hipLaunchKernelGGL(k_hsync_refine<float>, dim3((n + 3) / 4), dim3(256), 0, st, static_cast<const float*>(d05),
                            ll, ctl, nf_override, fmt.spfp, fmt.pulse_threshold_hz, ous, fmt.active_line_start,
-                           fmt.hsync_width, reo);
+                           fmt.hsync_width, reo, fmt.lines_per_frame);
     else
         hipLaunchKernelGGL(k_hsync_refine<double>, dim3((n + 3) / 4), dim3(256), 0, st,
                            static_cast<const double*>(d05), ll, ctl, nf_override, fmt.spfp, fmt.pulse_threshold_hz,
-                           ous, fmt.active_line_start, fmt.hsync_width, reo);
+                           ous, fmt.active_line_start, fmt.hsync_width, reo, fmt.lines_per_frame);
     return hipGetLastError();
 }
 
 hipError_t run_tbc_resample(const void* demod, int f32, const double* ll, uint16_t* luma, const ChunkCtl* ctl,
                             int nf_override, int max_fields, const Fmt& fmt, hipStream_t st, const ChunkPlan* plan) {
-    long long n = (long long)max_fields * kFieldLines * kLineLen;
+    long long n = (long long)max_fields * fmt.field_lines * fmt.line_len;
     const dim3 grid((unsigned)((n + 255) / 256));
     if (f32)
         hipLaunchKernelGGL(k_tbc_resample<float>, grid, dim3(256), 0, st, static_cast<const float*>(demod), ll, luma,
                            ctl, nf_override, fmt.spfp, fmt.ire0, fmt.hz_ire, fmt.vsync_ire, fmt.output_zero,
-                           fmt.output_scale, plan);
+                           fmt.output_scale, plan, fmt.lines_per_frame, fmt.field_lines, fmt.line_len);
     else
         hipLaunchKernelGGL(k_tbc_resample<double>, grid, dim3(256), 0, st, static_cast<const double*>(demod), ll,
                            luma, ctl, nf_override, fmt.spfp, fmt.ire0, fmt.hz_ire, fmt.vsync_ire, fmt.output_zero,
-                           fmt.output_scale, plan);
+                           fmt.output_scale, plan, fmt.lines_per_frame, fmt.field_lines, fmt.line_len);
     return hipGetLastError();
 }
 

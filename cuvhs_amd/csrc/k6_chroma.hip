@@ -5,6 +5,8 @@
 // 1024-point FFT band-pass -> inverse FFT -> burst metric up to three
 // times, copying one int to the host per pass, and detects the first
 // track and the NTSC phase from field 0 with two more host round trips.
+// PAL runs the same logic on PAL's constants (1135-sample lines, 313 lines
+// per field, a 2048-point line transform, 4.43 MHz subcarrier).
 //
 // Here the decision logic needs only per-field burst metrics for both track
 // hypotheses.  For even output lines the heterodyne phase is the same for
@@ -24,16 +26,18 @@
 
 namespace cuvhs {
 
-constexpr int kBurstLen = 36;      // 112 - 76 at 4 fsc
 constexpr int kSkip = 16;          // chroma_decode.cu:255
-constexpr int kPairs = (kFieldLines - 2 * kSkip - 1) / 2;  // 115
+// line pairs of the burst metric: (field lines - 2 kSkip - 1) / 2, 115 (NTSC)
+// or 140 (PAL)
+constexpr int kMaxPairs = (kMaxFieldLines - 2 * kSkip - 1) / 2;
+__device__ __forceinline__ int metric_pairs(int field_lines) { return (field_lines - 2 * kSkip - 1) / 2; }
 
 // Resampled sample of one line (chroma_decode.cu:75-137), before the
 // heterodyne.
 __device__ __forceinline__ double chroma_resample(const ChromaLaunch& a, double ls, double le, int col,
                                                  long long total) {
-    if (col >= kLineLen) return 0.0;
-    double frac = (double)col / (double)kLineLen;
+    if (col >= a.line_len) return 0.0;
+    double frac = (double)col / (double)a.line_len;
     double coord = ls + frac * (le - ls);
     int ci = (int)coord;
     double x = coord - (double)ci;
@@ -45,15 +49,15 @@ __device__ __forceinline__ double chroma_resample(const ChromaLaunch& a, double 
     return p1 + 0.5 * x * (aa + x * (bb + x * cc));
 }
 
-// Start of the heterodyne -cos(2 pi het_scale (line * 910 + col) + phase) of
-// chroma_decode.cu:143-148 at column col = t, the first of the columns
-// t + 64 r, r = 0..15, that thread t owns; het_step moves (s, c) on to the
+// Start of the heterodyne -cos(2 pi het_scale (line * L + col) + phase) of
+// chroma_decode.cu:143-148 (L the output line length) at column col = t,
+// the first of the columns t + 64 r that thread t owns; het_step moves (s, c) on to the
 // next of them by the fixed rotation exp(i 2 pi het_scale 64) (het_dc,
 // het_ds).  The rotation differs from a direct cos by ~1e-14 relative; the
 // reference's own FP64 argument, up to 4.4e5 rad, is rounded at ~5e-11 rad.
 __device__ __forceinline__ void het_start(const ChromaLaunch& a, int line, double phase_offset, double* s,
                                           double* c) {
-    sincos(2.0 * M_PI * a.het_scale * (double)(line * kLineLen + (int)threadIdx.x) + phase_offset, s, c);
+    sincos(2.0 * M_PI * a.het_scale * (double)(line * a.line_len + (int)threadIdx.x) + phase_offset, s, c);
 }
 __device__ __forceinline__ void het_step(const ChromaLaunch& a, double* s, double* c) {
     const double cn = *c * a.het_dc - *s * a.het_ds;
@@ -62,24 +66,31 @@ __device__ __forceinline__ void het_step(const ChromaLaunch& a, double* s, doubl
 }
 
 // One wavefront per pair of lines (la, la + 1): resample + heterodyne with
-// track 0 (chroma_decode.cu:75-157), 1024-point FFT, band-pass (:163-182),
-// inverse FFT; the burst window (FP64) and the line (FP32, ~1e-3 LSB of the
-// u16 output) are stored for the metric and output kernels.  The two real
-// lines go in as the real and imaginary parts of one complex transform; the
+// track 0 (chroma_decode.cu:75-157), NF-point FFT (NF = 1024 for NTSC's
+// 910-sample lines, 2048 for PAL's 1135), band-pass (:163-182), inverse
+// FFT; the burst window (FP64) and the line (FP32, ~1e-3 LSB of the u16
+// output) are stored for the metric and output kernels.  The two real lines
+// go in as the real and imaginary parts of one complex transform; the
 // band-pass keeps bins by |frequency|, so the band-passed line la comes back
 // as the real part and la + 1 as the imaginary part.  (The reference
 // transforms each line on its own; the results agree to rounding.)  Thread t
-// computes columns t + 64 r, the register layout filter1024_wave takes and
-// returns, so the lines never pass through LDS outside the transform.
-// T: the transform's type.  The resampling coordinate and the heterodyne
-// stay FP64 in both (positions reach 46 M samples, the heterodyne argument
-// 4.4e5 rad); T = float is the reduced-precision forms' line filter, with
-// whole complex elements in LDS (the same 8.7 KB as the FP64 filter's real
-// and imaginary parts in turn).
-template <typename T>
-__device__ __forceinline__ void chroma_line(const ChromaLaunch& a, void* buf, const typename C2<T>::type* tw) {
+// computes columns t + 64 r, r < NF / 64, the register layout
+// filter1024_wave takes and returns, so the lines never pass through LDS
+// outside the transform.  NF = 2048 splits into even and odd bins first
+// (x[n] +- x[n + 1024], the odd half times W2048^n: a radix-2 step inside
+// each lane), filters each half as a 1024-point block with the bins it
+// holds, and joins them the same way.  T: the transform's type.  The
+// resampling coordinate and the heterodyne stay FP64 in both (positions
+// reach 46 M samples, the heterodyne argument 4.4e5 rad); T = float is the
+// reduced-precision forms' line filter, with whole complex elements in LDS
+// (the same 8.7 KB as the FP64 filter's real and imaginary parts in turn).
+template <typename T, int NF>
+__device__ __forceinline__ void chroma_line(const ChromaLaunch& a, void* buf, const typename C2<T>::type* tw,
+                                            const typename C2<T>::type* tw2) {
     using V = typename C2<T>::type;
     constexpr bool whole = sizeof(T) == 4;
+    constexpr int R = NF / 64;
+    static_assert(NF == 1024 || NF == 2048, "line transform size");
     const int f = blockIdx.y;
     const int nf = a.nf_override > 0 ? a.nf_override : a.ctl->nf;
     if (f >= nf) return;
@@ -88,15 +99,15 @@ __device__ __forceinline__ void chroma_line(const ChromaLaunch& a, void* buf, co
     const int lb = la + 1;
     const bool has_b = lb < a.line_end;
     const long long total = (long long)nf * a.spfp;
-    const double* ll = a.linelocs + (size_t)f * kLinesPerFrame;
+    const double* ll = a.linelocs + (size_t)f * a.lpf;
     const int rot = 3;  // track 0
-    V v[16];
+    V v[R];
     {
         const double ls = ll[la], le = ll[la + 1];
         double s, c;
         het_start(a, la, (double)((la * rot + 0) & 3) * (M_PI * 0.5), &s, &c);
 #pragma unroll
-        for (int r = 0; r < 16; r++) {
+        for (int r = 0; r < R; r++) {
             v[r].x = (T)(chroma_resample(a, ls, le, (int)threadIdx.x + 64 * r, total) * -c);
             het_step(a, &s, &c);
         }
@@ -106,50 +117,78 @@ __device__ __forceinline__ void chroma_line(const ChromaLaunch& a, void* buf, co
         double s, c;
         het_start(a, lb, (double)((lb * rot + 0) & 3) * (M_PI * 0.5), &s, &c);
 #pragma unroll
-        for (int r = 0; r < 16; r++) {
+        for (int r = 0; r < R; r++) {
             v[r].y = (T)(chroma_resample(a, ls, le, (int)threadIdx.x + 64 * r, total) * -c);
             het_step(a, &s, &c);
         }
     } else {
 #pragma unroll
-        for (int r = 0; r < 16; r++) v[r].y = (T)0;
+        for (int r = 0; r < R; r++) v[r].y = (T)0;
     }
     // band-pass: bins with lo <= |k| <= hi kept (chroma_decode.cu:163-182)
     const int lo = a.lo, hi = a.hi;
-    filter1024_wave<T, whole>(buf, v, tw, [&](V z, int k) {
-        const int kk = k <= kChromaFFT / 2 ? k : kChromaFFT - k;
+    auto keep = [&](V z, int k) {
+        const int kk = k <= NF / 2 ? k : NF - k;
         if (kk < lo || kk > hi) {
             z.x = (T)0;
             z.y = (T)0;
         }
         return z;
-    });
-    const T scale = (T)a.fft_scale;
-    double* da = a.burst + ((size_t)f * kFieldLines + la) * kBurstLen;
-    float* fa = a.filt + ((size_t)f * kFieldLines + la) * kLineLen;
+    };
+    if constexpr (NF == 1024) {
+        filter1024_wave<T, whole>(buf, v, tw, keep);
+    } else {
+        const int l = (int)threadIdx.x & 63;
 #pragma unroll
-    for (int r = 0; r < 16; r++) {
+        for (int r = 0; r < 16; r++) {
+            const V x0 = v[r], x1 = v[r + 16];
+            v[r] = cadd(x0, x1);
+            v[r + 16] = cmul(csub(x0, x1), tw2[64 * r + l]);   // W2048^n, n = 64 r + l
+        }
+        filter1024_wave<T, whole>(buf, v, tw, [&](V z, int k) { return keep(z, 2 * k); });
+        filter1024_wave<T, whole>(buf, v + 16, tw, [&](V z, int k) { return keep(z, 2 * k + 1); });
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            const V e = v[r], o = cmul(v[r + 16], cconj(tw2[64 * r + l]));
+            v[r] = cadd(e, o);
+            v[r + 16] = csub(e, o);
+        }
+    }
+    const T scale = (T)a.fft_scale;
+    double* da = a.burst + ((size_t)f * a.field_lines + la) * a.burst_len;
+    float* fa = a.filt + ((size_t)f * a.field_lines + la) * a.line_len;
+#pragma unroll
+    for (int r = 0; r < R; r++) {
         const int col = (int)threadIdx.x + 64 * r;
         const T xa = v[r].x * scale, xb = v[r].y * scale;
-        if (col < kLineLen) {
+        if (col < a.line_len) {
             fa[col] = (float)xa;
-            if (has_b) fa[kLineLen + col] = (float)xb;
+            if (has_b) fa[a.line_len + col] = (float)xb;
         }
         const int bi = col - a.burst_start;
-        if (bi >= 0 && bi < kBurstLen) {
+        if (bi >= 0 && bi < a.burst_len) {
             da[bi] = (double)xa;
-            if (has_b) da[kBurstLen + bi] = (double)xb;
+            if (has_b) da[a.burst_len + bi] = (double)xb;
         }
     }
 }
 
 __global__ __launch_bounds__(64, 3) void k_chroma_line(ChromaLaunch a) {
-    __shared__ __align__(16) double buf[lds_elems(kChromaFFT)];
-    chroma_line<double>(a, buf, a.tw);
+    __shared__ __align__(16) double buf[lds_elems(1024)];
+    chroma_line<double, 1024>(a, buf, a.tw, nullptr);
 }
 __global__ __launch_bounds__(64, 4) void k_chroma_line_f32(ChromaLaunch a) {
-    __shared__ __align__(16) float2 buf[lds_elems(kChromaFFT)];
-    chroma_line<float>(a, buf, a.tw_f);
+    __shared__ __align__(16) float2 buf[lds_elems(1024)];
+    chroma_line<float, 1024>(a, buf, a.tw_f, nullptr);
+}
+// PAL: 2048-point line transforms (twice the registers per lane)
+__global__ __launch_bounds__(64) void k_chroma_line2048(ChromaLaunch a) {
+    __shared__ __align__(16) double buf[lds_elems(1024)];
+    chroma_line<double, 2048>(a, buf, a.tw, a.tw2);
+}
+__global__ __launch_bounds__(64) void k_chroma_line2048_f32(ChromaLaunch a) {
+    __shared__ __align__(16) float2 buf[lds_elems(1024)];
+    chroma_line<float, 2048>(a, buf, a.tw_f, a.tw2_f);
 }
 
 // ACC + u16 of every line (chroma_decode.cu:188-231) with the resolved track:
@@ -159,25 +198,26 @@ __global__ __launch_bounds__(256) void k_chroma_out(ChromaLaunch a) {
     const int f = blockIdx.y, line = blockIdx.x;
     const int nf = a.nf_override > 0 ? a.nf_override : a.ctl->nf;
     if (f >= nf) return;
-    const size_t row = (size_t)f * kFieldLines + line;
+    const size_t row = (size_t)f * a.field_lines + line;
     // burst RMS (chroma_decode.cu:205-221, whose atomic sum has no fixed
     // order; here, as in the oracle, sample order): every wavefront loads the
-    // 36 burst samples at once and sums their squares through lane reads, so
-    // no lane waits on a workgroup barrier
+    // 36 (NTSC) or 40 (PAL) burst samples at once and sums their squares
+    // through lane reads, so no lane waits on a workgroup barrier
     const int lane = (int)threadIdx.x & 63;
-    const double* da = a.burst + row * kBurstLen;
-    const double bv = lane < kBurstLen ? da[lane] : 0.0;
+    const int bl = a.burst_len;
+    const double* da = a.burst + row * bl;
+    const double bv = lane < bl ? da[lane] : 0.0;
     double ssq = 0.0;
-    for (int i = 0; i < kBurstLen; i++) {
+    for (int i = 0; i < bl; i++) {
         const double d = lane_value(bv, i);
         ssq += d * d;
     }
-    const double rms = sqrt(ssq / (double)kBurstLen);
+    const double rms = sqrt(ssq / (double)bl);
     const double s_acc = (rms > 1.0) ? (a.burst_abs_ref / rms) : 0.0;
     const double acc = ((line & 1) && a.track[f]) ? -s_acc : s_acc;
-    const float* fa = a.filt + row * kLineLen;
-    uint16_t* o = (a.plan ? a.plan->chroma : a.out) + row * kLineLen;
-    for (int col = threadIdx.x; col < kLineLen; col += 256) {
+    const float* fa = a.filt + row * a.line_len;
+    uint16_t* o = (a.plan ? a.plan->chroma : a.out) + row * a.line_len;
+    for (int col = threadIdx.x; col < a.line_len; col += 256) {
         double ov = (double)fa[col] * acc + 32768.0;
         if (ov < 0.0) ov = 0.0;
         if (ov > 65535.0) ov = 65535.0;
@@ -189,34 +229,36 @@ __global__ __launch_bounds__(256) void k_chroma_out(ChromaLaunch a) {
 // per-line burst I/Q.  One wavefront per field: lanes evaluate pairs, lane 0
 // accumulates them in pair order (chroma_decode.cu:383-411, :415-450).
 __global__ __launch_bounds__(64) void k_chroma_metrics(const double* burst, const ChunkCtl* ctl, int nf_override,
-                                                       double* m0, double* m1, const double* cs_tab, double* iq0) {
-    __shared__ double s0[128], s1[128];
+                                                       double* m0, double* m1, const double* cs_tab, double* iq0,
+                                                       int field_lines, int bl) {
+    __shared__ double s0[kMaxPairs], s1[kMaxPairs];
     const int nf = nf_override > 0 ? nf_override : ctl->nf;
     const int f = blockIdx.x;
     if (f >= nf) return;
-    for (int p = threadIdx.x; p < kPairs; p += 64) {
-        const double* ra = burst + ((size_t)f * kFieldLines + kSkip + 2 * p) * kBurstLen;
-        const double* rb = ra + kBurstLen;
+    const int pairs = metric_pairs(field_lines);
+    for (int p = threadIdx.x; p < pairs; p += 64) {
+        const double* ra = burst + ((size_t)f * field_lines + kSkip + 2 * p) * bl;
+        const double* rb = ra + bl;
         double a0 = 0.0, a1 = 0.0;
-        for (int i = 0; i < kBurstLen; i++) {
+        for (int i = 0; i < bl; i++) {
             a0 += fabs(ra[i] + rb[i]);
             a1 += fabs(ra[i] + (-rb[i]));
         }
-        s0[p] = a0 / kBurstLen;
-        s1[p] = a1 / kBurstLen;
+        s0[p] = a0 / bl;
+        s1[p] = a1 / bl;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
         double t0 = 0.0, t1 = 0.0;
-        for (int p = 0; p < kPairs; p++) { t0 += s0[p]; t1 += s1[p]; }
-        m0[f] = t0 / kPairs;
-        m1[f] = t1 / kPairs;
+        for (int p = 0; p < pairs; p++) { t0 += s0[p]; t1 += s1[p]; }
+        m0[f] = t0 / pairs;
+        m1[f] = t1 / pairs;
     }
     if (f == 0 && iq0) {
-        for (int line = kSkip + threadIdx.x; line < kFieldLines - kSkip; line += 64) {
-            const double* r = burst + (size_t)line * kBurstLen;
+        for (int line = kSkip + threadIdx.x; line < field_lines - kSkip; line += 64) {
+            const double* r = burst + (size_t)line * bl;
             double I = 0.0, Q = 0.0;
-            for (int i = 0; i < kBurstLen; i++) {
+            for (int i = 0; i < bl; i++) {
                 I += r[i] * cs_tab[2 * i];
                 Q += r[i] * cs_tab[2 * i + 1];
             }
@@ -243,10 +285,10 @@ __device__ int ntsc_phase_id_first(int quadrant) {
 // dependent reads and rewrites wait on LDS rather than on L2.
 constexpr int kResolveLds = 1024;
 __device__ void resolve_tracks(DecodeState* dst, int nf, const double* m0, const double* m1, const double* iq0,
-                               int* out_track, int* phase_ids);
+                               int* out_track, int* phase_ids, int field_lines);
 __global__ void k_chroma_resolve(const ChunkCtl* ctl, DecodeState* dst, int nf_override, const double* g_m0,
                                  const double* g_m1, const double* iq0, int* g_track, int* phase_ids,
-                                 const ChunkPlan* plan) {
+                                 const ChunkPlan* plan, int field_lines) {
     __shared__ double s_m0[kResolveLds], s_m1[kResolveLds];
     if (plan) phase_ids = plan->phase_id;   // a replayed graph: the chunk's place in the outputs
     __shared__ int s_track[kResolveLds];
@@ -264,14 +306,14 @@ __global__ void k_chroma_resolve(const ChunkCtl* ctl, DecodeState* dst, int nf_o
     const double* m0 = staged ? s_m0 : g_m0;
     const double* m1 = staged ? s_m1 : g_m1;
     int* out_track = staged ? s_track : g_track;
-    if (threadIdx.x == 0) resolve_tracks(dst, nf, m0, m1, iq0, out_track, phase_ids);
+    if (threadIdx.x == 0) resolve_tracks(dst, nf, m0, m1, iq0, out_track, phase_ids, field_lines);
     __syncthreads();
     if (staged)
         for (int i = threadIdx.x; i < nf; i += blockDim.x) g_track[i] = s_track[i];
 }
 
 __device__ void resolve_tracks(DecodeState* dst, int nf, const double* m0, const double* m1, const double* iq0,
-                               int* out_track, int* phase_ids) {
+                               int* out_track, int* phase_ids, int field_lines) {
     ChromaState cs = dst->chroma;
     const int cycle[8] = {3, 2, 1, 4, 3, 2, 1, 4};
     int det, cycle_start;
@@ -285,7 +327,7 @@ __device__ void resolve_tracks(DecodeState* dst, int nf, const double* m0, const
         det = (metric[1] < metric[0]) ? 1 : 0;
         thr = metric[det] * 4.0;
         double It = 0.0, Qt = 0.0;
-        for (int line = kSkip; line < kFieldLines - kSkip; line++) {
+        for (int line = kSkip; line < field_lines - kSkip; line++) {
             double I = iq0[2 * line], Q = iq0[2 * line + 1];
             if (det == 1 && (line & 1)) { I = -I; Q = -Q; }
             double mag = sqrt(I * I + Q * Q);
@@ -306,10 +348,10 @@ __device__ void resolve_tracks(DecodeState* dst, int nf, const double* m0, const
     }
     // out_track doubles as h_track; the tracks of the last computed pass of
     // each sub-chunk are what the output uses.
-    int fpc = (4096 / kFieldLines) * kFieldLines;
-    int tl = nf * kFieldLines;
+    int fpc = (4096 / field_lines) * field_lines;
+    int tl = nf * field_lines;
     int cl = tl < fpc ? tl : fpc;
-    fpc = cl / kFieldLines;
+    fpc = cl / field_lines;
     for (int fs = 0; fs < nf; fs += fpc) {
         int fthis = min(fpc, nf - fs);
         int used[16];
@@ -346,20 +388,28 @@ __device__ void resolve_tracks(DecodeState* dst, int nf, const double* m0, const
 hipError_t run_chroma(ChromaLaunch a, DecodeState* dst, int max_fields, double* m0, double* m1,
                       const double* cs_tab, double* iq0, int* out_track, int* phase_ids, int need_iq,
                       hipStream_t st) {
-    // band-pass of every line with track 0; metrics from lines 16..246
+    // band-pass of every line with track 0; metrics from lines 16 .. field
+    // lines - 17
+    if (a.nfft != 1024 && a.nfft != 2048) return hipErrorInvalidValue;
+    if (a.burst_len > kMaxBurstLen || a.field_lines > kMaxFieldLines) return hipErrorInvalidValue;
     ChromaLaunch m = a;
     m.line0 = 0;
-    m.line_end = kFieldLines;
+    m.line_end = a.field_lines;
     const dim3 lg((m.line_end - m.line0 + 1) / 2, max_fields);
-    if (a.f32) hipLaunchKernelGGL(k_chroma_line_f32, lg, dim3(64), 0, st, m);
-    else hipLaunchKernelGGL(k_chroma_line, lg, dim3(64), 0, st, m);
+    if (a.nfft == 2048) {
+        if (a.f32) hipLaunchKernelGGL(k_chroma_line2048_f32, lg, dim3(64), 0, st, m);
+        else hipLaunchKernelGGL(k_chroma_line2048, lg, dim3(64), 0, st, m);
+    } else {
+        if (a.f32) hipLaunchKernelGGL(k_chroma_line_f32, lg, dim3(64), 0, st, m);
+        else hipLaunchKernelGGL(k_chroma_line, lg, dim3(64), 0, st, m);
+    }
     hipLaunchKernelGGL(k_chroma_metrics, dim3(max_fields), dim3(64), 0, st, a.burst, a.ctl,
-                       a.nf_override, m0, m1, cs_tab, need_iq ? iq0 : nullptr);
+                       a.nf_override, m0, m1, cs_tab, need_iq ? iq0 : nullptr, a.field_lines, a.burst_len);
     hipLaunchKernelGGL(k_chroma_resolve, dim3(1), dim3(64), 0, st, a.ctl, dst, a.nf_override, m0, m1, iq0, out_track,
-                       phase_ids, a.plan);
+                       phase_ids, a.plan, a.field_lines);
     ChromaLaunch o = a;
     o.track = out_track;
-    hipLaunchKernelGGL(k_chroma_out, dim3(kFieldLines, max_fields), dim3(256), 0, st, o);
+    hipLaunchKernelGGL(k_chroma_out, dim3(a.field_lines, max_fields), dim3(256), 0, st, o);
     return hipGetLastError();
 }
 

@@ -22,7 +22,7 @@ namespace cuvhs {
 
 constexpr int kDoThreads = 256;
 constexpr int kEnvBlock = 16;            // dropout_detect.cu:11
-constexpr int kMaxClassBlocks = 45056;   // >= spfp / 16 for captures up to 40 MHz
+constexpr int kMaxClassBlocks = 53504;   // >= spfp / 16 for captures up to 40 MHz (PAL: 53,280)
 constexpr int kMaxEvents = 2048;
 
 __device__ __forceinline__ int find_tbc_line(const double* ll, int als, int n_lines, double rf) {
@@ -48,12 +48,13 @@ struct DoWindow {
 };
 
 // scan window of dropout_detect.cu:106-114 (field_start uses the padded stride)
-__device__ __forceinline__ DoWindow do_window(const double* linelocs, int f, int spfp, int als) {
-    const double* fll = linelocs + (size_t)f * kLinesPerFrame;
+__device__ __forceinline__ DoWindow do_window(const double* linelocs, int f, int spfp, int als, int lpf,
+                                              int field_lines) {
+    const double* fll = linelocs + (size_t)f * lpf;
     const int field_start = f * spfp;
     DoWindow w;
     w.start = max((int)fll[als], field_start);
-    w.end = min((int)fll[als + kFieldLines], field_start + spfp);
+    w.end = min((int)fll[als + field_lines], field_start + spfp);
     return w;
 }
 
@@ -64,12 +65,12 @@ __device__ __forceinline__ DoWindow do_window(const double* linelocs, int f, int
 template <typename S>
 __global__ __launch_bounds__(kDoThreads) void k_do_sum(const S* __restrict__ env, const double* __restrict__ linelocs,
                                                         const ChunkCtl* ctl, int nf_override, int spfp, int als,
-                                                        double* partial) {
+                                                        int lpf, int field_lines, double* partial) {
     __shared__ double red[kDoThreads];
     const int f = blockIdx.x, sl = blockIdx.y, t = threadIdx.x;
     const int nf = nf_override > 0 ? nf_override : ctl->nf;
     if (f >= nf) return;
-    DoWindow w = do_window(linelocs, f, spfp, als);
+    DoWindow w = do_window(linelocs, f, spfp, als, lpf, field_lines);
     if (w.end <= w.start) return;
     const int L = (spfp + kSumSlices - 1) / kSumSlices;
     const int a = sl * L, b = min(spfp, a + L);
@@ -95,11 +96,12 @@ __device__ __forceinline__ double field_mean(const double* partial, int f, int s
 template <typename S>
 __global__ __launch_bounds__(kDoThreads) void k_do_class(const S* __restrict__ env, const double* __restrict__ linelocs,
                                                           const ChunkCtl* ctl, int nf_override, int spfp, int als,
-                                                          const double* partial, unsigned char* cls) {
+                                                          int lpf, int field_lines, const double* partial,
+                                                          unsigned char* cls) {
     const int f = blockIdx.x, sl = blockIdx.y;
     const int nf = nf_override > 0 ? nf_override : ctl->nf;
     if (f >= nf) return;
-    DoWindow w = do_window(linelocs, f, spfp, als);
+    DoWindow w = do_window(linelocs, f, spfp, als, lpf, field_lines);
     if (w.end <= w.start) return;
     __shared__ double s_mean;
     if (threadIdx.x == 0) s_mean = field_mean(partial, f, spfp);
@@ -131,9 +133,9 @@ __global__ __launch_bounds__(kDoThreads) void k_do_class(const S* __restrict__ e
 __global__ __launch_bounds__(kDoThreads) void k_dropouts(const unsigned char* __restrict__ gcls,
                                                           const double* __restrict__ linelocs,
                                                           const ChunkCtl* ctl, int nf_override, int spfp, int als,
-                                                          uint16_t* luma, uint16_t* chroma, int* do_lines,
-                                                          int* do_starts, int* do_ends, int* do_count,
-                                                          const ChunkPlan* plan) {
+                                                          int lpf, int field_lines, int line_len, uint16_t* luma,
+                                                          uint16_t* chroma, int* do_lines, int* do_starts,
+                                                          int* do_ends, int* do_count, const ChunkPlan* plan) {
     __shared__ unsigned char cls[kMaxClassBlocks];
     if (plan) {   // a replayed graph: the chunk's place in the output arrays
         luma = plan->luma;
@@ -153,10 +155,10 @@ __global__ __launch_bounds__(kDoThreads) void k_dropouts(const unsigned char* __
     const int nf = nf_override > 0 ? nf_override : ctl->nf;
     if (f >= nf) return;
     const int t = threadIdx.x;
-    const double* fll = linelocs + (size_t)f * kLinesPerFrame;
+    const double* fll = linelocs + (size_t)f * lpf;
     const int field_start = f * spfp;
     const int scan_start = max((int)fll[als], field_start);
-    const int scan_end = min((int)fll[als + kFieldLines], field_start + spfp);
+    const int scan_end = min((int)fll[als + field_lines], field_start + spfp);
     if (t == 0) s_nent = 0;
     if (scan_end > scan_start) {
         const int nblk = (scan_end - scan_start + kEnvBlock - 1) / kEnvBlock;
@@ -239,19 +241,19 @@ __global__ __launch_bounds__(kDoThreads) void k_dropouts(const unsigned char* __
             int tc = 0;
             for (int i = 0; i < m && tc < kMaxDropouts; i++) {
                 double srf = (double)rs[i], erf = (double)re[i];
-                int sl = find_tbc_line(fll, als, kFieldLines, srf);
+                int sl = find_tbc_line(fll, als, field_lines, srf);
                 if (sl < 0) continue;
-                int el = find_tbc_line(fll, als, kFieldLines, erf - 1.0);
-                if (el < 0) el = kFieldLines - 1;
+                int el = find_tbc_line(fll, als, field_lines, erf - 1.0);
+                if (el < 0) el = field_lines - 1;
                 for (int l = sl; l <= el && tc < kMaxDropouts; l++) {
                     double lls = fll[als + l], lle = fll[als + l + 1];
                     double len = lle - lls;
                     if (len <= 0.0) continue;
                     int sx, ex;
-                    if (l == sl) { sx = (int)((srf - lls) / len * (double)kLineLen); sx = max(0, sx); }
+                    if (l == sl) { sx = (int)((srf - lls) / len * (double)line_len); sx = max(0, sx); }
                     else sx = 0;
-                    if (l == el) { ex = (int)((erf - lls) / len * (double)kLineLen + 0.999); ex = min(kLineLen, ex); }
-                    else ex = kLineLen;
+                    if (l == el) { ex = (int)((erf - lls) / len * (double)line_len + 0.999); ex = min(line_len, ex); }
+                    else ex = line_len;
                     if (ex > sx) {
                         s_ent[3 * tc] = l;
                         s_ent[3 * tc + 1] = sx;
@@ -278,16 +280,16 @@ __global__ __launch_bounds__(kDoThreads) void k_dropouts(const unsigned char* __
     // handled by one thread in program order.  Starting each entry at sx + t
     // instead would hand a column to different threads in entries with
     // different sx, a race with no barrier between the entries.
-    uint16_t* lu = luma + (size_t)f * kFieldLines * kLineLen;
-    uint16_t* ch = chroma + (size_t)f * kFieldLines * kLineLen;
+    uint16_t* lu = luma + (size_t)f * field_lines * line_len;
+    uint16_t* ch = chroma + (size_t)f * field_lines * line_len;
     for (int d = 0; d < ne; d++) {
         int line = s_ent[3 * d], sx = s_ent[3 * d + 1], ex = s_ent[3 * d + 2];
         int src = (line > 0) ? line - 1 : line + 1;
-        if (src >= kFieldLines) continue;
+        if (src >= field_lines) continue;
         for (int col = t; col < ex; col += kDoThreads) {
             if (col < sx) continue;
-            lu[line * kLineLen + col] = lu[src * kLineLen + col];
-            ch[line * kLineLen + col] = ch[src * kLineLen + col];
+            lu[line * line_len + col] = lu[src * line_len + col];
+            ch[line * line_len + col] = ch[src * line_len + col];
         }
     }
 }
@@ -326,15 +328,15 @@ hipError_t run_dropout_classes(const void* env, int f32, const double* ll, const
     if (f32) {
         const float* e = static_cast<const float*>(env);
         hipLaunchKernelGGL(k_do_sum<float>, gs, dim3(kDoThreads), 0, st, e, ll, ctl, nf_override, fmt.spfp,
-                           fmt.active_line_start, partial);
+                           fmt.active_line_start, fmt.lines_per_frame, fmt.field_lines, partial);
         hipLaunchKernelGGL(k_do_class<float>, gc, dim3(kDoThreads), 0, st, e, ll, ctl, nf_override, fmt.spfp,
-                           fmt.active_line_start, partial, cls);
+                           fmt.active_line_start, fmt.lines_per_frame, fmt.field_lines, partial, cls);
     } else {
         const double* e = static_cast<const double*>(env);
         hipLaunchKernelGGL(k_do_sum<double>, gs, dim3(kDoThreads), 0, st, e, ll, ctl, nf_override, fmt.spfp,
-                           fmt.active_line_start, partial);
+                           fmt.active_line_start, fmt.lines_per_frame, fmt.field_lines, partial);
         hipLaunchKernelGGL(k_do_class<double>, gc, dim3(kDoThreads), 0, st, e, ll, ctl, nf_override, fmt.spfp,
-                           fmt.active_line_start, partial, cls);
+                           fmt.active_line_start, fmt.lines_per_frame, fmt.field_lines, partial, cls);
     }
     return hipGetLastError();
 }
@@ -343,7 +345,8 @@ hipError_t run_dropout_scan(const unsigned char* cls, const double* ll, const Ch
                             int max_fields, const Fmt& fmt, uint16_t* luma, uint16_t* chroma, int* lines, int* starts,
                             int* ends, int* count, hipStream_t st, const ChunkPlan* plan) {
     hipLaunchKernelGGL(k_dropouts, dim3(max_fields), dim3(kDoThreads), 0, st, cls, ll, ctl, nf_override, fmt.spfp,
-                       fmt.active_line_start, luma, chroma, lines, starts, ends, count, plan);
+                       fmt.active_line_start, fmt.lines_per_frame, fmt.field_lines, fmt.line_len, luma, chroma, lines,
+                       starts, ends, count, plan);
     return hipGetLastError();
 }
 

@@ -140,13 +140,18 @@ struct ChromaLaunch {
     double het_scale, fft_scale, burst_abs_ref;
     double het_dc, het_ds;      // cos / sin of 2 pi het_scale 64 (heterodyne step of 64 columns)
     int burst_start, lo, hi;
+    int burst_len;              // burst window samples (36 NTSC, 40 PAL)
+    int lpf, field_lines, line_len;   // line locations per frame; output lines, samples
+    int nfft;                   // line transform points (1024 NTSC, 2048 PAL)
     const int* track;           // resolved track per field (output kernel)
     int line0, line_end;        // lines [line0, line_end) of each field
     double* burst;              // burst window of every line, track 0, scaled
-    float* filt;                // band-passed lines, track 0, scaled: [field][263][910]
+    float* filt;                // band-passed lines, track 0, scaled: [field][lines][line_len]
     uint16_t* out;
-    const double2* tw;
+    const double2* tw;          // W1024^m, m < 1024
     const float2* tw_f;         // the same twiddles in FP32 (f32)
+    const double2* tw2;         // W2048^m, m < 2048 (nfft 2048)
+    const float2* tw2_f;
     const double2* mask;        // band-pass bins: (1, 0) kept, (0, 0) removed
     int f32;                    // line filter in FP32 (the reduced-precision forms)
     const ChunkPlan* plan;      // if set: out (and the phase IDs) from the plan

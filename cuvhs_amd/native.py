@@ -11,9 +11,12 @@ LIB_DIR = os.path.join(HERE, "lib")
 
 MAX_PULSES = 800
 MAX_DROPOUTS = 512
-LINES_PER_FRAME = 525
+LINES_PER_FRAME = 525      # NTSC geometry; a decoder's own is Decoder.geometry
 FIELD_LINES = 263
 LINE_LEN = 910
+SYSTEMS = {"ntsc": 0, "pal": 1}
+# (line locations per frame, output lines per field, samples per output line)
+GEOMETRY = {"ntsc": (525, 263, 910), "pal": (625, 313, 1135)}
 
 E_ARG, E_HIP, E_ALLOC, E_UNSUPPORTED, E_CAPACITY = -1, -2, -3, -4, -5
 INPUT_U8, INPUT_S16, INPUT_U16 = 0, 1, 2
@@ -44,7 +47,8 @@ class Format(ctypes.Structure):
     _fields_ = [("sample_rate", _d), ("samples_per_line", _i), ("samples_per_field", _i), ("spfp", _i),
                 ("pulse_threshold_hz", _d), ("ire0", _d), ("hz_ire", _d), ("output_scale", _d),
                 ("output_zero", _d), ("output_rate", _d), ("black16b", _d), ("white16b", _d),
-                ("burst_start", _i), ("burst_end", _i)]
+                ("burst_start", _i), ("burst_end", _i), ("system", _i), ("lines_per_frame", _i),
+                ("field_lines", _i), ("line_len", _i), ("fsc", _d)]
 
 
 class ChunkInfo(ctypes.Structure):
@@ -61,7 +65,7 @@ class GenParams(ctypes.Structure):
     _fields_ = [("sample_rate_mhz", _d), ("n_samples", ctypes.c_size_t), ("seed", _u64), ("lead_lines", _d),
                 ("snr_lo_db", _d), ("snr_hi_db", _d), ("chroma_db", _d), ("dropouts_per_field", _d),
                 ("dropout_len_lo_us", _d), ("dropout_len_hi_us", _d), ("wow_max_us", _d), ("wow_f_lo_hz", _d),
-                ("wow_f_hi_hz", _d), ("headswitch_max_us", _d), ("headswitch_group_max", _i)]
+                ("wow_f_hi_hz", _d), ("headswitch_max_us", _d), ("headswitch_group_max", _i), ("system", _i)]
 
 
 _SIGS = {
@@ -69,6 +73,7 @@ _SIGS = {
     "cuvhs_destroy": (_i, [_vp]),
     "cuvhs_get_format": (_i, [_vp, ctypes.POINTER(Format)]),
     "cuvhs_format_for": (_i, [_d, ctypes.POINTER(Format)]),
+    "cuvhs_format_for_system": (_i, [_i, _d, ctypes.POINTER(Format)]),
     "cuvhs_reset": (_i, [_vp]),
     "cuvhs_decode_chunk": (_i, [_vp, _vp, _u64, _u64, _i, ctypes.POINTER(ChunkInfo)]),
     "cuvhs_chunk_outputs_get": (_i, [_vp, ctypes.POINTER(ChunkOutputs)]),
@@ -134,24 +139,29 @@ def gen_lib():
         G = ctypes.CDLL(path)
         G.cuvhs_gen_defaults.argtypes = [ctypes.POINTER(GenParams), _i]
         G.cuvhs_gen_defaults.restype = None
+        G.cuvhs_gen_defaults_system.argtypes = [ctypes.POINTER(GenParams), _i, _i]
+        G.cuvhs_gen_defaults_system.restype = None
         G.cuvhs_gen.argtypes = [ctypes.POINTER(GenParams), _vp]
         G.cuvhs_gen.restype = _i
         _gen = G
     return _gen
 
 
-def gen_params(config, **overrides):
+def gen_params(config, system="ntsc", **overrides):
+    """Generator settings of a benchmark configuration; system "pal" gives
+    the same signal model on PAL's line structure and levels (n_samples
+    then counts PAL fields)."""
     p = GenParams()
-    gen_lib().cuvhs_gen_defaults(ctypes.byref(p), int(config))
+    gen_lib().cuvhs_gen_defaults_system(ctypes.byref(p), int(config), SYSTEMS[system])
     for k, v in overrides.items():
         setattr(p, k, v)
     return p
 
 
-def generate(config=0, **overrides):
+def generate(config=0, system="ntsc", **overrides):
     """Synthetic u8 capture of one of the benchmark configurations (numpy array)."""
     import numpy as np
-    p = gen_params(config, **overrides)
+    p = gen_params(config, system, **overrides)
     out = np.empty(p.n_samples, dtype=np.uint8)
     rc = gen_lib().cuvhs_gen(ctypes.byref(p), out.ctypes.data)
     if rc != 0:
@@ -186,17 +196,17 @@ def next_7smooth(n):
         n += 1
 
 
-def k1_fft_size(mhz):
+def k1_fft_size(mhz, system="ntsc"):
     """The reference's K1 transform length: a padded field window plus the
     2048-sample overlap, rounded up to 7-smooth (fm_demod.cu:273-280)."""
-    return next_7smooth(int(format_for(mhz).spfp) + 2048)
+    return next_7smooth(int(format_for(mhz, system).spfp) + 2048)
 
 
-def format_for(mhz):
+def format_for(mhz, system="ntsc"):
     f = Format()
-    rc = lib().cuvhs_format_for(float(mhz), ctypes.byref(f))
+    rc = lib().cuvhs_format_for_system(SYSTEMS[system], float(mhz), ctypes.byref(f))
     if rc != 0:
-        raise CuvhsError(f"cuvhs_format_for({mhz}) -> {rc}")
+        raise CuvhsError(f"cuvhs_format_for_system({system}, {mhz}) -> {rc}")
     return f
 
 
@@ -216,14 +226,17 @@ class Decoder:
     """One decode context (Pipeline, pipeline.h:28-114) on a HIP device."""
 
     def __init__(self, sample_rate_mhz=28.0, batch_fields=64, device=0, input_format="u8", dc_correct=False,
-                 line_locator="k2b", precision="fp64", blocks=None, graphs=False):
+                 line_locator="k2b", precision="fp64", blocks=None, graphs=False, system="ntsc"):
         """blocks: optional (k1a, k1b, pre) block sizes, 0 for the tuned
         or modelled choice (cuvhs_config.block_*).  graphs: decode_capture
-        replays captured stage graphs (cuvhs_config.graphs)."""
+        replays captured stage graphs (cuvhs_config.graphs).  system: "ntsc"
+        or "pal" (cuvhs_config.system)."""
         self.input_format = INPUT_FORMATS[input_format]
         self.precision = precision
+        self.system = system
+        self.geometry = GEOMETRY[system]
         ba, bb, bp = blocks or (0, 0, 0)
-        cfg = Config(0, float(sample_rate_mhz), int(batch_fields), PRECISIONS[precision], None, int(device),
+        cfg = Config(SYSTEMS[system], float(sample_rate_mhz), int(batch_fields), PRECISIONS[precision], None, int(device),
                      self.input_format, 1 if dc_correct else 0, LINE_LOCATORS[line_locator], int(ba), int(bb),
                      int(bp), 1 if graphs else 0)
         h = _vp()
@@ -333,7 +346,8 @@ class Decoder:
             o = ChunkOutputs()
             check(lib().cuvhs_chunk_outputs_get(self.h, ctypes.byref(o)), self.h, "chunk_outputs_get")
             k = info.fields
-            shapes = {"luma": ((k, FIELD_LINES, LINE_LEN), torch.uint16), "chroma": ((k, FIELD_LINES, LINE_LEN), torch.uint16),
+            _, fl, lw = self.geometry
+            shapes = {"luma": ((k, fl, lw), torch.uint16), "chroma": ((k, fl, lw), torch.uint16),
                       "file_loc": ((k,), torch.uint64), "is_first": ((k,), torch.int32), "phase_id": ((k,), torch.int32),
                       "do_count": ((k,), torch.int32), "do_line": ((k, MAX_DROPOUTS), torch.int32),
                       "do_startx": ((k, MAX_DROPOUTS), torch.int32), "do_endx": ((k, MAX_DROPOUTS), torch.int32)}
@@ -377,7 +391,8 @@ class Decoder:
         if max_fields is None:
             max_fields = self.default_capacity(n)
         dev = d_capture.device
-        luma = torch.empty((max_fields, FIELD_LINES, LINE_LEN), dtype=torch.uint16, device=dev)
+        _, fl, lw = self.geometry
+        luma = torch.empty((max_fields, fl, lw), dtype=torch.uint16, device=dev)
         chroma = torch.empty_like(luma)
         file_loc = torch.empty(max_fields, dtype=torch.uint64, device=dev)
         is_first = torch.empty(max_fields, dtype=torch.int32, device=dev)

@@ -35,6 +35,11 @@ extern "C" {
 #define CUVHS_LINES_PER_FRAME 525     /* linelocs per field (video_format.cpp:26) */
 #define CUVHS_FIELD_LINES 263         /* TBC rows per field */
 #define CUVHS_LINE_LEN 910            /* TBC samples per row */
+/* The same for PAL (video_format.cpp:49-61); cuvhs_format reports the
+ * context's own. */
+#define CUVHS_PAL_LINES_PER_FRAME 625
+#define CUVHS_PAL_FIELD_LINES 313
+#define CUVHS_PAL_LINE_LEN 1135
 
 enum {
     CUVHS_OK = 0,
@@ -51,7 +56,13 @@ enum {
     CUVHS_W_EOF = 4                   /* pipeline.cu:1535: fewer than one field left */
 };
 
-enum { CUVHS_SYSTEM_NTSC = 0 };
+/* Video system (VideoSystem, video_format.h:4; --system, main.cpp:54).  PAL
+ * is the reference's PAL format: its NTSC pipeline and K1 filters with PAL's
+ * constants (video_format.cpp:48-87): 625 lines, 313 output lines of 1135
+ * samples per field, 5 EQ pulses, PAL IRE levels, 4.43 MHz subcarrier and
+ * 626,953 Hz colour-under, a 2048-point chroma line transform, and PAL's
+ * branches of K2b (pipeline.cu:707-717, 811) and K3 (line_locs.cu:219-235). */
+enum { CUVHS_SYSTEM_NTSC = 0, CUVHS_SYSTEM_PAL = 1 };
 /* Arithmetic of the decode (the reference computes in FP64 with cuFFT,
  * fm_demod.cu:396-486, chroma_decode.cu:568-635).
  * FP64: the reference's precision, the default.
@@ -80,7 +91,7 @@ enum { CUVHS_LINELOC_K2B = 0, CUVHS_LINELOC_K3 = 1 };
 typedef struct cuvhs_ctx cuvhs_ctx;
 
 typedef struct cuvhs_config {
-    int system;               /* CUVHS_SYSTEM_NTSC (PAL: CUVHS_E_UNSUPPORTED) */
+    int system;               /* CUVHS_SYSTEM_NTSC or CUVHS_SYSTEM_PAL */
     double sample_rate_mhz;   /* capture rate, e.g. 28.0 */
     int batch_fields;         /* F: fields per chunk; results depend on F */
     int precision;            /* CUVHS_PRECISION_* */
@@ -116,12 +127,17 @@ typedef struct cuvhs_config {
 /* Derived timing constants (VideoFormat, video_format.cpp:23-104). */
 typedef struct cuvhs_format {
     double sample_rate;
-    int samples_per_line;     /* round(fs / 15734.264) */
-    int samples_per_field;    /* samples_per_line * 263 */
+    int samples_per_line;     /* round(fs / line rate): 15734.264 Hz NTSC, 15625 Hz PAL */
+    int samples_per_field;    /* samples_per_line * 263 (PAL 313) */
     int spfp;                 /* samples_per_field + 20 lines (pipeline.cu:1339-1342) */
     double pulse_threshold_hz, ire0, hz_ire, output_scale, output_zero, output_rate;
     double black16b, white16b;
     int burst_start, burst_end;
+    int system;               /* CUVHS_SYSTEM_* */
+    int lines_per_frame;      /* line locations per field: 525 / 625 */
+    int field_lines;          /* TBC rows per field: 263 / 313 */
+    int line_len;             /* TBC samples per row: 910 / 1135 */
+    double fsc;               /* colour subcarrier (Hz) */
 } cuvhs_format;
 
 /* Create/destroy.  Replaces Pipeline::allocate_buffers + FMDemodState::init
@@ -129,7 +145,8 @@ typedef struct cuvhs_format {
 int cuvhs_create(const cuvhs_config* cfg, cuvhs_ctx** out);
 int cuvhs_destroy(cuvhs_ctx* ctx);
 int cuvhs_get_format(const cuvhs_ctx* ctx, cuvhs_format* out);
-int cuvhs_format_for(double sample_rate_mhz, cuvhs_format* out);
+int cuvhs_format_for(double sample_rate_mhz, cuvhs_format* out);   /* NTSC */
+int cuvhs_format_for_system(int system, double sample_rate_mhz, cuvhs_format* out);
 /* Clears the state carried across chunks (K2b cadence, ChromaState, field
  * parity: pipeline.h:93-97, chroma_decode.h:9-15, pipeline.cu:2575). */
 int cuvhs_reset(cuvhs_ctx* ctx);
@@ -148,7 +165,7 @@ typedef struct cuvhs_chunk_info {
 
 /* Device pointers to the last chunk's outputs, owned by the context and valid
  * until the next decode call.  Field i of the chunk:
- *   luma/chroma: [fields][263][910] u16
+ *   luma/chroma: [fields][263][910] u16 (PAL: [fields][313][1135])
  *   file_loc[i] (u64), is_first[i], phase_id[i], do_count[i],
  *   do_line/do_startx/do_endx: [fields][512] */
 typedef struct cuvhs_chunk_outputs {

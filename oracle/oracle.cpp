@@ -57,9 +57,10 @@ const int MAX_DO = OR_MAX_DROPOUTS;
 enum { PULSE_HSYNC = 0, PULSE_EQ1 = 1, PULSE_VSYNC = 2, PULSE_EQ2 = 3 };  // line_locs.h:6
 
 // ---------------------------------------------------------------------------
-// VideoFormat, NTSC branch (video_format.cpp:23-104)
+// VideoFormat (video_format.cpp:23-104), NTSC (system 0) or PAL (system 1)
 // ---------------------------------------------------------------------------
 struct Fmt {
+    int system;
     double sample_rate;
     int lines_per_frame, lines_per_field;
     double line_rate, fsc, chroma_under;
@@ -75,40 +76,72 @@ struct Fmt {
     int spfp;
 };
 
-Fmt make_fmt(double mhz) {
+Fmt make_fmt(double mhz, int system = 0) {
     Fmt f;
+    f.system = system;
     f.sample_rate = mhz * 1e6;
-    f.lines_per_frame = 525;
-    f.lines_per_field = 263;
-    f.line_rate = 15734.264;
-    f.fsc = 3579545.0;
-    f.chroma_under = (525.0 * (30.0 / 1.001)) * 40.0;
-    f.burst_abs_ref = 4416.0;
-    f.burst_start_us = 5.3;
-    f.burst_end_us = 7.8;
-    f.output_line_len = 910;
-    f.output_field_lines = 263;
-    f.num_eq_pulses = 6;
-    f.field_lines_first = 263;
-    f.field_lines_second = 262;
-    f.hz_ire = 1e6 / 140.0;
-    f.ire0 = 4.4e6 - (f.hz_ire * 100.0);
-    f.vsync_ire = -40.0;
+    if (system == 1) {
+        // PAL branch (video_format.cpp:48-70, :83-87)
+        f.lines_per_frame = 625;
+        f.lines_per_field = 313;
+        f.line_rate = 15625.0;
+        f.fsc = 4433618.75;
+        f.chroma_under = ((625.0 * 25.0) * 40.0) + 1953.0;
+        f.burst_abs_ref = 5000.0;
+        f.burst_start_us = 5.6;
+        f.burst_end_us = 7.85;
+        f.output_line_len = 1135;
+        f.output_field_lines = 313;
+        f.num_eq_pulses = 5;
+        f.field_lines_first = 312;
+        f.field_lines_second = 313;
+        f.vsync_ire = -0.3 * (100.0 / 0.7);
+        f.hz_ire = 1e6 / (100.0 + (-f.vsync_ire));
+        f.ire0 = 4.8e6 - (f.hz_ire * 100.0);
+    } else {
+        f.lines_per_frame = 525;
+        f.lines_per_field = 263;
+        f.line_rate = 15734.264;
+        f.fsc = 3579545.0;
+        f.chroma_under = (525.0 * (30.0 / 1.001)) * 40.0;
+        f.burst_abs_ref = 4416.0;
+        f.burst_start_us = 5.3;
+        f.burst_end_us = 7.8;
+        f.output_line_len = 910;
+        f.output_field_lines = 263;
+        f.num_eq_pulses = 6;
+        f.field_lines_first = 263;
+        f.field_lines_second = 262;
+        f.hz_ire = 1e6 / 140.0;
+        f.ire0 = 4.4e6 - (f.hz_ire * 100.0);
+        f.vsync_ire = -40.0;
+    }
     f.sync_tip_hz = f.ire0 + f.hz_ire * f.vsync_ire;
     f.pulse_threshold_hz = f.ire0 + f.hz_ire * (-20.0);
-    f.output_zero = 1024.0;
-    f.output_scale = (51200.0 - 1024.0) / (100.0 - f.vsync_ire);
-    f.active_line_start = 10;
+    if (system == 1) {
+        f.output_zero = 256.0;
+        f.output_scale = (54016.0 - 256.0) / (100.0 - f.vsync_ire);
+        f.active_line_start = 8;
+    } else {
+        f.output_zero = 1024.0;
+        f.output_scale = (51200.0 - 1024.0) / (100.0 - f.vsync_ire);
+        f.active_line_start = 10;
+    }
     f.output_rate = 4.0 * f.fsc;
     f.samples_per_line = static_cast<int>(round(f.sample_rate / f.line_rate));
     f.samples_per_field = f.samples_per_line * f.lines_per_field;
     f.hsync_width = 4.7e-6 * f.sample_rate;
     f.eq_pulse_width = 2.3e-6 * f.sample_rate;
     f.bp_width = 1.5e-6 * f.sample_rate;
-    f.vsync_width = 27.1e-6 * f.sample_rate;
+    f.vsync_width = (system == 1 ? 27.3e-6 : 27.1e-6) * f.sample_rate;
     f.spfp = f.samples_per_field + FIELD_MARGIN_LINES * f.samples_per_line;
     return f;
 }
+
+// active output columns of the K5 counters and the JSON (tbc_resample.cu:156-157,
+// tbc_writer.cpp:106-107)
+int active_video_start(const Fmt& f) { return f.system == 1 ? 185 : 134; }
+int active_video_end(const Fmt& f) { return f.system == 1 ? 1107 : 894; }
 
 // ---------------------------------------------------------------------------
 // K1 filter design helpers (fm_demod.cu:35-127)
@@ -607,9 +640,19 @@ HostK2bAnchor host_get_first_hsync_loc_exactish(const std::vector<HostValidPulse
         last_pulse = i;
     }
 
+    // pipeline.cu:707-717
+    const bool pal = fmt.system == 1;
     double first_field_lengths[4] = {1, 0.5, 0.5, 1};
     double second_field_lengths[4] = {0.5, 1, 1, 0.5};
     double progressive_field_lengths[4] = {1, 0.5, 1, 0.5};
+    if (pal) {
+        const double pf[4] = {0.5, 0.5, 1, 1}, ps[4] = {1, 1, 0.5, 0.5}, pp[4] = {0.5, 0.5, 0.5, 0.5};
+        for (int i = 0; i < 4; i++) {
+            first_field_lengths[i] = pf[i];
+            second_field_lengths[i] = ps[i];
+            progressive_field_lengths[i] = pp[i];
+        }
+    }
     double ic = 0.0, id = 0.0, pc = 0.0, pd = 0.0;
     for (int i = 0; i < 4; i++) {
         double fl = field_order_lengths[i];
@@ -694,7 +737,7 @@ HostK2bAnchor host_get_first_hsync_loc_exactish(const std::vector<HostValidPulse
         offset = lv_off;
     }
 
-    double est_field_lines = previous_field_lines;  // NTSC branch (pipeline.cu:811)
+    double est_field_lines = pal ? current_field_lines : previous_field_lines;  // pipeline.cu:811
     int estimated_hsync_loc = (int)std::llround(
         (last_field_offset_lines + est_field_lines + prev_first_hsync_loc / meanlinelen) * meanlinelen);
     bool used_estimated = false;
@@ -1478,15 +1521,18 @@ static void k3_field(const Fmt& fmt, const int* starts, const int* lengths, int 
             }
         }
     }
-    // Step 1b: field parity (:198-229), NTSC
+    // Step 1b: field parity (:198-235)
+    const bool ntsc = fmt.system != 1;
     int field_parity = -1;
     if (ref_pulse_idx >= 0 && last_hsync_idx >= 0 && first_eq1_idx >= 0) {
         double entry_lines = (double)(starts[first_eq1_idx] - starts[last_hsync_idx]) / inlinelen;
-        field_parity = (entry_lines > 0.75) ? 1 : 0;
+        if (ntsc) field_parity = (entry_lines > 0.75) ? 1 : 0;
+        else field_parity = (entry_lines > 0.75) ? 0 : 1;
     }
     if (field_parity < 0 && ref_pulse_idx >= 0 && last_eq2_idx >= 0) {
         double exit_lines = (double)(starts[ref_pulse_idx] - starts[last_eq2_idx]) / inlinelen;
-        field_parity = (exit_lines > 0.75) ? 1 : 0;
+        if (ntsc) field_parity = (exit_lines > 0.75) ? 1 : 0;
+        else field_parity = (exit_lines > 0.75) ? 0 : 1;
     }
     *is_first = field_parity;
     // Step 2: mean line length from the longest HSYNC run (:231-277)
@@ -1654,7 +1700,8 @@ static void k5_counts(const Fmt& fmt, const double* demod, int total, const doub
                 double p0 = demod[ci - 1], p1 = demod[ci], p2 = demod[ci + 1], p3 = demod[ci + 2];
                 double a = p2 - p0, bb = 2.0 * p0 - 5.0 * p1 + 4.0 * p2 - p3, c = 3.0 * (p1 - p2) + p3 - p0;
                 double hz = p1 + 0.5 * x * (a + x * (bb + x * c));
-                if (col >= 134 && col < 894 && hz <= fmt.pulse_threshold_hz) out[2]++;
+                if (col >= active_video_start(fmt) && col < active_video_end(fmt) && hz <= fmt.pulse_threshold_hz)
+                    out[2]++;
             }
         }
     }
@@ -1833,9 +1880,11 @@ static int write_json_meta(const char* path, const std::vector<or_dec::Meta>& me
 
 extern "C" {
 
-int or_format_init(double mhz, or_format* o) {
-    if (!o || !(mhz > 0.0)) return -1;
-    Fmt f = make_fmt(mhz);
+int or_format_init(double mhz, or_format* o) { return or_format_init_sys(0, mhz, o); }
+
+int or_format_init_sys(int system, double mhz, or_format* o) {
+    if (!o || !(mhz > 0.0) || system < 0 || system > 1) return -1;
+    Fmt f = make_fmt(mhz, system);
     o->sample_rate = f.sample_rate;
     o->samples_per_line = f.samples_per_line;
     o->samples_per_field = f.samples_per_field;
@@ -1925,10 +1974,12 @@ int or_tbc_resample(double mhz, const double* demod, int total, const double* li
     return 0;
 }
 
-or_dec* or_create(double mhz, int batch, int record_trace) {
-    if (!(mhz > 0.0) || batch < 1) return nullptr;
+or_dec* or_create(double mhz, int batch, int record_trace) { return or_create_sys(0, mhz, batch, record_trace); }
+
+or_dec* or_create_sys(int system, double mhz, int batch, int record_trace) {
+    if (!(mhz > 0.0) || batch < 1 || system < 0 || system > 1) return nullptr;
     or_dec* d = new or_dec();
-    d->fmt = make_fmt(mhz);
+    d->fmt = make_fmt(mhz, system);
     d->batch = batch;
     d->trace = record_trace != 0;
     d->chroma.keep_trial = d->trace;
@@ -1960,7 +2011,13 @@ int or_chunk_diagnostics(const or_dec* d, int c, int* out9) {
 
 int or_k3_linelocs(double mhz, const int* starts, const int* lens, const int* count, int nf, int* types,
                    double* linelocs, int* is_first) {
-    Fmt fmt = make_fmt(mhz);
+    return or_k3_linelocs_sys(0, mhz, starts, lens, count, nf, types, linelocs, is_first);
+}
+
+int or_k3_linelocs_sys(int system, double mhz, const int* starts, const int* lens, const int* count, int nf,
+                       int* types, double* linelocs, int* is_first) {
+    if (system < 0 || system > 1) return -1;
+    Fmt fmt = make_fmt(mhz, system);
     K3Thresholds t3 = k3_thresholds(fmt);
     for (int f = 0; f < nf; f++) {
         const int* st = starts + (size_t)f * MAX_PULSES;
@@ -2053,8 +2110,15 @@ int or_chroma_trial_lines(const or_dec* d, int which, double* out) {
 int or_k2b_chunk(double mhz, or_k2b_state* st, uint64_t raw_offset, const uint64_t* offsets, int nf, int* starts,
                  int* lens, int* count, int* types, unsigned char* good, double* linelocs, int* is_first,
                  double* anchor) {
-    if (!st || nf < 0 || !(mhz > 0.0)) return -1;
-    Fmt fmt = make_fmt(mhz);
+    return or_k2b_chunk_sys(0, mhz, st, raw_offset, offsets, nf, starts, lens, count, types, good, linelocs, is_first,
+                            anchor);
+}
+
+int or_k2b_chunk_sys(int system, double mhz, or_k2b_state* st, uint64_t raw_offset, const uint64_t* offsets, int nf,
+                     int* starts, int* lens, int* count, int* types, unsigned char* good, double* linelocs,
+                     int* is_first, double* anchor) {
+    if (!st || nf < 0 || !(mhz > 0.0) || system < 0 || system > 1) return -1;
+    Fmt fmt = make_fmt(mhz, system);
     K2bState k;
     k.prev_first_field = st->prev_first_field;
     k.prev_progressive_field = st->prev_progressive_field;
@@ -2074,13 +2138,17 @@ int or_k2b_chunk(double mhz, or_k2b_state* st, uint64_t raw_offset, const uint64
 
 int or_lookup_ntsc_phase(int is_first, int quadrant, int delta) { return lookup_ntsc_phase_id(is_first, quadrant, delta); }
 
-double or_burst_metric(double mhz, const double* lines) {
-    Fmt fmt = make_fmt(mhz);
+double or_burst_metric(double mhz, const double* lines) { return or_burst_metric_sys(0, mhz, lines); }
+
+double or_burst_phase(double mhz, const double* lines) { return or_burst_phase_sys(0, mhz, lines); }
+
+double or_burst_metric_sys(int system, double mhz, const double* lines) {
+    Fmt fmt = make_fmt(mhz, system == 1 ? 1 : 0);
     return burst_metric(lines, fmt, chroma_consts(fmt));
 }
 
-double or_burst_phase(double mhz, const double* lines) {
-    Fmt fmt = make_fmt(mhz);
+double or_burst_phase_sys(int system, double mhz, const double* lines) {
+    Fmt fmt = make_fmt(mhz, system == 1 ? 1 : 0);
     return burst_phase(lines, fmt, chroma_consts(fmt));
 }
 
@@ -2145,6 +2213,14 @@ int or_write_outputs(const or_dec* d, const char* out_base) {
 int or_write_json_meta(double mhz, const char* path, int n, const uint64_t* file_loc, const int* is_first,
                        const int* phase_id, const int* do_count, const int* do_line, const int* do_startx,
                        const int* do_endx) {
+    return or_write_json_meta_sys(0, mhz, path, n, file_loc, is_first, phase_id, do_count, do_line, do_startx,
+                                  do_endx);
+}
+
+int or_write_json_meta_sys(int system, double mhz, const char* path, int n, const uint64_t* file_loc,
+                           const int* is_first, const int* phase_id, const int* do_count, const int* do_line,
+                           const int* do_startx, const int* do_endx) {
+    if (system < 0 || system > 1) return -1;
     std::vector<or_dec::Meta> meta((size_t)n);
     int k = 0;
     for (int i = 0; i < n; i++) {
@@ -2157,7 +2233,7 @@ int or_write_json_meta(double mhz, const char* path, int n, const uint64_t* file
             meta[(size_t)i].de.push_back(do_endx[k]);
         }
     }
-    return write_json_meta(path, meta, make_fmt(mhz));
+    return write_json_meta(path, meta, make_fmt(mhz, system));
 }
 
 int or_decode_file(const char* in_path, const char* out_base, double mhz, int batch) {
@@ -2198,7 +2274,9 @@ static int write_json_meta(const char* path, const std::vector<or_dec::Meta>& me
     int burst_end = (int)(fmt.burst_end_us * 1e-6 * fmt.output_rate + 0.5);
     int n = (int)meta.size();
     fprintf(fp, "{\n  \"videoParameters\": {\n");
-    fprintf(fp, "    \"system\": \"NTSC\",\n    \"isSubcarrierLocked\": false,\n    \"isSourcePal\": false,\n");
+    const bool pal = fmt.system == 1;
+    fprintf(fp, "    \"system\": \"%s\",\n    \"isSubcarrierLocked\": false,\n    \"isSourcePal\": %s,\n",
+            pal ? "PAL" : "NTSC", pal ? "true" : "false");
     fprintf(fp, "    \"numberOfSequentialFields\": %d,\n", n);
     fprintf(fp, "    \"black16bIre\": %.1f,\n", black16b);
     fprintf(fp, "    \"white16bIre\": %.1f,\n", white16b);
@@ -2207,8 +2285,8 @@ static int write_json_meta(const char* path, const std::vector<or_dec::Meta>& me
     fprintf(fp, "    \"fieldHeight\": %d,\n", fmt.output_field_lines);
     fprintf(fp, "    \"colourBurstStart\": %d,\n", burst_start);
     fprintf(fp, "    \"colourBurstEnd\": %d,\n", burst_end);
-    fprintf(fp, "    \"activeVideoStart\": %d,\n", 134);
-    fprintf(fp, "    \"activeVideoEnd\": %d,\n", 894);
+    fprintf(fp, "    \"activeVideoStart\": %d,\n", active_video_start(fmt));
+    fprintf(fp, "    \"activeVideoEnd\": %d,\n", active_video_end(fmt));
     fprintf(fp, "    \"tapeFormat\": \"VHS\",\n    \"isMapped\": false\n  },\n");
     fprintf(fp, "  \"fields\": [\n");
     for (int i = 0; i < n; i++) {

@@ -27,9 +27,9 @@ extern "C" {
 
 #define OR_MAX_PULSES 800          /* sync_pulses.h:6 */
 #define OR_MAX_DROPOUTS 512        /* dropout_detect.h:5 */
-#define OR_LINES_PER_FRAME 525
-#define OR_FIELD_LINES 263
-#define OR_LINE_LEN 910
+#define OR_LINES_PER_FRAME 525     /* NTSC; PAL 625 */
+#define OR_FIELD_LINES 263         /* NTSC; PAL 313 */
+#define OR_LINE_LEN 910            /* NTSC; PAL 1135 */
 
 typedef struct or_format {
     double sample_rate;
@@ -46,6 +46,9 @@ typedef struct or_format {
 } or_format;
 
 int or_format_init(double mhz, or_format* out);
+/* system: 0 NTSC, 1 PAL (video_format.cpp:48-87); the _sys variants below
+ * take it, the others are NTSC */
+int or_format_init_sys(int system, double mhz, or_format* out);
 
 /* --- filter tables (for design checks) --------------------------------- */
 /* which: 0 = K1 RF (real, fb bins), 1 = K1 Fvideo (complex, interleaved),
@@ -75,6 +78,7 @@ int or_tbc_resample(double mhz, const double* demod, int total, const double* li
 /* --- whole decode ------------------------------------------------------- */
 typedef struct or_dec or_dec;
 or_dec* or_create(double mhz, int batch, int record_trace);
+or_dec* or_create_sys(int system, double mhz, int batch, int record_trace);
 void or_destroy(or_dec* d);
 /* Input sample format (0 u8, 1 s16, 2 u16; 16-bit samples little-endian as
  * the host stores them) and per-read DC removal (raw_reader.cpp:71-100).
@@ -98,6 +102,8 @@ int or_diag_counts(double mhz, int which, const double* a, const double* b, cons
  * line locations and field parity (1 first, 0 second, -1 unknown). */
 int or_k3_linelocs(double mhz, const int* starts, const int* lens, const int* count, int nf, int* types,
                    double* linelocs, int* is_first);
+int or_k3_linelocs_sys(int system, double mhz, const int* starts, const int* lens, const int* count, int nf,
+                       int* types, double* linelocs, int* is_first);
 /* The conversion alone: n samples (n or 2n bytes) -> out[n] doubles. */
 int or_convert(int format, int dc_correct, const uint8_t* data, size_t n, double* out);
 /* Decode a whole capture held in memory: n samples (n or 2n bytes).  Returns
@@ -148,12 +154,18 @@ typedef struct or_k2b_state {
 int or_k2b_chunk(double mhz, or_k2b_state* st, uint64_t raw_offset, const uint64_t* offsets, int nf, int* starts,
                  int* lens, int* count, int* types, unsigned char* good, double* linelocs, int* is_first,
                  double* anchor);
+int or_k2b_chunk_sys(int system, double mhz, or_k2b_state* st, uint64_t raw_offset, const uint64_t* offsets, int nf,
+                     int* starts, int* lens, int* count, int* types, unsigned char* good, double* linelocs,
+                     int* is_first, double* anchor);
 /* lookup_ntsc_phase (chroma_decode.cu:359-381): the phase ID */
 int or_lookup_ntsc_phase(int is_first, int quadrant, int delta);
 /* measure_burst_cancellation / measure_burst_phase (chroma_decode.cu:383-450)
- * on one field's band-passed lines [263][1024] */
+ * on one field's band-passed lines [263][1024] (NTSC) */
 double or_burst_metric(double mhz, const double* lines);
 double or_burst_phase(double mhz, const double* lines);
+/* the same with the system's geometry (PAL: [313][2048] lines) */
+double or_burst_metric_sys(int system, double mhz, const double* lines);
+double or_burst_phase_sys(int system, double mhz, const double* lines);
 
 /* Seconds spent per stage over the decodes of d (9 values: conversion, K1,
  * VSYNC discovery + clustering, K2, K2b or K3, K4, K5, K6, K7); returns 9. */
@@ -165,6 +177,9 @@ int or_set_threads(int n);
 int or_write_json_meta(double mhz, const char* path, int n, const uint64_t* file_loc, const int* is_first,
                        const int* phase_id, const int* do_count, const int* do_line, const int* do_startx,
                        const int* do_endx);
+int or_write_json_meta_sys(int system, double mhz, const char* path, int n, const uint64_t* file_loc,
+                           const int* is_first, const int* phase_id, const int* do_count, const int* do_line,
+                           const int* do_startx, const int* do_endx);
 
 /* CLI-equivalent: decode file -> out_base.{tbc,_chroma.tbc,tbc.json} */
 int or_decode_file(const char* in_path, const char* out_base, double mhz, int batch);

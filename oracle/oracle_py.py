@@ -9,6 +9,10 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 MAX_PULSES = 800
 MAX_DO = 512
 LPF = 525
+SYSTEMS = {"ntsc": 0, "pal": 1}
+# (line locations per frame, output lines per field, samples per output line,
+# chroma line transform points)
+GEOMETRY = {"ntsc": (525, 263, 910, 1024), "pal": (625, 313, 1135, 2048)}
 
 _vp = ctypes.c_void_p
 _L = None
@@ -32,6 +36,11 @@ def lib():
         L = ctypes.CDLL(path)
         L.or_create.restype = _vp
         L.or_create.argtypes = [ctypes.c_double, ctypes.c_int, ctypes.c_int]
+        L.or_create_sys.restype = _vp
+        L.or_create_sys.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int]
+        L.or_format_init_sys.argtypes = [ctypes.c_int, ctypes.c_double, _vp]
+        L.or_k3_linelocs_sys.argtypes = [ctypes.c_int, ctypes.c_double, _vp, _vp, _vp, ctypes.c_int, _vp, _vp, _vp]
+        L.or_k2b_chunk_sys.argtypes = [ctypes.c_int, ctypes.c_double, _vp, ctypes.c_uint64, _vp, ctypes.c_int] + [_vp] * 8
         L.or_destroy.argtypes = [_vp]
         L.or_decode.argtypes = [_vp, _vp, ctypes.c_size_t]
         L.or_num_fields.argtypes = [_vp]
@@ -72,6 +81,10 @@ def lib():
         L.or_burst_metric.restype = ctypes.c_double
         L.or_burst_phase.argtypes = [ctypes.c_double, _vp]
         L.or_burst_phase.restype = ctypes.c_double
+        L.or_burst_metric_sys.argtypes = [ctypes.c_int, ctypes.c_double, _vp]
+        L.or_burst_metric_sys.restype = ctypes.c_double
+        L.or_burst_phase_sys.argtypes = [ctypes.c_int, ctypes.c_double, _vp]
+        L.or_burst_phase_sys.restype = ctypes.c_double
         _L = L
     return _L
 
@@ -88,10 +101,11 @@ def set_threads(n):
 FORMATS = {np.dtype(np.uint8): 0, np.dtype(np.int16): 1, np.dtype(np.uint16): 2}
 
 
-def decode(data, mhz, batch, trace=True, dc_correct=False, line_locator=0, diagnostics=False):
+def decode(data, mhz, batch, trace=True, dc_correct=False, line_locator=0, diagnostics=False, system="ntsc"):
     """Decode a u8 capture; returns a dict of numpy outputs (+ per-chunk traces)."""
     L = lib()
-    d = L.or_create(float(mhz), int(batch), 1 if trace else 0)
+    lpf, rows, width, nfft = GEOMETRY[system]
+    d = L.or_create_sys(SYSTEMS[system], float(mhz), int(batch), 1 if trace else 0)
     try:
         data = np.ascontiguousarray(data)
         if data.dtype not in FORMATS:
@@ -103,13 +117,13 @@ def decode(data, mhz, batch, trace=True, dc_correct=False, line_locator=0, diagn
         st = np.zeros(9)
         L.or_stage_seconds(d, st.ctypes.data)
         out["stage_seconds"] = dict(zip(STAGES, st.tolist()))
-        tbc = n * 263 * 910
+        tbc = n * rows * width
         for key, fn in (("luma", L.or_luma), ("chroma", L.or_chroma)):
             if n == 0:  # no field was produced: the plane buffers were never allocated
-                out[key] = np.zeros((0, 263, 910), np.uint16)
+                out[key] = np.zeros((0, rows, width), np.uint16)
             else:
                 ptr = ctypes.cast(fn(d), ctypes.POINTER(ctypes.c_uint16))
-                out[key] = np.ctypeslib.as_array(ptr, (tbc,)).reshape(n, 263, 910).copy()
+                out[key] = np.ctypeslib.as_array(ptr, (tbc,)).reshape(n, rows, width).copy()
         fl = np.zeros(n, np.uint64)
         isf = np.zeros(n, np.int32)
         ph = np.zeros(n, np.int32)
@@ -137,8 +151,8 @@ def decode(data, mhz, batch, trace=True, dc_correct=False, line_locator=0, diagn
                 vs = np.zeros(B * MAX_PULSES, np.int32)
                 vl = np.zeros(B * MAX_PULSES, np.int32)
                 vt = np.zeros(B * MAX_PULSES, np.int32)
-                llc = np.zeros(B * LPF)
-                llr = np.zeros(B * LPF)
+                llc = np.zeros(B * lpf)
+                llr = np.zeros(B * lpf)
                 isk = np.zeros(B, np.int32)
                 nf = L.or_trace_chunk(d, c, *[a.ctypes.data for a in (ro, fk, no, offs, pc, ps, pl, vc, vs, vl, vt, llc, llr, isk)])
                 chunks.append(dict(raw_offset=int(ro[0]), fields_k1=int(fk[0]), next_offset=int(no[0]), nf=nf,
@@ -155,9 +169,9 @@ def decode(data, mhz, batch, trace=True, dc_correct=False, line_locator=0, diagn
             if trace:
                 lines = []
                 for which in range(3):
-                    buf = np.zeros(263 * 1024)
+                    buf = np.zeros(rows * nfft)
                     if L.or_chroma_trial_lines(d, which, buf.ctypes.data) == buf.size:
-                        lines.append(buf.reshape(263, 1024))
+                        lines.append(buf.reshape(rows, nfft))
                 if len(lines) == 3:
                     out["track_trial"]["lines"] = lines
         if diagnostics:
@@ -194,18 +208,18 @@ def convert(data, dc_correct=False):
     return out
 
 
-def k3_linelocs(starts, lens, count, mhz):
+def k3_linelocs(starts, lens, count, mhz, system="ntsc"):
     """K3 line location on K2 pulse lists ([nf, 800] int arrays, count [nf]):
-    (nominal types, linelocs [nf, 525], is_first [nf])."""
+    (nominal types, linelocs [nf, 525 or 625], is_first [nf])."""
     starts = np.ascontiguousarray(starts, np.int32)
     lens = np.ascontiguousarray(lens, np.int32)
     count = np.ascontiguousarray(count, np.int32)
     nf = count.size
     types = np.zeros((nf, MAX_PULSES), np.int32)
-    ll = np.zeros((nf, 525))
+    ll = np.zeros((nf, GEOMETRY[system][0]))
     isf = np.zeros(nf, np.int32)
-    lib().or_k3_linelocs(float(mhz), starts.ctypes.data, lens.ctypes.data, count.ctypes.data, nf, types.ctypes.data,
-                         ll.ctypes.data, isf.ctypes.data)
+    lib().or_k3_linelocs_sys(SYSTEMS[system], float(mhz), starts.ctypes.data, lens.ctypes.data, count.ctypes.data, nf,
+                             types.ctypes.data, ll.ctypes.data, isf.ctypes.data)
     return types, ll, isf
 
 
@@ -236,9 +250,10 @@ class K2bState(ctypes.Structure):
         return cls(-1, -1, -1, -1.0, -1.0)
 
 
-def k2b_chunk_with(fn, mhz, state, raw_offset, offsets, count, starts, lens):
+def k2b_chunk_with(fn, mhz, state, raw_offset, offsets, count, starts, lens, system=None):
     """Run one chunk of K2b through fn (or_k2b_chunk, or the reference's
-    ref_k2b_chunk with the same signature) on K2 pulses count[nf],
+    ref_k2b_chunk with the same signature; with system "ntsc" / "pal" the
+    _sys forms, which take the system first) on K2 pulses count[nf],
     starts/lens[nf][<=800]; returns the validated pulses, types, good flags,
     line locations, is_first and the anchor values per field."""
     nf = len(offsets)
@@ -250,10 +265,11 @@ def k2b_chunk_with(fn, mhz, state, raw_offset, offsets, count, starts, lens):
     offs = np.ascontiguousarray(offsets, np.uint64)
     types = np.zeros((nf, MAX_PULSES), np.int32)
     good = np.zeros((nf, MAX_PULSES), np.uint8)
-    ll = np.zeros((nf, LPF))
+    ll = np.zeros((nf, GEOMETRY[system or "ntsc"][0]))
     isf = np.zeros(nf, np.int32)
     anchor = np.zeros((nf, 9))
-    rc = fn(ctypes.c_double(float(mhz)), ctypes.byref(state), ctypes.c_uint64(int(raw_offset)), offs.ctypes.data,
+    pre = () if system is None else (ctypes.c_int(SYSTEMS[system]),)
+    rc = fn(*pre, ctypes.c_double(float(mhz)), ctypes.byref(state), ctypes.c_uint64(int(raw_offset)), offs.ctypes.data,
             nf, st.ctypes.data, ln.ctypes.data, cnt.ctypes.data, types.ctypes.data, good.ctypes.data,
             ll.ctypes.data, isf.ctypes.data, anchor.ctypes.data)
     if rc != 0:
@@ -261,23 +277,23 @@ def k2b_chunk_with(fn, mhz, state, raw_offset, offsets, count, starts, lens):
     return dict(count=cnt, starts=st, lens=ln, types=types, good=good, linelocs=ll, is_first=isf, anchor=anchor)
 
 
-def k2b_chunk(mhz, state, raw_offset, offsets, count, starts, lens):
+def k2b_chunk(mhz, state, raw_offset, offsets, count, starts, lens, system="ntsc"):
     """The oracle's K2b (pipeline.cu:1717-1778 restated) on one chunk."""
-    return k2b_chunk_with(lib().or_k2b_chunk, mhz, state, raw_offset, offsets, count, starts, lens)
+    return k2b_chunk_with(lib().or_k2b_chunk_sys, mhz, state, raw_offset, offsets, count, starts, lens, system)
 
 
 def lookup_ntsc_phase(is_first, quadrant, delta):
     return lib().or_lookup_ntsc_phase(int(is_first), int(quadrant), int(delta))
 
 
-def burst_metric(mhz, lines):
+def burst_metric(mhz, lines, system="ntsc"):
     lines = np.ascontiguousarray(lines, np.float64)
-    return lib().or_burst_metric(float(mhz), lines.ctypes.data)
+    return lib().or_burst_metric_sys(SYSTEMS[system], float(mhz), lines.ctypes.data)
 
 
-def burst_phase(mhz, lines):
+def burst_phase(mhz, lines, system="ntsc"):
     lines = np.ascontiguousarray(lines, np.float64)
-    return lib().or_burst_phase(float(mhz), lines.ctypes.data)
+    return lib().or_burst_phase_sys(SYSTEMS[system], float(mhz), lines.ctypes.data)
 
 
 _R = None
@@ -294,6 +310,7 @@ def ref_k2b_lib():
             return None
         R = ctypes.CDLL(path)
         R.ref_k2b_chunk.argtypes = [ctypes.c_double, _vp, ctypes.c_uint64, _vp, ctypes.c_int] + [_vp] * 8
+        R.ref_k2b_chunk_sys.argtypes = [ctypes.c_int, ctypes.c_double, _vp, ctypes.c_uint64, _vp, ctypes.c_int] + [_vp] * 8
         R.ref_lookup_ntsc_phase.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp]
         R.ref_burst_cancellation.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                              ctypes.c_int, ctypes.c_double]
@@ -302,18 +319,22 @@ def ref_k2b_lib():
                                       ctypes.c_double, ctypes.c_double, ctypes.c_double]
         R.ref_burst_phase.restype = ctypes.c_double
         R.ref_chroma_params.argtypes = [ctypes.c_double, _vp]
+        R.ref_chroma_params_sys.argtypes = [ctypes.c_int, ctypes.c_double, _vp]
         _R = R
     return _R
 
 
-def ref_chroma_trial(R, mhz, lines):
+def ref_chroma_trial(R, mhz, lines, system="ntsc"):
     """The reference's measure_burst_cancellation on the track-0 and track-1
     trial lines and measure_burst_phase on the detected-track lines
     (chroma_decode.cu:688-731), with its own format constants."""
     p = np.zeros(6)
-    R.ref_chroma_params(float(mhz), p.ctypes.data)
+    R.ref_chroma_params_sys(SYSTEMS[system], float(mhz), p.ctypes.data)
     fsc, rate, L, rows, bs, be = p
-    args = (int(rows), 1024, int(L), int(bs), int(be), 1.0 / 1024)
+    nfft = 1
+    while nfft < int(L):
+        nfft *= 2
+    args = (int(rows), nfft, int(L), int(bs), int(be), 1.0 / nfft)
     m = [R.ref_burst_cancellation(np.ascontiguousarray(lines[t]).ctypes.data, *args) for t in (0, 1)]
     ph = R.ref_burst_phase(np.ascontiguousarray(lines[2]).ctypes.data, *args, fsc, rate)
     return m[0], m[1], ph

@@ -5,9 +5,9 @@
 // compare the oracle's and the decoder's restatements with the reference's
 // compiled code.  Test infrastructure only.
 //
-// usage: ref_host format <mhz>
+// usage: ref_host format <mhz> [pal]
 //        ref_host convert <file> [u8|s16|u16] [dc]   (one converted value per line)
-//        ref_host json <out_base>              (fixed metadata, see below)
+//        ref_host json <out_base> [pal]        (fixed metadata, see below)
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -19,8 +19,10 @@
 
 int main(int argc, char** argv) {
     if (argc < 3) return 2;
+    const bool pal = argc > 3 && !strcmp(argv[3], "pal");
+    const VideoSystem sys = pal ? VideoSystem::PAL : VideoSystem::NTSC;
     if (!strcmp(argv[1], "format")) {
-        VideoFormat f(VideoSystem::NTSC, atof(argv[2]));
+        VideoFormat f(sys, atof(argv[2]));
         printf("sample_rate=%.17g\nsamples_per_line=%d\nsamples_per_field=%d\n", f.sample_rate, f.samples_per_line,
                f.samples_per_field);
         printf("hsync_width=%.17g\neq_pulse_width=%.17g\nvsync_width=%.17g\n", f.hsync_width, f.eq_pulse_width,
@@ -30,6 +32,11 @@ int main(int argc, char** argv) {
         printf("output_zero=%.17g\noutput_scale=%.17g\noutput_rate=%.17g\nfsc=%.17g\nchroma_under=%.17g\n",
                f.output_zero, f.output_scale, f.output_rate, f.fsc, f.chroma_under);
         printf("burst_abs_ref=%.17g\n", f.burst_abs_ref);
+        printf("lines_per_frame=%d\noutput_line_len=%d\noutput_field_lines=%d\nnum_eq_pulses=%d\n",
+               f.lines_per_frame, f.output_line_len, f.output_field_lines, f.num_eq_pulses);
+        printf("field_lines_first=%d\nfield_lines_second=%d\nactive_line_start=%d\n", f.field_lines_first,
+               f.field_lines_second, f.active_line_start);
+        printf("burst_start_us=%.17g\nburst_end_us=%.17g\n", f.burst_start_us, f.burst_end_us);
         return 0;
     }
     if (!strcmp(argv[1], "convert")) {
@@ -52,9 +59,9 @@ int main(int argc, char** argv) {
         // three fields: alternating parity from false, phase IDs 1,4,0 (0 = unset),
         // dropouts only on the second field
         TBCWriter w;
-        VideoFormat f(VideoSystem::NTSC, 28.0);
+        VideoFormat f(sys, 28.0);
         if (!w.open(argv[2], f, true)) return 1;
-        std::vector<uint16_t> blank((size_t)910 * 263, 1024);
+        std::vector<uint16_t> blank((size_t)f.output_line_len * f.output_field_lines, 1024);
         const int phase[3] = {1, 4, 0};
         const size_t loc[3] = {32032, 499165, 966298};
         for (int i = 0; i < 3; i++) {

@@ -46,15 +46,15 @@ enum { kAnchorVals = 9 };
 // One chunk of K2b as process_chunk runs it (pipeline.cu:1717-1778).
 // starts/lens/count ([nf][800], count[nf]) hold the K2 pulses on entry and the
 // validated pulses on return; types [nf][800]; good [nf][800] (the validated
-// pulses' good flags, 0/1); linelocs [nf][525]; is_first [nf]; anchor
-// [nf][9]: valid, line0loc, first_hsync_loc, first_hsync_line, next_field,
-// first_field, progressive_field, prev_hsync_diff, meanlinelen.  Any of
-// good / anchor may be NULL.
-int ref_k2b_chunk(double mhz, k2b_state_io* st, uint64_t raw_offset, const uint64_t* offsets, int nf, int* starts,
-                  int* lens, int* count, int* types, unsigned char* good, double* linelocs, int* is_first,
-                  double* anchor) {
-    if (!st || nf < 0) return -1;
-    VideoFormat fmt(VideoSystem::NTSC, mhz);
+// pulses' good flags, 0/1); linelocs [nf][525] (PAL [nf][625]); is_first
+// [nf]; anchor [nf][9]: valid, line0loc, first_hsync_loc, first_hsync_line,
+// next_field, first_field, progressive_field, prev_hsync_diff, meanlinelen.
+// Any of good / anchor may be NULL.  system: 0 NTSC, 1 PAL.
+int ref_k2b_chunk_sys(int system, double mhz, k2b_state_io* st, uint64_t raw_offset, const uint64_t* offsets, int nf,
+                      int* starts, int* lens, int* count, int* types, unsigned char* good, double* linelocs,
+                      int* is_first, double* anchor) {
+    if (!st || nf < 0 || system < 0 || system > 1) return -1;
+    VideoFormat fmt(system == 1 ? VideoSystem::PAL : VideoSystem::NTSC, mhz);
     const int LPF = fmt.lines_per_frame;
     for (int field = 0; field < nf; field++) {
         int* fs = starts + (size_t)field * MAX_PULSES;
@@ -109,11 +109,22 @@ int ref_k2b_chunk(double mhz, k2b_state_io* st, uint64_t raw_offset, const uint6
     return 0;
 }
 
+int ref_k2b_chunk(double mhz, k2b_state_io* st, uint64_t raw_offset, const uint64_t* offsets, int nf, int* starts,
+                  int* lens, int* count, int* types, unsigned char* good, double* linelocs, int* is_first,
+                  double* anchor) {
+    return ref_k2b_chunk_sys(0, mhz, st, raw_offset, offsets, nf, starts, lens, count, types, good, linelocs, is_first,
+                             anchor);
+}
+
+int ref_chroma_params_sys(int system, double mhz, double* out);
+
 // Chroma constants of the reference's VideoFormat for a sample rate: fsc,
 // output_rate, output_line_len, output_field_lines, and the burst window in
 // output samples as chroma_decode() derives it (chroma_decode.cu:548-549).
-int ref_chroma_params(double mhz, double* out) {
-    VideoFormat fmt(VideoSystem::NTSC, mhz);
+int ref_chroma_params(double mhz, double* out) { return ref_chroma_params_sys(0, mhz, out); }
+
+int ref_chroma_params_sys(int system, double mhz, double* out) {
+    VideoFormat fmt(system == 1 ? VideoSystem::PAL : VideoSystem::NTSC, mhz);
     out[0] = fmt.fsc;
     out[1] = fmt.output_rate;
     out[2] = (double)fmt.output_line_len;

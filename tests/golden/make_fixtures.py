@@ -6,13 +6,15 @@
                          Butterworth designs and the Hamming sync FIR at
                          sampled bins, 28 and 40 MHz
 * ref_format.json        VideoFormat constants printed by the reference's own
-                         video_format.cpp (oracle/_ref/ref_host format)
+                         video_format.cpp (oracle/_ref/ref_host format), NTSC
+                         and PAL ("pal_28" ...)
 * ref_convert_u8.txt     RawReader::read_at conversion of all 256 u8 values
 * ref_convert_formats.npz  RawReader::read_at conversion of 4096 random u8,
                          s16 and u16 samples, with and without --dc-correct
                          by the reference's own raw_reader.cpp
 * ref_writer_3fields.tbc.json  .tbc.json written by the reference's own
-                         tbc_writer.cpp for fixed metadata (ref_host json)
+                         tbc_writer.cpp for fixed metadata (ref_host json);
+                         ref_writer_3fields_pal.tbc.json the same for PAL
 
 The reference-derived files need oracle/_ref/ref_host (make -C oracle ref,
 only where the reference sources are mounted).
@@ -100,9 +102,11 @@ def reference_files():
         print("oracle/_ref/ref_host missing: reference-derived fixtures not regenerated")
         return
     fmts = {}
-    for mhz in ("28", "35.8", "40"):
-        out = subprocess.run([REF, "format", mhz], capture_output=True, text=True, check=True).stdout
-        fmts[mhz] = {k: float(v) for k, v in (l.split("=") for l in out.split())}
+    for system in ("ntsc", "pal"):
+        for mhz in ("28", "35.8", "40"):
+            args = [REF, "format", mhz] + (["pal"] if system == "pal" else [])
+            out = subprocess.run(args, capture_output=True, text=True, check=True).stdout
+            fmts[("pal_" if system == "pal" else "") + mhz] = {k: float(v) for k, v in (l.split("=") for l in out.split())}
     with open(os.path.join(HERE, "ref_format.json"), "w") as f:
         json.dump(fmts, f, indent=1, sort_keys=True)
     tmp = "/tmp/_all_u8.u8"
@@ -110,12 +114,14 @@ def reference_files():
     out = subprocess.run([REF, "convert", tmp], capture_output=True, text=True, check=True).stdout
     with open(os.path.join(HERE, "ref_convert_u8.txt"), "w") as f:
         f.write(out)
-    base = "/tmp/_ref_writer"
-    subprocess.run([REF, "json", base], capture_output=True, text=True, check=True)
-    with open(base + ".tbc.json") as f:
-        text = f.read()
-    with open(os.path.join(HERE, "ref_writer_3fields.tbc.json"), "w") as f:
-        f.write(text)
+    for system, name in (("ntsc", "ref_writer_3fields.tbc.json"), ("pal", "ref_writer_3fields_pal.tbc.json")):
+        base = "/tmp/_ref_writer"
+        args = [REF, "json", base] + (["pal"] if system == "pal" else [])
+        subprocess.run(args, capture_output=True, text=True, check=True)
+        with open(base + ".tbc.json") as f:
+            text = f.read()
+        with open(os.path.join(HERE, name), "w") as f:
+            f.write(text)
 
 
 def convert_formats():

@@ -31,7 +31,8 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "oracle"))
 
-# name -> (sample rate MHz, generator config, overrides, fields, field cut out of the tape or None)
+# name -> (sample rate MHz, generator config, overrides, fields, field cut out
+# of the tape or None[, system])
 K2B_CASES = {
     "cfg1_601f": (28.0, 1, {}, None, None),
     "cfg2_640f_40mhz": (40.0, 2, {}, 640, None),
@@ -40,26 +41,36 @@ K2B_CASES = {
     "dropouts8_96f": (28.0, 1, dict(dropouts_per_field=8.0, seed=7), 96, None),
     "cut_field_70f": (28.0, 1, dict(seed=9), 71, 6),
     "cfg1_130f_35_8mhz": (35.8, 1, dict(sample_rate_mhz=35.8, seed=8), 130, None),
+    # PAL (the reference's PAL branches of pipeline.cu:707-717, 811)
+    "pal_cfg1_200f": (28.0, 1, {}, 200, None, "pal"),
+    "pal_cfg2_192f_40mhz": (40.0, 2, {}, 192, None, "pal"),
+    "pal_snr18_96f": (28.0, 1, dict(snr_lo_db=18.0, snr_hi_db=20.0, seed=16), 96, None, "pal"),
+    "pal_cut_field_70f": (28.0, 1, dict(seed=19), 71, 5, "pal"),
 }
 BATCH = 64
 
-# chroma trial captures: (MHz, config, overrides, fields, start field)
+# chroma trial captures: (MHz, config, overrides, fields, start field[, system])
 CHROMA_CASES = [(28.0, 1, {}, 3, s) for s in range(4)] + [
     (28.0, 0, {}, 4, 0),
     (40.0, 2, {}, 3, 0),
     (40.0, 2, {}, 4, 1),
     (35.8, 1, dict(sample_rate_mhz=35.8, seed=8), 3, 0),
     (28.0, 1, dict(snr_lo_db=18.0, snr_hi_db=20.0, seed=6), 3, 2),
-]
+] + [(28.0, 1, {}, 3, s, "pal") for s in range(4)] + [(40.0, 2, {}, 3, 1, "pal")]
 
 
-def capture(mhz, config, over, fields, cut=None, start=0):
+def case_system(case, n):
+    """The system of a case tuple with n fixed elements ("ntsc" by default)."""
+    return case[n] if len(case) > n else "ntsc"
+
+
+def capture(mhz, config, over, fields, cut=None, start=0, system="ntsc"):
     from cuvhs_amd import native
-    spf = native.format_for(mhz).samples_per_field
+    spf = native.format_for(mhz, system).samples_per_field
     kw = dict(over)
     if fields is not None:
         kw["n_samples"] = (fields + start) * spf
-    g = native.generate(config, **kw)
+    g = native.generate(config, system=system, **kw)
     if cut is not None:
         g = np.concatenate([g[:cut * spf], g[(cut + 1) * spf:]])
     return np.ascontiguousarray(g[start * spf:])
@@ -79,7 +90,7 @@ def _undelta(d):
     return np.cumsum(np.asarray(d, np.int64)).astype(np.int32)
 
 
-def pack_k2b(name, mhz, chunks, ref_fn, keep=None):
+def pack_k2b(name, mhz, chunks, ref_fn, keep=None, system="ntsc"):
     """Inputs of the first `keep` chunks and the outputs of ref_fn (the
     reference's K2b chunk runner) on them, plus the cadence state entering
     each chunk.  Every chunk is also run through the oracle's K2b; returns
@@ -96,8 +107,9 @@ def pack_k2b(name, mhz, chunks, ref_fn, keep=None):
         state_in = [st.prev_first_field, st.prev_progressive_field, st.prev_first_hsync_readloc,
                     st.prev_first_hsync_loc, st.prev_first_hsync_diff]
         out = O.k2b_chunk_with(ref_fn, mhz, st, c["raw_offset"], c["offsets"], c["k2_count"], c["k2_starts"],
-                               c["k2_lens"])
-        mine = O.k2b_chunk(mhz, so, c["raw_offset"], c["offsets"], c["k2_count"], c["k2_starts"], c["k2_lens"])
+                               c["k2_lens"], system)
+        mine = O.k2b_chunk(mhz, so, c["raw_offset"], c["offsets"], c["k2_count"], c["k2_starts"], c["k2_lens"],
+                           system)
         if any(not np.array_equal(out[k], mine[k]) for k in out):
             differ += 1
         if keep is not None and ci >= keep:
@@ -126,6 +138,7 @@ def pack_k2b(name, mhz, chunks, ref_fn, keep=None):
     cat["raw_offset"] = cat["raw_offset"].astype(np.int64)
     cat["nf"] = cat["nf"].astype(np.int64)
     cat["mhz"] = np.float64(mhz)
+    cat["system"] = np.int64(O.SYSTEMS[system])
     cat["batch"] = np.int64(BATCH)
     cat["chunks_total"] = np.int64(len(chunks))
     cat["fields_total"] = np.int64(sum(c["nf"] for c in chunks))
@@ -170,11 +183,11 @@ def unpack_k2b(z):
     return chunks
 
 
-def chroma_trial(mhz, config, over, fields, start):
+def chroma_trial(mhz, config, over, fields, start, system="ntsc"):
     """The oracle's field-0 track trial lines of a small capture."""
     import oracle_py as O
-    cap = capture(mhz, config, over, fields, start=start)
-    r = O.decode(cap, mhz, BATCH, trace=True)
+    cap = capture(mhz, config, over, fields, start=start, system=system)
+    r = O.decode(cap, mhz, BATCH, trace=True, system=system)
     return r
 
 
@@ -184,13 +197,15 @@ def main():
     if R is None:
         sys.exit("oracle/_ref/libref_k2b.so is missing: run `make -C oracle refk2b` with the reference mounted")
     only = sys.argv[1:]
-    for name, (mhz, config, over, fields, cut) in K2B_CASES.items():
+    for name, case in K2B_CASES.items():
         if only and name not in only:
             continue
+        mhz, config, over, fields, cut = case[:5]
+        system = case_system(case, 5)
         t = time.perf_counter()
-        cap = capture(mhz, config, over, fields, cut)
-        r = O.decode(cap, mhz, BATCH, trace=True)
-        rec, differ = pack_k2b(name, mhz, r["chunks"], R.ref_k2b_chunk, KEEP_CHUNKS.get(name))
+        cap = capture(mhz, config, over, fields, cut, system=system)
+        r = O.decode(cap, mhz, BATCH, trace=True, system=system)
+        rec, differ = pack_k2b(name, mhz, r["chunks"], R.ref_k2b_chunk_sys, KEEP_CHUNKS.get(name), system)
         path = os.path.join(HERE, f"ref_k2b_{name}.npz")
         np.savez_compressed(path, **rec)
         print(f"{name}: {r['fields']} fields, {len(r['chunks'])} chunks ({len(rec['nf'])} kept), "
@@ -199,11 +214,14 @@ def main():
     if only and "chroma" not in only:
         return
     rows = []
-    for (mhz, config, over, fields, start) in CHROMA_CASES:
-        r = chroma_trial(mhz, config, over, fields, start)
-        m0, m1, ph = O.ref_chroma_trial(R, mhz, r["track_trial"]["lines"])
-        rows.append([mhz, config, fields, start, m0, m1, ph])
-        print(f"chroma {mhz} cfg{config} start {start}: metrics {m0:.4f} {m1:.4f} phase {ph:.3f}", flush=True)
+    for case in CHROMA_CASES:
+        mhz, config, over, fields, start = case[:5]
+        system = case_system(case, 5)
+        r = chroma_trial(mhz, config, over, fields, start, system)
+        m0, m1, ph = O.ref_chroma_trial(R, mhz, r["track_trial"]["lines"], system)
+        rows.append([mhz, config, fields, start, m0, m1, ph, O.SYSTEMS[system]])
+        print(f"chroma {system} {mhz} cfg{config} start {start}: metrics {m0:.4f} {m1:.4f} phase {ph:.3f}",
+              flush=True)
     table = np.zeros((2, 4, 5, 2), np.int32)   # [is_first][quadrant][delta + 1] -> (phase_id, offset)
     import ctypes
     for isf in range(2):

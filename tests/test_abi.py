@@ -22,7 +22,7 @@ def test_header_symbols_exported():
 
 def test_create_rejects_unsupported_configs():
     h = ctypes.c_void_p()
-    for cfg in (native.Config(1, 28.0, 64, 0, None, 0),     # PAL
+    for cfg in (native.Config(2, 28.0, 64, 0, None, 0),     # no such system
                 native.Config(0, 50.0, 64, 0, None, 0),     # above 40 MHz
                 native.Config(0, 28.0, 64, 7, None, 0)):    # unknown precision
         assert native.lib().cuvhs_create(ctypes.byref(cfg), ctypes.byref(h)) == native.E_UNSUPPORTED
@@ -51,3 +51,24 @@ def test_format_known_answers(mhz, spl):
     assert f.samples_per_field == spl * 263
     assert f.spfp == spl * 283
     assert abs(f.pulse_threshold_hz - 3542857.142857143) < 1e-6
+
+
+@pytest.mark.parametrize("mhz,spl", [(28.0, 1792), (35.8, 2291), (40.0, 2560)])
+def test_pal_format_known_answers(mhz, spl):
+    """PAL (video_format.cpp:48-87): 15625 Hz lines, 313 per field, the
+    -20 IRE threshold at 4.1 MHz - 20 x 7000 Hz, 1135 x 313 output fields;
+    the NTSC entry point keeps NTSC."""
+    f = native.format_for(mhz, "pal")
+    assert f.samples_per_line == spl
+    assert f.samples_per_field == spl * 313
+    assert f.spfp == spl * 333
+    assert (f.system, f.lines_per_frame, f.field_lines, f.line_len) == (1, 625, 313, 1135)
+    assert abs(f.hz_ire - 7000.0) < 1e-9 and abs(f.ire0 - 4.1e6) < 1e-6
+    assert abs(f.pulse_threshold_hz - 3.96e6) < 1e-6
+    assert f.output_zero == 256.0 and abs(f.output_scale - (54016.0 - 256.0) / (100.0 + 300.0 / 7.0)) < 1e-9
+    assert (f.burst_start, f.burst_end) == (99, 139)
+    assert f.fsc == 4433618.75 and f.output_rate == 4.0 * 4433618.75
+    n = native.format_for(mhz)
+    assert (n.system, n.lines_per_frame, n.field_lines, n.line_len) == (0, 525, 263, 910)
+    bad = native.Format()
+    assert native.lib().cuvhs_format_for_system(2, mhz, ctypes.byref(bad)) == native.E_ARG

@@ -11,15 +11,19 @@ captures are committed as vectors (tests/golden/make_ref_k2b_fixtures.py):
   full 601-field 28 MHz benchmark capture with SNR 25-40 dB, dropouts and
   wow/flutter; 640 fields of the 40 MHz head-switch capture; 25 dB; 18-20 dB;
   eight dropouts per field; a capture with a recorded field cut out; 35.8
-  MHz), chunks of 64 fields with the cadence state carried across chunks
-  (the two long captures are committed as their first three chunks; all of
-  their chunks were compared when the vectors were made).
+  MHz), and four PAL captures (200 fields at 28 MHz, 192 fields of the 40
+  MHz head-switch model, 18-20 dB, a field cut out), which take the
+  reference's PAL branches (pipeline.cu:707-717, 811); chunks of 64 fields
+  with the cadence state carried across chunks (the two long NTSC captures
+  are committed as their first three chunks; all of their chunks were
+  compared when the vectors were made).
   The oracle's K2b must give the same validated pulses, types, good flags,
   line locations, is_first and anchor values (valid, line0loc,
   first_hsync_loc, first_hsync_line, next_field, first_field,
   progressive_field, prev_hsync_diff, meanlinelen), bit for bit.
 * Chroma: on the oracle's band-passed lines of the field-0 track trial of
-  nine small captures (both tracks, 28 / 35.8 / 40 MHz, 18-20 dB), the
+  fourteen small captures (both tracks, 28 / 35.8 / 40 MHz, 18-20 dB, NTSC
+  and PAL: 313 lines of 2048-point transforms), the
   reference's measure_burst_cancellation must equal the oracle's metric for
   each track, and the reference's measure_burst_phase, quantised to a
   quadrant and looked up in the reference's own table, must give the phase ID
@@ -67,6 +71,7 @@ def _compare(got, exp, where):
 def test_oracle_k2b_equals_reference(oracle, path):
     z = np.load(path)
     mhz = float(z["mhz"])
+    system = "pal" if "system" in z.files and int(z["system"]) == 1 else "ntsc"
     chunks = FX.unpack_k2b(z)
     assert len(chunks) >= 1
     st = oracle.K2bState.initial()
@@ -77,11 +82,12 @@ def test_oracle_k2b_equals_reference(oracle, path):
         # the cadence state the reference carried into this chunk
         assert (st.prev_first_field, st.prev_progressive_field, st.prev_first_hsync_readloc,
                 st.prev_first_hsync_loc, st.prev_first_hsync_diff) == c["state_in"], ci
-        got = oracle.k2b_chunk(mhz, st, c["raw_offset"], c["offsets"], c["k2_count"], c["k2_starts"], c["k2_lens"])
+        got = oracle.k2b_chunk(mhz, st, c["raw_offset"], c["offsets"], c["k2_count"], c["k2_starts"], c["k2_lens"],
+                               system)
         _compare(got, c["expected"], f"chunk {ci}")
         if R is not None:
-            live = oracle.k2b_chunk_with(R.ref_k2b_chunk, mhz, sr, c["raw_offset"], c["offsets"], c["k2_count"],
-                                         c["k2_starts"], c["k2_lens"])
+            live = oracle.k2b_chunk_with(R.ref_k2b_chunk_sys, mhz, sr, c["raw_offset"], c["offsets"], c["k2_count"],
+                                         c["k2_starts"], c["k2_lens"], system)
             _compare(live, c["expected"], f"chunk {ci} (reference library)")
         fields += c["nf"]
     assert fields == len(z["is_first"])
@@ -89,7 +95,7 @@ def test_oracle_k2b_equals_reference(oracle, path):
 
 def test_vectors_cover_the_whole_captures():
     """When the vectors were made, the oracle's K2b was run beside the
-    reference's on every chunk of every capture (1,794 fields; the two long
+    reference's on every chunk of every capture (2,352 fields, 558 of them PAL; the two long
     captures keep only their first three chunks in the committed vectors) and
     no chunk differed.  The committed fields hold the validation quirks of
     noisy vertical intervals: validated pulses flagged not good (partial
@@ -106,24 +112,27 @@ def test_vectors_cover_the_whole_captures():
 @pytest.mark.parametrize("i", range(len(FX.CHROMA_CASES)))
 def test_chroma_trial_and_first_phase_id_equal_reference(oracle, i):
     z = np.load(os.path.join(GOLDEN, "ref_chroma_phase.npz"))
-    mhz, config, over, fields, start = FX.CHROMA_CASES[i]
+    case = FX.CHROMA_CASES[i]
+    mhz, config, over, fields, start = case[:5]
+    system = FX.case_system(case, 5)
     row = z["cases"][i]
-    assert (row[0], row[1], row[2], row[3]) == (mhz, config, fields, start)
-    r = FX.chroma_trial(mhz, config, over, fields, start)
+    assert (row[0], row[1], row[2], row[3], row[7]) == (mhz, config, fields, start, oracle.SYSTEMS[system])
+    r = FX.chroma_trial(mhz, config, over, fields, start, system)
     lines = r["track_trial"]["lines"]
+    assert lines[0].shape == (oracle.GEOMETRY[system][1], oracle.GEOMETRY[system][3])
     m = r["track_trial"]["metrics"]
     # the metric of each track, bit for bit (same operation order)
     assert m[0] == row[4] and m[1] == row[5], (m, row[4:6])
-    assert oracle.burst_metric(mhz, lines[0]) == row[4]
+    assert oracle.burst_metric(mhz, lines[0], system) == row[4]
     # the reference's burst phase -> quadrant -> its own lookup table -> the
     # phase ID of field 0 (chroma_decode.cu:728-734)
-    assert oracle.burst_phase(mhz, lines[2]) == row[6]
+    assert oracle.burst_phase(mhz, lines[2], system) == row[6]
     quadrant = int(row[6] / 90.0 + 0.5) % 4
     pid0 = int(z["lookup"][1, quadrant, 0, 0])
     assert int(r["phase_id"][0]) == pid0, (r["phase_id"][:4], quadrant)
     R = oracle.ref_k2b_lib()
     if R is not None:
-        live = oracle.ref_chroma_trial(R, mhz, lines)
+        live = oracle.ref_chroma_trial(R, mhz, lines, system)
         assert live == (row[4], row[5], row[6])
 
 

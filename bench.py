@@ -15,7 +15,9 @@ reference's arithmetic, the one checked against the oracle), fp32 (K1
 transforms in FP32) and mixed-bf16 -- each reduced form with its error and
 decision report against the FP64 form (cuvhs_amd.precision), the mixed
 form's MFMA utilisation and its speed over the plain FP32 form;
---no-forms skips them.
+--no-forms skips them.  --workload picks another capture for a secondary
+line (the 40 MHz head-switch workload, or the same 600-field signal model
+on PAL's line structure and levels).
 """
 import argparse
 import json
@@ -37,7 +39,9 @@ DTYPES = {"fp64": "f64", "fp32": "f32 (K1), f64 (rest)",
           "mixed-bf16": "bf16 hi/lo MFMA (K1 part A), f32 (K1 part B), f64 (guard band, rest)"}
 
 LINE_RATE = 15734.264      # NTSC line rate (video_format.cpp)
+PAL_LINE_RATE = 15625.0
 TBC_BYTES = 2 * 2 * 910 * 263  # luma + chroma u16 per field
+PAL_TBC_BYTES = 2 * 2 * 1135 * 313
 
 # The stages a roofline line can be taken on (their stage names in the
 # decoder's profile); the dominant one is the largest serialised time.
@@ -50,9 +54,12 @@ ROOFLINE_STAGES = ("k1_analytic", "k1_video", "k6_prebpf", "k6_chroma")
 ALG_BYTES_PER_FIELD = {28.0: 1424455, 35.8: 1553968, 40.0: 1624655}
 
 
-def alg_bytes_per_field(mhz):
+def alg_bytes_per_field(mhz, system="ntsc"):
     """Compulsory bytes of one field (the table above; for other rates one
-    field period of u8 samples, fs * 262.5 / line rate, plus the planes)."""
+    field period of u8 samples, fs * 262.5 / line rate, plus the planes;
+    PAL: fs * 312.5 / 15625 plus its 1135 x 313 planes)."""
+    if system == "pal":
+        return round(mhz * 1e6 * 312.5 / PAL_LINE_RATE) + PAL_TBC_BYTES
     return ALG_BYTES_PER_FIELD.get(round(mhz, 1), round(mhz * 1e6 * 262.5 / LINE_RATE) + TBC_BYTES)
 
 
@@ -82,6 +89,9 @@ WORKLOADS = {
     "6000f_40mhz": {"config": 2, "mhz": 40.0, "fields": 6000,
                     "desc": "6000 synthetic NTSC fields of u8 RF at 40 MHz (4.0 GB; SNR 25-40 dB, dropouts, "
                             "wow/flutter, 0-4 us head-switch offsets in groups of 1-8 fields)"},
+    "600f_28mhz_pal": {"config": 1, "mhz": 28.0, "fields": 600, "system": "pal",
+                       "desc": "600 synthetic PAL fields of u8 RF at 28 MHz (the 600-field signal model on PAL's "
+                               "312.5-line fields and levels; 1135 x 313 output fields)"},
 }
 
 
@@ -91,13 +101,13 @@ def _oracle():
     return oracle_py
 
 
-def cpu_decode(cap, mhz, batch, threads):
+def cpu_decode(cap, mhz, batch, threads, system="ntsc"):
     """One oracle decode of cap with the given OpenMP thread count: fields/s
     and per-stage ms per field."""
     O = _oracle()
     used = O.set_threads(threads)
     t = time.perf_counter()
-    ref = O.decode(cap, mhz, batch, trace=False)
+    ref = O.decode(cap, mhz, batch, trace=False, system=system)
     dt = time.perf_counter() - t
     n = max(ref["fields"], 1)
     return {"fields": ref["fields"], "seconds": round(dt, 2), "fields_per_s": round(ref["fields"] / dt, 3),
@@ -108,18 +118,19 @@ def host_threads():
     return int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
 
 
-def cpu_baseline(cap, fmt, batch, mhz):
+def cpu_baseline(cap, fmt, batch, mhz, system="ntsc"):
     """The oracle (the reference's stage logic restated in C++, FP64) on the
     host cores: the first two chunks of the workload with every host thread,
     and the 4-field capture with one thread, each with per-stage ms/field."""
     from cuvhs_amd import native
     threads = host_threads()
     sample = cap[: 2 * batch * fmt.spfp]
-    multi = cpu_decode(sample, mhz, batch, threads)
+    multi = cpu_decode(sample, mhz, batch, threads, system)
     one = cpu_decode(native.generate(0), 28.0, batch, 1)
     _oracle().set_threads(threads)
     return {"value": multi["fields_per_s"], "unit": "fields/s", "cores": multi["threads"], "kind": "port",
-            "sample": f"first {multi['fields']} fields of the same {mhz:g} MHz capture ({sample.size} u8 samples), "
+            "sample": f"first {multi['fields']} fields of the same {mhz:g} MHz {system.upper()} capture "
+                      f"({sample.size} u8 samples), "
                       f"oracle liboracle.so (FP64, own FFT), OpenMP over fields, {multi['seconds']} s",
             "stage_ms_per_field": multi["stage_ms_per_field"],
             "one_thread": {"workload": "4-field 28 MHz capture", **one}}
@@ -149,7 +160,7 @@ def cpu_baseline_full(args):
     print(json.dumps(line), flush=True)
 
 
-def cli_end_to_end(cap, mhz, batch, form):
+def cli_end_to_end(cap, mhz, batch, form, system="ntsc"):
     """The `cuvhs` CLI on the capture written to a file: disk read, host to
     device copies, decode, device to host copies and .tbc/.tbc.json writes."""
     import subprocess
@@ -162,7 +173,8 @@ def cli_end_to_end(cap, mhz, batch, form):
         runs = []
         for _ in range(2):   # the first run pays for kernel loading
             t = time.perf_counter()
-            r = subprocess.run([exe, "-f", f"{mhz:g}", "--batch", str(batch), "--precision", form, "--overwrite", inp, out],
+            r = subprocess.run([exe, "--system", system.upper(), "-f", f"{mhz:g}", "--batch", str(batch),
+                                "--precision", form, "--overwrite", inp, out],
                                stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)
             runs.append(time.perf_counter() - t)
             if r.returncode != 0:
@@ -218,10 +230,10 @@ def guard_blocks(dec, d_cap):
     return [sum(c["guard_blocks"][0] for c in ch), sum(c["guard_blocks"][1] for c in ch)]
 
 
-def time_form(native, torch, d_cap, mhz, args, form):
+def time_form(native, torch, d_cap, mhz, args, form, system="ntsc"):
     """One precision form on the same capture and schedule as the headline:
     warmup, K timed decodes, one serialised decode for stage times."""
-    d = native.Decoder(mhz, args.batch, device=0, precision=form, graphs=bool(args.graphs))
+    d = native.Decoder(mhz, args.batch, device=0, precision=form, graphs=bool(args.graphs), system=system)
     n = 0
     for _ in range(args.warmup):
         d.reset()
@@ -299,14 +311,16 @@ def main():
     t = time.perf_counter()
     wl = WORKLOADS[args.workload]
     mhz = wl["mhz"]
+    system = wl.get("system", "ntsc")
     if wl["fields"]:
-        cap = native.generate(wl["config"], n_samples=wl["fields"] * native.format_for(mhz).samples_per_field)
+        cap = native.generate(wl["config"], system=system,
+                              n_samples=wl["fields"] * native.format_for(mhz, system).samples_per_field)
     else:
         cap = native.generate(wl["config"])
     log(f"generated {cap.size} samples in {time.perf_counter() - t:.1f} s")
     dev = torch.device("cuda:0")
     d_cap = torch.from_numpy(cap).to(dev)
-    dec = native.Decoder(mhz, args.batch, device=0, precision=args.form, graphs=bool(args.graphs))
+    dec = native.Decoder(mhz, args.batch, device=0, precision=args.form, graphs=bool(args.graphs), system=system)
     fmt = dec.fmt
     matrix_flops_per_seg = dec.k1_matrix_flops()
     fields = 0
@@ -339,7 +353,7 @@ def main():
     dec.close()
     ms_per_step = dt * 1000.0 / args.steps
     value = fields * args.steps / dt
-    bytes_per_field = alg_bytes_per_field(mhz)
+    bytes_per_field = alg_bytes_per_field(mhz, system)
     log(f"{fields} fields/step, {ms_per_step:.2f} ms/step, {value:.1f} fields/s")
     stage_ms = dict(prof)
     log("per-step stage ms (overlapped): " + ", ".join(f"{k}={v:.3f}" for k, v in stage_ms.items()))
@@ -352,7 +366,7 @@ def main():
     # launch time, from HIP events on the stream it runs on.
     dom = max((k for k in ROOFLINE_STAGES if k in iso), key=lambda k: iso[k])
     launches = chunks                      # one launch per chunk of the profiled decode
-    bpf = alg_bytes_per_field(mhz)
+    bpf = alg_bytes_per_field(mhz, system)
     fields_per_launch = fields / launches
     alg_bytes_per_launch = bpf * fields_per_launch
     avg_launch_s = stage_ms[dom] / 1000.0 / launches
@@ -383,7 +397,7 @@ def main():
     if dom in ("k1_analytic", "k1_video") and args.form == "fp64":
         from cuvhs_amd import native as _n
         segs = k1_samples / fmt.spfp / launches
-        n_fft = _n.k1_fft_size(mhz)
+        n_fft = _n.k1_fft_size(mhz, system)
         flops = segs * k1_fft_flops_per_segment(dom, n_fft)
         roofline_fp64 = {"bound": "fp64", "kernel": dom, "fft_points": n_fft, "segments_per_launch": round(segs, 2),
                          "alg_flops_per_launch": int(flops), "unit": "TFLOP/s", "peak": PEAK_FP64_TFLOPS,
@@ -408,34 +422,35 @@ def main():
     head_report = None
     if args.form != "fp64" and not args.no_forms:
         log(f"{args.form}: comparing with fp64 ...")
-        head_report = precision.compare(d_cap, mhz, args.batch, args.form)
+        head_report = precision.compare(d_cap, mhz, args.batch, args.form, system=system)
     forms = None
     if not args.no_forms:
         forms = {}
         for form in FORMS:
             if form == args.form:
                 continue
-            r = time_form(native, torch, d_cap, mhz, args, form)
+            r = time_form(native, torch, d_cap, mhz, args, form, system)
             log(f"{form} form: {r['value']:.1f} fields/s")
             r["mfma"] = mfma_use(r.pop("matrix_flops_per_segment"), r["stage_ms_per_step_isolated"])
             if form != "fp64":
                 log(f"{form}: comparing with fp64 ...")
-                r["vs_fp64"] = precision.compare(d_cap, mhz, args.batch, form)
+                r["vs_fp64"] = precision.compare(d_cap, mhz, args.batch, form, system=system)
             forms[form] = r
 
     cli = None
     if not args.no_cli and wl["fields"] != 6000:   # the CLI figure would write a 4 GB file first
-        cli = cli_end_to_end(cap, mhz, args.batch, args.form)
+        cli = cli_end_to_end(cap, mhz, args.batch, args.form, system)
         log(f"cli end to end: {cli}")
 
     cpu = None
     if not args.no_cpu_baseline:
         log("running CPU baseline ...")
-        cpu = cpu_baseline(cap, fmt, args.batch, mhz)
+        cpu = cpu_baseline(cap, fmt, args.batch, mhz, system)
         log(f"cpu baseline {cpu['value']:.2f} fields/s on {cpu['cores']} threads")
 
     line = {
-        "metric": METRIC if mhz == 28.0 else METRIC.replace("28 MHz", f"{mhz:g} MHz"),
+        "metric": (METRIC if mhz == 28.0 else METRIC.replace("28 MHz", f"{mhz:g} MHz")).replace(
+            "NTSC", system.upper()),
         "value": round(value, 2),
         "unit": "fields/s",
         "n_gpus": 1,

@@ -147,7 +147,6 @@ struct ChunkCtl {
     // K3 bad/isolated spacing, large jump; K4 large delta, isolated jump,
     // refined sync-like; K5 out-of-bounds pixels, bad rows, sync-like pixels
     int diag[9];
-    int guard_lines;          // lines K4 refined again on FP64 demod_05 (guard band)
 };
 
 // State carried from chunk to chunk; only the line-location, chroma and

@@ -650,7 +650,8 @@ int tail_stage(cuvhs_ctx* c, Slot& s, const OutArrays* out, hipStream_t fork_str
                                  hipMemcpyDeviceToDevice, st));
     }
     mark(c, s, 8, st);
-    HIPCHK(c, run_hsync_refine(s.d05, c->d05_f32, c->ll, s.ctl, 0, s.max_fields, f, st));
+    HIPCHK(c, run_hsync_refine(s.d05, c->d05_f32, c->ll, s.ctl, 0, s.max_fields, f, st,
+                               c->cfg.precision != CUVHS_PRECISION_FP64));
     if (c->diag) HIPCHK(c, run_diag_k4(s.d05, c->ll_coarse, c->ll, s.ctl, s.max_fields, f, st));
     mark(c, s, 9, st);
     // outputs go straight to the caller's arrays where given (the field count
@@ -822,7 +823,8 @@ int seq_tail1(cuvhs_ctx* c, Slot& s, hipStream_t st) {
                                   (const K2bField*)s.summ, (K2bAnchorOut*)c->anchors, c->ps, c->pl, c->pc, c->pt,
                                   c->ll, c->isf_k2b, st));
     }
-    HIPCHK(c, run_hsync_refine(s.d05, c->d05_f32, c->ll, s.ctl, 0, c->F, f, st));
+    HIPCHK(c, run_hsync_refine(s.d05, c->d05_f32, c->ll, s.ctl, 0, c->F, f, st,
+                               c->cfg.precision != CUVHS_PRECISION_FP64));
     return 0;
 }
 int seq_fork(cuvhs_ctx* c, Slot& s, hipStream_t st) {
@@ -977,7 +979,7 @@ int tail_stage_graph(cuvhs_ctx* c, Slot& s) {
 // cuvhs_dropouts_noalloc takes its device scratch (the per-slice envelope
 // sums and the 16-sample block classes of K7, the buffers a context
 // allocates for cuvhs_dropouts) from this file-scope array instead of
-// hipMalloc: kNoAllocMaxFields fields, 5.9 MB, held on every device the
+// hipMalloc: kNoAllocMaxFields fields, 7.0 MB, held on every device the
 // library runs on until the process ends.  Host code never uses the array's
 // name as an address: the device address is taken once per device with
 // hipGetSymbolAddress (with that device current) and kept in
@@ -985,7 +987,7 @@ int tail_stage_graph(cuvhs_ctx* c, Slot& s) {
 // overlap on one device race; the kernels write every scratch element they
 // read in the same call, so nothing needs clearing between calls.
 constexpr int kNoAllocMaxFields = 128;
-constexpr size_t kNoAllocBytes = 5898240;  // 128 fields x (45,056 class bytes + 128 partial sums x 8 B)
+constexpr size_t kNoAllocBytes = 6979584;  // 128 fields x (53,504 class bytes + 128 partial sums x 8 B)
 constexpr int kNoAllocDevices = 64;
 static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
 std::mutex g_noalloc_mu;

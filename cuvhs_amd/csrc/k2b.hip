@@ -713,12 +713,15 @@ __global__ void k_k2b_chain(ChunkCtl* ctl, DecodeState* dst, int nf_override, co
     ctl->vp_overflow = ovf;
 }
 
-// Line locations + pulse outputs, one workgroup (256 threads) per field.
+// Line locations + pulse outputs, one workgroup (256 threads) per field;
+// LPF line locations per frame (525 NTSC, 625 PAL).
+template <int LPF>
 __global__ __launch_bounds__(256) void k_k2b_linelocs(const ChunkCtl* ctl, int nf_override, const K2bField* summ,
                                                       const K2bAnchorOut* anchors, const int* g_vp_start,
                                                       const int* g_vp_len, const unsigned char* g_vp_tg,
                                                       int* out_starts, int* out_lens, int* out_count,
-                                                      int* out_types, double* linelocs, int lpf) {
+                                                      int* out_types, double* linelocs) {
+    constexpr int lpf = LPF;
     __shared__ int key[kVpCap];
     const int f = blockIdx.x;
     const int nf = nf_override > 0 ? nf_override : ctl->nf;
@@ -785,11 +788,11 @@ __global__ __launch_bounds__(256) void k_k2b_linelocs(const ChunkCtl* ctl, int n
     // end clamped to n-2, if within tolerance.  Wave 0 finds g for every line
     // by binary search (lane j: lines j + 64 k), then carries the cursor
     // through the frame's lines reading those results from the lanes.
-    __shared__ int s_pick[kMaxLinesPerFrame];
+    __shared__ int s_pick[LPF];
     const double max_allowed = mll / 1.5;
     if (threadIdx.x < 64) {
         const int lane = threadIdx.x;
-        constexpr int K = (kMaxLinesPerFrame + 63) / 64;
+        constexpr int K = (LPF + 63) / 64;
         int g[K];
         int ok[K];
 #pragma unroll
@@ -866,9 +869,14 @@ hipError_t run_k2b_cadence(const ChunkCtl* ctl_dev, ChunkCtl* ctl_mut, DecodeSta
     hipLaunchKernelGGL(k_k2b_chain, dim3(1), dim3(64), 0, st, ctl_mut, dst, nf_override, offsets, summ, vp_start, vp_tg,
                        anchors, is_first, fmt.num_eq_pulses, fmt.field_lines_first, fmt.field_lines_second,
                        fmt.system == kSystemPal);
-    hipLaunchKernelGGL(k_k2b_linelocs, dim3(max_fields), dim3(256), 0, st, ctl_dev, nf_override, summ, anchors,
-                       vp_start, vp_len, vp_tg, out_starts, out_lens, out_count, out_types, linelocs,
-                       fmt.lines_per_frame);
+    if (fmt.lines_per_frame == 625)
+        hipLaunchKernelGGL(k_k2b_linelocs<625>, dim3(max_fields), dim3(256), 0, st, ctl_dev, nf_override, summ,
+                           anchors, vp_start, vp_len, vp_tg, out_starts, out_lens, out_count, out_types, linelocs);
+    else if (fmt.lines_per_frame == 525)
+        hipLaunchKernelGGL(k_k2b_linelocs<525>, dim3(max_fields), dim3(256), 0, st, ctl_dev, nf_override, summ,
+                           anchors, vp_start, vp_len, vp_tg, out_starts, out_lens, out_count, out_types, linelocs);
+    else
+        return hipErrorInvalidValue;
     return hipGetLastError();
 }
 

@@ -33,9 +33,17 @@ __device__ __forceinline__ double wv_find_falling(const S* sig, int start, int c
     return -1.0;
 }
 
+// quirk: the reference's interpolation guard (a + b != 0, hsync_refine.cu:
+// 63-64), which returns the sample itself when the two samples lie
+// symmetrically about the target.  In FP64 that has measure zero; on the
+// reduced-precision forms' FP32-quantized demod_05 it happens whenever the
+// target is representable (PAL's 3,960,000 Hz threshold) and the samples
+// straddle it by equal steps, a half-sample jump the FP64 values never take.
+// Those forms therefore interpolate without it (quirk = false), the
+// continuous value the FP64 form gives.
 template <typename S>
 __device__ __forceinline__ double wv_find_rising(const S* sig, int start, int count, double target, int total,
-                                                 int lane) {
+                                                 int lane, bool quirk = true) {
     if (start < 0 || start + count > total || count < 2) return -1.0;
     if ((double)sig[start] > target) return -1.0;
     for (int b = start + 1; b < start + count; b += 64) {
@@ -46,7 +54,7 @@ __device__ __forceinline__ double wv_find_rising(const S* sig, int start, int co
         if (m) {
             int f = b + __ffsll((long long)m) - 1;   // hsync_refine.cu:57-67
             double a = target - (double)sig[f - 1], bb = target - (double)sig[f];
-            double frac = (a + bb != 0.0) ? (a / (a - bb)) : 0.0;
+            double frac = (quirk ? (a + bb != 0.0) : (a - bb != 0.0)) ? (a / (a - bb)) : 0.0;
             return (double)(f - 1) + frac;
         }
     }
@@ -84,7 +92,7 @@ template <typename S>
 __global__ __launch_bounds__(256) void k_hsync_refine(const S* __restrict__ d05, double* __restrict__ ll,
                                                       const ChunkCtl* ctl, int nf_override, int total_per_field,
                                                       double thr, double one_usec_samples, int als, double hw,
-                                                      double right_edge_offset, int lpf) {
+                                                      double right_edge_offset, int lpf, int quirk) {
     __shared__ __align__(16) double s_win[4][64];
     const int nf = nf_override > 0 ? nf_override : ctl->nf;
     const int lane = threadIdx.x & 63;
@@ -117,7 +125,7 @@ __global__ __launch_bounds__(256) void k_hsync_refine(const S* __restrict__ d05,
     double right = -1.0;
     const int rs0 = (int)(original + hw - one_usec_samples);
     const int rc0 = (int)(hw * 2);
-    double rcross = wv_find_rising(d05, rs0, rc0, thr, total, lane);
+    double rcross = wv_find_rising(d05, rs0, rc0, thr, total, lane, quirk != 0);
     if (rcross >= 0.0) {
         double zc_fr = rcross - hw;
         int zfi = (int)(zc_fr + 0.5);
@@ -127,7 +135,7 @@ __global__ __launch_bounds__(256) void k_hsync_refine(const S* __restrict__ d05,
                                          total, lane, wl);
         if (rporch > rsync) {
             double rmid = (rsync + rporch) / 2.0;
-            double zc2r = wv_find_rising(d05, rs0, rc0, rmid, total, lane);
+            double zc2r = wv_find_rising(d05, rs0, rc0, rmid, total, lane, quirk != 0);
             if (zc2r >= 0.0 && fabs(zc2r - rcross) <= one_usec_samples * 0.5) {
                 double cand = rcross - hw + right_edge_offset;
                 double ref = (left >= 0.0) ? left : original;
@@ -188,18 +196,19 @@ __global__ __launch_bounds__(256) void k_tbc_resample(const S* __restrict__ demo
 }
 
 hipError_t run_hsync_refine(const void* d05, int f32, double* ll, const ChunkCtl* ctl, int nf_override,
-                            int max_fields, const Fmt& fmt, hipStream_t st) {
+                            int max_fields, const Fmt& fmt, hipStream_t st, int reduced) {
+    const int quirk = reduced ? 0 : 1;
     int n = max_fields * fmt.lines_per_frame;
     double ous = 1.0e-6 * fmt.sample_rate;
     double reo = 2.25 * ((fmt.sample_rate / 1.0e6) / 40.0);
     if (f32)
         hipLaunchKernelGGL(k_hsync_refine<float>, dim3((n + 3) / 4), dim3(256), 0, st, static_cast<const float*>(d05),
                            ll, ctl, nf_override, fmt.spfp, fmt.pulse_threshold_hz, ous, fmt.active_line_start,
-                           fmt.hsync_width, reo, fmt.lines_per_frame);
+                           fmt.hsync_width, reo, fmt.lines_per_frame, quirk);
     else
         hipLaunchKernelGGL(k_hsync_refine<double>, dim3((n + 3) / 4), dim3(256), 0, st,
                            static_cast<const double*>(d05), ll, ctl, nf_override, fmt.spfp, fmt.pulse_threshold_hz,
-                           ous, fmt.active_line_start, fmt.hsync_width, reo, fmt.lines_per_frame);
+                           ous, fmt.active_line_start, fmt.hsync_width, reo, fmt.lines_per_frame, quirk);
     return hipGetLastError();
 }
 

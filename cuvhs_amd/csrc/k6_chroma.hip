@@ -27,17 +27,26 @@
 namespace cuvhs {
 
 constexpr int kSkip = 16;          // chroma_decode.cu:255
-// line pairs of the burst metric: (field lines - 2 kSkip - 1) / 2, 115 (NTSC)
-// or 140 (PAL)
-constexpr int kMaxPairs = (kMaxFieldLines - 2 * kSkip - 1) / 2;
-__device__ __forceinline__ int metric_pairs(int field_lines) { return (field_lines - 2 * kSkip - 1) / 2; }
 
-// Resampled sample of one line (chroma_decode.cu:75-137), before the
-// heterodyne.
+// The geometry the chroma kernels are compiled for (the launcher checks it
+// against the launch's): samples per output line, burst window samples,
+// output lines per field, line locations per frame, line transform points.
+template <int LL_, int BL_, int FL_, int LPF_, int NF_>
+struct ChromaGeo {
+    static constexpr int LL = LL_, BL = BL_, FL = FL_, LPF = LPF_, NF = NF_;
+    // line pairs of the burst metric (chroma_decode.cu:383-411)
+    static constexpr int PAIRS = (FL_ - 2 * kSkip - 1) / 2;
+};
+using NtscChroma = ChromaGeo<910, 36, 263, 525, 1024>;     // 115 pairs
+using PalChroma = ChromaGeo<1135, 40, 313, 625, 2048>;     // 140 pairs
+
+// Resampled sample of one line of LL samples (chroma_decode.cu:75-137),
+// before the heterodyne.
+template <int LL>
 __device__ __forceinline__ double chroma_resample(const ChromaLaunch& a, double ls, double le, int col,
                                                  long long total) {
-    if (col >= a.line_len) return 0.0;
-    double frac = (double)col / (double)a.line_len;
+    if (col >= LL) return 0.0;
+    double frac = (double)col / (double)LL;
     double coord = ls + frac * (le - ls);
     int ci = (int)coord;
     double x = coord - (double)ci;
@@ -55,9 +64,10 @@ __device__ __forceinline__ double chroma_resample(const ChromaLaunch& a, double 
 // next of them by the fixed rotation exp(i 2 pi het_scale 64) (het_dc,
 // het_ds).  The rotation differs from a direct cos by ~1e-14 relative; the
 // reference's own FP64 argument, up to 4.4e5 rad, is rounded at ~5e-11 rad.
+template <int LL>
 __device__ __forceinline__ void het_start(const ChromaLaunch& a, int line, double phase_offset, double* s,
                                           double* c) {
-    sincos(2.0 * M_PI * a.het_scale * (double)(line * a.line_len + (int)threadIdx.x) + phase_offset, s, c);
+    sincos(2.0 * M_PI * a.het_scale * (double)(line * LL + (int)threadIdx.x) + phase_offset, s, c);
 }
 __device__ __forceinline__ void het_step(const ChromaLaunch& a, double* s, double* c) {
     const double cn = *c * a.het_dc - *s * a.het_ds;
@@ -84,11 +94,12 @@ __device__ __forceinline__ void het_step(const ChromaLaunch& a, double* s, doubl
 // reach 46 M samples, the heterodyne argument 4.4e5 rad); T = float is the
 // reduced-precision forms' line filter, with whole complex elements in LDS
 // (the same 8.7 KB as the FP64 filter's real and imaginary parts in turn).
-template <typename T, int NF>
+template <typename T, typename G>
 __device__ __forceinline__ void chroma_line(const ChromaLaunch& a, void* buf, const typename C2<T>::type* tw,
                                             const typename C2<T>::type* tw2) {
     using V = typename C2<T>::type;
     constexpr bool whole = sizeof(T) == 4;
+    constexpr int NF = G::NF, LL = G::LL, BL = G::BL;
     constexpr int R = NF / 64;
     static_assert(NF == 1024 || NF == 2048, "line transform size");
     const int f = blockIdx.y;
@@ -99,26 +110,26 @@ __device__ __forceinline__ void chroma_line(const ChromaLaunch& a, void* buf, co
     const int lb = la + 1;
     const bool has_b = lb < a.line_end;
     const long long total = (long long)nf * a.spfp;
-    const double* ll = a.linelocs + (size_t)f * a.lpf;
+    const double* ll = a.linelocs + (size_t)f * G::LPF;
     const int rot = 3;  // track 0
     V v[R];
     {
         const double ls = ll[la], le = ll[la + 1];
         double s, c;
-        het_start(a, la, (double)((la * rot + 0) & 3) * (M_PI * 0.5), &s, &c);
+        het_start<LL>(a, la, (double)((la * rot + 0) & 3) * (M_PI * 0.5), &s, &c);
 #pragma unroll
         for (int r = 0; r < R; r++) {
-            v[r].x = (T)(chroma_resample(a, ls, le, (int)threadIdx.x + 64 * r, total) * -c);
+            v[r].x = (T)(chroma_resample<LL>(a, ls, le, (int)threadIdx.x + 64 * r, total) * -c);
             het_step(a, &s, &c);
         }
     }
     if (has_b) {
         const double ls = ll[lb], le = ll[lb + 1];
         double s, c;
-        het_start(a, lb, (double)((lb * rot + 0) & 3) * (M_PI * 0.5), &s, &c);
+        het_start<LL>(a, lb, (double)((lb * rot + 0) & 3) * (M_PI * 0.5), &s, &c);
 #pragma unroll
         for (int r = 0; r < R; r++) {
-            v[r].y = (T)(chroma_resample(a, ls, le, (int)threadIdx.x + 64 * r, total) * -c);
+            v[r].y = (T)(chroma_resample<LL>(a, ls, le, (int)threadIdx.x + 64 * r, total) * -c);
             het_step(a, &s, &c);
         }
     } else {
@@ -155,56 +166,57 @@ __device__ __forceinline__ void chroma_line(const ChromaLaunch& a, void* buf, co
         }
     }
     const T scale = (T)a.fft_scale;
-    double* da = a.burst + ((size_t)f * a.field_lines + la) * a.burst_len;
-    float* fa = a.filt + ((size_t)f * a.field_lines + la) * a.line_len;
+    double* da = a.burst + ((size_t)f * G::FL + la) * BL;
+    float* fa = a.filt + ((size_t)f * G::FL + la) * LL;
 #pragma unroll
     for (int r = 0; r < R; r++) {
         const int col = (int)threadIdx.x + 64 * r;
         const T xa = v[r].x * scale, xb = v[r].y * scale;
-        if (col < a.line_len) {
+        if (col < LL) {
             fa[col] = (float)xa;
-            if (has_b) fa[a.line_len + col] = (float)xb;
+            if (has_b) fa[LL + col] = (float)xb;
         }
         const int bi = col - a.burst_start;
-        if (bi >= 0 && bi < a.burst_len) {
+        if (bi >= 0 && bi < BL) {
             da[bi] = (double)xa;
-            if (has_b) da[a.burst_len + bi] = (double)xb;
+            if (has_b) da[BL + bi] = (double)xb;
         }
     }
 }
 
 __global__ __launch_bounds__(64, 3) void k_chroma_line(ChromaLaunch a) {
     __shared__ __align__(16) double buf[lds_elems(1024)];
-    chroma_line<double, 1024>(a, buf, a.tw, nullptr);
+    chroma_line<double, NtscChroma>(a, buf, a.tw, nullptr);
 }
 __global__ __launch_bounds__(64, 4) void k_chroma_line_f32(ChromaLaunch a) {
     __shared__ __align__(16) float2 buf[lds_elems(1024)];
-    chroma_line<float, 1024>(a, buf, a.tw_f, nullptr);
+    chroma_line<float, NtscChroma>(a, buf, a.tw_f, nullptr);
 }
 // PAL: 2048-point line transforms (twice the registers per lane)
 __global__ __launch_bounds__(64) void k_chroma_line2048(ChromaLaunch a) {
     __shared__ __align__(16) double buf[lds_elems(1024)];
-    chroma_line<double, 2048>(a, buf, a.tw, a.tw2);
+    chroma_line<double, PalChroma>(a, buf, a.tw, a.tw2);
 }
 __global__ __launch_bounds__(64) void k_chroma_line2048_f32(ChromaLaunch a) {
     __shared__ __align__(16) float2 buf[lds_elems(1024)];
-    chroma_line<float, 2048>(a, buf, a.tw_f, a.tw2_f);
+    chroma_line<float, PalChroma>(a, buf, a.tw_f, a.tw2_f);
 }
 
 // ACC + u16 of every line (chroma_decode.cu:188-231) with the resolved track:
 // an odd line of track 1 is the negated track-0 line.  One workgroup per
 // (line, field).
+template <typename G>
 __global__ __launch_bounds__(256) void k_chroma_out(ChromaLaunch a) {
     const int f = blockIdx.y, line = blockIdx.x;
     const int nf = a.nf_override > 0 ? a.nf_override : a.ctl->nf;
     if (f >= nf) return;
-    const size_t row = (size_t)f * a.field_lines + line;
+    const size_t row = (size_t)f * G::FL + line;
     // burst RMS (chroma_decode.cu:205-221, whose atomic sum has no fixed
     // order; here, as in the oracle, sample order): every wavefront loads the
     // 36 (NTSC) or 40 (PAL) burst samples at once and sums their squares
     // through lane reads, so no lane waits on a workgroup barrier
     const int lane = (int)threadIdx.x & 63;
-    const int bl = a.burst_len;
+    constexpr int bl = G::BL;
     const double* da = a.burst + row * bl;
     const double bv = lane < bl ? da[lane] : 0.0;
     double ssq = 0.0;
@@ -215,9 +227,9 @@ __global__ __launch_bounds__(256) void k_chroma_out(ChromaLaunch a) {
     const double rms = sqrt(ssq / (double)bl);
     const double s_acc = (rms > 1.0) ? (a.burst_abs_ref / rms) : 0.0;
     const double acc = ((line & 1) && a.track[f]) ? -s_acc : s_acc;
-    const float* fa = a.filt + row * a.line_len;
-    uint16_t* o = (a.plan ? a.plan->chroma : a.out) + row * a.line_len;
-    for (int col = threadIdx.x; col < a.line_len; col += 256) {
+    const float* fa = a.filt + row * G::LL;
+    uint16_t* o = (a.plan ? a.plan->chroma : a.out) + row * G::LL;
+    for (int col = threadIdx.x; col < G::LL; col += 256) {
         double ov = (double)fa[col] * acc + 32768.0;
         if (ov < 0.0) ov = 0.0;
         if (ov > 65535.0) ov = 65535.0;
@@ -228,14 +240,14 @@ __global__ __launch_bounds__(256) void k_chroma_out(ChromaLaunch a) {
 // Per-field burst-cancellation metrics for both tracks and, for field 0,
 // per-line burst I/Q.  One wavefront per field: lanes evaluate pairs, lane 0
 // accumulates them in pair order (chroma_decode.cu:383-411, :415-450).
+template <typename G>
 __global__ __launch_bounds__(64) void k_chroma_metrics(const double* burst, const ChunkCtl* ctl, int nf_override,
-                                                       double* m0, double* m1, const double* cs_tab, double* iq0,
-                                                       int field_lines, int bl) {
-    __shared__ double s0[kMaxPairs], s1[kMaxPairs];
+                                                       double* m0, double* m1, const double* cs_tab, double* iq0) {
+    constexpr int field_lines = G::FL, bl = G::BL, pairs = G::PAIRS;
+    __shared__ double s0[pairs], s1[pairs];
     const int nf = nf_override > 0 ? nf_override : ctl->nf;
     const int f = blockIdx.x;
     if (f >= nf) return;
-    const int pairs = metric_pairs(field_lines);
     for (int p = threadIdx.x; p < pairs; p += 64) {
         const double* ra = burst + ((size_t)f * field_lines + kSkip + 2 * p) * bl;
         const double* rb = ra + bl;
@@ -389,27 +401,35 @@ hipError_t run_chroma(ChromaLaunch a, DecodeState* dst, int max_fields, double* 
                       const double* cs_tab, double* iq0, int* out_track, int* phase_ids, int need_iq,
                       hipStream_t st) {
     // band-pass of every line with track 0; metrics from lines 16 .. field
-    // lines - 17
-    if (a.nfft != 1024 && a.nfft != 2048) return hipErrorInvalidValue;
-    if (a.burst_len > kMaxBurstLen || a.field_lines > kMaxFieldLines) return hipErrorInvalidValue;
+    // lines - 17.  The kernels are compiled for NTSC's and PAL's geometry.
+    auto is = [&](auto g) {
+        using G = decltype(g);
+        return a.nfft == G::NF && a.line_len == G::LL && a.burst_len == G::BL && a.field_lines == G::FL &&
+               a.lpf == G::LPF;
+    };
+    const bool pal = is(PalChroma{});
+    if (!pal && !is(NtscChroma{})) return hipErrorInvalidValue;
     ChromaLaunch m = a;
     m.line0 = 0;
     m.line_end = a.field_lines;
     const dim3 lg((m.line_end - m.line0 + 1) / 2, max_fields);
-    if (a.nfft == 2048) {
+    if (pal) {
         if (a.f32) hipLaunchKernelGGL(k_chroma_line2048_f32, lg, dim3(64), 0, st, m);
         else hipLaunchKernelGGL(k_chroma_line2048, lg, dim3(64), 0, st, m);
+        hipLaunchKernelGGL(k_chroma_metrics<PalChroma>, dim3(max_fields), dim3(64), 0, st, a.burst, a.ctl,
+                           a.nf_override, m0, m1, cs_tab, need_iq ? iq0 : nullptr);
     } else {
         if (a.f32) hipLaunchKernelGGL(k_chroma_line_f32, lg, dim3(64), 0, st, m);
         else hipLaunchKernelGGL(k_chroma_line, lg, dim3(64), 0, st, m);
+        hipLaunchKernelGGL(k_chroma_metrics<NtscChroma>, dim3(max_fields), dim3(64), 0, st, a.burst, a.ctl,
+                           a.nf_override, m0, m1, cs_tab, need_iq ? iq0 : nullptr);
     }
-    hipLaunchKernelGGL(k_chroma_metrics, dim3(max_fields), dim3(64), 0, st, a.burst, a.ctl,
-                       a.nf_override, m0, m1, cs_tab, need_iq ? iq0 : nullptr, a.field_lines, a.burst_len);
     hipLaunchKernelGGL(k_chroma_resolve, dim3(1), dim3(64), 0, st, a.ctl, dst, a.nf_override, m0, m1, iq0, out_track,
                        phase_ids, a.plan, a.field_lines);
     ChromaLaunch o = a;
     o.track = out_track;
-    hipLaunchKernelGGL(k_chroma_out, dim3(a.field_lines, max_fields), dim3(256), 0, st, o);
+    if (pal) hipLaunchKernelGGL(k_chroma_out<PalChroma>, dim3(a.field_lines, max_fields), dim3(256), 0, st, o);
+    else hipLaunchKernelGGL(k_chroma_out<NtscChroma>, dim3(a.field_lines, max_fields), dim3(256), 0, st, o);
     return hipGetLastError();
 }
 

@@ -230,9 +230,11 @@ hipError_t run_k2b_cadence(const ChunkCtl* ctl_dev, ChunkCtl* ctl_mut, DecodeSta
                            int* is_first, hipStream_t st);
 size_t k2b_field_summary_bytes();
 size_t k2b_anchor_bytes();
-// f32: the signal is stored as float (reduced-precision forms), else double
+// f32: the signal is stored as float (reduced-precision forms), else double;
+// reduced: demod_05 comes from a reduced-precision K1 (FP32-quantized), so
+// the rising-edge interpolation drops the reference's symmetric-sample guard
 hipError_t run_hsync_refine(const void* d05, int f32, double* ll, const ChunkCtl* ctl, int nf_override,
-                            int max_fields, const Fmt& fmt, hipStream_t st);
+                            int max_fields, const Fmt& fmt, hipStream_t st, int reduced = 0);
 hipError_t run_tbc_resample(const void* demod, int f32, const double* ll, uint16_t* luma, const ChunkCtl* ctl,
                             int nf_override, int max_fields, const Fmt& fmt, hipStream_t st,
                             const ChunkPlan* plan = nullptr);

@@ -53,12 +53,12 @@ def _err(a, b):
     return {"max": float(d.max()), "rms": float(d.square().mean().sqrt())}
 
 
-def compare(d_capture, sample_rate_mhz=28.0, batch_fields=64, form="fp32", ref=None):
+def compare(d_capture, sample_rate_mhz=28.0, batch_fields=64, form="fp32", ref=None, system="ntsc"):
     """Decode d_capture (device u8 tensor) in `form` and in FP64; return the
     report dict.  ref: an FP64 decode_chunked result to reuse."""
     import torch
-    dec64 = N.Decoder(sample_rate_mhz, batch_fields, precision="fp64")
-    dec = N.Decoder(sample_rate_mhz, batch_fields, precision=form)
+    dec64 = N.Decoder(sample_rate_mhz, batch_fields, precision="fp64", system=system)
+    dec = N.Decoder(sample_rate_mhz, batch_fields, precision=form, system=system)
     try:
         r = ref if ref is not None else dec64.decode_chunked(d_capture)
         o = dec.decode_chunked(d_capture)

@@ -145,7 +145,9 @@ typedef struct cuvhs_format {
 int cuvhs_create(const cuvhs_config* cfg, cuvhs_ctx** out);
 int cuvhs_destroy(cuvhs_ctx* ctx);
 int cuvhs_get_format(const cuvhs_ctx* ctx, cuvhs_format* out);
-int cuvhs_format_for(double sample_rate_mhz, cuvhs_format* out);   /* NTSC */
+/* VideoFormat::VideoFormat (video_format.cpp:23-104) for NTSC, and for a
+ * given system (PAL: video_format.cpp:48-87) */
+int cuvhs_format_for(double sample_rate_mhz, cuvhs_format* out);
 int cuvhs_format_for_system(int system, double sample_rate_mhz, cuvhs_format* out);
 /* Clears the state carried across chunks (K2b cadence, ChromaState, field
  * parity: pipeline.h:93-97, chroma_decode.h:9-15, pipeline.cu:2575). */
@@ -245,7 +247,7 @@ int cuvhs_dropouts(cuvhs_ctx* ctx, const double* d_env, const double* d_linelocs
  * in the reference: for callers that cannot allocate device memory, such as
  * code under HIP graph capture or tight serving loops).  Same inputs,
  * outputs and results as cuvhs_dropouts for a capture at sample_rate_mhz;
- * the kernels' scratch is a file-scope device array of 5.9 MB (held on each
+ * the kernels' scratch is a file-scope device array of 7.0 MB (held on each
  * device until the process ends) that holds at most 128 fields: more return
  * CUVHS_E_CAPACITY, with the reason in cuvhs_last_error(NULL).  Enqueues on
  * `stream` (hipStream_t, NULL = the current device's default stream) and

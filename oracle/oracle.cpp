@@ -19,7 +19,12 @@
 // locations, is_first) and the chroma burst metric, burst phase and NTSC
 // phase table (chroma_decode.cu:329-450) bit for bit against the reference's
 // own host functions, compiled as they lie (oracle/ref_k2b_driver.cpp), on
-// seven degraded captures (tests/test_reference_k2b.py); FFT against numpy
+// seven degraded NTSC captures and four PAL captures, the PAL format and
+// writer included (tests/test_reference_k2b.py, tests/test_reference_host.py:
+// system 1 runs the reference's PAL branches, video_format.cpp:48-87,
+// pipeline.cu:707-717, 811); K3 (NTSC and PAL parity rules,
+// line_locs.cu:219-235) on ideal pulse trains of both field orders
+// (tests/test_oracle_k3.py); FFT against numpy
 // and brute-force DFTs; filter design against scipy.signal; demod, VSYNC, K2,
 // K4, K5 by closed forms and brute force; K7 by RF dips placed at known
 // samples.  Pinned by invariants: the VSYNC clustering (field offsets one

@@ -1,8 +1,9 @@
 """GPU K2b line location (cuvhs_linelocs) against the reference's own host K2b.
 
-The vectors in tests/golden/ref_k2b_*.npz hold, for seven degraded captures
-(28 / 35.8 / 40 MHz, SNR down to 18 dB, eight dropouts per field, a field cut
-out of the tape), the K2 pulse lists of each 64-field chunk and what the
+The vectors in tests/golden/ref_k2b_*.npz hold, for seven degraded NTSC
+captures (28 / 35.8 / 40 MHz, SNR down to 18 dB, eight dropouts per field, a
+field cut out of the tape) and four PAL captures (the reference's PAL
+branches, pipeline.cu:707-717, 811), the K2 pulse lists of each 64-field chunk and what the
 reference's host K2b (pipeline.cu:133-148, 410-893, compiled as it lies; see
 tests/golden/make_ref_k2b_fixtures.py) returns for them, with the cadence
 state carried across chunks.  The HIP K2b gets the same pulses chunk after
@@ -39,9 +40,11 @@ def test_gpu_k2b_equals_reference(torch, path):
     from cuvhs_amd import native as N
     z = np.load(path)
     mhz = float(z["mhz"])
+    system = "pal" if "system" in z.files and int(z["system"]) == 1 else "ntsc"
+    lpf = N.GEOMETRY[system][0]
     chunks = FX.unpack_k2b(z)
-    print(f"case {os.path.basename(path)}: {len(chunks)} chunks at {mhz} MHz", flush=True)
-    dec = N.Decoder(mhz, int(z["batch"]))
+    print(f"case {os.path.basename(path)}: {len(chunks)} chunks at {mhz} MHz, {system}", flush=True)
+    dec = N.Decoder(mhz, int(z["batch"]), system=system)
     try:
         for ci, c in enumerate(chunks):
             nf = c["nf"]
@@ -51,7 +54,7 @@ def test_gpu_k2b_equals_reference(torch, path):
             pl = dev(c["k2_lens"].reshape(-1))
             pc = dev(c["k2_count"])
             pt = torch.zeros_like(ps)
-            ll = torch.zeros(nf * 525, dtype=torch.float64, device="cuda:0")
+            ll = torch.zeros(nf * lpf, dtype=torch.float64, device="cuda:0")
             isf = torch.zeros(nf, dtype=torch.int32, device="cuda:0")
             rc = N.lib().cuvhs_linelocs(dec.h, c["raw_offset"], N.ptr(offs), nf, N.ptr(ps), N.ptr(pl), N.ptr(pc),
                                         N.ptr(pt), N.ptr(ll), N.ptr(isf))
@@ -64,7 +67,7 @@ def test_gpu_k2b_equals_reference(torch, path):
                 a = t.cpu().numpy().reshape(nf, 800)
                 for f in range(nf):
                     assert np.array_equal(a[f, :cnt[f]], e[k][f, :cnt[f]]), (ci, f, k)
-            assert np.array_equal(ll.cpu().numpy().reshape(nf, 525), e["linelocs"]), ci
+            assert np.array_equal(ll.cpu().numpy().reshape(nf, lpf), e["linelocs"]), ci
             assert np.array_equal(isf.cpu().numpy(), e["is_first"]), ci
     finally:
         dec.close()

@@ -1,7 +1,7 @@
 """The oracle's K3 line location (line_locs.cu) and the reference's
 K3/K4/K5 debug counters, pinned on constructed inputs whose answers follow
-from the algorithm's definitions: ideal NTSC pulse trains of both field
-parities, and line grids with planted spacing faults counted by hand."""
+from the algorithm's definitions: ideal NTSC and PAL pulse trains of both
+field parities, and line grids with planted spacing faults counted by hand."""
 import numpy as np
 import pytest
 
@@ -55,6 +55,63 @@ def test_k3_ideal_field(oracle, first):
     # lines past the pulse list continue the cadence
     if 10 + m < 525:
         assert np.allclose(np.diff(ll[0, 10 + m - 1:]), SPL)
+
+
+PAL_SPL, PAL_SPF = 1792, 560896
+PAL_VW = 764                      # 27.3 us at 28 MHz, rounded
+
+
+def ideal_pal_field(first, lead_lines=5, start=3000):
+    """K2 pulse list of one PAL field window: lead_lines HSYNCs, then 5 EQ,
+    5 broad and 5 EQ pulses at half-line spacing, then HSYNCs.  The first EQ
+    follows the last lead HSYNC by 1/2 line (first field) or 1 line, and the
+    first HSYNC follows the last EQ by the same (PAL's field order,
+    pipeline.cu:707-717)."""
+    step = 0 if first else PAL_SPL // 2
+    starts, lens = [], []
+    p = start
+    for _ in range(lead_lines):
+        starts.append(p)
+        lens.append(HW)
+        p += PAL_SPL
+    p += step - PAL_SPL // 2
+    for w in [EQW] * 5 + [PAL_VW] * 5 + [EQW] * 5:
+        starts.append(p)
+        lens.append(w)
+        p += PAL_SPL // 2
+    p += step
+    first_h = len(starts)
+    while p < start + PAL_SPF:
+        starts.append(p)
+        lens.append(HW)
+        p += PAL_SPL
+    st = np.zeros((1, 800), np.int32)
+    ln = np.zeros((1, 800), np.int32)
+    st[0, :len(starts)] = starts
+    ln[0, :len(lens)] = lens
+    return st, ln, np.array([len(starts)], np.int32), first_h
+
+
+@pytest.mark.parametrize("first", [True, False])
+def test_k3_ideal_pal_field(oracle, first):
+    """PAL's parity rule is NTSC's inverted (line_locs.cu:219-235): a short
+    entry spacing (1/2 line) is a first field; line 10 is the first HSYNC
+    after the vblank and the 625 line locations follow the cadence."""
+    st, ln, ct, first_h = ideal_pal_field(first)
+    types, ll, isf = oracle.k3_linelocs(st, ln, ct, MHZ, system="pal")
+    assert isf[0] == (1 if first else 0)
+    n = ct[0]
+    exp_types = np.where(ln[0, :n] == PAL_VW, 2, np.where(ln[0, :n] == EQW, 1, 0))
+    assert np.array_equal(types[0, :n], exp_types)
+    hs = st[0, first_h:n]
+    m = min(hs.size, 625 - 10)
+    assert ll.shape == (1, 625)
+    assert np.array_equal(ll[0, 10:10 + m], hs[:m].astype(float))
+    if 10 + m < 625:
+        assert np.allclose(np.diff(ll[0, 10 + m - 1:]), PAL_SPL)
+    # the same pulses read as NTSC take NTSC's rule: the parity flips
+    _, _, isf_n = oracle.k3_linelocs(st, ln, ct, MHZ)
+    assert isf_n[0] == (0 if first else 1)
 
 
 def test_k3_without_vblank_uses_first_hsync(oracle):

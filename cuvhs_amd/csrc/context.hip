@@ -132,6 +132,7 @@ struct cuvhs_ctx {
     // mixed forms: part A tap operands, and the FP64 guard band
     uint4* m_taps = nullptr;
     int m_nc = 0, m_nx = 0, m_nblk = 0;
+    int m_glo0 = 0, m_glo1 = 0;        // groups of four tap steps run with their lo parts
     double m_scale = 0.0;
     K1Design k1g;              // the FP64 design the guard band recomputes with
     void *G_ga = nullptr, *G_gb = nullptr, *tw_ga = nullptr, *tw_gb = nullptr;
@@ -366,6 +367,8 @@ int k1_launch(cuvhs_ctx* c, Slot& s, const void* chunk, long long n_valid, int n
         m.lp = c->k1.rf.lp;
         m.nc = c->m_nc;
         m.nx = c->m_nx;
+        m.glo0 = c->m_glo0;
+        m.glo1 = c->m_glo1;
         m.a_lds_offset = mfma_a_lds_offset(c->m_nx);
         m.scale = (float)c->m_scale;
         m.fs_over_tau = (float)fs_tau;
@@ -1203,7 +1206,8 @@ int cuvhs_create(const cuvhs_config* cfg, cuvhs_ctx** out) {
         if ((e = upload_twiddles(c, &c->tw_ga, c->k1g.rf.B, 0)) != hipSuccess) return bad(e, "twiddles");
         if ((e = upload_twiddles(c, &c->tw_gb, c->k1g.video.B, 0)) != hipSuccess) return bad(e, "twiddles");
         std::vector<uint4> taps;
-        c->m_nc = build_mfma_taps(c->k1.rf.taps, c->k1.rf.lm, c->k1.rf.lp, c->mixed_bf16, taps, &c->m_scale);
+        c->m_nc = build_mfma_taps(c->k1.rf.taps, c->k1.rf.lm, c->k1.rf.lp, c->mixed_bf16, taps, &c->m_scale,
+                                  &c->m_glo0, &c->m_glo1);
         c->m_scale *= 256.0;   // u8 sample minus 128, times 256 (raw_reader.cpp:71-76)
         c->m_nx = mfma_input_elems(c->m_nc);
         c->m_nblk = mfma_blocks_per_segment(f.spfp);
@@ -1591,9 +1595,12 @@ int cuvhs_profile_work(const cuvhs_ctx* c, uint64_t* samples, int* chunks, int* 
 
 int cuvhs_k1_matrix_flops(const cuvhs_ctx* c, double* per_segment) {
     if (!c || !per_segment) return CUVHS_E_ARG;
-    // workgroups per segment x 4 waves x steps x 16 products (4 tiles x
-    // real/imaginary x hi/lo) x 2 x 32 x 32 x 16
-    *per_segment = c->mixed ? (double)mfma_blocks_per_segment(c->fmt.spfp) * 4.0 * c->m_nc * 16.0 * 32768.0 : 0.0;
+    // workgroups per segment x 4 waves x (steps of the groups with lo x 16
+    // products: 4 tiles x real/imaginary x hi/lo; the other steps x 8) x
+    // 2 x 32 x 32 x 16
+    const double nlo = 4.0 * (c->m_glo1 - c->m_glo0);   // steps with lo parts
+    const double prods = 16.0 * nlo + 8.0 * (c->m_nc - nlo);
+    *per_segment = c->mixed ? (double)mfma_blocks_per_segment(c->fmt.spfp) * 4.0 * prods * 32768.0 : 0.0;
     return 0;
 }
 

@@ -14,7 +14,13 @@
 // one v_mfma_f32_32x32x16 per (c, operand).  The capture's u8 samples minus
 // 128 are small integers, exact in f16 and bf16, so only the taps need
 // splitting: w = hi + lo (each f16 or bf16, scaled by a power of two into
-// range), two products per step, accumulated in FP32.  A workgroup of four
+// range), two products per step, accumulated in FP32.  The taps decay by
+// orders of magnitude away from the kernel's centre; where every tap a group
+// of four steps holds is below 2^-16 (bf16: 2^-19) of the largest, its lo
+// part (at most 2^-11 of the tap in f16, 2^-8 in bf16) is below the FP32
+// accumulation's own rounding and is dropped: those groups, at both ends of
+// the kernel, take one product per step, the groups [glo0, glo1) between
+// them two.  A workgroup of four
 // waves computes 16,384 consecutive outputs (the first only as the phase
 // predecessor of the second), each wave four tiles that share every A
 // operand load.  Lane layout of the 32x32x16 operations (gfx950): lane l
@@ -26,6 +32,7 @@
 #include <cmath>
 #include <complex>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "common.h"
@@ -226,9 +233,12 @@ __global__ __launch_bounds__(256, 2) __attribute__((amdgpu_waves_per_eu(2, 2))) 
     V8 bc[kMTiles], bn[kMTiles];
 #pragma unroll
     for (int p = 0; p < 4; p++) ac[p] = sA[p * 64 + l];
+    // B operand offset of group g: 80 elements per group of four steps
+    auto gofs = [&](int g) { return 80 * g; };
 #pragma unroll
-    for (int k = 0; k < kMTiles; k++) bc[k] = *reinterpret_cast<const V8*>(xlane + 1280 * k);
-    for (int g = 0; g < ngroups; g++) {
+    for (int k = 0; k < kMTiles; k++) bc[k] = *reinterpret_cast<const V8*>(xlane + gofs(0) + 1280 * k);
+    auto run_group = [&](int g, auto lo_tag) {
+        constexpr bool LO = decltype(lo_tag)::value;
         const int buf = g & 1;
         // the last group reloads itself into the idle buffer and prefetches
         // its own first step again (no branch; the values go unused)
@@ -245,30 +255,33 @@ __global__ __launch_bounds__(256, 2) __attribute__((amdgpu_waves_per_eu(2, 2))) 
                 for (int p = 0; p < 4; p++) an[p] = sa[((cc + 1) * 4 + p) * 64 + l];
 #pragma unroll
                 for (int k = 0; k < kMTiles; k++)
-                    bn[k] = *reinterpret_cast<const V8*>(xlane + 80 * g + kStepOff[cc + 1] + 1280 * k);
+                    bn[k] = *reinterpret_cast<const V8*>(xlane + gofs(g) + kStepOff[cc + 1] + 1280 * k);
             } else {
 #pragma unroll
                 for (int p = 0; p < 4; p++) an[p] = sn[p * 64 + l];
 #pragma unroll
-                for (int k = 0; k < kMTiles; k++) bn[k] = *reinterpret_cast<const V8*>(xlane + 80 * gn + 1280 * k);
+                for (int k = 0; k < kMTiles; k++) bn[k] = *reinterpret_cast<const V8*>(xlane + gofs(gn) + 1280 * k);
             }
 #pragma unroll
             for (int k = 0; k < kMTiles; k++) {
                 acc[0][k] = Ops::mma(__builtin_bit_cast(V8, ac[0]), bc[k], acc[0][k]);
                 acc[1][k] = Ops::mma(__builtin_bit_cast(V8, ac[2]), bc[k], acc[1][k]);
             }
+            if constexpr (LO) {
 #pragma unroll
-            for (int k = 0; k < kMTiles; k++) {
-                acc[0][k] = Ops::mma(__builtin_bit_cast(V8, ac[1]), bc[k], acc[0][k]);
-                acc[1][k] = Ops::mma(__builtin_bit_cast(V8, ac[3]), bc[k], acc[1][k]);
+                for (int k = 0; k < kMTiles; k++) {
+                    acc[0][k] = Ops::mma(__builtin_bit_cast(V8, ac[1]), bc[k], acc[0][k]);
+                    acc[1][k] = Ops::mma(__builtin_bit_cast(V8, ac[3]), bc[k], acc[1][k]);
+                }
             }
             // issue order: (at a group's start, the four global tap loads),
-            // then two products and one read of the next step's operands,
-            // eight times: the reads complete under the products
+            // then two products (one without lo) and one read of the next
+            // step's operands, eight times: the reads complete under the
+            // products
             if (cc == 0) __builtin_amdgcn_sched_group_barrier(0x020, 4, 0);
 #pragma unroll
             for (int z = 0; z < 8; z++) {
-                __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);
+                __builtin_amdgcn_sched_group_barrier(0x008, LO ? 2 : 1, 0);
                 __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
             }
             if (cc == kMGroup - 2) {
@@ -284,7 +297,10 @@ __global__ __launch_bounds__(256, 2) __attribute__((amdgpu_waves_per_eu(2, 2))) 
 #pragma unroll
             for (int k = 0; k < kMTiles; k++) bc[k] = bn[k];
         }
-    }
+    };
+    for (int g = 0; g < a.glo0; g++) run_group(g, std::false_type{});
+    for (int g = a.glo0; g < a.glo1; g++) run_group(g, std::true_type{});
+    for (int g = a.glo1; g < ngroups; g++) run_group(g, std::false_type{});
     __syncthreads();   // every wave's reads of xw end before the epilogue reuses it
 
     // epilogue: envelope |acc| scale, phase arg(acc) (the scale is a
@@ -400,10 +416,12 @@ int mfma_valid_per_block() { return kMValid; }
 // elements, lane l element k = w[16 c + 8 (l / 32) + k - l % 32] (zero
 // outside the kernel), w[u] = h[lp - u] scaled by 2^e so that the largest
 // magnitude is at most 2^14 (f16 range), split into hi = E(w) and lo =
-// E(w - hi).  Returns the step count; *scale receives 2^-e.
+// E(w - hi).  [*glo0, *glo1): the groups of kMGroup steps from the first to
+// the last one holding a tap above lo_rel of the largest (their lo parts
+// matter).  Returns the step count; *scale receives 2^-e.
 template <typename E>
 static int build_taps(const std::vector<std::complex<double>>& taps, int lm, int lp, std::vector<uint4>& out,
-                      double* scale) {
+                      double* scale, double lo_rel, int* glo0, int* glo1) {
     const int L = lm + lp + 1;
     const int nc = (L + 31 + 15) / 16;
     const int ncp = (nc + kMGroup - 1) / kMGroup * kMGroup;   // padded with zero steps to whole groups
@@ -413,6 +431,18 @@ static int build_taps(const std::vector<std::complex<double>>& taps, int lm, int
     if (mx > 0.0) e = 14 - (int)std::ceil(std::log2(mx));
     const double S = std::ldexp(1.0, e);
     *scale = std::ldexp(1.0, -e);
+    *glo0 = ncp / kMGroup;
+    *glo1 = 0;
+    for (int c = 0; c < nc; c++)
+        for (int u = 16 * c - 31; u <= 16 * c + 15; u++) {
+            if (u < 0 || u >= L) continue;
+            const std::complex<double> w = taps[(size_t)(L - 1 - u)];
+            if (std::max(std::fabs(w.real()), std::fabs(w.imag())) > lo_rel * mx) {
+                *glo0 = std::min(*glo0, c / kMGroup);
+                *glo1 = std::max(*glo1, c / kMGroup + 1);
+            }
+        }
+    if (*glo1 <= *glo0) *glo0 = *glo1 = 0;
     out.assign((size_t)ncp * 4 * 64, uint4{});
     for (int c = 0; c < nc; c++)
         for (int l = 0; l < 64; l++) {
@@ -438,8 +468,11 @@ static int build_taps(const std::vector<std::complex<double>>& taps, int lm, int
 }
 
 int build_mfma_taps(const std::vector<std::complex<double>>& taps, int lm, int lp, int bf16, std::vector<uint4>& out,
-                    double* scale) {
-    return bf16 ? build_taps<__bf16>(taps, lm, lp, out, scale) : build_taps<_Float16>(taps, lm, lp, out, scale);
+                    double* scale, int* glo0, int* glo1) {
+    // lo parts dropped below 2^-16 (f16) / 2^-19 (bf16) of the largest tap:
+    // there they are at most 2^-27 of it
+    return bf16 ? build_taps<__bf16>(taps, lm, lp, out, scale, std::ldexp(1.0, -19), glo0, glo1)
+                : build_taps<_Float16>(taps, lm, lp, out, scale, std::ldexp(1.0, -16), glo0, glo1);
 }
 
 // LDS elements of the staged input: outputs of the workgroup plus the
@@ -454,7 +487,8 @@ hipError_t run_k1_part_a_mfma(int bf16, const K1MLaunch& a, int nf, hipStream_t 
     const int nb = mfma_blocks_per_segment(a.spfp);
     const size_t sh = mfma_lds_bytes(a.nx);
     if (sh > 160 * 1024 || sh < (size_t)kMWaves * 2 * kEpiRow * 4 || a.a_lds_offset != mfma_a_lds_offset(a.nx) ||
-        a.nc % kMGroup || a.nx % 16 || a.nx > kStagePasses * 256 * 16)
+        a.nc % kMGroup || a.nx % 16 || a.nx > kStagePasses * 256 * 16 || a.glo0 < 0 || a.glo0 > a.glo1 ||
+        a.glo1 > a.nc / kMGroup)
         return hipErrorInvalidValue;
     const void* kern = bf16 ? (const void*)k1_part_a_mfma<__bf16> : (const void*)k1_part_a_mfma<_Float16>;
     (void)hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);

@@ -64,6 +64,7 @@ struct K1MLaunch {
     long long n_valid, total;
     int spfp, N, lp;
     int nc, nx;                 // steps of 16 taps (whole groups), staged input elements
+    int glo0, glo1;             // groups of four steps run with their lo parts
     int a_lds_offset;           // byte offset of the tap operands in LDS (mfma_a_lds_offset)
     float scale;                // 256 (u8 scale) / tap scale
     float fs_over_tau;          // sample rate / 2 pi
@@ -169,9 +170,10 @@ int mfma_valid_per_block();
 int mfma_input_elems(int nc);
 int mfma_a_lds_offset(int nx);
 // A operands of the MFMA part A for an impulse response taps[m + lm], m =
-// -lm..lp; returns the step count, *scale the tap scale to undo
+// -lm..lp; returns the step count, *scale the tap scale to undo and
+// [*glo0, *glo1) the groups of four steps whose lo parts are kept
 int build_mfma_taps(const std::vector<std::complex<double>>& taps, int lm, int lp, int bf16, std::vector<uint4>& out,
-                    double* scale);
+                    double* scale, int* glo0, int* glo1);
 // n samples of a float (f32) or double signal into a double array
 hipError_t run_to_f64(const void* src, int f32, double* dst, long long n, hipStream_t st);
 // the guard band's FP64 block lists (marks, compaction, counts)

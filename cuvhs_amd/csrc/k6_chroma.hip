@@ -84,12 +84,13 @@ __device__ __forceinline__ void het_step(const ChromaLaunch& a, double* s, doubl
 // band-pass keeps bins by |frequency|, so the band-passed line la comes back
 // as the real part and la + 1 as the imaginary part.  (The reference
 // transforms each line on its own; the results agree to rounding.)  Thread t
-// computes columns t + 64 r, r < NF / 64, the register layout
+// computes columns t + 64 r, the register layout
 // filter1024_wave takes and returns, so the lines never pass through LDS
-// outside the transform.  NF = 2048 splits into even and odd bins first
-// (x[n] +- x[n + 1024], the odd half times W2048^n: a radix-2 step inside
-// each lane), filters each half as a 1024-point block with the bins it
-// holds, and joins them the same way.  T: the transform's type.  The
+// outside the transform; rows past the line's last sample are zero and are
+// neither computed nor stored.  NF = 2048 splits into even and odd bins
+// first (x[n] +- x[n + 1024], the odd half times W2048^n: a radix-2 step
+// inside each lane), filters each half as a 1024-point block with the bins
+// it holds, and joins them the same way.  T: the transform's type.  The
 // resampling coordinate and the heterodyne stay FP64 in both (positions
 // reach 46 M samples, the heterodyne argument 4.4e5 rad); T = float is the
 // reduced-precision forms' line filter, with whole complex elements in LDS
@@ -100,8 +101,12 @@ __device__ __forceinline__ void chroma_line(const ChromaLaunch& a, void* buf, co
     using V = typename C2<T>::type;
     constexpr bool whole = sizeof(T) == 4;
     constexpr int NF = G::NF, LL = G::LL, BL = G::BL;
-    constexpr int R = NF / 64;
     static_assert(NF == 1024 || NF == 2048, "line transform size");
+    // rows r (columns t + 64 r) that hold samples: 15 of NTSC's 16, 18 of
+    // PAL's 32; the rest of the transform's input is zero
+    constexpr int RD = (LL + 63) / 64;
+    constexpr int RX = RD > 16 ? RD - 16 : 0;   // data rows past the first 1024 points
+    static_assert(RD <= 16 || NF == 2048, "a line fits its transform");
     const int f = blockIdx.y;
     const int nf = a.nf_override > 0 ? a.nf_override : a.ctl->nf;
     if (f >= nf) return;
@@ -112,14 +117,28 @@ __device__ __forceinline__ void chroma_line(const ChromaLaunch& a, void* buf, co
     const long long total = (long long)nf * a.spfp;
     const double* ll = a.linelocs + (size_t)f * G::LPF;
     const int rot = 3;  // track 0
-    V v[R];
+    const int l = (int)threadIdx.x & 63;
+    V v[16];        // rows 0..15
+    V x2[RX > 0 ? RX : 1];   // rows 16 .. RD - 1
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        v[r].x = (T)0;
+        v[r].y = (T)0;
+    }
+#pragma unroll
+    for (int r = 0; r < (RX > 0 ? RX : 1); r++) {
+        x2[r].x = (T)0;
+        x2[r].y = (T)0;
+    }
     {
         const double ls = ll[la], le = ll[la + 1];
         double s, c;
         het_start<LL>(a, la, (double)((la * rot + 0) & 3) * (M_PI * 0.5), &s, &c);
 #pragma unroll
-        for (int r = 0; r < R; r++) {
-            v[r].x = (T)(chroma_resample<LL>(a, ls, le, (int)threadIdx.x + 64 * r, total) * -c);
+        for (int r = 0; r < RD; r++) {
+            const T x = (T)(chroma_resample<LL>(a, ls, le, l + 64 * r, total) * -c);
+            if (r < 16) v[r].x = x;
+            else x2[r - 16].x = x;
             het_step(a, &s, &c);
         }
     }
@@ -128,13 +147,12 @@ __device__ __forceinline__ void chroma_line(const ChromaLaunch& a, void* buf, co
         double s, c;
         het_start<LL>(a, lb, (double)((lb * rot + 0) & 3) * (M_PI * 0.5), &s, &c);
 #pragma unroll
-        for (int r = 0; r < R; r++) {
-            v[r].y = (T)(chroma_resample<LL>(a, ls, le, (int)threadIdx.x + 64 * r, total) * -c);
+        for (int r = 0; r < RD; r++) {
+            const T x = (T)(chroma_resample<LL>(a, ls, le, l + 64 * r, total) * -c);
+            if (r < 16) v[r].y = x;
+            else x2[r - 16].y = x;
             het_step(a, &s, &c);
         }
-    } else {
-#pragma unroll
-        for (int r = 0; r < R; r++) v[r].y = (T)0;
     }
     // band-pass: bins with lo <= |k| <= hi kept (chroma_decode.cu:163-182)
     const int lo = a.lo, hi = a.hi;
@@ -149,29 +167,39 @@ __device__ __forceinline__ void chroma_line(const ChromaLaunch& a, void* buf, co
     if constexpr (NF == 1024) {
         filter1024_wave<T, whole>(buf, v, tw, keep);
     } else {
-        const int l = (int)threadIdx.x & 63;
+        // even bins: a[n] = x[n] + x[n + 1024]; odd bins: b[n] = (x[n] -
+        // x[n + 1024]) W2048^n, n = 64 r + l; x[n + 1024] is zero past the
+        // RX data rows
+        V w[16];
 #pragma unroll
         for (int r = 0; r < 16; r++) {
-            const V x0 = v[r], x1 = v[r + 16];
-            v[r] = cadd(x0, x1);
-            v[r + 16] = cmul(csub(x0, x1), tw2[64 * r + l]);   // W2048^n, n = 64 r + l
+            V x0 = v[r], d = v[r];
+            if (r < RX) {
+                x0 = cadd(v[r], x2[r]);
+                d = csub(v[r], x2[r]);
+            }
+            v[r] = x0;
+            w[r] = cmul(d, tw2[64 * r + l]);
         }
         filter1024_wave<T, whole>(buf, v, tw, [&](V z, int k) { return keep(z, 2 * k); });
-        filter1024_wave<T, whole>(buf, v + 16, tw, [&](V z, int k) { return keep(z, 2 * k + 1); });
+        filter1024_wave<T, whole>(buf, w, tw, [&](V z, int k) { return keep(z, 2 * k + 1); });
+        // y[n] = a'[n] + W2048^-n b'[n], y[n + 1024] = a'[n] - W2048^-n b'[n]
+        // (the latter only where the line has samples)
 #pragma unroll
         for (int r = 0; r < 16; r++) {
-            const V e = v[r], o = cmul(v[r + 16], cconj(tw2[64 * r + l]));
-            v[r] = cadd(e, o);
-            v[r + 16] = csub(e, o);
+            const V o = cmul(w[r], cconj(tw2[64 * r + l]));
+            if (r < RX) x2[r] = csub(v[r], o);
+            v[r] = cadd(v[r], o);
         }
     }
     const T scale = (T)a.fft_scale;
     double* da = a.burst + ((size_t)f * G::FL + la) * BL;
     float* fa = a.filt + ((size_t)f * G::FL + la) * LL;
 #pragma unroll
-    for (int r = 0; r < R; r++) {
-        const int col = (int)threadIdx.x + 64 * r;
-        const T xa = v[r].x * scale, xb = v[r].y * scale;
+    for (int r = 0; r < RD; r++) {
+        const int col = l + 64 * r;
+        const V y = r < 16 ? v[r] : x2[r - 16];
+        const T xa = y.x * scale, xb = y.y * scale;
         if (col < LL) {
             fa[col] = (float)xa;
             if (has_b) fa[LL + col] = (float)xb;
@@ -197,7 +225,7 @@ __global__ __launch_bounds__(64) void k_chroma_line2048(ChromaLaunch a) {
     __shared__ __align__(16) double buf[lds_elems(1024)];
     chroma_line<double, PalChroma>(a, buf, a.tw, a.tw2);
 }
-__global__ __launch_bounds__(64) void k_chroma_line2048_f32(ChromaLaunch a) {
+__global__ __launch_bounds__(64, 2) void k_chroma_line2048_f32(ChromaLaunch a) {
     __shared__ __align__(16) float2 buf[lds_elems(1024)];
     chroma_line<float, PalChroma>(a, buf, a.tw_f, a.tw2_f);
 }

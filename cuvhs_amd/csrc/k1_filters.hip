@@ -21,6 +21,9 @@
 
 #include <algorithm>
 #include <atomic>
+#include <mutex>
+#include <set>
+#include <utility>
 
 #include "atan_f64.h"
 #include "common.h"
@@ -454,15 +457,19 @@ __global__ void k_fill_f64(double* p, long long n, double v) {
 // ---- host launchers ----
 // Dynamic LDS above 64 KB needs a per-kernel attribute; set it once per
 // device rather than before every launch.
+// Keyed by the kernel itself: kernels of one signature share a function
+// type, so a static per template instance would cover all of them at once.
 template <typename K>
 static void ensure_lds(K* kern, size_t sh) {
-    static std::atomic<unsigned> done{0};
+    static std::mutex mu;
+    static std::set<std::pair<const void*, int>> done;
     int d = 0;
     (void)hipGetDevice(&d);
-    const unsigned bit = 1u << (d & 31);
-    if (done.load(std::memory_order_relaxed) & bit) return;
+    const std::pair<const void*, int> key{(const void*)kern, d};
+    std::lock_guard<std::mutex> lock(mu);
+    if (done.count(key)) return;
     (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
-    done.fetch_or(bit);
+    done.insert(key);
 }
 template <int B, typename T>
 static hipError_t launch_a(const K1ALaunch& a, int nf, hipStream_t st) {

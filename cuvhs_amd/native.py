@@ -217,9 +217,24 @@ def check(rc, ctx=None, what="call"):
     return rc
 
 
+def _addr(t):
+    """Device address of a torch tensor, torch's stream already synchronised."""
+    return ctypes.c_void_p(t.data_ptr())
+
+
 def ptr(t):
-    """Device address of a torch tensor (or None)."""
-    return None if t is None else ctypes.c_void_p(t.data_ptr())
+    """Device address of a torch tensor (or None), for a library call.
+
+    The library runs on its own non-blocking streams, which do not wait for
+    torch's: the tensor's memory may still be written (a fill, a copy) or
+    read (an earlier tensor that held it) by work queued on torch's stream,
+    so that stream is synchronised first."""
+    if t is None:
+        return None
+    if t.is_cuda:
+        import torch
+        torch.cuda.current_stream(t.device).synchronize()
+    return ctypes.c_void_p(t.data_ptr())
 
 
 class Decoder:
@@ -358,6 +373,12 @@ class Decoder:
                 parts[key].append(t)
             off = info.next_raw_offset
         out = {k: (torch.cat(v) if v else None) for k, v in parts.items()}
+        # the concatenation runs on torch's stream and the parts' memory goes
+        # back to torch's allocator: the next buffer a library call writes on
+        # the decoder's (non-blocking) streams can be that memory, so the
+        # copies must have read it first
+        del parts
+        torch.cuda.current_stream(d_capture.device).synchronize()
         out["fields"] = sum(i["fields"] for i in infos)
         out["chunks"] = infos
         return out
@@ -405,8 +426,9 @@ class Decoder:
         # the capture was written on torch's stream; the decoder's streams are
         # its own (a device-wide sync would also wait for other decoders)
         torch.cuda.current_stream(dev).synchronize()
-        rc = lib().cuvhs_decode_capture(self.h, ptr(d_capture), n, max_fields, ptr(luma), ptr(chroma), ptr(file_loc),
-                                        ptr(is_first), ptr(phase), ptr(doc), ptr(dol), ptr(dos), ptr(doe),
+        rp = _addr   # torch's stream was synchronised just above
+        rc = lib().cuvhs_decode_capture(self.h, rp(d_capture), n, max_fields, rp(luma), rp(chroma), rp(file_loc),
+                                        rp(is_first), rp(phase), rp(doc), rp(dol), rp(dos), rp(doe),
                                         ctypes.byref(nf))
         check(rc, self.h, "decode_capture")
         k = nf.value

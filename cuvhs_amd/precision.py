@@ -39,6 +39,9 @@ def _chunk_k1(dec, d_capture, raw_offset, fields_k1):
     nv = min(n, dec.batch * spfp)
     shape = (fields_k1 * spfp,)
     out = [torch.empty(shape, dtype=torch.float64, device=d_capture.device) for _ in range(3)]
+    # the library writes them on the decoder's non-blocking stream: torch's
+    # work on their memory (earlier tensors it held) must be done first
+    torch.cuda.current_stream(d_capture.device).synchronize()
     base = d_capture.data_ptr() + raw_offset * d_capture.element_size()
     N.check(N.lib().cuvhs_fm_demod(dec.h, ctypes.c_void_p(base), nv, fields_k1, *[N.ptr(t) for t in out]), dec.h,
             "fm_demod")
